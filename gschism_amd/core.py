@@ -1,0 +1,275 @@
+"""Python host API over libgsh: contexts, meshes, solvers and the per-step operations.
+
+The objects own library handles; every method calls the C ABI (include/gsh.h). Arrays
+cross the boundary as NumPy host arrays; device pointers are available for zero-copy use.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import check, dptr
+
+import atexit
+import weakref
+
+# Library handles still open at interpreter exit are closed newest first (solvers and meshes
+# before their context), before the HIP runtime tears itself down.
+_live: list = []
+
+
+def _register(obj) -> None:
+    _live.append(weakref.ref(obj))
+
+
+@atexit.register
+def _close_all() -> None:
+    while _live:
+        obj = _live.pop()()
+        if obj is not None:
+            try:
+                obj.close()
+            except Exception:
+                pass
+
+
+class Context:
+    """One GPU and one HIP stream (gsh_ctx)."""
+
+    def __init__(self, device: int = 0):
+        h = C.c_void_p()
+        check(L.lib().gsh_ctx_create(device, C.byref(h)))
+        self.h = h
+        self.device = device
+        _register(self)
+
+    def close(self):
+        if getattr(self, "h", None):
+            L.lib().gsh_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def mem_info(self):
+        f, t = C.c_int64(), C.c_int64()
+        check(L.lib().gsh_ctx_mem_info(self.h, C.byref(f), C.byref(t)))
+        return f.value, t.value
+
+    def synchronize(self):
+        check(L.lib().gsh_ctx_synchronize(self.h))
+
+
+_FIELD_SPEC = {
+    L.F_ETA2: (np.float64, "n", ()), L.F_ETA1: (np.float64, "n", ()),
+    L.F_NODE_UVW: (np.float64, "n", ("nvrt", 3)), L.F_SIDE_U2: (np.float64, "ns", ("nvrt", 2)),
+    L.F_BTU: (np.float64, "ns", ("nvrt", 2)), L.F_FOOT: (np.int32, "ns", ("nvrt",)),
+    L.F_BSTATUS: (np.int32, "ns", ("nvrt",)), L.F_CD: (np.float64, "n", ()),
+    L.F_GRAV2: (np.float64, "n", ()), L.F_ETH: (np.float64, "n", ()), L.F_VSOURCE: (np.float64, "ne", ()),
+    L.F_SIDE_U: (np.float64, "ns", (2,)), L.F_SIDE_HHAT: (np.float64, "ns", ()),
+    L.F_SIDE_CHI: (np.float64, "ns", ()), L.F_GHAT1: (np.float64, "ne", (2,)),
+    L.F_ZC: (np.float64, "ns", ("nvrt",)), L.F_ELBC: (np.float64, "n", ()),
+}
+
+
+class Mesh:
+    """A triangular mesh with its device-resident state (gsh_mesh).
+
+    With renumber=True the nodes are put in reverse Cuthill-McKee order (elements follow);
+    every array in and out of the mesh is in that order, and `node_perm[new] = old`.
+    ctx=None builds a host-only mesh (topology and geometry, no device state).
+    """
+
+    def __init__(self, ctx: Context | None, xy, elnode, depth, sigma, open_bnd=None, kb=None, renumber=True):
+        xy = np.ascontiguousarray(xy, dtype=np.float64).reshape(-1, 2)
+        elnode = np.ascontiguousarray(elnode, dtype=np.int32).reshape(-1, 3)
+        depth = np.ascontiguousarray(depth, dtype=np.float64)
+        sigma = np.ascontiguousarray(sigma, dtype=np.float64)
+        n, ne = xy.shape[0], elnode.shape[0]
+        ob = None if open_bnd is None else np.ascontiguousarray(open_bnd, dtype=np.int8)
+        kbv = None if kb is None else np.ascontiguousarray(kb, dtype=np.int32)
+        d = L.MeshDesc()
+        d.n, d.ne, d.nvrt = n, ne, sigma.size
+        d.xy = dptr(xy)
+        d.elnode = dptr(elnode, C.c_int32)
+        d.depth = dptr(depth)
+        d.open_bnd = ob.ctypes.data_as(C.POINTER(C.c_int8)) if ob is not None else None
+        d.sigma = dptr(sigma)
+        d.kb = dptr(kbv, C.c_int32) if kbv is not None else None
+        d.renumber = 1 if renumber else 0
+        self.node_perm = np.empty(n, dtype=np.int32)
+        self.elem_perm = np.empty(ne, dtype=np.int32)
+        h = C.c_void_p()
+        check(L.lib().gsh_mesh_create(ctx.h if ctx is not None else None, C.byref(d), C.byref(h),
+                                      dptr(self.node_perm, C.c_int32), dptr(self.elem_perm, C.c_int32)))
+        self.h = h
+        self.ctx = ctx
+        _register(self)
+        info = L.MeshInfo()
+        check(L.lib().gsh_mesh_get_info(self.h, C.byref(info)))
+        self.n, self.ne, self.ns, self.nvrt = info.n, info.ne, info.ns, info.nvrt
+        self.nnz, self.max_row, self.n_open = info.nnz, info.max_row, info.n_open
+
+    def close(self):
+        if getattr(self, "h", None):
+            L.lib().gsh_mesh_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    # ---------------------------------------------------------------- topology
+    def topology(self, code: int) -> np.ndarray:
+        nbr = self.nnz - self.n
+        spec = {
+            L.T_XY: (np.float64, (self.n, 2)), L.T_ELNODE: (np.int32, (self.ne, 3)),
+            L.T_ELSIDE: (np.int32, (self.ne, 3)), L.T_ISDEL: (np.int32, (self.ns, 2)),
+            L.T_ISIDENODE: (np.int32, (self.ns, 2)), L.T_BALL_PTR: (np.int32, (self.n + 1,)),
+            L.T_BALL_ELEM: (np.int32, (3 * self.ne,)), L.T_NBR_NODE: (np.int32, (nbr,)),
+            L.T_CSR_INDPTR: (np.int32, (self.n + 1,)), L.T_CSR_INDICES: (np.int32, (self.nnz,)),
+            L.T_CSR_SLOT: (np.int16, (nbr,)), L.T_DEPTH: (np.float64, (self.n,)),
+            L.T_OPEN_BND: (np.int8, (self.n,)), L.T_KB: (np.int32, (self.n,)),
+            L.T_SIDE_TYPE: (np.int8, (self.ns,)),
+        }
+        dt, shp = spec[code]
+        out = np.empty(shp, dtype=dt)
+        check(L.lib().gsh_mesh_get_topology(self.h, code, out.ctypes.data_as(C.c_void_p)))
+        return out
+
+    # ---------------------------------------------------------------- fields
+    def _shape(self, code):
+        if code == L.F_WET:
+            return np.int8, (self.n + self.ne + self.ns,)
+        dt, lead, tail = _FIELD_SPEC[code]
+        shp = (getattr(self, lead),) + tuple(getattr(self, t) if isinstance(t, str) else t for t in tail)
+        return dt, shp
+
+    def set(self, code: int, arr) -> None:
+        dt, shp = self._shape(code)
+        a = np.ascontiguousarray(arr, dtype=dt).reshape(shp)
+        check(L.lib().gsh_mesh_set_field(self.h, code, a.ctypes.data_as(C.c_void_p), a.size))
+
+    def get(self, code: int) -> np.ndarray:
+        dt, shp = self._shape(code)
+        out = np.empty(shp, dtype=dt)
+        check(L.lib().gsh_mesh_get_field(self.h, code, out.ctypes.data_as(C.c_void_p), out.size))
+        return out
+
+    def field_ptr(self, code: int) -> int:
+        p, cnt = C.c_void_p(), C.c_int64()
+        check(L.lib().gsh_mesh_field_ptr(self.h, code, C.byref(p), C.byref(cnt)))
+        return p.value
+
+    # ---------------------------------------------------------------- step operations
+    def update_levels(self, params) -> None:
+        check(L.lib().gsh_update_levels(self.h, C.byref(params)))
+
+    def backtrack(self, params, copy_out: bool = False):
+        """backtrackAllNodes: every side x level; returns (btu, foot) when copy_out."""
+        if copy_out:
+            btu = np.empty((self.ns, self.nvrt, 2), dtype=np.float64)
+            foot = np.empty((self.ns, self.nvrt), dtype=np.int32)
+            check(L.lib().gsh_backtrack(self.h, C.byref(params), dptr(btu), dptr(foot, C.c_int32)))
+            return btu, foot
+        check(L.lib().gsh_backtrack(self.h, C.byref(params), None, None))
+        return None
+
+    def backtrack_list(self, params, sides, levels) -> None:
+        s = np.ascontiguousarray(sides, dtype=np.int32)
+        k = np.ascontiguousarray(levels, dtype=np.int32)
+        check(L.lib().gsh_backtrack_list(self.h, C.byref(params), dptr(s, C.c_int32), dptr(k, C.c_int32), s.size))
+
+    def rhs_producers(self, params) -> None:
+        check(L.lib().gsh_rhs_producers(self.h, C.byref(params)))
+
+
+class Solver:
+    """Jacobi PCG on the device (gsh_solver), bound to a mesh or to a caller CSR pattern."""
+
+    def __init__(self, mesh: Mesh | None = None, precision: int = L.PREC_FP64, *, ctx: Context | None = None,
+                 indptr=None, indices=None):
+        h = C.c_void_p()
+        if mesh is not None:
+            check(L.lib().gsh_solver_create(mesh.h, precision, L.PRECOND_JACOBI, C.byref(h)))
+            self.n, self.nnz = mesh.n, mesh.nnz
+        else:
+            if ctx is None or indptr is None or indices is None:
+                raise ValueError("a stand-alone solver needs ctx, indptr and indices")
+            ip = np.ascontiguousarray(indptr, dtype=np.int32)
+            ix = np.ascontiguousarray(indices, dtype=np.int32)
+            self.n, self.nnz = ip.size - 1, ix.size
+            check(L.lib().gsh_solver_create_csr(ctx.h, self.n, self.nnz, dptr(ip, C.c_int32), dptr(ix, C.c_int32),
+                                                precision, L.PRECOND_JACOBI, C.byref(h)))
+        self.h = h
+        self.mesh = mesh
+        self.ctx = ctx if ctx is not None else (mesh.ctx if mesh is not None else None)
+        self.precision = precision
+        _register(self)
+
+    def close(self):
+        if getattr(self, "h", None):
+            L.lib().gsh_solver_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def csr_ptrs(self):
+        d, i, p, nnz = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_int64()
+        check(L.lib().gsh_solver_csr(self.h, C.byref(d), C.byref(i), C.byref(p), C.byref(nnz)))
+        return d.value, i.value, p.value, nnz.value
+
+    def set_values(self, val) -> None:
+        v = np.ascontiguousarray(val, dtype=np.float64)
+        if v.size != self.nnz:
+            raise ValueError("value array has the wrong length")
+        check(L.lib().gsh_solver_set_values(self.h, dptr(v)))
+
+    def get_values(self) -> np.ndarray:
+        v = np.empty(self.nnz, dtype=np.float64)
+        check(L.lib().gsh_solver_get_values(self.h, dptr(v)))
+        return v
+
+    def assemble(self, params, time: float = 0.0) -> None:
+        if self.mesh is None:
+            raise ValueError("solver is not bound to a mesh")
+        check(L.lib().gsh_assemble(self.mesh.h, self.h, C.byref(params), time))
+
+    def solve(self, b=None, rtol: float = 1e-12, maxiter: int = 0, check_every: int = 10, timeout_s: float = 60.0,
+              want_x: bool = True):
+        """Returns (x or None, iters, status)."""
+        bb = None if b is None else np.ascontiguousarray(b, dtype=np.float64)
+        x = np.empty(self.n, dtype=np.float64) if want_x else None
+        it, st = C.c_int(), C.c_int()
+        check(L.lib().gsh_solver_solve(self.h, dptr(bb) if bb is not None else None,
+                                       dptr(x) if x is not None else None, rtol, maxiter, check_every, timeout_s,
+                                       C.byref(it), C.byref(st)))
+        return x, it.value, st.value
+
+    def check(self) -> float:
+        r = C.c_double()
+        check(L.lib().gsh_solver_check(self.h, C.byref(r)))
+        return r.value
+
+    def reset_history(self) -> None:
+        check(L.lib().gsh_solver_reset_history(self.h))
+
+    def stats(self) -> L.SolveStats:
+        s = L.SolveStats()
+        check(L.lib().gsh_solver_last_stats(self.h, C.byref(s)))
+        return s
+
+    def set_tuning(self, rows_per_thread: int = 4, inner_rtol: float = 1e-5) -> None:
+        check(L.lib().gsh_solver_set_tuning(self.h, rows_per_thread, inner_rtol))
+
+    def update_elevation(self) -> None:
+        check(L.lib().gsh_update_elevation(self.mesh.h, self.h))
+
+
+def batch_solve(solvers, rtol=1e-12, maxiter=0, check_every=10, timeout_s=60.0):
+    arr = (C.c_void_p * len(solvers))(*[s.h.value for s in solvers])
+    its = (C.c_int * len(solvers))()
+    check(L.lib().gsh_batch_solve(arr, len(solvers), rtol, maxiter, check_every, timeout_s, its))
+    return list(its)

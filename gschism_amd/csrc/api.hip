@@ -1,0 +1,1342 @@
+// api.hip -- the extern "C" boundary of libgsh: contexts, meshes, step operations, the PCG
+// solver driver (single, batched, mixed precision) and the continuous-batching server.
+//
+// Host-side counterpart of the reference's PCGSolverMultiGPU (pcg_solver.py:538-1151) and its
+// compiled loop NativePCG (pcg_loop.cpp:152-465): the iteration loop issues the two fused PCG
+// phases per iteration on one HIP stream and reads back one small state record per check
+// window, asynchronously, so the GPU never idles waiting for the host.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <thread>
+#include <vector>
+
+#include "abi_util.h"
+#include "device.h"
+
+namespace gsh {
+
+static thread_local std::string g_last_error;
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+void clear_last_error() { g_last_error.clear(); }
+
+static void hip_check(hipError_t e, const char* what) {
+  if (e == hipSuccess) return;
+  const int code = (e == hipErrorOutOfMemory) ? GSH_E_OOM : GSH_E_DEVICE;
+  throw Error(code, std::string(what) + ": " + hipGetErrorString(e));
+}
+#define HIPCHK(x) hip_check((x), #x)
+
+using Clock = std::chrono::steady_clock;
+
+// Device allocations owned by one object, released together.
+struct DevPool {
+  std::vector<void*> ptrs;
+  int64_t bytes = 0;
+  template <class T>
+  T* alloc(int64_t count, bool zero = true) {
+    if (count <= 0) count = 1;
+    void* p = nullptr;
+    const size_t nb = (size_t)count * sizeof(T);
+    HIPCHK(hipMalloc(&p, nb));
+    ptrs.push_back(p);
+    bytes += (int64_t)nb;
+    if (zero) HIPCHK(hipMemset(p, 0, nb));
+    return static_cast<T*>(p);
+  }
+  template <class T>
+  T* upload(const std::vector<T>& v) {
+    T* p = alloc<T>((int64_t)v.size(), false);
+    if (!v.empty()) HIPCHK(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return p;
+  }
+  void release() {
+    for (void* p : ptrs) (void)hipFree(p);
+    ptrs.clear();
+    bytes = 0;
+  }
+  ~DevPool() { release(); }
+};
+
+// Waits for an event with a time limit (pcg_loop.cpp:444-456 raises instead of hanging).
+static void wait_event(hipEvent_t ev, double timeout_s, const char* what) {
+  const auto t0 = Clock::now();
+  int spins = 0;
+  for (;;) {
+    const hipError_t r = hipEventQuery(ev);
+    if (r == hipSuccess) return;
+    if (r != hipErrorNotReady) hip_check(r, what);
+    if (std::chrono::duration<double>(Clock::now() - t0).count() > timeout_s)
+      throw Error(GSH_E_TIMEOUT, std::string(what) + ": the GPU did not finish within the time limit");
+    if (++spins > 64) std::this_thread::sleep_for(std::chrono::microseconds(20));
+  }
+}
+
+template <class VT>
+__global__ void k_gather_states(const PcgSys<VT>* __restrict__ sys, int nsys, PcgState* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < nsys) out[i] = *sys[i].st;
+}
+
+__global__ void k_extrapolate(double* __restrict__ x, const double* __restrict__ h0, const double* __restrict__ h1,
+                              const double* __restrict__ h2, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = (3.0 * h0[i] - 3.0 * h1[i]) + h2[i];
+}
+
+__global__ void k_pack_uvw(const double* __restrict__ in, double* __restrict__ out, int64_t cnt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= cnt) return;
+  out[4 * i] = in[3 * i];
+  out[4 * i + 1] = in[3 * i + 1];
+  out[4 * i + 2] = in[3 * i + 2];
+  out[4 * i + 3] = 0.0;
+}
+
+__global__ void k_unpack_uvw(const double* __restrict__ in, double* __restrict__ out, int64_t cnt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= cnt) return;
+  out[3 * i] = in[4 * i];
+  out[3 * i + 1] = in[4 * i + 1];
+  out[3 * i + 2] = in[4 * i + 2];
+}
+
+}  // namespace gsh
+
+using namespace gsh;
+
+// ====================================================================== context
+struct gsh_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  PcgState* pinned = nullptr;   // host-visible readback slots
+  int pinned_cap = 0;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  void set() const { HIPCHK(hipSetDevice(device)); }
+  PcgState* pinned_slots(int per_slot) {
+    if (2 * per_slot > pinned_cap) {
+      if (pinned) (void)hipHostFree(pinned);
+      pinned = nullptr;
+      const int cap = std::max(64, 2 * per_slot);
+      HIPCHK(hipHostMalloc((void**)&pinned, sizeof(PcgState) * cap, hipHostMallocDefault));
+      pinned_cap = cap;
+    }
+    return pinned;
+  }
+};
+
+// ====================================================================== mesh
+struct gsh_mesh {
+  gsh_ctx* ctx = nullptr;
+  HostMesh hm;
+  Topology topo;
+  Geometry geo;
+  DMesh d{};
+  DevPool pool;
+  int32_t n_open = 0;
+};
+
+// ====================================================================== solver
+struct gsh_solver {
+  gsh_ctx* ctx = nullptr;
+  gsh_mesh* mesh = nullptr;
+  int precision = GSH_PREC_FP64;
+  int rpt = 4;                 // rows per thread of a tile
+  double inner_rtol = 1e-5;    // MIXED: relative tolerance of each inner FP32 solve
+  int32_t n = 0, ntiles = 0;
+  int64_t nnz = 0;
+  DevPool pool;
+  int32_t *rowptr = nullptr, *col = nullptr, *diag = nullptr;
+  double *val = nullptr, *b = nullptr, *x = nullptr, *h[3] = {nullptr, nullptr, nullptr};
+  PcgSys<double> hs{};         // host copy of the FP64 system descriptor
+  PcgSys<double>* ds = nullptr;
+  TileRef* dtiles = nullptr;   // tiles of this system alone
+  double* red2 = nullptr;      // scratch for two reduced scalars
+  // mixed precision
+  float* val32 = nullptr;
+  double* r64 = nullptr;
+  PcgSys<float> hs32{};
+  PcgSys<float>* ds32 = nullptr;
+  gsh_solve_stats last{};
+
+  int tiles_for(int rows) const { return (rows + kTpb * rpt - 1) / (kTpb * rpt); }
+};
+
+static void solver_alloc(gsh_solver* s, int32_t n, int64_t nnz, const std::vector<int32_t>& indptr,
+                         const std::vector<int32_t>& indices, const std::vector<int32_t>& diag_abs) {
+  s->n = n;
+  s->nnz = nnz;
+  s->ntiles = s->tiles_for(n);
+  auto& P = s->pool;
+  const int nt = std::max(1, (n + kTpb - 1) / kTpb);  // room for the smallest tile size
+  s->rowptr = P.upload(indptr);
+  s->col = P.upload(indices);
+  s->diag = P.upload(diag_abs);
+  s->val = P.alloc<double>(nnz);
+  s->b = P.alloc<double>(n);
+  s->x = P.alloc<double>(n);
+  for (auto& hh : s->h) hh = P.alloc<double>(n);
+  PcgSys<double>& h = s->hs;
+  h.n = n;
+  h.ntiles = s->ntiles;
+  h.rowptr = s->rowptr;
+  h.col = s->col;
+  h.val = s->val;
+  h.diag = s->diag;
+  h.b = s->b;
+  h.x = s->x;
+  h.r = P.alloc<double>(n);
+  h.z = P.alloc<double>(n);
+  h.pa = P.alloc<double>(n);
+  h.pb = P.alloc<double>(n);
+  h.q = P.alloc<double>(n);
+  h.invd = P.alloc<double>(n);
+  h.part_rr = P.alloc<double>(nt);
+  h.part_rz = P.alloc<double>(nt);
+  h.part_pq = P.alloc<double>(nt);
+  h.part_bb = P.alloc<double>(nt);
+  h.st = P.alloc<PcgState>(1);
+  s->ds = P.alloc<PcgSys<double>>(1);
+  s->dtiles = P.alloc<TileRef>(nt);
+  s->red2 = P.alloc<double>(2);
+  if (s->precision == GSH_PREC_MIXED) {
+    s->val32 = P.alloc<float>(nnz);
+    s->r64 = P.alloc<double>(n);
+    PcgSys<float>& g = s->hs32;
+    g.n = n;
+    g.ntiles = s->ntiles;
+    g.rowptr = s->rowptr;
+    g.col = s->col;
+    g.val = s->val32;
+    g.diag = s->diag;
+    g.b = s->r64;
+    g.x = P.alloc<float>(n);
+    g.r = P.alloc<float>(n);
+    g.z = P.alloc<float>(n);
+    g.pa = P.alloc<float>(n);
+    g.pb = P.alloc<float>(n);
+    g.q = P.alloc<float>(n);
+    g.invd = P.alloc<float>(n);
+    g.h0 = g.h1 = g.h2 = nullptr;
+    g.part_rr = P.alloc<double>(nt);
+    g.part_rz = P.alloc<double>(nt);
+    g.part_pq = P.alloc<double>(nt);
+    g.part_bb = P.alloc<double>(nt);
+    g.st = P.alloc<PcgState>(1);
+    s->ds32 = P.alloc<PcgSys<float>>(1);
+  }
+}
+
+static std::vector<TileRef> tile_table(int sys, int ntiles) {
+  std::vector<TileRef> t(ntiles);
+  for (int i = 0; i < ntiles; ++i) t[i] = {sys, i};
+  return t;
+}
+
+// Re-uploads the descriptors and the single-system tile table (pointers rotate between solves).
+static void solver_sync_desc(gsh_solver* s) {
+  hipStream_t st = s->ctx->stream;
+  s->hs.x = s->x;
+  s->hs.h0 = s->h[0];
+  s->hs.h1 = s->h[1];
+  s->hs.h2 = s->h[2];
+  s->hs.ntiles = s->ntiles;
+  HIPCHK(hipMemcpyAsync(s->ds, &s->hs, sizeof(s->hs), hipMemcpyHostToDevice, st));
+  if (s->ds32) {
+    s->hs32.ntiles = s->ntiles;
+    HIPCHK(hipMemcpyAsync(s->ds32, &s->hs32, sizeof(s->hs32), hipMemcpyHostToDevice, st));
+  }
+  auto t = tile_table(0, s->ntiles);
+  HIPCHK(hipMemcpyAsync(s->dtiles, t.data(), t.size() * sizeof(TileRef), hipMemcpyHostToDevice, st));
+  HIPCHK(hipStreamSynchronize(st));  // host staging buffers above are stack/heap temporaries
+}
+
+// rollHistory (pcg_solver.py:941-947) by pointer rotation: x2 <- x1 <- x0 <- x.
+static void solver_roll(gsh_solver* s) {
+  double* old2 = s->h[2];
+  s->h[2] = s->h[1];
+  s->h[1] = s->h[0];
+  s->h[0] = s->x;
+  s->x = old2;
+}
+
+// ---------------------------------------------------------------------- PCG driver
+struct PcgOutcome {
+  std::vector<PcgState> st;
+};
+
+// Runs the PCG iteration for nsys systems described by dsys (device) over the tile table
+// dtiles. Each system stops at its own first check with r.r <= atol^2 (pcg_loop.cpp:405-412).
+template <class VT>
+static void run_pcg(gsh_ctx* ctx, const PcgSys<VT>* dsys, int nsys, const TileRef* dtiles, int ntiles, int rpt,
+                    double rtol, int maxiter, int check_every, double timeout_s, PcgOutcome& out,
+                    const TileRef* dinit_tiles = nullptr, int ninit_tiles = -1) {
+  hipStream_t st = ctx->stream;
+  if (check_every < 1) check_every = 1;
+  if (timeout_s <= 0.0) timeout_s = 60.0;
+  PcgState* slots = ctx->pinned_slots(nsys);
+  PcgState* dgather = nullptr;
+  HIPCHK(hipMallocAsync((void**)&dgather, sizeof(PcgState) * 2 * nsys, st));
+  launch_pcg_reset<VT>(dsys, nsys, rtol, maxiter, check_every, st);
+  if (dinit_tiles)
+    launch_pcg_init<VT>(dsys, dinit_tiles, ninit_tiles, rpt, st);
+  else
+    launch_pcg_init<VT>(dsys, dtiles, ntiles, rpt, st);
+  HIPCHK(hipGetLastError());
+  int issued = 0, window = 0;
+  std::deque<int> inflight;  // window slots pending readback
+  bool all_done = false;
+  const auto t0 = Clock::now();
+  auto check_slot = [&](int slot) {
+    wait_event(ctx->ev[slot], timeout_s, "PCG window");
+    bool done = true;
+    for (int i = 0; i < nsys; ++i) done = done && (slots[slot * nsys + i].done >= 0);
+    return done;
+  };
+  while (issued < maxiter && !all_done) {
+    const int w = std::min(check_every, maxiter - issued);
+    for (int i = 0; i < w; ++i) launch_pcg_iter<VT>(dsys, dtiles, ntiles, rpt, st);
+    launch_pcg_check<VT>(dsys, nsys, st);
+    issued += w;
+    const int slot = window & 1;
+    if (inflight.size() == 2) {
+      const int old = inflight.front();
+      inflight.pop_front();
+      if (check_slot(old)) all_done = true;
+    }
+    hipLaunchKernelGGL(k_gather_states<VT>, dim3((nsys + 63) / 64), dim3(64), 0, st, dsys, nsys,
+                       dgather + slot * nsys);
+    HIPCHK(hipMemcpyAsync(slots + slot * nsys, dgather + slot * nsys, sizeof(PcgState) * nsys,
+                          hipMemcpyDeviceToHost, st));
+    HIPCHK(hipEventRecord(ctx->ev[slot], st));
+    inflight.push_back(slot);
+    ++window;
+    if (std::chrono::duration<double>(Clock::now() - t0).count() > timeout_s)
+      throw Error(GSH_E_TIMEOUT, "PCG: time limit exceeded while issuing iterations");
+    HIPCHK(hipGetLastError());
+  }
+  launch_pcg_finish<VT>(dsys, dtiles, ntiles, rpt, st);
+  hipLaunchKernelGGL(k_gather_states<VT>, dim3((nsys + 63) / 64), dim3(64), 0, st, dsys, nsys, dgather);
+  HIPCHK(hipMemcpyAsync(slots, dgather, sizeof(PcgState) * nsys, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipEventRecord(ctx->ev[0], st));
+  wait_event(ctx->ev[0], timeout_s, "PCG finish");
+  HIPCHK(hipFreeAsync(dgather, st));
+  out.st.assign(slots, slots + nsys);
+}
+
+static int iters_of(const PcgState& s, int maxiter) { return s.done >= 0 ? s.done : maxiter; }
+
+// Mixed precision: FP64 true residual + refinement around FP32 PCG solves (FP64 dots).
+static void solve_mixed(gsh_solver* s, double rtol, int maxiter, int check_every, double timeout_s, int* iters_out,
+                        int* status_out) {
+  gsh_ctx* ctx = s->ctx;
+  hipStream_t st = ctx->stream;
+  const int n = s->n;
+  launch_cast_values(s->val, s->val32, s->nnz, st);
+  hipLaunchKernelGGL(k_extrapolate, dim3((n + 255) / 256), dim3(256), 0, st, s->x, s->h[0], s->h[1], s->h[2], n);
+  double hostred[2];
+  int total_inner = 0, outer = 0;
+  double atol2 = -1.0, rr = 0.0;
+  bool converged = false;
+  const int max_outer = 30;
+  for (; outer < max_outer; ++outer) {
+    launch_true_residual(s->ds, s->dtiles, s->ntiles, s->rpt, s->x, s->r64, st);
+    launch_reduce_pair(s->hs.part_rr, s->hs.part_bb, s->ntiles, s->red2, st);
+    HIPCHK(hipMemcpyAsync(hostred, s->red2, sizeof(hostred), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipEventRecord(ctx->ev[0], st));
+    wait_event(ctx->ev[0], timeout_s, "refinement residual");
+    rr = hostred[0];
+    if (outer == 0) {
+      const double atol = rtol * std::sqrt(hostred[1]);
+      atol2 = atol * atol;
+      if (atol == 0.0) {
+        launch_fill_f64(s->x, 0.0, n, st);
+        converged = true;
+        break;
+      }
+    }
+    if (rr <= atol2) {
+      converged = true;
+      break;
+    }
+    if (total_inner >= maxiter) break;
+    // inner FP32 solve of A d = r to inner_rtol (relative to ||r||), started from d = 0
+    PcgOutcome o;
+    const double in_rtol = std::max(s->inner_rtol, 0.5 * std::sqrt(atol2 / rr));
+    run_pcg<float>(ctx, s->ds32, 1, s->dtiles, s->ntiles, s->rpt, in_rtol, maxiter - total_inner, check_every,
+                   timeout_s, o);
+    const int it = iters_of(o.st[0], maxiter - total_inner);
+    total_inner += it;
+    launch_refine_update(s->x, s->hs32.x, n, st);
+    if (it == 0) break;
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  s->last.iters = total_inner;
+  s->last.outer = outer + 1;
+  s->last.rr = rr;
+  s->last.atol2 = atol2;
+  *iters_out = total_inner;
+  *status_out = converged ? 0 : GSH_W_MAXITER;
+}
+
+// ====================================================================== C ABI
+extern "C" {
+
+const char* gsh_last_error(void) { return g_last_error.c_str(); }
+int gsh_version(void) { return 100; }
+
+int gsh_ctx_create(int device, gsh_ctx** out) {
+  GSH_API_BODY({
+    GSH_REQUIRE(out, GSH_E_ARG, "out is NULL");
+    int count = 0;
+    HIPCHK(hipGetDeviceCount(&count));
+    GSH_REQUIRE(device >= 0 && device < count, GSH_E_ARG, "no such device");
+    auto c = std::make_unique<gsh_ctx>();
+    c->device = device;
+    c->set();
+    HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&c->ev[0], hipEventDisableTiming));
+    HIPCHK(hipEventCreateWithFlags(&c->ev[1], hipEventDisableTiming));
+    *out = c.release();
+  })
+}
+
+int gsh_ctx_destroy(gsh_ctx* ctx) {
+  if (!ctx) return GSH_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+  for (auto& e : ctx->ev)
+    if (e) (void)hipEventDestroy(e);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return GSH_OK;
+}
+
+int gsh_ctx_mem_info(gsh_ctx* ctx, int64_t* free_bytes, int64_t* total_bytes) {
+  GSH_API_BODY({
+    GSH_REQUIRE(ctx, GSH_E_ARG, "ctx is NULL");
+    ctx->set();
+    size_t f = 0, t = 0;
+    HIPCHK(hipMemGetInfo(&f, &t));
+    if (free_bytes) *free_bytes = (int64_t)f;
+    if (total_bytes) *total_bytes = (int64_t)t;
+  })
+}
+
+int gsh_ctx_synchronize(gsh_ctx* ctx) {
+  GSH_API_BODY({
+    GSH_REQUIRE(ctx, GSH_E_ARG, "ctx is NULL");
+    ctx->set();
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+  })
+}
+
+void gsh_step_params_default(gsh_step_params* p) {
+  if (!p) return;
+  // defaults of readNamelist.py:98-106, 181-193 and constants_template.h
+  p->dt = 150.0;
+  p->theta = 0.6;
+  p->h0 = 0.01;
+  p->ramp = 1.0;
+  p->dtb_min = 10.0;
+  p->dtb_max = 30.0;
+  p->velmin_btrack = 1e-4;
+  p->ihhat = 1;
+  p->iettype = 2;
+}
+
+static StepArgs step_args(const gsh_step_params* p) {
+  GSH_REQUIRE(p, GSH_E_ARG, "step params are NULL");
+  GSH_REQUIRE(p->dt > 0.0 && p->dtb_min > 0.0 && p->dtb_max > 0.0, GSH_E_ARG, "dt, dtb_min, dtb_max must be > 0");
+  GSH_REQUIRE(p->theta >= 0.0 && p->theta <= 1.0, GSH_E_ARG, "theta must be in [0, 1]");
+  StepArgs a{};
+  a.dt = p->dt;
+  a.theta = p->theta;
+  a.h0 = p->h0;
+  a.ramp = p->ramp;
+  a.dtb_min = p->dtb_min;
+  a.dtb_max = p->dtb_max;
+  a.velmin2 = p->velmin_btrack * p->velmin_btrack;
+  a.ihhat = p->ihhat;
+  a.iettype = p->iettype;
+  // backtrackAllNodes (meshdata.cu:3550-3551)
+  a.ndelt_max = (int)std::fmax(1.0, p->dt / p->dtb_min);
+  a.ndelt_min = (int)std::fmax(1.0, p->dt / p->dtb_max);
+  return a;
+}
+
+int gsh_mesh_create(gsh_ctx* ctx, const gsh_mesh_desc* desc, gsh_mesh** out, int32_t* node_perm_out,
+                    int32_t* elem_perm_out) {
+  GSH_API_BODY({
+    GSH_REQUIRE(desc && out, GSH_E_ARG, "NULL argument");
+    GSH_REQUIRE(desc->n >= 3 && desc->ne >= 1, GSH_E_ARG, "mesh needs at least 3 nodes and 1 element");
+    GSH_REQUIRE(desc->nvrt >= 2, GSH_E_ARG, "nvrt must be >= 2");
+    GSH_REQUIRE(desc->xy && desc->elnode && desc->depth && desc->sigma, GSH_E_ARG, "xy, elnode, depth, sigma required");
+    if (ctx) ctx->set();
+    auto m = std::make_unique<gsh_mesh>();
+    m->ctx = ctx;
+    HostMesh& hm = m->hm;
+    hm.n = desc->n;
+    hm.ne = desc->ne;
+    hm.nvrt = desc->nvrt;
+    hm.x.resize(hm.n);
+    hm.y.resize(hm.n);
+    for (int32_t i = 0; i < hm.n; ++i) hm.x[i] = desc->xy[2 * i], hm.y[i] = desc->xy[2 * i + 1];
+    hm.depth.assign(desc->depth, desc->depth + hm.n);
+    hm.sigma.assign(desc->sigma, desc->sigma + hm.nvrt);
+    hm.elnode.assign(desc->elnode, desc->elnode + 3 * (size_t)hm.ne);
+    hm.open_bnd.assign(hm.n, 0);
+    if (desc->open_bnd)
+      for (int32_t i = 0; i < hm.n; ++i) hm.open_bnd[i] = desc->open_bnd[i] ? 1 : 0;
+    hm.kb.assign(hm.n, 0);
+    if (desc->kb) hm.kb.assign(desc->kb, desc->kb + hm.n);
+    for (int32_t i = 0; i < hm.n; ++i)
+      GSH_REQUIRE(hm.kb[i] >= 0 && hm.kb[i] <= hm.nvrt - 2, GSH_E_ARG, "kb must be in [0, nvrt-2]");
+    for (int k = 1; k < hm.nvrt; ++k) GSH_REQUIRE(hm.sigma[k] > hm.sigma[k - 1], GSH_E_ARG, "sigma must increase");
+    std::vector<int32_t> nperm(hm.n), eperm(hm.ne);
+    for (int32_t i = 0; i < hm.n; ++i) nperm[i] = i;
+    for (int32_t i = 0; i < hm.ne; ++i) eperm[i] = i;
+    if (desc->renumber) {
+      for (int32_t k = 0; k < 3 * hm.ne; ++k)
+        GSH_REQUIRE(hm.elnode[k] >= 0 && hm.elnode[k] < hm.n, GSH_E_TOPOLOGY, "element with an invalid node id");
+      nperm = rcm_order(hm.n, hm.ne, hm.elnode);
+      renumber(hm, nperm, eperm);
+    }
+    build_topology(hm, m->topo);
+    build_geometry(hm, m->topo, m->geo);
+    const Topology& t = m->topo;
+    const int32_t n = hm.n, ne = hm.ne, ns = t.ns, nv = hm.nvrt;
+    m->n_open = 0;
+    for (auto v : hm.open_bnd) m->n_open += v;
+    // ball -> CSR slots of the two other nodes of each ball element
+    std::vector<int16_t> ball_slot(2 * t.ball_elem.size());
+    for (int32_t i = 0; i < n; ++i) {
+      const int32_t row = t.indptr[i], len = t.indptr[i + 1] - row;
+      for (int32_t j = t.ball_ptr[i]; j < t.ball_ptr[i + 1]; ++j) {
+        const int32_t e = t.ball_elem[j];
+        const int id1 = t.ball_loc[j];
+        for (int q = 0; q < 2; ++q) {
+          const int32_t other = hm.elnode[3 * e + (id1 + 1 + q) % 3];
+          int16_t slot = -1;
+          for (int32_t k = 0; k < len; ++k)
+            if (t.indices[row + k] == other) slot = (int16_t)k;
+          GSH_REQUIRE(slot >= 0, GSH_E_TOPOLOGY, "ball element node missing from CSR row");
+          ball_slot[2 * j + q] = slot;
+        }
+      }
+    }
+    if (node_perm_out) std::memcpy(node_perm_out, nperm.data(), sizeof(int32_t) * n);
+    if (elem_perm_out) std::memcpy(elem_perm_out, eperm.data(), sizeof(int32_t) * ne);
+    if (!ctx) {  // host-only mesh: topology and geometry without device state
+      *out = m.release();
+      return GSH_OK;
+    }
+    std::vector<int4> erec(2 * (size_t)ne);
+    for (int32_t e = 0; e < ne; ++e) {
+      erec[2 * e] = make_int4(hm.elnode[3 * e], hm.elnode[3 * e + 1], hm.elnode[3 * e + 2], t.elside[3 * e]);
+      erec[2 * e + 1] = make_int4(t.elside[3 * e + 1], t.elside[3 * e + 2], 0, 0);
+    }
+    std::vector<int4> srec(ns);
+    for (int32_t s = 0; s < ns; ++s)
+      srec[s] = make_int4(t.isidenode[2 * s], t.isidenode[2 * s + 1], t.isdel[2 * s], t.isdel[2 * s + 1]);
+    DevPool& P = m->pool;
+    DMesh& d = m->d;
+    d.n = n;
+    d.ne = ne;
+    d.ns = ns;
+    d.nvrt = nv;
+    d.x = P.upload(hm.x);
+    d.y = P.upload(hm.y);
+    d.depth = P.upload(hm.depth);
+    d.kb = P.upload(hm.kb);
+    d.sigma = P.upload(hm.sigma);
+    d.open_bnd = P.upload(hm.open_bnd);
+    d.eta2 = P.alloc<double>(n);
+    d.eta1 = P.alloc<double>(n);
+    d.grav2 = P.upload(std::vector<double>(n, 9.81));
+    d.cd = P.upload(std::vector<double>(n, 2.5e-3));
+    d.eth = P.alloc<double>(n);
+    d.elbc = P.alloc<double>(n);
+    d.uvw = P.alloc<double>(4 * (int64_t)n * nv);
+    d.node_wet = P.alloc<int8_t>(n);
+    d.elem = P.upload(erec);
+    d.area = P.upload(m->geo.area);
+    d.gradphi = P.upload(m->geo.gradphi);
+    d.elem_wet = P.alloc<int8_t>(ne);
+    d.elem_kb = P.alloc<int32_t>(ne);
+    d.ghat1 = P.alloc<double>(2 * (int64_t)ne);
+    d.vsource = P.alloc<double>(ne);
+    d.side = P.upload(srec);
+    d.side_type = P.upload(t.side_type);
+    d.side_cx = P.upload(m->geo.side_cx);
+    d.side_cy = P.upload(m->geo.side_cy);
+    d.side_len = P.upload(m->geo.side_len);
+    d.side_kb = P.alloc<int32_t>(ns);
+    d.side_wet = P.alloc<int8_t>(ns);
+    d.zc = P.alloc<double>((int64_t)ns * nv);
+    d.su2 = P.alloc<double>(2 * (int64_t)ns * nv);
+    d.btu = P.alloc<double>(2 * (int64_t)ns * nv);
+    d.foot = P.alloc<int32_t>((int64_t)ns * nv);
+    d.bstatus = P.alloc<int32_t>((int64_t)ns * nv);
+    d.sU = P.alloc<double>(2 * (int64_t)ns);
+    d.hhat = P.alloc<double>(ns);
+    d.chi = P.alloc<double>(ns);
+    d.ball_ptr = P.upload(t.ball_ptr);
+    d.ball_elem = P.upload(t.ball_elem);
+    d.ball_loc = P.upload(t.ball_loc);
+    d.ball_slot = P.upload(ball_slot);
+    d.nbr_ptr = P.upload(t.nbr_ptr);
+    d.nbr_node = P.upload(t.nbr_node);
+    d.csr_slot = P.upload(t.csr_slot);
+    d.indptr = P.upload(t.indptr);
+    d.diag_off = P.upload(t.diag_off);
+    // initial levels with eta2 = 0
+    gsh_step_params p;
+    gsh_step_params_default(&p);
+    launch_levels(d, step_args(&p), ctx->stream);
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    *out = m.release();
+  })
+}
+
+int gsh_mesh_destroy(gsh_mesh* mesh) {
+  if (!mesh) return GSH_OK;
+  if (mesh->ctx) {
+    (void)hipSetDevice(mesh->ctx->device);
+    (void)hipStreamSynchronize(mesh->ctx->stream);
+  }
+  delete mesh;
+  return GSH_OK;
+}
+
+int gsh_mesh_get_info(const gsh_mesh* m, gsh_mesh_info* out) {
+  GSH_API_BODY({
+    GSH_REQUIRE(m && out, GSH_E_ARG, "NULL argument");
+    out->n = m->hm.n;
+    out->ne = m->hm.ne;
+    out->ns = m->topo.ns;
+    out->nvrt = m->hm.nvrt;
+    out->nnz = m->topo.indptr.back();
+    out->max_row = m->topo.max_row;
+    out->n_open = m->n_open;
+  })
+}
+
+int gsh_mesh_get_topology(const gsh_mesh* m, int what, void* out) {
+  GSH_API_BODY({
+    GSH_REQUIRE(m && out, GSH_E_ARG, "NULL argument");
+    const Topology& t = m->topo;
+    auto put = [&](const auto& v) { std::memcpy(out, v.data(), v.size() * sizeof(v[0])); };
+    switch (what) {
+      case GSH_T_XY: {
+        double* o = static_cast<double*>(out);
+        for (int32_t i = 0; i < m->hm.n; ++i) o[2 * i] = m->hm.x[i], o[2 * i + 1] = m->hm.y[i];
+        break;
+      }
+      case GSH_T_ELNODE: put(m->hm.elnode); break;
+      case GSH_T_ELSIDE: put(t.elside); break;
+      case GSH_T_ISDEL: put(t.isdel); break;
+      case GSH_T_ISIDENODE: put(t.isidenode); break;
+      case GSH_T_BALL_PTR: put(t.ball_ptr); break;
+      case GSH_T_BALL_ELEM: put(t.ball_elem); break;
+      case GSH_T_NBR_NODE: put(t.nbr_node); break;
+      case GSH_T_CSR_INDPTR: put(t.indptr); break;
+      case GSH_T_CSR_INDICES: put(t.indices); break;
+      case GSH_T_CSR_SLOT: put(t.csr_slot); break;
+      case GSH_T_DEPTH: put(m->hm.depth); break;
+      case GSH_T_OPEN_BND: put(m->hm.open_bnd); break;
+      case GSH_T_KB: put(m->hm.kb); break;
+      case GSH_T_SIDE_TYPE: put(t.side_type); break;
+      default: throw Error(GSH_E_ARG, "unknown topology array");
+    }
+  })
+}
+
+struct FieldRef {
+  void* ptr;
+  int64_t count;  // public element count
+  int elem;       // bytes per public element
+};
+
+static FieldRef field_ref(const gsh_mesh* m, int field) {
+  const DMesh& d = m->d;
+  const int64_t n = d.n, ne = d.ne, ns = d.ns, nv = d.nvrt;
+  switch (field) {
+    case GSH_F_ETA2: return {d.eta2, n, 8};
+    case GSH_F_ETA1: return {d.eta1, n, 8};
+    case GSH_F_NODE_UVW: return {d.uvw, 3 * n * nv, 8};
+    case GSH_F_SIDE_U2: return {d.su2, 2 * ns * nv, 8};
+    case GSH_F_BTU: return {d.btu, 2 * ns * nv, 8};
+    case GSH_F_FOOT: return {d.foot, ns * nv, 4};
+    case GSH_F_BSTATUS: return {d.bstatus, ns * nv, 4};
+    case GSH_F_CD: return {d.cd, n, 8};
+    case GSH_F_GRAV2: return {d.grav2, n, 8};
+    case GSH_F_ETH: return {d.eth, n, 8};
+    case GSH_F_VSOURCE: return {d.vsource, ne, 8};
+    case GSH_F_SIDE_U: return {d.sU, 2 * ns, 8};
+    case GSH_F_SIDE_HHAT: return {d.hhat, ns, 8};
+    case GSH_F_SIDE_CHI: return {d.chi, ns, 8};
+    case GSH_F_GHAT1: return {d.ghat1, 2 * ne, 8};
+    case GSH_F_ZC: return {d.zc, ns * nv, 8};
+    case GSH_F_ELBC: return {d.elbc, n, 8};
+    case GSH_F_WET: return {nullptr, n + ne + ns, 1};
+    default: throw Error(GSH_E_ARG, "unknown field");
+  }
+}
+
+int gsh_mesh_set_field(gsh_mesh* m, int field, const void* host, int64_t count) {
+  GSH_API_BODY({
+    GSH_REQUIRE(m && host, GSH_E_ARG, "NULL argument");
+    GSH_REQUIRE(m->ctx, GSH_E_STATE, "host-only mesh (created without a context)");
+    m->ctx->set();
+    GSH_REQUIRE(field != GSH_F_WET, GSH_E_ARG, "wet flags are derived by gsh_update_levels");
+    FieldRef f = field_ref(m, field);
+    GSH_REQUIRE(count == f.count, GSH_E_ARG, "field size mismatch");
+    hipStream_t st = m->ctx->stream;
+    if (field == GSH_F_NODE_UVW) {
+      double* tmp = nullptr;
+      HIPCHK(hipMallocAsync((void**)&tmp, sizeof(double) * count, st));
+      HIPCHK(hipMemcpyAsync(tmp, host, sizeof(double) * count, hipMemcpyHostToDevice, st));
+      const int64_t recs = count / 3;
+      hipLaunchKernelGGL(k_pack_uvw, dim3((unsigned)((recs + 255) / 256)), dim3(256), 0, st, tmp, m->d.uvw, recs);
+      HIPCHK(hipFreeAsync(tmp, st));
+    } else {
+      HIPCHK(hipMemcpyAsync(f.ptr, host, (size_t)count * f.elem, hipMemcpyHostToDevice, st));
+    }
+    HIPCHK(hipStreamSynchronize(st));
+  })
+}
+
+int gsh_mesh_get_field(const gsh_mesh* m, int field, void* host, int64_t count) {
+  GSH_API_BODY({
+    GSH_REQUIRE(m && host, GSH_E_ARG, "NULL argument");
+    GSH_REQUIRE(m->ctx, GSH_E_STATE, "host-only mesh (created without a context)");
+    m->ctx->set();
+    FieldRef f = field_ref(m, field);
+    GSH_REQUIRE(count == f.count, GSH_E_ARG, "field size mismatch");
+    hipStream_t st = m->ctx->stream;
+    if (field == GSH_F_NODE_UVW) {
+      double* tmp = nullptr;
+      HIPCHK(hipMallocAsync((void**)&tmp, sizeof(double) * count, st));
+      const int64_t recs = count / 3;
+      hipLaunchKernelGGL(k_unpack_uvw, dim3((unsigned)((recs + 255) / 256)), dim3(256), 0, st, m->d.uvw, tmp, recs);
+      HIPCHK(hipMemcpyAsync(host, tmp, sizeof(double) * count, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipFreeAsync(tmp, st));
+    } else if (field == GSH_F_WET) {
+      int8_t* h = static_cast<int8_t*>(host);
+      HIPCHK(hipMemcpyAsync(h, m->d.node_wet, m->d.n, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipMemcpyAsync(h + m->d.n, m->d.elem_wet, m->d.ne, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipMemcpyAsync(h + m->d.n + m->d.ne, m->d.side_wet, m->d.ns, hipMemcpyDeviceToHost, st));
+    } else {
+      HIPCHK(hipMemcpyAsync(host, f.ptr, (size_t)count * f.elem, hipMemcpyDeviceToHost, st));
+    }
+    HIPCHK(hipStreamSynchronize(st));
+  })
+}
+
+int gsh_mesh_field_ptr(gsh_mesh* m, int field, void** dev_ptr, int64_t* count) {
+  GSH_API_BODY({
+    GSH_REQUIRE(m && dev_ptr, GSH_E_ARG, "NULL argument");
+    GSH_REQUIRE(field != GSH_F_WET && field != GSH_F_NODE_UVW, GSH_E_ARG,
+                "this field has a device layout of its own; use gsh_mesh_get_field");
+    FieldRef f = field_ref(m, field);
+    *dev_ptr = f.ptr;
+    if (count) *count = f.count;
+  })
+}
+
+int gsh_update_levels(gsh_mesh* m, const gsh_step_params* p) {
+  GSH_API_BODY({
+    GSH_REQUIRE(m, GSH_E_ARG, "mesh is NULL");
+    GSH_REQUIRE(m->ctx, GSH_E_STATE, "host-only mesh (created without a context)");
+    m->ctx->set();
+    launch_levels(m->d, step_args(p), m->ctx->stream);
+    HIPCHK(hipGetLastError());
+  })
+}
+
+int gsh_backtrack(gsh_mesh* m, const gsh_step_params* p, double* btu_host, int32_t* foot_host) {
+  GSH_API_BODY({
+    GSH_REQUIRE(m, GSH_E_ARG, "mesh is NULL");
+    GSH_REQUIRE(m->ctx, GSH_E_STATE, "host-only mesh (created without a context)");
+    m->ctx->set();
+    hipStream_t st = m->ctx->stream;
+    launch_backtrack(m->d, step_args(p), nullptr, nullptr, 0, st);
+    HIPCHK(hipGetLastError());
+    const int64_t cnt = (int64_t)m->d.ns * m->d.nvrt;
+    if (btu_host) HIPCHK(hipMemcpyAsync(btu_host, m->d.btu, 16 * cnt, hipMemcpyDeviceToHost, st));
+    if (foot_host) HIPCHK(hipMemcpyAsync(foot_host, m->d.foot, 4 * cnt, hipMemcpyDeviceToHost, st));
+    if (btu_host || foot_host) HIPCHK(hipStreamSynchronize(st));
+  })
+}
+
+int gsh_backtrack_list(gsh_mesh* m, const gsh_step_params* p, const int32_t* sides, const int32_t* levels,
+                       int32_t count) {
+  GSH_API_BODY({
+    GSH_REQUIRE(m && sides && levels && count >= 0, GSH_E_ARG, "bad arguments");
+    GSH_REQUIRE(m->ctx, GSH_E_STATE, "host-only mesh (created without a context)");
+    m->ctx->set();
+    for (int32_t i = 0; i < count; ++i)
+      GSH_REQUIRE(sides[i] >= 0 && sides[i] < m->d.ns && levels[i] >= 0 && levels[i] < m->d.nvrt, GSH_E_ARG,
+                  "start point out of range");
+    if (count == 0) return GSH_OK;
+    hipStream_t st = m->ctx->stream;
+    int32_t* buf = nullptr;
+    HIPCHK(hipMallocAsync((void**)&buf, 8 * (size_t)count, st));
+    HIPCHK(hipMemcpyAsync(buf, sides, 4 * (size_t)count, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(buf + count, levels, 4 * (size_t)count, hipMemcpyHostToDevice, st));
+    launch_backtrack(m->d, step_args(p), buf, buf + count, count, st);
+    HIPCHK(hipFreeAsync(buf, st));
+    HIPCHK(hipStreamSynchronize(st));
+  })
+}
+
+int gsh_rhs_producers(gsh_mesh* m, const gsh_step_params* p) {
+  GSH_API_BODY({
+    GSH_REQUIRE(m, GSH_E_ARG, "mesh is NULL");
+    GSH_REQUIRE(m->ctx, GSH_E_STATE, "host-only mesh (created without a context)");
+    m->ctx->set();
+    launch_rhs(m->d, step_args(p), m->ctx->stream);
+    HIPCHK(hipGetLastError());
+  })
+}
+
+// ---------------------------------------------------------------------- solver entry points
+static std::vector<int32_t> diag_abs_of(const std::vector<int32_t>& indptr, const std::vector<int32_t>& indices,
+                                        int32_t n) {
+  std::vector<int32_t> d(n, -1);
+  for (int32_t i = 0; i < n; ++i)
+    for (int32_t k = indptr[i]; k < indptr[i + 1]; ++k)
+      if (indices[k] == i) d[i] = k;
+  for (int32_t i = 0; i < n; ++i)
+    if (d[i] < 0) throw Error(GSH_E_ARG, "CSR row " + std::to_string(i) + " has no diagonal entry");
+  return d;
+}
+
+int gsh_solver_create(gsh_mesh* mesh, int precision, int precond, gsh_solver** out) {
+  GSH_API_BODY({
+    GSH_REQUIRE(mesh && out, GSH_E_ARG, "NULL argument");
+    GSH_REQUIRE(precision == GSH_PREC_FP64 || precision == GSH_PREC_MIXED, GSH_E_ARG, "unknown precision");
+    GSH_REQUIRE(precond == GSH_PRECOND_JACOBI, GSH_E_ARG, "unknown preconditioner");
+    GSH_REQUIRE(mesh->ctx, GSH_E_STATE, "host-only mesh (created without a context)");
+    mesh->ctx->set();
+    auto s = std::make_unique<gsh_solver>();
+    s->ctx = mesh->ctx;
+    s->mesh = mesh;
+    s->precision = precision;
+    const Topology& t = mesh->topo;
+    std::vector<int32_t> dabs(mesh->hm.n);
+    for (int32_t i = 0; i < mesh->hm.n; ++i) dabs[i] = t.indptr[i] + t.diag_off[i];
+    solver_alloc(s.get(), mesh->hm.n, t.indptr.back(), t.indptr, t.indices, dabs);
+    *out = s.release();
+  })
+}
+
+int gsh_solver_create_csr(gsh_ctx* ctx, int32_t n, int64_t nnz, const int32_t* indptr, const int32_t* indices,
+                          int precision, int precond, gsh_solver** out) {
+  GSH_API_BODY({
+    GSH_REQUIRE(ctx && indptr && indices && out && n > 0 && nnz > 0, GSH_E_ARG, "bad arguments");
+    GSH_REQUIRE(precision == GSH_PREC_FP64 || precision == GSH_PREC_MIXED, GSH_E_ARG, "unknown precision");
+    GSH_REQUIRE(precond == GSH_PRECOND_JACOBI, GSH_E_ARG, "unknown preconditioner");
+    GSH_REQUIRE(nnz < (int64_t)1 << 31, GSH_E_ARG, "nnz exceeds int32 indexing");
+    GSH_REQUIRE(indptr[0] == 0 && indptr[n] == nnz, GSH_E_ARG, "indptr must start at 0 and end at nnz");
+    for (int32_t i = 0; i < n; ++i) GSH_REQUIRE(indptr[i + 1] >= indptr[i], GSH_E_ARG, "indptr must be non-decreasing");
+    for (int64_t k = 0; k < nnz; ++k) GSH_REQUIRE(indices[k] >= 0 && indices[k] < n, GSH_E_ARG, "column out of range");
+    ctx->set();
+    auto s = std::make_unique<gsh_solver>();
+    s->ctx = ctx;
+    s->precision = precision;
+    std::vector<int32_t> ip(indptr, indptr + n + 1), ix(indices, indices + nnz);
+    solver_alloc(s.get(), n, nnz, ip, ix, diag_abs_of(ip, ix, n));
+    *out = s.release();
+  })
+}
+
+int gsh_solver_destroy(gsh_solver* s) {
+  if (!s) return GSH_OK;
+  (void)hipSetDevice(s->ctx->device);
+  (void)hipStreamSynchronize(s->ctx->stream);
+  delete s;
+  return GSH_OK;
+}
+
+int gsh_solver_csr(gsh_solver* s, double** data, int32_t** indices, int32_t** indptr, int64_t* nnz) {
+  GSH_API_BODY({
+    GSH_REQUIRE(s, GSH_E_ARG, "solver is NULL");
+    if (data) *data = s->val;
+    if (indices) *indices = s->col;
+    if (indptr) *indptr = s->rowptr;
+    if (nnz) *nnz = s->nnz;
+  })
+}
+
+int gsh_solver_set_values(gsh_solver* s, const double* data) {
+  GSH_API_BODY({
+    GSH_REQUIRE(s && data, GSH_E_ARG, "NULL argument");
+    s->ctx->set();
+    HIPCHK(hipMemcpy(s->val, data, sizeof(double) * s->nnz, hipMemcpyHostToDevice));
+  })
+}
+
+int gsh_solver_get_values(gsh_solver* s, double* data) {
+  GSH_API_BODY({
+    GSH_REQUIRE(s && data, GSH_E_ARG, "NULL argument");
+    s->ctx->set();
+    HIPCHK(hipStreamSynchronize(s->ctx->stream));
+    HIPCHK(hipMemcpy(data, s->val, sizeof(double) * s->nnz, hipMemcpyDeviceToHost));
+  })
+}
+
+int gsh_solver_vec_ptr(gsh_solver* s, double** b, double** x) {
+  GSH_API_BODY({
+    GSH_REQUIRE(s, GSH_E_ARG, "solver is NULL");
+    if (b) *b = s->b;
+    if (x) *x = s->x;
+  })
+}
+
+int gsh_solver_reset_history(gsh_solver* s) {
+  GSH_API_BODY({
+    GSH_REQUIRE(s, GSH_E_ARG, "solver is NULL");
+    s->ctx->set();
+    for (auto* h : s->h) launch_fill_f64(h, 0.0, s->n, s->ctx->stream);
+    HIPCHK(hipStreamSynchronize(s->ctx->stream));
+  })
+}
+
+int gsh_solver_set_tuning(gsh_solver* s, int rows_per_thread, double inner_rtol) {
+  GSH_API_BODY({
+    GSH_REQUIRE(s, GSH_E_ARG, "solver is NULL");
+    GSH_REQUIRE(rows_per_thread == 1 || rows_per_thread == 2 || rows_per_thread == 4 || rows_per_thread == 8,
+                GSH_E_ARG, "rows_per_thread must be 1, 2, 4 or 8");
+    GSH_REQUIRE(inner_rtol > 0.0 && inner_rtol < 1.0, GSH_E_ARG, "inner_rtol must be in (0, 1)");
+    s->rpt = rows_per_thread;
+    s->ntiles = s->tiles_for(s->n);
+    s->hs.ntiles = s->ntiles;
+    s->hs32.ntiles = s->ntiles;
+    s->inner_rtol = inner_rtol;
+  })
+}
+
+int gsh_assemble(gsh_mesh* m, gsh_solver* s, const gsh_step_params* p, double time) {
+  (void)time;  // only bed deformation (imm != 0) depends on time (meshdata.cu:3244-3258)
+  GSH_API_BODY({
+    GSH_REQUIRE(m && s, GSH_E_ARG, "NULL argument");
+    GSH_REQUIRE(s->mesh == m, GSH_E_ARG, "solver is not bound to this mesh");
+    GSH_REQUIRE(m->ctx, GSH_E_STATE, "host-only mesh (created without a context)");
+    m->ctx->set();
+    launch_assemble(m->d, step_args(p), s->val, s->b, m->ctx->stream);
+    HIPCHK(hipGetLastError());
+  })
+}
+
+int gsh_solver_solve(gsh_solver* s, const double* b_host, double* x_host, double rtol, int maxiter,
+                     int check_every, double timeout_s, int* iters_out, int* status_out) {
+  GSH_API_BODY({
+    GSH_REQUIRE(s, GSH_E_ARG, "solver is NULL");
+    GSH_REQUIRE(rtol >= 0.0, GSH_E_ARG, "rtol must be >= 0");
+    if (maxiter <= 0) maxiter = 10 * s->n;  // pcg_solver.py:1082-1083
+    s->ctx->set();
+    hipStream_t st = s->ctx->stream;
+    const auto t0 = Clock::now();
+    if (b_host) HIPCHK(hipMemcpyAsync(s->b, b_host, sizeof(double) * s->n, hipMemcpyHostToDevice, st));
+    solver_sync_desc(s);
+    int iters = 0, status = 0;
+    if (s->precision == GSH_PREC_MIXED) {
+      solve_mixed(s, rtol, maxiter, check_every, timeout_s, &iters, &status);
+    } else {
+      PcgOutcome o;
+      run_pcg<double>(s->ctx, s->ds, 1, s->dtiles, s->ntiles, s->rpt, rtol, maxiter, check_every, timeout_s, o);
+      const PcgState& r = o.st[0];
+      iters = iters_of(r, maxiter);
+      status = (r.done >= 0) ? 0 : GSH_W_MAXITER;
+      s->last.iters = iters;
+      s->last.outer = 1;
+      s->last.rr = r.rr_last;
+      s->last.atol2 = r.atol2;
+    }
+    s->last.true_rel = -1.0;
+    if (x_host) HIPCHK(hipMemcpyAsync(x_host, s->x, sizeof(double) * s->n, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    solver_roll(s);
+    s->last.ms = std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
+    if (iters_out) *iters_out = iters;
+    if (status_out) *status_out = status;
+  })
+}
+
+int gsh_solver_check(gsh_solver* s, double* rel_out) {
+  GSH_API_BODY({
+    GSH_REQUIRE(s && rel_out, GSH_E_ARG, "NULL argument");
+    s->ctx->set();
+    hipStream_t st = s->ctx->stream;
+    // the last solution is the newest history entry after the roll
+    solver_sync_desc(s);
+    launch_true_residual(s->ds, s->dtiles, s->ntiles, s->rpt, s->h[0], nullptr, st);
+    launch_reduce_pair(s->hs.part_rr, s->hs.part_bb, s->ntiles, s->red2, st);
+    double h[2];
+    HIPCHK(hipMemcpyAsync(h, s->red2, sizeof(h), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    const double rn = std::sqrt(h[0]), bn = std::sqrt(h[1]);
+    *rel_out = (bn > 0.0) ? rn / bn : rn;
+    s->last.true_rel = *rel_out;
+  })
+}
+
+int gsh_solver_last_stats(gsh_solver* s, gsh_solve_stats* out) {
+  GSH_API_BODY({
+    GSH_REQUIRE(s && out, GSH_E_ARG, "NULL argument");
+    *out = s->last;
+  })
+}
+
+int gsh_update_elevation(gsh_mesh* m, gsh_solver* s) {
+  GSH_API_BODY({
+    GSH_REQUIRE(m && s && s->mesh == m, GSH_E_ARG, "solver is not bound to this mesh");
+    GSH_REQUIRE(m->ctx, GSH_E_STATE, "host-only mesh (created without a context)");
+    m->ctx->set();
+    // after the solve the solution is the newest history entry (scatterXToMesh)
+    launch_update_elevation(m->d, s->h[0], m->ctx->stream);
+    HIPCHK(hipStreamSynchronize(m->ctx->stream));
+  })
+}
+
+// ---------------------------------------------------------------------- batching
+int gsh_batch_solve(gsh_solver* const* solvers, int count, double rtol, int maxiter, int check_every,
+                    double timeout_s, int* iters_out) {
+  GSH_API_BODY({
+    GSH_REQUIRE(solvers && count > 0, GSH_E_ARG, "bad arguments");
+    gsh_ctx* ctx = solvers[0]->ctx;
+    for (int i = 0; i < count; ++i) {
+      GSH_REQUIRE(solvers[i] && solvers[i]->ctx == ctx, GSH_E_ARG, "all solvers must share one context");
+      GSH_REQUIRE(solvers[i]->precision == GSH_PREC_FP64, GSH_E_ARG, "batched solve supports FP64 solvers");
+      for (int j = 0; j < i; ++j) GSH_REQUIRE(solvers[j] != solvers[i], GSH_E_ARG, "a solver appears twice");
+    }
+    ctx->set();
+    hipStream_t st = ctx->stream;
+    const int rpt = solvers[0]->rpt;
+    int mi = maxiter;
+    std::vector<PcgSys<double>> hs(count);
+    std::vector<TileRef> tiles;
+    for (int i = 0; i < count; ++i) {
+      gsh_solver* s = solvers[i];
+      s->rpt = rpt;
+      s->ntiles = s->tiles_for(s->n);
+      s->hs.x = s->x;
+      s->hs.h0 = s->h[0];
+      s->hs.h1 = s->h[1];
+      s->hs.h2 = s->h[2];
+      s->hs.ntiles = s->ntiles;
+      hs[i] = s->hs;
+      if (mi <= 0) mi = std::max(mi, 10 * s->n);
+      for (int t = 0; t < s->ntiles; ++t) tiles.push_back({i, t});
+    }
+    if (maxiter <= 0) {
+      maxiter = 0;
+      for (int i = 0; i < count; ++i) maxiter = std::max(maxiter, 10 * solvers[i]->n);
+    }
+    PcgSys<double>* dsys = nullptr;
+    TileRef* dt = nullptr;
+    HIPCHK(hipMallocAsync((void**)&dsys, sizeof(PcgSys<double>) * count, st));
+    HIPCHK(hipMallocAsync((void**)&dt, sizeof(TileRef) * tiles.size(), st));
+    HIPCHK(hipMemcpyAsync(dsys, hs.data(), sizeof(PcgSys<double>) * count, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dt, tiles.data(), sizeof(TileRef) * tiles.size(), hipMemcpyHostToDevice, st));
+    PcgOutcome o;
+    run_pcg<double>(ctx, dsys, count, dt, (int)tiles.size(), rpt, rtol, maxiter, check_every, timeout_s, o);
+    HIPCHK(hipFreeAsync(dsys, st));
+    HIPCHK(hipFreeAsync(dt, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (int i = 0; i < count; ++i) {
+      gsh_solver* s = solvers[i];
+      const int it = iters_of(o.st[i], maxiter);
+      if (iters_out) iters_out[i] = it;
+      s->last.iters = it;
+      s->last.outer = 1;
+      s->last.rr = o.st[i].rr_last;
+      s->last.atol2 = o.st[i].atol2;
+      solver_roll(s);
+    }
+  })
+}
+
+}  // extern "C"
+
+// ====================================================================== serving
+// Continuous batching: requests are admitted between check windows as device memory and
+// slots allow; all admitted systems advance together, one lock-step launch per PCG phase,
+// each with its own iteration counter and stopping test; finished systems leave at the next
+// window boundary. Per-system reductions use per-system tiles only, so a request's result is
+// identical alone and in any mix.
+namespace {
+struct Request {
+  int64_t ticket = 0;
+  int32_t mesh_id = 0;
+  std::vector<double> b;
+  bool has_b = false, with_bt = false;
+  Clock::time_point t_submit;
+  int status = 1;  // 1 queued/active, 0 done, GSH_E_* failed
+  bool active = false;
+  std::vector<double> x;
+  int32_t iters = 0;
+  int32_t run = 0;  // iterations issued for this request
+  double latency_ms = 0.0;
+  gsh_solver* solver = nullptr;
+  int64_t bytes = 0;
+};
+}  // namespace
+
+struct gsh_server {
+  gsh_ctx* ctx = nullptr;
+  gsh_server_config cfg{};
+  int64_t budget = 0, used = 0;
+  std::vector<gsh_mesh*> meshes;
+  std::vector<std::vector<gsh_solver*>> pool;  // idle solvers per mesh
+  std::deque<Request*> queue;
+  std::vector<Request*> active;
+  std::map<int64_t, std::unique_ptr<Request>> reqs;
+  int64_t next_ticket = 1;
+  PcgSys<double>* dsys = nullptr;
+  int dsys_cap = 0;
+  TileRef* dtiles = nullptr;
+  int dtiles_cap = 0;
+  TileRef* dfresh = nullptr;
+  int dfresh_cap = 0;
+  PcgState* dgather = nullptr;
+  int dgather_cap = 0;
+  ~gsh_server() {
+    for (auto& v : pool)
+      for (auto* s : v) gsh_solver_destroy(s);
+    for (auto& kv : reqs)
+      if (kv.second->solver) gsh_solver_destroy(kv.second->solver);
+    (void)hipFree(dsys);
+    (void)hipFree(dtiles);
+    (void)hipFree(dfresh);
+    (void)hipFree(dgather);
+  }
+};
+
+static int64_t solver_bytes(const gsh_mesh* m) {
+  const int64_t n = m->hm.n, nnz = m->topo.indptr.back();
+  return 12 * nnz + 4 * (n + 1) + 4 * n + 8 * 14 * n + 4 * 8 * ((n + kTpb - 1) / kTpb) + 4096;
+}
+
+template <class T>
+static void ensure_cap(T*& p, int& cap, int need) {
+  if (need <= cap) return;
+  if (p) HIPCHK(hipFree(p));
+  p = nullptr;
+  const int c = std::max(need, 2 * cap);
+  HIPCHK(hipMalloc((void**)&p, sizeof(T) * c));
+  cap = c;
+}
+
+extern "C" {
+
+int gsh_server_create(gsh_ctx* ctx, const gsh_server_config* cfg, gsh_server** out) {
+  GSH_API_BODY({
+    GSH_REQUIRE(ctx && cfg && out, GSH_E_ARG, "NULL argument");
+    GSH_REQUIRE(cfg->precision == GSH_PREC_FP64, GSH_E_ARG, "the server runs FP64 solves");
+    GSH_REQUIRE(cfg->check_every >= 1 && cfg->maxiter >= 1 && cfg->maxiter % cfg->check_every == 0, GSH_E_ARG,
+                "maxiter must be a positive multiple of check_every");
+    GSH_REQUIRE(cfg->rtol >= 0.0, GSH_E_ARG, "rtol must be >= 0");
+    ctx->set();
+    auto s = std::make_unique<gsh_server>();
+    s->ctx = ctx;
+    s->cfg = *cfg;
+    size_t f = 0, t = 0;
+    HIPCHK(hipMemGetInfo(&f, &t));
+    s->budget = cfg->hbm_budget_bytes > 0 ? cfg->hbm_budget_bytes : (int64_t)f;
+    *out = s.release();
+  })
+}
+
+int gsh_server_destroy(gsh_server* srv) {
+  if (!srv) return GSH_OK;
+  (void)hipSetDevice(srv->ctx->device);
+  (void)hipStreamSynchronize(srv->ctx->stream);
+  delete srv;
+  return GSH_OK;
+}
+
+int gsh_server_add_mesh(gsh_server* srv, gsh_mesh* mesh, int32_t* id) {
+  GSH_API_BODY({
+    GSH_REQUIRE(srv && mesh && id, GSH_E_ARG, "NULL argument");
+    GSH_REQUIRE(mesh->ctx == srv->ctx, GSH_E_ARG, "mesh belongs to another context");
+    srv->meshes.push_back(mesh);
+    srv->pool.emplace_back();
+    *id = (int32_t)srv->meshes.size() - 1;
+  })
+}
+
+int gsh_server_submit(gsh_server* srv, int32_t mesh_id, const double* b, int with_bt, int64_t* ticket) {
+  GSH_API_BODY({
+    GSH_REQUIRE(srv && ticket, GSH_E_ARG, "NULL argument");
+    GSH_REQUIRE(mesh_id >= 0 && mesh_id < (int32_t)srv->meshes.size(), GSH_E_ARG, "unknown mesh id");
+    auto r = std::make_unique<Request>();
+    r->ticket = srv->next_ticket++;
+    r->mesh_id = mesh_id;
+    r->with_bt = with_bt != 0;
+    if (b) {
+      r->has_b = true;
+      r->b.assign(b, b + srv->meshes[mesh_id]->hm.n);
+    }
+    r->t_submit = Clock::now();
+    r->bytes = solver_bytes(srv->meshes[mesh_id]);
+    *ticket = r->ticket;
+    srv->queue.push_back(r.get());
+    srv->reqs[r->ticket] = std::move(r);
+  })
+}
+
+int gsh_server_step(gsh_server* srv, int32_t* active_out, int32_t* queued_out) {
+  GSH_API_BODY({
+    GSH_REQUIRE(srv, GSH_E_ARG, "server is NULL");
+    gsh_ctx* ctx = srv->ctx;
+    ctx->set();
+    hipStream_t st = ctx->stream;
+    const auto& cfg = srv->cfg;
+    // ---- admission
+    std::vector<Request*> fresh;
+    while (!srv->queue.empty()) {
+      Request* r = srv->queue.front();
+      if (cfg.max_active > 0 && (int)srv->active.size() >= cfg.max_active) break;
+      if (srv->used + r->bytes > srv->budget) {
+        if (srv->active.empty() && fresh.empty()) {  // can never fit
+          r->status = GSH_E_ADMISSION;
+          srv->queue.pop_front();
+          continue;
+        }
+        break;
+      }
+      srv->queue.pop_front();
+      gsh_mesh* m = srv->meshes[r->mesh_id];
+      auto& idle = srv->pool[r->mesh_id];
+      gsh_solver* s = nullptr;
+      if (!idle.empty()) {
+        s = idle.back();
+        idle.pop_back();
+      } else {
+        if (gsh_solver_create(m, GSH_PREC_FP64, GSH_PRECOND_JACOBI, &s) != GSH_OK)
+          throw Error(GSH_E_OOM, std::string("server admission: ") + g_last_error);
+      }
+      for (auto* hh : s->h) launch_fill_f64(hh, 0.0, s->n, st);  // each request is a cold start
+      if (r->with_bt) {
+        gsh_step_params p;
+        gsh_step_params_default(&p);
+        const StepArgs a = step_args(&p);
+        launch_backtrack(m->d, a, nullptr, nullptr, 0, st);
+        launch_rhs(m->d, a, st);
+      }
+      if (r->has_b) {
+        HIPCHK(hipMemcpyAsync(s->b, r->b.data(), sizeof(double) * s->n, hipMemcpyHostToDevice, st));
+      } else {
+        gsh_step_params p;
+        gsh_step_params_default(&p);
+        launch_assemble(m->d, step_args(&p), s->val, s->b, st);
+      }
+      s->rpt = 4;
+      s->ntiles = s->tiles_for(s->n);
+      s->hs.x = s->x;
+      s->hs.h0 = s->h[0];
+      s->hs.h1 = s->h[1];
+      s->hs.h2 = s->h[2];
+      s->hs.ntiles = s->ntiles;
+      r->solver = s;
+      r->active = true;
+      r->run = 0;
+      srv->used += r->bytes;
+      fresh.push_back(r);
+    }
+    for (auto* r : fresh) srv->active.push_back(r);
+    const int na = (int)srv->active.size();
+    if (na > 0) {
+      // ---- descriptors and tile tables (fresh systems are the last ones)
+      std::vector<PcgSys<double>> hs(na);
+      std::vector<TileRef> tiles, ftiles;
+      for (int i = 0; i < na; ++i) {
+        gsh_solver* s = srv->active[i]->solver;
+        hs[i] = s->hs;
+        const bool is_fresh = i >= na - (int)fresh.size();
+        for (int t = 0; t < s->ntiles; ++t) {
+          tiles.push_back({i, t});
+          if (is_fresh) ftiles.push_back({i, t});
+        }
+      }
+      ensure_cap(srv->dsys, srv->dsys_cap, na);
+      ensure_cap(srv->dtiles, srv->dtiles_cap, (int)tiles.size());
+      ensure_cap(srv->dgather, srv->dgather_cap, na);
+      HIPCHK(hipMemcpyAsync(srv->dsys, hs.data(), sizeof(PcgSys<double>) * na, hipMemcpyHostToDevice, st));
+      HIPCHK(hipMemcpyAsync(srv->dtiles, tiles.data(), sizeof(TileRef) * tiles.size(), hipMemcpyHostToDevice, st));
+      if (!fresh.empty()) {
+        ensure_cap(srv->dfresh, srv->dfresh_cap, (int)ftiles.size());
+        HIPCHK(hipMemcpyAsync(srv->dfresh, ftiles.data(), sizeof(TileRef) * ftiles.size(), hipMemcpyHostToDevice,
+                              st));
+        const int f0 = na - (int)fresh.size();
+        launch_pcg_reset<double>(srv->dsys + f0, (int)fresh.size(), cfg.rtol, cfg.maxiter, cfg.check_every, st);
+        launch_pcg_init<double>(srv->dsys, srv->dfresh, (int)ftiles.size(), 4, st);
+      }
+      // ---- one check window for every active system
+      for (int i = 0; i < cfg.check_every; ++i) launch_pcg_iter<double>(srv->dsys, srv->dtiles, (int)tiles.size(), 4, st);
+      launch_pcg_check<double>(srv->dsys, na, st);
+      for (auto* r : srv->active) r->run += cfg.check_every;
+      hipLaunchKernelGGL(k_gather_states<double>, dim3((na + 63) / 64), dim3(64), 0, st, srv->dsys, na, srv->dgather);
+      std::vector<PcgState> sts(na);
+      HIPCHK(hipMemcpyAsync(sts.data(), srv->dgather, sizeof(PcgState) * na, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipStreamSynchronize(st));
+      HIPCHK(hipGetLastError());
+      // ---- retire finished systems
+      std::vector<Request*> keep;
+      for (int i = 0; i < na; ++i) {
+        Request* r = srv->active[i];
+        const PcgState& s = sts[i];
+        const bool done = s.done >= 0;
+        const bool maxed = r->run >= cfg.maxiter;
+        if (!done && !maxed) {
+          keep.push_back(r);
+          continue;
+        }
+        gsh_solver* sol = r->solver;
+        r->x.resize(sol->n);
+        if (s.zero)
+          std::fill(r->x.begin(), r->x.end(), 0.0);
+        else
+          HIPCHK(hipMemcpy(r->x.data(), sol->x, sizeof(double) * sol->n, hipMemcpyDeviceToHost));
+        r->iters = done ? s.done : cfg.maxiter;
+        r->status = 0;
+        r->active = false;
+        r->latency_ms = std::chrono::duration<double, std::milli>(Clock::now() - r->t_submit).count();
+        srv->pool[r->mesh_id].push_back(sol);
+        r->solver = nullptr;
+        srv->used -= r->bytes;
+      }
+      srv->active.swap(keep);
+    }
+    if (active_out) *active_out = (int32_t)srv->active.size();
+    if (queued_out) *queued_out = (int32_t)srv->queue.size();
+  })
+}
+
+int gsh_server_poll(gsh_server* srv, int64_t ticket, double* x_host, int32_t* iters_out, int32_t* status_out,
+                    double* latency_ms_out) {
+  GSH_API_BODY({
+    GSH_REQUIRE(srv, GSH_E_ARG, "server is NULL");
+    auto it = srv->reqs.find(ticket);
+    GSH_REQUIRE(it != srv->reqs.end(), GSH_E_NOTFOUND, "unknown ticket");
+    Request* r = it->second.get();
+    if (status_out) *status_out = r->status;
+    if (r->status == 0) {
+      if (x_host) std::memcpy(x_host, r->x.data(), sizeof(double) * r->x.size());
+      if (iters_out) *iters_out = r->iters;
+      if (latency_ms_out) *latency_ms_out = r->latency_ms;
+    }
+  })
+}
+
+}  // extern "C"

@@ -1,0 +1,147 @@
+// device.h -- device-resident data layout (SoA in HBM) and launch wrappers of libgsh.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace gsh {
+
+// Wavefront and workgroup geometry of the CDNA kernels.
+constexpr int kWave = 64;
+constexpr int kTpb = 256;              // 4 waves per workgroup
+constexpr int kSpmvLdsCap = 4096;      // products staged per 256-row CSR chunk (32 KiB FP64)
+
+// Mesh state on the device. Per-level arrays are [entity][level].
+struct DMesh {
+  int32_t n, ne, ns, nvrt;
+  // nodes
+  const double* x;
+  const double* y;
+  const double* depth;
+  const int32_t* kb;
+  const double* sigma;      // [nvrt]
+  const int8_t* open_bnd;
+  double* eta2;
+  double* eta1;
+  double* grav2;
+  double* cd;
+  double* eth;
+  double* elbc;
+  double* uvw;              // [n][nvrt][4]: u, v, w, 0
+  int8_t* node_wet;
+  // elements
+  const int4* elem;         // [ne][2]: (n0, n1, n2, s0), (s1, s2, -, -)
+  const double* area;
+  const double* gradphi;    // [ne][3][2]
+  int8_t* elem_wet;
+  int32_t* elem_kb;
+  double* ghat1;            // [ne][2]
+  double* vsource;
+  // sides
+  const int4* side;         // [ns]: (n1, n2, e0, e1)
+  const int8_t* side_type;  // 0 internal, 1 land, 2 open
+  const double* side_cx;
+  const double* side_cy;
+  const double* side_len;
+  int32_t* side_kb;
+  int8_t* side_wet;
+  double* zc;               // [ns][nvrt]
+  double* su2;              // [ns][nvrt][2]
+  double* btu;              // [ns][nvrt][2]
+  int32_t* foot;            // [ns][nvrt]
+  int32_t* bstatus;         // [ns][nvrt]
+  double* sU;               // [ns][2]
+  double* hhat;
+  double* chi;
+  // balls and CSR pattern
+  const int32_t* ball_ptr;
+  const int32_t* ball_elem;
+  const int8_t* ball_loc;
+  const int16_t* ball_slot;  // [ball][2]: CSR slots of the two other element nodes
+  const int32_t* nbr_ptr;
+  const int32_t* nbr_node;
+  const int16_t* csr_slot;
+  const int32_t* indptr;
+  const int32_t* diag_off;
+};
+
+struct StepArgs {
+  double dt, theta, h0, ramp, dtb_min, dtb_max, velmin2;
+  int32_t ihhat, iettype, ndelt_min, ndelt_max;
+};
+
+// ---------------------------------------------------------------- PCG
+// Scalar state of one system, device resident. k / kk are the iteration counters read by
+// the two per-iteration phases (each phase reads one and its workgroup 0 writes the other).
+struct PcgState {
+  int32_t k, kk, done, zero;
+  int32_t maxiter, check_every, pad0, pad1;
+  double rtol, atol2, rr_last, bb;
+  double rho[2];
+};
+
+// One linear system of a (possibly batched) PCG launch. VT is the arithmetic type of the
+// matrix/vector updates; dot products are accumulated in FP64.
+template <class VT>
+struct PcgSys {
+  int32_t n, ntiles;
+  const int32_t* rowptr;
+  const int32_t* col;
+  const VT* val;
+  const int32_t* diag;       // absolute index of the diagonal entry of each row
+  const double* b;           // FP64 right-hand side (MIXED inner solve: FP64 residual)
+  VT* x;
+  VT* r;
+  VT* z;
+  VT* pa;
+  VT* pb;
+  VT* q;
+  VT* invd;
+  const double* h0;          // extrapolation history x0, x1, x2 (FP64), NULL = zero guess
+  const double* h1;
+  const double* h2;
+  double* part_rr;
+  double* part_rz;
+  double* part_pq;
+  double* part_bb;
+  PcgState* st;
+};
+
+// Launch table entry: which system and which tile a workgroup works on.
+struct TileRef {
+  int32_t sys, tile;
+};
+
+// ---------------------------------------------------------------- launch wrappers
+void launch_levels(const DMesh& m, const StepArgs& a, hipStream_t s);
+void launch_backtrack(const DMesh& m, const StepArgs& a, const int32_t* list_side, const int32_t* list_lev,
+                      int32_t count, hipStream_t s);
+void launch_rhs(const DMesh& m, const StepArgs& a, hipStream_t s);
+void launch_assemble(const DMesh& m, const StepArgs& a, double* val, double* b, hipStream_t s);
+void launch_update_elevation(const DMesh& m, const double* x, hipStream_t s);
+
+template <class VT>
+void launch_pcg_reset(const PcgSys<VT>* sys, int nsys, double rtol, int maxiter, int check_every, hipStream_t s);
+template <class VT>
+void launch_pcg_init(const PcgSys<VT>* sys, const TileRef* tiles, int ntiles, int rpt, hipStream_t s);
+template <class VT>
+void launch_pcg_iter(const PcgSys<VT>* sys, const TileRef* tiles, int ntiles, int rpt, hipStream_t s);
+template <class VT>
+void launch_pcg_check(const PcgSys<VT>* sys, int nsys, hipStream_t s);
+template <class VT>
+void launch_pcg_finish(const PcgSys<VT>* sys, const TileRef* tiles, int ntiles, int rpt, hipStream_t s);
+// q = A x (FP64), then per-tile partials of (b - q).(b - q) and b.b for checkSolution.
+void launch_true_residual(const PcgSys<double>* sys, const TileRef* tiles, int ntiles, int rpt, const double* xv,
+                          double* rvec, hipStream_t s);
+void launch_reduce_pair(const double* pa, const double* pb, int ntiles, double* out2, hipStream_t s);
+
+// Mixed precision helpers.
+void launch_cast_values(const double* in, float* out, int64_t count, hipStream_t s);
+void launch_refine_residual(const PcgSys<double>* sys64, const TileRef* tiles, int ntiles, int rpt, double* r64,
+                            hipStream_t s);
+void launch_refine_update(double* x64, const float* d32, int32_t n, hipStream_t s);
+void launch_copy_f64(double* dst, const double* src, int64_t count, hipStream_t s);
+void launch_fill_f64(double* dst, double v, int64_t count, hipStream_t s);
+
+}  // namespace gsh

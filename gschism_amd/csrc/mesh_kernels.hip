@@ -1,0 +1,553 @@
+// mesh_kernels.hip -- gfx950 kernels of the per-step mesh work: vertical levels and wet/dry,
+// ELM backtracking, RHS producers, wave-equation assembly with essential BCs, elevation update.
+//
+// Arithmetic follows the reference's operation order term by term (no contraction: the library
+// is built with -ffp-contract=off), so the CPU oracle reproduces every result bit for bit.
+#include <hip/hip_runtime.h>
+
+#include "device.h"
+
+namespace gsh {
+
+#define GSH_BT_WALK_CAP 1
+#define GSH_BT_ITER38 2
+#define GSH_BT_STUCK 4
+#define GSH_BT_OPEN_EXIT 8
+#define GSH_BT_SLIDE 16
+
+constexpr double kSmall = 1.0e-7;  // meshdata.h:10
+
+// compZCoord for ivcor=1 (meshdata.cu:1744-1807): z of level k at a node.
+__device__ __forceinline__ double zcoord(const DMesh& m, double h0, int node, int k) {
+  const double eta = m.eta2[node], h = m.depth[node];
+  if (eta + h <= h0) return 0.0;  // isDry2
+  const int kb = m.kb[node];
+  if (k <= kb) return -h;
+  if (k == m.nvrt - 1) return eta;
+  return (eta + h) * m.sigma[k] + eta;
+}
+
+// ---------------------------------------------------------------- a3: levels and wet/dry
+// ul0ComputeDryElements (meshdata.cu:5550-5580)
+__global__ void k_elem_levels(DMesh m, double h0) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m.ne) return;
+  const int4 r = m.elem[2 * e];
+  const int nd[3] = {r.x, r.y, r.z};
+  bool dry = false;
+  for (int j = 0; j < 3; ++j) dry = dry || (m.eta2[nd[j]] + m.depth[nd[j]] <= h0);
+  m.elem_wet[e] = dry ? 0 : 1;
+  int kb = 0;
+  if (!dry) kb = min(min(m.kb[nd[0]], m.kb[nd[1]]), m.kb[nd[2]]);
+  m.elem_kb[e] = kb;
+}
+
+// ul0ComputeDryNodes (meshdata.cu:5596-5612)
+__global__ void k_node_wet(DMesh m) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.n) return;
+  int8_t wet = 0;
+  for (int j = m.ball_ptr[i]; j < m.ball_ptr[i + 1]; ++j)
+    if (m.elem_wet[m.ball_elem[j]]) {
+      wet = 1;
+      break;
+    }
+  m.node_wet[i] = wet;
+}
+
+// ul0UpdateSides / ul0UpdateSides2 (meshdata.cu:5679-5760): side kb, zc, wet flag.
+__global__ void k_side_levels(DMesh m, double h0) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= m.ns) return;
+  const int4 r = m.side[s];
+  const bool wet = m.elem_wet[r.z] || (r.w >= 0 && m.elem_wet[r.w]);
+  int kb = 0;
+  if (wet) {
+    const int kb1 = m.kb[r.x], kb2 = m.kb[r.y];
+    kb = min(kb1, kb2);
+    double* zc = m.zc + (int64_t)s * m.nvrt;
+    for (int k = 0; k < kb; ++k) zc[k] = 0.0;
+    for (int k = kb; k < m.nvrt; ++k)
+      zc[k] = 0.5 * (zcoord(m, h0, r.x, max(k, kb1)) + zcoord(m, h0, r.y, max(k, kb2)));
+  }
+  m.side_kb[s] = kb;
+  m.side_wet[s] = wet ? 1 : 0;
+}
+
+void launch_levels(const DMesh& m, const StepArgs& a, hipStream_t st) {
+  if (m.ne > 0) hipLaunchKernelGGL(k_elem_levels, dim3((m.ne + kTpb - 1) / kTpb), dim3(kTpb), 0, st, m, a.h0);
+  if (m.n > 0) hipLaunchKernelGGL(k_node_wet, dim3((m.n + kTpb - 1) / kTpb), dim3(kTpb), 0, st, m);
+  if (m.ns > 0) hipLaunchKernelGGL(k_side_levels, dim3((m.ns + kTpb - 1) / kTpb), dim3(kTpb), 0, st, m, a.h0);
+}
+
+// ---------------------------------------------------------------- a4-a7: ELM backtracking
+__device__ __forceinline__ double signed_area(double p1x, double p1y, double p2x, double p2y, double p3x,
+                                              double p3y) {
+  return 0.5 * ((p1x - p3x) * (p2y - p3y) - (p2x - p3x) * (p1y - p3y));
+}
+
+// Element::interpolateVelocities for triangles, ics=1 (meshdata.cu:1209-1291) with
+// computeShapeCoords(fiAnywhere) (meshdata.cu:990-1043).
+__device__ void interp_velocity(const DMesh& m, double h0, int e, double px, double py, double pz,
+                                double& uo, double& vo, double& wo) {
+  const int4 r = m.elem[2 * e];
+  const int n0 = r.x, n1 = r.y, n2 = r.z;
+  const double x0 = m.x[n0], y0 = m.y[n0], x1 = m.x[n1], y1 = m.y[n1], x2 = m.x[n2], y2 = m.y[n2];
+  const double area = m.area[e];
+  double s0 = signed_area(px, py, x1, y1, x2, y2) / area;
+  double s1 = signed_area(x0, y0, px, py, x2, y2) / area;
+  double s2 = 1.0 - s0 - s1;
+  const double tmin = fmin(s0, fmin(s1, s2));
+  if (tmin < 0.0) {
+    const double aa = -tmin + 1.0e-8;
+    const double bb = 1.0 / (s0 + s1 + s2 + 3.0 * aa);
+    s0 = (s0 + aa) * bb;
+    s1 = (s1 + aa) * bb;
+    s2 = (s2 + aa) * bb;
+  }
+  const int kb0 = m.kb[n0], kb1 = m.kb[n1], kb2 = m.kb[n2];
+  const int nv = m.nvrt;
+  int L = m.elem_kb[e] - 1, R = nv - 1;
+  double zL = s0 * zcoord(m, h0, n0, max(L, kb0)) + s1 * zcoord(m, h0, n1, max(L, kb1)) +
+              s2 * zcoord(m, h0, n2, max(L, kb2)) + 0.0;
+  double zR = s0 * zcoord(m, h0, n0, max(R, kb0)) + s1 * zcoord(m, h0, n1, max(R, kb1)) +
+              s2 * zcoord(m, h0, n2, max(R, kb2)) + 0.0;
+  double w1, w2;
+  if (pz <= zL) {
+    w1 = 1.0;
+    w2 = 0.0;
+  } else if (pz >= zR) {
+    w1 = 0.0;
+    w2 = 1.0;
+  } else {
+    while (R - L > 1) {
+      const int M = (L + R) / 2;
+      const double zM = s0 * zcoord(m, h0, n0, max(M, kb0)) + s1 * zcoord(m, h0, n1, max(M, kb1)) +
+                        s2 * zcoord(m, h0, n2, max(M, kb2)) + 0.0;
+      if (zM < pz) {
+        L = M;
+        zL = zM;
+      } else {
+        R = M;
+        zR = zM;
+      }
+    }
+    w2 = (pz - zL) / (zR - zL);
+    w1 = 1.0 - w2;
+  }
+  const double4* U = reinterpret_cast<const double4*>(m.uvw);
+  const double4 a0 = U[(int64_t)n0 * nv + max(L, kb0)], a1 = U[(int64_t)n1 * nv + max(L, kb1)],
+                a2 = U[(int64_t)n2 * nv + max(L, kb2)];
+  const double4 b0 = U[(int64_t)n0 * nv + max(R, kb0)], b1 = U[(int64_t)n1 * nv + max(R, kb1)],
+                b2 = U[(int64_t)n2 * nv + max(R, kb2)];
+  const double uL = s0 * a0.x + s1 * a1.x + s2 * a2.x + 0.0, vL = s0 * a0.y + s1 * a1.y + s2 * a2.y + 0.0;
+  const double uR = s0 * b0.x + s1 * b1.x + s2 * b2.x + 0.0, vR = s0 * b0.y + s1 * b1.y + s2 * b2.y + 0.0;
+  const double wL = s0 * a0.z + s1 * a1.z + s2 * a2.z + 0.0, wR = s0 * b0.z + s1 * b1.z + s2 * b2.z + 0.0;
+  uo = w1 * uL + w2 * uR;
+  vo = w1 * vL + w2 * vR;
+  wo = w1 * wL + w2 * wR;
+}
+
+// Leaded celestial walk (meshdata.cu:85-127) with Element::computePredicates (meshdata.cu:651-746).
+// Corner j of a triangle is getNodeInOrder(j) = nodes[(j+1)%3]; walk side j is element side j.
+__device__ bool celestial_walk(const DMesh& m, int& cur, double x0x, double x0y, double xtx, double xty,
+                               int& last_cross, int& status) {
+  double fac = 0.00001;
+  for (int iter = 0; iter < 40; ++iter) {
+    const int4 r0 = m.elem[2 * cur], r1 = m.elem[2 * cur + 1];
+    const double C0x = m.x[r0.y], C0y = m.y[r0.y];
+    const double C1x = m.x[r0.z], C1y = m.y[r0.z];
+    const double C2x = m.x[r0.x], C2y = m.y[r0.x];
+    const double vx = xtx - x0x, vy = xty - x0y;
+    const double d0 = vx * (C0x - x0x) + vy * (C0y - x0y);
+    const double d1 = vx * (C1x - x0x) + vy * (C1y - x0y);
+    const double d2 = vx * (C2x - x0x) + vy * (C2y - x0y);
+    double maxp = fmax(d0, d1);
+    maxp = fmax(maxp, d2);
+    fac = fmax(1.00001 * fac, fmin(1.0, (maxp / (vx * vx + vy * vy)) * 1.00001));
+    const double ix = x0x + (xtx - x0x) * fac, iy = x0y + (xty - x0y) * fac;
+    const double s0x = C1x - C0x, s0y = C1y - C0y;
+    const double s1x = C2x - C1x, s1y = C2y - C1y;
+    const double s2x = C0x - C2x, s2y = C0y - C2y;
+    const bool left0 = (s0x * (iy - C0y) - s0y * (ix - C0x)) > -kSmall;
+    const bool left1 = (s1x * (iy - C1y) - s1y * (ix - C1x)) > -kSmall;
+    const bool left2 = (s2x * (iy - C2y) - s2y * (ix - C2x)) > -kSmall;
+    const bool obt0 = (s0x * s1x + s0y * s1y) > 0.0;
+    const bool obt1 = (s1x * s2x + s1y * s2y) > 0.0;
+    const bool obt2 = (s2x * s0x + s2y * s0y) > 0.0;
+    const bool bis0 = ((ix - C1x) * (s0x + s1x) + (iy - C1y) * (s0y + s1y)) < 0.0;
+    const bool bis1 = ((ix - C2x) * (s1x + s2x) + (iy - C2y) * (s1y + s2y)) < 0.0;
+    const bool bis2 = ((ix - C0x) * (s2x + s0x) + (iy - C0y) * (s2y + s0y)) < 0.0;
+    const bool v0 = !left0 && (!obt0 || bis0);
+    const bool v1 = !left1 && (!obt1 || bis1);
+    const bool v2 = !left2 && (!obt2 || bis2);
+    if (!(v0 || v1 || v2)) return true;
+    const int side = v0 ? r0.w : (v1 ? r1.x : r1.y);
+    last_cross = side;
+    const int4 sr = m.side[side];
+    const int nxt = (sr.z == cur) ? sr.w : sr.z;
+    if (nxt < 0 || !m.elem_wet[nxt]) return false;
+    cur = nxt;
+  }
+  status |= GSH_BT_WALK_CAP;
+  return true;
+}
+
+// Side::btrack (meshdata.cu:1449-1629). Returns the final velocity and the foot element.
+__device__ void btrack(const DMesh& m, const StepArgs& a, int s, int k, double dtbk, int cur, double& ux,
+                       double& uy, int& foot, int& status) {
+  const double h0 = a.h0;
+  double x0x = m.side_cx[s], x0y = m.side_cy[s], x0z = m.zc[(int64_t)s * m.nvrt + k];
+  double time_rm = a.dt, t1 = 1e10, t2 = 2e10, t3 = 3e10;
+  const int4 sr = m.side[s];
+  const double4* U = reinterpret_cast<const double4*>(m.uvw);
+  double uix = m.su2[2 * ((int64_t)s * m.nvrt + k)], uiy = m.su2[2 * ((int64_t)s * m.nvrt + k) + 1];
+  double uiz = 0.5 * (U[(int64_t)sr.x * m.nvrt + k].z + U[(int64_t)sr.y * m.nvrt + k].z);
+  double xtx = -1.0e25, xty = -1.0e25, xtz = -1.0e25;
+  int iter = 0;
+  while (true) {
+    const double dtb = fmin(dtbk, time_rm);
+    xtx = x0x - dtb * uix;
+    xty = x0y - dtb * uiy;
+    xtz = x0z - dtb * uiz;
+    int last = -1;
+    const bool found = celestial_walk(m, cur, x0x, x0y, xtx, xty, last, status);
+    if (t3 <= time_rm) {
+      xtx = x0x;
+      xty = x0y;
+      xtz = x0z;
+      status |= GSH_BT_STUCK;
+      break;
+    }
+    if (!found) {
+      const int4 cs = m.side[last];
+      const double p1x = m.x[cs.x], p1y = m.y[cs.x], p2x = m.x[cs.y], p2y = m.y[cs.y];
+      const double v1x = xtx - x0x, v1y = xty - x0y;
+      const double v2x = p2x - p1x, v2y = p2y - p1y;
+      const double denom = v2x * v1y - v2y * v1x;
+      double t, sp;
+      if (fabs(denom) < kSmall) {
+        t = fmin(fmax(((xtx - p1x) * v2x + (xty - p1y) * v2y) / (v2x * v2x + v2y * v2y), 0.0), 1.0);
+        if (t < 0.0000000001) t = 0.0;
+        if (t > 0.9999999999) t = 1.0;
+        sp = fmin(fmax(((p1x + t * v2x - x0x) * v1x + (p1y + t * v2y - x0y) * v1y) / (v1x * v1x + v1y * v1y), 0.0),
+                  1.0);
+      } else {
+        t = fmin(fmax(((x0x - p1x) * v1y - (x0y - p1y) * v1x) / denom, 0.0), 1.0);
+        if (t < 0.0000000001) t = 0.0;
+        if (t > 0.9999999999) t = 1.0;
+        sp = fmin(fmax(((p1x + t * v2x - x0x) * v1x + (p1y + t * v2y - x0y) * v1y) / (v1x * v1x + v1y * v1y), 0.0),
+                  1.0);
+        if (sp < 0.0000000001) sp = 0.0;
+      }
+      time_rm -= sp * dtb;
+      const double nz = x0z + sp * (xtz - x0z);
+      xtx = p1x + t * v2x;
+      xty = p1y + t * v2y;
+      xtz = nz;
+      if (m.side_type[last] == 2) {
+        status |= GSH_BT_OPEN_EXIT;
+        break;
+      }
+      const double len = sqrt(v2x * v2x + v2y * v2y);
+      const double tx = v2x / len, ty = v2y / len;
+      double iu, iv, iw;
+      interp_velocity(m, h0, cur, xtx, xty, xtz, iu, iv, iw);
+      const double d = iu * tx + iv * ty;
+      uix = tx * d;
+      uiy = ty * d;
+      uiz = iw;
+      status |= GSH_BT_SLIDE;
+    } else {
+      time_rm -= dtb;
+      interp_velocity(m, h0, cur, xtx, xty, xtz, uix, uiy, uiz);
+    }
+    if (time_rm <= 0.0) break;
+    if (uix * uix + uiy * uiy <= a.velmin2) {
+      ux = uix;
+      uy = uiy;
+      foot = cur;
+      return;
+    }
+    x0x = xtx;
+    x0y = xty;
+    x0z = xtz;
+    ++iter;
+    if (iter == 38) {
+      status |= GSH_BT_ITER38;
+      ux = uix;
+      uy = uiy;
+      foot = cur;
+      return;
+    }
+    t3 = t2;
+    t2 = t1;
+    t1 = time_rm;
+  }
+  double iw;
+  interp_velocity(m, h0, cur, xtx, xty, xtz, ux, uy, iw);
+  foot = cur;
+}
+
+// backtrackAllNodes (meshdata.cu:3547-3660), one thread per (side, level): consecutive
+// lanes are consecutive levels of one side, so the nodal [node][level] gathers of a wave
+// fall on a few contiguous cache lines.
+__global__ void __launch_bounds__(256) k_backtrack(DMesh m, StepArgs a, const int32_t* list_side,
+                                                   const int32_t* list_lev, int32_t count) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int s, k;
+  if (list_side) {
+    if (t >= count) return;
+    s = list_side[t];
+    k = list_lev[t];
+  } else {
+    if (t >= (int64_t)m.ns * m.nvrt) return;
+    s = (int)(t / m.nvrt);
+    k = (int)(t - (int64_t)s * m.nvrt);
+  }
+  const int64_t idx = (int64_t)s * m.nvrt + k;
+  const double ux0 = m.su2[2 * idx], uy0 = m.su2[2 * idx + 1];
+  double bx = ux0, by = uy0;
+  int foot = -1, status = 0;
+  if (m.side_wet[s] && k >= m.side_kb[s] && !(ux0 * ux0 + uy0 * uy0 <= a.velmin2)) {
+    const int4 sr = m.side[s];
+    const double4* U = reinterpret_cast<const double4*>(m.uvw);
+    double suma = 0.0;
+    int icount = 0;
+    for (int l = 0; l < 2; ++l) {
+      const int e = (l == 0) ? sr.z : sr.w;
+      if (e < 0) continue;
+      ++icount;
+      const int4 r = m.elem[2 * e];
+      const double4 q0 = U[(int64_t)r.x * m.nvrt + k], q1 = U[(int64_t)r.y * m.nvrt + k],
+                    q2 = U[(int64_t)r.z * m.nvrt + k];
+      const double* g = m.gradphi + 6 * (int64_t)e;
+      const double gux = q0.x * g[0] + q1.x * g[2] + q2.x * g[4];
+      const double guy = q0.x * g[1] + q1.x * g[3] + q2.x * g[5];
+      const double gvx = q0.y * g[0] + q1.y * g[2] + q2.y * g[4];
+      const double gvy = q0.y * g[1] + q1.y * g[3] + q2.y * g[5];
+      suma += a.dt * sqrt((gux * gux + guy * guy) + (gvx * gvx + gvy * gvy));
+    }
+    const int nd = (int)(suma / icount) * 4;
+    const int ndelt = max(a.ndelt_min, min(nd, a.ndelt_max));
+    const double dtbk = a.dt / ndelt;
+    const int start = m.elem_wet[sr.z] ? sr.z : sr.w;
+    btrack(m, a, s, k, dtbk, start, bx, by, foot, status);
+  }
+  m.btu[2 * idx] = bx;
+  m.btu[2 * idx + 1] = by;
+  m.foot[idx] = foot;
+  m.bstatus[idx] = status;
+}
+
+void launch_backtrack(const DMesh& m, const StepArgs& a, const int32_t* ls, const int32_t* ll, int32_t count,
+                      hipStream_t st) {
+  const int64_t total = ls ? (int64_t)count : (int64_t)m.ns * m.nvrt;
+  if (total <= 0) return;
+  hipLaunchKernelGGL(k_backtrack, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, m, a, ls, ll, count);
+}
+
+// ---------------------------------------------------------------- a8-a10, elevation BC
+// computeVerticallyIntegratedVelocity (meshdata.cu:4459-4476) and
+// computeVegetationAndDragParameters for iveg=0, nchi=0 (meshdata.cu:4615-4725).
+__global__ void k_side_rhs(DMesh m, StepArgs a) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= m.ns) return;
+  const int4 r = m.side[s];
+  const int nv = m.nvrt;
+  double Ux = 0.0, Uy = 0.0, hhat = 0.0, chi = 0.0;
+  if (m.side_wet[s]) {
+    const int kb = m.side_kb[s];
+    const double* zc = m.zc + (int64_t)s * nv;
+    const double* u2 = m.su2 + 2 * (int64_t)s * nv;
+    for (int k = kb; k < nv - 1; ++k) {
+      const double ucx = 0.5 * (u2[2 * k] + u2[2 * (k + 1)]);
+      const double ucy = 0.5 * (u2[2 * k + 1] + u2[2 * (k + 1) + 1]);
+      const double dz = zc[k + 1] - zc[k];
+      Ux += dz * ucx;
+      Uy += dz * ucy;
+    }
+    const double htot = 0.5 * (m.eta2[r.x] + m.eta2[r.y]) + 0.5 * (m.depth[r.x] + m.depth[r.y]);
+    const double* bt = m.btu + 2 * (int64_t)s * nv;
+    const double bx = bt[2 * (kb + 1)], by = bt[2 * (kb + 1) + 1];
+    const double vmag1 = sqrt(bx * bx + by * by);
+    const double cd = 0.5 * (m.cd[r.x] + m.cd[r.y]);
+    const double chi2 = cd * vmag1;
+    chi = chi2;
+    if (kb == nv - 2) {
+      const double fricH = htot + chi2 * a.dt;
+      hhat = htot * htot / fricH;
+    } else {
+      hhat = htot - chi * a.dt;
+      if (a.ihhat == 1) hhat = fmax(0.0, hhat);
+    }
+  }
+  m.sU[2 * s] = Ux;
+  m.sU[2 * s + 1] = Uy;
+  m.hhat[s] = hhat;
+  m.chi[s] = chi;
+}
+
+// computeElemAverageGhat (meshdata.cu:6183-6424) for the barotropic, ics=1, iveg=0 subset with
+// no Coriolis, wind stress, horizontal viscosity, atmospheric pressure or tidal potential:
+// those terms are identically zero here and are kept as explicit zero operands.
+__global__ void k_elem_ghat(DMesh m, StepArgs a) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m.ne) return;
+  double gx = 0.0, gy = 0.0;
+  if (m.elem_wet[e]) {
+    const int nv = m.nvrt;
+    const int4 r0 = m.elem[2 * e], r1 = m.elem[2 * e + 1];
+    const int sides[3] = {r0.w, r1.x, r1.y};
+    const double dt = a.dt;
+    for (int j = 0; j < 3; ++j) {
+      const int s = sides[j];
+      const int4 sr = m.side[s];
+      const double chi = m.chi[s];
+      const double htot = 0.5 * (m.eta2[sr.x] + m.eta2[sr.y]) + 0.5 * (m.depth[sr.x] + m.depth[sr.y]);
+      const int kb = m.side_kb[s];
+      double cff1, cff2, cff3;
+      if (kb == nv - 2) {
+        cff1 = m.hhat[s] / htot;
+        cff2 = 0.0;
+        cff3 = 0.0;
+      } else {
+        cff1 = 1.0;
+        cff2 = 1.0;
+        cff3 = 0.0;
+      }
+      const double* zc = m.zc + (int64_t)s * nv;
+      const double* bt = m.btu + 2 * (int64_t)s * nv;
+      double usx = 0.0, usy = 0.0;
+      if (kb == nv - 2) {
+        const double dz = zc[nv - 1] - zc[nv - 2];
+        usx += bt[2 * (nv - 1)] * dz;
+        usy += bt[2 * (nv - 1) + 1] * dz;
+      } else {
+        for (int k = kb + 1; k < nv; ++k) {
+          const double dz = zc[k] - zc[k - 1];
+          usx += (bt[2 * k] + bt[2 * (k - 1)]) / 2.0 * dz;
+          usy += (bt[2 * k + 1] + bt[2 * (k - 1) + 1]) / 2.0 * dz;
+        }
+      }
+      gx += cff1 * usx - cff3 * 0.0;
+      gy += cff1 * usy - cff3 * 0.0;
+      const double ubx = bt[2 * (kb + 1)], uby = bt[2 * (kb + 1) + 1];
+      gx -= chi * dt * cff2 * ubx + dt * cff1 * 0.0;
+      gy -= chi * dt * cff2 * uby + dt * cff1 * 0.0;
+    }
+    gx = (1.0 / 3) * gx;
+    gy = (1.0 / 3) * gy;
+  }
+  m.ghat1[2 * e] = gx;
+  m.ghat1[2 * e + 1] = gy;
+}
+
+// computeElevationBoundaryConditions for iettype 1, 2, 4 without ramp-down of an initial
+// elevation (nramp_elev = 0) (meshdata.cu:4276-4339).
+__global__ void k_elbc(DMesh m, StepArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.n) return;
+  if (!m.open_bnd[i]) return;
+  const double eic = 0.0;
+  const double r_eic = (1.0 - a.ramp) * eic;
+  const double r_eth = a.ramp * m.eth[i];
+  m.elbc[i] = r_eic + r_eth;
+}
+
+void launch_rhs(const DMesh& m, const StepArgs& a, hipStream_t st) {
+  if (m.n > 0) hipLaunchKernelGGL(k_elbc, dim3((m.n + kTpb - 1) / kTpb), dim3(kTpb), 0, st, m, a);
+  if (m.ns > 0) hipLaunchKernelGGL(k_side_rhs, dim3((m.ns + kTpb - 1) / kTpb), dim3(kTpb), 0, st, m, a);
+  if (m.ne > 0) hipLaunchKernelGGL(k_elem_ghat, dim3((m.ne + kTpb - 1) / kTpb), dim3(kTpb), 0, st, m, a);
+}
+
+// ---------------------------------------------------------------- a11+a12: assembly + BC
+// setupCoefficientMatrixAndRHS for triangles, imm=0 (meshdata.cu:3182-3311), fused with
+// setEssentialBC (meshdata.cu:4400-4446). One thread per node owns its CSR row (node-gather,
+// no atomics). Open boundaries carry an essential BC here, so the natural-BC integrals I3/I5
+// of open nodes (meshdata.cu:3380-3470) are overwritten and not evaluated.
+__global__ void __launch_bounds__(256) k_assemble(DMesh m, StepArgs a, double* __restrict__ val,
+                                                  double* __restrict__ b) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.n) return;
+  const int row = m.indptr[i];
+  const int len = m.indptr[i + 1] - row;
+  double* rv = val + row;
+  for (int k = 0; k < len; ++k) rv[k] = 0.0;
+  double qel = 0.0;
+  const int diag = m.diag_off[i];
+  const double dt = a.dt, th = a.theta;
+  for (int j = m.ball_ptr[i]; j < m.ball_ptr[i + 1]; ++j) {
+    const int e = m.ball_elem[j];
+    const int id1 = m.ball_loc[j];
+    const int4 r0 = m.elem[2 * e], r1 = m.elem[2 * e + 1];
+    const int nd[3] = {r0.x, r0.y, r0.z};
+    const int sd[3] = {r0.w, r1.x, r1.y};
+    const double grav3 = (m.grav2[nd[0]] + m.grav2[nd[1]] + m.grav2[nd[2]] + 0.0) / 3;
+    const double hhatb = (m.hhat[sd[0]] + m.hhat[sd[1]] + m.hhat[sd[2]] + 0.0) / 3;
+    const double area = m.area[e];
+    const double Ux = (m.sU[2 * sd[0]] + m.sU[2 * sd[1]] + m.sU[2 * sd[2]] + 0.0) / 3;
+    const double Uy = (m.sU[2 * sd[0] + 1] + m.sU[2 * sd[1] + 1] + m.sU[2 * sd[2] + 1] + 0.0) / 3;
+    const double* g = m.gradphi + 6 * (int64_t)e;
+    const double g1x = g[2 * id1], g1y = g[2 * id1 + 1];
+    const double I41 = g1x * Ux + g1y * Uy;
+    const double I42 = g1x * m.ghat1[2 * e] + g1y * m.ghat1[2 * e + 1];
+    qel += dt * area * ((1 - th) * I41 + th * I42);
+    const double fric = grav3 * th * (1.0 - th) * dt * dt * hhatb;
+    const int id2 = (id1 + 1) % 3, id3 = (id1 + 2) % 3;
+    const double x0x = m.x[nd[id1]], x0y = m.y[nd[id1]];
+    const double x1x = m.x[nd[id2]], x1y = m.y[nd[id2]];
+    const double x2x = m.x[nd[id3]], x2y = m.y[nd[id3]];
+    const double ipp_ii = (1.0 / 6.0) * area, ipp_ij = (1.0 / 12.0) * area, ipp_ik = (1.0 / 12.0) * area;
+    const double ax = x2x - x1x, ay = x2y - x1y;
+    const double igpgp_ii = ((1.0 / 4.0) / area) * (ax * ax + ay * ay);
+    const double igpgp_ij = ((1.0 / 4.0) / area) * (ax * (x0x - x2x) + ay * (x0y - x2y));
+    const double igpgp_ik = -igpgp_ii - igpgp_ij;
+    const double c = grav3 * th * th * dt * dt * hhatb;
+    rv[diag] += ipp_ii + c * igpgp_ii;
+    rv[m.ball_slot[2 * j]] += ipp_ij + c * igpgp_ij;
+    rv[m.ball_slot[2 * j + 1]] += ipp_ik + c * igpgp_ik;
+    const double wet = m.elem_wet[e] ? 1.0 : 0.0;
+    for (int l = 0; l < 3; ++l) {
+      const double eta = m.eta2[nd[l]];
+      const double ipp = ((id1 == l) ? (2.0 / 12.0) : (1.0 / 12.0)) * area;
+      const double igpgp = area * (g1x * g[2 * l] + g1y * g[2 * l + 1]);
+      qel += (eta + 0.0) * ipp - eta * fric * igpgp * wet;
+    }
+    qel += (dt / 3) * m.vsource[e];
+  }
+  const bool ess = a.iettype != 0;
+  if (ess && m.open_bnd[i]) {
+    for (int k = 0; k < len; ++k) rv[k] = 0.0;
+    rv[diag] = 1.0;
+    qel = m.elbc[i];
+  } else if (ess) {
+    for (int j = m.nbr_ptr[i]; j < m.nbr_ptr[i + 1]; ++j) {
+      const int nb = m.nbr_node[j];
+      if (m.open_bnd[nb]) {
+        const int sl = m.csr_slot[j];
+        qel -= rv[sl] * m.elbc[nb];
+        rv[sl] = 0.0;
+      }
+    }
+  }
+  b[i] = qel;
+}
+
+void launch_assemble(const DMesh& m, const StepArgs& a, double* val, double* b, hipStream_t st) {
+  if (m.n > 0) hipLaunchKernelGGL(k_assemble, dim3((m.n + kTpb - 1) / kTpb), dim3(kTpb), 0, st, m, a, val, b);
+}
+
+// rollUpElevation + updateElevation (meshdata.cu:3538-3544, 4449-4456)
+__global__ void k_update_elevation(DMesh m, const double* __restrict__ x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.n) return;
+  m.eta1[i] = m.eta2[i];
+  m.eta2[i] = x[i];
+}
+
+void launch_update_elevation(const DMesh& m, const double* x, hipStream_t st) {
+  if (m.n > 0) hipLaunchKernelGGL(k_update_elevation, dim3((m.n + kTpb - 1) / kTpb), dim3(kTpb), 0, st, m, x);
+}
+
+}  // namespace gsh

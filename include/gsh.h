@@ -1,0 +1,288 @@
+/*
+ * gsh.h -- C ABI of the MI355X-native SCHISM wave-equation hot path.
+ *
+ * One per-timestep path of SCHISM/gschism is rebuilt here:
+ *   - ELM backtracking from side centres     (reference: src/kernels/meshdata.cu:3547-3660, 1449-1629)
+ *   - RHS producers U, chi/H^, G^            (meshdata.cu:4459-4476, 4615-4725, 6183-6424)
+ *   - wave-equation assembly + essential BC  (meshdata.cu:3182-3474, 4276-4339, 4400-4446)
+ *   - Jacobi-preconditioned CG solve         (src/solver/pcg_solver.py:1075-1124, pcg_loop.cpp:334-416)
+ *   - elevation update                       (meshdata.cu:3538-3544, 4449-4456)
+ *
+ * Conventions
+ *   - Every entry point returns GSH_OK (0) or a GSH_E_* status; the message of the last
+ *     failure on the calling thread is returned by gsh_last_error(). No C++ exception crosses
+ *     this boundary.
+ *   - Plain pointers and sizes only. Pointers named *_host are host memory; pointers returned
+ *     by *_ptr / gsh_solver_csr are device (HBM) addresses on the context's GPU.
+ *   - Node, element and side indices are 0-based. Per-level arrays are stored entity-major,
+ *     level-minor ([entity][level]) as in the reference's Node/Side structs
+ *     (meshdata.h:1577-1580, 1324-1325).
+ *   - A handle is single-stream and not re-entrant; distinct handles may be used from
+ *     different threads.
+ */
+#ifndef GSH_H
+#define GSH_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status codes */
+#define GSH_OK            0
+#define GSH_E_ARG         1   /* invalid argument (reference: ValueError / RuntimeError on bad input) */
+#define GSH_E_DEVICE      2   /* HIP runtime failure (reference: cuCheck, pcg_loop.cpp:46-54) */
+#define GSH_E_OOM         3   /* device allocation failed */
+#define GSH_E_TIMEOUT     4   /* GPU did not finish in time (reference: waitStream, pcg_loop.cpp:444-456) */
+#define GSH_E_ADMISSION   5   /* serving: request does not fit the device memory budget */
+#define GSH_E_TOPOLOGY    6   /* broken ball / hanging node / orientation (read_and_process_grid.cpp:47-58, 154-159, 334-341) */
+#define GSH_E_STATE       7   /* call out of order (e.g. solve before a matrix exists) */
+#define GSH_E_NOTFOUND    8   /* serving: unknown ticket */
+#define GSH_W_MAXITER     64  /* flag OR-ed into *status_out of a solve that stopped at maxiter
+                                 (the reference returns maxiter silently, pcg_solver.py:1068) */
+
+#define GSH_PREC_FP64     0   /* plain FP64 PCG */
+#define GSH_PREC_MIXED    1   /* FP32 SpMV/precond/axpy, FP64 dots + true residual + refinement */
+
+#define GSH_PRECOND_JACOBI 0  /* reference preconditioner (pcg_solver.py:933-938) */
+
+/* ---------------------------------------------------------------- context */
+typedef struct gsh_ctx gsh_ctx;
+typedef struct gsh_mesh gsh_mesh;
+typedef struct gsh_solver gsh_solver;
+
+int  gsh_ctx_create(int device, gsh_ctx** out);
+int  gsh_ctx_destroy(gsh_ctx* ctx);
+/* Message of the last failure on this thread ("" if none). */
+const char* gsh_last_error(void);
+/* Library version as 10000*major + 100*minor + patch. */
+int  gsh_version(void);
+/* Device memory figures of the context's GPU, in bytes. */
+int  gsh_ctx_mem_info(gsh_ctx* ctx, int64_t* free_bytes, int64_t* total_bytes);
+/* Waits until all work issued on the context's stream has finished. */
+int  gsh_ctx_synchronize(gsh_ctx* ctx);
+
+/* ---------------------------------------------------------------- mesh */
+/* Host description of a triangular mesh. Replaces the hgrid/vgrid arrays that
+ * readHGrid/readVGrid hand to populateNodes/populateSides/populateElements
+ * (read_and_process_grid.cpp:829-1249, meshdata.cu:1903-2462). */
+typedef struct gsh_mesh_desc {
+  int32_t n;              /* nodes */
+  int32_t ne;             /* triangles */
+  int32_t nvrt;           /* vertical levels (>= 2) */
+  const double*  xy;      /* [n][2] node coordinates (ics=1 Cartesian, metres) */
+  const int32_t* elnode;  /* [ne][3] counter-clockwise node ids */
+  const double*  depth;   /* [n] depth h (positive down, metres) */
+  const int8_t*  open_bnd;/* [n] 1 = node on an open boundary with an elevation (essential) BC; NULL = none */
+  const double*  sigma;   /* [nvrt] sigma levels, sigma[0]=-1 ... sigma[nvrt-1]=0 (ivcor=1, meshdata.cu:1798-1803) */
+  const int32_t* kb;      /* [n] bottom level index per node; NULL = 0 everywhere */
+  int32_t renumber;       /* 1 = reverse Cuthill-McKee renumbering of nodes (elements follow) */
+} gsh_mesh_desc;
+
+typedef struct gsh_mesh_info {
+  int32_t n, ne, ns, nvrt;
+  int64_t nnz;            /* CSR nonzeros = sum(numNodes+1) (main.py:262-279) */
+  int32_t max_row;        /* longest CSR row */
+  int32_t n_open;         /* nodes with an essential BC */
+} gsh_mesh_info;
+
+/* Builds the mesh on the device: topology (CCW balls, sides, CSR pattern) exactly as
+ * read_and_process_grid.cpp:27-174, 272-437, 496-574 and meshdata.cu:2094-2141 define it,
+ * element/side geometry (meshdata.cu:2274-2321, 2350-2462). With desc->renumber the node
+ * order is RCM; node_perm_out[new] = old node id and elem_perm_out[new] = old element id
+ * (either may be NULL). */
+int gsh_mesh_create(gsh_ctx* ctx, const gsh_mesh_desc* desc, gsh_mesh** out,
+                    int32_t* node_perm_out, int32_t* elem_perm_out);
+int gsh_mesh_destroy(gsh_mesh* mesh);
+int gsh_mesh_get_info(const gsh_mesh* mesh, gsh_mesh_info* out);
+
+/* Topology arrays (host copies), selected by GSH_T_*; sizes from gsh_mesh_get_info. */
+#define GSH_T_XY           0   /* double [n][2] (renumbered)                          */
+#define GSH_T_ELNODE       1   /* int32  [ne][3]                                       */
+#define GSH_T_ELSIDE       2   /* int32  [ne][3]  side j is opposite node j            */
+#define GSH_T_ISDEL        3   /* int32  [ns][2]  elements of a side, -1 = none        */
+#define GSH_T_ISIDENODE    4   /* int32  [ns][2]                                       */
+#define GSH_T_BALL_PTR     5   /* int32  [n+1]    CCW ball of elements                 */
+#define GSH_T_BALL_ELEM    6   /* int32  [ball_ptr[n]]                                 */
+#define GSH_T_NBR_NODE     7   /* int32  [ball-node count]  (row i spans indptr[i]-i .. indptr[i+1]-i-1) */
+#define GSH_T_CSR_INDPTR   8   /* int32  [n+1]                                         */
+#define GSH_T_CSR_INDICES  9   /* int32  [nnz]                                         */
+#define GSH_T_CSR_SLOT    10   /* int16  [nnz-n]  slot of neighbour j in its CSR row   */
+#define GSH_T_DEPTH       11   /* double [n] (renumbered)                              */
+#define GSH_T_OPEN_BND    12   /* int8   [n]                                           */
+#define GSH_T_KB          13   /* int32  [n]                                           */
+#define GSH_T_SIDE_TYPE   14   /* int8   [ns] 0 internal, 1 land boundary, 2 open boundary */
+int gsh_mesh_get_topology(const gsh_mesh* mesh, int what, void* host_out);
+
+/* Per-entity state fields (device resident, host copies in and out). */
+#define GSH_F_ETA2        0   /* double [n]          elevation at the current step     */
+#define GSH_F_ETA1        1   /* double [n]          elevation at the previous step    */
+#define GSH_F_NODE_UVW    2   /* double [n][nvrt][3] nodal u2.x, u2.y, w2              */
+#define GSH_F_SIDE_U2     3   /* double [ns][nvrt][2] side velocity u2                 */
+#define GSH_F_BTU         4   /* double [ns][nvrt][2] backtracked velocity (output)    */
+#define GSH_F_FOOT        5   /* int32  [ns][nvrt]   foot element of each backtrack, -1 = not tracked */
+#define GSH_F_BSTATUS     6   /* int32  [ns][nvrt]   backtrack flags GSH_BT_*          */
+#define GSH_F_CD          7   /* double [n]          bottom drag coefficient (nchi=0)  */
+#define GSH_F_GRAV2       8   /* double [n]          gravity                           */
+#define GSH_F_ETH         9   /* double [n]          essential-BC elevation target     */
+#define GSH_F_VSOURCE    10   /* double [ne]         volumetric source                 */
+#define GSH_F_SIDE_U     11   /* double [ns][2]      vertically integrated velocity    */
+#define GSH_F_SIDE_HHAT  12   /* double [ns]                                           */
+#define GSH_F_SIDE_CHI   13   /* double [ns]                                           */
+#define GSH_F_GHAT1      14   /* double [ne][2]                                        */
+#define GSH_F_QEL        15   /* double [n]          assembled RHS                     */
+#define GSH_F_ZC         16   /* double [ns][nvrt]   side-centre z                     */
+#define GSH_F_ELBC       17   /* double [n]          computed boundary elevation       */
+#define GSH_F_WET        18   /* int8   [n]/[ne]/[ns] packed: nodes, then elements, then sides (1 = wet) */
+int gsh_mesh_set_field(gsh_mesh* mesh, int field, const void* host, int64_t count);
+int gsh_mesh_get_field(const gsh_mesh* mesh, int field, void* host, int64_t count);
+int gsh_mesh_field_ptr(gsh_mesh* mesh, int field, void** dev_ptr, int64_t* count);
+
+/* Backtrack status bits (GSH_F_BSTATUS) for paths the reference only asserts on. */
+#define GSH_BT_WALK_CAP   1   /* celestial walk hit its 40-step cap (meshdata.cu:87,126)   */
+#define GSH_BT_ITER38     2   /* btrack returned at iter==38 (meshdata.cu:1611)           */
+#define GSH_BT_STUCK      4   /* time_rm stopped decreasing (meshdata.cu:1496-1500)        */
+#define GSH_BT_OPEN_EXIT  8   /* track left the domain through an open boundary (:1567)    */
+#define GSH_BT_SLIDE     16   /* track slid along a land boundary or dry edge (:1569-1584) */
+
+/* Physical parameters of one step (the constexpr values of constants_template.h:140-168). */
+typedef struct gsh_step_params {
+  double dt;          /* time step [s]                                 */
+  double theta;       /* implicitness thetai                           */
+  double h0;          /* wet/dry threshold [m]                         */
+  double ramp;        /* boundary ramp factor (1 = none)               */
+  double dtb_min;     /* ELM minimum sub-step [s]                      */
+  double dtb_max;     /* ELM maximum sub-step [s]                      */
+  double velmin_btrack; /* ELM minimum velocity [m/s]                  */
+  int32_t ihhat;      /* 1 = clip H^ >= 0 in 3D                        */
+  int32_t iettype;    /* elevation BC type of the open boundary: 1,2,4 = eth; 0 = none */
+} gsh_step_params;
+void gsh_step_params_default(gsh_step_params* p);
+
+/* a3: node z-levels, wet/dry of nodes, elements and sides, side kb and zc
+ * (meshdata.cu:5521-5594, 5679-5760, 1744-1807). Call after eta2 changes. */
+int gsh_update_levels(gsh_mesh* mesh, const gsh_step_params* p);
+
+/* a4-a7: backtrackAllNodes (meshdata.cu:3547-3660) over every side x level. Results stay on
+ * the device (GSH_F_BTU, GSH_F_FOOT, GSH_F_BSTATUS); btu_host / foot_host may be NULL. */
+int gsh_backtrack(gsh_mesh* mesh, const gsh_step_params* p, double* btu_host, int32_t* foot_host);
+
+/* Backtracks an explicit list of (side, level) start points (the same walk as gsh_backtrack,
+ * results written to the same per-(side, level) slots). */
+int gsh_backtrack_list(gsh_mesh* mesh, const gsh_step_params* p, const int32_t* sides_host,
+                       const int32_t* levels_host, int32_t count);
+
+/* a8-a10 and the elevation BC: computeVerticallyIntegratedVelocity, computeVegetationAndDragParameters,
+ * computeElemAverageGhat and computeElevationBoundaryConditions (meshdata.cu:4459-4476, 4615-4725,
+ * 6183-6424, 4276-4339). */
+int gsh_rhs_producers(gsh_mesh* mesh, const gsh_step_params* p);
+
+/* ---------------------------------------------------------------- solver */
+/* PCGSolverMultiGPU(configCuda, N, NNZ) equivalent bound to a mesh's CSR pattern
+ * (pcg_solver.py:547-671). */
+int gsh_solver_create(gsh_mesh* mesh, int precision, int precond, gsh_solver** out);
+/* Stand-alone solver over a caller CSR pattern (int32 indptr/indices, host arrays); values
+ * are set with gsh_solver_set_values or written through gsh_solver_csr. */
+int gsh_solver_create_csr(gsh_ctx* ctx, int32_t n, int64_t nnz, const int32_t* indptr_host,
+                          const int32_t* indices_host, int precision, int precond, gsh_solver** out);
+int gsh_solver_destroy(gsh_solver* s);
+/* csr_data / csr_indices / csr_indptr (pcg_solver.py:1146-1148): device pointers. */
+int gsh_solver_csr(gsh_solver* s, double** data, int32_t** indices, int32_t** indptr, int64_t* nnz);
+int gsh_solver_set_values(gsh_solver* s, const double* data_host);
+int gsh_solver_get_values(gsh_solver* s, double* data_host);
+/* Right-hand side b and solution x as device vectors (length n). */
+int gsh_solver_vec_ptr(gsh_solver* s, double** b, double** x);
+
+/* a11+a12: setupCoefficientMatrixAndRHS + setEssentialBC (meshdata.cu:3182-3474, 4400-4446)
+ * into the solver's CSR values and b (= qel). */
+int gsh_assemble(gsh_mesh* mesh, gsh_solver* s, const gsh_step_params* p, double time);
+
+/* solve(rtol, maxiter, checkEvery) (pcg_solver.py:1075-1124; native loop pcg_loop.cpp:253-291):
+ * b from b_host if non-NULL, else the device b (the assembled qel, gatherBFromMesh);
+ * initial guess 3x0-3x1+x2 from the solver's history; returns the iteration count in
+ * *iters_out (maxiter if not converged, GSH_W_MAXITER set in *status_out). x_host may be NULL.
+ * timeout_s <= 0 means 60 s. */
+int gsh_solver_solve(gsh_solver* s, const double* b_host, double* x_host, double rtol,
+                     int maxiter, int check_every, double timeout_s, int* iters_out, int* status_out);
+/* checkSolution (pcg_solver.py:1127-1143): ||b - A x|| / ||b||. */
+int gsh_solver_check(gsh_solver* s, double* rel_out);
+/* Clears the extrapolation history x0=x1=x2=0 (cold start, main.py:352). */
+int gsh_solver_reset_history(gsh_solver* s);
+/* Per-solve statistics of the last solve. */
+typedef struct gsh_solve_stats {
+  int32_t iters;          /* PCG iterations (inner iterations for MIXED)      */
+  int32_t outer;          /* refinement steps (MIXED), 1 for FP64             */
+  double  rr;             /* final recursive ||r||^2                          */
+  double  atol2;          /* (rtol ||b||)^2                                   */
+  double  true_rel;       /* ||b-Ax||/||b|| if measured, else -1              */
+  double  ms;             /* wall time of the solve                           */
+} gsh_solve_stats;
+int gsh_solver_last_stats(gsh_solver* s, gsh_solve_stats* out);
+/* Tuning knobs: rows per thread of the PCG tiles (1, 2, 4, 8) and the inner tolerance of the
+ * mixed-precision refinement. */
+int gsh_solver_set_tuning(gsh_solver* s, int rows_per_thread, double inner_rtol);
+
+/* updateElevation (meshdata.cu:4449-4456) after rollUpElevation (:3538-3544):
+ * eta1 <- eta2, eta2 <- x. */
+int gsh_update_elevation(gsh_mesh* mesh, gsh_solver* s);
+
+/* ---------------------------------------------------------------- batching */
+/* Solves several independent systems in lock-step launches: every PCG phase is one launch over
+ * the tiles of all systems that have not converged. Each system keeps the single-solve
+ * stopping rule, so results equal solving them one by one. */
+int gsh_batch_solve(gsh_solver* const* solvers, int count, double rtol, int maxiter,
+                    int check_every, double timeout_s, int* iters_out);
+
+/* ---------------------------------------------------------------- serving */
+typedef struct gsh_server gsh_server;
+typedef struct gsh_server_config {
+  int64_t hbm_budget_bytes;   /* admission budget (0 = free device memory at creation) */
+  int32_t max_active;         /* max systems in flight (0 = unlimited)             */
+  int32_t precision;          /* GSH_PREC_*                                         */
+  double  rtol;               /* solve tolerance                                    */
+  int32_t maxiter;
+  int32_t check_every;
+} gsh_server_config;
+int gsh_server_create(gsh_ctx* ctx, const gsh_server_config* cfg, gsh_server** out);
+int gsh_server_destroy(gsh_server* srv);
+/* Registers a mesh so that requests can reference it; the mesh stays device resident. */
+int gsh_server_add_mesh(gsh_server* srv, gsh_mesh* mesh, int32_t* mesh_id_out);
+/* Submits one solve (+ optional backtrack) request: eta2/b host vectors are copied at submit.
+ * with_backtrack != 0 runs gsh_backtrack on the mesh's current velocities before the solve.
+ * b_host = NULL assembles the RHS from the mesh state. */
+int gsh_server_submit(gsh_server* srv, int32_t mesh_id, const double* b_host, int with_backtrack,
+                      int64_t* ticket_out);
+/* Advances the service: admits queued requests as memory and slots allow, runs one check
+ * window of lock-step PCG iterations for every active request, retires finished ones. */
+int gsh_server_step(gsh_server* srv, int32_t* active_out, int32_t* queued_out);
+/* Result of a finished request: status 0 = done, 1 = pending; x_host may be NULL. */
+int gsh_server_poll(gsh_server* srv, int64_t ticket, double* x_host, int32_t* iters_out,
+                    int32_t* status_out, double* latency_ms_out);
+
+/* ---------------------------------------------------------------- synthetic inputs */
+/* Jittered-grid Delaunay channel: nx*ny nodes spaced dx, interior nodes
+ * jittered uniformly by +-jitter*dx (boundary nodes only along the boundary), cells split and
+ * Lawson-flipped to Delaunay. The west edge (x = 0) is the open boundary. */
+typedef struct gsh_gen gsh_gen;
+int gsh_gen_channel(int32_t nx, int32_t ny, double dx, double jitter, uint64_t seed, gsh_gen** out);
+int gsh_gen_sizes(const gsh_gen* g, int32_t* n, int32_t* ne);
+int gsh_gen_get(const gsh_gen* g, double* xy, int32_t* elnode, int8_t* open_bnd);
+int gsh_gen_destroy(gsh_gen* g);
+/* Log-normal bathymetry from a Gaussian random field (Gaussian covariance, correlation length
+ * corr_cells grid cells) on the nx*ny generator grid, mapped to [hmin, hmax]. */
+int gsh_gen_bathymetry(int32_t nx, int32_t ny, double corr_cells, double hmin, double hmax,
+                       uint64_t seed, double* depth_out);
+/* Divergence-free velocity from a stream function of nmodes Fourier modes, scaled so that
+ * max |u| over the domain is umax, times the vertical shear (1 + shear*sigma_k); w = 0.
+ * out is [npts][nvrt][ncomp] with ncomp 2 (u,v) or 3 (u,v,w). */
+int gsh_gen_velocity(int32_t npts, const double* xy, int32_t nvrt, const double* sigma,
+                     double lx, double ly, double dx, int32_t nmodes, double umax, double shear,
+                     uint64_t seed, int32_t ncomp, double* out);
+/* Normal deviates N(mean, std^2) from the library's xoshiro256** / Box-Muller stream. */
+int gsh_gen_normal(uint64_t seed, int64_t count, double mean, double std, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GSH_H */

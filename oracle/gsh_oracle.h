@@ -1,0 +1,61 @@
+/*
+ * gsh_oracle.h -- CPU FP64 restatement of the SCHISM wave-equation hot path (test
+ * infrastructure only: the parity checker and the CPU baseline, never the product path).
+ *
+ * Every function follows the reference's algorithm line by line; citations are in
+ * gsh_oracle.c. Array layouts and field/array codes are the same as include/gsh.h.
+ */
+#ifndef GSH_ORACLE_H
+#define GSH_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct or_mesh or_mesh;
+
+typedef struct or_params {
+  double dt, theta, h0, ramp, dtb_min, dtb_max, velmin_btrack;
+  int32_t ihhat, iettype;
+} or_params;
+
+const char* or_error(void);
+/* Builds topology and geometry; NULL on failure (message in or_error()). kb may be NULL. */
+or_mesh* or_mesh_new(int32_t n, int32_t ne, int32_t nvrt, const double* xy, const int32_t* elnode,
+                     const double* depth, const int8_t* open_bnd, const int32_t* kb, const double* sigma);
+void or_mesh_free(or_mesh* m);
+int32_t or_mesh_ns(const or_mesh* m);
+int64_t or_mesh_nnz(const or_mesh* m);
+int32_t or_mesh_nbr_count(const or_mesh* m);
+/* Copies a topology array (GSH_T_* codes) out. Returns 0 on success. */
+int or_mesh_get(const or_mesh* m, int what, void* out);
+/* State fields (GSH_F_* codes and layouts). Returns 0 on success. */
+int or_set_field(or_mesh* m, int field, const void* data);
+int or_get_field(const or_mesh* m, int field, void* out);
+
+void or_levels(or_mesh* m, const or_params* p);
+/* Backtracks every side x level (list == NULL) or the given (side, level) pairs. */
+void or_backtrack(or_mesh* m, const or_params* p, const int32_t* list_side, const int32_t* list_lev,
+                  int64_t count, int nthreads);
+void or_rhs(or_mesh* m, const or_params* p);
+/* Assembly + essential BC into val (CSR of the mesh) and b. */
+void or_assemble(or_mesh* m, const or_params* p, double* val, double* b, int nthreads);
+
+/* Jacobi PCG with the reference's recurrences; dot products in the fixed tile/wave tree of the
+ * device kernels (rows_per_thread as the device solver's). h0/h1/h2 may be NULL (zero guess).
+ * Returns the iteration count; *rr_last = last checked r.r, *atol2 = (rtol ||b||)^2. */
+int32_t or_pcg(int32_t n, const int32_t* indptr, const int32_t* indices, const double* val, const double* b,
+               const double* h0, const double* h1, const double* h2, double rtol, int32_t maxiter,
+               int32_t check_every, int32_t rows_per_thread, double* x_out, double* rr_last, double* atol2,
+               int nthreads);
+/* Dot product in the device's fixed reduction order. */
+double or_dot_canon(int32_t n, const double* a, const double* b, int32_t rows_per_thread);
+/* ||b - A x|| / ||b|| in the same order as the device check. */
+double or_true_rel(int32_t n, const int32_t* indptr, const int32_t* indices, const double* val, const double* b,
+                   const double* x, int32_t rows_per_thread);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
