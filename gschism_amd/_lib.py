@@ -110,6 +110,7 @@ SIGNATURES = {
     "gsh_solver_set_values": (_I, [_P, _PD]),
     "gsh_solver_get_values": (_I, [_P, _PD]),
     "gsh_solver_vec_ptr": (_I, [_P, _PP, _PP]),
+    "gsh_solver_get_rhs": (_I, [_P, _PD]),
     "gsh_assemble": (_I, [_P, _P, C.POINTER(StepParams), _D]),
     "gsh_solver_solve": (_I, [_P, _PD, _PD, _D, _I, _I, _D, C.POINTER(_I), C.POINTER(_I)]),
     "gsh_solver_check": (_I, [_P, _PD]),
@@ -124,6 +125,8 @@ SIGNATURES = {
     "gsh_server_submit": (_I, [_P, _I32, _PD, _I, _PI64]),
     "gsh_server_step": (_I, [_P, _PI32, _PI32]),
     "gsh_server_poll": (_I, [_P, _I64, _PD, _PI32, _PI32, _PD]),
+    "gsh_copy_probe": (_I, [_P, _I64, _I, _PD]),
+    "gsh_solver_profile": (_I, [_P, _D, _I, _I, _PD, _PD, C.POINTER(_I)]),
     "gsh_gen_channel": (_I, [_I32, _I32, _D, _D, C.c_uint64, _PP]),
     "gsh_gen_sizes": (_I, [_P, _PI32, _PI32]),
     "gsh_gen_get": (_I, [_P, _PD, _PI32, C.POINTER(C.c_int8)]),

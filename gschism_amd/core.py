@@ -232,6 +232,11 @@ class Solver:
         check(L.lib().gsh_solver_get_values(self.h, dptr(v)))
         return v
 
+    def get_rhs(self) -> np.ndarray:
+        b = np.empty(self.n, dtype=np.float64)
+        check(L.lib().gsh_solver_get_rhs(self.h, dptr(b)))
+        return b
+
     def assemble(self, params, time: float = 0.0) -> None:
         if self.mesh is None:
             raise ValueError("solver is not bound to a mesh")
@@ -273,3 +278,58 @@ def batch_solve(solvers, rtol=1e-12, maxiter=0, check_every=10, timeout_s=60.0):
     its = (C.c_int * len(solvers))()
     check(L.lib().gsh_batch_solve(arr, len(solvers), rtol, maxiter, check_every, timeout_s, its))
     return list(its)
+
+
+class Server:
+    """Continuous-batching solve service (gsh_server): requests join and leave the lock-step
+    PCG launches at check-window boundaries."""
+
+    def __init__(self, ctx: Context, rtol: float = 1e-12, maxiter: int = 3000, check_every: int = 10,
+                 max_active: int = 0, hbm_budget_bytes: int = 0):
+        cfg = L.ServerConfig()
+        cfg.hbm_budget_bytes = hbm_budget_bytes
+        cfg.max_active = max_active
+        cfg.precision = L.PREC_FP64
+        cfg.rtol = rtol
+        cfg.maxiter = maxiter
+        cfg.check_every = check_every
+        h = C.c_void_p()
+        check(L.lib().gsh_server_create(ctx.h, C.byref(cfg), C.byref(h)))
+        self.h = h
+        self.ctx = ctx
+        self.meshes: list[Mesh] = []
+        _register(self)
+
+    def close(self):
+        if getattr(self, "h", None):
+            L.lib().gsh_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def add_mesh(self, mesh: Mesh) -> int:
+        i = C.c_int32()
+        check(L.lib().gsh_server_add_mesh(self.h, mesh.h, C.byref(i)))
+        self.meshes.append(mesh)
+        return i.value
+
+    def submit(self, mesh_id: int, b=None, with_backtrack: bool = False) -> int:
+        bb = None if b is None else np.ascontiguousarray(b, dtype=np.float64)
+        t = C.c_int64()
+        check(L.lib().gsh_server_submit(self.h, mesh_id, dptr(bb) if bb is not None else None,
+                                        1 if with_backtrack else 0, C.byref(t)))
+        return t.value
+
+    def step(self):
+        a, q = C.c_int32(), C.c_int32()
+        check(L.lib().gsh_server_step(self.h, C.byref(a), C.byref(q)))
+        return a.value, q.value
+
+    def poll(self, ticket: int, n: int | None = None):
+        """(status, x or None, iters, latency_ms); status 0 = done, 1 = pending, else GSH_E_*."""
+        x = np.empty(n, dtype=np.float64) if n else None
+        it, st, lat = C.c_int32(), C.c_int32(), C.c_double()
+        check(L.lib().gsh_server_poll(self.h, ticket, dptr(x) if x is not None else None, C.byref(it), C.byref(st),
+                                      C.byref(lat)))
+        return st.value, (x if st.value == 0 else None), it.value, lat.value

@@ -273,10 +273,16 @@ struct PcgOutcome {
 
 // Runs the PCG iteration for nsys systems described by dsys (device) over the tile table
 // dtiles. Each system stops at its own first check with r.r <= atol^2 (pcg_loop.cpp:405-412).
+// Per-phase launch timing collected with HIP events on the solver stream.
+struct PhaseTimes {
+  double ms_a = 0.0, ms_b = 0.0;
+  int64_t launches = 0;
+};
+
 template <class VT>
 static void run_pcg(gsh_ctx* ctx, const PcgSys<VT>* dsys, int nsys, const TileRef* dtiles, int ntiles, int rpt,
                     double rtol, int maxiter, int check_every, double timeout_s, PcgOutcome& out,
-                    const TileRef* dinit_tiles = nullptr, int ninit_tiles = -1) {
+                    const TileRef* dinit_tiles = nullptr, int ninit_tiles = -1, PhaseTimes* prof = nullptr) {
   hipStream_t st = ctx->stream;
   if (check_every < 1) check_every = 1;
   if (timeout_s <= 0.0) timeout_s = 60.0;
@@ -301,7 +307,30 @@ static void run_pcg(gsh_ctx* ctx, const PcgSys<VT>* dsys, int nsys, const TileRe
   };
   while (issued < maxiter && !all_done) {
     const int w = std::min(check_every, maxiter - issued);
-    for (int i = 0; i < w; ++i) launch_pcg_iter<VT>(dsys, dtiles, ntiles, rpt, st);
+    if (prof) {
+      // timed window: events around each phase, read back before the next window
+      std::vector<hipEvent_t> evs(2 * w + 1);
+      for (auto& e : evs) HIPCHK(hipEventCreate(&e));
+      HIPCHK(hipEventRecord(evs[0], st));
+      for (int i = 0; i < w; ++i) {
+        launch_pcg_phase<VT>(0, dsys, dtiles, ntiles, rpt, st);
+        HIPCHK(hipEventRecord(evs[2 * i + 1], st));
+        launch_pcg_phase<VT>(1, dsys, dtiles, ntiles, rpt, st);
+        HIPCHK(hipEventRecord(evs[2 * i + 2], st));
+      }
+      HIPCHK(hipEventSynchronize(evs[2 * w]));
+      for (int i = 0; i < w; ++i) {
+        float a = 0.f, b = 0.f;
+        HIPCHK(hipEventElapsedTime(&a, evs[2 * i], evs[2 * i + 1]));
+        HIPCHK(hipEventElapsedTime(&b, evs[2 * i + 1], evs[2 * i + 2]));
+        prof->ms_a += a;
+        prof->ms_b += b;
+        prof->launches += 1;
+      }
+      for (auto& e : evs) (void)hipEventDestroy(e);
+    } else {
+      for (int i = 0; i < w; ++i) launch_pcg_iter<VT>(dsys, dtiles, ntiles, rpt, st);
+    }
     launch_pcg_check<VT>(dsys, nsys, st);
     issued += w;
     const int slot = window & 1;
@@ -1234,13 +1263,12 @@ int gsh_server_step(gsh_server* srv, int32_t* active_out, int32_t* queued_out) {
         launch_backtrack(m->d, a, nullptr, nullptr, 0, st);
         launch_rhs(m->d, a, st);
       }
-      if (r->has_b) {
-        HIPCHK(hipMemcpyAsync(s->b, r->b.data(), sizeof(double) * s->n, hipMemcpyHostToDevice, st));
-      } else {
+      {  // the request's matrix (and right-hand side unless given) from the mesh state
         gsh_step_params p;
         gsh_step_params_default(&p);
         launch_assemble(m->d, step_args(&p), s->val, s->b, st);
       }
+      if (r->has_b) HIPCHK(hipMemcpyAsync(s->b, r->b.data(), sizeof(double) * s->n, hipMemcpyHostToDevice, st));
       s->rpt = 4;
       s->ntiles = s->tiles_for(s->n);
       s->hs.x = s->x;
@@ -1340,3 +1368,76 @@ int gsh_server_poll(gsh_server* srv, int64_t ticket, double* x_host, int32_t* it
 }
 
 }  // extern "C"
+
+// ====================================================================== measurement helpers
+namespace gsh {
+__global__ void k_copy_probe(const double4* __restrict__ src, double4* __restrict__ dst, int64_t n4) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) dst[i] = src[i];
+}
+}  // namespace gsh
+
+extern "C" {
+
+int gsh_copy_probe(gsh_ctx* ctx, int64_t bytes, int reps, double* ms_avg) {
+  GSH_API_BODY({
+    GSH_REQUIRE(ctx && ms_avg && bytes >= 32 && reps >= 1, GSH_E_ARG, "bad arguments");
+    ctx->set();
+    hipStream_t st = ctx->stream;
+    const int64_t n4 = bytes / 32;
+    double4 *a = nullptr, *b = nullptr;
+    HIPCHK(hipMalloc(&a, n4 * 32));
+    HIPCHK(hipMalloc(&b, n4 * 32));
+    HIPCHK(hipMemsetAsync(a, 0, n4 * 32, st));
+    hipEvent_t e0, e1;
+    HIPCHK(hipEventCreate(&e0));
+    HIPCHK(hipEventCreate(&e1));
+    int dev_cus = 0;
+    HIPCHK(hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    const int grid = std::max(1, dev_cus) * 16;
+    hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(256), 0, st, a, b, n4);  // warm-up
+    HIPCHK(hipEventRecord(e0, st));
+    for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(256), 0, st, a, b, n4);
+    HIPCHK(hipEventRecord(e1, st));
+    HIPCHK(hipEventSynchronize(e1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+    *ms_avg = ms / reps;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+  })
+}
+
+int gsh_solver_profile(gsh_solver* s, double rtol, int maxiter, int check_every, double* ms_a, double* ms_b,
+                       int* iters_out) {
+  GSH_API_BODY({
+    GSH_REQUIRE(s && ms_a && ms_b, GSH_E_ARG, "NULL argument");
+    GSH_REQUIRE(s->precision == GSH_PREC_FP64, GSH_E_ARG, "phase profiling runs the FP64 solver");
+    if (maxiter <= 0) maxiter = 10 * s->n;
+    s->ctx->set();
+    solver_sync_desc(s);
+    PcgOutcome o;
+    PhaseTimes pt;
+    run_pcg<double>(s->ctx, s->ds, 1, s->dtiles, s->ntiles, s->rpt, rtol, maxiter, check_every, 60.0, o, nullptr,
+                    -1, &pt);
+    const int it = iters_of(o.st[0], maxiter);
+    HIPCHK(hipStreamSynchronize(s->ctx->stream));
+    solver_roll(s);
+    *ms_a = pt.launches ? pt.ms_a / pt.launches : 0.0;
+    *ms_b = pt.launches ? pt.ms_b / pt.launches : 0.0;
+    if (iters_out) *iters_out = it;
+  })
+}
+
+}  // extern "C"
+
+extern "C" int gsh_solver_get_rhs(gsh_solver* s, double* b_host) {
+  GSH_API_BODY({
+    GSH_REQUIRE(s && b_host, GSH_E_ARG, "NULL argument");
+    s->ctx->set();
+    HIPCHK(hipStreamSynchronize(s->ctx->stream));
+    HIPCHK(hipMemcpy(b_host, s->b, sizeof(double) * s->n, hipMemcpyDeviceToHost));
+  })
+}

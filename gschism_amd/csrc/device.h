@@ -128,6 +128,8 @@ void launch_pcg_init(const PcgSys<VT>* sys, const TileRef* tiles, int ntiles, in
 template <class VT>
 void launch_pcg_iter(const PcgSys<VT>* sys, const TileRef* tiles, int ntiles, int rpt, hipStream_t s);
 template <class VT>
+void launch_pcg_phase(int which, const PcgSys<VT>* sys, const TileRef* tiles, int ntiles, int rpt, hipStream_t s);
+template <class VT>
 void launch_pcg_check(const PcgSys<VT>* sys, int nsys, hipStream_t s);
 template <class VT>
 void launch_pcg_finish(const PcgSys<VT>* sys, const TileRef* tiles, int ntiles, int rpt, hipStream_t s);

@@ -391,6 +391,16 @@ void launch_pcg_iter(const PcgSys<VT>* sys, const TileRef* tiles, int ntiles, in
 }
 
 template <class VT>
+void launch_pcg_phase(int which, const PcgSys<VT>* sys, const TileRef* tiles, int ntiles, int rpt, hipStream_t s) {
+  if (ntiles <= 0) return;
+  if (which == 0) {
+    GSH_DISPATCH_R(rpt, k_pcg_phase_a, VT, dim3(ntiles), dim3(kTpb), 0, s, sys, tiles);
+  } else {
+    GSH_DISPATCH_R(rpt, k_pcg_phase_b, VT, dim3(ntiles), dim3(kTpb), 0, s, sys, tiles);
+  }
+}
+
+template <class VT>
 void launch_pcg_check(const PcgSys<VT>* sys, int nsys, hipStream_t s) {
   if (nsys <= 0) return;
   hipLaunchKernelGGL(k_pcg_check<VT>, dim3(nsys), dim3(kTpb), 0, s, sys);
@@ -408,6 +418,8 @@ template void launch_pcg_init<double>(const PcgSys<double>*, const TileRef*, int
 template void launch_pcg_init<float>(const PcgSys<float>*, const TileRef*, int, int, hipStream_t);
 template void launch_pcg_iter<double>(const PcgSys<double>*, const TileRef*, int, int, hipStream_t);
 template void launch_pcg_iter<float>(const PcgSys<float>*, const TileRef*, int, int, hipStream_t);
+template void launch_pcg_phase<double>(int, const PcgSys<double>*, const TileRef*, int, int, hipStream_t);
+template void launch_pcg_phase<float>(int, const PcgSys<float>*, const TileRef*, int, int, hipStream_t);
 template void launch_pcg_check<double>(const PcgSys<double>*, int, hipStream_t);
 template void launch_pcg_check<float>(const PcgSys<float>*, int, hipStream_t);
 template void launch_pcg_finish<double>(const PcgSys<double>*, const TileRef*, int, int, hipStream_t);

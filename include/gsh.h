@@ -193,6 +193,8 @@ int gsh_solver_set_values(gsh_solver* s, const double* data_host);
 int gsh_solver_get_values(gsh_solver* s, double* data_host);
 /* Right-hand side b and solution x as device vectors (length n). */
 int gsh_solver_vec_ptr(gsh_solver* s, double** b, double** x);
+/* Host copy of the right-hand side b (gatherBFromMesh target, pcg_solver.py:950-954). */
+int gsh_solver_get_rhs(gsh_solver* s, double* b_host);
 
 /* a11+a12: setupCoefficientMatrixAndRHS + setEssentialBC (meshdata.cu:3182-3474, 4400-4446)
  * into the solver's CSR values and b (= qel). */
@@ -259,6 +261,16 @@ int gsh_server_step(gsh_server* srv, int32_t* active_out, int32_t* queued_out);
 /* Result of a finished request: status 0 = done, 1 = pending; x_host may be NULL. */
 int gsh_server_poll(gsh_server* srv, int64_t ticket, double* x_host, int32_t* iters_out,
                     int32_t* status_out, double* latency_ms_out);
+
+/* ---------------------------------------------------------------- measurement */
+/* No counterpart in the reference: measurement helpers of this library.
+ * gsh_copy_probe: average time of one device-to-device copy of `bytes` (read + write) by a
+ * streaming kernel, the achievable-bandwidth reference. gsh_solver_profile: one solve (same
+ * arithmetic as gsh_solver_solve) with HIP events around every PCG phase launch; returns the
+ * mean duration of phase A (p update + SpMV + p.q) and phase B (x, r, z updates + r.r, r.z). */
+int gsh_copy_probe(gsh_ctx* ctx, int64_t bytes, int reps, double* ms_avg);
+int gsh_solver_profile(gsh_solver* s, double rtol, int maxiter, int check_every, double* ms_phase_a,
+                       double* ms_phase_b, int* iters_out);
 
 /* ---------------------------------------------------------------- synthetic inputs */
 /* Jittered-grid Delaunay channel: nx*ny nodes spaced dx, interior nodes

@@ -1,0 +1,233 @@
+"""GPU parity: every stage of the hot path through the C ABI against the CPU oracle, bit for bit."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import assert_bitwise, oracle_mesh_for
+from gschism_amd import _lib as L
+from gschism_amd import cases, core
+from oracle import pyoracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _setup(ctx, spec, kb_fn=None, depth_fn=None):
+    data = cases.make_case(spec)
+    if depth_fn is not None:
+        data.depth = depth_fn(data.xy)
+    kb = None if kb_fn is None else kb_fn(data.depth)
+    mesh = core.Mesh(ctx, data.xy, data.elnode, data.depth, data.sigma, open_bnd=data.open_bnd, kb=kb)
+    xy = mesh.topology(L.T_XY)
+    eta, uvw, su2 = cases.initial_state(spec, xy, cases.side_centres(mesh), data.sigma)
+    eta = eta + cases.normal(cases.SEED_ETA, spec.n, 0.0, spec.eta_noise)[mesh.node_perm]
+    p = cases.step_params(spec)
+    mesh.set(L.F_ETA2, eta)
+    mesh.set(L.F_NODE_UVW, uvw)
+    mesh.set(L.F_SIDE_U2, su2)
+    mesh.update_levels(p)
+    om = oracle_mesh_for(mesh, data.sigma)
+    om.set(L.F_ETA2, eta)
+    om.set(L.F_NODE_UVW, uvw)
+    om.set(L.F_SIDE_U2, su2)
+    om.levels(p)
+    return mesh, om, p
+
+
+def _full_chain(ctx, mesh, om, p, rows_per_thread=4):
+    print(f"[gpu] chain n={mesh.n} ns={mesh.ns} nvrt={mesh.nvrt}", flush=True)
+    assert_bitwise(mesh.get(L.F_ZC), om.get(L.F_ZC), "zc")
+    assert_bitwise(mesh.get(L.F_WET), om.get(L.F_WET), "wet flags")
+    mesh.backtrack(p)
+    om.backtrack(p, nthreads=8)
+    assert_bitwise(mesh.get(L.F_FOOT), om.get(L.F_FOOT), "foot elements")
+    assert_bitwise(mesh.get(L.F_BTU), om.get(L.F_BTU), "backtracked velocity")
+    assert_bitwise(mesh.get(L.F_BSTATUS), om.get(L.F_BSTATUS), "backtrack status")
+    mesh.rhs_producers(p)
+    om.rhs(p)
+    for f, name in ((L.F_SIDE_U, "U"), (L.F_SIDE_HHAT, "hhat"), (L.F_SIDE_CHI, "chi"), (L.F_GHAT1, "ghat1"),
+                    (L.F_ELBC, "elbc")):
+        assert_bitwise(mesh.get(f), om.get(f), name)
+    solver = core.Solver(mesh)
+    solver.set_tuning(rows_per_thread)
+    solver.assemble(p)
+    val, b = om.assemble(p)
+    assert_bitwise(solver.get_values(), val, "CSR values")
+    assert_bitwise(solver.get_rhs(), b, "RHS qel")
+    x, it, st = solver.solve(None, rtol=1e-12, maxiter=1500, check_every=10)
+    xo, ito, rr, at = O.pcg(mesh.topology(L.T_CSR_INDPTR), mesh.topology(L.T_CSR_INDICES), val, b, rtol=1e-12,
+                            maxiter=1500, check_every=10, rows_per_thread=rows_per_thread)
+    assert st == 0 and it == ito, (it, ito, st)
+    assert_bitwise(x, xo, "PCG solution")
+    rel = solver.check()
+    assert rel < 1e-10
+    stt = solver.stats()
+    assert stt.atol2 == at and stt.rr == rr
+    return solver, x, val, b
+
+
+def test_chain_channel_200x80(ctx):
+    spec = cases.spec_cpu_reference()
+    mesh, om, p = _setup(ctx, spec)
+    _full_chain(ctx, mesh, om, p)
+    st = mesh.get(L.F_BSTATUS)
+    assert (st & L.BT_OPEN_EXIT).any() and (st & L.BT_SLIDE).any()
+
+
+def _island_depth(xy):
+    cx, cy = xy[:, 0].mean(), xy[:, 1].mean()
+    r2 = ((xy[:, 0] - cx) ** 2 + (xy[:, 1] - cy) ** 2) / (4000.0 ** 2)
+    return 12.0 - 16.0 * np.exp(-r2)
+
+
+def _lsc_kb(depth):
+    return np.clip((6 - depth).astype(np.int32), 0, 6)
+
+
+def test_chain_island_dry_and_kb(ctx):
+    """Dry elements (an island above the water) and node-varying bottom levels."""
+    spec = cases.CaseSpec("island-48x32", 48, 32, 400.0, nvrt=12, depth=10.0, substeps=6)
+    mesh, om, p = _setup(ctx, spec, kb_fn=_lsc_kb, depth_fn=_island_depth)
+    wet = mesh.get(L.F_WET)
+    assert (wet == 0).any(), "the island should leave dry entities"
+    assert (mesh.topology(L.T_KB) > 0).any()
+    _full_chain(ctx, mesh, om, p)
+
+
+@pytest.mark.parametrize("rpt", [1, 2, 8])
+def test_rows_per_thread(ctx, rpt):
+    spec = cases.CaseSpec("channel-64x40", 64, 40, 500.0, nvrt=6, depth=None, substeps=4)
+    mesh, om, p = _setup(ctx, spec)
+    _full_chain(ctx, mesh, om, p, rows_per_thread=rpt)
+
+
+def test_three_steps_with_history(ctx):
+    """Consecutive steps: extrapolated initial guess 3x0-3x1+x2, eta update, levels update."""
+    spec = cases.CaseSpec("channel-60x30", 60, 30, 500.0, nvrt=8, depth=None, substeps=5)
+    mesh, om, p = _setup(ctx, spec)
+    solver = core.Solver(mesh)
+    ip, ix = mesh.topology(L.T_CSR_INDPTR), mesh.topology(L.T_CSR_INDICES)
+    hist = [np.zeros(mesh.n)] * 3
+    for step in range(3):
+        print(f"[gpu] step {step}", flush=True)
+        mesh.backtrack(p)
+        om.backtrack(p)
+        mesh.rhs_producers(p)
+        om.rhs(p)
+        solver.assemble(p)
+        val, b = om.assemble(p)
+        x, it, st = solver.solve(None, rtol=1e-12, maxiter=1500)
+        xo, ito, _, _ = O.pcg(ip, ix, val, b, hist=hist, rtol=1e-12, maxiter=1500)
+        assert it == ito
+        assert_bitwise(x, xo, f"step {step} solution")
+        hist = [xo, hist[0], hist[1]]
+        eta_old = mesh.get(L.F_ETA2)
+        solver.update_elevation()
+        assert_bitwise(mesh.get(L.F_ETA1), eta_old, "eta1 roll")
+        assert_bitwise(mesh.get(L.F_ETA2), xo, "eta2 update")
+        om.set(L.F_ETA2, xo)
+        mesh.update_levels(p)
+        om.levels(p)
+        assert_bitwise(mesh.get(L.F_ZC), om.get(L.F_ZC), f"step {step} zc")
+
+
+def test_backtrack_list_matches_full(ctx):
+    spec = cases.CaseSpec("channel-50x30", 50, 30, 500.0, nvrt=10, depth=10.0, substeps=5)
+    mesh, om, p = _setup(ctx, spec)
+    mesh.backtrack(p)
+    full_b, full_f = mesh.get(L.F_BTU), mesh.get(L.F_FOOT)
+    rng = np.random.default_rng(5)
+    sides = rng.integers(0, mesh.ns, 500).astype(np.int32)
+    levs = rng.integers(0, mesh.nvrt, 500).astype(np.int32)
+    mesh.set(L.F_BTU, np.zeros_like(full_b))
+    mesh.backtrack_list(p, sides, levs)
+    got_b, got_f = mesh.get(L.F_BTU), mesh.get(L.F_FOOT)
+    assert_bitwise(got_b[sides, levs], full_b[sides, levs], "list-mode btu")
+    assert_bitwise(got_f[sides, levs], full_f[sides, levs], "list-mode foot")
+    om.backtrack(p, sides, levs)
+    assert_bitwise(om.get(L.F_FOOT)[sides, levs], full_f[sides, levs], "oracle list-mode foot")
+
+
+def _laplace_csr(nx, ny, shift):
+    n = nx * ny
+    rows, cols, vals = [], [], []
+    for j in range(ny):
+        for i in range(nx):
+            k = j * nx + i
+            rows.append(k), cols.append(k), vals.append(4.0 + shift)
+            for di, dj in ((1, 0), (-1, 0), (0, 1), (0, -1)):
+                a, bb = i + di, j + dj
+                if 0 <= a < nx and 0 <= bb < ny:
+                    rows.append(k), cols.append(bb * nx + a), vals.append(-1.0)
+    import scipy.sparse as sp
+
+    A = sp.csr_matrix((vals, (rows, cols)), shape=(n, n))
+    A.sort_indices()
+    return A
+
+
+def test_standalone_csr_against_scipy_and_oracle(ctx):
+    import scipy.sparse.linalg as sla
+
+    A = _laplace_csr(70, 50, 0.05)
+    b = cases.normal(11, A.shape[0])
+    s = core.Solver(None, ctx=ctx, indptr=A.indptr, indices=A.indices)
+    s.set_values(A.data)
+    x, it, st = s.solve(b, rtol=1e-12, maxiter=5000)
+    xo, ito, _, _ = O.pcg(A.indptr, A.indices, A.data, b, rtol=1e-12, maxiter=5000)
+    assert st == 0 and it == ito
+    assert_bitwise(x, xo, "stand-alone solution")
+    xs = sla.spsolve(A.tocsc(), b)
+    assert np.linalg.norm(x - xs) / np.linalg.norm(xs) < 1e-9
+
+
+def test_zero_rhs_and_maxiter(ctx):
+    A = _laplace_csr(30, 20, 0.1)
+    s = core.Solver(None, ctx=ctx, indptr=A.indptr, indices=A.indices)
+    s.set_values(A.data)
+    x, it, st = s.solve(np.zeros(A.shape[0]), rtol=1e-12, maxiter=100)
+    assert it == 0 and st == 0 and not x.any()
+    b = cases.normal(12, A.shape[0])
+    s.reset_history()
+    x, it, st = s.solve(b, rtol=1e-30, maxiter=25, check_every=10)
+    assert it == 25 and st == L.GSH_W_MAXITER
+    xo, ito, _, _ = O.pcg(A.indptr, A.indices, A.data, b, rtol=1e-30, maxiter=25, check_every=10)
+    assert ito == 25
+    assert_bitwise(x, xo, "maxiter solution")
+
+
+def test_mixed_precision_accuracy(ctx):
+    spec = cases.CaseSpec("channel-96x64-lognormal", 96, 64, 1000.0, nvrt=6, depth=None, substeps=5)
+    mesh, om, p = _setup(ctx, spec)
+    mesh.backtrack(p)
+    mesh.rhs_producers(p)
+    s64 = core.Solver(mesh, L.PREC_FP64)
+    s64.assemble(p)
+    x64, it64, _ = s64.solve(None, rtol=1e-12, maxiter=5000)
+    smx = core.Solver(mesh, L.PREC_MIXED)
+    smx.assemble(p)
+    xm, itm, stm = smx.solve(None, rtol=1e-12, maxiter=5000)
+    assert stm == 0
+    err = np.linalg.norm(xm - x64) / np.linalg.norm(x64)
+    print(f"[gpu] mixed rel err {err:.3e}, inner iters {itm} vs fp64 {it64}, outer {smx.stats().outer}")
+    assert err <= 1e-8
+
+
+def test_batch_equals_solo(ctx):
+    solvers, solo = [], []
+    for k, (nx, ny) in enumerate(((40, 30), (64, 20), (33, 47))):
+        spec = cases.CaseSpec(f"b{k}", nx, ny, 500.0, nvrt=5, depth=None, substeps=4)
+        mesh, om, p = _setup(ctx, spec)
+        mesh.backtrack(p)
+        mesh.rhs_producers(p)
+        s = core.Solver(mesh)
+        s.assemble(p)
+        x, it, st = s.solve(None, rtol=1e-12, maxiter=3000)
+        s.reset_history()
+        solo.append((x, it))
+        solvers.append((s, mesh))
+    its = core.batch_solve([s for s, _ in solvers], rtol=1e-12, maxiter=3000)
+    for (s, mesh), (x, it), itb in zip(solvers, solo, its):
+        assert itb == it
+        s.update_elevation()
+        assert_bitwise(mesh.get(L.F_ETA2), x, "batched solution")
