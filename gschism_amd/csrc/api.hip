@@ -153,6 +153,7 @@ struct gsh_solver {
   int64_t nnz = 0;
   DevPool pool;
   int32_t *rowptr = nullptr, *col = nullptr, *diag = nullptr;
+  int16_t* cdel = nullptr;
   double *val = nullptr, *b = nullptr, *x = nullptr, *h[3] = {nullptr, nullptr, nullptr};
   PcgSys<double> hs{};         // host copy of the FP64 system descriptor
   PcgSys<double>* ds = nullptr;
@@ -178,6 +179,21 @@ static void solver_alloc(gsh_solver* s, int32_t n, int64_t nnz, const std::vecto
   s->rowptr = P.upload(indptr);
   s->col = P.upload(indices);
   s->diag = P.upload(diag_abs);
+  // int16 column offsets from the first row of each 64-row chunk, when the bandwidth allows
+  {
+    std::vector<int16_t> cdel(nnz);
+    bool ok = true;
+    for (int32_t i = 0; i < n && ok; ++i)
+      for (int32_t k = indptr[i]; k < indptr[i + 1]; ++k) {
+        const int32_t d = indices[k] - (i & ~63);
+        if (d < -32768 || d > 32767) {
+          ok = false;
+          break;
+        }
+        cdel[k] = (int16_t)d;
+      }
+    s->cdel = ok ? P.upload(cdel) : nullptr;
+  }
   s->val = P.alloc<double>(nnz);
   s->b = P.alloc<double>(n);
   s->x = P.alloc<double>(n);
@@ -189,6 +205,7 @@ static void solver_alloc(gsh_solver* s, int32_t n, int64_t nnz, const std::vecto
   h.col = s->col;
   h.val = s->val;
   h.diag = s->diag;
+  h.cdel = s->cdel;
   h.b = s->b;
   h.x = s->x;
   h.r = P.alloc<double>(n);
@@ -215,6 +232,7 @@ static void solver_alloc(gsh_solver* s, int32_t n, int64_t nnz, const std::vecto
     g.col = s->col;
     g.val = s->val32;
     g.diag = s->diag;
+    g.cdel = s->cdel;
     g.b = s->r64;
     g.x = P.alloc<float>(n);
     g.r = P.alloc<float>(n);

@@ -78,7 +78,7 @@ struct PcgState {
   int32_t k, kk, done, zero;
   int32_t maxiter, check_every, pad0, pad1;
   double rtol, atol2, rr_last, bb;
-  double rho[2];
+  double rho[2];    // r.z used by iteration k (slot k & 1)
 };
 
 // One linear system of a (possibly batched) PCG launch. VT is the arithmetic type of the
@@ -90,6 +90,7 @@ struct PcgSys {
   const int32_t* col;
   const VT* val;
   const int32_t* diag;       // absolute index of the diagonal entry of each row
+  const int16_t* cdel;       // column - (row & ~63) when every offset fits int16, else NULL
   const double* b;           // FP64 right-hand side (MIXED inner solve: FP64 residual)
   VT* x;
   VT* r;

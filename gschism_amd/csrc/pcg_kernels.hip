@@ -4,14 +4,18 @@
 // f_update_xrz; pcg_loop.cpp:367-403) and sums dot products with one atomicAdd per block
 // (pcg_solver.py:134-167), so its results are not reproducible. Here every iteration is
 //   phase A  beta; p_new = z + beta p_old (own row and, on the fly, every column of the
-//            stencil); q = A p_new (CSR chunks staged through LDS, rows summed in column
-//            order); per-tile partial of p.q
-//   phase B  alpha; x += alpha p; r -= alpha q; z = invD r; per-tile partials of r.r, r.z
+//            stencil); q = A p_new (rows summed in column order); p.q
+//   phase B  alpha; x += alpha p; r -= alpha q; z = invD r; r.r, r.z; stopping test
 // with the same scalar recurrences as the reference (f_update_p / f_update_xrz,
-// pcg_solver.py:208-276). Dot products use one fixed tree: a tile of 256*R rows is summed
-// per thread over its R rows, then by a wave64 shuffle tree, then across the 4 waves; the
-// tile partials are summed by every consumer workgroup with the same tree. Results are
-// therefore bitwise reproducible and independent of how many workgroups run concurrently.
+// pcg_solver.py:208-276).
+//
+// Dot products use one fixed tree. A tile of 256*R rows is summed per thread over its R rows
+// (thread tau takes rows tile*256R + 256 j + tau), then by a wave64 shuffle tree, then as
+// (w0 + w1) + (w2 + w3) over the 4 waves; every consumer workgroup sums the tile partials with
+// the same tree (thread t takes partials t, t+256, ...). A single-writer-per-scalar scheme with
+// device-scope counters was measured slower: thousands of workgroups serialize on one atomic
+// address across the XCDs. Results are bitwise reproducible and independent of how many
+// workgroups run concurrently.
 //
 // A workgroup finds its system and tile through a launch table, so one launch advances any
 // number of independent systems (batched and served solves); finished systems drop out
@@ -21,6 +25,64 @@
 #include "device.h"
 
 namespace gsh {
+
+// Pointers read from a descriptor in device memory are generic to the compiler; casting them to
+// the global address space makes every access a global_load/global_store.
+template <class T>
+using gptr = __attribute__((address_space(1))) T*;
+template <class T>
+__device__ __forceinline__ gptr<T> G(T* p) {
+  return (gptr<T>)p;
+}
+
+template <class VT>
+struct SysView {
+  int n, ntiles;
+  gptr<const int32_t> rowptr, col, diag;
+  gptr<const int16_t> cdel;
+  gptr<const VT> val;
+  gptr<const double> b, h0, h1, h2;
+  gptr<VT> x, r, z, pa, pb, q, invd;
+  gptr<double> part_rr, part_rz, part_pq, part_bb;
+  gptr<PcgState> st;
+  bool has_hist, has_cdel;
+};
+
+template <class VT>
+__device__ __forceinline__ SysView<VT> view(const PcgSys<VT>* __restrict__ sys, int i) {
+  const PcgSys<VT>& S = sys[i];
+  SysView<VT> v;
+  v.n = S.n;
+  v.ntiles = S.ntiles;
+  v.rowptr = G(S.rowptr);
+  v.col = G(S.col);
+  v.diag = G(S.diag);
+  v.has_cdel = S.cdel != nullptr;
+  v.cdel = G(S.cdel);
+  v.val = G(S.val);
+  v.b = G(S.b);
+  v.has_hist = S.h0 != nullptr;
+  v.h0 = G(S.h0);
+  v.h1 = G(S.h1);
+  v.h2 = G(S.h2);
+  v.x = G(S.x);
+  v.r = G(S.r);
+  v.z = G(S.z);
+  v.pa = G(S.pa);
+  v.pb = G(S.pb);
+  v.q = G(S.q);
+  v.invd = G(S.invd);
+  v.part_rr = G(S.part_rr);
+  v.part_rz = G(S.part_rz);
+  v.part_pq = G(S.part_pq);
+  v.part_bb = G(S.part_bb);
+  v.st = G(S.st);
+  return v;
+}
+
+__device__ __forceinline__ int32_t load_done(gptr<PcgState> st) {
+  return *(volatile __attribute__((address_space(1))) int32_t*)&st->done;
+}
 
 __device__ __forceinline__ double wave_tree(double v) {
   v += __shfl_down(v, 32, 64);
@@ -45,7 +107,7 @@ __device__ __forceinline__ double block_tree(double v, double* red) {
 }
 
 // Sum of the tile partials P[0..T): thread t sums P[t], P[t+256], ... then block_tree.
-__device__ __forceinline__ double final_reduce(const double* __restrict__ P, int T, double* red) {
+__device__ __forceinline__ double final_reduce(gptr<const double> P, int T, double* red) {
   double u = 0.0;
   for (int m = threadIdx.x; m < T; m += kTpb) u += P[m];
   return block_tree(u, red);
@@ -56,7 +118,7 @@ template <class VT>
 __global__ void k_pcg_reset(const PcgSys<VT>* sys, int nsys, double rtol, int maxiter, int check_every) {
   const int sidx = blockIdx.x * blockDim.x + threadIdx.x;
   if (sidx >= nsys) return;
-  PcgState* st = sys[sidx].st;
+  gptr<PcgState> st = G(sys[sidx].st);
   st->k = 0;
   st->kk = 0;
   st->done = -1;
@@ -71,12 +133,12 @@ __global__ void k_pcg_reset(const PcgSys<VT>* sys, int nsys, double rtol, int ma
   st->rho[1] = 0.0;
 }
 
-// ------------------------------------------------------------------ CSR row sums
+// ------------------------------------------------------------------ CSR row sums (block staged)
 // Sums row i = r0 + threadIdx.x of A*v, v(c) given by `vf`, in column order. The chunk's
 // products are staged in LDS with coalesced loads of val/col when they fit.
 template <class VT, class VF>
-__device__ __forceinline__ VT csr_chunk(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
-                                        const VT* __restrict__ val, int n, int r0, VT* lds, VF vf) {
+__device__ __forceinline__ VT csr_chunk(gptr<const int32_t> rowptr, gptr<const int32_t> col, gptr<const VT> val,
+                                        int n, int r0, VT* lds, VF vf) {
   const int i = r0 + threadIdx.x;
   const int rend = min(r0 + kTpb, n);
   const int s0 = rowptr[r0], e0 = rowptr[rend];
@@ -105,13 +167,12 @@ __global__ void __launch_bounds__(256) k_pcg_init(const PcgSys<VT>* __restrict__
   __shared__ double red[4];
   __shared__ VT lds[kSpmvLdsCap];
   const TileRef tr = tiles[blockIdx.x];
-  const PcgSys<VT>& S = sys[tr.sys];
+  const SysView<VT> S = view(sys, tr.sys);
   const int n = S.n;
-  const double* h0 = S.h0;
-  const double* h1 = S.h1;
-  const double* h2 = S.h2;
+  const gptr<const double> h0 = S.h0, h1 = S.h1, h2 = S.h2;
+  const bool hist = S.has_hist;
   auto xe = [&](int c) -> VT {
-    if (!h0) return (VT)0;
+    if (!hist) return (VT)0;
     return (VT)((3.0 * h0[c] - 3.0 * h1[c]) + h2[c]);
   };
   double rr = 0.0, rz = 0.0, bb = 0.0;
@@ -149,20 +210,69 @@ __global__ void __launch_bounds__(256) k_pcg_init(const PcgSys<VT>* __restrict__
 
 // ------------------------------------------------------------------ phase A
 // f_update_p + SpMV + f_dot (pcg_solver.py:208-249): beta = rz(k-1)/rz(k-2) (0 at k = 0),
-// p = z + beta p, q = A p, p.q partial. Workgroup 0 of each system also records rho, runs the
-// stopping test of the previous iteration (pcg_loop.cpp:405-412) and, at k = 0,
-// atol = rtol ||b|| (pcg_solver.py:928-930, 1089-1095).
+// p = z + beta p, q = A p, p.q.
+//
+// Each wave owns the 64 consecutive rows r0w..r0w+63 of a sub-round (thread tau = 64w + lane
+// handles row base + 256 j + tau, the rows of the fixed reduction order). The wave loads the
+// chunk's nonzeros coalesced (lane l takes entries s0 + l + 64 m) together with the z and p_old
+// gathers; the products go to the wave's LDS slice and each lane sums its own row in column
+// order. Columns are read as int16 offsets from the chunk's first row when they fit.
+constexpr int kWaveCap = 640;   // nonzeros of a 64-row chunk held in registers (10 per lane)
+constexpr int kWaveM = kWaveCap / 64;
+
+template <class VT>
+struct Chunk {
+  int s0, e0;
+  VT v[kWaveM], zc[kWaveM], pc[kWaveM];
+};
+
+template <class VT>
+__device__ __forceinline__ int col_of(const SysView<VT>& S, int kk, int base64) {
+  return S.has_cdel ? base64 + (int)S.cdel[kk] : S.col[kk];
+}
+
+template <class VT>
+__device__ __forceinline__ void load_chunk(const SysView<VT>& S, gptr<const VT> po, int r0w, int lane, Chunk<VT>& c) {
+  const int rend = min(r0w + 64, S.n);
+  c.s0 = S.rowptr[r0w];
+  c.e0 = S.rowptr[rend];
+  if (c.e0 - c.s0 > kWaveCap) return;
+#pragma unroll
+  for (int m = 0; m < kWaveM; ++m) {
+    const int kk = c.s0 + lane + 64 * m;
+    if (kk < c.e0) {
+      const int col = col_of(S, kk, r0w);
+      c.v[m] = S.val[kk];
+      c.zc[m] = S.z[col];
+      c.pc[m] = po[col];
+    }
+  }
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <class VT, int R>
 __global__ void __launch_bounds__(256) k_pcg_phase_a(const PcgSys<VT>* __restrict__ sys,
                                                      const TileRef* __restrict__ tiles) {
   __shared__ double red[4];
-  __shared__ VT lds[kSpmvLdsCap];
+  __shared__ VT lds[4 * kWaveCap];
   const TileRef tr = tiles[blockIdx.x];
-  const PcgSys<VT>& S = sys[tr.sys];
-  PcgState* st = S.st;
-  if (*(volatile int32_t*)&st->done >= 0) return;
+  const SysView<VT> S = view(sys, tr.sys);
+  gptr<PcgState> st = S.st;
+  if (load_done(st) >= 0) return;
   const int k = st->k;
   const int n = S.n;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const gptr<VT> pn = (k & 1) ? S.pb : S.pa;
+  const gptr<const VT> po = (k & 1) ? S.pa : S.pb;
+  const int base = tr.tile * kTpb * R;
+  Chunk<VT> c;
+  const int r00 = base + w * 64;
+  if (r00 < n) load_chunk(S, po, r00, lane, c);
   const double rz_cur = final_reduce(S.part_rz, S.ntiles, red);
   double beta = 0.0;
   if (k > 0) {
@@ -170,6 +280,8 @@ __global__ void __launch_bounds__(256) k_pcg_phase_a(const PcgSys<VT>* __restric
     beta = (rz_old == 0.0) ? 0.0 : rz_cur / rz_old;
   }
   if (tr.tile == 0) {
+    // workgroup 0: atol = rtol ||b|| at k = 0 (pcg_solver.py:928-930, 1089-1095), otherwise the
+    // stopping test of iteration k-1 (pcg_loop.cpp:405-412)
     if (k == 0) {
       const double bb = final_reduce(S.part_bb, S.ntiles, red);
       if (threadIdx.x == 0) {
@@ -197,19 +309,36 @@ __global__ void __launch_bounds__(256) k_pcg_phase_a(const PcgSys<VT>* __restric
     }
   }
   const VT bv = (VT)beta;
-  VT* __restrict__ pn = (k & 1) ? S.pb : S.pa;
-  const VT* __restrict__ po = (k & 1) ? S.pa : S.pb;
-  const VT* __restrict__ z = S.z;
-  auto pf = [&](int c) -> VT { return z[c] + bv * po[c]; };
+  const gptr<const VT> z = S.z;
+  VT* my = lds + w * kWaveCap;
   double pq = 0.0;
-  const int base = tr.tile * kTpb * R;
   for (int j = 0; j < R; ++j) {
-    const int r0 = base + j * kTpb;
-    if (r0 >= n) break;
-    const VT q = csr_chunk<VT>(S.rowptr, S.col, S.val, n, r0, lds, pf);
-    const int i = r0 + threadIdx.x;
+    const int r0w = base + j * kTpb + w * 64;
+    if (r0w >= n) break;
+    if (j > 0) load_chunk(S, po, r0w, lane, c);
+    const int i = r0w + lane;
+    VT q = (VT)0;
+    if (c.e0 - c.s0 <= kWaveCap) {
+#pragma unroll
+      for (int m = 0; m < kWaveM; ++m) {
+        const int kk = c.s0 + lane + 64 * m;
+        if (kk < c.e0) my[kk - c.s0] = c.v[m] * (c.zc[m] + bv * c.pc[m]);
+      }
+      wave_lds_sync();
+      if (i < n) {
+        const int a = S.rowptr[i], b = S.rowptr[i + 1];
+        for (int kk = a; kk < b; ++kk) q += my[kk - c.s0];
+      }
+      wave_lds_sync();
+    } else if (i < n) {
+      const int a = S.rowptr[i], b = S.rowptr[i + 1];
+      for (int kk = a; kk < b; ++kk) {
+        const int col = col_of(S, kk, r0w);
+        q += S.val[kk] * (z[col] + bv * po[col]);
+      }
+    }
     if (i < n) {
-      const VT pi = pf(i);
+      const VT pi = z[i] + bv * po[i];
       pn[i] = pi;
       S.q[i] = q;
       pq += (double)pi * (double)q;
@@ -220,32 +349,45 @@ __global__ void __launch_bounds__(256) k_pcg_phase_a(const PcgSys<VT>* __restric
 }
 
 // ------------------------------------------------------------------ phase B
-// f_update_xrz (pcg_solver.py:253-276): alpha = rz / pq (0 if pq == 0).
+// f_update_xrz (pcg_solver.py:253-276): alpha = rz / pq (0 if pq == 0). The row loads are
+// issued before the scalar reduction, which only the arithmetic waits for.
 template <class VT, int R>
 __global__ void __launch_bounds__(256) k_pcg_phase_b(const PcgSys<VT>* __restrict__ sys,
                                                      const TileRef* __restrict__ tiles) {
   __shared__ double red[4];
   const TileRef tr = tiles[blockIdx.x];
-  const PcgSys<VT>& S = sys[tr.sys];
-  PcgState* st = S.st;
-  if (*(volatile int32_t*)&st->done >= 0) return;
+  const SysView<VT> S = view(sys, tr.sys);
+  gptr<PcgState> st = S.st;
+  if (load_done(st) >= 0) return;
   const int k = st->kk;
   const int n = S.n;
+  const gptr<const VT> p = (k & 1) ? S.pb : S.pa;
+  const int base = tr.tile * kTpb * R;
+  VT xv[R], pv[R], rv[R], qv[R], dv[R];
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    const int i = base + j * kTpb + threadIdx.x;
+    if (i < n) {
+      xv[j] = S.x[i];
+      pv[j] = p[i];
+      rv[j] = S.r[i];
+      qv[j] = S.q[i];
+      dv[j] = S.invd[i];
+    }
+  }
   const double pq = final_reduce(S.part_pq, S.ntiles, red);
   const double rz = st->rho[k & 1];
   const double alpha = (pq != 0.0) ? rz / pq : 0.0;
   if (tr.tile == 0 && threadIdx.x == 0) st->k = k + 1;
   const VT av = (VT)alpha;
-  const VT* __restrict__ p = (k & 1) ? S.pb : S.pa;
   double rr = 0.0, rzs = 0.0;
-  const int base = tr.tile * kTpb * R;
 #pragma unroll
   for (int j = 0; j < R; ++j) {
     const int i = base + j * kTpb + threadIdx.x;
     if (i < n) {
-      S.x[i] += av * p[i];
-      const VT ri = S.r[i] - av * S.q[i];
-      const VT zi = S.invd[i] * ri;
+      S.x[i] = xv[j] + av * pv[j];
+      const VT ri = rv[j] - av * qv[j];
+      const VT zi = dv[j] * ri;
       S.r[i] = ri;
       S.z[i] = zi;
       rr += (double)ri * (double)ri;
@@ -267,9 +409,9 @@ __global__ void __launch_bounds__(256) k_pcg_phase_b(const PcgSys<VT>* __restric
 template <class VT>
 __global__ void __launch_bounds__(256) k_pcg_check(const PcgSys<VT>* __restrict__ sys) {
   __shared__ double red[4];
-  const PcgSys<VT>& S = sys[blockIdx.x];
-  PcgState* st = S.st;
-  if (*(volatile int32_t*)&st->done >= 0) return;
+  const SysView<VT> S = view(sys, blockIdx.x);
+  gptr<PcgState> st = S.st;
+  if (load_done(st) >= 0) return;
   const int k = st->k;
   if (k < 1) return;
   const int it = k - 1;
@@ -287,7 +429,7 @@ template <class VT, int R>
 __global__ void __launch_bounds__(256) k_pcg_finish(const PcgSys<VT>* __restrict__ sys,
                                                     const TileRef* __restrict__ tiles) {
   const TileRef tr = tiles[blockIdx.x];
-  const PcgSys<VT>& S = sys[tr.sys];
+  const SysView<VT> S = view(sys, tr.sys);
   if (!S.st->zero) return;
   const int base = tr.tile * kTpb * R;
   for (int j = 0; j < R; ++j) {
@@ -301,12 +443,15 @@ __global__ void __launch_bounds__(256) k_pcg_finish(const PcgSys<VT>* __restrict
 template <int R>
 __global__ void __launch_bounds__(256) k_true_residual(const PcgSys<double>* __restrict__ sys,
                                                        const TileRef* __restrict__ tiles,
-                                                       const double* __restrict__ xv, double* __restrict__ rvec) {
+                                                       const double* __restrict__ xv_, double* __restrict__ rvec_) {
   __shared__ double red[4];
   __shared__ double lds[kSpmvLdsCap];
   const TileRef tr = tiles[blockIdx.x];
-  const PcgSys<double>& S = sys[tr.sys];
+  const SysView<double> S = view(sys, tr.sys);
   const int n = S.n;
+  const gptr<const double> xv = G(xv_);
+  const gptr<double> rvec = G(rvec_);
+  const bool want_r = rvec_ != nullptr;
   auto xf = [&](int c) -> double { return xv[c]; };
   double rr = 0.0, bb = 0.0;
   const int base = tr.tile * kTpb * R;
@@ -317,7 +462,7 @@ __global__ void __launch_bounds__(256) k_true_residual(const PcgSys<double>* __r
     const int i = r0 + threadIdx.x;
     if (i < n) {
       const double ri = S.b[i] - q;
-      if (rvec) rvec[i] = ri;
+      if (want_r) rvec[i] = ri;
       rr += ri * ri;
       bb += S.b[i] * S.b[i];
     }
@@ -333,8 +478,8 @@ __global__ void __launch_bounds__(256) k_true_residual(const PcgSys<double>* __r
 __global__ void k_reduce_pair(const double* __restrict__ pa, const double* __restrict__ pb, int T,
                               double* __restrict__ out) {
   __shared__ double red[4];
-  const double a = final_reduce(pa, T, red);
-  const double b = final_reduce(pb, T, red);
+  const double a = final_reduce(G(pa), T, red);
+  const double b = final_reduce(G(pb), T, red);
   if (threadIdx.x == 0) {
     out[0] = a;
     out[1] = b;

@@ -1217,7 +1217,7 @@ int32_t or_pcg(int32_t n, const int32_t* indptr, const int32_t* indices, const d
     rz_old = rz_cur;
     rz_cur = rz_new;
     if ((it + 1) % checkEvery == 0 || it == maxiter - 1) {
-      if (it != maxiter - 1) *rr_last = rr;
+      *rr_last = rr;
       if (rr <= atol2) {
         result = it + 1;
         break;
