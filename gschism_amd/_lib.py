@@ -111,6 +111,7 @@ SIGNATURES = {
     "gsh_solver_get_values": (_I, [_P, _PD]),
     "gsh_solver_vec_ptr": (_I, [_P, _PP, _PP]),
     "gsh_solver_get_rhs": (_I, [_P, _PD]),
+    "gsh_solver_get_solution": (_I, [_P, _PD]),
     "gsh_assemble": (_I, [_P, _P, C.POINTER(StepParams), _D]),
     "gsh_solver_solve": (_I, [_P, _PD, _PD, _D, _I, _I, _D, C.POINTER(_I), C.POINTER(_I)]),
     "gsh_solver_check": (_I, [_P, _PD]),

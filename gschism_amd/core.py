@@ -232,6 +232,11 @@ class Solver:
         check(L.lib().gsh_solver_get_values(self.h, dptr(v)))
         return v
 
+    def get_solution(self) -> np.ndarray:
+        x = np.empty(self.n, dtype=np.float64)
+        check(L.lib().gsh_solver_get_solution(self.h, dptr(x)))
+        return x
+
     def get_rhs(self) -> np.ndarray:
         b = np.empty(self.n, dtype=np.float64)
         check(L.lib().gsh_solver_get_rhs(self.h, dptr(b)))

@@ -1459,3 +1459,13 @@ extern "C" int gsh_solver_get_rhs(gsh_solver* s, double* b_host) {
     HIPCHK(hipMemcpy(b_host, s->b, sizeof(double) * s->n, hipMemcpyDeviceToHost));
   })
 }
+
+extern "C" int gsh_solver_get_solution(gsh_solver* s, double* x_host) {
+  GSH_API_BODY({
+    GSH_REQUIRE(s && x_host, GSH_E_ARG, "NULL argument");
+    s->ctx->set();
+    HIPCHK(hipStreamSynchronize(s->ctx->stream));
+    // after a solve the solution is the newest history entry (pointer rotation of rollHistory)
+    HIPCHK(hipMemcpy(x_host, s->h[0], sizeof(double) * s->n, hipMemcpyDeviceToHost));
+  })
+}

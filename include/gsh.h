@@ -195,6 +195,8 @@ int gsh_solver_get_values(gsh_solver* s, double* data_host);
 int gsh_solver_vec_ptr(gsh_solver* s, double** b, double** x);
 /* Host copy of the right-hand side b (gatherBFromMesh target, pcg_solver.py:950-954). */
 int gsh_solver_get_rhs(gsh_solver* s, double* b_host);
+/* Host copy of the last solution (scatterXToMesh source, pcg_solver.py:957-961). */
+int gsh_solver_get_solution(gsh_solver* s, double* x_host);
 
 /* a11+a12: setupCoefficientMatrixAndRHS + setEssentialBC (meshdata.cu:3182-3474, 4400-4446)
  * into the solver's CSR values and b (= qel). */

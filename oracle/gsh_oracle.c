@@ -1248,3 +1248,23 @@ double or_true_rel(int32_t n, const int32_t* indptr, const int32_t* indices, con
   const double rn = sqrt(rr), bn = sqrt(bb);
   return bn > 0.0 ? rn / bn : rn;
 }
+
+/* ------------------------------------------------------------------ test hooks */
+int or_geometry(const or_mesh* m, int what, double* out) {
+  switch (what) {
+    case 0: memcpy(out, m->area, 8 * (size_t)m->ne); return 0;
+    case 1: memcpy(out, m->gradphi, 48 * (size_t)m->ne); return 0;
+    case 2:
+      for (int s = 0; s < m->ns; ++s) out[2 * s] = m->scx[s], out[2 * s + 1] = m->scy[s];
+      return 0;
+    case 3: memcpy(out, m->slen, 8 * (size_t)m->ns); return 0;
+    case 4:
+      for (int s = 0; s < m->ns; ++s) out[2 * s] = m->snx[s], out[2 * s + 1] = m->sny[s];
+      return 0;
+    default: set_err("unknown geometry array %d", what); return -1;
+  }
+}
+
+void or_interp(const or_mesh* m, int32_t e, double x, double y, double z, double h0, double out[3]) {
+  interpolate_velocities(m, e, x, y, z, h0, out);
+}

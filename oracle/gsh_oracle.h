@@ -42,6 +42,12 @@ void or_rhs(or_mesh* m, const or_params* p);
 /* Assembly + essential BC into val (CSR of the mesh) and b. */
 void or_assemble(or_mesh* m, const or_params* p, double* val, double* b, int nthreads);
 
+/* Geometry arrays: 0 area [ne], 1 gradphi [ne][3][2], 2 side centre [ns][2], 3 side length [ns],
+ * 4 side normal [ns][2]. Returns 0 on success. */
+int or_geometry(const or_mesh* m, int what, double* out);
+/* Element::interpolateVelocities at (x, y, z) in element e with the current state (test hook). */
+void or_interp(const or_mesh* m, int32_t e, double x, double y, double z, double h0, double out[3]);
+
 /* Jacobi PCG with the reference's recurrences; dot products in the fixed tile/wave tree of the
  * device kernels (rows_per_thread as the device solver's). h0/h1/h2 may be NULL (zero guess).
  * Returns the iteration count; *rr_last = last checked r.r, *atol2 = (rtol ||b||)^2. */

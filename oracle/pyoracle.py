@@ -38,6 +38,8 @@ _SIG = {
     "or_assemble": (None, [_P, C.POINTER(OrParams), _PD, _PD, C.c_int]),
     "or_pcg": (C.c_int32, [C.c_int32, _PI32, _PI32, _PD, _PD, _PD, _PD, _PD, C.c_double, C.c_int32, C.c_int32,
                            C.c_int32, _PD, _PD, _PD, C.c_int]),
+    "or_geometry": (C.c_int, [_P, C.c_int, _PD]),
+    "or_interp": (None, [_P, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double, _PD]),
     "or_dot_canon": (C.c_double, [C.c_int32, _PD, _PD, C.c_int32]),
     "or_true_rel": (C.c_double, [C.c_int32, _PI32, _PI32, _PD, _PD, _PD, C.c_int32]),
 }
@@ -154,6 +156,18 @@ class OracleMesh:
             s = np.ascontiguousarray(sides, dtype=np.int32)
             k = np.ascontiguousarray(levels, dtype=np.int32)
             lib().or_backtrack(self.h, C.byref(params(p)), _i(s), _i(k), s.size, nthreads)
+
+    def geometry(self, what: int) -> np.ndarray:
+        shp = {0: (self.ne,), 1: (self.ne, 3, 2), 2: (self.ns, 2), 3: (self.ns,), 4: (self.ns, 2)}[what]
+        out = np.empty(shp, dtype=np.float64)
+        if lib().or_geometry(self.h, what, _d(out)):
+            raise RuntimeError(lib().or_error().decode())
+        return out
+
+    def interp(self, e: int, x: float, y: float, z: float, h0: float = 0.01) -> np.ndarray:
+        out = np.empty(3, dtype=np.float64)
+        lib().or_interp(self.h, e, x, y, z, h0, _d(out))
+        return out
 
     def rhs(self, p) -> None:
         lib().or_rhs(self.h, C.byref(params(p)))

@@ -231,3 +231,51 @@ def test_batch_equals_solo(ctx):
         assert itb == it
         s.update_elevation()
         assert_bitwise(mesh.get(L.F_ETA2), x, "batched solution")
+
+
+def test_full_size_1m_channel(ctx):
+    """The 1024 x 1024 log-normal channel (1.05M nodes, 31 levels): a sample of backtracks
+    against the oracle's list mode; RHS, assembly and the whole FP64 solve bit for bit."""
+    spec = cases.spec_gpu_1m()
+    data = cases.make_case(spec)
+    print(f"[gpu] 1M case {spec.name}: building", flush=True)
+    mesh, p, (eta, uvw, su2) = cases.load_mesh(ctx, data)
+    om = oracle_mesh_for(mesh, data.sigma)
+    om.set(L.F_ETA2, eta)
+    om.set(L.F_NODE_UVW, uvw)
+    om.set(L.F_SIDE_U2, su2)
+    om.levels(p)
+    del uvw
+    print("[gpu] 1M backtrack", flush=True)
+    mesh.backtrack(p)
+    rng = np.random.default_rng(11)
+    cnt = 1 << 15
+    pick = np.unique(rng.integers(0, mesh.ns * mesh.nvrt, cnt))
+    s = (pick // mesh.nvrt).astype(np.int32)
+    k = (pick % mesh.nvrt).astype(np.int32)
+    om.backtrack(p, sides=s, levels=k, nthreads=16)
+    foot, btu, bst = mesh.get(L.F_FOOT), mesh.get(L.F_BTU), mesh.get(L.F_BSTATUS)
+    assert_bitwise(foot[s, k], om.get(L.F_FOOT)[s, k], "1M sample foot elements")
+    assert_bitwise(btu[s, k], om.get(L.F_BTU)[s, k], "1M sample backtracked velocity")
+    assert_bitwise(bst[s, k], om.get(L.F_BSTATUS)[s, k], "1M sample status")
+    assert (foot >= 0).mean() > 0.9
+    # the oracle continues from the device's (sample-checked) backtracked velocities
+    om.set(L.F_BTU, btu)
+    del foot, btu, bst
+    print("[gpu] 1M rhs + assembly", flush=True)
+    mesh.rhs_producers(p)
+    om.rhs(p)
+    solver = core.Solver(mesh)
+    solver.assemble(p)
+    val, b = om.assemble(p, nthreads=16)
+    assert_bitwise(solver.get_values(), val, "1M CSR values")
+    assert_bitwise(solver.get_rhs(), b, "1M RHS")
+    print("[gpu] 1M solve", flush=True)
+    x, it, st = solver.solve(None, rtol=1e-12, maxiter=3000)
+    xo, ito, _, _ = O.pcg(mesh.topology(L.T_CSR_INDPTR), mesh.topology(L.T_CSR_INDICES), val, b, rtol=1e-12,
+                          maxiter=3000, nthreads=16)
+    assert st == 0 and it == ito, (it, ito, st)
+    assert_bitwise(x, xo, "1M PCG solution")
+    assert solver.check() < 1e-10
+    solver.close()
+    mesh.close()
