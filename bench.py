@@ -114,7 +114,12 @@ def main() -> int:
     L.check(L.lib().gsh_solver_profile(solver.h, rtol, maxiter, check_every, ma, mb, itp))
     ms_a, ms_b = ma.value, mb.value
     n, nnz = mesh.n, mesh.nnz
-    bytes_a = 12 * nnz + 4 * (n + 1) + 32 * n      # val+col, rowptr, z, p_old, p_new, q
+    ip_h, ix_h = mesh.topology(L.T_CSR_INDPTR), mesh.topology(L.T_CSR_INDICES)
+    cdel = ix_h - (np.repeat(np.arange(n, dtype=np.int64), np.diff(ip_h)) & ~63)
+    col_bytes = 2 if (cdel.min() >= -32768 and cdel.max() <= 32767) else 4   # the solver's column format
+    del cdel
+    # phase A compulsory bytes: values + columns, row pointers, read z and p_old, write p_new and q
+    bytes_a = (8 + col_bytes) * nnz + 4 * (n + 1) + 32 * n
     bytes_b = 64 * n                               # read x r p q invD, write x r z
     bytes_iter_survey = 12 * nnz + 4 * (n + 1) + 104 * n
     gbs_a = bytes_a / (ms_a * 1e-3) / 1e9
@@ -172,8 +177,7 @@ def main() -> int:
     if not args.no_cpu_baseline:
         from oracle import pyoracle as O
 
-        ip = mesh.topology(L.T_CSR_INDPTR)
-        ix = mesh.topology(L.T_CSR_INDICES)
+        ip, ix = ip_h, ix_h
         val = solver.get_values()
         om_b = solver.get_rhs()
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
@@ -208,7 +212,7 @@ def main() -> int:
                    "dt": p.dt, "theta": p.theta},
         "roofline": {"bound": "hbm", "kernel": "k_pcg_phase_a", "achieved": round(gbs_a, 1), "peak": PEAK_HBM_GBS,
                      "unit": "GB/s", "frac": round(gbs_a / PEAK_HBM_GBS, 4), "traffic": traffic,
-                     "bytes_per_launch": bytes_a, "us_per_launch": round(1e3 * ms_a, 2)},
+                     "bytes_per_launch": bytes_a, "column_bytes": col_bytes, "us_per_launch": round(1e3 * ms_a, 2)},
         "phase_b": {"achieved_gbs": round(gbs_b, 1), "bytes_per_launch": bytes_b, "us_per_launch": round(1e3 * ms_b, 2)},
         "iteration": {"us": round(1e3 * ms_iter, 2), "gbs_192n": round(gbs_iter, 1)},
         "copy_probe_gbs": round(copy_gbs, 1),

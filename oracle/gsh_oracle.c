@@ -395,6 +395,12 @@ or_mesh* or_mesh_new(int32_t n, int32_t ne, int32_t nvrt, const double* xy, cons
     for (int i = 0; i < n; ++i) m->open[i] = open_bnd[i] ? 1 : 0;
   m->kb = xcalloc(n, sizeof(int32_t));
   if (kb) memcpy(m->kb, kb, sizeof(int32_t) * n);
+  for (int i = 0; i < n; ++i)
+    if (m->kb[i] < 0 || m->kb[i] > nvrt - 2) {
+      set_err("kb must be in [0, nvrt-2] (node %d)", i);
+      or_mesh_free(m);
+      return NULL;
+    }
   m->elnode = xcalloc(3 * (size_t)ne, sizeof(int32_t));
   memcpy(m->elnode, elnode, sizeof(int32_t) * 3 * (size_t)ne);
   if (build_topology(m) || build_geometry(m)) {
@@ -1247,6 +1253,345 @@ double or_true_rel(int32_t n, const int32_t* indptr, const int32_t* indices, con
   free(q);
   const double rn = sqrt(rr), bn = sqrt(bb);
   return bn > 0.0 ? rn / bn : rn;
+}
+
+/* ------------------------------------------------------------------ multicolour SSOR PCG
+ * The reference has only the Jacobi preconditioner (pcg_solver.py:933-938); the SSOR variant
+ * keeps its driver (extrapolated guess, atol = rtol ||b||, stopping rule checked every
+ * check_every iterations on ||b - A x||^2, beta/alpha recurrences, pcg_solver.py:881-947,
+ * 1075-1124, pcg_loop.cpp:361-413) and replaces the preconditioner by symmetric SOR
+ *   M = (D/w + L) (D/w)^-1 (D/w + U)
+ * in a multicolour ordering, applied with Eisenstat's trick (S. C. Eisenstat, "Efficient
+ * implementation of a class of preconditioned conjugate gradient methods", SIAM J. Sci. Stat.
+ * Comput. 2 (1981) 1-4): CG runs on A^ = (D~+L)^-1 A (D~+U)^-1 (D~ = D/w) with preconditioner D~,
+ *   A^ v = t + (D~+L)^-1 (v - (2D~ - D) t),   t = (D~+U)^-1 v,
+ * so one iteration reads the off-diagonal matrix once. Iterates map back as x = (D~+U)^-1 y and
+ * the true residual is b - A x = (D~+L) r.
+ *
+ * Layout: nodes are coloured greedily (first fit) in smallest-last order (D. W. Matula and
+ * L. L. Beck, J. ACM 30 (1983) 417-427), ties broken by the smaller index; rows are renumbered
+ * colour-major, ascending original index inside a colour, and each colour block is padded
+ * to a multiple of the 256*R-row reduction tile with identity rows (d = 1, b = 0). Dot products
+ * use the fixed tile tree over the padded order. Row sums run over ascending new column index. */
+
+typedef struct {
+  int64_t key; /* degree << 32 | index */
+} hent;
+
+static void heap_push(hent* h, int64_t* len, int64_t key) {
+  int64_t i = (*len)++;
+  while (i > 0) {
+    const int64_t par = (i - 1) / 2;
+    if (h[par].key <= key) break;
+    h[i] = h[par];
+    i = par;
+  }
+  h[i].key = key;
+}
+
+static int64_t heap_pop(hent* h, int64_t* len) {
+  const int64_t top = h[0].key;
+  const int64_t last = h[--(*len)].key;
+  int64_t i = 0;
+  for (;;) {
+    int64_t c = 2 * i + 1;
+    if (c >= *len) break;
+    if (c + 1 < *len && h[c + 1].key < h[c].key) c++;
+    if (last <= h[c].key) break;
+    h[i] = h[c];
+    i = c;
+  }
+  if (*len > 0) h[i].key = last;
+  return top;
+}
+
+int32_t or_colour(int32_t n, const int32_t* indptr, const int32_t* indices, int32_t* colour) {
+  int32_t* deg = xcalloc(n > 0 ? n : 1, sizeof(int32_t));
+  char* removed = xcalloc(n > 0 ? n : 1, 1);
+  int32_t* order = xcalloc(n > 0 ? n : 1, sizeof(int32_t));
+  int64_t cap = (int64_t)n + indptr[n] + 1, len = 0;
+  hent* h = xcalloc(cap, sizeof(hent));
+  for (int32_t i = 0; i < n; ++i) {
+    for (int32_t k = indptr[i]; k < indptr[i + 1]; ++k)
+      if (indices[k] != i) deg[i]++;
+    heap_push(h, &len, ((int64_t)deg[i] << 32) | i);
+  }
+  int32_t cnt = 0;
+  while (len > 0) {
+    const int64_t key = heap_pop(h, &len);
+    const int32_t i = (int32_t)(key & 0xffffffff), d = (int32_t)(key >> 32);
+    if (removed[i] || d != deg[i]) continue;
+    removed[i] = 1;
+    order[cnt++] = i;
+    for (int32_t k = indptr[i]; k < indptr[i + 1]; ++k) {
+      const int32_t j = indices[k];
+      if (j != i && !removed[j]) {
+        deg[j]--;
+        heap_push(h, &len, ((int64_t)deg[j] << 32) | j);
+      }
+    }
+  }
+  int32_t ncol = 0;
+  int32_t* stamp = xcalloc((size_t)n + 2, sizeof(int32_t));
+  for (int32_t i = 0; i < n; ++i) colour[i] = -1;
+  for (int32_t m = n - 1; m >= 0; --m) {
+    const int32_t i = order[m];
+    for (int32_t k = indptr[i]; k < indptr[i + 1]; ++k) {
+      const int32_t j = indices[k];
+      if (j != i && colour[j] >= 0) stamp[colour[j]] = m + 1;
+    }
+    int32_t c = 0;
+    while (stamp[c] == m + 1) c++;
+    colour[i] = c;
+    if (c + 1 > ncol) ncol = c + 1;
+  }
+  free(stamp);
+  free(h);
+  free(order);
+  free(removed);
+  free(deg);
+  return ncol;
+}
+
+typedef struct {
+  int32_t n, np, ncol;
+  int32_t* cstart;  /* [ncol+1] padded block starts */
+  int32_t* perm;    /* [np] new -> old, -1 for padding */
+  int32_t* newpos;  /* [n] old -> new */
+  int32_t *lptr, *lcol, *uptr, *ucol; /* [np+1], [*] */
+  double *lval, *uval, *d;            /* off-diagonal values, diagonal [np] */
+} ssor_layout;
+
+static int cmp_pair(const void* a, const void* b) {
+  const int64_t x = *(const int64_t*)a, y = *(const int64_t*)b;
+  return (x > y) - (x < y);
+}
+
+static int ssor_build(ssor_layout* L, int32_t n, const int32_t* indptr, const int32_t* indices, const double* val,
+                      int32_t R) {
+  memset(L, 0, sizeof(*L));
+  const int32_t T = 256 * R;
+  int32_t* colour = xcalloc(n > 0 ? n : 1, sizeof(int32_t));
+  const int32_t C = n > 0 ? or_colour(n, indptr, indices, colour) : 0;
+  int32_t* count = xcalloc((size_t)C + 1, sizeof(int32_t));
+  for (int32_t i = 0; i < n; ++i) count[colour[i]]++;
+  L->n = n;
+  L->ncol = C;
+  L->cstart = xcalloc((size_t)C + 1, sizeof(int32_t));
+  for (int32_t c = 0; c < C; ++c) L->cstart[c + 1] = L->cstart[c] + (count[c] + T - 1) / T * T;
+  const int32_t np = L->cstart[C];
+  L->np = np;
+  L->perm = xcalloc(np > 0 ? np : 1, sizeof(int32_t));
+  L->newpos = xcalloc(n > 0 ? n : 1, sizeof(int32_t));
+  for (int32_t v = 0; v < np; ++v) L->perm[v] = -1;
+  int32_t* fill = xcalloc((size_t)C + 1, sizeof(int32_t));
+  for (int32_t i = 0; i < n; ++i) {
+    const int32_t c = colour[i];
+    const int32_t v = L->cstart[c] + fill[c]++;
+    L->perm[v] = i;
+    L->newpos[i] = v;
+  }
+  L->lptr = xcalloc((size_t)np + 1, sizeof(int32_t));
+  L->uptr = xcalloc((size_t)np + 1, sizeof(int32_t));
+  L->d = xcalloc(np > 0 ? np : 1, sizeof(double));
+  int bad = 0;
+  for (int32_t v = 0; v < np; ++v) {
+    const int32_t i = L->perm[v];
+    int32_t nl = 0, nu = 0;
+    if (i >= 0) {
+      const int32_t c = colour[i];
+      int has_diag = 0;
+      for (int32_t k = indptr[i]; k < indptr[i + 1]; ++k) {
+        const int32_t j = indices[k];
+        if (j == i) {
+          has_diag = 1;
+          continue;
+        }
+        const int32_t w = L->newpos[j];
+        if (w < L->cstart[c]) nl++;
+        else if (w >= L->cstart[c + 1]) nu++;
+        else bad = 1;
+      }
+      if (!has_diag) bad = 1;
+    }
+    L->lptr[v + 1] = L->lptr[v] + nl;
+    L->uptr[v + 1] = L->uptr[v] + nu;
+  }
+  L->lcol = xcalloc((size_t)L->lptr[np] + 1, sizeof(int32_t));
+  L->ucol = xcalloc((size_t)L->uptr[np] + 1, sizeof(int32_t));
+  L->lval = xcalloc((size_t)L->lptr[np] + 1, sizeof(double));
+  L->uval = xcalloc((size_t)L->uptr[np] + 1, sizeof(double));
+  int64_t* tmp = xcalloc((size_t)64 + (indptr[n] > 0 ? indptr[n] : 1), sizeof(int64_t));
+  for (int32_t v = 0; v < np; ++v) {
+    const int32_t i = L->perm[v];
+    if (i < 0) {
+      L->d[v] = 1.0;
+      continue;
+    }
+    const int32_t c = colour[i];
+    int32_t m = 0;
+    for (int32_t k = indptr[i]; k < indptr[i + 1]; ++k) {
+      const int32_t j = indices[k];
+      if (j == i) {
+        L->d[v] = val[k];
+        continue;
+      }
+      tmp[m++] = ((int64_t)L->newpos[j] << 32) | (int64_t)(k - indptr[i]);
+    }
+    qsort(tmp, m, sizeof(int64_t), cmp_pair);
+    int32_t a = L->lptr[v], b = L->uptr[v];
+    for (int32_t q = 0; q < m; ++q) {
+      const int32_t w = (int32_t)(tmp[q] >> 32), k = indptr[i] + (int32_t)(tmp[q] & 0xffffffff);
+      if (w < L->cstart[c]) L->lcol[a] = w, L->lval[a++] = val[k];
+      else L->ucol[b] = w, L->uval[b++] = val[k];
+    }
+  }
+  free(tmp);
+  free(fill);
+  free(count);
+  free(colour);
+  if (bad) set_err("SSOR layout: a row lacks its diagonal or the colouring is invalid");
+  return bad ? -1 : 0;
+}
+
+static void ssor_free(ssor_layout* L) {
+  free(L->cstart);
+  free(L->perm);
+  free(L->newpos);
+  free(L->lptr);
+  free(L->lcol);
+  free(L->uptr);
+  free(L->ucol);
+  free(L->lval);
+  free(L->uval);
+  free(L->d);
+}
+
+static inline double lsum(const ssor_layout* L, int32_t v, const double* x) {
+  double acc = 0.0;
+  for (int32_t k = L->lptr[v]; k < L->lptr[v + 1]; ++k) acc += L->lval[k] * x[L->lcol[k]];
+  return acc;
+}
+
+static inline double usum(const ssor_layout* L, int32_t v, const double* x) {
+  double acc = 0.0;
+  for (int32_t k = L->uptr[v]; k < L->uptr[v + 1]; ++k) acc += L->uval[k] * x[L->ucol[k]];
+  return acc;
+}
+
+int32_t or_pcg_ssor(int32_t n, const int32_t* indptr, const int32_t* indices, const double* val, const double* b,
+                    const double* h0, const double* h1, const double* h2, double omega, double rtol,
+                    int32_t maxiter, int32_t checkEvery, int32_t R, double* x, double* rr_last, double* atol2_out,
+                    int32_t* ncolours, int nthreads) {
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+  (void)nthreads;
+  if (maxiter <= 0) maxiter = 10 * n;
+  if (checkEvery < 1) checkEvery = 1;
+  ssor_layout L;
+  if (ssor_build(&L, n, indptr, indices, val, R) != 0) {
+    ssor_free(&L);
+    return -1;
+  }
+  if (ncolours) *ncolours = L.ncol;
+  const int32_t np = L.np, C = L.ncol;
+  const int64_t T = np / (256 * (int64_t)R);
+  double* part = xcalloc(T > 0 ? T : 1, sizeof(double));
+  double *dt = xcalloc(np + 1, sizeof(double)), *kd = xcalloc(np + 1, sizeof(double));
+  double *bp = xcalloc(np + 1, sizeof(double)), *xp = xcalloc(np + 1, sizeof(double));
+  double *y = xcalloc(np + 1, sizeof(double)), *r = xcalloc(np + 1, sizeof(double));
+  double *z = xcalloc(np + 1, sizeof(double)), *p = xcalloc(np + 1, sizeof(double));
+  double *t = xcalloc(np + 1, sizeof(double)), *s = xcalloc(np + 1, sizeof(double));
+  double *q = xcalloc(np + 1, sizeof(double)), *w = xcalloc(np + 1, sizeof(double));
+  for (int32_t v = 0; v < np; ++v) {
+    dt[v] = L.d[v] / omega;
+    kd[v] = 2.0 * dt[v] - L.d[v];
+    const int32_t i = L.perm[v];
+    bp[v] = i >= 0 ? b[i] : 0.0;
+    xp[v] = (i >= 0 && h0) ? (3.0 * h0[i] - 3.0 * h1[i]) + h2[i] : 0.0;
+  }
+  const double atol = rtol * sqrt(canon_dot(np, bp, bp, R, part));
+  const double atol2 = atol * atol;
+  *atol2_out = atol2;
+  *rr_last = -1.0;
+  int32_t result;
+  if (atol == 0.0) {
+    for (int32_t i = 0; i < n; ++i) x[i] = 0.0;
+    result = 0;
+    goto done;
+  }
+  /* r_true = b - A x0 and y0 = (D~+U) x0 */
+  for (int32_t v = 0; v < np; ++v) {
+    const double au = usum(&L, v, xp);
+    const double ax = (lsum(&L, v, xp) + L.d[v] * xp[v]) + au;
+    w[v] = bp[v] - ax;
+    y[v] = dt[v] * xp[v] + au;
+  }
+  /* r0 = (D~+L)^-1 r_true, colour by colour */
+  for (int32_t c = 0; c < C; ++c)
+    for (int32_t v = L.cstart[c]; v < L.cstart[c + 1]; ++v) r[v] = (w[v] - lsum(&L, v, r)) / dt[v];
+  for (int32_t v = 0; v < np; ++v) {
+    z[v] = dt[v] * r[v];
+    p[v] = 0.0;
+  }
+  double rz_cur = canon_dot(np, r, z, R, part), rz_old = 0.0;
+  result = maxiter;
+  for (int32_t it = 0; it < maxiter; ++it) {
+    const int k = it;
+    const double beta = (k == 0 || rz_old == 0.0) ? 0.0 : rz_cur / rz_old;
+    /* t = (D~+U)^-1 p, p = z + beta p: last colour first */
+    for (int32_t c = C - 1; c >= 0; --c)
+      for (int32_t v = L.cstart[c]; v < L.cstart[c + 1]; ++v) {
+        p[v] = dt[v] * r[v] + beta * p[v];
+        t[v] = (p[v] - usum(&L, v, t)) / dt[v];
+      }
+    /* s = (D~+L)^-1 (p - (2D~ - D) t), q = t + s: first colour first */
+    for (int32_t c = 0; c < C; ++c)
+      for (int32_t v = L.cstart[c]; v < L.cstart[c + 1]; ++v) {
+        s[v] = ((p[v] - kd[v] * t[v]) - lsum(&L, v, s)) / dt[v];
+        q[v] = t[v] + s[v];
+      }
+    const double pq = canon_dot(np, p, q, R, part);
+    const double alpha = (pq != 0.0) ? rz_cur / pq : 0.0;
+    for (int32_t v = 0; v < np; ++v) {
+      y[v] += alpha * p[v];
+      r[v] = r[v] - alpha * q[v];
+      z[v] = dt[v] * r[v];
+    }
+    const double rz_new = canon_dot(np, r, z, R, part);
+    rz_old = rz_cur;
+    rz_cur = rz_new;
+    if ((it + 1) % checkEvery == 0 || it == maxiter - 1) {
+      for (int32_t v = 0; v < np; ++v) w[v] = dt[v] * r[v] + lsum(&L, v, r);
+      const double rr = canon_dot(np, w, w, R, part);
+      *rr_last = rr;
+      if (rr <= atol2) {
+        result = it + 1;
+        break;
+      }
+    }
+  }
+  /* x = (D~+U)^-1 y, back to the original numbering */
+  for (int32_t c = C - 1; c >= 0; --c)
+    for (int32_t v = L.cstart[c]; v < L.cstart[c + 1]; ++v) xp[v] = (y[v] - usum(&L, v, xp)) / dt[v];
+  for (int32_t i = 0; i < n; ++i) x[i] = xp[L.newpos[i]];
+done:
+  free(part);
+  free(dt);
+  free(kd);
+  free(bp);
+  free(xp);
+  free(y);
+  free(r);
+  free(z);
+  free(p);
+  free(t);
+  free(s);
+  free(q);
+  free(w);
+  ssor_free(&L);
+  return result;
 }
 
 /* ------------------------------------------------------------------ test hooks */

@@ -55,6 +55,16 @@ int32_t or_pcg(int32_t n, const int32_t* indptr, const int32_t* indices, const d
                const double* h0, const double* h1, const double* h2, double rtol, int32_t maxiter,
                int32_t check_every, int32_t rows_per_thread, double* x_out, double* rr_last, double* atol2,
                int nthreads);
+/* Greedy first-fit colouring in smallest-last order of the graph of a CSR pattern (diagonal
+ * ignored; ties by smaller index). Returns the number of colours. */
+int32_t or_colour(int32_t n, const int32_t* indptr, const int32_t* indices, int32_t* colour);
+/* PCG with the multicolour SSOR(omega) preconditioner applied by Eisenstat's trick, the Jacobi
+ * driver's stopping rule on ||b - A x||, dot products in the device tree over the colour-major
+ * padded order. Returns the iteration count, -1 on a bad pattern (or_error()). */
+int32_t or_pcg_ssor(int32_t n, const int32_t* indptr, const int32_t* indices, const double* val, const double* b,
+                    const double* h0, const double* h1, const double* h2, double omega, double rtol,
+                    int32_t maxiter, int32_t check_every, int32_t rows_per_thread, double* x_out, double* rr_last,
+                    double* atol2, int32_t* ncolours, int nthreads);
 /* Dot product in the device's fixed reduction order. */
 double or_dot_canon(int32_t n, const double* a, const double* b, int32_t rows_per_thread);
 /* ||b - A x|| / ||b|| in the same order as the device check. */

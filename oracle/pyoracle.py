@@ -38,6 +38,9 @@ _SIG = {
     "or_assemble": (None, [_P, C.POINTER(OrParams), _PD, _PD, C.c_int]),
     "or_pcg": (C.c_int32, [C.c_int32, _PI32, _PI32, _PD, _PD, _PD, _PD, _PD, C.c_double, C.c_int32, C.c_int32,
                            C.c_int32, _PD, _PD, _PD, C.c_int]),
+    "or_colour": (C.c_int32, [C.c_int32, _PI32, _PI32, _PI32]),
+    "or_pcg_ssor": (C.c_int32, [C.c_int32, _PI32, _PI32, _PD, _PD, _PD, _PD, _PD, C.c_double, C.c_double,
+                                C.c_int32, C.c_int32, C.c_int32, _PD, _PD, _PD, _PI32, C.c_int]),
     "or_geometry": (C.c_int, [_P, C.c_int, _PD]),
     "or_interp": (None, [_P, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double, _PD]),
     "or_dot_canon": (C.c_double, [C.c_int32, _PD, _PD, C.c_int32]),
@@ -194,6 +197,35 @@ def pcg(indptr, indices, val, b, hist=None, rtol=1e-12, maxiter=1500, check_ever
     it = lib().or_pcg(n, _i(indptr), _i(indices), _d(val), _d(b), _d(h[0]), _d(h[1]), _d(h[2]), rtol, maxiter,
                       check_every, rows_per_thread, _d(x), C.byref(rr), C.byref(at), nthreads)
     return x, int(it), rr.value, at.value
+
+
+def colour(indptr, indices):
+    """Smallest-last greedy colouring of a CSR pattern; returns (colour [n], ncolours)."""
+    indptr = np.ascontiguousarray(indptr, dtype=np.int32)
+    indices = np.ascontiguousarray(indices, dtype=np.int32)
+    n = indptr.size - 1
+    col = np.empty(n, dtype=np.int32)
+    nc = lib().or_colour(n, _i(indptr), _i(indices), _i(col))
+    return col, int(nc)
+
+
+def pcg_ssor(indptr, indices, val, b, hist=None, omega=1.0, rtol=1e-12, maxiter=1500, check_every=10,
+             rows_per_thread=1, nthreads: int = 1):
+    """Multicolour SSOR PCG (Eisenstat form); returns (x, iters, rr_last, atol2, ncolours)."""
+    indptr = np.ascontiguousarray(indptr, dtype=np.int32)
+    indices = np.ascontiguousarray(indices, dtype=np.int32)
+    val = np.ascontiguousarray(val, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    n = b.size
+    x = np.zeros(n, dtype=np.float64)
+    rr, at, nc = C.c_double(), C.c_double(), C.c_int32()
+    h = [None, None, None] if hist is None else [np.ascontiguousarray(v, dtype=np.float64) for v in hist]
+    it = lib().or_pcg_ssor(n, _i(indptr), _i(indices), _d(val), _d(b), _d(h[0]), _d(h[1]), _d(h[2]), omega, rtol,
+                           maxiter, check_every, rows_per_thread, _d(x), C.byref(rr), C.byref(at), C.byref(nc),
+                           nthreads)
+    if it < 0:
+        raise ValueError(lib().or_error().decode())
+    return x, int(it), rr.value, at.value, nc.value
 
 
 def dot_canon(a, b, rows_per_thread=4) -> float:

@@ -32,15 +32,15 @@ def _island(nx, ny, dx):
     cx, cy = xy[:, 0].mean(), xy[:, 1].mean()
     r2 = ((xy[:, 0] - cx) ** 2 + (xy[:, 1] - cy) ** 2) / (4.0 * dx) ** 2
     depth = 12.0 - 16.0 * np.exp(-r2)
-    kb = np.clip((6 - depth).astype(np.int32), 0, 6)
-    return xy, en, ob, depth, kb
+    return xy, en, ob, depth
 
 
 def case_inputs(name):
     """Mesh, initial state and step parameters of one fixture case."""
     if name == "island-20x14":
         nx, ny, dx, nvrt = 20, 14, 400.0, 6
-        xy, en, ob, depth, kb = _island(nx, ny, dx)
+        xy, en, ob, depth = _island(nx, ny, dx)
+        kb = np.clip((6 - depth).astype(np.int32), 0, nvrt - 2)
         p = L.default_step_params(dt=150.0, theta=0.6, dtb_min=30.0, dtb_max=30.0)
     elif name == "lognormal-24x16":
         nx, ny, dx, nvrt = 24, 16, 1000.0, 5

@@ -328,3 +328,152 @@ def test_canonical_dot_is_accurate():
         exact = math.fsum(a * b)
         bound = n * 2.3e-16 * np.abs(a * b).sum()
         assert abs(d - exact) <= bound + 1e-300
+
+
+# ---------------------------------------------------------------- multicolour SSOR
+def _smallest_last_colouring(indptr, indices):
+    """Independent restatement: remove the vertex of least (degree, index) repeatedly, then
+    colour first-fit in reverse removal order."""
+    import heapq
+
+    n = indptr.size - 1
+    nb = [[j for j in indices[indptr[i]:indptr[i + 1]] if j != i] for i in range(n)]
+    deg = [len(x) for x in nb]
+    heap = [(deg[i], i) for i in range(n)]
+    heapq.heapify(heap)
+    gone = [False] * n
+    order = []
+    while heap:
+        d, i = heapq.heappop(heap)
+        if gone[i] or d != deg[i]:
+            continue
+        gone[i] = True
+        order.append(i)
+        for j in nb[i]:
+            if not gone[j]:
+                deg[j] -= 1
+                heapq.heappush(heap, (deg[j], j))
+    col = [-1] * n
+    for i in reversed(order):
+        used = {col[j] for j in nb[i]}
+        c = 0
+        while c in used:
+            c += 1
+        col[i] = c
+    return np.array(col, dtype=np.int32)
+
+
+def _ssor_pcg_reference(A, b, omega, rtol, maxiter, check_every, colour):
+    """Textbook PCG with z = M^-1 r, M = (D/w + L)(D/w)^-1(D/w + U) in colour-major order."""
+    perm = np.lexsort((np.arange(A.shape[0]), colour))
+    Ap = A[perm][:, perm].tocsr()
+    bp = b[perm]
+    d = Ap.diagonal() / omega
+    DL = (sp.diags(d) + sp.tril(Ap, -1)).tocsr()
+    DU = (sp.diags(d) + sp.triu(Ap, 1)).tocsr()
+
+    def minv(r):
+        y = sla.spsolve_triangular(DL, r, lower=True)
+        return sla.spsolve_triangular(DU, d * y, lower=False)
+
+    x = np.zeros_like(bp)
+    r = bp.copy()
+    z = minv(r)
+    p = np.zeros_like(bp)
+    rho, rho_old = r @ z, 0.0
+    atol2 = (rtol * np.linalg.norm(bp)) ** 2
+    for k in range(maxiter):
+        beta = 0.0 if (k == 0 or rho_old == 0.0) else rho / rho_old
+        p = z + beta * p
+        q = Ap @ p
+        alpha = rho / (p @ q)
+        x += alpha * p
+        r -= alpha * q
+        z = minv(r)
+        rho_old, rho = rho, r @ z
+        if (k + 1) % check_every == 0 or k == maxiter - 1:
+            if r @ r <= atol2:
+                break
+    out = np.empty_like(x)
+    out[perm] = x
+    return out, k + 1
+
+
+def _mesh_system(nx=30, ny=20):
+    m, xy, en, ob = _mesh(nx=nx, ny=ny, nvrt=4, depth=20.0)
+    m.set(L.F_ETA2, 0.1 * np.sin(xy[:, 0] / 3000.0))
+    p = _params(dt=300.0, theta=0.6, iettype=2)
+    m.levels(p)
+    m.set(L.F_SIDE_HHAT, np.full(m.ns, 20.0))
+    m.set(L.F_ELBC, np.zeros(m.n))
+    val, b = m.assemble(p)
+    return m.topology(L.T_CSR_INDPTR), m.topology(L.T_CSR_INDICES), val, b
+
+
+def test_colouring_is_valid_and_matches_restatement():
+    graphs = [_spd(400, 7)[0], _spd(1500, 8)[0]]
+    ip, ix, _, _ = _mesh_system()
+    graphs.append(sp.csr_matrix((np.ones(ix.size), ix, ip)))
+    for A in graphs:
+        col, nc = O.colour(A.indptr, A.indices)
+        assert nc == col.max() + 1
+        rows = np.repeat(np.arange(A.shape[0]), np.diff(A.indptr))
+        off = A.indices != rows
+        assert not (col[rows[off]] == col[A.indices[off]]).any()
+        assert np.array_equal(col, _smallest_last_colouring(A.indptr, A.indices))
+    # a planar triangulation is 5-degenerate, so smallest-last needs at most 6 colours
+    col, nc = O.colour(ip, ix)
+    assert nc <= 6
+
+
+@pytest.mark.parametrize("omega", [1.0, 1.4])
+def test_ssor_pcg_matches_textbook_pcg(omega):
+    ip, ix, val, b = _mesh_system()
+    A = sp.csr_matrix((val, ix, ip))
+    x, it, rr, atol2, nc = O.pcg_ssor(ip, ix, val, b, omega=omega, rtol=1e-10, maxiter=1000, check_every=1)
+    col, _ = O.colour(ip, ix)
+    xr, itr = _ssor_pcg_reference(A, b, omega, 1e-10, 1000, 1, col)
+    assert it == itr
+    assert np.linalg.norm(x - xr) / np.linalg.norm(xr) < 1e-9
+    xs = sla.spsolve(A.tocsc(), b)
+    assert np.linalg.norm(x - xs) / np.linalg.norm(xs) < 1e-8
+    # the stopping test is on the true residual b - A x
+    assert rr <= atol2
+    assert np.isclose(np.linalg.norm(b - A @ x) ** 2, rr, rtol=1e-3)
+    # SSOR needs fewer iterations than Jacobi on the wave-equation system
+    _, itj, _, _ = O.pcg(ip, ix, val, b, rtol=1e-10, maxiter=1000, check_every=1)
+    assert it < itj
+
+
+def test_ssor_pcg_special_cases():
+    A, b = _spd(700, 12)
+    x, it, _, _, _ = O.pcg_ssor(A.indptr, A.indices, A.data, np.zeros(700), rtol=1e-12)
+    assert it == 0 and not x.any()
+    xs = sla.spsolve(A.tocsc(), b)
+    for rpt in (1, 4):
+        x, it, _, _, _ = O.pcg_ssor(A.indptr, A.indices, A.data, b, rtol=1e-12, rows_per_thread=rpt)
+        assert it % 10 == 0
+        assert np.linalg.norm(x - xs) / np.linalg.norm(xs) < 1e-10
+    # exact guess from the history: converged at the first check
+    x, it, _, _, _ = O.pcg_ssor(A.indptr, A.indices, A.data, b, hist=(xs, xs, xs), rtol=1e-6, check_every=1)
+    assert it <= 1
+    # a pattern without a diagonal entry is rejected
+    B = sp.csr_matrix(np.array([[0.0, 1.0], [1.0, 2.0]]))
+    B.eliminate_zeros()
+    with pytest.raises(ValueError):
+        O.pcg_ssor(B.indptr, B.indices, B.data, np.ones(2))
+
+
+def test_bottom_level_range_is_checked():
+    """kb in [0, nvrt-2] (at least one layer), rejected otherwise like the library does."""
+    xy, en, ob = cases.channel(6, 5, 100.0)
+    kb = np.zeros(xy.shape[0], np.int32)
+    kb[3] = 4
+    with pytest.raises(Exception):
+        O.OracleMesh(xy, en, np.full(xy.shape[0], 5.0), ob, cases.sigma_levels(5), kb=kb)
+    from gschism_amd import core
+
+    with pytest.raises(L.GshError):
+        core.Mesh(None, xy, en, np.full(xy.shape[0], 5.0), cases.sigma_levels(5), open_bnd=ob, kb=kb)
+    kb[3] = 3
+    O.OracleMesh(xy, en, np.full(xy.shape[0], 5.0), ob, cases.sigma_levels(5), kb=kb)
