@@ -110,9 +110,8 @@ def main() -> int:
 
     # ---- per-phase kernel durations (HIP events on the solver stream) for the roofline
     solver.reset_history()
-    ma, mb, itp = C_double(), C_double(), C_int()
-    L.check(L.lib().gsh_solver_profile(solver.h, rtol, maxiter, check_every, ma, mb, itp))
-    ms_a, ms_b = ma.value, mb.value
+    ms_phase, _ = solver.profile(rtol, maxiter, check_every)
+    ms_a, ms_b = float(ms_phase[0]), float(ms_phase[1])
     n, nnz = mesh.n, mesh.nnz
     ip_h, ix_h = mesh.topology(L.T_CSR_INDPTR), mesh.topology(L.T_CSR_INDICES)
     cdel = ix_h - (np.repeat(np.arange(n, dtype=np.int64), np.diff(ip_h)) & ~63)

@@ -26,6 +26,7 @@ GSH_W_MAXITER = 64
 PREC_FP64 = 0
 PREC_MIXED = 1
 PRECOND_JACOBI = 0
+PRECOND_SSOR = 1
 
 # topology array codes
 T_XY, T_ELNODE, T_ELSIDE, T_ISDEL, T_ISIDENODE, T_BALL_PTR, T_BALL_ELEM, T_NBR_NODE = range(8)
@@ -127,7 +128,10 @@ SIGNATURES = {
     "gsh_server_step": (_I, [_P, _PI32, _PI32]),
     "gsh_server_poll": (_I, [_P, _I64, _PD, _PI32, _PI32, _PD]),
     "gsh_copy_probe": (_I, [_P, _I64, _I, _PD]),
-    "gsh_solver_profile": (_I, [_P, _D, _I, _I, _PD, _PD, C.POINTER(_I)]),
+    "gsh_solver_profile": (_I, [_P, _D, _I, _I, _PD, C.POINTER(_I)]),
+    "gsh_solver_set_omega": (_I, [_P, _D]),
+    "gsh_solver_ssor_info": (_I, [_P, _PI32, _PI32, _PI32]),
+    "gsh_colour_csr": (_I, [_I32, _PI32, _PI32, _PI32, _PI32]),
     "gsh_gen_channel": (_I, [_I32, _I32, _D, _D, C.c_uint64, _PP]),
     "gsh_gen_sizes": (_I, [_P, _PI32, _PI32]),
     "gsh_gen_get": (_I, [_P, _PD, _PI32, C.POINTER(C.c_int8)]),

@@ -186,13 +186,15 @@ class Mesh:
 
 
 class Solver:
-    """Jacobi PCG on the device (gsh_solver), bound to a mesh or to a caller CSR pattern."""
+    """PCG on the device (gsh_solver), bound to a mesh or to a caller CSR pattern; Jacobi
+    (the reference preconditioner) or multicolour SSOR."""
 
     def __init__(self, mesh: Mesh | None = None, precision: int = L.PREC_FP64, *, ctx: Context | None = None,
-                 indptr=None, indices=None):
+                 indptr=None, indices=None, precond: int = L.PRECOND_JACOBI):
         h = C.c_void_p()
+        self.precond = precond
         if mesh is not None:
-            check(L.lib().gsh_solver_create(mesh.h, precision, L.PRECOND_JACOBI, C.byref(h)))
+            check(L.lib().gsh_solver_create(mesh.h, precision, precond, C.byref(h)))
             self.n, self.nnz = mesh.n, mesh.nnz
         else:
             if ctx is None or indptr is None or indices is None:
@@ -201,7 +203,7 @@ class Solver:
             ix = np.ascontiguousarray(indices, dtype=np.int32)
             self.n, self.nnz = ip.size - 1, ix.size
             check(L.lib().gsh_solver_create_csr(ctx.h, self.n, self.nnz, dptr(ip, C.c_int32), dptr(ix, C.c_int32),
-                                                precision, L.PRECOND_JACOBI, C.byref(h)))
+                                                precision, precond, C.byref(h)))
         self.h = h
         self.mesh = mesh
         self.ctx = ctx if ctx is not None else (mesh.ctx if mesh is not None else None)
@@ -274,8 +276,36 @@ class Solver:
     def set_tuning(self, rows_per_thread: int = 4, inner_rtol: float = 1e-5) -> None:
         check(L.lib().gsh_solver_set_tuning(self.h, rows_per_thread, inner_rtol))
 
+    def set_omega(self, omega: float) -> None:
+        check(L.lib().gsh_solver_set_omega(self.h, omega))
+
+    def ssor_info(self):
+        """(ncolours, padded rows, colour of every row) of an SSOR solver."""
+        nc, npad = C.c_int32(), C.c_int32()
+        col = np.empty(self.n, dtype=np.int32)
+        check(L.lib().gsh_solver_ssor_info(self.h, C.byref(nc), C.byref(npad), dptr(col, C.c_int32)))
+        return nc.value, npad.value, col
+
+    def profile(self, rtol: float = 1e-12, maxiter: int = 0, check_every: int = 10):
+        """One FP64 solve with per-launch timing: (ms per iteration [4], iterations)."""
+        ms = np.zeros(4, dtype=np.float64)
+        it = C.c_int()
+        check(L.lib().gsh_solver_profile(self.h, rtol, maxiter, check_every, dptr(ms), C.byref(it)))
+        return ms, it.value
+
     def update_elevation(self) -> None:
         check(L.lib().gsh_update_elevation(self.mesh.h, self.h))
+
+
+def colour_csr(indptr, indices):
+    """The SSOR solver's colouring of a CSR pattern (host only): (colour [n], ncolours)."""
+    ip = np.ascontiguousarray(indptr, dtype=np.int32)
+    ix = np.ascontiguousarray(indices, dtype=np.int32)
+    col = np.empty(ip.size - 1, dtype=np.int32)
+    nc = C.c_int32()
+    check(L.lib().gsh_colour_csr(ip.size - 1, dptr(ip, C.c_int32), dptr(ix, C.c_int32), dptr(col, C.c_int32),
+                                 C.byref(nc)))
+    return col, nc.value
 
 
 def batch_solve(solvers, rtol=1e-12, maxiter=0, check_every=10, timeout_s=60.0):

@@ -19,6 +19,7 @@
 
 #include "abi_util.h"
 #include "device.h"
+#include "ssor_layout.h"
 
 namespace gsh {
 
@@ -143,6 +144,16 @@ struct gsh_mesh {
 };
 
 // ====================================================================== solver
+// Device side of the multicolour SSOR solver (built from the solver's CSR pattern).
+struct SsorState {
+  DevPool pool;
+  SsorLayout L;
+  SsorSys hs{};
+  SsorSys* ds = nullptr;
+  int rpt = 1, ntiles = 0;
+  std::vector<int> tile0, ntc;  // tiles of each colour
+};
+
 struct gsh_solver {
   gsh_ctx* ctx = nullptr;
   gsh_mesh* mesh = nullptr;
@@ -165,6 +176,11 @@ struct gsh_solver {
   PcgSys<float> hs32{};
   PcgSys<float>* ds32 = nullptr;
   gsh_solve_stats last{};
+  // preconditioner
+  int precond = GSH_PRECOND_JACOBI;
+  double omega = 1.0;
+  std::vector<int32_t> h_indptr, h_indices;  // host pattern kept for the SSOR layout
+  std::unique_ptr<SsorState> ssor;
 
   int tiles_for(int rows) const { return (rows + kTpb * rpt - 1) / (kTpb * rpt); }
 };
@@ -275,6 +291,56 @@ static void solver_sync_desc(gsh_solver* s) {
   HIPCHK(hipStreamSynchronize(st));  // host staging buffers above are stack/heap temporaries
 }
 
+// Colour-major layout, split matrix and work vectors of the SSOR solver for the current
+// rows-per-thread (the colour blocks are padded to whole tiles of 256 * rpt rows).
+static void ssor_build(gsh_solver* s) {
+  auto S = std::make_unique<SsorState>();
+  S->rpt = s->rpt;
+  const int T = kTpb * s->rpt;
+  build_ssor_layout(s->n, s->h_indptr.data(), s->h_indices.data(), T, S->L);
+  const SsorLayout& L = S->L;
+  auto& P = S->pool;
+  SsorSys& h = S->hs;
+  h.n = s->n;
+  h.np = L.np;
+  h.ntiles = L.np / T;
+  h.ncol = L.ncol;
+  h.omega = s->omega;
+  h.lptr = P.upload(L.lptr);
+  h.lcol = P.upload(L.lcol);
+  h.lsrc = P.upload(L.lsrc);
+  h.uptr = P.upload(L.uptr);
+  h.ucol = P.upload(L.ucol);
+  h.usrc = P.upload(L.usrc);
+  h.dsrc = P.upload(L.dsrc);
+  h.perm = P.upload(L.perm);
+  h.lval = P.alloc<double>((int64_t)L.lcol.size());
+  h.uval = P.alloc<double>((int64_t)L.ucol.size());
+  for (double** v : {&h.d, &h.dt, &h.kd, &h.bp, &h.y, &h.r, &h.p, &h.t, &h.s, &h.q, &h.w}) *v = P.alloc<double>(L.np);
+  for (double** v : {&h.part_rr, &h.part_rz, &h.part_pq, &h.part_bb}) *v = P.alloc<double>(h.ntiles);
+  h.st = P.alloc<PcgState>(1);
+  S->ds = P.alloc<SsorSys>(1);
+  S->ntiles = h.ntiles;
+  for (int c = 0; c < L.ncol; ++c) {
+    S->tile0.push_back(L.cstart[c] / T);
+    S->ntc.push_back((L.cstart[c + 1] - L.cstart[c]) / T);
+  }
+  s->ssor = std::move(S);
+}
+
+static void ssor_sync_desc(gsh_solver* s) {
+  SsorSys& h = s->ssor->hs;
+  h.aval = s->val;
+  h.b = s->b;
+  h.xout = s->x;
+  h.h0 = s->h[0];
+  h.h1 = s->h[1];
+  h.h2 = s->h[2];
+  h.omega = s->omega;
+  HIPCHK(hipMemcpyAsync(s->ssor->ds, &h, sizeof(h), hipMemcpyHostToDevice, s->ctx->stream));
+  HIPCHK(hipStreamSynchronize(s->ctx->stream));
+}
+
 // rollHistory (pcg_solver.py:941-947) by pointer rotation: x2 <- x1 <- x0 <- x.
 static void solver_roll(gsh_solver* s) {
   double* old2 = s->h[2];
@@ -378,6 +444,101 @@ static void run_pcg(gsh_ctx* ctx, const PcgSys<VT>* dsys, int nsys, const TileRe
 }
 
 static int iters_of(const PcgState& s, int maxiter) { return s.done >= 0 ? s.done : maxiter; }
+
+// Per-iteration kernel time of the SSOR solver by launch kind (HIP events, profiling only).
+struct SsorTimes {
+  double ms_bwd = 0.0, ms_fwd = 0.0, ms_upd = 0.0;
+  int64_t iters = 0;
+};
+
+// Multicolour SSOR PCG of one system: set-up, windows of check_every iterations (2 * ncolours
+// launches each) closed by the true-residual test, then x = (D~ + U)^-1 y.
+static PcgState run_ssor(gsh_solver* s, double rtol, int maxiter, int check_every, double timeout_s,
+                         SsorTimes* prof = nullptr) {
+  gsh_ctx* ctx = s->ctx;
+  hipStream_t st = ctx->stream;
+  if (check_every < 1) check_every = 1;
+  if (timeout_s <= 0.0) timeout_s = 60.0;
+  ssor_sync_desc(s);
+  SsorState& S = *s->ssor;
+  const SsorSys* ds = S.ds;
+  const int rpt = S.rpt, C = S.L.ncol, NT = S.ntiles;
+  PcgState* slots = ctx->pinned_slots(1);
+  launch_ssor_reset(ds, rtol, maxiter, check_every, st);
+  launch_ssor_setup(ds, NT, rpt, st);
+  launch_ssor_residual(ds, NT, rpt, st);
+  for (int c = 0; c < C; ++c) launch_ssor_r0(ds, S.tile0[c], S.ntc[c], rpt, st);
+  auto bwd = [&]() {
+    for (int c = C - 1; c >= 0; --c) launch_ssor_bwd(ds, S.tile0[c], S.ntc[c], c == C - 1, c == 0, rpt, st);
+  };
+  auto fwd = [&]() {
+    for (int c = 1; c < C; ++c) launch_ssor_fwd(ds, S.tile0[c], S.ntc[c], rpt, st);
+  };
+  HIPCHK(hipGetLastError());
+  int issued = 0, window = 0;
+  std::deque<int> inflight;
+  bool done = false;
+  const auto t0 = Clock::now();
+  auto check_slot = [&](int slot) {
+    wait_event(ctx->ev[slot], timeout_s, "SSOR PCG window");
+    return slots[slot].done >= 0;
+  };
+  while (issued < maxiter && !done) {
+    const int w = std::min(check_every, maxiter - issued);
+    if (prof) {
+      hipEvent_t e[4];
+      for (auto& x : e) HIPCHK(hipEventCreate(&x));
+      for (int i = 0; i < w; ++i) {
+        HIPCHK(hipEventRecord(e[0], st));
+        bwd();
+        HIPCHK(hipEventRecord(e[1], st));
+        fwd();
+        HIPCHK(hipEventRecord(e[2], st));
+        launch_ssor_update(ds, NT, rpt, st);
+        HIPCHK(hipEventRecord(e[3], st));
+        HIPCHK(hipEventSynchronize(e[3]));
+        float a = 0.f, b = 0.f, c = 0.f;
+        HIPCHK(hipEventElapsedTime(&a, e[0], e[1]));
+        HIPCHK(hipEventElapsedTime(&b, e[1], e[2]));
+        HIPCHK(hipEventElapsedTime(&c, e[2], e[3]));
+        prof->ms_bwd += a;
+        prof->ms_fwd += b;
+        prof->ms_upd += c;
+        prof->iters += 1;
+      }
+      for (auto& x : e) (void)hipEventDestroy(x);
+    } else {
+      for (int i = 0; i < w; ++i) {
+        bwd();
+        fwd();
+        launch_ssor_update(ds, NT, rpt, st);
+      }
+    }
+    launch_ssor_true_rr(ds, NT, rpt, st);
+    launch_ssor_check(ds, st);
+    issued += w;
+    const int slot = window & 1;
+    if (inflight.size() == 2) {
+      const int old = inflight.front();
+      inflight.pop_front();
+      if (check_slot(old)) done = true;
+    }
+    HIPCHK(hipMemcpyAsync(slots + slot, S.hs.st, sizeof(PcgState), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipEventRecord(ctx->ev[slot], st));
+    inflight.push_back(slot);
+    ++window;
+    if (std::chrono::duration<double>(Clock::now() - t0).count() > timeout_s)
+      throw Error(GSH_E_TIMEOUT, "SSOR PCG: time limit exceeded while issuing iterations");
+    HIPCHK(hipGetLastError());
+  }
+  for (int c = C - 1; c >= 0; --c) launch_ssor_xsolve(ds, S.tile0[c], S.ntc[c], rpt, st);
+  launch_ssor_scatter(ds, NT, rpt, st);
+  HIPCHK(hipMemcpyAsync(slots, S.hs.st, sizeof(PcgState), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipEventRecord(ctx->ev[0], st));
+  wait_event(ctx->ev[0], timeout_s, "SSOR PCG finish");
+  HIPCHK(hipGetLastError());
+  return slots[0];
+}
 
 // Mixed precision: FP64 true residual + refinement around FP32 PCG solves (FP64 dots).
 static void solve_mixed(gsh_solver* s, double rtol, int maxiter, int check_every, double timeout_s, int* iters_out,
@@ -871,17 +1032,28 @@ int gsh_solver_create(gsh_mesh* mesh, int precision, int precond, gsh_solver** o
   GSH_API_BODY({
     GSH_REQUIRE(mesh && out, GSH_E_ARG, "NULL argument");
     GSH_REQUIRE(precision == GSH_PREC_FP64 || precision == GSH_PREC_MIXED, GSH_E_ARG, "unknown precision");
-    GSH_REQUIRE(precond == GSH_PRECOND_JACOBI, GSH_E_ARG, "unknown preconditioner");
+    GSH_REQUIRE(precond == GSH_PRECOND_JACOBI || precond == GSH_PRECOND_SSOR, GSH_E_ARG, "unknown preconditioner");
+    GSH_REQUIRE(precond == GSH_PRECOND_JACOBI || precision == GSH_PREC_FP64, GSH_E_ARG,
+                "the SSOR preconditioner runs in FP64");
     GSH_REQUIRE(mesh->ctx, GSH_E_STATE, "host-only mesh (created without a context)");
     mesh->ctx->set();
     auto s = std::make_unique<gsh_solver>();
     s->ctx = mesh->ctx;
     s->mesh = mesh;
     s->precision = precision;
+    s->precond = precond;
     const Topology& t = mesh->topo;
     std::vector<int32_t> dabs(mesh->hm.n);
     for (int32_t i = 0; i < mesh->hm.n; ++i) dabs[i] = t.indptr[i] + t.diag_off[i];
     solver_alloc(s.get(), mesh->hm.n, t.indptr.back(), t.indptr, t.indices, dabs);
+    if (precond == GSH_PRECOND_SSOR) {
+      s->rpt = 1;
+      s->ntiles = s->tiles_for(s->n);
+      s->hs.ntiles = s->ntiles;
+      s->h_indptr = t.indptr;
+      s->h_indices = t.indices;
+      ssor_build(s.get());
+    }
     *out = s.release();
   })
 }
@@ -891,7 +1063,9 @@ int gsh_solver_create_csr(gsh_ctx* ctx, int32_t n, int64_t nnz, const int32_t* i
   GSH_API_BODY({
     GSH_REQUIRE(ctx && indptr && indices && out && n > 0 && nnz > 0, GSH_E_ARG, "bad arguments");
     GSH_REQUIRE(precision == GSH_PREC_FP64 || precision == GSH_PREC_MIXED, GSH_E_ARG, "unknown precision");
-    GSH_REQUIRE(precond == GSH_PRECOND_JACOBI, GSH_E_ARG, "unknown preconditioner");
+    GSH_REQUIRE(precond == GSH_PRECOND_JACOBI || precond == GSH_PRECOND_SSOR, GSH_E_ARG, "unknown preconditioner");
+    GSH_REQUIRE(precond == GSH_PRECOND_JACOBI || precision == GSH_PREC_FP64, GSH_E_ARG,
+                "the SSOR preconditioner runs in FP64");
     GSH_REQUIRE(nnz < (int64_t)1 << 31, GSH_E_ARG, "nnz exceeds int32 indexing");
     GSH_REQUIRE(indptr[0] == 0 && indptr[n] == nnz, GSH_E_ARG, "indptr must start at 0 and end at nnz");
     for (int32_t i = 0; i < n; ++i) GSH_REQUIRE(indptr[i + 1] >= indptr[i], GSH_E_ARG, "indptr must be non-decreasing");
@@ -900,8 +1074,17 @@ int gsh_solver_create_csr(gsh_ctx* ctx, int32_t n, int64_t nnz, const int32_t* i
     auto s = std::make_unique<gsh_solver>();
     s->ctx = ctx;
     s->precision = precision;
+    s->precond = precond;
     std::vector<int32_t> ip(indptr, indptr + n + 1), ix(indices, indices + nnz);
     solver_alloc(s.get(), n, nnz, ip, ix, diag_abs_of(ip, ix, n));
+    if (precond == GSH_PRECOND_SSOR) {
+      s->rpt = 1;
+      s->ntiles = s->tiles_for(s->n);
+      s->hs.ntiles = s->ntiles;
+      s->h_indptr = std::move(ip);
+      s->h_indices = std::move(ix);
+      ssor_build(s.get());
+    }
     *out = s.release();
   })
 }
@@ -964,11 +1147,49 @@ int gsh_solver_set_tuning(gsh_solver* s, int rows_per_thread, double inner_rtol)
     GSH_REQUIRE(rows_per_thread == 1 || rows_per_thread == 2 || rows_per_thread == 4 || rows_per_thread == 8,
                 GSH_E_ARG, "rows_per_thread must be 1, 2, 4 or 8");
     GSH_REQUIRE(inner_rtol > 0.0 && inner_rtol < 1.0, GSH_E_ARG, "inner_rtol must be in (0, 1)");
+    s->ctx->set();
+    const bool relayout = s->ssor && s->ssor->rpt != rows_per_thread;
     s->rpt = rows_per_thread;
     s->ntiles = s->tiles_for(s->n);
     s->hs.ntiles = s->ntiles;
     s->hs32.ntiles = s->ntiles;
     s->inner_rtol = inner_rtol;
+    if (relayout) {
+      HIPCHK(hipStreamSynchronize(s->ctx->stream));
+      ssor_build(s);
+    }
+  })
+}
+
+int gsh_solver_set_omega(gsh_solver* s, double omega) {
+  GSH_API_BODY({
+    GSH_REQUIRE(s, GSH_E_ARG, "solver is NULL");
+    GSH_REQUIRE(s->precond == GSH_PRECOND_SSOR, GSH_E_STATE, "omega applies to the SSOR preconditioner");
+    GSH_REQUIRE(omega > 0.0 && omega < 2.0, GSH_E_ARG, "omega must be in (0, 2)");
+    s->omega = omega;
+  })
+}
+
+int gsh_solver_ssor_info(gsh_solver* s, int32_t* ncolours, int32_t* padded_rows, int32_t* colour_out) {
+  GSH_API_BODY({
+    GSH_REQUIRE(s, GSH_E_ARG, "solver is NULL");
+    GSH_REQUIRE(s->ssor, GSH_E_STATE, "not an SSOR solver");
+    const SsorLayout& L = s->ssor->L;
+    if (ncolours) *ncolours = L.ncol;
+    if (padded_rows) *padded_rows = L.np;
+    if (colour_out) std::copy(L.colour.begin(), L.colour.end(), colour_out);
+  })
+}
+
+int gsh_colour_csr(int32_t n, const int32_t* indptr, const int32_t* indices, int32_t* colour_out,
+                   int32_t* ncolours) {
+  GSH_API_BODY({
+    GSH_REQUIRE(n >= 0 && indptr && indices && colour_out && ncolours, GSH_E_ARG, "bad arguments");
+    for (int32_t i = 0; i < n; ++i)
+      GSH_REQUIRE(indptr[i + 1] >= indptr[i], GSH_E_ARG, "indptr must be non-decreasing");
+    for (int64_t k = 0; k < indptr[n]; ++k)
+      GSH_REQUIRE(indices[k] >= 0 && indices[k] < n, GSH_E_ARG, "column out of range");
+    *ncolours = colour_smallest_last(n, indptr, indices, colour_out);
   })
 }
 
@@ -998,6 +1219,14 @@ int gsh_solver_solve(gsh_solver* s, const double* b_host, double* x_host, double
     int iters = 0, status = 0;
     if (s->precision == GSH_PREC_MIXED) {
       solve_mixed(s, rtol, maxiter, check_every, timeout_s, &iters, &status);
+    } else if (s->precond == GSH_PRECOND_SSOR) {
+      const PcgState r = run_ssor(s, rtol, maxiter, check_every, timeout_s);
+      iters = iters_of(r, maxiter);
+      status = (r.done >= 0) ? 0 : GSH_W_MAXITER;
+      s->last.iters = iters;
+      s->last.outer = 1;
+      s->last.rr = r.rr_last;
+      s->last.atol2 = r.atol2;
     } else {
       PcgOutcome o;
       run_pcg<double>(s->ctx, s->ds, 1, s->dtiles, s->ntiles, s->rpt, rtol, maxiter, check_every, timeout_s, o);
@@ -1064,6 +1293,7 @@ int gsh_batch_solve(gsh_solver* const* solvers, int count, double rtol, int maxi
     for (int i = 0; i < count; ++i) {
       GSH_REQUIRE(solvers[i] && solvers[i]->ctx == ctx, GSH_E_ARG, "all solvers must share one context");
       GSH_REQUIRE(solvers[i]->precision == GSH_PREC_FP64, GSH_E_ARG, "batched solve supports FP64 solvers");
+      GSH_REQUIRE(solvers[i]->precond == GSH_PRECOND_JACOBI, GSH_E_ARG, "batched solve supports Jacobi solvers");
       for (int j = 0; j < i; ++j) GSH_REQUIRE(solvers[j] != solvers[i], GSH_E_ARG, "a solver appears twice");
     }
     ctx->set();
@@ -1428,13 +1658,26 @@ int gsh_copy_probe(gsh_ctx* ctx, int64_t bytes, int reps, double* ms_avg) {
   })
 }
 
-int gsh_solver_profile(gsh_solver* s, double rtol, int maxiter, int check_every, double* ms_a, double* ms_b,
-                       int* iters_out) {
+int gsh_solver_profile(gsh_solver* s, double rtol, int maxiter, int check_every, double* ms, int* iters_out) {
   GSH_API_BODY({
-    GSH_REQUIRE(s && ms_a && ms_b, GSH_E_ARG, "NULL argument");
+    GSH_REQUIRE(s && ms, GSH_E_ARG, "NULL argument");
     GSH_REQUIRE(s->precision == GSH_PREC_FP64, GSH_E_ARG, "phase profiling runs the FP64 solver");
     if (maxiter <= 0) maxiter = 10 * s->n;
     s->ctx->set();
+    for (int i = 0; i < 4; ++i) ms[i] = 0.0;
+    if (s->precond == GSH_PRECOND_SSOR) {
+      SsorTimes tt;
+      const PcgState r = run_ssor(s, rtol, maxiter, check_every, 60.0, &tt);
+      HIPCHK(hipStreamSynchronize(s->ctx->stream));
+      solver_roll(s);
+      if (tt.iters) {
+        ms[0] = tt.ms_bwd / tt.iters;
+        ms[1] = tt.ms_fwd / tt.iters;
+        ms[2] = tt.ms_upd / tt.iters;
+      }
+      if (iters_out) *iters_out = iters_of(r, maxiter);
+      return GSH_OK;
+    }
     solver_sync_desc(s);
     PcgOutcome o;
     PhaseTimes pt;
@@ -1443,8 +1686,8 @@ int gsh_solver_profile(gsh_solver* s, double rtol, int maxiter, int check_every,
     const int it = iters_of(o.st[0], maxiter);
     HIPCHK(hipStreamSynchronize(s->ctx->stream));
     solver_roll(s);
-    *ms_a = pt.launches ? pt.ms_a / pt.launches : 0.0;
-    *ms_b = pt.launches ? pt.ms_b / pt.launches : 0.0;
+    ms[0] = pt.launches ? pt.ms_a / pt.launches : 0.0;
+    ms[1] = pt.launches ? pt.ms_b / pt.launches : 0.0;
     if (iters_out) *iters_out = it;
   })
 }

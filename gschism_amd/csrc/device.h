@@ -79,6 +79,8 @@ struct PcgState {
   int32_t maxiter, check_every, pad0, pad1;
   double rtol, atol2, rr_last, bb;
   double rho[2];    // r.z used by iteration k (slot k & 1)
+  double beta;      // SSOR: beta of iteration kk, for the sweeps after the first
+  double pad2;
 };
 
 // One linear system of a (possibly batched) PCG launch. VT is the arithmetic type of the
@@ -106,6 +108,26 @@ struct PcgSys {
   double* part_rz;
   double* part_pq;
   double* part_bb;
+  PcgState* st;
+};
+
+// Multicolour SSOR system: rows in colour-major order, each colour block padded to whole
+// tiles of 256*R rows with identity rows; the matrix split into the strictly lower part
+// (columns of lower colours) and the strictly upper part, both with ascending columns.
+struct SsorSys {
+  int32_t n, np, ntiles, ncol;
+  double omega;
+  const int32_t *lptr, *lcol, *uptr, *ucol;  // [np + 1], [nl] / [nu]
+  const int32_t *lsrc, *usrc, *dsrc;         // slots of each entry in the solver's CSR values
+  const int32_t* perm;                       // [np] new row -> original row, -1 for padding
+  const double* aval;                        // the solver's CSR values (original order)
+  const double* b;                           // right-hand side (original order)
+  const double *h0, *h1, *h2;                // extrapolation history (original order), NULL = zero guess
+  double* xout;                              // solution (original order)
+  double *lval, *uval;                       // gathered off-diagonal values
+  double *d, *dt, *kd, *bp;                  // diagonal, D/omega, 2 D/omega - D, permuted b
+  double *y, *r, *p, *t, *s, *q, *w;         // iteration vectors [np]
+  double *part_rr, *part_rz, *part_pq, *part_bb;
   PcgState* st;
 };
 
@@ -138,6 +160,19 @@ void launch_pcg_finish(const PcgSys<VT>* sys, const TileRef* tiles, int ntiles, 
 void launch_true_residual(const PcgSys<double>* sys, const TileRef* tiles, int ntiles, int rpt, const double* xv,
                           double* rvec, hipStream_t s);
 void launch_reduce_pair(const double* pa, const double* pb, int ntiles, double* out2, hipStream_t s);
+
+// Multicolour SSOR PCG (one system). Tiles of colour c are [tile0, tile0 + ntc).
+void launch_ssor_reset(const SsorSys* sys, double rtol, int maxiter, int check_every, hipStream_t s);
+void launch_ssor_setup(const SsorSys* sys, int ntiles, int rpt, hipStream_t s);
+void launch_ssor_residual(const SsorSys* sys, int ntiles, int rpt, hipStream_t s);
+void launch_ssor_r0(const SsorSys* sys, int tile0, int ntc, int rpt, hipStream_t s);
+void launch_ssor_bwd(const SsorSys* sys, int tile0, int ntc, int first, int mid, int rpt, hipStream_t s);
+void launch_ssor_fwd(const SsorSys* sys, int tile0, int ntc, int rpt, hipStream_t s);
+void launch_ssor_update(const SsorSys* sys, int ntiles, int rpt, hipStream_t s);
+void launch_ssor_true_rr(const SsorSys* sys, int ntiles, int rpt, hipStream_t s);
+void launch_ssor_check(const SsorSys* sys, hipStream_t s);
+void launch_ssor_xsolve(const SsorSys* sys, int tile0, int ntc, int rpt, hipStream_t s);
+void launch_ssor_scatter(const SsorSys* sys, int ntiles, int rpt, hipStream_t s);
 
 // Mixed precision helpers.
 void launch_cast_values(const double* in, float* out, int64_t count, hipStream_t s);

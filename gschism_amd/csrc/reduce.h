@@ -1,0 +1,54 @@
+// reduce.h -- device helpers shared by the solver kernels: global-address-space views and the
+// fixed reduction tree of every dot product (wave64 shuffle tree, then (w0 + w1) + (w2 + w3),
+// tile partials summed by thread t over t, t + 256, ...).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "device.h"
+
+namespace gsh {
+
+// Pointers read from a descriptor in device memory are generic to the compiler; casting them to
+// the global address space makes every access a global_load/global_store.
+template <class T>
+using gptr = __attribute__((address_space(1))) T*;
+template <class T>
+__device__ __forceinline__ gptr<T> G(T* p) {
+  return (gptr<T>)p;
+}
+
+__device__ __forceinline__ int32_t load_done(gptr<PcgState> st) {
+  return *(volatile __attribute__((address_space(1))) int32_t*)&st->done;
+}
+
+__device__ __forceinline__ double wave_tree(double v) {
+  v += __shfl_down(v, 32, 64);
+  v += __shfl_down(v, 16, 64);
+  v += __shfl_down(v, 8, 64);
+  v += __shfl_down(v, 4, 64);
+  v += __shfl_down(v, 2, 64);
+  v += __shfl_down(v, 1, 64);
+  return v;
+}
+
+// Sum over the 256 threads of a workgroup: wave trees, then (w0 + w1) + (w2 + w3).
+// The total is returned to every thread.
+__device__ __forceinline__ double block_tree(double v, double* red) {
+  v = wave_tree(v);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  const double t = (red[0] + red[1]) + (red[2] + red[3]);
+  __syncthreads();
+  return t;
+}
+
+// Sum of the tile partials P[0..T): thread t sums P[t], P[t+256], ... then block_tree.
+__device__ __forceinline__ double final_reduce(gptr<const double> P, int T, double* red) {
+  double u = 0.0;
+  for (int m = threadIdx.x; m < T; m += kTpb) u += P[m];
+  return block_tree(u, red);
+}
+
+}  // namespace gsh

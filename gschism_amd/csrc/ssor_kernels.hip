@@ -1,0 +1,423 @@
+// ssor_kernels.hip -- PCG with the multicolour SSOR(omega) preconditioner on gfx950.
+//
+// The reference solves with Jacobi PCG only (pcg_solver.py:933-938). This solver keeps the
+// reference's driver (extrapolated guess, atol = rtol ||b||, the stopping rule on ||b - A x||^2
+// every check_every iterations, the beta / alpha recurrences; pcg_solver.py:881-947,
+// 1075-1124, pcg_loop.cpp:361-413) and preconditions with
+//   M = (D~ + L) D~^-1 (D~ + U),   D~ = D / omega,
+// in a multicolour order, applied by Eisenstat's trick (SIAM J. Sci. Stat. Comput. 2 (1981)
+// 1-4): CG runs on A^ = (D~ + L)^-1 A (D~ + U)^-1 with the diagonal preconditioner D~ and
+//   A^ v = t + (D~ + L)^-1 (v - (2 D~ - D) t),   t = (D~ + U)^-1 v,
+// so one iteration reads every off-diagonal entry once. x = (D~ + U)^-1 y maps the iterate
+// back; the true residual b - A x is (D~ + L) r.
+//
+// Rows are in colour-major order; every colour is a set of independent rows, so each
+// triangular sweep is one launch per colour:
+//   bwd(C-1) ... bwd(1)   p = D~ r + beta p; t = (p - U t) / D~           (first also: beta)
+//   bwd(0)                 the same, then s = (p - K t) / D~, q = t + s     (K = 2 D~ - D)
+//   fwd(1) ... fwd(C-1)    s = (p - K t - L s) / D~, q = t + s, p.q partials
+//   update                 alpha; y += alpha p; r -= alpha q; r.(D~ r) partials
+// Colour blocks are padded to whole reduction tiles, so each tile's partial is written by
+// exactly one launch and every dot product uses the fixed tile tree of the Jacobi solver.
+// Row sums run over ascending columns from 0.0, in the oracle's order.
+#include <hip/hip_runtime.h>
+
+#include "device.h"
+#include "reduce.h"
+
+namespace gsh {
+
+struct SsorView {
+  int np, ntiles;
+  double omega;
+  gptr<const int32_t> lptr, lcol, uptr, ucol, lsrc, usrc, dsrc, perm;
+  gptr<const double> aval, b, h0, h1, h2;
+  gptr<double> xout, lval, uval, d, dt, kd, bp, y, r, p, t, s, q, w;
+  gptr<double> part_rr, part_rz, part_pq, part_bb;
+  gptr<PcgState> st;
+  bool has_hist;
+};
+
+__device__ __forceinline__ SsorView sview(const SsorSys* __restrict__ sys) {
+  const SsorSys& S = *sys;
+  SsorView v;
+  v.np = S.np;
+  v.ntiles = S.ntiles;
+  v.omega = S.omega;
+  v.lptr = G(S.lptr);
+  v.lcol = G(S.lcol);
+  v.uptr = G(S.uptr);
+  v.ucol = G(S.ucol);
+  v.lsrc = G(S.lsrc);
+  v.usrc = G(S.usrc);
+  v.dsrc = G(S.dsrc);
+  v.perm = G(S.perm);
+  v.aval = G(S.aval);
+  v.b = G(S.b);
+  v.has_hist = S.h0 != nullptr;
+  v.h0 = G(S.h0);
+  v.h1 = G(S.h1);
+  v.h2 = G(S.h2);
+  v.xout = G(S.xout);
+  v.lval = G(S.lval);
+  v.uval = G(S.uval);
+  v.d = G(S.d);
+  v.dt = G(S.dt);
+  v.kd = G(S.kd);
+  v.bp = G(S.bp);
+  v.y = G(S.y);
+  v.r = G(S.r);
+  v.p = G(S.p);
+  v.t = G(S.t);
+  v.s = G(S.s);
+  v.q = G(S.q);
+  v.w = G(S.w);
+  v.part_rr = G(S.part_rr);
+  v.part_rz = G(S.part_rz);
+  v.part_pq = G(S.part_pq);
+  v.part_bb = G(S.part_bb);
+  v.st = G(S.st);
+  return v;
+}
+
+// sum over the strictly lower / upper part of row v of a_vj x_j, ascending j, from 0.0
+__device__ __forceinline__ double lower_sum(const SsorView& S, int v, gptr<const double> x) {
+  double acc = 0.0;
+  const int a = S.lptr[v], e = S.lptr[v + 1];
+  for (int k = a; k < e; ++k) acc += S.lval[k] * x[S.lcol[k]];
+  return acc;
+}
+
+__device__ __forceinline__ double upper_sum(const SsorView& S, int v, gptr<const double> x) {
+  double acc = 0.0;
+  const int a = S.uptr[v], e = S.uptr[v + 1];
+  for (int k = a; k < e; ++k) acc += S.uval[k] * x[S.ucol[k]];
+  return acc;
+}
+
+template <class T>
+__device__ __forceinline__ gptr<const T> C(gptr<T> p) {
+  return (gptr<const T>)p;
+}
+
+// row of thread threadIdx.x in sub-round j of a tile (the reduction order of every partial)
+template <int R>
+__device__ __forceinline__ int tile_row(int tile, int j) {
+  return tile * kTpb * R + j * kTpb + threadIdx.x;
+}
+
+// ------------------------------------------------------------------ setup
+// Gathers the split matrix and the diagonal from the CSR values, b and the extrapolated guess
+// x0 = 3 x0 - 3 x1 + x2 (pcg_solver.py:881-886) into the colour-major order; b.b partials.
+template <int R>
+__global__ void __launch_bounds__(256) k_ssor_setup(const SsorSys* __restrict__ sys) {
+  __shared__ double red[4];
+  const SsorView S = sview(sys);
+  const int tile = blockIdx.x;
+  double bb = 0.0;
+  for (int j = 0; j < R; ++j) {
+    const int v = tile_row<R>(tile, j);
+    const int ds = S.dsrc[v];
+    const double dv = ds >= 0 ? S.aval[ds] : 1.0;
+    const double dtv = dv / S.omega;
+    S.d[v] = dv;
+    S.dt[v] = dtv;
+    S.kd[v] = 2.0 * dtv - dv;
+    for (int k = S.lptr[v]; k < S.lptr[v + 1]; ++k) S.lval[k] = S.aval[S.lsrc[k]];
+    for (int k = S.uptr[v]; k < S.uptr[v + 1]; ++k) S.uval[k] = S.aval[S.usrc[k]];
+    const int i = S.perm[v];
+    const double bv = i >= 0 ? S.b[i] : 0.0;
+    S.bp[v] = bv;
+    S.w[v] = (i >= 0 && S.has_hist) ? (3.0 * S.h0[i] - 3.0 * S.h1[i]) + S.h2[i] : 0.0;
+    bb += bv * bv;
+  }
+  bb = block_tree(bb, red);
+  if (threadIdx.x == 0) S.part_bb[tile] = bb;
+}
+
+// r_true = b - A x0 (computeResiduals, pcg_solver.py:889-895) into t; y0 = (D~ + U) x0; p = 0.
+template <int R>
+__global__ void __launch_bounds__(256) k_ssor_residual(const SsorSys* __restrict__ sys) {
+  const SsorView S = sview(sys);
+  const gptr<const double> x0 = C(S.w);
+  for (int j = 0; j < R; ++j) {
+    const int v = tile_row<R>(blockIdx.x, j);
+    const double xv = x0[v];
+    const double au = upper_sum(S, v, x0);
+    const double ax = (lower_sum(S, v, x0) + S.d[v] * xv) + au;
+    S.t[v] = S.bp[v] - ax;
+    S.y[v] = S.dt[v] * xv + au;
+    S.p[v] = 0.0;
+  }
+}
+
+// r0 = (D~ + L)^-1 r_true for the rows of one colour; partials of r.(D~ r).
+template <int R>
+__global__ void __launch_bounds__(256) k_ssor_r0(const SsorSys* __restrict__ sys, int tile0) {
+  __shared__ double red[4];
+  const SsorView S = sview(sys);
+  const int tile = tile0 + blockIdx.x;
+  const gptr<const double> r = C(S.r);
+  double rz = 0.0;
+  for (int j = 0; j < R; ++j) {
+    const int v = tile_row<R>(tile, j);
+    const double dtv = S.dt[v];
+    const double rv = (S.t[v] - lower_sum(S, v, r)) / dtv;
+    S.r[v] = rv;
+    const double zv = dtv * rv;
+    rz += rv * zv;
+  }
+  rz = block_tree(rz, red);
+  if (threadIdx.x == 0) S.part_rz[tile] = rz;
+}
+
+// ------------------------------------------------------------------ backward sweep
+// t = (D~ + U)^-1 p for one colour, with p = z + beta p (f_update_p, pcg_solver.py:208-235)
+// formed on the way. The first sweep of an iteration reduces r.z, forms beta and, in its
+// workgroup 0, keeps the scalar state: atol at k = 0 (pcg_solver.py:928-930, 1089-1095),
+// rho, kk and beta for the later sweeps. The sweep of colour 0 also starts the forward sweep.
+template <int R>
+__global__ void __launch_bounds__(256) k_ssor_bwd(const SsorSys* __restrict__ sys, int tile0, int first, int mid) {
+  __shared__ double red[4];
+  const SsorView S = sview(sys);
+  gptr<PcgState> st = S.st;
+  if (load_done(st) >= 0) return;
+  const int tile = tile0 + blockIdx.x;
+  double beta;
+  if (first) {
+    const int k = st->k;
+    const double rz_cur = final_reduce(C(S.part_rz), S.ntiles, red);
+    beta = 0.0;
+    if (k > 0) {
+      const double rz_old = st->rho[(k - 1) & 1];
+      beta = (rz_old == 0.0) ? 0.0 : rz_cur / rz_old;
+    }
+    if (blockIdx.x == 0) {
+      if (k == 0) {
+        const double bb = final_reduce(C(S.part_bb), S.ntiles, red);
+        if (threadIdx.x == 0) {
+          const double atol = st->rtol * sqrt(bb);
+          st->bb = bb;
+          st->atol2 = atol * atol;
+          if (atol == 0.0) {
+            st->zero = 1;
+            st->done = 0;
+          }
+        }
+      }
+      if (threadIdx.x == 0) {
+        st->rho[k & 1] = rz_cur;
+        st->kk = k;
+        st->beta = beta;
+      }
+    }
+  } else {
+    beta = st->beta;
+  }
+  const gptr<const double> t = C(S.t);
+  double pq = 0.0;
+  for (int j = 0; j < R; ++j) {
+    const int v = tile_row<R>(tile, j);
+    const double dtv = S.dt[v];
+    const double pv = dtv * S.r[v] + beta * S.p[v];
+    S.p[v] = pv;
+    const double tv = (pv - upper_sum(S, v, t)) / dtv;
+    S.t[v] = tv;
+    if (mid) {
+      const double sv = ((pv - S.kd[v] * tv) - lower_sum(S, v, C(S.s))) / dtv;
+      S.s[v] = sv;
+      const double qv = tv + sv;
+      S.q[v] = qv;
+      pq += pv * qv;
+    }
+  }
+  if (mid) {
+    pq = block_tree(pq, red);
+    if (threadIdx.x == 0) S.part_pq[tile] = pq;
+  }
+}
+
+// ------------------------------------------------------------------ forward sweep
+// s = (D~ + L)^-1 (p - K t) for one colour, q = t + s = A^ p, partials of p.q (f_dot).
+template <int R>
+__global__ void __launch_bounds__(256) k_ssor_fwd(const SsorSys* __restrict__ sys, int tile0) {
+  __shared__ double red[4];
+  const SsorView S = sview(sys);
+  if (load_done(S.st) >= 0) return;
+  const int tile = tile0 + blockIdx.x;
+  const gptr<const double> s = C(S.s);
+  double pq = 0.0;
+  for (int j = 0; j < R; ++j) {
+    const int v = tile_row<R>(tile, j);
+    const double pv = S.p[v], tv = S.t[v];
+    const double sv = ((pv - S.kd[v] * tv) - lower_sum(S, v, s)) / S.dt[v];
+    S.s[v] = sv;
+    const double qv = tv + sv;
+    S.q[v] = qv;
+    pq += pv * qv;
+  }
+  pq = block_tree(pq, red);
+  if (threadIdx.x == 0) S.part_pq[tile] = pq;
+}
+
+// ------------------------------------------------------------------ update
+// f_update_xrz (pcg_solver.py:253-276) in the transformed variables: alpha = rz / pq (0 if
+// pq == 0); y += alpha p; r -= alpha q; z = D~ r; partials of r.z.
+template <int R>
+__global__ void __launch_bounds__(256) k_ssor_update(const SsorSys* __restrict__ sys) {
+  __shared__ double red[4];
+  const SsorView S = sview(sys);
+  gptr<PcgState> st = S.st;
+  if (load_done(st) >= 0) return;
+  const int kk = st->kk;
+  double yv[R], pv[R], rv[R], qv[R], dv[R];
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    const int v = tile_row<R>(blockIdx.x, j);
+    yv[j] = S.y[v];
+    pv[j] = S.p[v];
+    rv[j] = S.r[v];
+    qv[j] = S.q[v];
+    dv[j] = S.dt[v];
+  }
+  const double pq = final_reduce(C(S.part_pq), S.ntiles, red);
+  const double rz = st->rho[kk & 1];
+  const double alpha = (pq != 0.0) ? rz / pq : 0.0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) st->k = kk + 1;
+  double rzs = 0.0;
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    const int v = tile_row<R>(blockIdx.x, j);
+    S.y[v] = yv[j] + alpha * pv[j];
+    const double ri = rv[j] - alpha * qv[j];
+    S.r[v] = ri;
+    const double zi = dv[j] * ri;
+    rzs += ri * zi;
+  }
+  rzs = block_tree(rzs, red);
+  if (threadIdx.x == 0) S.part_rz[blockIdx.x] = rzs;
+}
+
+// ------------------------------------------------------------------ stopping test
+// ||b - A x||^2 = ||(D~ + L) r||^2 partials, then the test of iteration k-1 (pcg_loop.cpp:405-412)
+// in one workgroup. Issued after the last iteration of every window of check_every.
+template <int R>
+__global__ void __launch_bounds__(256) k_ssor_true_rr(const SsorSys* __restrict__ sys) {
+  __shared__ double red[4];
+  const SsorView S = sview(sys);
+  if (load_done(S.st) >= 0) return;
+  const gptr<const double> r = C(S.r);
+  double rr = 0.0;
+  for (int j = 0; j < R; ++j) {
+    const int v = tile_row<R>(blockIdx.x, j);
+    const double wv = S.dt[v] * r[v] + lower_sum(S, v, r);
+    rr += wv * wv;
+  }
+  rr = block_tree(rr, red);
+  if (threadIdx.x == 0) S.part_rr[blockIdx.x] = rr;
+}
+
+__global__ void __launch_bounds__(256) k_ssor_check(const SsorSys* __restrict__ sys) {
+  __shared__ double red[4];
+  const SsorView S = sview(sys);
+  gptr<PcgState> st = S.st;
+  if (load_done(st) >= 0) return;
+  const int k = st->k;
+  if (k < 1) return;
+  const int it = k - 1;
+  if (!((it + 1) % st->check_every == 0 || it == st->maxiter - 1)) return;
+  const double rr = final_reduce(C(S.part_rr), S.ntiles, red);
+  if (threadIdx.x == 0) {
+    st->rr_last = rr;
+    if (rr <= st->atol2) st->done = k;
+  }
+}
+
+// ------------------------------------------------------------------ back to x
+// x = (D~ + U)^-1 y for one colour (into w), then scattered to the original order; x = 0 when
+// atol == 0 (pcg_solver.py:1089-1095).
+template <int R>
+__global__ void __launch_bounds__(256) k_ssor_xsolve(const SsorSys* __restrict__ sys, int tile0) {
+  const SsorView S = sview(sys);
+  if (S.st->zero) return;
+  const int tile = tile0 + blockIdx.x;
+  const gptr<const double> x = C(S.w);
+  for (int j = 0; j < R; ++j) {
+    const int v = tile_row<R>(tile, j);
+    S.w[v] = (S.y[v] - upper_sum(S, v, x)) / S.dt[v];
+  }
+}
+
+template <int R>
+__global__ void __launch_bounds__(256) k_ssor_scatter(const SsorSys* __restrict__ sys) {
+  const SsorView S = sview(sys);
+  const bool zero = S.st->zero != 0;
+  for (int j = 0; j < R; ++j) {
+    const int v = tile_row<R>(blockIdx.x, j);
+    const int i = S.perm[v];
+    if (i >= 0) S.xout[i] = zero ? 0.0 : S.w[v];
+  }
+}
+
+__global__ void k_ssor_reset(const SsorSys* __restrict__ sys, double rtol, int maxiter, int check_every) {
+  if (threadIdx.x != 0) return;
+  gptr<PcgState> st = G(sys->st);
+  st->k = 0;
+  st->kk = 0;
+  st->done = -1;
+  st->zero = 0;
+  st->maxiter = maxiter;
+  st->check_every = check_every;
+  st->rtol = rtol;
+  st->atol2 = 0.0;
+  st->rr_last = -1.0;
+  st->bb = 0.0;
+  st->rho[0] = 0.0;
+  st->rho[1] = 0.0;
+  st->beta = 0.0;
+}
+
+// ------------------------------------------------------------------ launch wrappers
+#define GSH_SSOR_R(rpt, KER, GRID, ...)                                                  \
+  switch (rpt) {                                                                         \
+    case 1: hipLaunchKernelGGL(KER<1>, dim3(GRID), dim3(kTpb), 0, __VA_ARGS__); break;   \
+    case 2: hipLaunchKernelGGL(KER<2>, dim3(GRID), dim3(kTpb), 0, __VA_ARGS__); break;   \
+    case 8: hipLaunchKernelGGL(KER<8>, dim3(GRID), dim3(kTpb), 0, __VA_ARGS__); break;   \
+    default: hipLaunchKernelGGL(KER<4>, dim3(GRID), dim3(kTpb), 0, __VA_ARGS__); break;  \
+  }
+
+void launch_ssor_reset(const SsorSys* sys, double rtol, int maxiter, int check_every, hipStream_t s) {
+  hipLaunchKernelGGL(k_ssor_reset, dim3(1), dim3(64), 0, s, sys, rtol, maxiter, check_every);
+}
+void launch_ssor_setup(const SsorSys* sys, int ntiles, int rpt, hipStream_t s) {
+  if (ntiles > 0) GSH_SSOR_R(rpt, k_ssor_setup, ntiles, s, sys);
+}
+void launch_ssor_residual(const SsorSys* sys, int ntiles, int rpt, hipStream_t s) {
+  if (ntiles > 0) GSH_SSOR_R(rpt, k_ssor_residual, ntiles, s, sys);
+}
+void launch_ssor_r0(const SsorSys* sys, int tile0, int ntc, int rpt, hipStream_t s) {
+  if (ntc > 0) GSH_SSOR_R(rpt, k_ssor_r0, ntc, s, sys, tile0);
+}
+void launch_ssor_bwd(const SsorSys* sys, int tile0, int ntc, int first, int mid, int rpt, hipStream_t s) {
+  if (ntc > 0) GSH_SSOR_R(rpt, k_ssor_bwd, ntc, s, sys, tile0, first, mid);
+}
+void launch_ssor_fwd(const SsorSys* sys, int tile0, int ntc, int rpt, hipStream_t s) {
+  if (ntc > 0) GSH_SSOR_R(rpt, k_ssor_fwd, ntc, s, sys, tile0);
+}
+void launch_ssor_update(const SsorSys* sys, int ntiles, int rpt, hipStream_t s) {
+  if (ntiles > 0) GSH_SSOR_R(rpt, k_ssor_update, ntiles, s, sys);
+}
+void launch_ssor_true_rr(const SsorSys* sys, int ntiles, int rpt, hipStream_t s) {
+  if (ntiles > 0) GSH_SSOR_R(rpt, k_ssor_true_rr, ntiles, s, sys);
+}
+void launch_ssor_check(const SsorSys* sys, hipStream_t s) {
+  hipLaunchKernelGGL(k_ssor_check, dim3(1), dim3(kTpb), 0, s, sys);
+}
+void launch_ssor_xsolve(const SsorSys* sys, int tile0, int ntc, int rpt, hipStream_t s) {
+  if (ntc > 0) GSH_SSOR_R(rpt, k_ssor_xsolve, ntc, s, sys, tile0);
+}
+void launch_ssor_scatter(const SsorSys* sys, int ntiles, int rpt, hipStream_t s) {
+  if (ntiles > 0) GSH_SSOR_R(rpt, k_ssor_scatter, ntiles, s, sys);
+}
+
+}  // namespace gsh

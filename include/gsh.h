@@ -46,6 +46,8 @@ extern "C" {
 #define GSH_PREC_MIXED    1   /* FP32 SpMV/precond/axpy, FP64 dots + true residual + refinement */
 
 #define GSH_PRECOND_JACOBI 0  /* reference preconditioner (pcg_solver.py:933-938) */
+#define GSH_PRECOND_SSOR   1  /* multicolour symmetric SOR, FP64 only (no counterpart in the reference,
+                                 whose only preconditioner is Jacobi; same driver and stopping rule) */
 
 /* ---------------------------------------------------------------- context */
 typedef struct gsh_ctx gsh_ctx;
@@ -226,6 +228,15 @@ int gsh_solver_last_stats(gsh_solver* s, gsh_solve_stats* out);
 /* Tuning knobs: rows per thread of the PCG tiles (1, 2, 4, 8) and the inner tolerance of the
  * mixed-precision refinement. */
 int gsh_solver_set_tuning(gsh_solver* s, int rows_per_thread, double inner_rtol);
+/* SSOR relaxation factor omega in (0, 2), default 1 (symmetric Gauss-Seidel). */
+int gsh_solver_set_omega(gsh_solver* s, double omega);
+/* SSOR layout: number of colours, rows after padding every colour block to whole reduction
+ * tiles, and the colour of every row (n entries, may be NULL). */
+int gsh_solver_ssor_info(gsh_solver* s, int32_t* ncolours, int32_t* padded_rows, int32_t* colour_out);
+/* Host only: the SSOR solver's colouring of a CSR pattern (greedy first fit in smallest-last
+ * order, ties by smaller index; the diagonal is ignored). */
+int gsh_colour_csr(int32_t n, const int32_t* indptr, const int32_t* indices, int32_t* colour_out,
+                   int32_t* ncolours);
 
 /* updateElevation (meshdata.cu:4449-4456) after rollUpElevation (:3538-3544):
  * eta1 <- eta2, eta2 <- x. */
@@ -267,15 +278,16 @@ int gsh_server_poll(gsh_server* srv, int64_t ticket, double* x_host, int32_t* it
 /* ---------------------------------------------------------------- measurement */
 /* No counterpart in the reference: measurement helpers of this library.
  * gsh_copy_probe: average time of one device-to-device copy of `bytes` (read + write) by a
- * streaming kernel, the achievable-bandwidth reference. gsh_solver_profile: one solve (same
- * arithmetic as gsh_solver_solve) with HIP events around every PCG phase launch; returns the
- * mean duration of phase A (p update + SpMV + p.q) and phase B (x, r, z updates + r.r, r.z). */
+ * streaming kernel, the achievable-bandwidth reference. gsh_solver_profile: one FP64 solve (same
+ * arithmetic as gsh_solver_solve) with HIP events around the launches of every iteration;
+ * ms[4] receives the mean time per iteration of each launch kind:
+ *   Jacobi: ms[0] phase A (p update + SpMV + p.q), ms[1] phase B (x, r, z updates + r.r, r.z);
+ *   SSOR:   ms[0] backward sweeps (all colours), ms[1] forward sweeps, ms[2] vector update. */
 int gsh_copy_probe(gsh_ctx* ctx, int64_t bytes, int reps, double* ms_avg);
-int gsh_solver_profile(gsh_solver* s, double rtol, int maxiter, int check_every, double* ms_phase_a,
-                       double* ms_phase_b, int* iters_out);
+int gsh_solver_profile(gsh_solver* s, double rtol, int maxiter, int check_every, double* ms, int* iters_out);
 
 /* ---------------------------------------------------------------- synthetic inputs */
-/* Jittered-grid Delaunay channel (BASELINE configs): nx*ny nodes spaced dx, interior nodes
+/* Jittered-grid Delaunay channel: nx*ny nodes spaced dx, interior nodes
  * jittered uniformly by +-jitter*dx (boundary nodes only along the boundary), cells split and
  * Lawson-flipped to Delaunay. The west edge (x = 0) is the open boundary. */
 typedef struct gsh_gen gsh_gen;

@@ -131,3 +131,25 @@ def test_rcm_reduces_bandwidth():
     assert bw <= 2 * 40 + 5
     # permutation maps back to the input coordinates
     assert np.array_equal(m.topology(L.T_XY), xy_s[m.node_perm])
+
+
+def test_ssor_colouring_matches_oracle():
+    """The library's SSOR colouring (host code) equals the oracle's on mesh and random patterns."""
+    import scipy.sparse as sp
+
+    from gschism_amd import cases, core
+    from oracle import pyoracle as O
+
+    xy, en, ob = cases.channel(40, 30, 100.0)
+    m = core.Mesh(None, xy, en, np.full(xy.shape[0], 5.0), cases.sigma_levels(3), open_bnd=ob)
+    pats = [(m.topology(L.T_CSR_INDPTR), m.topology(L.T_CSR_INDICES))]
+    A = sp.random(3000, 3000, density=4e-3, random_state=3, format="csr")
+    A = (A + A.T + sp.eye(3000)).tocsr()
+    pats.append((A.indptr, A.indices))
+    for ip, ix in pats:
+        col, nc = core.colour_csr(ip, ix)
+        ocol, onc = O.colour(ip, ix)
+        assert nc == onc
+        assert np.array_equal(col, ocol)
+    with pytest.raises(L.GshError):
+        core.colour_csr(np.array([0, 1], np.int32), np.array([5], np.int32))

@@ -1,0 +1,117 @@
+"""GPU parity of the multicolour SSOR solver against the oracle, bit for bit."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from conftest import assert_bitwise
+from gschism_amd import _lib as L
+from gschism_amd import cases, core
+from oracle import pyoracle as O
+from test_gpu_parity import _setup
+
+pytestmark = pytest.mark.gpu
+
+
+def _assembled(ctx, spec):
+    mesh, om, p = _setup(ctx, spec)
+    mesh.backtrack(p)
+    mesh.rhs_producers(p)
+    om.set(L.F_BTU, mesh.get(L.F_BTU))
+    om.rhs(p)
+    val, b = om.assemble(p, nthreads=8)
+    return mesh, om, p, val, b
+
+
+def _compare(solver, ip, ix, val, b, hist=None, omega=1.0, rpt=1, rtol=1e-12, maxiter=3000, check_every=10,
+             what=""):
+    x, it, st = solver.solve(None, rtol=rtol, maxiter=maxiter, check_every=check_every)
+    xo, ito, rr, at, nc = O.pcg_ssor(ip, ix, val, b, hist=hist, omega=omega, rtol=rtol, maxiter=maxiter,
+                                     check_every=check_every, rows_per_thread=rpt, nthreads=8)
+    assert it == ito, (what, it, ito)
+    assert_bitwise(x, xo, f"{what} SSOR solution")
+    s = solver.stats()
+    assert s.atol2 == at and s.rr == rr, what
+    return x, it, st
+
+
+@pytest.mark.parametrize("rpt", [1, 4])
+def test_ssor_channel_200x80(ctx, rpt):
+    spec = cases.spec_cpu_reference()
+    print(f"[gpu] SSOR {spec.name} rpt={rpt}", flush=True)
+    mesh, om, p, val, b = _assembled(ctx, spec)
+    solver = core.Solver(mesh, precond=L.PRECOND_SSOR)
+    solver.set_tuning(rpt)
+    solver.assemble(p)
+    assert_bitwise(solver.get_values(), val, "CSR values")
+    ip, ix = mesh.topology(L.T_CSR_INDPTR), mesh.topology(L.T_CSR_INDICES)
+    nc, npad, col = solver.ssor_info()
+    ocol, onc = O.colour(ip, ix)
+    assert nc == onc and np.array_equal(col, ocol)
+    assert npad % (256 * rpt) == 0
+    x, it, st = _compare(solver, ip, ix, val, b, rpt=rpt, what=f"rpt={rpt}")
+    assert st == 0
+    assert solver.check() < 1e-10
+    # fewer iterations than the Jacobi solver on the same system
+    jac = core.Solver(mesh)
+    jac.assemble(p)
+    _, itj, _ = jac.solve(None, rtol=1e-12, maxiter=3000)
+    assert it < itj
+    print(f"[gpu] SSOR {it} iterations, Jacobi {itj}, {nc} colours", flush=True)
+
+
+def test_ssor_history_omega_and_steps(ctx):
+    spec = cases.CaseSpec("channel-60x30", 60, 30, 500.0, nvrt=8, depth=None, substeps=5)
+    mesh, om, p, val, b = _assembled(ctx, spec)
+    solver = core.Solver(mesh, precond=L.PRECOND_SSOR)
+    solver.set_omega(1.3)
+    ip, ix = mesh.topology(L.T_CSR_INDPTR), mesh.topology(L.T_CSR_INDICES)
+    hist = [np.zeros(mesh.n)] * 3
+    for step in range(3):
+        print(f"[gpu] SSOR omega 1.3 step {step}", flush=True)
+        if step > 0:
+            mesh.backtrack(p)
+            mesh.rhs_producers(p)
+            om.set(L.F_BTU, mesh.get(L.F_BTU))
+            om.rhs(p)
+            val, b = om.assemble(p)
+        solver.assemble(p)
+        x, it, st = _compare(solver, ip, ix, val, b, hist=hist, omega=1.3, what=f"step {step}")
+        hist = [x, hist[0], hist[1]]
+        solver.update_elevation()
+        om.set(L.F_ETA2, x)
+        mesh.update_levels(p)
+        om.levels(p)
+
+
+def test_ssor_standalone_csr_and_special_cases(ctx):
+    rng = np.random.default_rng(5)
+    n = 5000
+    A = sp.random(n, n, density=5.0 / n, random_state=6, format="csr")
+    A = A + A.T
+    A = (A + sp.diags(np.abs(A).sum(1).A1 + 0.5)).tocsr()
+    A.sort_indices()
+    b = rng.normal(size=n)
+    print("[gpu] SSOR standalone CSR", flush=True)
+    s = core.Solver(None, ctx=ctx, indptr=A.indptr, indices=A.indices, precond=L.PRECOND_SSOR)
+    s.set_values(A.data)
+    x, it, st = s.solve(b, rtol=1e-12, maxiter=2000)
+    xo, ito, _, _, _ = O.pcg_ssor(A.indptr, A.indices, A.data, b, rtol=1e-12, maxiter=2000, rows_per_thread=1)
+    assert it == ito
+    assert_bitwise(x, xo, "standalone SSOR")
+    # zero right-hand side: atol = 0, x = 0, no iterations
+    x, it, st = s.solve(np.zeros(n), rtol=1e-12, maxiter=100)
+    assert it == 0 and st == 0 and not x.any()
+    # maxiter reached without convergence: status flag, iterations = maxiter
+    s.reset_history()
+    x, it, st = s.solve(b, rtol=1e-30, maxiter=25, check_every=10)
+    assert it == 25 and st & L.GSH_W_MAXITER
+    xo, ito, _, _, _ = O.pcg_ssor(A.indptr, A.indices, A.data, b, rtol=1e-30, maxiter=25, check_every=10,
+                                  rows_per_thread=1)
+    assert ito == 25
+    assert_bitwise(x, xo, "SSOR at maxiter")
+    with pytest.raises(L.GshError):
+        s.set_omega(2.5)
+    with pytest.raises(L.GshError):
+        core.Solver(None, L.PREC_MIXED, ctx=ctx, indptr=A.indptr, indices=A.indices, precond=L.PRECOND_SSOR)
