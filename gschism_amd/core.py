@@ -320,11 +320,11 @@ class Server:
     PCG launches at check-window boundaries."""
 
     def __init__(self, ctx: Context, rtol: float = 1e-12, maxiter: int = 3000, check_every: int = 10,
-                 max_active: int = 0, hbm_budget_bytes: int = 0):
+                 max_active: int = 0, hbm_budget_bytes: int = 0, precision: int = L.PREC_FP64):
         cfg = L.ServerConfig()
         cfg.hbm_budget_bytes = hbm_budget_bytes
         cfg.max_active = max_active
-        cfg.precision = L.PREC_FP64
+        cfg.precision = precision
         cfg.rtol = rtol
         cfg.maxiter = maxiter
         cfg.check_every = check_every

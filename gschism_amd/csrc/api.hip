@@ -306,10 +306,13 @@ static void ssor_build(gsh_solver* s) {
   h.ntiles = L.np / T;
   h.ncol = L.ncol;
   h.omega = s->omega;
-  h.lptr = P.upload(L.lptr);
+  std::vector<SsorColour> cols(L.ncol);
+  for (int c = 0; c < L.ncol; ++c)
+    cols[c] = {L.cstart[c], L.cstart[c + 1] - L.cstart[c], L.wl[c], L.wu[c], (int32_t)L.offl[c], (int32_t)L.offu[c], 0, 0};
+  h.colours = P.upload(cols);
+  h.tile_colour = P.upload(L.tile_colour);
   h.lcol = P.upload(L.lcol);
   h.lsrc = P.upload(L.lsrc);
-  h.uptr = P.upload(L.uptr);
   h.ucol = P.upload(L.ucol);
   h.usrc = P.upload(L.usrc);
   h.dsrc = P.upload(L.dsrc);
@@ -337,6 +340,7 @@ static void ssor_sync_desc(gsh_solver* s) {
   h.h1 = s->h[1];
   h.h2 = s->h[2];
   h.omega = s->omega;
+  h.unit_omega = s->omega == 1.0;
   HIPCHK(hipMemcpyAsync(s->ssor->ds, &h, sizeof(h), hipMemcpyHostToDevice, s->ctx->stream));
   HIPCHK(hipStreamSynchronize(s->ctx->stream));
 }
@@ -366,14 +370,15 @@ struct PhaseTimes {
 template <class VT>
 static void run_pcg(gsh_ctx* ctx, const PcgSys<VT>* dsys, int nsys, const TileRef* dtiles, int ntiles, int rpt,
                     double rtol, int maxiter, int check_every, double timeout_s, PcgOutcome& out,
-                    const TileRef* dinit_tiles = nullptr, int ninit_tiles = -1, PhaseTimes* prof = nullptr) {
+                    const TileRef* dinit_tiles = nullptr, int ninit_tiles = -1, PhaseTimes* prof = nullptr,
+                    const double* rtols_dev = nullptr) {
   hipStream_t st = ctx->stream;
   if (check_every < 1) check_every = 1;
   if (timeout_s <= 0.0) timeout_s = 60.0;
   PcgState* slots = ctx->pinned_slots(nsys);
   PcgState* dgather = nullptr;
   HIPCHK(hipMallocAsync((void**)&dgather, sizeof(PcgState) * 2 * nsys, st));
-  launch_pcg_reset<VT>(dsys, nsys, rtol, maxiter, check_every, st);
+  launch_pcg_reset<VT>(dsys, nsys, rtol, maxiter, check_every, st, rtols_dev);
   if (dinit_tiles)
     launch_pcg_init<VT>(dsys, dinit_tiles, ninit_tiles, rpt, st);
   else
@@ -392,21 +397,19 @@ static void run_pcg(gsh_ctx* ctx, const PcgSys<VT>* dsys, int nsys, const TileRe
   while (issued < maxiter && !all_done) {
     const int w = std::min(check_every, maxiter - issued);
     if (prof) {
-      // timed window: events around each phase, read back before the next window
-      std::vector<hipEvent_t> evs(2 * w + 1);
+      // timed window: the runtime stamps each phase kernel's start and end into its own event
+      // pair; read back before the next window
+      std::vector<hipEvent_t> evs(4 * w);
       for (auto& e : evs) HIPCHK(hipEventCreate(&e));
-      HIPCHK(hipEventRecord(evs[0], st));
       for (int i = 0; i < w; ++i) {
-        launch_pcg_phase<VT>(0, dsys, dtiles, ntiles, rpt, st);
-        HIPCHK(hipEventRecord(evs[2 * i + 1], st));
-        launch_pcg_phase<VT>(1, dsys, dtiles, ntiles, rpt, st);
-        HIPCHK(hipEventRecord(evs[2 * i + 2], st));
+        launch_pcg_phase<VT>(0, dsys, dtiles, ntiles, rpt, st, evs[4 * i], evs[4 * i + 1]);
+        launch_pcg_phase<VT>(1, dsys, dtiles, ntiles, rpt, st, evs[4 * i + 2], evs[4 * i + 3]);
       }
-      HIPCHK(hipEventSynchronize(evs[2 * w]));
+      HIPCHK(hipEventSynchronize(evs[4 * w - 1]));
       for (int i = 0; i < w; ++i) {
         float a = 0.f, b = 0.f;
-        HIPCHK(hipEventElapsedTime(&a, evs[2 * i], evs[2 * i + 1]));
-        HIPCHK(hipEventElapsedTime(&b, evs[2 * i + 1], evs[2 * i + 2]));
+        HIPCHK(hipEventElapsedTime(&a, evs[4 * i], evs[4 * i + 1]));
+        HIPCHK(hipEventElapsedTime(&b, evs[4 * i + 2], evs[4 * i + 3]));
         prof->ms_a += a;
         prof->ms_b += b;
         prof->launches += 1;
@@ -467,12 +470,12 @@ static PcgState run_ssor(gsh_solver* s, double rtol, int maxiter, int check_ever
   launch_ssor_reset(ds, rtol, maxiter, check_every, st);
   launch_ssor_setup(ds, NT, rpt, st);
   launch_ssor_residual(ds, NT, rpt, st);
-  for (int c = 0; c < C; ++c) launch_ssor_r0(ds, S.tile0[c], S.ntc[c], rpt, st);
+  for (int c = 0; c < C; ++c) launch_ssor_r0(ds, S.tile0[c], S.ntc[c], c, rpt, st);
   auto bwd = [&]() {
-    for (int c = C - 1; c >= 0; --c) launch_ssor_bwd(ds, S.tile0[c], S.ntc[c], c == C - 1, c == 0, rpt, st);
+    for (int c = C - 1; c >= 0; --c) launch_ssor_bwd(ds, S.tile0[c], S.ntc[c], c, c == C - 1, c == 0, rpt, st);
   };
   auto fwd = [&]() {
-    for (int c = 1; c < C; ++c) launch_ssor_fwd(ds, S.tile0[c], S.ntc[c], rpt, st);
+    for (int c = 1; c < C; ++c) launch_ssor_fwd(ds, S.tile0[c], S.ntc[c], c, rpt, st);
   };
   HIPCHK(hipGetLastError());
   int issued = 0, window = 0;
@@ -486,24 +489,23 @@ static PcgState run_ssor(gsh_solver* s, double rtol, int maxiter, int check_ever
   while (issued < maxiter && !done) {
     const int w = std::min(check_every, maxiter - issued);
     if (prof) {
-      hipEvent_t e[4];
+      // every launch stamps its own kernel start and end (hipExtLaunchKernelGGL events)
+      const int nl = 2 * C + 1;
+      std::vector<hipEvent_t> e(2 * nl);
       for (auto& x : e) HIPCHK(hipEventCreate(&x));
       for (int i = 0; i < w; ++i) {
-        HIPCHK(hipEventRecord(e[0], st));
-        bwd();
-        HIPCHK(hipEventRecord(e[1], st));
-        fwd();
-        HIPCHK(hipEventRecord(e[2], st));
-        launch_ssor_update(ds, NT, rpt, st);
-        HIPCHK(hipEventRecord(e[3], st));
-        HIPCHK(hipEventSynchronize(e[3]));
-        float a = 0.f, b = 0.f, c = 0.f;
-        HIPCHK(hipEventElapsedTime(&a, e[0], e[1]));
-        HIPCHK(hipEventElapsedTime(&b, e[1], e[2]));
-        HIPCHK(hipEventElapsedTime(&c, e[2], e[3]));
-        prof->ms_bwd += a;
-        prof->ms_fwd += b;
-        prof->ms_upd += c;
+        int l = 0;
+        for (int c = C - 1; c >= 0; --c, ++l)
+          launch_ssor_bwd(ds, S.tile0[c], S.ntc[c], c, c == C - 1, c == 0, rpt, st, e[2 * l], e[2 * l + 1]);
+        for (int c = 1; c < C; ++c, ++l)
+          launch_ssor_fwd(ds, S.tile0[c], S.ntc[c], c, rpt, st, e[2 * l], e[2 * l + 1]);
+        launch_ssor_update(ds, NT, rpt, st, e[2 * l], e[2 * l + 1]);
+        HIPCHK(hipEventSynchronize(e[2 * l + 1]));
+        for (int m = 0; m <= l; ++m) {
+          float a = 0.f;
+          HIPCHK(hipEventElapsedTime(&a, e[2 * m], e[2 * m + 1]));
+          (m < C ? prof->ms_bwd : (m < 2 * C - 1 ? prof->ms_fwd : prof->ms_upd)) += a;
+        }
         prof->iters += 1;
       }
       for (auto& x : e) (void)hipEventDestroy(x);
@@ -531,7 +533,7 @@ static PcgState run_ssor(gsh_solver* s, double rtol, int maxiter, int check_ever
       throw Error(GSH_E_TIMEOUT, "SSOR PCG: time limit exceeded while issuing iterations");
     HIPCHK(hipGetLastError());
   }
-  for (int c = C - 1; c >= 0; --c) launch_ssor_xsolve(ds, S.tile0[c], S.ntc[c], rpt, st);
+  for (int c = C - 1; c >= 0; --c) launch_ssor_xsolve(ds, S.tile0[c], S.ntc[c], c, rpt, st);
   launch_ssor_scatter(ds, NT, rpt, st);
   HIPCHK(hipMemcpyAsync(slots, S.hs.st, sizeof(PcgState), hipMemcpyDeviceToHost, st));
   HIPCHK(hipEventRecord(ctx->ev[0], st));
@@ -540,57 +542,132 @@ static PcgState run_ssor(gsh_solver* s, double rtol, int maxiter, int check_ever
   return slots[0];
 }
 
-// Mixed precision: FP64 true residual + refinement around FP32 PCG solves (FP64 dots).
-static void solve_mixed(gsh_solver* s, double rtol, int maxiter, int check_every, double timeout_s, int* iters_out,
-                        int* status_out) {
-  gsh_ctx* ctx = s->ctx;
+// Mixed precision for `count` systems in lock-step: FP64 true residual and refinement around FP32
+// PCG inner solves (FP64 dot products). Every system keeps its own outer state: its inner
+// tolerance is max(inner_rtol, 0.5 sqrt(atol^2 / r.r)) from its own residual, each inner solve is
+// bounded by maxiter, and the inner iterations add up to at most maxiter plus one inner solve,
+// so a system's result does not depend on the other systems of the batch.
+static void run_mixed(gsh_ctx* ctx, gsh_solver* const* S, int count, double rtol, int maxiter, int check_every,
+                      double timeout_s, std::vector<int>& iters, std::vector<int>& status) {
   hipStream_t st = ctx->stream;
-  const int n = s->n;
-  launch_cast_values(s->val, s->val32, s->nnz, st);
-  hipLaunchKernelGGL(k_extrapolate, dim3((n + 255) / 256), dim3(256), 0, st, s->x, s->h[0], s->h[1], s->h[2], n);
-  double hostred[2];
-  int total_inner = 0, outer = 0;
-  double atol2 = -1.0, rr = 0.0;
-  bool converged = false;
+  const int rpt = S[0]->rpt;
+  std::vector<PcgSys<double>> h64(count);
+  std::vector<PcgSys<float>> h32(count);
+  for (int i = 0; i < count; ++i) {
+    gsh_solver* s = S[i];
+    launch_cast_values(s->val, s->val32, s->nnz, st);
+    hipLaunchKernelGGL(k_extrapolate, dim3((s->n + 255) / 256), dim3(256), 0, st, s->x, s->h[0], s->h[1], s->h[2],
+                       s->n);
+    h64[i] = s->hs;
+    h64[i].x = s->x;
+    h64[i].r = s->r64;
+    h64[i].ntiles = s->ntiles;
+    h32[i] = s->hs32;
+    h32[i].ntiles = s->ntiles;
+  }
+  PcgSys<double>* d64 = nullptr;
+  PcgSys<double>* d64s = nullptr;
+  PcgSys<float>* d32s = nullptr;
+  TileRef* dtl = nullptr;
+  double* dred = nullptr;
+  double* drt = nullptr;
+  int64_t all_tiles = 0;
+  for (int i = 0; i < count; ++i) all_tiles += S[i]->ntiles;
+  HIPCHK(hipMallocAsync((void**)&d64, sizeof(PcgSys<double>) * count, st));
+  HIPCHK(hipMallocAsync((void**)&d64s, sizeof(PcgSys<double>) * count, st));
+  HIPCHK(hipMallocAsync((void**)&d32s, sizeof(PcgSys<float>) * count, st));
+  HIPCHK(hipMallocAsync((void**)&dtl, sizeof(TileRef) * std::max<int64_t>(1, all_tiles), st));
+  HIPCHK(hipMallocAsync((void**)&dred, sizeof(double) * 2 * count, st));
+  HIPCHK(hipMallocAsync((void**)&drt, sizeof(double) * count, st));
+  HIPCHK(hipMemcpyAsync(d64, h64.data(), sizeof(PcgSys<double>) * count, hipMemcpyHostToDevice, st));
+  std::vector<double> atol2(count, -1.0), rr(count, 0.0), red(2 * count);
+  std::vector<int> outer(count, 0), inner(count, 0), state(count, 0);  // 0 running, 1 converged, 2 stopped
   const int max_outer = 30;
-  for (; outer < max_outer; ++outer) {
-    launch_true_residual(s->ds, s->dtiles, s->ntiles, s->rpt, s->x, s->r64, st);
-    launch_reduce_pair(s->hs.part_rr, s->hs.part_bb, s->ntiles, s->red2, st);
-    HIPCHK(hipMemcpyAsync(hostred, s->red2, sizeof(hostred), hipMemcpyDeviceToHost, st));
+  for (int round = 0; round < max_outer; ++round) {
+    // FP64 true residual of every running system
+    std::vector<TileRef> tl;
+    for (int i = 0; i < count; ++i)
+      if (state[i] == 0)
+        for (int t = 0; t < S[i]->ntiles; ++t) tl.push_back({i, t});
+    if (tl.empty()) break;
+    HIPCHK(hipMemcpyAsync(dtl, tl.data(), sizeof(TileRef) * tl.size(), hipMemcpyHostToDevice, st));
+    launch_residual_batch(d64, dtl, (int)tl.size(), rpt, st);
+    launch_reduce_systems(d64, count, dred, st);
+    HIPCHK(hipMemcpyAsync(red.data(), dred, sizeof(double) * 2 * count, hipMemcpyDeviceToHost, st));
     HIPCHK(hipEventRecord(ctx->ev[0], st));
     wait_event(ctx->ev[0], timeout_s, "refinement residual");
-    rr = hostred[0];
-    if (outer == 0) {
-      const double atol = rtol * std::sqrt(hostred[1]);
-      atol2 = atol * atol;
-      if (atol == 0.0) {
-        launch_fill_f64(s->x, 0.0, n, st);
-        converged = true;
-        break;
+    // outer decisions; the running systems that need an inner solve, in index order
+    std::vector<int> sub;
+    std::vector<double> rts;
+    for (int i = 0; i < count; ++i) {
+      if (state[i] != 0) continue;
+      rr[i] = red[2 * i];
+      outer[i] += 1;
+      if (round == 0) {
+        const double atol = rtol * std::sqrt(red[2 * i + 1]);
+        atol2[i] = atol * atol;
+        if (atol == 0.0) {  // pcg_solver.py:1089-1095
+          launch_fill_f64(S[i]->x, 0.0, S[i]->n, st);
+          state[i] = 1;
+          continue;
+        }
+      }
+      if (rr[i] <= atol2[i]) {
+        state[i] = 1;
+      } else if (inner[i] >= maxiter) {
+        state[i] = 2;
+      } else {
+        sub.push_back(i);
+        rts.push_back(std::max(S[i]->inner_rtol, 0.5 * std::sqrt(atol2[i] / rr[i])));
       }
     }
-    if (rr <= atol2) {
-      converged = true;
-      break;
+    if (sub.empty()) break;
+    // inner FP32 solves A d = r of the subset, lock-step, from d = 0
+    const int ns = (int)sub.size();
+    std::vector<PcgSys<double>> s64(ns);
+    std::vector<PcgSys<float>> s32(ns);
+    std::vector<TileRef> stl;
+    for (int j = 0; j < ns; ++j) {
+      s64[j] = h64[sub[j]];
+      s32[j] = h32[sub[j]];
+      for (int t = 0; t < S[sub[j]]->ntiles; ++t) stl.push_back({j, t});
     }
-    if (total_inner >= maxiter) break;
-    // inner FP32 solve of A d = r to inner_rtol (relative to ||r||), started from d = 0
+    HIPCHK(hipMemcpyAsync(d64s, s64.data(), sizeof(PcgSys<double>) * ns, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(d32s, s32.data(), sizeof(PcgSys<float>) * ns, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(dtl, stl.data(), sizeof(TileRef) * stl.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(drt, rts.data(), sizeof(double) * ns, hipMemcpyHostToDevice, st));
     PcgOutcome o;
-    const double in_rtol = std::max(s->inner_rtol, 0.5 * std::sqrt(atol2 / rr));
-    run_pcg<float>(ctx, s->ds32, 1, s->dtiles, s->ntiles, s->rpt, in_rtol, maxiter - total_inner, check_every,
-                   timeout_s, o);
-    const int it = iters_of(o.st[0], maxiter - total_inner);
-    total_inner += it;
-    launch_refine_update(s->x, s->hs32.x, n, st);
-    if (it == 0) break;
+    run_pcg<float>(ctx, d32s, ns, dtl, (int)stl.size(), rpt, rtol, maxiter, check_every, timeout_s, o, nullptr, -1,
+                   nullptr, drt);
+    launch_refine_batch(d64s, d32s, dtl, (int)stl.size(), rpt, st);
+    for (int j = 0; j < ns; ++j) {
+      const int it = iters_of(o.st[j], maxiter);
+      inner[sub[j]] += it;
+      if (it == 0) state[sub[j]] = 2;  // no progress possible
+    }
   }
   HIPCHK(hipStreamSynchronize(st));
-  s->last.iters = total_inner;
-  s->last.outer = outer + 1;
-  s->last.rr = rr;
-  s->last.atol2 = atol2;
-  *iters_out = total_inner;
-  *status_out = converged ? 0 : GSH_W_MAXITER;
+  for (void* p : {(void*)d64, (void*)d64s, (void*)d32s, (void*)dtl, (void*)dred, (void*)drt}) (void)hipFree(p);
+  iters.assign(count, 0);
+  status.assign(count, 0);
+  for (int i = 0; i < count; ++i) {
+    gsh_solver* s = S[i];
+    iters[i] = inner[i];
+    status[i] = state[i] == 1 ? 0 : GSH_W_MAXITER;
+    s->last.iters = inner[i];
+    s->last.outer = outer[i];
+    s->last.rr = rr[i];
+    s->last.atol2 = atol2[i];
+  }
+}
+
+static void solve_mixed(gsh_solver* s, double rtol, int maxiter, int check_every, double timeout_s, int* iters_out,
+                        int* status_out) {
+  std::vector<int> it, stv;
+  gsh_solver* one[1] = {s};
+  run_mixed(s->ctx, one, 1, rtol, maxiter, check_every, timeout_s, it, stv);
+  *iters_out = it[0];
+  *status_out = stv[0];
 }
 
 // ====================================================================== C ABI
@@ -1292,7 +1369,8 @@ int gsh_batch_solve(gsh_solver* const* solvers, int count, double rtol, int maxi
     gsh_ctx* ctx = solvers[0]->ctx;
     for (int i = 0; i < count; ++i) {
       GSH_REQUIRE(solvers[i] && solvers[i]->ctx == ctx, GSH_E_ARG, "all solvers must share one context");
-      GSH_REQUIRE(solvers[i]->precision == GSH_PREC_FP64, GSH_E_ARG, "batched solve supports FP64 solvers");
+      GSH_REQUIRE(solvers[i]->precision == solvers[0]->precision, GSH_E_ARG,
+                  "all solvers of a batch need the same precision");
       GSH_REQUIRE(solvers[i]->precond == GSH_PRECOND_JACOBI, GSH_E_ARG, "batched solve supports Jacobi solvers");
       for (int j = 0; j < i; ++j) GSH_REQUIRE(solvers[j] != solvers[i], GSH_E_ARG, "a solver appears twice");
     }
@@ -1318,6 +1396,17 @@ int gsh_batch_solve(gsh_solver* const* solvers, int count, double rtol, int maxi
     if (maxiter <= 0) {
       maxiter = 0;
       for (int i = 0; i < count; ++i) maxiter = std::max(maxiter, 10 * solvers[i]->n);
+    }
+    if (solvers[0]->precision == GSH_PREC_MIXED) {
+      for (int i = 0; i < count; ++i) solvers[i]->hs32.ntiles = solvers[i]->ntiles;
+      std::vector<int> its, sts;
+      run_mixed(ctx, solvers, count, rtol, maxiter, check_every, timeout_s, its, sts);
+      for (int i = 0; i < count; ++i) {
+        if (iters_out) iters_out[i] = its[i];
+        solvers[i]->last.true_rel = -1.0;
+        solver_roll(solvers[i]);
+      }
+      return GSH_OK;
     }
     PcgSys<double>* dsys = nullptr;
     TileRef* dt = nullptr;
@@ -1366,6 +1455,11 @@ struct Request {
   double latency_ms = 0.0;
   gsh_solver* solver = nullptr;
   int64_t bytes = 0;
+  // mixed precision: outer refinement state
+  int phase = 0;          // 0: FP64 residual check due, 1: FP32 inner solve running
+  bool inner_fresh = false;
+  int outer = 0, inner_total = 0;
+  double atol2 = -1.0, in_rtol = 0.0;
 };
 }  // namespace
 
@@ -1387,6 +1481,10 @@ struct gsh_server {
   int dfresh_cap = 0;
   PcgState* dgather = nullptr;
   int dgather_cap = 0;
+  PcgSys<float>* dsys32 = nullptr;
+  int dsys32_cap = 0;
+  double* dred = nullptr;
+  int dred_cap = 0;
   ~gsh_server() {
     for (auto& v : pool)
       for (auto* s : v) gsh_solver_destroy(s);
@@ -1396,13 +1494,10 @@ struct gsh_server {
     (void)hipFree(dtiles);
     (void)hipFree(dfresh);
     (void)hipFree(dgather);
+    (void)hipFree(dsys32);
+    (void)hipFree(dred);
   }
 };
-
-static int64_t solver_bytes(const gsh_mesh* m) {
-  const int64_t n = m->hm.n, nnz = m->topo.indptr.back();
-  return 12 * nnz + 4 * (n + 1) + 4 * n + 8 * 14 * n + 4 * 8 * ((n + kTpb - 1) / kTpb) + 4096;
-}
 
 template <class T>
 static void ensure_cap(T*& p, int& cap, int need) {
@@ -1414,12 +1509,169 @@ static void ensure_cap(T*& p, int& cap, int need) {
   cap = c;
 }
 
+static int64_t solver_bytes(const gsh_mesh* m, int precision) {
+  const int64_t n = m->hm.n, nnz = m->topo.indptr.back();
+  int64_t b = 14 * nnz + 4 * (n + 1) + 4 * n + 8 * 14 * n + 4 * 8 * ((n + kTpb - 1) / kTpb) + 4096;
+  if (precision == GSH_PREC_MIXED) b += 4 * nnz + 8 * n + 4 * 7 * n + 4 * 8 * ((n + kTpb - 1) / kTpb);
+  return b;
+}
+
+// Retires a request: its solution to the host, its solver back to the mesh's pool.
+static void server_retire(gsh_server* srv, Request* r, int32_t iters) {
+  gsh_solver* sol = r->solver;
+  r->x.resize(sol->n);
+  hipStream_t st = srv->ctx->stream;
+  HIPCHK(hipMemcpyAsync(r->x.data(), sol->x, sizeof(double) * sol->n, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  r->iters = iters;
+  r->status = 0;
+  r->active = false;
+  r->latency_ms = std::chrono::duration<double, std::milli>(Clock::now() - r->t_submit).count();
+  srv->pool[r->mesh_id].push_back(sol);
+  r->solver = nullptr;
+  srv->used -= r->bytes;
+}
+
+// One step of a mixed-precision server: FP64 residual checks of the requests due one, then one
+// window of FP32 inner PCG iterations for every request with an inner solve running. A request
+// follows the same sequence of operations as gsh_solver_solve in mixed precision, so its result
+// does not depend on the other requests.
+static void server_step_mixed(gsh_server* srv, const std::vector<Request*>& fresh) {
+  gsh_ctx* ctx = srv->ctx;
+  hipStream_t st = ctx->stream;
+  const auto& cfg = srv->cfg;
+  for (Request* r : fresh) {
+    gsh_solver* s = r->solver;
+    launch_cast_values(s->val, s->val32, s->nnz, st);
+    launch_fill_f64(s->x, 0.0, s->n, st);
+    s->hs32.ntiles = s->ntiles;
+    r->phase = 0;
+    r->outer = 0;
+    r->inner_total = 0;
+    r->atol2 = -1.0;
+  }
+  // ---- FP64 residual checks
+  std::vector<Request*> chk;
+  for (Request* r : srv->active)
+    if (r->phase == 0) chk.push_back(r);
+  std::vector<Request*> done_now;
+  if (!chk.empty()) {
+    const int nc = (int)chk.size();
+    std::vector<PcgSys<double>> h64(nc);
+    std::vector<TileRef> tl;
+    for (int j = 0; j < nc; ++j) {
+      gsh_solver* s = chk[j]->solver;
+      h64[j] = s->hs;
+      h64[j].x = s->x;
+      h64[j].r = s->r64;
+      for (int t = 0; t < s->ntiles; ++t) tl.push_back({j, t});
+    }
+    ensure_cap(srv->dsys, srv->dsys_cap, nc);
+    ensure_cap(srv->dtiles, srv->dtiles_cap, (int)tl.size());
+    ensure_cap(srv->dred, srv->dred_cap, 2 * nc);
+    HIPCHK(hipMemcpyAsync(srv->dsys, h64.data(), sizeof(PcgSys<double>) * nc, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(srv->dtiles, tl.data(), sizeof(TileRef) * tl.size(), hipMemcpyHostToDevice, st));
+    launch_residual_batch(srv->dsys, srv->dtiles, (int)tl.size(), 4, st);
+    launch_reduce_systems(srv->dsys, nc, srv->dred, st);
+    std::vector<double> red(2 * nc);
+    HIPCHK(hipMemcpyAsync(red.data(), srv->dred, sizeof(double) * 2 * nc, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (int j = 0; j < nc; ++j) {
+      Request* r = chk[j];
+      const double rr = red[2 * j];
+      r->outer += 1;
+      if (r->atol2 < 0.0) {
+        const double atol = cfg.rtol * std::sqrt(red[2 * j + 1]);
+        r->atol2 = atol * atol;
+        if (atol == 0.0) {  // pcg_solver.py:1089-1095
+          launch_fill_f64(r->solver->x, 0.0, r->solver->n, st);
+          done_now.push_back(r);
+          continue;
+        }
+      }
+      if (rr <= r->atol2 || r->inner_total >= cfg.maxiter || r->outer > 30) {
+        done_now.push_back(r);
+      } else {
+        r->in_rtol = std::max(r->solver->inner_rtol, 0.5 * std::sqrt(r->atol2 / rr));
+        r->phase = 1;
+        r->inner_fresh = true;
+        r->run = 0;
+      }
+    }
+  }
+  // ---- one window of inner iterations (fresh inner solves last)
+  std::vector<Request*> inr;
+  for (Request* r : srv->active)
+    if (r->phase == 1 && !r->inner_fresh) inr.push_back(r);
+  const int n_old = (int)inr.size();
+  for (Request* r : srv->active)
+    if (r->phase == 1 && r->inner_fresh) inr.push_back(r);
+  if (!inr.empty()) {
+    const int ni = (int)inr.size();
+    std::vector<PcgSys<float>> h32(ni);
+    std::vector<TileRef> tl, ftl;
+    std::vector<double> rts;
+    for (int j = 0; j < ni; ++j) {
+      gsh_solver* s = inr[j]->solver;
+      h32[j] = s->hs32;
+      for (int t = 0; t < s->ntiles; ++t) {
+        tl.push_back({j, t});
+        if (j >= n_old) ftl.push_back({j, t});
+      }
+      if (j >= n_old) rts.push_back(inr[j]->in_rtol);
+    }
+    ensure_cap(srv->dsys32, srv->dsys32_cap, ni);
+    ensure_cap(srv->dtiles, srv->dtiles_cap, (int)tl.size());
+    ensure_cap(srv->dgather, srv->dgather_cap, ni);
+    HIPCHK(hipMemcpyAsync(srv->dsys32, h32.data(), sizeof(PcgSys<float>) * ni, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(srv->dtiles, tl.data(), sizeof(TileRef) * tl.size(), hipMemcpyHostToDevice, st));
+    if (!ftl.empty()) {
+      ensure_cap(srv->dfresh, srv->dfresh_cap, (int)ftl.size());
+      ensure_cap(srv->dred, srv->dred_cap, (int)rts.size());
+      HIPCHK(hipMemcpyAsync(srv->dfresh, ftl.data(), sizeof(TileRef) * ftl.size(), hipMemcpyHostToDevice, st));
+      HIPCHK(hipMemcpyAsync(srv->dred, rts.data(), sizeof(double) * rts.size(), hipMemcpyHostToDevice, st));
+      launch_pcg_reset<float>(srv->dsys32 + n_old, ni - n_old, cfg.rtol, cfg.maxiter, cfg.check_every, st, srv->dred);
+      launch_pcg_init<float>(srv->dsys32, srv->dfresh, (int)ftl.size(), 4, st);
+    }
+    for (int i = 0; i < cfg.check_every; ++i)
+      launch_pcg_iter<float>(srv->dsys32, srv->dtiles, (int)tl.size(), 4, st);
+    launch_pcg_check<float>(srv->dsys32, ni, st);
+    hipLaunchKernelGGL(k_gather_states<float>, dim3((ni + 63) / 64), dim3(64), 0, st, srv->dsys32, ni, srv->dgather);
+    std::vector<PcgState> sts(ni);
+    HIPCHK(hipMemcpyAsync(sts.data(), srv->dgather, sizeof(PcgState) * ni, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (int j = 0; j < ni; ++j) {
+      Request* r = inr[j];
+      r->inner_fresh = false;
+      r->run += cfg.check_every;
+      const bool done = sts[j].done >= 0, maxed = r->run >= cfg.maxiter;
+      if (!done && !maxed) continue;
+      const int it = done ? sts[j].done : cfg.maxiter;
+      launch_refine_update(r->solver->x, r->solver->hs32.x, r->solver->n, st);
+      r->inner_total += it;
+      r->phase = 0;
+      if (it == 0) done_now.push_back(r);
+    }
+  }
+  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipGetLastError());
+  if (!done_now.empty()) {
+    std::vector<Request*> keep;
+    for (Request* r : srv->active)
+      if (std::find(done_now.begin(), done_now.end(), r) == done_now.end()) keep.push_back(r);
+    for (Request* r : done_now) server_retire(srv, r, r->inner_total);
+    srv->active.swap(keep);
+  }
+}
+
+
 extern "C" {
 
 int gsh_server_create(gsh_ctx* ctx, const gsh_server_config* cfg, gsh_server** out) {
   GSH_API_BODY({
     GSH_REQUIRE(ctx && cfg && out, GSH_E_ARG, "NULL argument");
-    GSH_REQUIRE(cfg->precision == GSH_PREC_FP64, GSH_E_ARG, "the server runs FP64 solves");
+    GSH_REQUIRE(cfg->precision == GSH_PREC_FP64 || cfg->precision == GSH_PREC_MIXED, GSH_E_ARG,
+                "unknown precision");
     GSH_REQUIRE(cfg->check_every >= 1 && cfg->maxiter >= 1 && cfg->maxiter % cfg->check_every == 0, GSH_E_ARG,
                 "maxiter must be a positive multiple of check_every");
     GSH_REQUIRE(cfg->rtol >= 0.0, GSH_E_ARG, "rtol must be >= 0");
@@ -1465,7 +1717,7 @@ int gsh_server_submit(gsh_server* srv, int32_t mesh_id, const double* b, int wit
       r->b.assign(b, b + srv->meshes[mesh_id]->hm.n);
     }
     r->t_submit = Clock::now();
-    r->bytes = solver_bytes(srv->meshes[mesh_id]);
+    r->bytes = solver_bytes(srv->meshes[mesh_id], srv->cfg.precision);
     *ticket = r->ticket;
     srv->queue.push_back(r.get());
     srv->reqs[r->ticket] = std::move(r);
@@ -1500,7 +1752,7 @@ int gsh_server_step(gsh_server* srv, int32_t* active_out, int32_t* queued_out) {
         s = idle.back();
         idle.pop_back();
       } else {
-        if (gsh_solver_create(m, GSH_PREC_FP64, GSH_PRECOND_JACOBI, &s) != GSH_OK)
+        if (gsh_solver_create(m, cfg.precision, GSH_PRECOND_JACOBI, &s) != GSH_OK)
           throw Error(GSH_E_OOM, std::string("server admission: ") + g_last_error);
       }
       for (auto* hh : s->h) launch_fill_f64(hh, 0.0, s->n, st);  // each request is a cold start
@@ -1531,7 +1783,8 @@ int gsh_server_step(gsh_server* srv, int32_t* active_out, int32_t* queued_out) {
       fresh.push_back(r);
     }
     for (auto* r : fresh) srv->active.push_back(r);
-    const int na = (int)srv->active.size();
+    const int na = cfg.precision == GSH_PREC_MIXED ? 0 : (int)srv->active.size();
+    if (cfg.precision == GSH_PREC_MIXED) server_step_mixed(srv, fresh);
     if (na > 0) {
       // ---- descriptors and tile tables (fresh systems are the last ones)
       std::vector<PcgSys<double>> hs(na);
@@ -1578,19 +1831,8 @@ int gsh_server_step(gsh_server* srv, int32_t* active_out, int32_t* queued_out) {
           keep.push_back(r);
           continue;
         }
-        gsh_solver* sol = r->solver;
-        r->x.resize(sol->n);
-        if (s.zero)
-          std::fill(r->x.begin(), r->x.end(), 0.0);
-        else
-          HIPCHK(hipMemcpy(r->x.data(), sol->x, sizeof(double) * sol->n, hipMemcpyDeviceToHost));
-        r->iters = done ? s.done : cfg.maxiter;
-        r->status = 0;
-        r->active = false;
-        r->latency_ms = std::chrono::duration<double, std::milli>(Clock::now() - r->t_submit).count();
-        srv->pool[r->mesh_id].push_back(sol);
-        r->solver = nullptr;
-        srv->used -= r->bytes;
+        if (s.zero) launch_fill_f64(r->solver->x, 0.0, r->solver->n, st);
+        server_retire(srv, r, done ? s.done : cfg.maxiter);
       }
       srv->active.swap(keep);
     }

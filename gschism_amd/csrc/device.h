@@ -112,19 +112,28 @@ struct PcgSys {
 };
 
 // Multicolour SSOR system: rows in colour-major order, each colour block padded to whole
-// tiles of 256*R rows with identity rows; the matrix split into the strictly lower part
-// (columns of lower colours) and the strictly upper part, both with ascending columns.
+// tiles of 256*R rows with identity rows; the off-diagonal part split into the strictly lower
+// part (columns of lower colours) and the strictly upper part, stored ELL per colour (entry m of
+// colour-offset row r at off + m * rows + r, column -1 where a row has fewer entries).
+struct SsorColour {
+  int32_t start, rows, wl, wu;   // first row, padded row count, ELL widths
+  int32_t offl, offu, pad0, pad1;  // ELL offsets of the lower / upper part
+};
+
 struct SsorSys {
   int32_t n, np, ntiles, ncol;
   double omega;
-  const int32_t *lptr, *lcol, *uptr, *ucol;  // [np + 1], [nl] / [nu]
-  const int32_t *lsrc, *usrc, *dsrc;         // slots of each entry in the solver's CSR values
+  int32_t unit_omega, pad;                   // omega == 1: 2 D~ - D equals D~ exactly
+  const SsorColour* colours;                 // [ncol]
+  const int32_t* tile_colour;                // [ntiles]
+  const int32_t *lcol, *ucol;                // ELL columns (-1 unused)
+  const int32_t *lsrc, *usrc, *dsrc;         // slots of each entry in the solver's CSR values (-1 unused)
   const int32_t* perm;                       // [np] new row -> original row, -1 for padding
   const double* aval;                        // the solver's CSR values (original order)
   const double* b;                           // right-hand side (original order)
   const double *h0, *h1, *h2;                // extrapolation history (original order), NULL = zero guess
   double* xout;                              // solution (original order)
-  double *lval, *uval;                       // gathered off-diagonal values
+  double *lval, *uval;                       // gathered ELL values (0 where unused)
   double *d, *dt, *kd, *bp;                  // diagonal, D/omega, 2 D/omega - D, permuted b
   double *y, *r, *p, *t, *s, *q, *w;         // iteration vectors [np]
   double *part_rr, *part_rz, *part_pq, *part_bb;
@@ -145,13 +154,15 @@ void launch_assemble(const DMesh& m, const StepArgs& a, double* val, double* b, 
 void launch_update_elevation(const DMesh& m, const double* x, hipStream_t s);
 
 template <class VT>
-void launch_pcg_reset(const PcgSys<VT>* sys, int nsys, double rtol, int maxiter, int check_every, hipStream_t s);
+void launch_pcg_reset(const PcgSys<VT>* sys, int nsys, double rtol, int maxiter, int check_every, hipStream_t s,
+                      const double* rtols = nullptr);
 template <class VT>
 void launch_pcg_init(const PcgSys<VT>* sys, const TileRef* tiles, int ntiles, int rpt, hipStream_t s);
 template <class VT>
 void launch_pcg_iter(const PcgSys<VT>* sys, const TileRef* tiles, int ntiles, int rpt, hipStream_t s);
 template <class VT>
-void launch_pcg_phase(int which, const PcgSys<VT>* sys, const TileRef* tiles, int ntiles, int rpt, hipStream_t s);
+void launch_pcg_phase(int which, const PcgSys<VT>* sys, const TileRef* tiles, int ntiles, int rpt, hipStream_t s,
+                      hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 template <class VT>
 void launch_pcg_check(const PcgSys<VT>* sys, int nsys, hipStream_t s);
 template <class VT>
@@ -165,19 +176,26 @@ void launch_reduce_pair(const double* pa, const double* pb, int ntiles, double* 
 void launch_ssor_reset(const SsorSys* sys, double rtol, int maxiter, int check_every, hipStream_t s);
 void launch_ssor_setup(const SsorSys* sys, int ntiles, int rpt, hipStream_t s);
 void launch_ssor_residual(const SsorSys* sys, int ntiles, int rpt, hipStream_t s);
-void launch_ssor_r0(const SsorSys* sys, int tile0, int ntc, int rpt, hipStream_t s);
-void launch_ssor_bwd(const SsorSys* sys, int tile0, int ntc, int first, int mid, int rpt, hipStream_t s);
-void launch_ssor_fwd(const SsorSys* sys, int tile0, int ntc, int rpt, hipStream_t s);
-void launch_ssor_update(const SsorSys* sys, int ntiles, int rpt, hipStream_t s);
+void launch_ssor_r0(const SsorSys* sys, int tile0, int ntc, int c, int rpt, hipStream_t s);
+void launch_ssor_bwd(const SsorSys* sys, int tile0, int ntc, int c, int first, int mid, int rpt, hipStream_t s,
+                     hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
+void launch_ssor_fwd(const SsorSys* sys, int tile0, int ntc, int c, int rpt, hipStream_t s, hipEvent_t e0 = nullptr,
+                     hipEvent_t e1 = nullptr);
+void launch_ssor_update(const SsorSys* sys, int ntiles, int rpt, hipStream_t s, hipEvent_t e0 = nullptr,
+                        hipEvent_t e1 = nullptr);
 void launch_ssor_true_rr(const SsorSys* sys, int ntiles, int rpt, hipStream_t s);
 void launch_ssor_check(const SsorSys* sys, hipStream_t s);
-void launch_ssor_xsolve(const SsorSys* sys, int tile0, int ntc, int rpt, hipStream_t s);
+void launch_ssor_xsolve(const SsorSys* sys, int tile0, int ntc, int c, int rpt, hipStream_t s);
 void launch_ssor_scatter(const SsorSys* sys, int ntiles, int rpt, hipStream_t s);
 
 // Mixed precision helpers.
 void launch_cast_values(const double* in, float* out, int64_t count, hipStream_t s);
-void launch_refine_residual(const PcgSys<double>* sys64, const TileRef* tiles, int ntiles, int rpt, double* r64,
-                            hipStream_t s);
+// Batched mixed-precision steps over a launch table: r = b - A x (FP64, into sys.r) with r.r / b.b
+// partials; per-system totals of those partials; x64 += d32.
+void launch_residual_batch(const PcgSys<double>* sys, const TileRef* tiles, int ntiles, int rpt, hipStream_t s);
+void launch_reduce_systems(const PcgSys<double>* sys, int nsys, double* out, hipStream_t s);
+void launch_refine_batch(const PcgSys<double>* sys64, const PcgSys<float>* sys32, const TileRef* tiles, int ntiles,
+                         int rpt, hipStream_t s);
 void launch_refine_update(double* x64, const float* d32, int32_t n, hipStream_t s);
 void launch_copy_f64(double* dst, const double* src, int64_t count, hipStream_t s);
 void launch_fill_f64(double* dst, double v, int64_t count, hipStream_t s);

@@ -20,6 +20,7 @@
 // A workgroup finds its system and tile through a launch table, so one launch advances any
 // number of independent systems (batched and served solves); finished systems drop out
 // through their device-side `done` flag.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "device.h"
@@ -73,10 +74,13 @@ __device__ __forceinline__ SysView<VT> view(const PcgSys<VT>* __restrict__ sys, 
 }
 
 // ------------------------------------------------------------------ state reset
+// rtols (device, may be NULL): per-system relative tolerances instead of rtol.
 template <class VT>
-__global__ void k_pcg_reset(const PcgSys<VT>* sys, int nsys, double rtol, int maxiter, int check_every) {
+__global__ void k_pcg_reset(const PcgSys<VT>* sys, int nsys, double rtol, int maxiter, int check_every,
+                            const double* __restrict__ rtols) {
   const int sidx = blockIdx.x * blockDim.x + threadIdx.x;
   if (sidx >= nsys) return;
+  if (rtols) rtol = rtols[sidx];
   gptr<PcgState> st = G(sys[sidx].st);
   st->k = 0;
   st->kk = 0;
@@ -206,12 +210,6 @@ __device__ __forceinline__ void load_chunk(const SysView<VT>& S, gptr<const VT> 
       c.pc[m] = po[col];
     }
   }
-}
-
-__device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 template <class VT, int R>
@@ -434,6 +432,69 @@ __global__ void __launch_bounds__(256) k_true_residual(const PcgSys<double>* __r
   }
 }
 
+// Batched r = b - A x (x = S.x, r written to S.r) over a launch table; partials of r.r and b.b.
+template <int R>
+__global__ void __launch_bounds__(256) k_residual_batch(const PcgSys<double>* __restrict__ sys,
+                                                        const TileRef* __restrict__ tiles) {
+  __shared__ double red[4];
+  __shared__ double lds[kSpmvLdsCap];
+  const TileRef tr = tiles[blockIdx.x];
+  const SysView<double> S = view(sys, tr.sys);
+  const int n = S.n;
+  const gptr<const double> xv = S.x;
+  auto xf = [&](int c) -> double { return xv[c]; };
+  double rr = 0.0, bb = 0.0;
+  const int base = tr.tile * kTpb * R;
+  for (int j = 0; j < R; ++j) {
+    const int r0 = base + j * kTpb;
+    if (r0 >= n) break;
+    const double q = csr_chunk<double>(S.rowptr, S.col, S.val, n, r0, lds, xf);
+    const int i = r0 + threadIdx.x;
+    if (i < n) {
+      const double bi = S.b[i];
+      const double ri = bi - q;
+      S.r[i] = ri;
+      rr += ri * ri;
+      bb += bi * bi;
+    }
+  }
+  rr = block_tree(rr, red);
+  bb = block_tree(bb, red);
+  if (threadIdx.x == 0) {
+    S.part_rr[tr.tile] = rr;
+    S.part_bb[tr.tile] = bb;
+  }
+}
+
+// Per-system totals of the r.r and b.b partials: out[2 i], out[2 i + 1] (one workgroup per system).
+__global__ void __launch_bounds__(256) k_reduce_systems(const PcgSys<double>* __restrict__ sys,
+                                                        double* __restrict__ out) {
+  __shared__ double red[4];
+  const PcgSys<double>& S = sys[blockIdx.x];
+  const double a = final_reduce(G((const double*)S.part_rr), S.ntiles, red);
+  const double b = final_reduce(G((const double*)S.part_bb), S.ntiles, red);
+  if (threadIdx.x == 0) {
+    out[2 * blockIdx.x] = a;
+    out[2 * blockIdx.x + 1] = b;
+  }
+}
+
+// x64 += d32 for every system of a launch table (sys64[i].x, sys32[i].x).
+template <int R>
+__global__ void __launch_bounds__(256) k_refine_batch(const PcgSys<double>* __restrict__ sys64,
+                                                      const PcgSys<float>* __restrict__ sys32,
+                                                      const TileRef* __restrict__ tiles) {
+  const TileRef tr = tiles[blockIdx.x];
+  const gptr<double> x = G(sys64[tr.sys].x);
+  const gptr<const float> d = G((const float*)sys32[tr.sys].x);
+  const int n = sys64[tr.sys].n;
+  const int base = tr.tile * kTpb * R;
+  for (int j = 0; j < R; ++j) {
+    const int i = base + j * kTpb + threadIdx.x;
+    if (i < n) x[i] = x[i] + (double)d[i];
+  }
+}
+
 __global__ void k_reduce_pair(const double* __restrict__ pa, const double* __restrict__ pb, int T,
                               double* __restrict__ out) {
   __shared__ double red[4];
@@ -476,9 +537,11 @@ __global__ void k_fill_f64(double* __restrict__ dst, double v, int64_t count) {
   }
 
 template <class VT>
-void launch_pcg_reset(const PcgSys<VT>* sys, int nsys, double rtol, int maxiter, int check_every, hipStream_t s) {
+void launch_pcg_reset(const PcgSys<VT>* sys, int nsys, double rtol, int maxiter, int check_every, hipStream_t s,
+                      const double* rtols) {
   if (nsys <= 0) return;
-  hipLaunchKernelGGL(k_pcg_reset<VT>, dim3((nsys + 63) / 64), dim3(64), 0, s, sys, nsys, rtol, maxiter, check_every);
+  hipLaunchKernelGGL(k_pcg_reset<VT>, dim3((nsys + 63) / 64), dim3(64), 0, s, sys, nsys, rtol, maxiter, check_every,
+                     rtols);
 }
 
 template <class VT>
@@ -494,13 +557,23 @@ void launch_pcg_iter(const PcgSys<VT>* sys, const TileRef* tiles, int ntiles, in
   GSH_DISPATCH_R(rpt, k_pcg_phase_b, VT, dim3(ntiles), dim3(kTpb), 0, s, sys, tiles);
 }
 
+// Launch with kernel start/end stamps in e0/e1 (hipExtLaunchKernelGGL), for profiling.
+#define GSH_DISPATCH_RX(rpt, KER, VT, E0, E1, GRID, STREAM, ...)                                                \
+  switch (rpt) {                                                                                             \
+    case 1: hipExtLaunchKernelGGL(KER<VT, 1>, dim3(GRID), dim3(kTpb), 0, STREAM, E0, E1, 0, __VA_ARGS__); break;  \
+    case 2: hipExtLaunchKernelGGL(KER<VT, 2>, dim3(GRID), dim3(kTpb), 0, STREAM, E0, E1, 0, __VA_ARGS__); break;  \
+    case 8: hipExtLaunchKernelGGL(KER<VT, 8>, dim3(GRID), dim3(kTpb), 0, STREAM, E0, E1, 0, __VA_ARGS__); break;  \
+    default: hipExtLaunchKernelGGL(KER<VT, 4>, dim3(GRID), dim3(kTpb), 0, STREAM, E0, E1, 0, __VA_ARGS__); break; \
+  }
+
 template <class VT>
-void launch_pcg_phase(int which, const PcgSys<VT>* sys, const TileRef* tiles, int ntiles, int rpt, hipStream_t s) {
+void launch_pcg_phase(int which, const PcgSys<VT>* sys, const TileRef* tiles, int ntiles, int rpt, hipStream_t s,
+                      hipEvent_t e0, hipEvent_t e1) {
   if (ntiles <= 0) return;
   if (which == 0) {
-    GSH_DISPATCH_R(rpt, k_pcg_phase_a, VT, dim3(ntiles), dim3(kTpb), 0, s, sys, tiles);
+    GSH_DISPATCH_RX(rpt, k_pcg_phase_a, VT, e0, e1, ntiles, s, sys, tiles);
   } else {
-    GSH_DISPATCH_R(rpt, k_pcg_phase_b, VT, dim3(ntiles), dim3(kTpb), 0, s, sys, tiles);
+    GSH_DISPATCH_RX(rpt, k_pcg_phase_b, VT, e0, e1, ntiles, s, sys, tiles);
   }
 }
 
@@ -516,14 +589,16 @@ void launch_pcg_finish(const PcgSys<VT>* sys, const TileRef* tiles, int ntiles, 
   GSH_DISPATCH_R(rpt, k_pcg_finish, VT, dim3(ntiles), dim3(kTpb), 0, s, sys, tiles);
 }
 
-template void launch_pcg_reset<double>(const PcgSys<double>*, int, double, int, int, hipStream_t);
-template void launch_pcg_reset<float>(const PcgSys<float>*, int, double, int, int, hipStream_t);
+template void launch_pcg_reset<double>(const PcgSys<double>*, int, double, int, int, hipStream_t, const double*);
+template void launch_pcg_reset<float>(const PcgSys<float>*, int, double, int, int, hipStream_t, const double*);
 template void launch_pcg_init<double>(const PcgSys<double>*, const TileRef*, int, int, hipStream_t);
 template void launch_pcg_init<float>(const PcgSys<float>*, const TileRef*, int, int, hipStream_t);
 template void launch_pcg_iter<double>(const PcgSys<double>*, const TileRef*, int, int, hipStream_t);
 template void launch_pcg_iter<float>(const PcgSys<float>*, const TileRef*, int, int, hipStream_t);
-template void launch_pcg_phase<double>(int, const PcgSys<double>*, const TileRef*, int, int, hipStream_t);
-template void launch_pcg_phase<float>(int, const PcgSys<float>*, const TileRef*, int, int, hipStream_t);
+template void launch_pcg_phase<double>(int, const PcgSys<double>*, const TileRef*, int, int, hipStream_t, hipEvent_t,
+                                       hipEvent_t);
+template void launch_pcg_phase<float>(int, const PcgSys<float>*, const TileRef*, int, int, hipStream_t, hipEvent_t,
+                                      hipEvent_t);
 template void launch_pcg_check<double>(const PcgSys<double>*, int, hipStream_t);
 template void launch_pcg_check<float>(const PcgSys<float>*, int, hipStream_t);
 template void launch_pcg_finish<double>(const PcgSys<double>*, const TileRef*, int, int, hipStream_t);
@@ -537,6 +612,31 @@ void launch_true_residual(const PcgSys<double>* sys, const TileRef* tiles, int n
     case 2: hipLaunchKernelGGL(k_true_residual<2>, dim3(ntiles), dim3(kTpb), 0, s, sys, tiles, xv, rvec); break;
     case 8: hipLaunchKernelGGL(k_true_residual<8>, dim3(ntiles), dim3(kTpb), 0, s, sys, tiles, xv, rvec); break;
     default: hipLaunchKernelGGL(k_true_residual<4>, dim3(ntiles), dim3(kTpb), 0, s, sys, tiles, xv, rvec); break;
+  }
+}
+
+void launch_residual_batch(const PcgSys<double>* sys, const TileRef* tiles, int ntiles, int rpt, hipStream_t s) {
+  if (ntiles <= 0) return;
+  switch (rpt) {
+    case 1: hipLaunchKernelGGL(k_residual_batch<1>, dim3(ntiles), dim3(kTpb), 0, s, sys, tiles); break;
+    case 2: hipLaunchKernelGGL(k_residual_batch<2>, dim3(ntiles), dim3(kTpb), 0, s, sys, tiles); break;
+    case 8: hipLaunchKernelGGL(k_residual_batch<8>, dim3(ntiles), dim3(kTpb), 0, s, sys, tiles); break;
+    default: hipLaunchKernelGGL(k_residual_batch<4>, dim3(ntiles), dim3(kTpb), 0, s, sys, tiles); break;
+  }
+}
+
+void launch_reduce_systems(const PcgSys<double>* sys, int nsys, double* out, hipStream_t s) {
+  if (nsys > 0) hipLaunchKernelGGL(k_reduce_systems, dim3(nsys), dim3(kTpb), 0, s, sys, out);
+}
+
+void launch_refine_batch(const PcgSys<double>* sys64, const PcgSys<float>* sys32, const TileRef* tiles, int ntiles,
+                         int rpt, hipStream_t s) {
+  if (ntiles <= 0) return;
+  switch (rpt) {
+    case 1: hipLaunchKernelGGL(k_refine_batch<1>, dim3(ntiles), dim3(kTpb), 0, s, sys64, sys32, tiles); break;
+    case 2: hipLaunchKernelGGL(k_refine_batch<2>, dim3(ntiles), dim3(kTpb), 0, s, sys64, sys32, tiles); break;
+    case 8: hipLaunchKernelGGL(k_refine_batch<8>, dim3(ntiles), dim3(kTpb), 0, s, sys64, sys32, tiles); break;
+    default: hipLaunchKernelGGL(k_refine_batch<4>, dim3(ntiles), dim3(kTpb), 0, s, sys64, sys32, tiles); break;
   }
 }
 

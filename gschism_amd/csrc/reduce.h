@@ -44,6 +44,13 @@ __device__ __forceinline__ double block_tree(double v, double* red) {
   return t;
 }
 
+// Orders LDS writes before reads among the lanes of one wave.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Sum of the tile partials P[0..T): thread t sums P[t], P[t+256], ... then block_tree.
 __device__ __forceinline__ double final_reduce(gptr<const double> P, int T, double* red) {
   double u = 0.0;

@@ -19,7 +19,10 @@
 //   update                 alpha; y += alpha p; r -= alpha q; r.(D~ r) partials
 // Colour blocks are padded to whole reduction tiles, so each tile's partial is written by
 // exactly one launch and every dot product uses the fixed tile tree of the Jacobi solver.
-// Row sums run over ascending columns from 0.0, in the oracle's order.
+// Row sums run over ascending columns from 0.0, in the oracle's order. The two triangular parts
+// are ELL per colour: a wave reads entry m of its 64 rows with one coalesced load, so a row sum
+// is two dependent memory steps (columns, then values and gathers together).
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include "device.h"
@@ -30,12 +33,13 @@ namespace gsh {
 struct SsorView {
   int np, ntiles;
   double omega;
-  gptr<const int32_t> lptr, lcol, uptr, ucol, lsrc, usrc, dsrc, perm;
+  gptr<const SsorColour> colours;
+  gptr<const int32_t> tile_colour, lcol, ucol, lsrc, usrc, dsrc, perm;
   gptr<const double> aval, b, h0, h1, h2;
   gptr<double> xout, lval, uval, d, dt, kd, bp, y, r, p, t, s, q, w;
   gptr<double> part_rr, part_rz, part_pq, part_bb;
   gptr<PcgState> st;
-  bool has_hist;
+  bool has_hist, unit_omega;
 };
 
 __device__ __forceinline__ SsorView sview(const SsorSys* __restrict__ sys) {
@@ -44,9 +48,10 @@ __device__ __forceinline__ SsorView sview(const SsorSys* __restrict__ sys) {
   v.np = S.np;
   v.ntiles = S.ntiles;
   v.omega = S.omega;
-  v.lptr = G(S.lptr);
+  v.unit_omega = S.unit_omega != 0;
+  v.colours = G(S.colours);
+  v.tile_colour = G(S.tile_colour);
   v.lcol = G(S.lcol);
-  v.uptr = G(S.uptr);
   v.ucol = G(S.ucol);
   v.lsrc = G(S.lsrc);
   v.usrc = G(S.usrc);
@@ -80,21 +85,6 @@ __device__ __forceinline__ SsorView sview(const SsorSys* __restrict__ sys) {
   return v;
 }
 
-// sum over the strictly lower / upper part of row v of a_vj x_j, ascending j, from 0.0
-__device__ __forceinline__ double lower_sum(const SsorView& S, int v, gptr<const double> x) {
-  double acc = 0.0;
-  const int a = S.lptr[v], e = S.lptr[v + 1];
-  for (int k = a; k < e; ++k) acc += S.lval[k] * x[S.lcol[k]];
-  return acc;
-}
-
-__device__ __forceinline__ double upper_sum(const SsorView& S, int v, gptr<const double> x) {
-  double acc = 0.0;
-  const int a = S.uptr[v], e = S.uptr[v + 1];
-  for (int k = a; k < e; ++k) acc += S.uval[k] * x[S.ucol[k]];
-  return acc;
-}
-
 template <class T>
 __device__ __forceinline__ gptr<const T> C(gptr<T> p) {
   return (gptr<const T>)p;
@@ -106,14 +96,57 @@ __device__ __forceinline__ int tile_row(int tile, int j) {
   return tile * kTpb * R + j * kTpb + threadIdx.x;
 }
 
+struct ColourInfo {
+  int start, rows, wl, wu, offl, offu;
+};
+
+__device__ __forceinline__ ColourInfo colour_info(const SsorView& S, int c) {
+  const gptr<const int32_t> k = (gptr<const int32_t>)(S.colours + c);
+  return {k[0], k[1], k[2], k[3], k[4], k[5]};
+}
+
+// sum over one ELL row part of a_vj x_j in entry (= ascending column) order, from 0.0; the
+// columns of 8 entries are loaded together, then their values and gathers together
+constexpr int kEllChunk = 8;
+
+__device__ __forceinline__ double ell_sum(gptr<const int32_t> ec, gptr<const double> ev, gptr<const double> x,
+                                          int base, int stride, int width) {
+  double acc = 0.0;
+  for (int m0 = 0; m0 < width; m0 += kEllChunk) {
+    int c[kEllChunk];
+    double vv[kEllChunk], xx[kEllChunk];
+#pragma unroll
+    for (int m = 0; m < kEllChunk; ++m) c[m] = (m0 + m < width) ? ec[base + (m0 + m) * stride] : -1;
+#pragma unroll
+    for (int m = 0; m < kEllChunk; ++m)
+      if (c[m] >= 0) {
+        vv[m] = ev[base + (m0 + m) * stride];
+        xx[m] = x[c[m]];
+      }
+#pragma unroll
+    for (int m = 0; m < kEllChunk; ++m)
+      if (c[m] >= 0) acc += vv[m] * xx[m];
+  }
+  return acc;
+}
+
+__device__ __forceinline__ double lower_sum(const SsorView& S, const ColourInfo& ci, int v, gptr<const double> x) {
+  return ell_sum(S.lcol, C(S.lval), x, ci.offl + (v - ci.start), ci.rows, ci.wl);
+}
+
+__device__ __forceinline__ double upper_sum(const SsorView& S, const ColourInfo& ci, int v, gptr<const double> x) {
+  return ell_sum(S.ucol, C(S.uval), x, ci.offu + (v - ci.start), ci.rows, ci.wu);
+}
+
 // ------------------------------------------------------------------ setup
-// Gathers the split matrix and the diagonal from the CSR values, b and the extrapolated guess
+// Gathers the ELL values and the diagonal from the CSR values, b and the extrapolated guess
 // x0 = 3 x0 - 3 x1 + x2 (pcg_solver.py:881-886) into the colour-major order; b.b partials.
 template <int R>
 __global__ void __launch_bounds__(256) k_ssor_setup(const SsorSys* __restrict__ sys) {
   __shared__ double red[4];
   const SsorView S = sview(sys);
   const int tile = blockIdx.x;
+  const ColourInfo ci = colour_info(S, S.tile_colour[tile]);
   double bb = 0.0;
   for (int j = 0; j < R; ++j) {
     const int v = tile_row<R>(tile, j);
@@ -123,8 +156,17 @@ __global__ void __launch_bounds__(256) k_ssor_setup(const SsorSys* __restrict__ 
     S.d[v] = dv;
     S.dt[v] = dtv;
     S.kd[v] = 2.0 * dtv - dv;
-    for (int k = S.lptr[v]; k < S.lptr[v + 1]; ++k) S.lval[k] = S.aval[S.lsrc[k]];
-    for (int k = S.uptr[v]; k < S.uptr[v + 1]; ++k) S.uval[k] = S.aval[S.usrc[k]];
+    const int ro = v - ci.start;
+    for (int m = 0; m < ci.wl; ++m) {
+      const int k = ci.offl + m * ci.rows + ro;
+      const int src = S.lsrc[k];
+      S.lval[k] = src >= 0 ? S.aval[src] : 0.0;
+    }
+    for (int m = 0; m < ci.wu; ++m) {
+      const int k = ci.offu + m * ci.rows + ro;
+      const int src = S.usrc[k];
+      S.uval[k] = src >= 0 ? S.aval[src] : 0.0;
+    }
     const int i = S.perm[v];
     const double bv = i >= 0 ? S.b[i] : 0.0;
     S.bp[v] = bv;
@@ -139,12 +181,13 @@ __global__ void __launch_bounds__(256) k_ssor_setup(const SsorSys* __restrict__ 
 template <int R>
 __global__ void __launch_bounds__(256) k_ssor_residual(const SsorSys* __restrict__ sys) {
   const SsorView S = sview(sys);
+  const ColourInfo ci = colour_info(S, S.tile_colour[blockIdx.x]);
   const gptr<const double> x0 = C(S.w);
   for (int j = 0; j < R; ++j) {
     const int v = tile_row<R>(blockIdx.x, j);
     const double xv = x0[v];
-    const double au = upper_sum(S, v, x0);
-    const double ax = (lower_sum(S, v, x0) + S.d[v] * xv) + au;
+    const double au = upper_sum(S, ci, v, x0);
+    const double ax = (lower_sum(S, ci, v, x0) + S.d[v] * xv) + au;
     S.t[v] = S.bp[v] - ax;
     S.y[v] = S.dt[v] * xv + au;
     S.p[v] = 0.0;
@@ -153,16 +196,17 @@ __global__ void __launch_bounds__(256) k_ssor_residual(const SsorSys* __restrict
 
 // r0 = (D~ + L)^-1 r_true for the rows of one colour; partials of r.(D~ r).
 template <int R>
-__global__ void __launch_bounds__(256) k_ssor_r0(const SsorSys* __restrict__ sys, int tile0) {
+__global__ void __launch_bounds__(256) k_ssor_r0(const SsorSys* __restrict__ sys, int tile0, int c) {
   __shared__ double red[4];
   const SsorView S = sview(sys);
+  const ColourInfo ci = colour_info(S, c);
   const int tile = tile0 + blockIdx.x;
   const gptr<const double> r = C(S.r);
   double rz = 0.0;
   for (int j = 0; j < R; ++j) {
     const int v = tile_row<R>(tile, j);
     const double dtv = S.dt[v];
-    const double rv = (S.t[v] - lower_sum(S, v, r)) / dtv;
+    const double rv = (S.t[v] - lower_sum(S, ci, v, r)) / dtv;
     S.r[v] = rv;
     const double zv = dtv * rv;
     rz += rv * zv;
@@ -175,14 +219,27 @@ __global__ void __launch_bounds__(256) k_ssor_r0(const SsorSys* __restrict__ sys
 // t = (D~ + U)^-1 p for one colour, with p = z + beta p (f_update_p, pcg_solver.py:208-235)
 // formed on the way. The first sweep of an iteration reduces r.z, forms beta and, in its
 // workgroup 0, keeps the scalar state: atol at k = 0 (pcg_solver.py:928-930, 1089-1095),
-// rho, kk and beta for the later sweeps. The sweep of colour 0 also starts the forward sweep.
+// rho, kk and beta for the later sweeps. The sweep of colour 0 (no lower colours) also does
+// the forward step of its rows: s = (p - K t) / D~, q = t + s, p.q partials.
 template <int R>
-__global__ void __launch_bounds__(256) k_ssor_bwd(const SsorSys* __restrict__ sys, int tile0, int first, int mid) {
+__global__ void __launch_bounds__(256) k_ssor_bwd(const SsorSys* __restrict__ sys, int tile0, int c, int first,
+                                                  int mid) {
   __shared__ double red[4];
   const SsorView S = sview(sys);
   gptr<PcgState> st = S.st;
   if (load_done(st) >= 0) return;
+  const ColourInfo ci = colour_info(S, c);
   const int tile = tile0 + blockIdx.x;
+  const gptr<const double> t = C(S.t);
+  double usum[R], dtv[R], rv[R], pv[R];
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    const int v = tile_row<R>(tile, j);
+    dtv[j] = S.dt[v];
+    rv[j] = S.r[v];
+    pv[j] = S.p[v];
+    usum[j] = upper_sum(S, ci, v, t);
+  }
   double beta;
   if (first) {
     const int k = st->k;
@@ -214,21 +271,21 @@ __global__ void __launch_bounds__(256) k_ssor_bwd(const SsorSys* __restrict__ sy
   } else {
     beta = st->beta;
   }
-  const gptr<const double> t = C(S.t);
   double pq = 0.0;
+#pragma unroll
   for (int j = 0; j < R; ++j) {
     const int v = tile_row<R>(tile, j);
-    const double dtv = S.dt[v];
-    const double pv = dtv * S.r[v] + beta * S.p[v];
-    S.p[v] = pv;
-    const double tv = (pv - upper_sum(S, v, t)) / dtv;
+    const double pn = dtv[j] * rv[j] + beta * pv[j];
+    const double tv = (pn - usum[j]) / dtv[j];
+    S.p[v] = pn;
     S.t[v] = tv;
     if (mid) {
-      const double sv = ((pv - S.kd[v] * tv) - lower_sum(S, v, C(S.s))) / dtv;
+      const double kdv = S.unit_omega ? dtv[j] : S.kd[v];
+      const double sv = ((pn - kdv * tv) - 0.0) / dtv[j];
       S.s[v] = sv;
       const double qv = tv + sv;
       S.q[v] = qv;
-      pq += pv * qv;
+      pq += pn * qv;
     }
   }
   if (mid) {
@@ -240,21 +297,32 @@ __global__ void __launch_bounds__(256) k_ssor_bwd(const SsorSys* __restrict__ sy
 // ------------------------------------------------------------------ forward sweep
 // s = (D~ + L)^-1 (p - K t) for one colour, q = t + s = A^ p, partials of p.q (f_dot).
 template <int R>
-__global__ void __launch_bounds__(256) k_ssor_fwd(const SsorSys* __restrict__ sys, int tile0) {
+__global__ void __launch_bounds__(256) k_ssor_fwd(const SsorSys* __restrict__ sys, int tile0, int c) {
   __shared__ double red[4];
   const SsorView S = sview(sys);
   if (load_done(S.st) >= 0) return;
+  const ColourInfo ci = colour_info(S, c);
   const int tile = tile0 + blockIdx.x;
   const gptr<const double> s = C(S.s);
-  double pq = 0.0;
+  double lsum[R], dtv[R], tv[R], pv[R], kdv[R];
+#pragma unroll
   for (int j = 0; j < R; ++j) {
     const int v = tile_row<R>(tile, j);
-    const double pv = S.p[v], tv = S.t[v];
-    const double sv = ((pv - S.kd[v] * tv) - lower_sum(S, v, s)) / S.dt[v];
+    dtv[j] = S.dt[v];
+    tv[j] = S.t[v];
+    pv[j] = S.p[v];
+    kdv[j] = S.unit_omega ? dtv[j] : S.kd[v];
+    lsum[j] = lower_sum(S, ci, v, s);
+  }
+  double pq = 0.0;
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    const int v = tile_row<R>(tile, j);
+    const double sv = ((pv[j] - kdv[j] * tv[j]) - lsum[j]) / dtv[j];
     S.s[v] = sv;
-    const double qv = tv + sv;
+    const double qv = tv[j] + sv;
     S.q[v] = qv;
-    pq += pv * qv;
+    pq += pv[j] * qv;
   }
   pq = block_tree(pq, red);
   if (threadIdx.x == 0) S.part_pq[tile] = pq;
@@ -306,11 +374,12 @@ __global__ void __launch_bounds__(256) k_ssor_true_rr(const SsorSys* __restrict_
   __shared__ double red[4];
   const SsorView S = sview(sys);
   if (load_done(S.st) >= 0) return;
+  const ColourInfo ci = colour_info(S, S.tile_colour[blockIdx.x]);
   const gptr<const double> r = C(S.r);
   double rr = 0.0;
   for (int j = 0; j < R; ++j) {
     const int v = tile_row<R>(blockIdx.x, j);
-    const double wv = S.dt[v] * r[v] + lower_sum(S, v, r);
+    const double wv = S.dt[v] * r[v] + lower_sum(S, ci, v, r);
     rr += wv * wv;
   }
   rr = block_tree(rr, red);
@@ -337,14 +406,15 @@ __global__ void __launch_bounds__(256) k_ssor_check(const SsorSys* __restrict__ 
 // x = (D~ + U)^-1 y for one colour (into w), then scattered to the original order; x = 0 when
 // atol == 0 (pcg_solver.py:1089-1095).
 template <int R>
-__global__ void __launch_bounds__(256) k_ssor_xsolve(const SsorSys* __restrict__ sys, int tile0) {
+__global__ void __launch_bounds__(256) k_ssor_xsolve(const SsorSys* __restrict__ sys, int tile0, int c) {
   const SsorView S = sview(sys);
   if (S.st->zero) return;
+  const ColourInfo ci = colour_info(S, c);
   const int tile = tile0 + blockIdx.x;
   const gptr<const double> x = C(S.w);
   for (int j = 0; j < R; ++j) {
     const int v = tile_row<R>(tile, j);
-    S.w[v] = (S.y[v] - upper_sum(S, v, x)) / S.dt[v];
+    S.w[v] = (S.y[v] - upper_sum(S, ci, v, x)) / S.dt[v];
   }
 }
 
@@ -378,13 +448,16 @@ __global__ void k_ssor_reset(const SsorSys* __restrict__ sys, double rtol, int m
 }
 
 // ------------------------------------------------------------------ launch wrappers
-#define GSH_SSOR_R(rpt, KER, GRID, ...)                                                  \
-  switch (rpt) {                                                                         \
-    case 1: hipLaunchKernelGGL(KER<1>, dim3(GRID), dim3(kTpb), 0, __VA_ARGS__); break;   \
-    case 2: hipLaunchKernelGGL(KER<2>, dim3(GRID), dim3(kTpb), 0, __VA_ARGS__); break;   \
-    case 8: hipLaunchKernelGGL(KER<8>, dim3(GRID), dim3(kTpb), 0, __VA_ARGS__); break;   \
-    default: hipLaunchKernelGGL(KER<4>, dim3(GRID), dim3(kTpb), 0, __VA_ARGS__); break;  \
+// Launches KER<rpt>; with events e0/e1 the runtime stamps the kernel's own start and end
+// (hipExtLaunchKernelGGL), which the profiling path reads.
+#define GSH_SSOR_RX(rpt, KER, GRID, E0, E1, STREAM, ...)                                                   \
+  switch (rpt) {                                                                                         \
+    case 1: hipExtLaunchKernelGGL(KER<1>, dim3(GRID), dim3(kTpb), 0, STREAM, E0, E1, 0, __VA_ARGS__); break;  \
+    case 2: hipExtLaunchKernelGGL(KER<2>, dim3(GRID), dim3(kTpb), 0, STREAM, E0, E1, 0, __VA_ARGS__); break;  \
+    case 8: hipExtLaunchKernelGGL(KER<8>, dim3(GRID), dim3(kTpb), 0, STREAM, E0, E1, 0, __VA_ARGS__); break;  \
+    default: hipExtLaunchKernelGGL(KER<4>, dim3(GRID), dim3(kTpb), 0, STREAM, E0, E1, 0, __VA_ARGS__); break; \
   }
+#define GSH_SSOR_R(rpt, KER, GRID, STREAM, ...) GSH_SSOR_RX(rpt, KER, GRID, nullptr, nullptr, STREAM, __VA_ARGS__)
 
 void launch_ssor_reset(const SsorSys* sys, double rtol, int maxiter, int check_every, hipStream_t s) {
   hipLaunchKernelGGL(k_ssor_reset, dim3(1), dim3(64), 0, s, sys, rtol, maxiter, check_every);
@@ -395,17 +468,19 @@ void launch_ssor_setup(const SsorSys* sys, int ntiles, int rpt, hipStream_t s) {
 void launch_ssor_residual(const SsorSys* sys, int ntiles, int rpt, hipStream_t s) {
   if (ntiles > 0) GSH_SSOR_R(rpt, k_ssor_residual, ntiles, s, sys);
 }
-void launch_ssor_r0(const SsorSys* sys, int tile0, int ntc, int rpt, hipStream_t s) {
-  if (ntc > 0) GSH_SSOR_R(rpt, k_ssor_r0, ntc, s, sys, tile0);
+void launch_ssor_r0(const SsorSys* sys, int tile0, int ntc, int c, int rpt, hipStream_t s) {
+  if (ntc > 0) GSH_SSOR_R(rpt, k_ssor_r0, ntc, s, sys, tile0, c);
 }
-void launch_ssor_bwd(const SsorSys* sys, int tile0, int ntc, int first, int mid, int rpt, hipStream_t s) {
-  if (ntc > 0) GSH_SSOR_R(rpt, k_ssor_bwd, ntc, s, sys, tile0, first, mid);
+void launch_ssor_bwd(const SsorSys* sys, int tile0, int ntc, int c, int first, int mid, int rpt, hipStream_t s,
+                     hipEvent_t e0, hipEvent_t e1) {
+  if (ntc > 0) GSH_SSOR_RX(rpt, k_ssor_bwd, ntc, e0, e1, s, sys, tile0, c, first, mid);
 }
-void launch_ssor_fwd(const SsorSys* sys, int tile0, int ntc, int rpt, hipStream_t s) {
-  if (ntc > 0) GSH_SSOR_R(rpt, k_ssor_fwd, ntc, s, sys, tile0);
+void launch_ssor_fwd(const SsorSys* sys, int tile0, int ntc, int c, int rpt, hipStream_t s, hipEvent_t e0,
+                     hipEvent_t e1) {
+  if (ntc > 0) GSH_SSOR_RX(rpt, k_ssor_fwd, ntc, e0, e1, s, sys, tile0, c);
 }
-void launch_ssor_update(const SsorSys* sys, int ntiles, int rpt, hipStream_t s) {
-  if (ntiles > 0) GSH_SSOR_R(rpt, k_ssor_update, ntiles, s, sys);
+void launch_ssor_update(const SsorSys* sys, int ntiles, int rpt, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
+  if (ntiles > 0) GSH_SSOR_RX(rpt, k_ssor_update, ntiles, e0, e1, s, sys);
 }
 void launch_ssor_true_rr(const SsorSys* sys, int ntiles, int rpt, hipStream_t s) {
   if (ntiles > 0) GSH_SSOR_R(rpt, k_ssor_true_rr, ntiles, s, sys);
@@ -413,8 +488,8 @@ void launch_ssor_true_rr(const SsorSys* sys, int ntiles, int rpt, hipStream_t s)
 void launch_ssor_check(const SsorSys* sys, hipStream_t s) {
   hipLaunchKernelGGL(k_ssor_check, dim3(1), dim3(kTpb), 0, s, sys);
 }
-void launch_ssor_xsolve(const SsorSys* sys, int tile0, int ntc, int rpt, hipStream_t s) {
-  if (ntc > 0) GSH_SSOR_R(rpt, k_ssor_xsolve, ntc, s, sys, tile0);
+void launch_ssor_xsolve(const SsorSys* sys, int tile0, int ntc, int c, int rpt, hipStream_t s) {
+  if (ntc > 0) GSH_SSOR_R(rpt, k_ssor_xsolve, ntc, s, sys, tile0, c);
 }
 void launch_ssor_scatter(const SsorSys* sys, int ntiles, int rpt, hipStream_t s) {
   if (ntiles > 0) GSH_SSOR_R(rpt, k_ssor_scatter, ntiles, s, sys);

@@ -1,4 +1,4 @@
-// ssor_layout.cpp -- smallest-last colouring and the colour-major split CSR of the SSOR solver.
+// ssor_layout.cpp -- smallest-last colouring and the colour-major split ELL of the SSOR solver.
 #include "ssor_layout.h"
 
 #include <algorithm>
@@ -76,34 +76,64 @@ void build_ssor_layout(int32_t n, const int32_t* indptr, const int32_t* indices,
     L.perm[v] = i;
     L.newpos[i] = v;
   }
-  L.lptr.assign(np + 1, 0);
-  L.uptr.assign(np + 1, 0);
   L.dsrc.assign(np, -1);
-  std::vector<std::pair<int32_t, int32_t>> lo, up;  // (new column, slot)
+  // sorted lower / upper lists of every row: (new column, slot)
+  std::vector<std::vector<std::pair<int32_t, int32_t>>> lo(np), up(np);
   for (int32_t v = 0; v < np; ++v) {
     const int32_t i = L.perm[v];
-    lo.clear();
-    up.clear();
-    if (i >= 0) {
-      const int32_t c = L.colour[i];
-      for (int32_t k = indptr[i]; k < indptr[i + 1]; ++k) {
-        const int32_t j = indices[k];
-        if (j == i) {
-          L.dsrc[v] = k;
-          continue;
-        }
-        const int32_t w = L.newpos[j];
-        (w < L.cstart[c] ? lo : up).push_back({w, k});
+    if (i < 0) continue;
+    const int32_t c = L.colour[i];
+    for (int32_t k = indptr[i]; k < indptr[i + 1]; ++k) {
+      const int32_t j = indices[k];
+      if (j == i) {
+        L.dsrc[v] = k;
+        continue;
       }
-      GSH_REQUIRE(L.dsrc[v] >= 0, GSH_E_ARG, "SSOR: every row needs a diagonal entry in the pattern");
-      std::sort(lo.begin(), lo.end());
-      std::sort(up.begin(), up.end());
+      const int32_t w = L.newpos[j];
+      (w < L.cstart[c] ? lo[v] : up[v]).push_back({w, k});
     }
-    for (const auto& e : lo) L.lcol.push_back(e.first), L.lsrc.push_back(e.second);
-    for (const auto& e : up) L.ucol.push_back(e.first), L.usrc.push_back(e.second);
-    L.lptr[v + 1] = (int32_t)L.lcol.size();
-    L.uptr[v + 1] = (int32_t)L.ucol.size();
+    GSH_REQUIRE(L.dsrc[v] >= 0, GSH_E_ARG, "SSOR: every row needs a diagonal entry in the pattern");
+    std::sort(lo[v].begin(), lo[v].end());
+    std::sort(up[v].begin(), up[v].end());
   }
+  L.wl.assign(C, 0);
+  L.wu.assign(C, 0);
+  L.offl.assign(C, 0);
+  L.offu.assign(C, 0);
+  int64_t tl = 0, tu = 0;
+  for (int32_t c = 0; c < C; ++c) {
+    for (int32_t v = L.cstart[c]; v < L.cstart[c + 1]; ++v) {
+      L.wl[c] = std::max<int32_t>(L.wl[c], (int32_t)lo[v].size());
+      L.wu[c] = std::max<int32_t>(L.wu[c], (int32_t)up[v].size());
+    }
+    const int64_t rows = L.cstart[c + 1] - L.cstart[c];
+    L.offl[c] = tl;
+    L.offu[c] = tu;
+    tl += rows * L.wl[c];
+    tu += rows * L.wu[c];
+  }
+  GSH_REQUIRE(tl < INT32_MAX && tu < INT32_MAX, GSH_E_ARG, "SSOR layout: ELL size exceeds int32");
+  L.lcol.assign(tl, -1);
+  L.lsrc.assign(tl, -1);
+  L.ucol.assign(tu, -1);
+  L.usrc.assign(tu, -1);
+  for (int32_t c = 0; c < C; ++c) {
+    const int64_t rows = L.cstart[c + 1] - L.cstart[c];
+    for (int32_t v = L.cstart[c]; v < L.cstart[c + 1]; ++v) {
+      const int64_t r = v - L.cstart[c];
+      for (size_t m = 0; m < lo[v].size(); ++m) {
+        L.lcol[L.offl[c] + m * rows + r] = lo[v][m].first;
+        L.lsrc[L.offl[c] + m * rows + r] = lo[v][m].second;
+      }
+      for (size_t m = 0; m < up[v].size(); ++m) {
+        L.ucol[L.offu[c] + m * rows + r] = up[v][m].first;
+        L.usrc[L.offu[c] + m * rows + r] = up[v][m].second;
+      }
+    }
+  }
+  L.tile_colour.assign(np / tile_rows, 0);
+  for (int32_t c = 0; c < C; ++c)
+    for (int32_t t = L.cstart[c] / tile_rows; t < L.cstart[c + 1] / tile_rows; ++t) L.tile_colour[t] = c;
 }
 
 }  // namespace gsh

@@ -1353,12 +1353,13 @@ int32_t or_colour(int32_t n, const int32_t* indptr, const int32_t* indices, int3
   return ncol;
 }
 
-typedef struct {
-  int32_t n, np, ncol;
+typedef struct or_ssor {
+  int32_t n, np, ncol, R;
   int32_t* cstart;  /* [ncol+1] padded block starts */
   int32_t* perm;    /* [np] new -> old, -1 for padding */
   int32_t* newpos;  /* [n] old -> new */
   int32_t *lptr, *lcol, *uptr, *ucol; /* [np+1], [*] */
+  int32_t *lsrc, *usrc, *dsrc;        /* slots of the entries in the CSR value array (-1: padding row) */
   double *lval, *uval, *d;            /* off-diagonal values, diagonal [np] */
 } ssor_layout;
 
@@ -1367,9 +1368,9 @@ static int cmp_pair(const void* a, const void* b) {
   return (x > y) - (x < y);
 }
 
-static int ssor_build(ssor_layout* L, int32_t n, const int32_t* indptr, const int32_t* indices, const double* val,
-                      int32_t R) {
+static int ssor_build(ssor_layout* L, int32_t n, const int32_t* indptr, const int32_t* indices, int32_t R) {
   memset(L, 0, sizeof(*L));
+  L->R = R;
   const int32_t T = 256 * R;
   int32_t* colour = xcalloc(n > 0 ? n : 1, sizeof(int32_t));
   const int32_t C = n > 0 ? or_colour(n, indptr, indices, colour) : 0;
@@ -1394,6 +1395,7 @@ static int ssor_build(ssor_layout* L, int32_t n, const int32_t* indptr, const in
   L->lptr = xcalloc((size_t)np + 1, sizeof(int32_t));
   L->uptr = xcalloc((size_t)np + 1, sizeof(int32_t));
   L->d = xcalloc(np > 0 ? np : 1, sizeof(double));
+  L->dsrc = xcalloc(np > 0 ? np : 1, sizeof(int32_t));
   int bad = 0;
   for (int32_t v = 0; v < np; ++v) {
     const int32_t i = L->perm[v];
@@ -1421,19 +1423,19 @@ static int ssor_build(ssor_layout* L, int32_t n, const int32_t* indptr, const in
   L->ucol = xcalloc((size_t)L->uptr[np] + 1, sizeof(int32_t));
   L->lval = xcalloc((size_t)L->lptr[np] + 1, sizeof(double));
   L->uval = xcalloc((size_t)L->uptr[np] + 1, sizeof(double));
+  L->lsrc = xcalloc((size_t)L->lptr[np] + 1, sizeof(int32_t));
+  L->usrc = xcalloc((size_t)L->uptr[np] + 1, sizeof(int32_t));
   int64_t* tmp = xcalloc((size_t)64 + (indptr[n] > 0 ? indptr[n] : 1), sizeof(int64_t));
   for (int32_t v = 0; v < np; ++v) {
     const int32_t i = L->perm[v];
-    if (i < 0) {
-      L->d[v] = 1.0;
-      continue;
-    }
+    L->dsrc[v] = -1;
+    if (i < 0) continue;
     const int32_t c = colour[i];
     int32_t m = 0;
     for (int32_t k = indptr[i]; k < indptr[i + 1]; ++k) {
       const int32_t j = indices[k];
       if (j == i) {
-        L->d[v] = val[k];
+        L->dsrc[v] = k;
         continue;
       }
       tmp[m++] = ((int64_t)L->newpos[j] << 32) | (int64_t)(k - indptr[i]);
@@ -1442,8 +1444,8 @@ static int ssor_build(ssor_layout* L, int32_t n, const int32_t* indptr, const in
     int32_t a = L->lptr[v], b = L->uptr[v];
     for (int32_t q = 0; q < m; ++q) {
       const int32_t w = (int32_t)(tmp[q] >> 32), k = indptr[i] + (int32_t)(tmp[q] & 0xffffffff);
-      if (w < L->cstart[c]) L->lcol[a] = w, L->lval[a++] = val[k];
-      else L->ucol[b] = w, L->uval[b++] = val[k];
+      if (w < L->cstart[c]) L->lcol[a] = w, L->lsrc[a++] = k;
+      else L->ucol[b] = w, L->usrc[b++] = k;
     }
   }
   free(tmp);
@@ -1454,7 +1456,17 @@ static int ssor_build(ssor_layout* L, int32_t n, const int32_t* indptr, const in
   return bad ? -1 : 0;
 }
 
+/* values of the split matrix from the CSR values (padding rows: d = 1) */
+static void ssor_fill(ssor_layout* L, const double* val) {
+  for (int32_t v = 0; v < L->np; ++v) L->d[v] = L->dsrc[v] >= 0 ? val[L->dsrc[v]] : 1.0;
+  for (int32_t k = 0; k < L->lptr[L->np]; ++k) L->lval[k] = val[L->lsrc[k]];
+  for (int32_t k = 0; k < L->uptr[L->np]; ++k) L->uval[k] = val[L->usrc[k]];
+}
+
 static void ssor_free(ssor_layout* L) {
+  free(L->lsrc);
+  free(L->usrc);
+  free(L->dsrc);
   free(L->cstart);
   free(L->perm);
   free(L->newpos);
@@ -1479,23 +1491,49 @@ static inline double usum(const ssor_layout* L, int32_t v, const double* x) {
   return acc;
 }
 
+or_ssor* or_ssor_new(int32_t n, const int32_t* indptr, const int32_t* indices, int32_t R) {
+  ssor_layout* L = xcalloc(1, sizeof(ssor_layout));
+  if (ssor_build(L, n, indptr, indices, R) != 0) {
+    ssor_free(L);
+    free(L);
+    return NULL;
+  }
+  return L;
+}
+
+void or_ssor_free(or_ssor* L) {
+  if (!L) return;
+  ssor_free(L);
+  free(L);
+}
+
+int32_t or_ssor_ncolours(const or_ssor* L) { return L->ncol; }
+
 int32_t or_pcg_ssor(int32_t n, const int32_t* indptr, const int32_t* indices, const double* val, const double* b,
                     const double* h0, const double* h1, const double* h2, double omega, double rtol,
                     int32_t maxiter, int32_t checkEvery, int32_t R, double* x, double* rr_last, double* atol2_out,
                     int32_t* ncolours, int nthreads) {
+  or_ssor* L = or_ssor_new(n, indptr, indices, R);
+  if (!L) return -1;
+  if (ncolours) *ncolours = L->ncol;
+  const int32_t it = or_ssor_solve(L, val, b, h0, h1, h2, omega, rtol, maxiter, checkEvery, x, rr_last, atol2_out,
+                                   nthreads);
+  or_ssor_free(L);
+  return it;
+}
+
+int32_t or_ssor_solve(or_ssor* L, const double* val, const double* b, const double* h0, const double* h1,
+                      const double* h2, double omega, double rtol, int32_t maxiter, int32_t checkEvery, double* x,
+                      double* rr_last, double* atol2_out, int nthreads) {
 #ifdef _OPENMP
   if (nthreads > 0) omp_set_num_threads(nthreads);
 #endif
   (void)nthreads;
+  const int32_t n = L->n, R = L->R;
   if (maxiter <= 0) maxiter = 10 * n;
   if (checkEvery < 1) checkEvery = 1;
-  ssor_layout L;
-  if (ssor_build(&L, n, indptr, indices, val, R) != 0) {
-    ssor_free(&L);
-    return -1;
-  }
-  if (ncolours) *ncolours = L.ncol;
-  const int32_t np = L.np, C = L.ncol;
+  ssor_fill(L, val);
+  const int32_t np = L->np, C = L->ncol;
   const int64_t T = np / (256 * (int64_t)R);
   double* part = xcalloc(T > 0 ? T : 1, sizeof(double));
   double *dt = xcalloc(np + 1, sizeof(double)), *kd = xcalloc(np + 1, sizeof(double));
@@ -1504,10 +1542,13 @@ int32_t or_pcg_ssor(int32_t n, const int32_t* indptr, const int32_t* indices, co
   double *z = xcalloc(np + 1, sizeof(double)), *p = xcalloc(np + 1, sizeof(double));
   double *t = xcalloc(np + 1, sizeof(double)), *s = xcalloc(np + 1, sizeof(double));
   double *q = xcalloc(np + 1, sizeof(double)), *w = xcalloc(np + 1, sizeof(double));
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
   for (int32_t v = 0; v < np; ++v) {
-    dt[v] = L.d[v] / omega;
-    kd[v] = 2.0 * dt[v] - L.d[v];
-    const int32_t i = L.perm[v];
+    dt[v] = L->d[v] / omega;
+    kd[v] = 2.0 * dt[v] - L->d[v];
+    const int32_t i = L->perm[v];
     bp[v] = i >= 0 ? b[i] : 0.0;
     xp[v] = (i >= 0 && h0) ? (3.0 * h0[i] - 3.0 * h1[i]) + h2[i] : 0.0;
   }
@@ -1522,15 +1563,22 @@ int32_t or_pcg_ssor(int32_t n, const int32_t* indptr, const int32_t* indices, co
     goto done;
   }
   /* r_true = b - A x0 and y0 = (D~+U) x0 */
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
   for (int32_t v = 0; v < np; ++v) {
-    const double au = usum(&L, v, xp);
-    const double ax = (lsum(&L, v, xp) + L.d[v] * xp[v]) + au;
+    const double au = usum(L, v, xp);
+    const double ax = (lsum(L, v, xp) + L->d[v] * xp[v]) + au;
     w[v] = bp[v] - ax;
     y[v] = dt[v] * xp[v] + au;
   }
   /* r0 = (D~+L)^-1 r_true, colour by colour */
-  for (int32_t c = 0; c < C; ++c)
-    for (int32_t v = L.cstart[c]; v < L.cstart[c + 1]; ++v) r[v] = (w[v] - lsum(&L, v, r)) / dt[v];
+  for (int32_t c = 0; c < C; ++c) {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+    for (int32_t v = L->cstart[c]; v < L->cstart[c + 1]; ++v) r[v] = (w[v] - lsum(L, v, r)) / dt[v];
+  }
   for (int32_t v = 0; v < np; ++v) {
     z[v] = dt[v] * r[v];
     p[v] = 0.0;
@@ -1542,18 +1590,27 @@ int32_t or_pcg_ssor(int32_t n, const int32_t* indptr, const int32_t* indices, co
     const double beta = (k == 0 || rz_old == 0.0) ? 0.0 : rz_cur / rz_old;
     /* t = (D~+U)^-1 p, p = z + beta p: last colour first */
     for (int32_t c = C - 1; c >= 0; --c)
-      for (int32_t v = L.cstart[c]; v < L.cstart[c + 1]; ++v) {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+      for (int32_t v = L->cstart[c]; v < L->cstart[c + 1]; ++v) {
         p[v] = dt[v] * r[v] + beta * p[v];
-        t[v] = (p[v] - usum(&L, v, t)) / dt[v];
+        t[v] = (p[v] - usum(L, v, t)) / dt[v];
       }
     /* s = (D~+L)^-1 (p - (2D~ - D) t), q = t + s: first colour first */
     for (int32_t c = 0; c < C; ++c)
-      for (int32_t v = L.cstart[c]; v < L.cstart[c + 1]; ++v) {
-        s[v] = ((p[v] - kd[v] * t[v]) - lsum(&L, v, s)) / dt[v];
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+      for (int32_t v = L->cstart[c]; v < L->cstart[c + 1]; ++v) {
+        s[v] = ((p[v] - kd[v] * t[v]) - lsum(L, v, s)) / dt[v];
         q[v] = t[v] + s[v];
       }
     const double pq = canon_dot(np, p, q, R, part);
     const double alpha = (pq != 0.0) ? rz_cur / pq : 0.0;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
     for (int32_t v = 0; v < np; ++v) {
       y[v] += alpha * p[v];
       r[v] = r[v] - alpha * q[v];
@@ -1563,7 +1620,10 @@ int32_t or_pcg_ssor(int32_t n, const int32_t* indptr, const int32_t* indices, co
     rz_old = rz_cur;
     rz_cur = rz_new;
     if ((it + 1) % checkEvery == 0 || it == maxiter - 1) {
-      for (int32_t v = 0; v < np; ++v) w[v] = dt[v] * r[v] + lsum(&L, v, r);
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+      for (int32_t v = 0; v < np; ++v) w[v] = dt[v] * r[v] + lsum(L, v, r);
       const double rr = canon_dot(np, w, w, R, part);
       *rr_last = rr;
       if (rr <= atol2) {
@@ -1573,9 +1633,13 @@ int32_t or_pcg_ssor(int32_t n, const int32_t* indptr, const int32_t* indices, co
     }
   }
   /* x = (D~+U)^-1 y, back to the original numbering */
-  for (int32_t c = C - 1; c >= 0; --c)
-    for (int32_t v = L.cstart[c]; v < L.cstart[c + 1]; ++v) xp[v] = (y[v] - usum(&L, v, xp)) / dt[v];
-  for (int32_t i = 0; i < n; ++i) x[i] = xp[L.newpos[i]];
+  for (int32_t c = C - 1; c >= 0; --c) {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static)
+#endif
+    for (int32_t v = L->cstart[c]; v < L->cstart[c + 1]; ++v) xp[v] = (y[v] - usum(L, v, xp)) / dt[v];
+  }
+  for (int32_t i = 0; i < n; ++i) x[i] = xp[L->newpos[i]];
 done:
   free(part);
   free(dt);
@@ -1590,7 +1654,6 @@ done:
   free(s);
   free(q);
   free(w);
-  ssor_free(&L);
   return result;
 }
 

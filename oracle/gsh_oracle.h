@@ -65,6 +65,14 @@ int32_t or_pcg_ssor(int32_t n, const int32_t* indptr, const int32_t* indices, co
                     const double* h0, const double* h1, const double* h2, double omega, double rtol,
                     int32_t maxiter, int32_t check_every, int32_t rows_per_thread, double* x_out, double* rr_last,
                     double* atol2, int32_t* ncolours, int nthreads);
+/* The same solver with its colouring and layout built once (rows_per_thread fixes the padding). */
+typedef struct or_ssor or_ssor;
+or_ssor* or_ssor_new(int32_t n, const int32_t* indptr, const int32_t* indices, int32_t rows_per_thread);
+int32_t or_ssor_ncolours(const or_ssor* s);
+int32_t or_ssor_solve(or_ssor* s, const double* val, const double* b, const double* h0, const double* h1,
+                      const double* h2, double omega, double rtol, int32_t maxiter, int32_t check_every,
+                      double* x_out, double* rr_last, double* atol2, int nthreads);
+void or_ssor_free(or_ssor* s);
 /* Dot product in the device's fixed reduction order. */
 double or_dot_canon(int32_t n, const double* a, const double* b, int32_t rows_per_thread);
 /* ||b - A x|| / ||b|| in the same order as the device check. */

@@ -41,6 +41,11 @@ _SIG = {
     "or_colour": (C.c_int32, [C.c_int32, _PI32, _PI32, _PI32]),
     "or_pcg_ssor": (C.c_int32, [C.c_int32, _PI32, _PI32, _PD, _PD, _PD, _PD, _PD, C.c_double, C.c_double,
                                 C.c_int32, C.c_int32, C.c_int32, _PD, _PD, _PD, _PI32, C.c_int]),
+    "or_ssor_new": (_P, [C.c_int32, _PI32, _PI32, C.c_int32]),
+    "or_ssor_ncolours": (C.c_int32, [_P]),
+    "or_ssor_solve": (C.c_int32, [_P, _PD, _PD, _PD, _PD, _PD, C.c_double, C.c_double, C.c_int32, C.c_int32, _PD, _PD,
+                                  _PD, C.c_int]),
+    "or_ssor_free": (None, [_P]),
     "or_geometry": (C.c_int, [_P, C.c_int, _PD]),
     "or_interp": (None, [_P, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double, _PD]),
     "or_dot_canon": (C.c_double, [C.c_int32, _PD, _PD, C.c_int32]),
@@ -226,6 +231,38 @@ def pcg_ssor(indptr, indices, val, b, hist=None, omega=1.0, rtol=1e-12, maxiter=
     if it < 0:
         raise ValueError(lib().or_error().decode())
     return x, int(it), rr.value, at.value, nc.value
+
+
+class OracleSsor:
+    """SSOR PCG with the colouring and layout built once (as the device solver keeps them)."""
+
+    def __init__(self, indptr, indices, rows_per_thread=1):
+        self._ip = np.ascontiguousarray(indptr, dtype=np.int32)
+        self._ix = np.ascontiguousarray(indices, dtype=np.int32)
+        self.n = self._ip.size - 1
+        self.h = lib().or_ssor_new(self.n, _i(self._ip), _i(self._ix), rows_per_thread)
+        if not self.h:
+            raise ValueError(lib().or_error().decode())
+        self.ncolours = int(lib().or_ssor_ncolours(self.h))
+
+    def solve(self, val, b, hist=None, omega=1.0, rtol=1e-12, maxiter=1500, check_every=10, nthreads: int = 1):
+        """Returns (x, iters, rr_last, atol2)."""
+        val = np.ascontiguousarray(val, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.zeros(self.n, dtype=np.float64)
+        rr, at = C.c_double(), C.c_double()
+        h = [None, None, None] if hist is None else [np.ascontiguousarray(v, dtype=np.float64) for v in hist]
+        it = lib().or_ssor_solve(self.h, _d(val), _d(b), _d(h[0]), _d(h[1]), _d(h[2]), omega, rtol, maxiter,
+                                 check_every, _d(x), C.byref(rr), C.byref(at), nthreads)
+        return x, int(it), rr.value, at.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().or_ssor_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
 
 
 def dot_canon(a, b, rows_per_thread=4) -> float:

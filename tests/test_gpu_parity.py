@@ -233,6 +233,32 @@ def test_batch_equals_solo(ctx):
         assert_bitwise(mesh.get(L.F_ETA2), x, "batched solution")
 
 
+def test_mixed_batch_equals_solo(ctx):
+    """Mixed precision in lock-step: each system's result is the one it gets alone, and within
+    1e-8 of the FP64 solution."""
+    systems = []
+    for k, (nx, ny) in enumerate(((50, 30), (28, 61), (70, 22), (45, 45))):
+        spec = cases.CaseSpec(f"m{k}", nx, ny, 800.0, nvrt=4, depth=None, substeps=4)
+        mesh, om, p = _setup(ctx, spec)
+        mesh.backtrack(p)
+        mesh.rhs_producers(p)
+        f64 = core.Solver(mesh)
+        f64.assemble(p)
+        x64, _, _ = f64.solve(None, rtol=1e-12, maxiter=3000)
+        mx = core.Solver(mesh, L.PREC_MIXED)
+        mx.assemble(p)
+        xm, itm, stm = mx.solve(None, rtol=1e-12, maxiter=3000)
+        assert stm == 0
+        assert np.linalg.norm(xm - x64) / np.linalg.norm(x64) < 1e-8
+        mx.reset_history()
+        systems.append((mesh, mx, xm, itm))
+    print("[gpu] mixed batch of 4", flush=True)
+    its = core.batch_solve([m for _, m, _, _ in systems], rtol=1e-12, maxiter=3000)
+    for (mesh, mx, xm, itm), itb in zip(systems, its):
+        assert itb == itm
+        assert_bitwise(mx.get_solution(), xm, "mixed batched solution")
+
+
 def test_full_size_1m_channel(ctx):
     """The 1024 x 1024 log-normal channel (1.05M nodes, 31 levels): a sample of backtracks
     against the oracle's list mode; RHS, assembly and the whole FP64 solve bit for bit."""

@@ -11,13 +11,13 @@ from gschism_amd import cases, core
 pytestmark = pytest.mark.gpu
 
 
-def _mesh(ctx, nx, ny, seed_shift):
+def _mesh(ctx, nx, ny, seed_shift, precision=L.PREC_FP64):
     spec = cases.CaseSpec(f"srv{nx}x{ny}", nx, ny, 700.0, nvrt=4, depth=None, substeps=4)
     data = cases.make_case(spec)
     mesh, p, _ = cases.load_mesh(ctx, data)
     mesh.backtrack(p)
     mesh.rhs_producers(p)
-    s = core.Solver(mesh)
+    s = core.Solver(mesh, precision)
     s.assemble(p)
     return mesh, s
 
@@ -63,3 +63,37 @@ def test_admission_budget(ctx):
     srv.step()
     st, x, it, lat = srv.poll(t, m.n)
     assert st == L.GSH_E_ADMISSION
+
+
+def test_mixed_alone_vs_mix(ctx):
+    """Mixed-precision serving: every result equals the request's solo mixed solve bit for bit
+    and is within 1e-8 of the FP64 solution."""
+    meshes = [_mesh(ctx, 44, 30, 0, L.PREC_MIXED), _mesh(ctx, 80, 26, 1, L.PREC_MIXED),
+              _mesh(ctx, 35, 52, 2, L.PREC_MIXED)]
+    rhs = [cases.normal(400 + k, m.n) for k, (m, _) in enumerate(meshes)]
+    alone = []
+    for (m, s), b in zip(meshes, rhs):
+        s.reset_history()
+        x, it, st = s.solve(b, rtol=1e-12, maxiter=3000, check_every=10)
+        assert st == 0
+        f64 = core.Solver(m)
+        f64.set_values(s.get_values())
+        x64, _, _ = f64.solve(b, rtol=1e-12, maxiter=3000)
+        assert np.linalg.norm(x - x64) / np.linalg.norm(x64) < 1e-8
+        alone.append((x, it))
+    srv = core.Server(ctx, rtol=1e-12, maxiter=3000, check_every=10, max_active=2, precision=L.PREC_MIXED)
+    ids = [srv.add_mesh(m) for m, _ in meshes]
+    tickets = []
+    for j, k in enumerate([2, 0, 1, 0, 2, 1, 1]):
+        tickets.append((k, srv.submit(ids[k], rhs[k])))
+        if j % 3 == 2:
+            srv.step()
+    for _ in range(5000):
+        a, q = srv.step()
+        if a == 0 and q == 0:
+            break
+    for k, t in tickets:
+        st, x, it, lat = srv.poll(t, meshes[k][0].n)
+        assert st == 0, st
+        assert it == alone[k][1], (k, it, alone[k][1])
+        assert np.array_equal(x, alone[k][0]), f"mixed request on mesh {k} differs from the solo solve"
