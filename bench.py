@@ -69,6 +69,159 @@ def ssor_bytes(mesh, solver):
     return {"bwd": bwd, "fwd": fwd, "upd": upd, "ncolours": nc, "colour0_rows": n0}
 
 
+def batched_leg(ctx, L, cases, core, members, nx, ny, rtol, mixed_rtol, check_every):
+    """Ensemble: `members` systems on one nx x ny mesh, bathymetry h (1 + 0.05 N(0,1)) per member
+    (seeds 100 + m), one random right-hand side each (seeds 200 + m). Plain = one FP64 Jacobi solve
+    after another; batched = all members in one lock-step launch per PCG phase; mixed = FP32 inner
+    iterations with FP64 residual and refinement."""
+    spec = cases.CaseSpec(f"ensemble-{nx}x{ny}", nx, ny, 1000.0, nvrt=11, depth=None, substeps=10)
+    data = cases.make_case(spec)
+    mesh, p, _ = cases.load_mesh(ctx, data)
+    depth0 = mesh.topology(L.T_DEPTH)
+    s64, smx = [], []
+    for m in range(members):
+        mesh.set(L.F_DEPTH, depth0 * (1.0 + 0.05 * cases.normal(100 + m, mesh.n)))
+        mesh.update_levels(p)
+        mesh.backtrack(p)
+        mesh.rhs_producers(p)
+        b = cases.normal(200 + m, mesh.n)
+        a = core.Solver(mesh)
+        a.assemble(p)
+        a.set_rhs(b)
+        c = core.Solver(mesh, L.PREC_MIXED)
+        c.set_values(a.get_values())
+        c.set_rhs(b)
+        s64.append(a)
+        smx.append(c)
+    mesh.set(L.F_DEPTH, depth0)
+    mesh.update_levels(p)
+
+    def seq(solvers, tol):
+        for s in solvers:
+            s.reset_history()
+        ctx.synchronize()
+        t = time.perf_counter()
+        its = [s.solve(None, tol, 20000, check_every, want_x=False)[1] for s in solvers]
+        ctx.synchronize()
+        return time.perf_counter() - t, its
+
+    def bat(solvers, tol):
+        for s in solvers:
+            s.reset_history()
+        ctx.synchronize()
+        t = time.perf_counter()
+        its = core.batch_solve(solvers, tol, 20000, check_every)
+        ctx.synchronize()
+        return time.perf_counter() - t, its
+
+    seq(s64[:2], rtol)  # warm
+    t_plain, it_plain = seq(s64, rtol)
+    x64 = [s.get_solution() for s in s64]
+    t_b64, it_b64 = bat(s64, rtol)
+    same = all(np.array_equal(s.get_solution(), x) for s, x in zip(s64, x64))
+    bat(smx[:2], mixed_rtol)  # warm
+    t_bmx, it_bmx = bat(smx, mixed_rtol)
+    err = max(float(np.linalg.norm(s.get_solution() - x) / np.linalg.norm(x)) for s, x in zip(smx, x64))
+    t_smx, _ = seq(smx, mixed_rtol)
+    out = {"workload": f"{members} members on {spec.name} (n={mesh.n}), per-member bathymetry and RHS",
+           "plain_fp64_solves_per_s": round(members / t_plain, 3),
+           "batched_fp64_solves_per_s": round(members / t_b64, 3),
+           "mixed_solves_per_s": round(members / t_smx, 3),
+           "batched_mixed_solves_per_s": round(members / t_bmx, 3),
+           "batched_mixed_vs_plain": round(t_plain / t_bmx, 3),
+           "fp64_iters_mean": float(np.mean(it_plain)), "mixed_inner_iters_mean": float(np.mean(it_bmx)),
+           "batched_fp64_bitwise_equal_to_plain": bool(same and it_b64 == it_plain),
+           "mixed_rtol": mixed_rtol, "mixed_max_rel_l2_vs_fp64": err}
+    for s in s64 + smx:
+        s.close()
+    mesh.close()
+    return out
+
+
+def served_leg(ctx, L, cases, core, n_req, rates, rtol, mixed_rtol, check_every):
+    """Served solves: a pool of 6 meshes with n log-uniform in [1e4, 2e6] (random bathymetry,
+    seeds 3 + k), requests drawn uniformly from the pool with Poisson arrivals (seed 7) and
+    right-hand sides N(0,1) (seeds 1000 + j); each request backtracks on its mesh and solves.
+    Plain = the same requests one after another in FP64; served = the continuous-batching
+    server in mixed precision at each arrival rate."""
+    from math import erfc, log, sqrt
+
+    sizes = [int(round(1e4 * 200 ** (k / 5))) for k in range(6)]
+    pool = []
+    for k, n in enumerate(sizes):
+        nx = int(round(sqrt(2 * n)))
+        ny = max(3, n // nx)
+        spec = cases.CaseSpec(f"pool{k}-{nx}x{ny}", nx, ny, 1000.0, nvrt=11, depth=None, substeps=10,
+                              corr_cells=20.0)
+        data = cases.make_case(spec)
+        data.depth = cases.bathymetry(nx, ny, 20.0, 5.0, 4000.0, seed=3 + k)
+        mesh, _, _ = cases.load_mesh(ctx, data)
+        pool.append(mesh)
+    pd = L.default_step_params()
+    z = cases.normal(cases.SEED_ARRIVALS, 2 * n_req)
+    u = np.array([0.5 * erfc(-v / sqrt(2.0)) for v in z])
+    which = np.minimum((u[:n_req] * len(pool)).astype(int), len(pool) - 1)
+    expo = np.array([-log(max(1e-300, 1.0 - v)) for v in u[n_req:]])
+    rhs = [cases.normal(1000 + j, pool[k].n) for j, k in enumerate(which)]
+    # plain: one FP64 request after another
+    plain = [core.Solver(m) for m in pool]
+    ctx.synchronize()
+    t = time.perf_counter()
+    x64 = []
+    for j, k in enumerate(which):
+        m = pool[k]
+        m.backtrack(pd)
+        m.rhs_producers(pd)
+        plain[k].assemble(pd)
+        plain[k].reset_history()
+        x, _, _ = plain[k].solve(rhs[j], rtol, 20000, check_every)
+        x64.append(x)
+    ctx.synchronize()
+    t_plain = time.perf_counter() - t
+    plain_rate = n_req / t_plain
+    for s in plain:
+        s.close()
+    res = {"workload": f"{n_req} requests over 6 meshes, n = {sizes}", "plain_fp64_solves_per_s": round(plain_rate, 3),
+           "mixed_rtol": mixed_rtol, "rates": []}
+    for mult in rates:
+        lam = mult * plain_rate
+        arrive = np.cumsum(expo) / lam
+        srv = core.Server(ctx, rtol=mixed_rtol, maxiter=20000, check_every=check_every, precision=L.PREC_MIXED)
+        ids = [srv.add_mesh(m) for m in pool]
+        tickets = {}
+        done = {}
+        t0 = time.perf_counter()
+        nxt = 0
+        while len(done) < n_req:
+            now = time.perf_counter() - t0
+            while nxt < n_req and arrive[nxt] <= now:
+                tickets[nxt] = srv.submit(ids[which[nxt]], rhs[nxt], with_backtrack=True)
+                nxt += 1
+            a, q = srv.step()
+            for j, tk in list(tickets.items()):
+                if j in done:
+                    continue
+                st, x, it, lat = srv.poll(tk, pool[which[j]].n)
+                if st == 0:
+                    done[j] = (x, it, lat, time.perf_counter() - t0)
+                elif st != 1:
+                    raise RuntimeError(f"request {j} failed with status {st}")
+            if a == 0 and q == 0 and nxt < n_req:
+                time.sleep(max(0.0, min(0.01, arrive[nxt] - (time.perf_counter() - t0))))
+        t_end = max(v[3] for v in done.values())
+        lats = np.array([done[j][2] for j in range(n_req)])
+        errs = [float(np.linalg.norm(done[j][0] - x64[j]) / np.linalg.norm(x64[j])) for j in range(n_req)]
+        res["rates"].append({"arrival_rate_per_s": round(lam, 3), "solves_per_s": round(n_req / (t_end - arrive[0]), 3),
+                             "p50_ms": round(float(np.percentile(lats, 50)), 3),
+                             "p99_ms": round(float(np.percentile(lats, 99)), 3),
+                             "max_rel_l2_vs_fp64": max(errs),
+                             "mean_inner_iters": float(np.mean([done[j][1] for j in range(n_req)]))})
+        srv.close()
+    for m in pool:
+        m.close()
+    return res
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
@@ -82,6 +235,12 @@ def main() -> int:
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-jacobi", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (0 = all visible cores, max 16)")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--members", type=int, default=64)
+    ap.add_argument("--ensemble-nx", type=int, default=708)
+    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--requests", type=int, default=120)
+    ap.add_argument("--mixed-rtol", type=float, default=1e-10)
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -232,6 +391,17 @@ def main() -> int:
                "gpu_bitwise_equal": bool(np.array_equal(xo, x_ssor)), "iters_equal": bool(ito == it_ssor)}
         log(f"[bench] cpu baseline: {cpu}")
 
+    batched = None
+    if not args.no_batched:
+        batched = batched_leg(ctx, L, cases, core, args.members, args.ensemble_nx, args.ensemble_nx, rtol,
+                              args.mixed_rtol, check_every)
+        log(f"[bench] batched: {batched}")
+    served = None
+    if not args.no_served:
+        served = served_leg(ctx, L, cases, core, args.requests, (1.0, 2.0, 4.0), rtol, args.mixed_rtol,
+                            check_every)
+        log(f"[bench] served: {served}")
+
     line = {
         "metric": "PCG solves/s + HBM GB/s at 1M nodes; mixed vs FP64; served p50/p99 latency",
         "value": round(solves_per_s, 4),
@@ -260,6 +430,8 @@ def main() -> int:
         "assembly_ms": round(ms_asm, 3),
         "rhs_ms": round(ms_rhs, 3),
         "mixed": mixed,
+        "batched": batched,
+        "served": served,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -33,7 +33,7 @@ T_XY, T_ELNODE, T_ELSIDE, T_ISDEL, T_ISIDENODE, T_BALL_PTR, T_BALL_ELEM, T_NBR_N
 T_CSR_INDPTR, T_CSR_INDICES, T_CSR_SLOT, T_DEPTH, T_OPEN_BND, T_KB, T_SIDE_TYPE = range(8, 15)
 # field codes
 (F_ETA2, F_ETA1, F_NODE_UVW, F_SIDE_U2, F_BTU, F_FOOT, F_BSTATUS, F_CD, F_GRAV2, F_ETH, F_VSOURCE,
- F_SIDE_U, F_SIDE_HHAT, F_SIDE_CHI, F_GHAT1, F_QEL, F_ZC, F_ELBC, F_WET) = range(19)
+ F_SIDE_U, F_SIDE_HHAT, F_SIDE_CHI, F_GHAT1, F_QEL, F_ZC, F_ELBC, F_WET, F_DEPTH) = range(20)
 
 BT_WALK_CAP, BT_ITER38, BT_STUCK, BT_OPEN_EXIT, BT_SLIDE = 1, 2, 4, 8, 16
 
@@ -112,6 +112,7 @@ SIGNATURES = {
     "gsh_solver_get_values": (_I, [_P, _PD]),
     "gsh_solver_vec_ptr": (_I, [_P, _PP, _PP]),
     "gsh_solver_get_rhs": (_I, [_P, _PD]),
+    "gsh_solver_set_rhs": (_I, [_P, _PD]),
     "gsh_solver_get_solution": (_I, [_P, _PD]),
     "gsh_assemble": (_I, [_P, _P, C.POINTER(StepParams), _D]),
     "gsh_solver_solve": (_I, [_P, _PD, _PD, _D, _I, _I, _D, C.POINTER(_I), C.POINTER(_I)]),
@@ -130,6 +131,7 @@ SIGNATURES = {
     "gsh_copy_probe": (_I, [_P, _I64, _I, _PD]),
     "gsh_solver_profile": (_I, [_P, _D, _I, _I, _PD, C.POINTER(_I)]),
     "gsh_solver_set_omega": (_I, [_P, _D]),
+    "gsh_solver_autotune": (_I, [_P, _I, C.POINTER(_I), _PD]),
     "gsh_solver_ssor_info": (_I, [_P, _PI32, _PI32, _PI32]),
     "gsh_colour_csr": (_I, [_I32, _PI32, _PI32, _PI32, _PI32]),
     "gsh_gen_channel": (_I, [_I32, _I32, _D, _D, C.c_uint64, _PP]),

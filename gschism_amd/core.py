@@ -71,6 +71,7 @@ _FIELD_SPEC = {
     L.F_SIDE_U: (np.float64, "ns", (2,)), L.F_SIDE_HHAT: (np.float64, "ns", ()),
     L.F_SIDE_CHI: (np.float64, "ns", ()), L.F_GHAT1: (np.float64, "ne", (2,)),
     L.F_ZC: (np.float64, "ns", ("nvrt",)), L.F_ELBC: (np.float64, "n", ()),
+    L.F_DEPTH: (np.float64, "n", ()),
 }
 
 
@@ -244,6 +245,12 @@ class Solver:
         check(L.lib().gsh_solver_get_rhs(self.h, dptr(b)))
         return b
 
+    def set_rhs(self, b) -> None:
+        v = np.ascontiguousarray(b, dtype=np.float64)
+        if v.size != self.n:
+            raise ValueError("right-hand side has the wrong length")
+        check(L.lib().gsh_solver_set_rhs(self.h, dptr(v)))
+
     def assemble(self, params, time: float = 0.0) -> None:
         if self.mesh is None:
             raise ValueError("solver is not bound to a mesh")
@@ -275,6 +282,13 @@ class Solver:
 
     def set_tuning(self, rows_per_thread: int = 4, inner_rtol: float = 1e-5) -> None:
         check(L.lib().gsh_solver_set_tuning(self.h, rows_per_thread, inner_rtol))
+
+    def autotune(self, iters: int = 30):
+        """Picks rows_per_thread by timing `iters` iterations of each; returns (best, us/iter [4])."""
+        best = C.c_int()
+        us = np.zeros(4, dtype=np.float64)
+        check(L.lib().gsh_solver_autotune(self.h, iters, C.byref(best), dptr(us)))
+        return best.value, us
 
     def set_omega(self, omega: float) -> None:
         check(L.lib().gsh_solver_set_omega(self.h, omega))

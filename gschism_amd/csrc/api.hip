@@ -972,6 +972,7 @@ static FieldRef field_ref(const gsh_mesh* m, int field) {
     case GSH_F_ZC: return {d.zc, ns * nv, 8};
     case GSH_F_ELBC: return {d.elbc, n, 8};
     case GSH_F_WET: return {nullptr, n + ne + ns, 1};
+    case GSH_F_DEPTH: return {const_cast<double*>(d.depth), n, 8};
     default: throw Error(GSH_E_ARG, "unknown field");
   }
 }
@@ -994,6 +995,10 @@ int gsh_mesh_set_field(gsh_mesh* m, int field, const void* host, int64_t count) 
       HIPCHK(hipFreeAsync(tmp, st));
     } else {
       HIPCHK(hipMemcpyAsync(f.ptr, host, (size_t)count * f.elem, hipMemcpyHostToDevice, st));
+    }
+    if (field == GSH_F_DEPTH) {  // the host copy answers GSH_T_DEPTH
+      const double* h = static_cast<const double*>(host);
+      m->hm.depth.assign(h, h + count);
     }
     HIPCHK(hipStreamSynchronize(st));
   })
@@ -1235,6 +1240,53 @@ int gsh_solver_set_tuning(gsh_solver* s, int rows_per_thread, double inner_rtol)
       HIPCHK(hipStreamSynchronize(s->ctx->stream));
       ssor_build(s);
     }
+  })
+}
+
+int gsh_solver_autotune(gsh_solver* s, int iters, int* best_rpt, double* us_per_iter) {
+  GSH_API_BODY({
+    GSH_REQUIRE(s, GSH_E_ARG, "solver is NULL");
+    GSH_REQUIRE(iters >= 2, GSH_E_ARG, "iters must be >= 2");
+    s->ctx->set();
+    hipStream_t st = s->ctx->stream;
+    const int cand[4] = {1, 2, 4, 8};
+    double best = 1e300;
+    int pick = s->rpt;
+    for (int c = 0; c < 4; ++c) {
+      const int r = cand[c];
+      s->rpt = r;
+      s->ntiles = s->tiles_for(s->n);
+      s->hs.ntiles = s->ntiles;
+      s->hs32.ntiles = s->ntiles;
+      double us = 0.0;
+      if (s->precond == GSH_PRECOND_SSOR) {
+        if (!s->ssor || s->ssor->rpt != r) {
+          HIPCHK(hipStreamSynchronize(st));
+          ssor_build(s);
+        }
+        SsorTimes tt;
+        run_ssor(s, 1e-300, iters, iters, 60.0, &tt);
+        us = tt.iters ? 1e3 * (tt.ms_bwd + tt.ms_fwd + tt.ms_upd) / tt.iters : 1e300;
+      } else {
+        solver_sync_desc(s);
+        PcgOutcome o;
+        PhaseTimes pt;
+        run_pcg<double>(s->ctx, s->ds, 1, s->dtiles, s->ntiles, r, 1e-300, iters, iters, 60.0, o, nullptr, -1, &pt);
+        us = pt.launches ? 1e3 * (pt.ms_a + pt.ms_b) / pt.launches : 1e300;
+      }
+      if (us_per_iter) us_per_iter[c] = us;
+      if (us < best) {
+        best = us;
+        pick = r;
+      }
+    }
+    HIPCHK(hipStreamSynchronize(st));
+    s->rpt = pick;
+    s->ntiles = s->tiles_for(s->n);
+    s->hs.ntiles = s->ntiles;
+    s->hs32.ntiles = s->ntiles;
+    if (s->precond == GSH_PRECOND_SSOR && s->ssor->rpt != pick) ssor_build(s);
+    if (best_rpt) *best_rpt = pick;
   })
 }
 
@@ -1942,6 +1994,15 @@ extern "C" int gsh_solver_get_rhs(gsh_solver* s, double* b_host) {
     s->ctx->set();
     HIPCHK(hipStreamSynchronize(s->ctx->stream));
     HIPCHK(hipMemcpy(b_host, s->b, sizeof(double) * s->n, hipMemcpyDeviceToHost));
+  })
+}
+
+extern "C" int gsh_solver_set_rhs(gsh_solver* s, const double* b_host) {
+  GSH_API_BODY({
+    GSH_REQUIRE(s && b_host, GSH_E_ARG, "NULL argument");
+    s->ctx->set();
+    HIPCHK(hipMemcpyAsync(s->b, b_host, sizeof(double) * s->n, hipMemcpyHostToDevice, s->ctx->stream));
+    HIPCHK(hipStreamSynchronize(s->ctx->stream));
   })
 }
 

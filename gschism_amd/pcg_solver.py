@@ -58,18 +58,19 @@ class DeviceArray:
 
 
 class PCGSolverMultiGPU:
-    """Jacobi-preconditioned CG on one MI355X (one row block; numParts must be 1).
+    """Preconditioned CG on one MI355X (one row block; numParts must be 1).
 
     Constructed either on a mesh (the wave-equation pattern, as main.py:281 does after the
     grid is read) or on a caller CSR pattern (indptr, indices). `precision` is "fp64" (the
-    reference arithmetic) or "mixed".
+    reference arithmetic) or "mixed"; `precond` is "jacobi" (the reference's) or "ssor"
+    (multicolour symmetric SOR, FP64).
     """
 
     TPB = 256
     nativeTimeout = 60.0
 
     def __init__(self, configCuda: CudaDeviceConfig, N, NNZ, *, mesh: Mesh | None = None, indptr=None, indices=None,
-                 ctx: Context | None = None, precision: str = "fp64"):
+                 ctx: Context | None = None, precision: str = "fp64", precond: str = "jacobi"):
         N, NNZ = list(N), list(NNZ)
         if len(NNZ) != len(N):
             raise ValueError(f"NNZ must have length numParts={len(N)}, got {len(NNZ)}")
@@ -78,10 +79,11 @@ class PCGSolverMultiGPU:
         if len(N) != 1:
             raise ValueError("this build runs one GPU per solver (numParts must be 1)")
         prec = {"fp64": L.PREC_FP64, "mixed": L.PREC_MIXED}[precision]
+        pc = {"jacobi": L.PRECOND_JACOBI, "ssor": L.PRECOND_SSOR}[precond]
         if mesh is not None:
             if mesh.n != N[0] or mesh.nnz != NNZ[0]:
                 raise ValueError("N/NNZ do not match the mesh pattern")
-            self._solver = Solver(mesh, prec)
+            self._solver = Solver(mesh, prec, precond=pc)
             self._indptr = mesh.topology(L.T_CSR_INDPTR)
             self._indices = mesh.topology(L.T_CSR_INDICES)
         else:
@@ -92,7 +94,8 @@ class PCGSolverMultiGPU:
             self._indices = np.ascontiguousarray(indices, dtype=np.int32)
             if self._indptr.size != N[0] + 1 or self._indices.size != NNZ[0]:
                 raise ValueError("N/NNZ do not match indptr/indices")
-            self._solver = Solver(None, prec, ctx=self._ctx, indptr=self._indptr, indices=self._indices)
+            self._solver = Solver(None, prec, ctx=self._ctx, indptr=self._indptr, indices=self._indices,
+                                  precond=pc)
         self.configCuda = configCuda
         self.numParts = 1
         self.N_m = N
@@ -122,6 +125,7 @@ class PCGSolverMultiGPU:
     def setRHS(self, b) -> None:
         """b for the next solve (the reference gathers it from the mesh's qel, gatherBFromMesh)."""
         self._b = np.ascontiguousarray(b, dtype=np.float64)
+        self._solver.set_rhs(self._b)
 
     # ------------------------------------------------------------------ public API
     def solve(self, rtol=1e-5, maxiter=None, ensureSymmetric=False, checkEvery=10):

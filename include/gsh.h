@@ -137,6 +137,8 @@ int gsh_mesh_get_topology(const gsh_mesh* mesh, int what, void* host_out);
 #define GSH_F_ZC         16   /* double [ns][nvrt]   side-centre z                     */
 #define GSH_F_ELBC       17   /* double [n]          computed boundary elevation       */
 #define GSH_F_WET        18   /* int8   [n]/[ne]/[ns] packed: nodes, then elements, then sides (1 = wet) */
+#define GSH_F_DEPTH      19   /* double [n]          still-water depth h (dp); levels and wet flags follow at
+                                 the next gsh_update_levels */
 int gsh_mesh_set_field(gsh_mesh* mesh, int field, const void* host, int64_t count);
 int gsh_mesh_get_field(const gsh_mesh* mesh, int field, void* host, int64_t count);
 int gsh_mesh_field_ptr(gsh_mesh* mesh, int field, void** dev_ptr, int64_t* count);
@@ -197,6 +199,8 @@ int gsh_solver_get_values(gsh_solver* s, double* data_host);
 int gsh_solver_vec_ptr(gsh_solver* s, double** b, double** x);
 /* Host copy of the right-hand side b (gatherBFromMesh target, pcg_solver.py:950-954). */
 int gsh_solver_get_rhs(gsh_solver* s, double* b_host);
+/* Sets b from the host (PCGSolverMultiGPU.setRHS of the mirror; the next solve uses it). */
+int gsh_solver_set_rhs(gsh_solver* s, const double* b_host);
 /* Host copy of the last solution (scatterXToMesh source, pcg_solver.py:957-961). */
 int gsh_solver_get_solution(gsh_solver* s, double* x_host);
 
@@ -228,6 +232,11 @@ int gsh_solver_last_stats(gsh_solver* s, gsh_solve_stats* out);
 /* Tuning knobs: rows per thread of the PCG tiles (1, 2, 4, 8) and the inner tolerance of the
  * mixed-precision refinement. */
 int gsh_solver_set_tuning(gsh_solver* s, int rows_per_thread, double inner_rtol);
+/* Autotuner (no counterpart in the reference): times `iters` FP64 iterations on the solver's
+ * current system for rows_per_thread 1, 2, 4 and 8 (kernel-stamped events) and keeps the fastest;
+ * us_per_iter[4] (may be NULL) receives the measured microseconds per iteration. The solution
+ * buffer is overwritten, the history of previous solutions is kept. */
+int gsh_solver_autotune(gsh_solver* s, int iters, int* best_rpt, double* us_per_iter);
 /* SSOR relaxation factor omega in (0, 2), default 1 (symmetric Gauss-Seidel). */
 int gsh_solver_set_omega(gsh_solver* s, double omega);
 /* SSOR layout: number of colours, rows after padding every colour block to whole reduction

@@ -487,6 +487,7 @@ static int field_info(const or_mesh* m, int f, void** ptr, size_t* bytes) {
     case 14: *ptr = m->ghat1; *bytes = 16 * ne; return 0;
     case 16: *ptr = m->zc; *bytes = 8 * ns * nv; return 0;
     case 17: *ptr = m->elbc; *bytes = 8 * n; return 0;
+    case 19: *ptr = m->depth; *bytes = 8 * n; return 0;
     default: set_err("unknown field %d", f); return -1;
   }
 }

@@ -96,6 +96,7 @@ _FIELDS = {
     12: (np.float64, lambda m: (m.ns,)), 13: (np.float64, lambda m: (m.ns,)),
     14: (np.float64, lambda m: (m.ne, 2)), 16: (np.float64, lambda m: (m.ns, m.nvrt)),
     17: (np.float64, lambda m: (m.n,)), 18: (np.int8, lambda m: (m.n + m.ne + m.ns,)),
+    19: (np.float64, lambda m: (m.n,)),
 }
 
 _TOPO = {

@@ -153,3 +153,17 @@ def test_ssor_colouring_matches_oracle():
         assert np.array_equal(col, ocol)
     with pytest.raises(L.GshError):
         core.colour_csr(np.array([0, 1], np.int32), np.array([5], np.int32))
+
+
+def test_mirror_constructor_checks():
+    """PCGSolverMultiGPU rejects inconsistent partition arguments before touching a device, as
+    the reference constructor does (pcg_solver.py:547-560)."""
+    from gschism_amd import pcg_solver as ps
+
+    cfg = ps.CudaDeviceConfig([0])
+    with pytest.raises(ValueError):
+        ps.PCGSolverMultiGPU(cfg, [10], [20, 30])
+    with pytest.raises(ValueError):
+        ps.PCGSolverMultiGPU(ps.CudaDeviceConfig([0, 1]), [10], [20])
+    with pytest.raises(ValueError):
+        ps.PCGSolverMultiGPU(ps.CudaDeviceConfig([0, 1]), [10, 10], [20, 20])

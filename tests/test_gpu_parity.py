@@ -233,6 +233,21 @@ def test_batch_equals_solo(ctx):
         assert_bitwise(mesh.get(L.F_ETA2), x, "batched solution")
 
 
+def test_depth_field_and_levels(ctx):
+    """Setting the bathymetry on the device changes levels and wet flags as the oracle's."""
+    spec = cases.CaseSpec("depth-40x30", 40, 30, 500.0, nvrt=6, depth=None, substeps=4)
+    mesh, om, p = _setup(ctx, spec)
+    d = mesh.topology(L.T_DEPTH) * (1.0 + 0.05 * cases.normal(123, mesh.n)) - 3.0
+    mesh.set(L.F_DEPTH, d)
+    om.set(L.F_DEPTH, d)
+    assert_bitwise(mesh.get(L.F_DEPTH), d, "depth readback")
+    assert_bitwise(mesh.topology(L.T_DEPTH), d, "depth topology copy")
+    mesh.update_levels(p)
+    om.levels(p)
+    assert_bitwise(mesh.get(L.F_ZC), om.get(L.F_ZC), "zc after depth change")
+    assert_bitwise(mesh.get(L.F_WET), om.get(L.F_WET), "wet flags after depth change")
+
+
 def test_mixed_batch_equals_solo(ctx):
     """Mixed precision in lock-step: each system's result is the one it gets alone, and within
     1e-8 of the FP64 solution."""

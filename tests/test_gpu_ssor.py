@@ -115,3 +115,24 @@ def test_ssor_standalone_csr_and_special_cases(ctx):
         s.set_omega(2.5)
     with pytest.raises(L.GshError):
         core.Solver(None, L.PREC_MIXED, ctx=ctx, indptr=A.indptr, indices=A.indices, precond=L.PRECOND_SSOR)
+
+
+def test_autotune_keeps_results_exact(ctx):
+    """The autotuner only picks the tile size; the solve afterwards still matches the oracle run
+    with that tile size (Jacobi and SSOR)."""
+    spec = cases.CaseSpec("tune-90x60", 90, 60, 600.0, nvrt=4, depth=None, substeps=4)
+    mesh, om, p, val, b = _assembled(ctx, spec)
+    ip, ix = mesh.topology(L.T_CSR_INDPTR), mesh.topology(L.T_CSR_INDICES)
+    for precond in (L.PRECOND_JACOBI, L.PRECOND_SSOR):
+        s = core.Solver(mesh, precond=precond)
+        s.assemble(p)
+        best, us = s.autotune(12)
+        print(f"[gpu] autotune precond={precond}: best rows_per_thread {best}, us/iter {us}", flush=True)
+        assert best in (1, 2, 4, 8) and (us > 0).all()
+        x, it, st = s.solve(None, rtol=1e-12, maxiter=3000)
+        if precond == L.PRECOND_SSOR:
+            xo, ito, _, _, _ = O.pcg_ssor(ip, ix, val, b, rtol=1e-12, maxiter=3000, rows_per_thread=best)
+        else:
+            xo, ito, _, _ = O.pcg(ip, ix, val, b, rtol=1e-12, maxiter=3000, rows_per_thread=best)
+        assert it == ito
+        assert_bitwise(x, xo, f"solution after autotune (precond {precond})")
