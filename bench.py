@@ -240,7 +240,7 @@ def main() -> int:
     ap.add_argument("--ensemble-nx", type=int, default=708)
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--requests", type=int, default=120)
-    ap.add_argument("--mixed-rtol", type=float, default=1e-10)
+    ap.add_argument("--mixed-rtol", type=float, default=1e-9)
     args = ap.parse_args()
 
     if args.gpus != 1:

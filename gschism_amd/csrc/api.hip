@@ -1537,6 +1537,11 @@ struct gsh_server {
   int dsys32_cap = 0;
   double* dred = nullptr;
   int dred_cap = 0;
+  // admission set-up (backtracking, assembly, casts) runs on its own stream and overlaps the
+  // solve window of the step that admitted the request; the request joins the next step
+  hipStream_t stage = nullptr;
+  hipEvent_t staged_ev = nullptr;
+  std::vector<Request*> staged;
   ~gsh_server() {
     for (auto& v : pool)
       for (auto* s : v) gsh_solver_destroy(s);
@@ -1548,6 +1553,8 @@ struct gsh_server {
     (void)hipFree(dgather);
     (void)hipFree(dsys32);
     (void)hipFree(dred);
+    if (staged_ev) (void)hipEventDestroy(staged_ev);
+    if (stage) (void)hipStreamDestroy(stage);
   }
 };
 
@@ -1593,9 +1600,7 @@ static void server_step_mixed(gsh_server* srv, const std::vector<Request*>& fres
   hipStream_t st = ctx->stream;
   const auto& cfg = srv->cfg;
   for (Request* r : fresh) {
-    gsh_solver* s = r->solver;
-    launch_cast_values(s->val, s->val32, s->nnz, st);
-    launch_fill_f64(s->x, 0.0, s->n, st);
+    gsh_solver* s = r->solver;  // values cast and x cleared at admission
     s->hs32.ntiles = s->ntiles;
     r->phase = 0;
     r->outer = 0;
@@ -1734,6 +1739,8 @@ int gsh_server_create(gsh_ctx* ctx, const gsh_server_config* cfg, gsh_server** o
     size_t f = 0, t = 0;
     HIPCHK(hipMemGetInfo(&f, &t));
     s->budget = cfg->hbm_budget_bytes > 0 ? cfg->hbm_budget_bytes : (int64_t)f;
+    HIPCHK(hipStreamCreateWithFlags(&s->stage, hipStreamNonBlocking));
+    HIPCHK(hipEventCreateWithFlags(&s->staged_ev, hipEventDisableTiming));
     *out = s.release();
   })
 }
@@ -1742,6 +1749,7 @@ int gsh_server_destroy(gsh_server* srv) {
   if (!srv) return GSH_OK;
   (void)hipSetDevice(srv->ctx->device);
   (void)hipStreamSynchronize(srv->ctx->stream);
+  if (srv->stage) (void)hipStreamSynchronize(srv->stage);
   delete srv;
   return GSH_OK;
 }
@@ -1783,13 +1791,20 @@ int gsh_server_step(gsh_server* srv, int32_t* active_out, int32_t* queued_out) {
     ctx->set();
     hipStream_t st = ctx->stream;
     const auto& cfg = srv->cfg;
-    // ---- admission
+    // ---- the requests set up during the previous step join now
     std::vector<Request*> fresh;
+    if (!srv->staged.empty()) {
+      HIPCHK(hipStreamWaitEvent(st, srv->staged_ev, 0));
+      fresh.swap(srv->staged);
+    }
+    // ---- admission; the set-up goes to the staging stream
+    hipStream_t sg = srv->stage;
     while (!srv->queue.empty()) {
       Request* r = srv->queue.front();
-      if (cfg.max_active > 0 && (int)srv->active.size() >= cfg.max_active) break;
+      const int in_flight = (int)(srv->active.size() + fresh.size() + srv->staged.size());
+      if (cfg.max_active > 0 && in_flight >= cfg.max_active) break;
       if (srv->used + r->bytes > srv->budget) {
-        if (srv->active.empty() && fresh.empty()) {  // can never fit
+        if (in_flight == 0) {  // can never fit
           r->status = GSH_E_ADMISSION;
           srv->queue.pop_front();
           continue;
@@ -1807,20 +1822,24 @@ int gsh_server_step(gsh_server* srv, int32_t* active_out, int32_t* queued_out) {
         if (gsh_solver_create(m, cfg.precision, GSH_PRECOND_JACOBI, &s) != GSH_OK)
           throw Error(GSH_E_OOM, std::string("server admission: ") + g_last_error);
       }
-      for (auto* hh : s->h) launch_fill_f64(hh, 0.0, s->n, st);  // each request is a cold start
+      for (auto* hh : s->h) launch_fill_f64(hh, 0.0, s->n, sg);  // each request is a cold start
       if (r->with_bt) {
         gsh_step_params p;
         gsh_step_params_default(&p);
         const StepArgs a = step_args(&p);
-        launch_backtrack(m->d, a, nullptr, nullptr, 0, st);
-        launch_rhs(m->d, a, st);
+        launch_backtrack(m->d, a, nullptr, nullptr, 0, sg);
+        launch_rhs(m->d, a, sg);
       }
       {  // the request's matrix (and right-hand side unless given) from the mesh state
         gsh_step_params p;
         gsh_step_params_default(&p);
-        launch_assemble(m->d, step_args(&p), s->val, s->b, st);
+        launch_assemble(m->d, step_args(&p), s->val, s->b, sg);
       }
-      if (r->has_b) HIPCHK(hipMemcpyAsync(s->b, r->b.data(), sizeof(double) * s->n, hipMemcpyHostToDevice, st));
+      if (r->has_b) HIPCHK(hipMemcpyAsync(s->b, r->b.data(), sizeof(double) * s->n, hipMemcpyHostToDevice, sg));
+      if (cfg.precision == GSH_PREC_MIXED) {
+        launch_cast_values(s->val, s->val32, s->nnz, sg);
+        launch_fill_f64(s->x, 0.0, s->n, sg);
+      }
       s->rpt = 4;
       s->ntiles = s->tiles_for(s->n);
       s->hs.x = s->x;
@@ -1832,8 +1851,9 @@ int gsh_server_step(gsh_server* srv, int32_t* active_out, int32_t* queued_out) {
       r->active = true;
       r->run = 0;
       srv->used += r->bytes;
-      fresh.push_back(r);
+      srv->staged.push_back(r);
     }
+    if (!srv->staged.empty()) HIPCHK(hipEventRecord(srv->staged_ev, sg));
     for (auto* r : fresh) srv->active.push_back(r);
     const int na = cfg.precision == GSH_PREC_MIXED ? 0 : (int)srv->active.size();
     if (cfg.precision == GSH_PREC_MIXED) server_step_mixed(srv, fresh);
@@ -1888,7 +1908,7 @@ int gsh_server_step(gsh_server* srv, int32_t* active_out, int32_t* queued_out) {
       }
       srv->active.swap(keep);
     }
-    if (active_out) *active_out = (int32_t)srv->active.size();
+    if (active_out) *active_out = (int32_t)(srv->active.size() + srv->staged.size());
     if (queued_out) *queued_out = (int32_t)srv->queue.size();
   })
 }

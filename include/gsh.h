@@ -277,8 +277,10 @@ int gsh_server_add_mesh(gsh_server* srv, gsh_mesh* mesh, int32_t* mesh_id_out);
  * b_host = NULL assembles the RHS from the mesh state. */
 int gsh_server_submit(gsh_server* srv, int32_t mesh_id, const double* b_host, int with_backtrack,
                       int64_t* ticket_out);
-/* Advances the service: admits queued requests as memory and slots allow, runs one check
- * window of lock-step PCG iterations for every active request, retires finished ones. */
+/* Advances the service: admits queued requests as memory and slots allow (their device set-up,
+ * i.e. backtracking, assembly and casts, runs on a second stream during this step's solve window
+ * and they join at the next step), runs one check window of lock-step PCG iterations for every
+ * active request, retires finished ones. active_out counts active and admitted requests. */
 int gsh_server_step(gsh_server* srv, int32_t* active_out, int32_t* queued_out);
 /* Result of a finished request: status 0 = done, 1 = pending; x_host may be NULL. */
 int gsh_server_poll(gsh_server* srv, int64_t ticket, double* x_host, int32_t* iters_out,
