@@ -138,6 +138,47 @@ def batched_leg(ctx, L, cases, core, members, nx, ny, rtol, mixed_rtol, check_ev
     return out
 
 
+def mixed_4m_leg(ctx, L, cases, core, steps, rtol, outer_rtol, check_every):
+    """Configuration 3: the 2048 x 2048 log-normal channel (4.2M nodes). FP64 PCG (multicolour
+    SSOR and the reference's Jacobi) to rtol 1e-12 against mixed precision (FP32 inner Jacobi
+    PCG, FP64 dot products, true residual and refinement) to the outer tolerance; relative L2
+    error against the FP64 solution and iteration counts."""
+    spec = cases.spec_mixed_4m()
+    data = cases.make_case(spec)
+    mesh, p, _ = cases.load_mesh(ctx, data)
+    mesh.backtrack(p)
+    mesh.rhs_producers(p)
+    out = {"workload": f"{spec.name} (n={mesh.n}, nnz={mesh.nnz})", "outer_rtol": outer_rtol}
+    ref = None
+    for name, prec, pc, tol in (("fp64_ssor", L.PREC_FP64, L.PRECOND_SSOR, rtol),
+                                ("fp64_jacobi", L.PREC_FP64, L.PRECOND_JACOBI, rtol),
+                                ("mixed_jacobi", L.PREC_MIXED, L.PRECOND_JACOBI, outer_rtol)):
+        s = core.Solver(mesh, prec, precond=pc)
+        s.set_tuning(2 if pc == L.PRECOND_SSOR else 4)
+        s.assemble(p)
+        s.reset_history()
+        x, it, st = s.solve(None, tol, 40000, check_every)
+        ctx.synchronize()
+        t = time.perf_counter()
+        for _ in range(steps):
+            s.reset_history()
+            s.solve(None, tol, 40000, check_every, want_x=False)
+        ctx.synchronize()
+        dt = (time.perf_counter() - t) / steps
+        rec = {"solves_per_s": round(1.0 / dt, 3), "ms_per_solve": round(1e3 * dt, 3), "iters": it, "status": st}
+        if prec == L.PREC_MIXED:
+            stt = s.stats()
+            rec["outer"] = int(stt.outer)
+        if ref is None:
+            ref = x
+        else:
+            rec["rel_l2_vs_fp64_ssor"] = float(np.linalg.norm(x - ref) / np.linalg.norm(ref))
+        out[name] = rec
+        s.close()
+    mesh.close()
+    return out
+
+
 def served_leg(ctx, L, cases, core, n_req, rates, rtol, mixed_rtol, check_every):
     """Served solves: a pool of 6 meshes with n log-uniform in [1e4, 2e6] (random bathymetry,
     seeds 3 + k), requests drawn uniformly from the pool with Poisson arrivals (seed 7) and
@@ -188,7 +229,7 @@ def served_leg(ctx, L, cases, core, n_req, rates, rtol, mixed_rtol, check_every)
         arrive = np.cumsum(expo) / lam
         srv = core.Server(ctx, rtol=mixed_rtol, maxiter=20000, check_every=check_every, precision=L.PREC_MIXED)
         ids = [srv.add_mesh(m) for m in pool]
-        tickets = {}
+        tickets, by_ticket = {}, {}
         done = {}
         t0 = time.perf_counter()
         nxt = 0
@@ -196,16 +237,15 @@ def served_leg(ctx, L, cases, core, n_req, rates, rtol, mixed_rtol, check_every)
             now = time.perf_counter() - t0
             while nxt < n_req and arrive[nxt] <= now:
                 tickets[nxt] = srv.submit(ids[which[nxt]], rhs[nxt], with_backtrack=True)
+                by_ticket[tickets[nxt]] = nxt
                 nxt += 1
             a, q = srv.step()
-            for j, tk in list(tickets.items()):
-                if j in done:
-                    continue
+            for tk in srv.finished():
+                j = by_ticket[tk]
                 st, x, it, lat = srv.poll(tk, pool[which[j]].n)
-                if st == 0:
-                    done[j] = (x, it, lat, time.perf_counter() - t0)
-                elif st != 1:
+                if st != 0:
                     raise RuntimeError(f"request {j} failed with status {st}")
+                done[j] = (x, it, lat, time.perf_counter() - t0)
             if a == 0 and q == 0 and nxt < n_req:
                 time.sleep(max(0.0, min(0.01, arrive[nxt] - (time.perf_counter() - t0))))
         t_end = max(v[3] for v in done.values())
@@ -241,6 +281,7 @@ def main() -> int:
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--requests", type=int, default=120)
     ap.add_argument("--mixed-rtol", type=float, default=1e-9)
+    ap.add_argument("--no-mixed-4m", action="store_true")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -391,6 +432,10 @@ def main() -> int:
                "gpu_bitwise_equal": bool(np.array_equal(xo, x_ssor)), "iters_equal": bool(ito == it_ssor)}
         log(f"[bench] cpu baseline: {cpu}")
 
+    mixed4m = None
+    if not args.no_mixed_4m:
+        mixed4m = mixed_4m_leg(ctx, L, cases, core, 3, rtol, 1e-10, check_every)
+        log(f"[bench] mixed 4M: {mixed4m}")
     batched = None
     if not args.no_batched:
         batched = batched_leg(ctx, L, cases, core, args.members, args.ensemble_nx, args.ensemble_nx, rtol,
@@ -430,6 +475,7 @@ def main() -> int:
         "assembly_ms": round(ms_asm, 3),
         "rhs_ms": round(ms_rhs, 3),
         "mixed": mixed,
+        "mixed_4m": mixed4m,
         "batched": batched,
         "served": served,
         "cpu_baseline": cpu,

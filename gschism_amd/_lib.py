@@ -127,6 +127,7 @@ SIGNATURES = {
     "gsh_server_add_mesh": (_I, [_P, _P, _PI32]),
     "gsh_server_submit": (_I, [_P, _I32, _PD, _I, _PI64]),
     "gsh_server_step": (_I, [_P, _PI32, _PI32]),
+    "gsh_server_finished": (_I, [_P, C.POINTER(C.c_int64), _I32, _PI32]),
     "gsh_server_poll": (_I, [_P, _I64, _PD, _PI32, _PI32, _PD]),
     "gsh_copy_probe": (_I, [_P, _I64, _I, _PD]),
     "gsh_solver_profile": (_I, [_P, _D, _I, _I, _PD, C.POINTER(_I)]),

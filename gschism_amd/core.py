@@ -375,6 +375,13 @@ class Server:
         check(L.lib().gsh_server_step(self.h, C.byref(a), C.byref(q)))
         return a.value, q.value
 
+    def finished(self, cap: int = 1024) -> list[int]:
+        """Tickets retired since the last call."""
+        buf = (C.c_int64 * cap)()
+        cnt = C.c_int32()
+        check(L.lib().gsh_server_finished(self.h, buf, cap, C.byref(cnt)))
+        return list(buf[:cnt.value])
+
     def poll(self, ticket: int, n: int | None = None):
         """(status, x or None, iters, latency_ms); status 0 = done, 1 = pending, else GSH_E_*."""
         x = np.empty(n, dtype=np.float64) if n else None

@@ -143,6 +143,56 @@ struct gsh_mesh {
   int32_t n_open = 0;
 };
 
+// ====================================================================== window graphs
+// A window of PCG launches (check_every iterations and the window-end test) captured once per
+// solver and replayed with one hipGraphLaunch: the kernels read the system descriptor from
+// device memory, so one graph serves every solve of the solver. If capture is not available the
+// window is launched directly.
+struct WindowGraph {
+  hipGraphExec_t exec = nullptr;
+  const void* key = nullptr;
+  int w = 0, rpt = 0;
+  bool failed = false;
+  void reset() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    exec = nullptr;
+    key = nullptr;
+    w = 0;
+  }
+  ~WindowGraph() { reset(); }
+};
+
+template <class F>
+static bool launch_window_graph(WindowGraph& g, hipStream_t st, const void* key, int w, int rpt, F&& issue) {
+  if (g.failed) return false;
+  if (!(g.exec && g.key == key && g.w == w && g.rpt == rpt)) {
+    g.reset();
+    if (hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+      (void)hipGetLastError();
+      g.failed = true;
+      return false;
+    }
+    issue();
+    hipGraph_t graph = nullptr;
+    const hipError_t e = hipStreamEndCapture(st, &graph);
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e2 =
+        (e == hipSuccess && graph) ? hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) : hipErrorUnknown;
+    if (graph) (void)hipGraphDestroy(graph);
+    if (e2 != hipSuccess) {
+      (void)hipGetLastError();
+      g.failed = true;
+      return false;
+    }
+    g.exec = exec;
+    g.key = key;
+    g.w = w;
+    g.rpt = rpt;
+  }
+  HIPCHK(hipGraphLaunch(g.exec, st));
+  return true;
+}
+
 // ====================================================================== solver
 // Device side of the multicolour SSOR solver (built from the solver's CSR pattern).
 struct SsorState {
@@ -181,6 +231,8 @@ struct gsh_solver {
   double omega = 1.0;
   std::vector<int32_t> h_indptr, h_indices;  // host pattern kept for the SSOR layout
   std::unique_ptr<SsorState> ssor;
+  // captured windows (declared after the memory they launch on, so they are destroyed first)
+  WindowGraph wg_jacobi, wg_ssor;
 
   int tiles_for(int rows) const { return (rows + kTpb * rpt - 1) / (kTpb * rpt); }
 };
@@ -371,7 +423,7 @@ template <class VT>
 static void run_pcg(gsh_ctx* ctx, const PcgSys<VT>* dsys, int nsys, const TileRef* dtiles, int ntiles, int rpt,
                     double rtol, int maxiter, int check_every, double timeout_s, PcgOutcome& out,
                     const TileRef* dinit_tiles = nullptr, int ninit_tiles = -1, PhaseTimes* prof = nullptr,
-                    const double* rtols_dev = nullptr) {
+                    const double* rtols_dev = nullptr, WindowGraph* graph = nullptr) {
   hipStream_t st = ctx->stream;
   if (check_every < 1) check_every = 1;
   if (timeout_s <= 0.0) timeout_s = 60.0;
@@ -415,10 +467,14 @@ static void run_pcg(gsh_ctx* ctx, const PcgSys<VT>* dsys, int nsys, const TileRe
         prof->launches += 1;
       }
       for (auto& e : evs) (void)hipEventDestroy(e);
+      launch_pcg_check<VT>(dsys, nsys, st);
     } else {
-      for (int i = 0; i < w; ++i) launch_pcg_iter<VT>(dsys, dtiles, ntiles, rpt, st);
+      auto issue = [&]() {
+        for (int i = 0; i < w; ++i) launch_pcg_iter<VT>(dsys, dtiles, ntiles, rpt, st);
+        launch_pcg_check<VT>(dsys, nsys, st);
+      };
+      if (!(graph && launch_window_graph(*graph, st, dsys, w, rpt, issue))) issue();
     }
-    launch_pcg_check<VT>(dsys, nsys, st);
     issued += w;
     const int slot = window & 1;
     if (inflight.size() == 2) {
@@ -457,7 +513,7 @@ struct SsorTimes {
 // Multicolour SSOR PCG of one system: set-up, windows of check_every iterations (2 * ncolours
 // launches each) closed by the true-residual test, then x = (D~ + U)^-1 y.
 static PcgState run_ssor(gsh_solver* s, double rtol, int maxiter, int check_every, double timeout_s,
-                         SsorTimes* prof = nullptr) {
+                         SsorTimes* prof = nullptr, bool use_graph = false) {
   gsh_ctx* ctx = s->ctx;
   hipStream_t st = ctx->stream;
   if (check_every < 1) check_every = 1;
@@ -509,15 +565,20 @@ static PcgState run_ssor(gsh_solver* s, double rtol, int maxiter, int check_ever
         prof->iters += 1;
       }
       for (auto& x : e) (void)hipEventDestroy(x);
+      launch_ssor_true_rr(ds, NT, rpt, st);
+      launch_ssor_check(ds, st);
     } else {
-      for (int i = 0; i < w; ++i) {
-        bwd();
-        fwd();
-        launch_ssor_update(ds, NT, rpt, st);
-      }
+      auto issue = [&]() {
+        for (int i = 0; i < w; ++i) {
+          bwd();
+          fwd();
+          launch_ssor_update(ds, NT, rpt, st);
+        }
+        launch_ssor_true_rr(ds, NT, rpt, st);
+        launch_ssor_check(ds, st);
+      };
+      if (!(use_graph && launch_window_graph(s->wg_ssor, st, ds, w, rpt, issue))) issue();
     }
-    launch_ssor_true_rr(ds, NT, rpt, st);
-    launch_ssor_check(ds, st);
     issued += w;
     const int slot = window & 1;
     if (inflight.size() == 2) {
@@ -1349,7 +1410,7 @@ int gsh_solver_solve(gsh_solver* s, const double* b_host, double* x_host, double
     if (s->precision == GSH_PREC_MIXED) {
       solve_mixed(s, rtol, maxiter, check_every, timeout_s, &iters, &status);
     } else if (s->precond == GSH_PRECOND_SSOR) {
-      const PcgState r = run_ssor(s, rtol, maxiter, check_every, timeout_s);
+      const PcgState r = run_ssor(s, rtol, maxiter, check_every, timeout_s, nullptr, true);
       iters = iters_of(r, maxiter);
       status = (r.done >= 0) ? 0 : GSH_W_MAXITER;
       s->last.iters = iters;
@@ -1358,7 +1419,8 @@ int gsh_solver_solve(gsh_solver* s, const double* b_host, double* x_host, double
       s->last.atol2 = r.atol2;
     } else {
       PcgOutcome o;
-      run_pcg<double>(s->ctx, s->ds, 1, s->dtiles, s->ntiles, s->rpt, rtol, maxiter, check_every, timeout_s, o);
+      run_pcg<double>(s->ctx, s->ds, 1, s->dtiles, s->ntiles, s->rpt, rtol, maxiter, check_every, timeout_s, o,
+                      nullptr, -1, nullptr, nullptr, &s->wg_jacobi);
       const PcgState& r = o.st[0];
       iters = iters_of(r, maxiter);
       status = (r.done >= 0) ? 0 : GSH_W_MAXITER;
@@ -1512,6 +1574,10 @@ struct Request {
   bool inner_fresh = false;
   int outer = 0, inner_total = 0;
   double atol2 = -1.0, in_rtol = 0.0;
+  hipEvent_t ready = nullptr;  // end of the request's set-up on the staging stream
+  ~Request() {
+    if (ready) (void)hipEventDestroy(ready);
+  }
 };
 }  // namespace
 
@@ -1540,8 +1606,8 @@ struct gsh_server {
   // admission set-up (backtracking, assembly, casts) runs on its own stream and overlaps the
   // solve window of the step that admitted the request; the request joins the next step
   hipStream_t stage = nullptr;
-  hipEvent_t staged_ev = nullptr;
   std::vector<Request*> staged;
+  std::deque<int64_t> finished;  // tickets retired and not yet reported by gsh_server_finished
   ~gsh_server() {
     for (auto& v : pool)
       for (auto* s : v) gsh_solver_destroy(s);
@@ -1553,7 +1619,6 @@ struct gsh_server {
     (void)hipFree(dgather);
     (void)hipFree(dsys32);
     (void)hipFree(dred);
-    if (staged_ev) (void)hipEventDestroy(staged_ev);
     if (stage) (void)hipStreamDestroy(stage);
   }
 };
@@ -1586,6 +1651,7 @@ static void server_retire(gsh_server* srv, Request* r, int32_t iters) {
   r->status = 0;
   r->active = false;
   r->latency_ms = std::chrono::duration<double, std::milli>(Clock::now() - r->t_submit).count();
+  srv->finished.push_back(r->ticket);
   srv->pool[r->mesh_id].push_back(sol);
   r->solver = nullptr;
   srv->used -= r->bytes;
@@ -1740,7 +1806,6 @@ int gsh_server_create(gsh_ctx* ctx, const gsh_server_config* cfg, gsh_server** o
     HIPCHK(hipMemGetInfo(&f, &t));
     s->budget = cfg->hbm_budget_bytes > 0 ? cfg->hbm_budget_bytes : (int64_t)f;
     HIPCHK(hipStreamCreateWithFlags(&s->stage, hipStreamNonBlocking));
-    HIPCHK(hipEventCreateWithFlags(&s->staged_ev, hipEventDisableTiming));
     *out = s.release();
   })
 }
@@ -1791,11 +1856,23 @@ int gsh_server_step(gsh_server* srv, int32_t* active_out, int32_t* queued_out) {
     ctx->set();
     hipStream_t st = ctx->stream;
     const auto& cfg = srv->cfg;
-    // ---- the requests set up during the previous step join now
+    // ---- requests whose set-up has finished join now (the oldest one is waited for when
+    // nothing else could run)
     std::vector<Request*> fresh;
-    if (!srv->staged.empty()) {
-      HIPCHK(hipStreamWaitEvent(st, srv->staged_ev, 0));
-      fresh.swap(srv->staged);
+    {
+      std::vector<Request*> still;
+      for (Request* r : srv->staged) {
+        const hipError_t q = hipEventQuery(r->ready);
+        if (q == hipSuccess || (q == hipErrorNotReady && srv->active.empty() && fresh.empty())) {
+          if (q != hipSuccess) HIPCHK(hipStreamWaitEvent(st, r->ready, 0));
+          fresh.push_back(r);
+        } else if (q == hipErrorNotReady) {
+          still.push_back(r);
+        } else {
+          hip_check(q, "request set-up");
+        }
+      }
+      srv->staged.swap(still);
     }
     // ---- admission; the set-up goes to the staging stream
     hipStream_t sg = srv->stage;
@@ -1851,9 +1928,10 @@ int gsh_server_step(gsh_server* srv, int32_t* active_out, int32_t* queued_out) {
       r->active = true;
       r->run = 0;
       srv->used += r->bytes;
+      if (!r->ready) HIPCHK(hipEventCreateWithFlags(&r->ready, hipEventDisableTiming));
+      HIPCHK(hipEventRecord(r->ready, sg));
       srv->staged.push_back(r);
     }
-    if (!srv->staged.empty()) HIPCHK(hipEventRecord(srv->staged_ev, sg));
     for (auto* r : fresh) srv->active.push_back(r);
     const int na = cfg.precision == GSH_PREC_MIXED ? 0 : (int)srv->active.size();
     if (cfg.precision == GSH_PREC_MIXED) server_step_mixed(srv, fresh);
@@ -1910,6 +1988,18 @@ int gsh_server_step(gsh_server* srv, int32_t* active_out, int32_t* queued_out) {
     }
     if (active_out) *active_out = (int32_t)(srv->active.size() + srv->staged.size());
     if (queued_out) *queued_out = (int32_t)srv->queue.size();
+  })
+}
+
+int gsh_server_finished(gsh_server* srv, int64_t* tickets, int32_t cap, int32_t* count) {
+  GSH_API_BODY({
+    GSH_REQUIRE(srv && tickets && count && cap >= 0, GSH_E_ARG, "bad arguments");
+    int32_t k = 0;
+    while (k < cap && !srv->finished.empty()) {
+      tickets[k++] = srv->finished.front();
+      srv->finished.pop_front();
+    }
+    *count = k;
   })
 }
 

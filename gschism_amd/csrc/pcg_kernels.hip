@@ -570,7 +570,13 @@ template <class VT>
 void launch_pcg_phase(int which, const PcgSys<VT>* sys, const TileRef* tiles, int ntiles, int rpt, hipStream_t s,
                       hipEvent_t e0, hipEvent_t e1) {
   if (ntiles <= 0) return;
-  if (which == 0) {
+  if (!e0 && !e1) {
+    if (which == 0) {
+      GSH_DISPATCH_R(rpt, k_pcg_phase_a, VT, dim3(ntiles), dim3(kTpb), 0, s, sys, tiles);
+    } else {
+      GSH_DISPATCH_R(rpt, k_pcg_phase_b, VT, dim3(ntiles), dim3(kTpb), 0, s, sys, tiles);
+    }
+  } else if (which == 0) {
     GSH_DISPATCH_RX(rpt, k_pcg_phase_a, VT, e0, e1, ntiles, s, sys, tiles);
   } else {
     GSH_DISPATCH_RX(rpt, k_pcg_phase_b, VT, e0, e1, ntiles, s, sys, tiles);

@@ -449,15 +449,29 @@ __global__ void k_ssor_reset(const SsorSys* __restrict__ sys, double rtol, int m
 
 // ------------------------------------------------------------------ launch wrappers
 // Launches KER<rpt>; with events e0/e1 the runtime stamps the kernel's own start and end
-// (hipExtLaunchKernelGGL), which the profiling path reads.
-#define GSH_SSOR_RX(rpt, KER, GRID, E0, E1, STREAM, ...)                                                   \
-  switch (rpt) {                                                                                         \
-    case 1: hipExtLaunchKernelGGL(KER<1>, dim3(GRID), dim3(kTpb), 0, STREAM, E0, E1, 0, __VA_ARGS__); break;  \
-    case 2: hipExtLaunchKernelGGL(KER<2>, dim3(GRID), dim3(kTpb), 0, STREAM, E0, E1, 0, __VA_ARGS__); break;  \
-    case 8: hipExtLaunchKernelGGL(KER<8>, dim3(GRID), dim3(kTpb), 0, STREAM, E0, E1, 0, __VA_ARGS__); break;  \
-    default: hipExtLaunchKernelGGL(KER<4>, dim3(GRID), dim3(kTpb), 0, STREAM, E0, E1, 0, __VA_ARGS__); break; \
+// (hipExtLaunchKernelGGL), which the profiling path reads; without them a plain launch, which
+// stream capture records into window graphs.
+#define GSH_SSOR_EXT(R, KER, GRID, E0, E1, STREAM, ...)                                               \
+  do {                                                                                               \
+    if (E0 || E1)                                                                                    \
+      hipExtLaunchKernelGGL(KER<R>, dim3(GRID), dim3(kTpb), 0, STREAM, E0, E1, 0, __VA_ARGS__);       \
+    else                                                                                             \
+      hipLaunchKernelGGL(KER<R>, dim3(GRID), dim3(kTpb), 0, STREAM, __VA_ARGS__);                     \
+  } while (0)
+#define GSH_SSOR_RX(rpt, KER, GRID, E0, E1, STREAM, ...)                         \
+  switch (rpt) {                                                               \
+    case 1: GSH_SSOR_EXT(1, KER, GRID, E0, E1, STREAM, __VA_ARGS__); break;    \
+    case 2: GSH_SSOR_EXT(2, KER, GRID, E0, E1, STREAM, __VA_ARGS__); break;    \
+    case 8: GSH_SSOR_EXT(8, KER, GRID, E0, E1, STREAM, __VA_ARGS__); break;    \
+    default: GSH_SSOR_EXT(4, KER, GRID, E0, E1, STREAM, __VA_ARGS__); break;   \
   }
-#define GSH_SSOR_R(rpt, KER, GRID, STREAM, ...) GSH_SSOR_RX(rpt, KER, GRID, nullptr, nullptr, STREAM, __VA_ARGS__)
+#define GSH_SSOR_R(rpt, KER, GRID, STREAM, ...)                                          \
+  switch (rpt) {                                                                       \
+    case 1: hipLaunchKernelGGL(KER<1>, dim3(GRID), dim3(kTpb), 0, STREAM, __VA_ARGS__); break;   \
+    case 2: hipLaunchKernelGGL(KER<2>, dim3(GRID), dim3(kTpb), 0, STREAM, __VA_ARGS__); break;   \
+    case 8: hipLaunchKernelGGL(KER<8>, dim3(GRID), dim3(kTpb), 0, STREAM, __VA_ARGS__); break;   \
+    default: hipLaunchKernelGGL(KER<4>, dim3(GRID), dim3(kTpb), 0, STREAM, __VA_ARGS__); break;  \
+  }
 
 void launch_ssor_reset(const SsorSys* sys, double rtol, int maxiter, int check_every, hipStream_t s) {
   hipLaunchKernelGGL(k_ssor_reset, dim3(1), dim3(64), 0, s, sys, rtol, maxiter, check_every);

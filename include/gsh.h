@@ -282,6 +282,9 @@ int gsh_server_submit(gsh_server* srv, int32_t mesh_id, const double* b_host, in
  * and they join at the next step), runs one check window of lock-step PCG iterations for every
  * active request, retires finished ones. active_out counts active and admitted requests. */
 int gsh_server_step(gsh_server* srv, int32_t* active_out, int32_t* queued_out);
+/* Tickets of the requests retired since the last call (up to cap, oldest first), so a caller
+ * polls only finished requests. Requests rejected at admission are reported by gsh_server_poll. */
+int gsh_server_finished(gsh_server* srv, int64_t* tickets, int32_t cap, int32_t* count);
 /* Result of a finished request: status 0 = done, 1 = pending; x_host may be NULL. */
 int gsh_server_poll(gsh_server* srv, int64_t ticket, double* x_host, int32_t* iters_out,
                     int32_t* status_out, double* latency_ms_out);
