@@ -53,20 +53,18 @@ def timed_solves(ctx, solver, steps, rtol, maxiter, check_every):
 
 
 def ssor_bytes(mesh, solver):
-    """Algorithmic bytes per iteration of the SSOR launch kinds (real rows, compulsory traffic):
-    backward sweeps read D~, r, p_old and the upper part (8 B value + 4 B column per entry), gather
-    t once, write p and t (colour 0 also s and q); forward sweeps (colours 1..C-1) read p, t, D~
-    and the lower part, gather s once, write s and q; the update reads y, p, r, q, D~ and writes
-    y and r."""
+    """Algorithmic bytes per iteration of the two SSOR launch kinds (real rows, compulsory
+    traffic; 8 B value + 4 B column per off-diagonal entry, each half of them in one sweep):
+    backward sweeps read D~, r, p, q, s, y, gather t once and write p, q, y, r, t (colour 0 also
+    u and s); forward sweeps (colours 1..C-1) read D~, t, r, gather u once and write u and s."""
     n, nnz = mesh.n, mesh.nnz
     nc, _, col = solver.ssor_info()
     half = (nnz - n) // 2
     n0 = int(np.count_nonzero(col == 0))
     nf = n - n0
-    bwd = 12 * half + 4 * n + 48 * n + 16 * n0
-    fwd = 12 * half + 4 * nf + 48 * nf
-    upd = 56 * n
-    return {"bwd": bwd, "fwd": fwd, "upd": upd, "ncolours": nc, "colour0_rows": n0}
+    bwd = 12 * half + 8 * n + 88 * n + 16 * n0
+    fwd = 12 * half + 48 * nf
+    return {"bwd": bwd, "fwd": fwd, "ncolours": nc, "colour0_rows": n0}
 
 
 def batched_leg(ctx, L, cases, core, members, nx, ny, rtol, mixed_rtol, check_every):
@@ -339,8 +337,7 @@ def main() -> int:
     kinds = ssor_bytes(mesh, solver)
     nc = kinds["ncolours"]
     per = {}
-    for i, (k, kern, launches) in enumerate((("bwd", "k_ssor_bwd", nc), ("fwd", "k_ssor_fwd", nc - 1),
-                                              ("upd", "k_ssor_update", 1))):
+    for i, (k, kern, launches) in enumerate((("bwd", "k_ssor_bwd", nc), ("fwd", "k_ssor_fwd", nc - 1))):
         ms = float(ms_kind[i])
         per[k] = {"kernel": kern, "launches_per_iter": launches, "us_per_iter": round(1e3 * ms, 2),
                   "bytes_per_iter": kinds[k], "gbs": round(kinds[k] / (ms * 1e-3) / 1e9, 1) if ms > 0 else None}

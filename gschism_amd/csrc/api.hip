@@ -371,7 +371,8 @@ static void ssor_build(gsh_solver* s) {
   h.perm = P.upload(L.perm);
   h.lval = P.alloc<double>((int64_t)L.lcol.size());
   h.uval = P.alloc<double>((int64_t)L.ucol.size());
-  for (double** v : {&h.d, &h.dt, &h.kd, &h.bp, &h.y, &h.r, &h.p, &h.t, &h.s, &h.q, &h.w}) *v = P.alloc<double>(L.np);
+  for (double** v : {&h.d, &h.dt, &h.kd, &h.bp, &h.y, &h.r, &h.p, &h.q, &h.t, &h.u, &h.s, &h.w})
+    *v = P.alloc<double>(L.np);
   for (double** v : {&h.part_rr, &h.part_rz, &h.part_pq, &h.part_bb}) *v = P.alloc<double>(h.ntiles);
   h.st = P.alloc<PcgState>(1);
   S->ds = P.alloc<SsorSys>(1);
@@ -527,12 +528,18 @@ static PcgState run_ssor(gsh_solver* s, double rtol, int maxiter, int check_ever
   launch_ssor_setup(ds, NT, rpt, st);
   launch_ssor_residual(ds, NT, rpt, st);
   for (int c = 0; c < C; ++c) launch_ssor_r0(ds, S.tile0[c], S.ntc[c], c, rpt, st);
+  // flags of the sweeps (see ssor_kernels.hip): 1 first, 2 colour 0, 4 last of the iteration, 8 set-up
+  auto bflags = [&](int c) { return (c == C - 1 ? 1 : 0) | (c == 0 ? 2 : 0) | (C == 1 ? 4 : 0); };
+  auto fflags = [&](int c) { return c == C - 1 ? 4 : 0; };
   auto bwd = [&]() {
-    for (int c = C - 1; c >= 0; --c) launch_ssor_bwd(ds, S.tile0[c], S.ntc[c], c, c == C - 1, c == 0, rpt, st);
+    for (int c = C - 1; c >= 0; --c) launch_ssor_bwd(ds, S.tile0[c], S.ntc[c], c, bflags(c), rpt, st);
   };
   auto fwd = [&]() {
-    for (int c = 1; c < C; ++c) launch_ssor_fwd(ds, S.tile0[c], S.ntc[c], c, rpt, st);
+    for (int c = 1; c < C; ++c) launch_ssor_fwd(ds, S.tile0[c], S.ntc[c], c, fflags(c), rpt, st);
   };
+  // set-up pass: z0 = D~ r0, s0 = A^ z0 and the gamma, delta partials of iteration 0
+  for (int c = C - 1; c >= 0; --c) launch_ssor_bwd(ds, S.tile0[c], S.ntc[c], c, bflags(c) | 8, rpt, st);
+  for (int c = 1; c < C; ++c) launch_ssor_fwd(ds, S.tile0[c], S.ntc[c], c, fflags(c) | 8, rpt, st);
   HIPCHK(hipGetLastError());
   int issued = 0, window = 0;
   std::deque<int> inflight;
@@ -546,21 +553,20 @@ static PcgState run_ssor(gsh_solver* s, double rtol, int maxiter, int check_ever
     const int w = std::min(check_every, maxiter - issued);
     if (prof) {
       // every launch stamps its own kernel start and end (hipExtLaunchKernelGGL events)
-      const int nl = 2 * C + 1;
+      const int nl = 2 * C - 1;
       std::vector<hipEvent_t> e(2 * nl);
       for (auto& x : e) HIPCHK(hipEventCreate(&x));
       for (int i = 0; i < w; ++i) {
         int l = 0;
         for (int c = C - 1; c >= 0; --c, ++l)
-          launch_ssor_bwd(ds, S.tile0[c], S.ntc[c], c, c == C - 1, c == 0, rpt, st, e[2 * l], e[2 * l + 1]);
+          launch_ssor_bwd(ds, S.tile0[c], S.ntc[c], c, bflags(c), rpt, st, e[2 * l], e[2 * l + 1]);
         for (int c = 1; c < C; ++c, ++l)
-          launch_ssor_fwd(ds, S.tile0[c], S.ntc[c], c, rpt, st, e[2 * l], e[2 * l + 1]);
-        launch_ssor_update(ds, NT, rpt, st, e[2 * l], e[2 * l + 1]);
-        HIPCHK(hipEventSynchronize(e[2 * l + 1]));
-        for (int m = 0; m <= l; ++m) {
+          launch_ssor_fwd(ds, S.tile0[c], S.ntc[c], c, fflags(c), rpt, st, e[2 * l], e[2 * l + 1]);
+        HIPCHK(hipEventSynchronize(e[2 * (l - 1) + 1]));
+        for (int m = 0; m < l; ++m) {
           float a = 0.f;
           HIPCHK(hipEventElapsedTime(&a, e[2 * m], e[2 * m + 1]));
-          (m < C ? prof->ms_bwd : (m < 2 * C - 1 ? prof->ms_fwd : prof->ms_upd)) += a;
+          (m < C ? prof->ms_bwd : prof->ms_fwd) += a;
         }
         prof->iters += 1;
       }
@@ -572,7 +578,6 @@ static PcgState run_ssor(gsh_solver* s, double rtol, int maxiter, int check_ever
         for (int i = 0; i < w; ++i) {
           bwd();
           fwd();
-          launch_ssor_update(ds, NT, rpt, st);
         }
         launch_ssor_true_rr(ds, NT, rpt, st);
         launch_ssor_check(ds, st);

@@ -81,6 +81,7 @@ struct PcgState {
   double rho[2];    // r.z used by iteration k (slot k & 1)
   double beta;      // SSOR: beta of iteration kk, for the sweeps after the first
   double pad2;
+  double alpha[2];  // SSOR: alpha of iteration k (slot k & 1)
 };
 
 // One linear system of a (possibly batched) PCG launch. VT is the arithmetic type of the
@@ -135,7 +136,7 @@ struct SsorSys {
   double* xout;                              // solution (original order)
   double *lval, *uval;                       // gathered ELL values (0 where unused)
   double *d, *dt, *kd, *bp;                  // diagonal, D/omega, 2 D/omega - D, permuted b
-  double *y, *r, *p, *t, *s, *q, *w;         // iteration vectors [np]
+  double *y, *r, *p, *q, *t, *u, *s, *w;     // iteration vectors [np]: CG y r p q, sweeps t u, s = A^ z
   double *part_rr, *part_rz, *part_pq, *part_bb;
   PcgState* st;
 };
@@ -177,12 +178,12 @@ void launch_ssor_reset(const SsorSys* sys, double rtol, int maxiter, int check_e
 void launch_ssor_setup(const SsorSys* sys, int ntiles, int rpt, hipStream_t s);
 void launch_ssor_residual(const SsorSys* sys, int ntiles, int rpt, hipStream_t s);
 void launch_ssor_r0(const SsorSys* sys, int tile0, int ntc, int c, int rpt, hipStream_t s);
-void launch_ssor_bwd(const SsorSys* sys, int tile0, int ntc, int c, int first, int mid, int rpt, hipStream_t s,
+// flags: 1 first sweep of the iteration, 2 colour 0 (starts the forward sweep), 4 last sweep
+// of the iteration, 8 the set-up pass (no vector update)
+void launch_ssor_bwd(const SsorSys* sys, int tile0, int ntc, int c, int flags, int rpt, hipStream_t s,
                      hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
-void launch_ssor_fwd(const SsorSys* sys, int tile0, int ntc, int c, int rpt, hipStream_t s, hipEvent_t e0 = nullptr,
-                     hipEvent_t e1 = nullptr);
-void launch_ssor_update(const SsorSys* sys, int ntiles, int rpt, hipStream_t s, hipEvent_t e0 = nullptr,
-                        hipEvent_t e1 = nullptr);
+void launch_ssor_fwd(const SsorSys* sys, int tile0, int ntc, int c, int flags, int rpt, hipStream_t s,
+                     hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 void launch_ssor_true_rr(const SsorSys* sys, int ntiles, int rpt, hipStream_t s);
 void launch_ssor_check(const SsorSys* sys, hipStream_t s);
 void launch_ssor_xsolve(const SsorSys* sys, int tile0, int ntc, int c, int rpt, hipStream_t s);

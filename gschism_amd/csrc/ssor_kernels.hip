@@ -2,8 +2,8 @@
 //
 // The reference solves with Jacobi PCG only (pcg_solver.py:933-938). This solver keeps the
 // reference's driver (extrapolated guess, atol = rtol ||b||, the stopping rule on ||b - A x||^2
-// every check_every iterations, the beta / alpha recurrences; pcg_solver.py:881-947,
-// 1075-1124, pcg_loop.cpp:361-413) and preconditions with
+// every check_every iterations; pcg_solver.py:881-947, 1075-1124, pcg_loop.cpp:361-413) and
+// preconditions with
 //   M = (D~ + L) D~^-1 (D~ + U),   D~ = D / omega,
 // in a multicolour order, applied by Eisenstat's trick (SIAM J. Sci. Stat. Comput. 2 (1981)
 // 1-4): CG runs on A^ = (D~ + L)^-1 A (D~ + U)^-1 with the diagonal preconditioner D~ and
@@ -11,12 +11,16 @@
 // so one iteration reads every off-diagonal entry once. x = (D~ + U)^-1 y maps the iterate
 // back; the true residual b - A x is (D~ + L) r.
 //
-// Rows are in colour-major order; every colour is a set of independent rows, so each
-// triangular sweep is one launch per colour:
-//   bwd(C-1) ... bwd(1)   p = D~ r + beta p; t = (p - U t) / D~           (first also: beta)
-//   bwd(0)                 the same, then s = (p - K t) / D~, q = t + s     (K = 2 D~ - D)
-//   fwd(1) ... fwd(C-1)    s = (p - K t - L s) / D~, q = t + s, p.q partials
-//   update                 alpha; y += alpha p; r -= alpha q; r.(D~ r) partials
+// CG runs in the Chronopoulos-Gear form (J. Comput. Appl. Math. 25 (1989) 153-168): with
+// z = D~ r and s = A^ z, gamma = r.z and delta = z.s come from one reduction,
+//   beta = gamma / gamma_prev,  alpha = gamma / (delta - beta gamma / alpha_prev),
+//   p = z + beta p,  q = s + beta q,  y += alpha p,  r -= alpha q,
+// so every vector update is row-local and folds into the next backward sweep. Rows are in
+// colour-major order; every colour is a set of independent rows, so each triangular sweep is
+// one launch per colour and an iteration is 2C - 1 launches:
+//   bwd(C-1) ... bwd(0)   alpha, beta (first); row updates; z = D~ r; t = (z - U t) / D~;
+//                         gamma partials; colour 0 (no lower colours) also u and s below
+//   fwd(1) ... fwd(C-1)   u = (z - K t - L u) / D~, s = t + u (K = 2 D~ - D); delta partials
 // Colour blocks are padded to whole reduction tiles, so each tile's partial is written by
 // exactly one launch and every dot product uses the fixed tile tree of the Jacobi solver.
 // Row sums run over ascending columns from 0.0, in the oracle's order. The two triangular parts
@@ -36,7 +40,7 @@ struct SsorView {
   gptr<const SsorColour> colours;
   gptr<const int32_t> tile_colour, lcol, ucol, lsrc, usrc, dsrc, perm;
   gptr<const double> aval, b, h0, h1, h2;
-  gptr<double> xout, lval, uval, d, dt, kd, bp, y, r, p, t, s, q, w;
+  gptr<double> xout, lval, uval, d, dt, kd, bp, y, r, p, q, t, u, s, w;
   gptr<double> part_rr, part_rz, part_pq, part_bb;
   gptr<PcgState> st;
   bool has_hist, unit_omega;
@@ -74,6 +78,7 @@ __device__ __forceinline__ SsorView sview(const SsorSys* __restrict__ sys) {
   v.r = G(S.r);
   v.p = G(S.p);
   v.t = G(S.t);
+  v.u = G(S.u);
   v.s = G(S.s);
   v.q = G(S.q);
   v.w = G(S.w);
@@ -191,179 +196,166 @@ __global__ void __launch_bounds__(256) k_ssor_residual(const SsorSys* __restrict
     S.t[v] = S.bp[v] - ax;
     S.y[v] = S.dt[v] * xv + au;
     S.p[v] = 0.0;
+    S.q[v] = 0.0;
   }
 }
 
-// r0 = (D~ + L)^-1 r_true for the rows of one colour; partials of r.(D~ r).
+// r0 = (D~ + L)^-1 r_true for the rows of one colour.
 template <int R>
 __global__ void __launch_bounds__(256) k_ssor_r0(const SsorSys* __restrict__ sys, int tile0, int c) {
-  __shared__ double red[4];
   const SsorView S = sview(sys);
   const ColourInfo ci = colour_info(S, c);
   const int tile = tile0 + blockIdx.x;
   const gptr<const double> r = C(S.r);
-  double rz = 0.0;
   for (int j = 0; j < R; ++j) {
     const int v = tile_row<R>(tile, j);
-    const double dtv = S.dt[v];
-    const double rv = (S.t[v] - lower_sum(S, ci, v, r)) / dtv;
-    S.r[v] = rv;
-    const double zv = dtv * rv;
-    rz += rv * zv;
+    S.r[v] = (S.t[v] - lower_sum(S, ci, v, r)) / S.dt[v];
   }
-  rz = block_tree(rz, red);
-  if (threadIdx.x == 0) S.part_rz[tile] = rz;
 }
 
 // ------------------------------------------------------------------ backward sweep
-// t = (D~ + U)^-1 p for one colour, with p = z + beta p (f_update_p, pcg_solver.py:208-235)
-// formed on the way. The first sweep of an iteration reduces r.z, forms beta and, in its
-// workgroup 0, keeps the scalar state: atol at k = 0 (pcg_solver.py:928-930, 1089-1095),
-// rho, kk and beta for the later sweeps. The sweep of colour 0 (no lower colours) also does
-// the forward step of its rows: s = (p - K t) / D~, q = t + s, p.q partials.
+constexpr int kFirst = 1, kMid = 2, kLast = 4, kInit = 8;
+
+// One colour of the backward sweep. Outside the set-up pass the first sweep of an iteration
+// (workgroup 0) turns the gamma and delta partials into alpha and beta and keeps them with
+// gamma and the iteration index; every sweep then applies p = z + beta p, q = s + beta q,
+// y += alpha p, r -= alpha q to its rows before t = (z - U t) / D~ with the new z = D~ r. The
+// sweep of colour 0 (no lower colours) also forms u and s = t + u for its rows.
 template <int R>
-__global__ void __launch_bounds__(256) k_ssor_bwd(const SsorSys* __restrict__ sys, int tile0, int c, int first,
-                                                  int mid) {
+__global__ void __launch_bounds__(256) k_ssor_bwd(const SsorSys* __restrict__ sys, int tile0, int c, int flags) {
   __shared__ double red[4];
   const SsorView S = sview(sys);
   gptr<PcgState> st = S.st;
   if (load_done(st) >= 0) return;
+  const bool init = flags & kInit, mid = flags & kMid;
   const ColourInfo ci = colour_info(S, c);
   const int tile = tile0 + blockIdx.x;
   const gptr<const double> t = C(S.t);
-  double usum[R], dtv[R], rv[R], pv[R];
+  double usum[R], dtv[R], rv[R], pv[R], qv[R], sv[R], yv[R];
 #pragma unroll
   for (int j = 0; j < R; ++j) {
     const int v = tile_row<R>(tile, j);
     dtv[j] = S.dt[v];
     rv[j] = S.r[v];
-    pv[j] = S.p[v];
-    usum[j] = upper_sum(S, ci, v, t);
-  }
-  double beta;
-  if (first) {
-    const int k = st->k;
-    const double rz_cur = final_reduce(C(S.part_rz), S.ntiles, red);
-    beta = 0.0;
-    if (k > 0) {
-      const double rz_old = st->rho[(k - 1) & 1];
-      beta = (rz_old == 0.0) ? 0.0 : rz_cur / rz_old;
+    if (!init) {
+      pv[j] = S.p[v];
+      qv[j] = S.q[v];
+      sv[j] = S.s[v];
+      yv[j] = S.y[v];
     }
-    if (blockIdx.x == 0) {
+  }
+  double alpha = 0.0, beta = 0.0;
+  if (!init) {
+    if (flags & kFirst) {
+      const int k = st->k;
+      const double gamma = final_reduce(C(S.part_rz), S.ntiles, red);
+      const double delta = final_reduce(C(S.part_pq), S.ntiles, red);
       if (k == 0) {
-        const double bb = final_reduce(C(S.part_bb), S.ntiles, red);
-        if (threadIdx.x == 0) {
-          const double atol = st->rtol * sqrt(bb);
-          st->bb = bb;
-          st->atol2 = atol * atol;
-          if (atol == 0.0) {
-            st->zero = 1;
-            st->done = 0;
+        alpha = (delta != 0.0) ? gamma / delta : 0.0;
+        if (blockIdx.x == 0) {
+          const double bb = final_reduce(C(S.part_bb), S.ntiles, red);
+          if (threadIdx.x == 0) {
+            const double atol = st->rtol * sqrt(bb);
+            st->bb = bb;
+            st->atol2 = atol * atol;
+            if (atol == 0.0) {
+              st->zero = 1;
+              st->done = 0;
+            }
           }
         }
+      } else {
+        const double gprev = st->rho[(k - 1) & 1], aprev = st->alpha[(k - 1) & 1];
+        beta = (gprev != 0.0) ? gamma / gprev : 0.0;
+        const double den = delta - beta * gamma / aprev;
+        alpha = (aprev != 0.0 && den != 0.0) ? gamma / den : 0.0;
       }
-      if (threadIdx.x == 0) {
-        st->rho[k & 1] = rz_cur;
-        st->kk = k;
+      if (blockIdx.x == 0 && threadIdx.x == 0) {
+        st->rho[k & 1] = gamma;
+        st->alpha[k & 1] = alpha;
         st->beta = beta;
+        st->kk = k;
       }
+    } else {
+      const int kk = st->kk;
+      alpha = st->alpha[kk & 1];
+      beta = st->beta;
     }
-  } else {
-    beta = st->beta;
   }
-  double pq = 0.0;
+#pragma unroll
+  for (int j = 0; j < R; ++j) usum[j] = upper_sum(S, ci, tile_row<R>(tile, j), t);
+  double g = 0.0, dl = 0.0;
 #pragma unroll
   for (int j = 0; j < R; ++j) {
     const int v = tile_row<R>(tile, j);
-    const double pn = dtv[j] * rv[j] + beta * pv[j];
-    const double tv = (pn - usum[j]) / dtv[j];
-    S.p[v] = pn;
+    double r1 = rv[j];
+    if (!init) {
+      const double zr = dtv[j] * rv[j];
+      const double pn = zr + beta * pv[j];
+      const double qn = sv[j] + beta * qv[j];
+      S.p[v] = pn;
+      S.q[v] = qn;
+      S.y[v] = yv[j] + alpha * pn;
+      r1 = rv[j] - alpha * qn;
+      S.r[v] = r1;
+    }
+    const double zn = dtv[j] * r1;
+    g += r1 * zn;
+    const double tv = (zn - usum[j]) / dtv[j];
     S.t[v] = tv;
     if (mid) {
       const double kdv = S.unit_omega ? dtv[j] : S.kd[v];
-      const double sv = ((pn - kdv * tv) - 0.0) / dtv[j];
-      S.s[v] = sv;
-      const double qv = tv + sv;
-      S.q[v] = qv;
-      pq += pn * qv;
+      const double uv = ((zn - kdv * tv) - 0.0) / dtv[j];
+      S.u[v] = uv;
+      const double sn = tv + uv;
+      S.s[v] = sn;
+      dl += zn * sn;
     }
   }
+  g = block_tree(g, red);
+  if (threadIdx.x == 0) S.part_rz[tile] = g;
   if (mid) {
-    pq = block_tree(pq, red);
-    if (threadIdx.x == 0) S.part_pq[tile] = pq;
+    dl = block_tree(dl, red);
+    if (threadIdx.x == 0) S.part_pq[tile] = dl;
+    if ((flags & kLast) && !init && blockIdx.x == 0 && threadIdx.x == 0) st->k = st->kk + 1;
   }
 }
 
 // ------------------------------------------------------------------ forward sweep
-// s = (D~ + L)^-1 (p - K t) for one colour, q = t + s = A^ p, partials of p.q (f_dot).
+// u = (D~ + L)^-1 (z - K t) for one colour, s = t + u = A^ z, partials of z.s. The last sweep of
+// an iteration advances the iteration counter.
 template <int R>
-__global__ void __launch_bounds__(256) k_ssor_fwd(const SsorSys* __restrict__ sys, int tile0, int c) {
+__global__ void __launch_bounds__(256) k_ssor_fwd(const SsorSys* __restrict__ sys, int tile0, int c, int flags) {
   __shared__ double red[4];
   const SsorView S = sview(sys);
   if (load_done(S.st) >= 0) return;
   const ColourInfo ci = colour_info(S, c);
   const int tile = tile0 + blockIdx.x;
-  const gptr<const double> s = C(S.s);
-  double lsum[R], dtv[R], tv[R], pv[R], kdv[R];
+  const gptr<const double> u = C(S.u);
+  double lsum[R], dtv[R], tv[R], rv[R], kdv[R];
 #pragma unroll
   for (int j = 0; j < R; ++j) {
     const int v = tile_row<R>(tile, j);
     dtv[j] = S.dt[v];
     tv[j] = S.t[v];
-    pv[j] = S.p[v];
+    rv[j] = S.r[v];
     kdv[j] = S.unit_omega ? dtv[j] : S.kd[v];
-    lsum[j] = lower_sum(S, ci, v, s);
+    lsum[j] = lower_sum(S, ci, v, u);
   }
-  double pq = 0.0;
+  double dl = 0.0;
 #pragma unroll
   for (int j = 0; j < R; ++j) {
     const int v = tile_row<R>(tile, j);
-    const double sv = ((pv[j] - kdv[j] * tv[j]) - lsum[j]) / dtv[j];
-    S.s[v] = sv;
-    const double qv = tv[j] + sv;
-    S.q[v] = qv;
-    pq += pv[j] * qv;
+    const double zn = dtv[j] * rv[j];
+    const double uv = ((zn - kdv[j] * tv[j]) - lsum[j]) / dtv[j];
+    S.u[v] = uv;
+    const double sn = tv[j] + uv;
+    S.s[v] = sn;
+    dl += zn * sn;
   }
-  pq = block_tree(pq, red);
-  if (threadIdx.x == 0) S.part_pq[tile] = pq;
-}
-
-// ------------------------------------------------------------------ update
-// f_update_xrz (pcg_solver.py:253-276) in the transformed variables: alpha = rz / pq (0 if
-// pq == 0); y += alpha p; r -= alpha q; z = D~ r; partials of r.z.
-template <int R>
-__global__ void __launch_bounds__(256) k_ssor_update(const SsorSys* __restrict__ sys) {
-  __shared__ double red[4];
-  const SsorView S = sview(sys);
-  gptr<PcgState> st = S.st;
-  if (load_done(st) >= 0) return;
-  const int kk = st->kk;
-  double yv[R], pv[R], rv[R], qv[R], dv[R];
-#pragma unroll
-  for (int j = 0; j < R; ++j) {
-    const int v = tile_row<R>(blockIdx.x, j);
-    yv[j] = S.y[v];
-    pv[j] = S.p[v];
-    rv[j] = S.r[v];
-    qv[j] = S.q[v];
-    dv[j] = S.dt[v];
-  }
-  const double pq = final_reduce(C(S.part_pq), S.ntiles, red);
-  const double rz = st->rho[kk & 1];
-  const double alpha = (pq != 0.0) ? rz / pq : 0.0;
-  if (blockIdx.x == 0 && threadIdx.x == 0) st->k = kk + 1;
-  double rzs = 0.0;
-#pragma unroll
-  for (int j = 0; j < R; ++j) {
-    const int v = tile_row<R>(blockIdx.x, j);
-    S.y[v] = yv[j] + alpha * pv[j];
-    const double ri = rv[j] - alpha * qv[j];
-    S.r[v] = ri;
-    const double zi = dv[j] * ri;
-    rzs += ri * zi;
-  }
-  rzs = block_tree(rzs, red);
-  if (threadIdx.x == 0) S.part_rz[blockIdx.x] = rzs;
+  dl = block_tree(dl, red);
+  if (threadIdx.x == 0) S.part_pq[tile] = dl;
+  if ((flags & kLast) && !(flags & kInit) && blockIdx.x == 0 && threadIdx.x == 0) S.st->k = S.st->kk + 1;
 }
 
 // ------------------------------------------------------------------ stopping test
@@ -445,6 +437,8 @@ __global__ void k_ssor_reset(const SsorSys* __restrict__ sys, double rtol, int m
   st->rho[0] = 0.0;
   st->rho[1] = 0.0;
   st->beta = 0.0;
+  st->alpha[0] = 0.0;
+  st->alpha[1] = 0.0;
 }
 
 // ------------------------------------------------------------------ launch wrappers
@@ -485,16 +479,13 @@ void launch_ssor_residual(const SsorSys* sys, int ntiles, int rpt, hipStream_t s
 void launch_ssor_r0(const SsorSys* sys, int tile0, int ntc, int c, int rpt, hipStream_t s) {
   if (ntc > 0) GSH_SSOR_R(rpt, k_ssor_r0, ntc, s, sys, tile0, c);
 }
-void launch_ssor_bwd(const SsorSys* sys, int tile0, int ntc, int c, int first, int mid, int rpt, hipStream_t s,
+void launch_ssor_bwd(const SsorSys* sys, int tile0, int ntc, int c, int flags, int rpt, hipStream_t s,
                      hipEvent_t e0, hipEvent_t e1) {
-  if (ntc > 0) GSH_SSOR_RX(rpt, k_ssor_bwd, ntc, e0, e1, s, sys, tile0, c, first, mid);
+  if (ntc > 0) GSH_SSOR_RX(rpt, k_ssor_bwd, ntc, e0, e1, s, sys, tile0, c, flags);
 }
-void launch_ssor_fwd(const SsorSys* sys, int tile0, int ntc, int c, int rpt, hipStream_t s, hipEvent_t e0,
-                     hipEvent_t e1) {
-  if (ntc > 0) GSH_SSOR_RX(rpt, k_ssor_fwd, ntc, e0, e1, s, sys, tile0, c);
-}
-void launch_ssor_update(const SsorSys* sys, int ntiles, int rpt, hipStream_t s, hipEvent_t e0, hipEvent_t e1) {
-  if (ntiles > 0) GSH_SSOR_RX(rpt, k_ssor_update, ntiles, e0, e1, s, sys);
+void launch_ssor_fwd(const SsorSys* sys, int tile0, int ntc, int c, int flags, int rpt, hipStream_t s,
+                     hipEvent_t e0, hipEvent_t e1) {
+  if (ntc > 0) GSH_SSOR_RX(rpt, k_ssor_fwd, ntc, e0, e1, s, sys, tile0, c, flags);
 }
 void launch_ssor_true_rr(const SsorSys* sys, int ntiles, int rpt, hipStream_t s) {
   if (ntiles > 0) GSH_SSOR_R(rpt, k_ssor_true_rr, ntiles, s, sys);

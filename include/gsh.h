@@ -296,7 +296,7 @@ int gsh_server_poll(gsh_server* srv, int64_t ticket, double* x_host, int32_t* it
  * arithmetic as gsh_solver_solve) with HIP events around the launches of every iteration;
  * ms[4] receives the mean time per iteration of each launch kind:
  *   Jacobi: ms[0] phase A (p update + SpMV + p.q), ms[1] phase B (x, r, z updates + r.r, r.z);
- *   SSOR:   ms[0] backward sweeps (all colours), ms[1] forward sweeps, ms[2] vector update. */
+ *   SSOR:   ms[0] backward sweeps (all colours, with the vector updates), ms[1] forward sweeps. */
 int gsh_copy_probe(gsh_ctx* ctx, int64_t bytes, int reps, double* ms_avg);
 int gsh_solver_profile(gsh_solver* s, double rtol, int maxiter, int check_every, double* ms, int* iters_out);
 

@@ -1259,8 +1259,11 @@ double or_true_rel(int32_t n, const int32_t* indptr, const int32_t* indices, con
 /* ------------------------------------------------------------------ multicolour SSOR PCG
  * The reference has only the Jacobi preconditioner (pcg_solver.py:933-938); the SSOR variant
  * keeps its driver (extrapolated guess, atol = rtol ||b||, stopping rule checked every
- * check_every iterations on ||b - A x||^2, beta/alpha recurrences, pcg_solver.py:881-947,
- * 1075-1124, pcg_loop.cpp:361-413) and replaces the preconditioner by symmetric SOR
+ * check_every iterations on ||b - A x||^2, pcg_solver.py:881-947, 1075-1124,
+ * pcg_loop.cpp:361-413), iterates with the Chronopoulos-Gear form of CG (A. T. Chronopoulos and
+ * C. W. Gear, J. Comput. Appl. Math. 25 (1989) 153-168: gamma = r.z and delta = z.Az in one
+ * reduction, beta = gamma/gamma_prev, alpha = gamma/(delta - beta gamma/alpha_prev), q = Az +
+ * beta q), and replaces the preconditioner by symmetric SOR
  *   M = (D/w + L) (D/w)^-1 (D/w + U)
  * in a multicolour ordering, applied with Eisenstat's trick (S. C. Eisenstat, "Efficient
  * implementation of a class of preconditioned conjugate gradient methods", SIAM J. Sci. Stat.
@@ -1543,6 +1546,7 @@ int32_t or_ssor_solve(or_ssor* L, const double* val, const double* b, const doub
   double *z = xcalloc(np + 1, sizeof(double)), *p = xcalloc(np + 1, sizeof(double));
   double *t = xcalloc(np + 1, sizeof(double)), *s = xcalloc(np + 1, sizeof(double));
   double *q = xcalloc(np + 1, sizeof(double)), *w = xcalloc(np + 1, sizeof(double));
+  double* q2 = xcalloc(np + 1, sizeof(double));
 #ifdef _OPENMP
 #pragma omp parallel for schedule(static)
 #endif
@@ -1580,52 +1584,59 @@ int32_t or_ssor_solve(or_ssor* L, const double* val, const double* b, const doub
 #endif
     for (int32_t v = L->cstart[c]; v < L->cstart[c + 1]; ++v) r[v] = (w[v] - lsum(L, v, r)) / dt[v];
   }
-  for (int32_t v = 0; v < np; ++v) {
-    z[v] = dt[v] * r[v];
-    p[v] = 0.0;
-  }
-  double rz_cur = canon_dot(np, r, z, R, part), rz_old = 0.0;
+  /* Chronopoulos-Gear recurrences: one global reduction (gamma = r.z, delta = z.A^z) per
+   * iteration, so the vector updates fold into the next backward sweep. z = D~ r. */
+  for (int32_t v = 0; v < np; ++v) p[v] = 0.0, q[v] = 0.0;
+  double alpha = 0.0, beta = 0.0, gamma_prev = 0.0, alpha_prev = 0.0;
   result = maxiter;
-  for (int32_t it = 0; it < maxiter; ++it) {
-    const int k = it;
-    const double beta = (k == 0 || rz_old == 0.0) ? 0.0 : rz_cur / rz_old;
-    /* t = (D~+U)^-1 p, p = z + beta p: last colour first */
+  for (int32_t it = -1; it < maxiter; ++it) {
+    const int update = it >= 0;
+    if (update) {
+      const double gamma = canon_dot(np, r, z, R, part), delta = canon_dot(np, z, s, R, part);
+      if (it == 0) {
+        beta = 0.0;
+        alpha = (delta != 0.0) ? gamma / delta : 0.0;
+      } else {
+        beta = (gamma_prev != 0.0) ? gamma / gamma_prev : 0.0;
+        const double den = delta - beta * gamma / alpha_prev;
+        alpha = (alpha_prev != 0.0 && den != 0.0) ? gamma / den : 0.0;
+      }
+      gamma_prev = gamma;
+      alpha_prev = alpha;
+    }
+    /* backward sweep, last colour first: p = z + beta p, q = A^z + beta q, y += alpha p,
+     * r -= alpha q (iterations only), then z = D~ r and t = (D~+U)^-1 z */
     for (int32_t c = C - 1; c >= 0; --c)
 #ifdef _OPENMP
 #pragma omp parallel for schedule(static)
 #endif
       for (int32_t v = L->cstart[c]; v < L->cstart[c + 1]; ++v) {
-        p[v] = dt[v] * r[v] + beta * p[v];
-        t[v] = (p[v] - usum(L, v, t)) / dt[v];
+        if (update) {
+          const double zr = dt[v] * r[v];
+          p[v] = zr + beta * p[v];
+          q[v] = s[v] + beta * q[v];
+          y[v] = y[v] + alpha * p[v];
+          r[v] = r[v] - alpha * q[v];
+        }
+        z[v] = dt[v] * r[v];
+        t[v] = (z[v] - usum(L, v, t)) / dt[v];
       }
-    /* s = (D~+L)^-1 (p - (2D~ - D) t), q = t + s: first colour first */
+    /* forward sweep, first colour first: u = (D~+L)^-1 (z - (2D~ - D) t), A^z = t + u */
     for (int32_t c = 0; c < C; ++c)
 #ifdef _OPENMP
 #pragma omp parallel for schedule(static)
 #endif
       for (int32_t v = L->cstart[c]; v < L->cstart[c + 1]; ++v) {
-        s[v] = ((p[v] - kd[v] * t[v]) - lsum(L, v, s)) / dt[v];
-        q[v] = t[v] + s[v];
+        w[v] = ((z[v] - kd[v] * t[v]) - lsum(L, v, w)) / dt[v];
+        s[v] = t[v] + w[v];
       }
-    const double pq = canon_dot(np, p, q, R, part);
-    const double alpha = (pq != 0.0) ? rz_cur / pq : 0.0;
+    if (update && ((it + 1) % checkEvery == 0 || it == maxiter - 1)) {
+      /* ||b - A x||^2 = ||(D~+L) r||^2 */
 #ifdef _OPENMP
 #pragma omp parallel for schedule(static)
 #endif
-    for (int32_t v = 0; v < np; ++v) {
-      y[v] += alpha * p[v];
-      r[v] = r[v] - alpha * q[v];
-      z[v] = dt[v] * r[v];
-    }
-    const double rz_new = canon_dot(np, r, z, R, part);
-    rz_old = rz_cur;
-    rz_cur = rz_new;
-    if ((it + 1) % checkEvery == 0 || it == maxiter - 1) {
-#ifdef _OPENMP
-#pragma omp parallel for schedule(static)
-#endif
-      for (int32_t v = 0; v < np; ++v) w[v] = dt[v] * r[v] + lsum(L, v, r);
-      const double rr = canon_dot(np, w, w, R, part);
+      for (int32_t v = 0; v < np; ++v) q2[v] = dt[v] * r[v] + lsum(L, v, r);
+      const double rr = canon_dot(np, q2, q2, R, part);
       *rr_last = rr;
       if (rr <= atol2) {
         result = it + 1;
@@ -1655,6 +1666,7 @@ done:
   free(s);
   free(q);
   free(w);
+  free(q2);
   return result;
 }
 
