@@ -373,7 +373,8 @@ static void ssor_build(gsh_solver* s) {
   h.uval = P.alloc<double>((int64_t)L.ucol.size());
   for (double** v : {&h.d, &h.dt, &h.kd, &h.bp, &h.y, &h.r, &h.p, &h.q, &h.t, &h.u, &h.s, &h.w})
     *v = P.alloc<double>(L.np);
-  for (double** v : {&h.part_rr, &h.part_rz, &h.part_pq, &h.part_bb}) *v = P.alloc<double>(h.ntiles);
+  for (double** v : {&h.part_rr, &h.part_bb}) *v = P.alloc<double>(h.ntiles);
+  for (double** v : {&h.part_rz, &h.part_pq}) *v = P.alloc<double>(2 * (int64_t)h.ntiles);  // two iteration slots
   h.st = P.alloc<PcgState>(1);
   S->ds = P.alloc<SsorSys>(1);
   S->ntiles = h.ntiles;

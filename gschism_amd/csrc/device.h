@@ -137,7 +137,8 @@ struct SsorSys {
   double *lval, *uval;                       // gathered ELL values (0 where unused)
   double *d, *dt, *kd, *bp;                  // diagonal, D/omega, 2 D/omega - D, permuted b
   double *y, *r, *p, *q, *t, *u, *s, *w;     // iteration vectors [np]: CG y r p q, sweeps t u, s = A^ z
-  double *part_rr, *part_rz, *part_pq, *part_bb;
+  double *part_rr, *part_bb;                 // [ntiles]
+  double *part_rz, *part_pq;                 // [2][ntiles]: gamma and delta partials, slot = iteration & 1
   PcgState* st;
 };
 

@@ -244,12 +244,16 @@ __global__ void __launch_bounds__(256) k_ssor_bwd(const SsorSys* __restrict__ sy
       yv[j] = S.y[v];
     }
   }
+  // the gamma / delta partials of iteration k live in slot k & 1, so the first sweep can read
+  // slot k & 1 while the sweeps write the partials of iteration k + 1 into the other slot
   double alpha = 0.0, beta = 0.0;
+  int wslot = 0;
   if (!init) {
     if (flags & kFirst) {
       const int k = st->k;
-      const double gamma = final_reduce(C(S.part_rz), S.ntiles, red);
-      const double delta = final_reduce(C(S.part_pq), S.ntiles, red);
+      wslot = (k + 1) & 1;
+      const double gamma = final_reduce(C(S.part_rz) + (k & 1) * S.ntiles, S.ntiles, red);
+      const double delta = final_reduce(C(S.part_pq) + (k & 1) * S.ntiles, S.ntiles, red);
       if (k == 0) {
         alpha = (delta != 0.0) ? gamma / delta : 0.0;
         if (blockIdx.x == 0) {
@@ -280,6 +284,7 @@ __global__ void __launch_bounds__(256) k_ssor_bwd(const SsorSys* __restrict__ sy
       const int kk = st->kk;
       alpha = st->alpha[kk & 1];
       beta = st->beta;
+      wslot = (kk + 1) & 1;
     }
   }
 #pragma unroll
@@ -313,10 +318,10 @@ __global__ void __launch_bounds__(256) k_ssor_bwd(const SsorSys* __restrict__ sy
     }
   }
   g = block_tree(g, red);
-  if (threadIdx.x == 0) S.part_rz[tile] = g;
+  if (threadIdx.x == 0) S.part_rz[wslot * S.ntiles + tile] = g;
   if (mid) {
     dl = block_tree(dl, red);
-    if (threadIdx.x == 0) S.part_pq[tile] = dl;
+    if (threadIdx.x == 0) S.part_pq[wslot * S.ntiles + tile] = dl;
     if ((flags & kLast) && !init && blockIdx.x == 0 && threadIdx.x == 0) st->k = st->kk + 1;
   }
 }
@@ -331,6 +336,7 @@ __global__ void __launch_bounds__(256) k_ssor_fwd(const SsorSys* __restrict__ sy
   if (load_done(S.st) >= 0) return;
   const ColourInfo ci = colour_info(S, c);
   const int tile = tile0 + blockIdx.x;
+  const int wslot = (flags & kInit) ? 0 : ((S.st->kk + 1) & 1);
   const gptr<const double> u = C(S.u);
   double lsum[R], dtv[R], tv[R], rv[R], kdv[R];
 #pragma unroll
@@ -354,7 +360,7 @@ __global__ void __launch_bounds__(256) k_ssor_fwd(const SsorSys* __restrict__ sy
     dl += zn * sn;
   }
   dl = block_tree(dl, red);
-  if (threadIdx.x == 0) S.part_pq[tile] = dl;
+  if (threadIdx.x == 0) S.part_pq[wslot * S.ntiles + tile] = dl;
   if ((flags & kLast) && !(flags & kInit) && blockIdx.x == 0 && threadIdx.x == 0) S.st->k = S.st->kk + 1;
 }
 

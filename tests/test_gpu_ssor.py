@@ -136,3 +136,29 @@ def test_autotune_keeps_results_exact(ctx):
             xo, ito, _, _ = O.pcg(ip, ix, val, b, rtol=1e-12, maxiter=3000, rows_per_thread=best)
         assert it == ito
         assert_bitwise(x, xo, f"solution after autotune (precond {precond})")
+
+
+def test_ssor_2m_matches_oracle(ctx):
+    """Two million rows: thousands of workgroups per sweep, so every partial a sweep reads must be
+    the previous iteration's (the sweeps write the next iteration's into the other slot)."""
+    spec = cases.CaseSpec("channel-1448x1448-lognormal", 1448, 1448, 1000.0, nvrt=4, depth=None, substeps=10)
+    data = cases.make_case(spec)
+    print(f"[gpu] SSOR 2M: building {spec.name}", flush=True)
+    mesh, p, _ = cases.load_mesh(ctx, data)
+    mesh.backtrack(p)
+    mesh.rhs_producers(p)
+    ip, ix = mesh.topology(L.T_CSR_INDPTR), mesh.topology(L.T_CSR_INDICES)
+    for rpt in (1, 2):
+        print(f"[gpu] SSOR 2M rows_per_thread={rpt}", flush=True)
+        s = core.Solver(mesh, precond=L.PRECOND_SSOR)
+        s.set_tuning(rpt)
+        s.assemble(p)
+        val, b = s.get_values(), s.get_rhs()
+        x, it, st = s.solve(None, rtol=1e-12, maxiter=5000)
+        assert st == 0
+        A = sp.csr_matrix((val, ix, ip), shape=(mesh.n, mesh.n))
+        assert np.linalg.norm(b - A @ x) / np.linalg.norm(b) < 1e-10
+        xo, ito, _, _, _ = O.pcg_ssor(ip, ix, val, b, rtol=1e-12, maxiter=5000, rows_per_thread=rpt, nthreads=16)
+        assert it == ito
+        assert_bitwise(x, xo, f"2M SSOR solution (rows_per_thread {rpt})")
+        s.close()
