@@ -177,6 +177,67 @@ def mixed_4m_leg(ctx, L, cases, core, steps, rtol, outer_rtol, check_every):
     return out
 
 
+def many_meshes_leg(ctx, L, cases, core, count, rtol, mixed_rtol, check_every):
+    """Configuration 4B: `count` independent meshes with n log-uniform in [5e3, 2e5] (sizes from
+    seed 7, bathymetry seeds 3 + k), one system each. Plain = one FP64 Jacobi solve after
+    another; batched = all systems in one launch per PCG phase (FP64, then mixed)."""
+    from math import erfc, sqrt
+
+    z = cases.normal(cases.SEED_ARRIVALS + 1000, count)
+    sizes = [int(round(5e3 * 40 ** (0.5 * erfc(-v / sqrt(2.0))))) for v in z]
+    meshes, s64, smx = [], [], []
+    for k, n in enumerate(sizes):
+        nx = max(8, int(round(sqrt(2 * n))))
+        ny = max(8, n // nx)
+        spec = cases.CaseSpec(f"m{k}-{nx}x{ny}", nx, ny, 1000.0, nvrt=2, depth=None, substeps=10)
+        data = cases.make_case(spec)
+        data.depth = cases.bathymetry(nx, ny, 20.0, 5.0, 4000.0, seed=3 + k)
+        mesh, p, _ = cases.load_mesh(ctx, data)
+        mesh.backtrack(p)
+        mesh.rhs_producers(p)
+        a = core.Solver(mesh)
+        a.assemble(p)
+        c = core.Solver(mesh, L.PREC_MIXED)
+        c.set_values(a.get_values())
+        c.set_rhs(a.get_rhs())
+        meshes.append(mesh)
+        s64.append(a)
+        smx.append(c)
+
+    def run(solvers, tol, batched):
+        for s in solvers:
+            s.reset_history()
+        ctx.synchronize()
+        t = time.perf_counter()
+        if batched:
+            core.batch_solve(solvers, tol, 20000, check_every)
+        else:
+            for s in solvers:
+                s.solve(None, tol, 20000, check_every, want_x=False)
+        ctx.synchronize()
+        return time.perf_counter() - t
+
+    run(s64[:4], rtol, False)
+    t_plain = run(s64, rtol, False)
+    x64 = [s.get_solution() for s in s64]
+    run(s64[:4], rtol, True)
+    t_b64 = run(s64, rtol, True)
+    same = all(np.array_equal(s.get_solution(), x) for s, x in zip(s64, x64))
+    run(smx[:4], mixed_rtol, True)
+    t_bmx = run(smx, mixed_rtol, True)
+    err = max(float(np.linalg.norm(s.get_solution() - x) / np.linalg.norm(x)) for s, x in zip(smx, x64))
+    out = {"workload": f"{count} meshes, n log-uniform in [5e3, 2e5] (total {sum(m.n for m in meshes)} nodes)",
+           "plain_fp64_solves_per_s": round(count / t_plain, 3), "batched_fp64_solves_per_s": round(count / t_b64, 3),
+           "batched_mixed_solves_per_s": round(count / t_bmx, 3), "batched_mixed_vs_plain": round(t_plain / t_bmx, 3),
+           "batched_fp64_bitwise_equal_to_plain": bool(same), "mixed_rtol": mixed_rtol,
+           "mixed_max_rel_l2_vs_fp64": err}
+    for s in s64 + smx:
+        s.close()
+    for m in meshes:
+        m.close()
+    return out
+
+
 def served_leg(ctx, L, cases, core, n_req, rates, rtol, mixed_rtol, check_every):
     """Served solves: a pool of 6 meshes with n log-uniform in [1e4, 2e6] (random bathymetry,
     seeds 3 + k), requests drawn uniformly from the pool with Poisson arrivals (seed 7) and
@@ -204,29 +265,32 @@ def served_leg(ctx, L, cases, core, n_req, rates, rtol, mixed_rtol, check_every)
     rhs = [cases.normal(1000 + j, pool[k].n) for j, k in enumerate(which)]
     # plain: one FP64 request after another
     plain = [core.Solver(m) for m in pool]
-    ctx.synchronize()
-    t = time.perf_counter()
-    x64 = []
-    for j, k in enumerate(which):
-        m = pool[k]
-        m.backtrack(pd)
-        m.rhs_producers(pd)
-        plain[k].assemble(pd)
-        plain[k].reset_history()
-        x, _, _ = plain[k].solve(rhs[j], rtol, 20000, check_every)
-        x64.append(x)
-    ctx.synchronize()
-    t_plain = time.perf_counter() - t
+    for rep in range(2):  # the first pass warms the solvers
+        ctx.synchronize()
+        t = time.perf_counter()
+        x64 = []
+        for j, k in enumerate(which):
+            m = pool[k]
+            m.backtrack(pd)
+            m.rhs_producers(pd)
+            plain[k].assemble(pd)
+            plain[k].reset_history()
+            x, _, _ = plain[k].solve(rhs[j], rtol, 20000, check_every)
+            x64.append(x)
+        ctx.synchronize()
+        t_plain = time.perf_counter() - t
     plain_rate = n_req / t_plain
     for s in plain:
         s.close()
     res = {"workload": f"{n_req} requests over 6 meshes, n = {sizes}", "plain_fp64_solves_per_s": round(plain_rate, 3),
            "mixed_rtol": mixed_rtol, "rates": []}
-    for mult in rates:
+    srv, ids, warm = None, None, True
+    for mult in (rates[-1],) + tuple(rates):  # the first pass warms the server's solver pools
         lam = mult * plain_rate
         arrive = np.cumsum(expo) / lam
-        srv = core.Server(ctx, rtol=mixed_rtol, maxiter=20000, check_every=check_every, precision=L.PREC_MIXED)
-        ids = [srv.add_mesh(m) for m in pool]
+        if srv is None:
+            srv = core.Server(ctx, rtol=mixed_rtol, maxiter=20000, check_every=check_every, precision=L.PREC_MIXED)
+            ids = [srv.add_mesh(m) for m in pool]
         tickets, by_ticket = {}, {}
         done = {}
         t0 = time.perf_counter()
@@ -249,12 +313,15 @@ def served_leg(ctx, L, cases, core, n_req, rates, rtol, mixed_rtol, check_every)
         t_end = max(v[3] for v in done.values())
         lats = np.array([done[j][2] for j in range(n_req)])
         errs = [float(np.linalg.norm(done[j][0] - x64[j]) / np.linalg.norm(x64[j])) for j in range(n_req)]
+        if warm:
+            warm = False
+            continue
         res["rates"].append({"arrival_rate_per_s": round(lam, 3), "solves_per_s": round(n_req / (t_end - arrive[0]), 3),
                              "p50_ms": round(float(np.percentile(lats, 50)), 3),
                              "p99_ms": round(float(np.percentile(lats, 99)), 3),
                              "max_rel_l2_vs_fp64": max(errs),
                              "mean_inner_iters": float(np.mean([done[j][1] for j in range(n_req)]))})
-        srv.close()
+    srv.close()
     for m in pool:
         m.close()
     return res
@@ -275,6 +342,7 @@ def main() -> int:
     ap.add_argument("--cpu-threads", type=int, default=0, help="oracle threads (0 = all visible cores, max 16)")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--members", type=int, default=64)
+    ap.add_argument("--meshes", type=int, default=256)
     ap.add_argument("--ensemble-nx", type=int, default=708)
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--requests", type=int, default=120)
@@ -438,6 +506,9 @@ def main() -> int:
         batched = batched_leg(ctx, L, cases, core, args.members, args.ensemble_nx, args.ensemble_nx, rtol,
                               args.mixed_rtol, check_every)
         log(f"[bench] batched: {batched}")
+        batched["many_meshes"] = many_meshes_leg(ctx, L, cases, core, args.meshes, rtol, args.mixed_rtol,
+                                                 check_every)
+        log(f"[bench] batched many meshes: {batched['many_meshes']}")
     served = None
     if not args.no_served:
         served = served_leg(ctx, L, cases, core, args.requests, (1.0, 2.0, 4.0), rtol, args.mixed_rtol,

@@ -38,18 +38,32 @@ using Clock = std::chrono::steady_clock;
 
 // Device allocations owned by one object, released together.
 struct DevPool {
+  // Bump allocator over zeroed chunks that double in size: an object with dozens of buffers
+  // costs a handful of hipMalloc / hipMemset calls instead of one pair per buffer.
   std::vector<void*> ptrs;
   int64_t bytes = 0;
+  char* cur = nullptr;
+  size_t left = 0, next_chunk = (size_t)1 << 20;
   template <class T>
   T* alloc(int64_t count, bool zero = true) {
+    (void)zero;  // chunk memory is zeroed when the chunk is allocated
     if (count <= 0) count = 1;
-    void* p = nullptr;
-    const size_t nb = (size_t)count * sizeof(T);
-    HIPCHK(hipMalloc(&p, nb));
-    ptrs.push_back(p);
+    const size_t nb = ((size_t)count * sizeof(T) + 255) & ~(size_t)255;
+    if (nb > left) {
+      const size_t cb = std::max(nb, next_chunk);
+      void* p = nullptr;
+      HIPCHK(hipMalloc(&p, cb));
+      HIPCHK(hipMemset(p, 0, cb));
+      ptrs.push_back(p);
+      cur = static_cast<char*>(p);
+      left = cb;
+      next_chunk = std::min<size_t>(2 * next_chunk, (size_t)1 << 30);
+    }
+    T* out = reinterpret_cast<T*>(cur);
+    cur += nb;
+    left -= nb;
     bytes += (int64_t)nb;
-    if (zero) HIPCHK(hipMemset(p, 0, nb));
-    return static_cast<T*>(p);
+    return out;
   }
   template <class T>
   T* upload(const std::vector<T>& v) {
@@ -61,6 +75,8 @@ struct DevPool {
     for (void* p : ptrs) (void)hipFree(p);
     ptrs.clear();
     bytes = 0;
+    cur = nullptr;
+    left = 0;
   }
   ~DevPool() { release(); }
 };
@@ -938,6 +954,7 @@ int gsh_mesh_create(gsh_ctx* ctx, const gsh_mesh_desc* desc, gsh_mesh** out, int
     d.foot = P.alloc<int32_t>((int64_t)ns * nv);
     d.bstatus = P.alloc<int32_t>((int64_t)ns * nv);
     d.sU = P.alloc<double>(2 * (int64_t)ns);
+    d.side_us = P.alloc<double>(2 * (int64_t)ns);
     d.hhat = P.alloc<double>(ns);
     d.chi = P.alloc<double>(ns);
     d.ball_ptr = P.upload(t.ball_ptr);

@@ -52,6 +52,7 @@ struct DMesh {
   int32_t* foot;            // [ns][nvrt]
   int32_t* bstatus;         // [ns][nvrt]
   double* sU;               // [ns][2]
+  double* side_us;          // [ns][2] vertical integral of btu of each side (computeElemAverageGhat)
   double* hhat;
   double* chi;
   // balls and CSR pattern

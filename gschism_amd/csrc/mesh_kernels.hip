@@ -369,6 +369,22 @@ __global__ void k_side_rhs(DMesh m, StepArgs a) {
     }
     const double htot = 0.5 * (m.eta2[r.x] + m.eta2[r.y]) + 0.5 * (m.depth[r.x] + m.depth[r.y]);
     const double* bt = m.btu + 2 * (int64_t)s * nv;
+    // the side's share of computeElemAverageGhat (meshdata.cu:6183-6424): vertical integral of btu,
+    // the same for both elements of the side
+    double usx = 0.0, usy = 0.0;
+    if (kb == nv - 2) {
+      const double dz = zc[nv - 1] - zc[nv - 2];
+      usx += bt[2 * (nv - 1)] * dz;
+      usy += bt[2 * (nv - 1) + 1] * dz;
+    } else {
+      for (int k = kb + 1; k < nv; ++k) {
+        const double dz = zc[k] - zc[k - 1];
+        usx += (bt[2 * k] + bt[2 * (k - 1)]) / 2.0 * dz;
+        usy += (bt[2 * k + 1] + bt[2 * (k - 1) + 1]) / 2.0 * dz;
+      }
+    }
+    m.side_us[2 * s] = usx;
+    m.side_us[2 * s + 1] = usy;
     const double bx = bt[2 * (kb + 1)], by = bt[2 * (kb + 1) + 1];
     const double vmag1 = sqrt(bx * bx + by * by);
     const double cd = 0.5 * (m.cd[r.x] + m.cd[r.y]);
@@ -416,20 +432,8 @@ __global__ void k_elem_ghat(DMesh m, StepArgs a) {
         cff2 = 1.0;
         cff3 = 0.0;
       }
-      const double* zc = m.zc + (int64_t)s * nv;
       const double* bt = m.btu + 2 * (int64_t)s * nv;
-      double usx = 0.0, usy = 0.0;
-      if (kb == nv - 2) {
-        const double dz = zc[nv - 1] - zc[nv - 2];
-        usx += bt[2 * (nv - 1)] * dz;
-        usy += bt[2 * (nv - 1) + 1] * dz;
-      } else {
-        for (int k = kb + 1; k < nv; ++k) {
-          const double dz = zc[k] - zc[k - 1];
-          usx += (bt[2 * k] + bt[2 * (k - 1)]) / 2.0 * dz;
-          usy += (bt[2 * k + 1] + bt[2 * (k - 1) + 1]) / 2.0 * dz;
-        }
-      }
+      const double usx = m.side_us[2 * s], usy = m.side_us[2 * s + 1];  // from k_side_rhs
       gx += cff1 * usx - cff3 * 0.0;
       gy += cff1 * usy - cff3 * 0.0;
       const double ubx = bt[2 * (kb + 1)], uby = bt[2 * (kb + 1) + 1];
