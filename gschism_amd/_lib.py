@@ -132,6 +132,7 @@ SIGNATURES = {
     "gsh_copy_probe": (_I, [_P, _I64, _I, _PD]),
     "gsh_solver_profile": (_I, [_P, _D, _I, _I, _PD, C.POINTER(_I)]),
     "gsh_solver_set_omega": (_I, [_P, _D]),
+    "gsh_solver_set_graphs": (_I, [_P, _I]),
     "gsh_solver_autotune": (_I, [_P, _I, C.POINTER(_I), _PD]),
     "gsh_solver_ssor_info": (_I, [_P, _PI32, _PI32, _PI32]),
     "gsh_colour_csr": (_I, [_I32, _PI32, _PI32, _PI32, _PI32]),

@@ -290,6 +290,9 @@ class Solver:
         check(L.lib().gsh_solver_autotune(self.h, iters, C.byref(best), dptr(us)))
         return best.value, us
 
+    def set_graphs(self, on: bool) -> None:
+        check(L.lib().gsh_solver_set_graphs(self.h, 1 if on else 0))
+
     def set_omega(self, omega: float) -> None:
         check(L.lib().gsh_solver_set_omega(self.h, omega))
 

@@ -237,6 +237,9 @@ int gsh_solver_set_tuning(gsh_solver* s, int rows_per_thread, double inner_rtol)
  * us_per_iter[4] (may be NULL) receives the measured microseconds per iteration. The solution
  * buffer is overwritten, the history of previous solutions is kept. */
 int gsh_solver_autotune(gsh_solver* s, int iters, int* best_rpt, double* us_per_iter);
+/* No counterpart in the reference: on != 0 captures each solve window (check_every iterations
+ * and the window-end test) of this solver into a HIP graph once and replays it; default off. */
+int gsh_solver_set_graphs(gsh_solver* s, int on);
 /* SSOR relaxation factor omega in (0, 2), default 1 (symmetric Gauss-Seidel). */
 int gsh_solver_set_omega(gsh_solver* s, double omega);
 /* SSOR layout: number of colours, rows after padding every colour block to whole reduction

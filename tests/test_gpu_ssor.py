@@ -162,3 +162,27 @@ def test_ssor_2m_matches_oracle(ctx):
         assert it == ito
         assert_bitwise(x, xo, f"2M SSOR solution (rows_per_thread {rpt})")
         s.close()
+
+
+def test_window_graphs_replay_exactly(ctx):
+    """With window graphs on, repeated solves (graph replays with eager work in between) give the
+    oracle's solutions, for Jacobi and SSOR."""
+    spec = cases.CaseSpec("graph-70x40", 70, 40, 600.0, nvrt=4, depth=None, substeps=4)
+    mesh, om, p, val, b = _assembled(ctx, spec)
+    ip, ix = mesh.topology(L.T_CSR_INDPTR), mesh.topology(L.T_CSR_INDICES)
+    for precond in (L.PRECOND_JACOBI, L.PRECOND_SSOR):
+        s = core.Solver(mesh, precond=precond)
+        s.set_graphs(True)
+        s.assemble(p)
+        for rep in range(3):
+            rhs = cases.normal(50 + rep, mesh.n)
+            s.reset_history()
+            x, it, st = s.solve(rhs, rtol=1e-12, maxiter=3000)
+            if precond == L.PRECOND_SSOR:
+                xo, ito, _, _, _ = O.pcg_ssor(ip, ix, val, rhs, rtol=1e-12, maxiter=3000, rows_per_thread=1)
+            else:
+                xo, ito, _, _ = O.pcg(ip, ix, val, rhs, rtol=1e-12, maxiter=3000)
+            assert it == ito
+            assert_bitwise(x, xo, f"graph replay {rep} (precond {precond})")
+            assert s.check() < 1e-10
+        s.close()
