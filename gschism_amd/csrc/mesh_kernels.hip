@@ -86,13 +86,23 @@ __device__ __forceinline__ double signed_area(double p1x, double p1y, double p2x
   return 0.5 * ((p1x - p3x) * (p2y - p3y) - (p2x - p3x) * (p1y - p3y));
 }
 
-// Element::interpolateVelocities for triangles, ics=1 (meshdata.cu:1209-1291) with
-// computeShapeCoords(fiAnywhere) (meshdata.cu:990-1043).
-__device__ void interp_velocity(const DMesh& m, double h0, int e, double px, double py, double pz,
-                                double& uo, double& vo, double& wo) {
+// Node ids and coordinates of one element, as the walk leaves them for the interpolation.
+struct ElemGeom {
+  int n0, n1, n2;
+  double x0, y0, x1, y1, x2, y2;
+};
+
+__device__ __forceinline__ ElemGeom elem_geom(const DMesh& m, int e) {
   const int4 r = m.elem[2 * e];
-  const int n0 = r.x, n1 = r.y, n2 = r.z;
-  const double x0 = m.x[n0], y0 = m.y[n0], x1 = m.x[n1], y1 = m.y[n1], x2 = m.x[n2], y2 = m.y[n2];
+  return {r.x, r.y, r.z, m.x[r.x], m.y[r.x], m.x[r.y], m.y[r.y], m.x[r.z], m.y[r.z]};
+}
+
+// Element::interpolateVelocities for triangles, ics=1 (meshdata.cu:1209-1291) with
+// computeShapeCoords(fiAnywhere) (meshdata.cu:990-1043); g is element e's geometry.
+__device__ void interp_velocity(const DMesh& m, double h0, int e, const ElemGeom& g, double px, double py, double pz,
+                                double& uo, double& vo, double& wo) {
+  const int n0 = g.n0, n1 = g.n1, n2 = g.n2;
+  const double x0 = g.x0, y0 = g.y0, x1 = g.x1, y1 = g.y1, x2 = g.x2, y2 = g.y2;
   const double area = m.area[e];
   double s0 = signed_area(px, py, x1, y1, x2, y2) / area;
   double s1 = signed_area(x0, y0, px, py, x2, y2) / area;
@@ -150,14 +160,16 @@ __device__ void interp_velocity(const DMesh& m, double h0, int e, double px, dou
 
 // Leaded celestial walk (meshdata.cu:85-127) with Element::computePredicates (meshdata.cu:651-746).
 // Corner j of a triangle is getNodeInOrder(j) = nodes[(j+1)%3]; walk side j is element side j.
+// On return, geo holds the geometry of the final element cur.
 __device__ bool celestial_walk(const DMesh& m, int& cur, double x0x, double x0y, double xtx, double xty,
-                               int& last_cross, int& status) {
+                               int& last_cross, int& status, ElemGeom& geo) {
   double fac = 0.00001;
   for (int iter = 0; iter < 40; ++iter) {
     const int4 r0 = m.elem[2 * cur], r1 = m.elem[2 * cur + 1];
     const double C0x = m.x[r0.y], C0y = m.y[r0.y];
     const double C1x = m.x[r0.z], C1y = m.y[r0.z];
     const double C2x = m.x[r0.x], C2y = m.y[r0.x];
+    geo = {r0.x, r0.y, r0.z, C2x, C2y, C0x, C0y, C1x, C1y};
     const double vx = xtx - x0x, vy = xty - x0y;
     const double d0 = vx * (C0x - x0x) + vy * (C0y - x0y);
     const double d1 = vx * (C1x - x0x) + vy * (C1y - x0y);
@@ -190,6 +202,7 @@ __device__ bool celestial_walk(const DMesh& m, int& cur, double x0x, double x0y,
     cur = nxt;
   }
   status |= GSH_BT_WALK_CAP;
+  geo = elem_geom(m, cur);
   return true;
 }
 
@@ -205,13 +218,14 @@ __device__ void btrack(const DMesh& m, const StepArgs& a, int s, int k, double d
   double uiz = 0.5 * (U[(int64_t)sr.x * m.nvrt + k].z + U[(int64_t)sr.y * m.nvrt + k].z);
   double xtx = -1.0e25, xty = -1.0e25, xtz = -1.0e25;
   int iter = 0;
+  ElemGeom geo;  // set by every walk before its first use
   while (true) {
     const double dtb = fmin(dtbk, time_rm);
     xtx = x0x - dtb * uix;
     xty = x0y - dtb * uiy;
     xtz = x0z - dtb * uiz;
     int last = -1;
-    const bool found = celestial_walk(m, cur, x0x, x0y, xtx, xty, last, status);
+    const bool found = celestial_walk(m, cur, x0x, x0y, xtx, xty, last, status, geo);
     if (t3 <= time_rm) {
       xtx = x0x;
       xty = x0y;
@@ -252,7 +266,7 @@ __device__ void btrack(const DMesh& m, const StepArgs& a, int s, int k, double d
       const double len = sqrt(v2x * v2x + v2y * v2y);
       const double tx = v2x / len, ty = v2y / len;
       double iu, iv, iw;
-      interp_velocity(m, h0, cur, xtx, xty, xtz, iu, iv, iw);
+      interp_velocity(m, h0, cur, geo, xtx, xty, xtz, iu, iv, iw);
       const double d = iu * tx + iv * ty;
       uix = tx * d;
       uiy = ty * d;
@@ -260,7 +274,7 @@ __device__ void btrack(const DMesh& m, const StepArgs& a, int s, int k, double d
       status |= GSH_BT_SLIDE;
     } else {
       time_rm -= dtb;
-      interp_velocity(m, h0, cur, xtx, xty, xtz, uix, uiy, uiz);
+      interp_velocity(m, h0, cur, geo, xtx, xty, xtz, uix, uiy, uiz);
     }
     if (time_rm <= 0.0) break;
     if (uix * uix + uiy * uiy <= a.velmin2) {
@@ -285,7 +299,7 @@ __device__ void btrack(const DMesh& m, const StepArgs& a, int s, int k, double d
     t1 = time_rm;
   }
   double iw;
-  interp_velocity(m, h0, cur, xtx, xty, xtz, ux, uy, iw);
+  interp_velocity(m, h0, cur, geo, xtx, xty, xtz, ux, uy, iw);
   foot = cur;
 }
 
