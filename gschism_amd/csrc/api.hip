@@ -849,7 +849,7 @@ int gsh_mesh_create(gsh_ctx* ctx, const gsh_mesh_desc* desc, gsh_mesh** out, int
   GSH_API_BODY({
     GSH_REQUIRE(desc && out, GSH_E_ARG, "NULL argument");
     GSH_REQUIRE(desc->n >= 3 && desc->ne >= 1, GSH_E_ARG, "mesh needs at least 3 nodes and 1 element");
-    GSH_REQUIRE(desc->nvrt >= 2, GSH_E_ARG, "nvrt must be >= 2");
+    GSH_REQUIRE(desc->nvrt >= 2 && desc->nvrt <= 1024, GSH_E_ARG, "nvrt must be in [2, 1024]");
     GSH_REQUIRE(desc->xy && desc->elnode && desc->depth && desc->sigma, GSH_E_ARG, "xy, elnode, depth, sigma required");
     if (ctx) ctx->set();
     auto m = std::make_unique<gsh_mesh>();

@@ -55,29 +55,31 @@ __global__ void k_node_wet(DMesh m) {
   m.node_wet[i] = wet;
 }
 
-// ul0UpdateSides / ul0UpdateSides2 (meshdata.cu:5679-5760): side kb, zc, wet flag.
+// ul0UpdateSides / ul0UpdateSides2 (meshdata.cu:5679-5760): side kb, zc, wet flag. One thread per
+// (side, level), so consecutive threads write consecutive zc entries.
 __global__ void k_side_levels(DMesh m, double h0) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= m.ns) return;
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t nv = (uint32_t)m.nvrt;
+  if (t >= (uint64_t)m.ns * nv) return;
+  const int s = (t < 0xffffffffull) ? (int)((uint32_t)t / nv) : (int)(t / nv);
+  const int k = (int)(t - (uint64_t)s * nv);
   const int4 r = m.side[s];
   const bool wet = m.elem_wet[r.z] || (r.w >= 0 && m.elem_wet[r.w]);
-  int kb = 0;
-  if (wet) {
-    const int kb1 = m.kb[r.x], kb2 = m.kb[r.y];
-    kb = min(kb1, kb2);
-    double* zc = m.zc + (int64_t)s * m.nvrt;
-    for (int k = 0; k < kb; ++k) zc[k] = 0.0;
-    for (int k = kb; k < m.nvrt; ++k)
-      zc[k] = 0.5 * (zcoord(m, h0, r.x, max(k, kb1)) + zcoord(m, h0, r.y, max(k, kb2)));
+  const int kb1 = m.kb[r.x], kb2 = m.kb[r.y];
+  const int kb = wet ? min(kb1, kb2) : 0;
+  if (wet)
+    m.zc[t] = (k < kb) ? 0.0 : 0.5 * (zcoord(m, h0, r.x, max(k, kb1)) + zcoord(m, h0, r.y, max(k, kb2)));
+  if (k == 0) {
+    m.side_kb[s] = kb;
+    m.side_wet[s] = wet ? 1 : 0;
   }
-  m.side_kb[s] = kb;
-  m.side_wet[s] = wet ? 1 : 0;
 }
 
 void launch_levels(const DMesh& m, const StepArgs& a, hipStream_t st) {
   if (m.ne > 0) hipLaunchKernelGGL(k_elem_levels, dim3((m.ne + kTpb - 1) / kTpb), dim3(kTpb), 0, st, m, a.h0);
   if (m.n > 0) hipLaunchKernelGGL(k_node_wet, dim3((m.n + kTpb - 1) / kTpb), dim3(kTpb), 0, st, m);
-  if (m.ns > 0) hipLaunchKernelGGL(k_side_levels, dim3((m.ns + kTpb - 1) / kTpb), dim3(kTpb), 0, st, m, a.h0);
+  const int64_t sl = (int64_t)m.ns * m.nvrt;
+  if (sl > 0) hipLaunchKernelGGL(k_side_levels, dim3((unsigned)((sl + kTpb - 1) / kTpb)), dim3(kTpb), 0, st, m, a.h0);
 }
 
 // ---------------------------------------------------------------- a4-a7: ELM backtracking
@@ -364,25 +366,58 @@ void launch_backtrack(const DMesh& m, const StepArgs& a, const int32_t* ls, cons
 // ---------------------------------------------------------------- a8-a10, elevation BC
 // computeVerticallyIntegratedVelocity (meshdata.cu:4459-4476) and
 // computeVegetationAndDragParameters for iveg=0, nchi=0 (meshdata.cu:4615-4725).
-__global__ void k_side_rhs(DMesh m, StepArgs a) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= m.ns) return;
-  const int4 r = m.side[s];
+// A workgroup takes g consecutive sides: its four waves copy the sides' zc and velocity rows
+// (contiguous in HBM) into LDS with coalesced loads, one thread per side then sums its levels in
+// the reference's order from LDS, and the velocity rows are replaced by the btu rows for the
+// second integral. Row strides in LDS are odd so the per-side reads spread over the banks.
+__device__ __forceinline__ void stage_rows(double* __restrict__ dst, int stride, const double* __restrict__ src,
+                                           int row_len, int rows) {
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int g = wv; g < rows; g += kTpb / 64)
+    for (int j = lane; j < row_len; j += 64) dst[g * stride + j] = src[(int64_t)g * row_len + j];
+}
+
+__global__ void __launch_bounds__(256) k_side_rhs(DMesh m, StepArgs a, int g) {
+  extern __shared__ double lds[];
   const int nv = m.nvrt;
-  double Ux = 0.0, Uy = 0.0, hhat = 0.0, chi = 0.0;
-  if (m.side_wet[s]) {
-    const int kb = m.side_kb[s];
-    const double* zc = m.zc + (int64_t)s * nv;
-    const double* u2 = m.su2 + 2 * (int64_t)s * nv;
-    for (int k = kb; k < nv - 1; ++k) {
-      const double ucx = 0.5 * (u2[2 * k] + u2[2 * (k + 1)]);
-      const double ucy = 0.5 * (u2[2 * k + 1] + u2[2 * (k + 1) + 1]);
-      const double dz = zc[k + 1] - zc[k];
-      Ux += dz * ucx;
-      Uy += dz * ucy;
+  const int zs = nv | 1, us = (2 * nv) | 1;
+  double* Lz = lds;           // [g][zs]: zc
+  double* Lu = lds + g * zs;  // [g][us]: side velocity u2, then btu
+  const int s0 = blockIdx.x * g;
+  const int gc = min(g, m.ns - s0);
+  stage_rows(Lz, zs, m.zc + (int64_t)s0 * nv, nv, gc);
+  stage_rows(Lu, us, m.su2 + 2 * (int64_t)s0 * nv, 2 * nv, gc);
+  __syncthreads();
+  const int t = threadIdx.x;
+  const bool mine = t < gc;
+  const int s = s0 + t;
+  bool wet = false;
+  int kb = 0;
+  double Ux = 0.0, Uy = 0.0;
+  const double* zc = Lz + t * zs;
+  const double* u = Lu + t * us;
+  if (mine) {
+    wet = m.side_wet[s] != 0;
+    if (wet) {
+      kb = m.side_kb[s];
+      for (int k = kb; k < nv - 1; ++k) {
+        const double ucx = 0.5 * (u[2 * k] + u[2 * (k + 1)]);
+        const double ucy = 0.5 * (u[2 * k + 1] + u[2 * (k + 1) + 1]);
+        const double dz = zc[k + 1] - zc[k];
+        Ux += dz * ucx;
+        Uy += dz * ucy;
+      }
     }
+  }
+  __syncthreads();
+  stage_rows(Lu, us, m.btu + 2 * (int64_t)s0 * nv, 2 * nv, gc);
+  __syncthreads();
+  if (!mine) return;
+  double hhat = 0.0, chi = 0.0;
+  if (wet) {
+    const int4 r = m.side[s];
     const double htot = 0.5 * (m.eta2[r.x] + m.eta2[r.y]) + 0.5 * (m.depth[r.x] + m.depth[r.y]);
-    const double* bt = m.btu + 2 * (int64_t)s * nv;
+    const double* bt = u;
     // the side's share of computeElemAverageGhat (meshdata.cu:6183-6424): vertical integral of btu,
     // the same for both elements of the side
     double usx = 0.0, usy = 0.0;
@@ -475,7 +510,12 @@ __global__ void k_elbc(DMesh m, StepArgs a) {
 
 void launch_rhs(const DMesh& m, const StepArgs& a, hipStream_t st) {
   if (m.n > 0) hipLaunchKernelGGL(k_elbc, dim3((m.n + kTpb - 1) / kTpb), dim3(kTpb), 0, st, m, a);
-  if (m.ns > 0) hipLaunchKernelGGL(k_side_rhs, dim3((m.ns + kTpb - 1) / kTpb), dim3(kTpb), 0, st, m, a);
+  if (m.ns > 0) {
+    // sides per workgroup: as many as fit 32 KB of LDS, at most one per thread
+    const int row = 8 * ((m.nvrt | 1) + ((2 * m.nvrt) | 1));
+    const int g = max(1, min(kTpb, (32 * 1024) / row));
+    hipLaunchKernelGGL(k_side_rhs, dim3((unsigned)((m.ns + g - 1) / g)), dim3(kTpb), (size_t)g * row, st, m, a, g);
+  }
   if (m.ne > 0) hipLaunchKernelGGL(k_elem_ghat, dim3((m.ne + kTpb - 1) / kTpb), dim3(kTpb), 0, st, m, a);
 }
 

@@ -72,7 +72,7 @@ int  gsh_ctx_synchronize(gsh_ctx* ctx);
 typedef struct gsh_mesh_desc {
   int32_t n;              /* nodes */
   int32_t ne;             /* triangles */
-  int32_t nvrt;           /* vertical levels (>= 2) */
+  int32_t nvrt;           /* vertical levels (2 .. 1024) */
   const double*  xy;      /* [n][2] node coordinates (ics=1 Cartesian, metres) */
   const int32_t* elnode;  /* [ne][3] counter-clockwise node ids */
   const double*  depth;   /* [n] depth h (positive down, metres) */
