@@ -937,6 +937,7 @@ int gsh_mesh_create(gsh_ctx* ctx, const gsh_mesh_desc* desc, gsh_mesh** out, int
     d.eth = P.alloc<double>(n);
     d.elbc = P.alloc<double>(n);
     d.uvw = P.alloc<double>(4 * (int64_t)n * nv);
+    d.nrec = P.alloc<double2>(2 * (int64_t)n);
     d.node_wet = P.alloc<int8_t>(n);
     d.elem = P.upload(erec);
     d.area = P.upload(m->geo.area);

@@ -29,6 +29,7 @@ struct DMesh {
   double* eth;
   double* elbc;
   double* uvw;              // [n][nvrt][4]: u, v, w, 0
+  double2* nrec;            // [n][2]: (x, y), (eta2, depth), packed for the backtracking gathers
   int8_t* node_wet;
   // elements
   const int4* elem;         // [ne][2]: (n0, n1, n2, s0), (s1, s2, -, -)

@@ -96,13 +96,33 @@ struct ElemGeom {
 
 __device__ __forceinline__ ElemGeom elem_geom(const DMesh& m, int e) {
   const int4 r = m.elem[2 * e];
-  return {r.x, r.y, r.z, m.x[r.x], m.y[r.x], m.x[r.y], m.y[r.y], m.x[r.z], m.y[r.z]};
+  const double2 p0 = m.nrec[2 * r.x], p1 = m.nrec[2 * r.y], p2 = m.nrec[2 * r.z];
+  return {r.x, r.y, r.z, p0.x, p0.y, p1.x, p1.y, p2.x, p2.y};
 }
 
 // Element::interpolateVelocities for triangles, ics=1 (meshdata.cu:1209-1291) with
 // computeShapeCoords(fiAnywhere) (meshdata.cu:990-1043); g is element e's geometry.
-__device__ void interp_velocity(const DMesh& m, double h0, int e, const ElemGeom& g, double px, double py, double pz,
-                                double& uo, double& vo, double& wo) {
+struct NodeZ {
+  double eta, h;
+  int kb;
+  bool dry;
+};
+
+__device__ __forceinline__ NodeZ node_z(const DMesh& m, double h0, int node) {
+  const double2 q = m.nrec[2 * node + 1];
+  return {q.x, q.y, m.kb[node], q.x + q.y <= h0};
+}
+
+// zcoord from a node's values held in registers
+__device__ __forceinline__ double zcoord_q(const NodeZ& q, const double* sig, int nv, int k) {
+  if (q.dry) return 0.0;
+  if (k <= q.kb) return -q.h;
+  if (k == nv - 1) return q.eta;
+  return (q.eta + q.h) * sig[k] + q.eta;
+}
+
+__device__ void interp_velocity(const DMesh& m, const double* sig, double h0, int e, const ElemGeom& g, double px,
+                                double py, double pz, double& uo, double& vo, double& wo) {
   const int n0 = g.n0, n1 = g.n1, n2 = g.n2;
   const double x0 = g.x0, y0 = g.y0, x1 = g.x1, y1 = g.y1, x2 = g.x2, y2 = g.y2;
   const double area = m.area[e];
@@ -117,13 +137,14 @@ __device__ void interp_velocity(const DMesh& m, double h0, int e, const ElemGeom
     s1 = (s1 + aa) * bb;
     s2 = (s2 + aa) * bb;
   }
-  const int kb0 = m.kb[n0], kb1 = m.kb[n1], kb2 = m.kb[n2];
   const int nv = m.nvrt;
+  const NodeZ q0 = node_z(m, h0, n0), q1 = node_z(m, h0, n1), q2 = node_z(m, h0, n2);
+  const int kb0 = q0.kb, kb1 = q1.kb, kb2 = q2.kb;
   int L = m.elem_kb[e] - 1, R = nv - 1;
-  double zL = s0 * zcoord(m, h0, n0, max(L, kb0)) + s1 * zcoord(m, h0, n1, max(L, kb1)) +
-              s2 * zcoord(m, h0, n2, max(L, kb2)) + 0.0;
-  double zR = s0 * zcoord(m, h0, n0, max(R, kb0)) + s1 * zcoord(m, h0, n1, max(R, kb1)) +
-              s2 * zcoord(m, h0, n2, max(R, kb2)) + 0.0;
+  double zL = s0 * zcoord_q(q0, sig, nv, max(L, kb0)) + s1 * zcoord_q(q1, sig, nv, max(L, kb1)) +
+              s2 * zcoord_q(q2, sig, nv, max(L, kb2)) + 0.0;
+  double zR = s0 * zcoord_q(q0, sig, nv, max(R, kb0)) + s1 * zcoord_q(q1, sig, nv, max(R, kb1)) +
+              s2 * zcoord_q(q2, sig, nv, max(R, kb2)) + 0.0;
   double w1, w2;
   if (pz <= zL) {
     w1 = 1.0;
@@ -134,8 +155,8 @@ __device__ void interp_velocity(const DMesh& m, double h0, int e, const ElemGeom
   } else {
     while (R - L > 1) {
       const int M = (L + R) / 2;
-      const double zM = s0 * zcoord(m, h0, n0, max(M, kb0)) + s1 * zcoord(m, h0, n1, max(M, kb1)) +
-                        s2 * zcoord(m, h0, n2, max(M, kb2)) + 0.0;
+      const double zM = s0 * zcoord_q(q0, sig, nv, max(M, kb0)) + s1 * zcoord_q(q1, sig, nv, max(M, kb1)) +
+                        s2 * zcoord_q(q2, sig, nv, max(M, kb2)) + 0.0;
       if (zM < pz) {
         L = M;
         zL = zM;
@@ -168,9 +189,10 @@ __device__ bool celestial_walk(const DMesh& m, int& cur, double x0x, double x0y,
   double fac = 0.00001;
   for (int iter = 0; iter < 40; ++iter) {
     const int4 r0 = m.elem[2 * cur], r1 = m.elem[2 * cur + 1];
-    const double C0x = m.x[r0.y], C0y = m.y[r0.y];
-    const double C1x = m.x[r0.z], C1y = m.y[r0.z];
-    const double C2x = m.x[r0.x], C2y = m.y[r0.x];
+    const double2 P0 = m.nrec[2 * r0.y], P1 = m.nrec[2 * r0.z], P2 = m.nrec[2 * r0.x];
+    const double C0x = P0.x, C0y = P0.y;
+    const double C1x = P1.x, C1y = P1.y;
+    const double C2x = P2.x, C2y = P2.y;
     geo = {r0.x, r0.y, r0.z, C2x, C2y, C0x, C0y, C1x, C1y};
     const double vx = xtx - x0x, vy = xty - x0y;
     const double d0 = vx * (C0x - x0x) + vy * (C0y - x0y);
@@ -209,7 +231,7 @@ __device__ bool celestial_walk(const DMesh& m, int& cur, double x0x, double x0y,
 }
 
 // Side::btrack (meshdata.cu:1449-1629). Returns the final velocity and the foot element.
-__device__ void btrack(const DMesh& m, const StepArgs& a, int s, int k, double dtbk, int cur, double& ux,
+__device__ void btrack(const DMesh& m, const double* sig, const StepArgs& a, int s, int k, double dtbk, int cur, double& ux,
                        double& uy, int& foot, int& status) {
   const double h0 = a.h0;
   double x0x = m.side_cx[s], x0y = m.side_cy[s], x0z = m.zc[(int64_t)s * m.nvrt + k];
@@ -237,7 +259,8 @@ __device__ void btrack(const DMesh& m, const StepArgs& a, int s, int k, double d
     }
     if (!found) {
       const int4 cs = m.side[last];
-      const double p1x = m.x[cs.x], p1y = m.y[cs.x], p2x = m.x[cs.y], p2y = m.y[cs.y];
+      const double2 P1 = m.nrec[2 * cs.x], P2 = m.nrec[2 * cs.y];
+      const double p1x = P1.x, p1y = P1.y, p2x = P2.x, p2y = P2.y;
       const double v1x = xtx - x0x, v1y = xty - x0y;
       const double v2x = p2x - p1x, v2y = p2y - p1y;
       const double denom = v2x * v1y - v2y * v1x;
@@ -268,7 +291,7 @@ __device__ void btrack(const DMesh& m, const StepArgs& a, int s, int k, double d
       const double len = sqrt(v2x * v2x + v2y * v2y);
       const double tx = v2x / len, ty = v2y / len;
       double iu, iv, iw;
-      interp_velocity(m, h0, cur, geo, xtx, xty, xtz, iu, iv, iw);
+      interp_velocity(m, sig, h0, cur, geo, xtx, xty, xtz, iu, iv, iw);
       const double d = iu * tx + iv * ty;
       uix = tx * d;
       uiy = ty * d;
@@ -276,7 +299,7 @@ __device__ void btrack(const DMesh& m, const StepArgs& a, int s, int k, double d
       status |= GSH_BT_SLIDE;
     } else {
       time_rm -= dtb;
-      interp_velocity(m, h0, cur, geo, xtx, xty, xtz, uix, uiy, uiz);
+      interp_velocity(m, sig, h0, cur, geo, xtx, xty, xtz, uix, uiy, uiz);
     }
     if (time_rm <= 0.0) break;
     if (uix * uix + uiy * uiy <= a.velmin2) {
@@ -301,7 +324,7 @@ __device__ void btrack(const DMesh& m, const StepArgs& a, int s, int k, double d
     t1 = time_rm;
   }
   double iw;
-  interp_velocity(m, h0, cur, geo, xtx, xty, xtz, ux, uy, iw);
+  interp_velocity(m, sig, h0, cur, geo, xtx, xty, xtz, ux, uy, iw);
   foot = cur;
 }
 
@@ -310,6 +333,9 @@ __device__ void btrack(const DMesh& m, const StepArgs& a, int s, int k, double d
 // fall on a few contiguous cache lines.
 __global__ void __launch_bounds__(256) k_backtrack(DMesh m, StepArgs a, const int32_t* list_side,
                                                    const int32_t* list_lev, int32_t count) {
+  __shared__ double sig[1024];
+  for (int j = threadIdx.x; j < m.nvrt; j += blockDim.x) sig[j] = m.sigma[j];
+  __syncthreads();
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int s, k;
   if (list_side) {
@@ -348,7 +374,7 @@ __global__ void __launch_bounds__(256) k_backtrack(DMesh m, StepArgs a, const in
     const int ndelt = max(a.ndelt_min, min(nd, a.ndelt_max));
     const double dtbk = a.dt / ndelt;
     const int start = m.elem_wet[sr.z] ? sr.z : sr.w;
-    btrack(m, a, s, k, dtbk, start, bx, by, foot, status);
+    btrack(m, sig, a, s, k, dtbk, start, bx, by, foot, status);
   }
   m.btu[2 * idx] = bx;
   m.btu[2 * idx + 1] = by;
@@ -356,8 +382,17 @@ __global__ void __launch_bounds__(256) k_backtrack(DMesh m, StepArgs a, const in
   m.bstatus[idx] = status;
 }
 
+// Packs (x, y) and (eta2, depth) of every node next to each other for the backtracking gathers.
+__global__ void k_pack_nodes(DMesh m) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.n) return;
+  m.nrec[2 * i] = make_double2(m.x[i], m.y[i]);
+  m.nrec[2 * i + 1] = make_double2(m.eta2[i], m.depth[i]);
+}
+
 void launch_backtrack(const DMesh& m, const StepArgs& a, const int32_t* ls, const int32_t* ll, int32_t count,
                       hipStream_t st) {
+  if (m.n > 0) hipLaunchKernelGGL(k_pack_nodes, dim3((m.n + kTpb - 1) / kTpb), dim3(kTpb), 0, st, m);
   const int64_t total = ls ? (int64_t)count : (int64_t)m.ns * m.nvrt;
   if (total <= 0) return;
   hipLaunchKernelGGL(k_backtrack, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, m, a, ls, ll, count);
