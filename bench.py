@@ -483,7 +483,7 @@ def main() -> int:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         osr = O.OracleSsor(ip_h, ix_h, args.ssor_rows_per_thread)
         reps, spent, xo, ito = 0, 0.0, None, 0
-        while reps < 10 and (spent < 10.0 or reps < 2):
+        while reps < 40 and (spent < 10.0 or reps < 2):
             a = time.perf_counter()
             xo, ito, _, _ = osr.solve(val, bvec, rtol=rtol, maxiter=maxiter, check_every=check_every,
                                       nthreads=threads)
