@@ -944,6 +944,7 @@ int gsh_mesh_create(gsh_ctx* ctx, const gsh_mesh_desc* desc, gsh_mesh** out, int
     d.gradphi = P.upload(m->geo.gradphi);
     d.elem_wet = P.alloc<int8_t>(ne);
     d.elem_kb = P.alloc<int32_t>(ne);
+    d.econ = P.alloc<double>(24 * (int64_t)ne);
     d.ghat1 = P.alloc<double>(2 * (int64_t)ne);
     d.vsource = P.alloc<double>(ne);
     d.side = P.upload(srec);

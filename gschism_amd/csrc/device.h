@@ -39,6 +39,7 @@ struct DMesh {
   int32_t* elem_kb;
   double* ghat1;            // [ne][2]
   double* vsource;
+  double* econ;             // [ne][3][8]: assembly terms of each (element, corner)
   // sides
   const int4* side;         // [ns]: (n1, n2, e0, e1)
   const int8_t* side_type;  // 0 internal, 1 land, 2 open

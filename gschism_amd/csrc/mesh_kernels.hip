@@ -556,78 +556,125 @@ void launch_rhs(const DMesh& m, const StepArgs& a, hipStream_t st) {
 
 // ---------------------------------------------------------------- a11+a12: assembly + BC
 // setupCoefficientMatrixAndRHS for triangles, imm=0 (meshdata.cu:3182-3311), fused with
-// setEssentialBC (meshdata.cu:4400-4446). One thread per node owns its CSR row (node-gather,
-// no atomics). Open boundaries carry an essential BC here, so the natural-BC integrals I3/I5
-// of open nodes (meshdata.cu:3380-3470) are overwritten and not evaluated.
-__global__ void __launch_bounds__(256) k_assemble(DMesh m, StepArgs a, double* __restrict__ val,
-                                                  double* __restrict__ b) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m.n) return;
-  const int row = m.indptr[i];
-  const int len = m.indptr[i + 1] - row;
-  double* rv = val + row;
-  for (int k = 0; k < len; ++k) rv[k] = 0.0;
-  double qel = 0.0;
-  const int diag = m.diag_off[i];
+// setEssentialBC (meshdata.cu:4400-4446), in two passes. k_elem_assemble evaluates, once per
+// element and corner id1, the terms the reference adds when node nodes[id1] visits the element
+// in its ball: the three matrix entries (diagonal, next, previous corner) and the five RHS
+// increments in their order (advective/ghat term, eta terms of corners 0-2, volume source).
+// k_assemble then gives each node its CSR row (node-gather, no atomics) by adding those terms
+// over its ball in ball order, so every sum has the reference's operands and order.
+// Open boundaries carry an essential BC here, so the natural-BC integrals I3/I5 of open nodes
+// (meshdata.cu:3380-3470) are overwritten and not evaluated.
+__global__ void __launch_bounds__(256) k_elem_assemble(DMesh m, StepArgs a) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m.ne) return;
+  const int4 r0 = m.elem[2 * e], r1 = m.elem[2 * e + 1];
+  const int nd[3] = {r0.x, r0.y, r0.z};
+  const int sd[3] = {r0.w, r1.x, r1.y};
   const double dt = a.dt, th = a.theta;
-  for (int j = m.ball_ptr[i]; j < m.ball_ptr[i + 1]; ++j) {
-    const int e = m.ball_elem[j];
-    const int id1 = m.ball_loc[j];
-    const int4 r0 = m.elem[2 * e], r1 = m.elem[2 * e + 1];
-    const int nd[3] = {r0.x, r0.y, r0.z};
-    const int sd[3] = {r0.w, r1.x, r1.y};
-    const double grav3 = (m.grav2[nd[0]] + m.grav2[nd[1]] + m.grav2[nd[2]] + 0.0) / 3;
-    const double hhatb = (m.hhat[sd[0]] + m.hhat[sd[1]] + m.hhat[sd[2]] + 0.0) / 3;
-    const double area = m.area[e];
-    const double Ux = (m.sU[2 * sd[0]] + m.sU[2 * sd[1]] + m.sU[2 * sd[2]] + 0.0) / 3;
-    const double Uy = (m.sU[2 * sd[0] + 1] + m.sU[2 * sd[1] + 1] + m.sU[2 * sd[2] + 1] + 0.0) / 3;
-    const double* g = m.gradphi + 6 * (int64_t)e;
-    const double g1x = g[2 * id1], g1y = g[2 * id1 + 1];
+  const double grav3 = (m.grav2[nd[0]] + m.grav2[nd[1]] + m.grav2[nd[2]] + 0.0) / 3;
+  const double hhatb = (m.hhat[sd[0]] + m.hhat[sd[1]] + m.hhat[sd[2]] + 0.0) / 3;
+  const double area = m.area[e];
+  const double Ux = (m.sU[2 * sd[0]] + m.sU[2 * sd[1]] + m.sU[2 * sd[2]] + 0.0) / 3;
+  const double Uy = (m.sU[2 * sd[0] + 1] + m.sU[2 * sd[1] + 1] + m.sU[2 * sd[2] + 1] + 0.0) / 3;
+  const double* g = m.gradphi + 6 * (int64_t)e;
+  double gp[6];
+  for (int l = 0; l < 6; ++l) gp[l] = g[l];
+  const double ghx = m.ghat1[2 * e], ghy = m.ghat1[2 * e + 1];
+  const double px[3] = {m.x[nd[0]], m.x[nd[1]], m.x[nd[2]]}, py[3] = {m.y[nd[0]], m.y[nd[1]], m.y[nd[2]]};
+  const double eta[3] = {m.eta2[nd[0]], m.eta2[nd[1]], m.eta2[nd[2]]};
+  const double wet = m.elem_wet[e] ? 1.0 : 0.0;
+  const double src = (dt / 3) * m.vsource[e];
+  const double fric = grav3 * th * (1.0 - th) * dt * dt * hhatb;
+  const double c = grav3 * th * th * dt * dt * hhatb;
+  double2* out = reinterpret_cast<double2*>(m.econ + 24 * (int64_t)e);
+  for (int id1 = 0; id1 < 3; ++id1) {
+    const double g1x = gp[2 * id1], g1y = gp[2 * id1 + 1];
     const double I41 = g1x * Ux + g1y * Uy;
-    const double I42 = g1x * m.ghat1[2 * e] + g1y * m.ghat1[2 * e + 1];
-    qel += dt * area * ((1 - th) * I41 + th * I42);
-    const double fric = grav3 * th * (1.0 - th) * dt * dt * hhatb;
+    const double I42 = g1x * ghx + g1y * ghy;
+    const double adv = dt * area * ((1 - th) * I41 + th * I42);
     const int id2 = (id1 + 1) % 3, id3 = (id1 + 2) % 3;
-    const double x0x = m.x[nd[id1]], x0y = m.y[nd[id1]];
-    const double x1x = m.x[nd[id2]], x1y = m.y[nd[id2]];
-    const double x2x = m.x[nd[id3]], x2y = m.y[nd[id3]];
+    const double x0x = px[id1], x0y = py[id1];
+    const double x1x = px[id2], x1y = py[id2];
+    const double x2x = px[id3], x2y = py[id3];
     const double ipp_ii = (1.0 / 6.0) * area, ipp_ij = (1.0 / 12.0) * area, ipp_ik = (1.0 / 12.0) * area;
     const double ax = x2x - x1x, ay = x2y - x1y;
     const double igpgp_ii = ((1.0 / 4.0) / area) * (ax * ax + ay * ay);
     const double igpgp_ij = ((1.0 / 4.0) / area) * (ax * (x0x - x2x) + ay * (x0y - x2y));
     const double igpgp_ik = -igpgp_ii - igpgp_ij;
-    const double c = grav3 * th * th * dt * dt * hhatb;
-    rv[diag] += ipp_ii + c * igpgp_ii;
-    rv[m.ball_slot[2 * j]] += ipp_ij + c * igpgp_ij;
-    rv[m.ball_slot[2 * j + 1]] += ipp_ik + c * igpgp_ik;
-    const double wet = m.elem_wet[e] ? 1.0 : 0.0;
+    double q[3];
     for (int l = 0; l < 3; ++l) {
-      const double eta = m.eta2[nd[l]];
       const double ipp = ((id1 == l) ? (2.0 / 12.0) : (1.0 / 12.0)) * area;
-      const double igpgp = area * (g1x * g[2 * l] + g1y * g[2 * l + 1]);
-      qel += (eta + 0.0) * ipp - eta * fric * igpgp * wet;
+      const double igpgp = area * (g1x * gp[2 * l] + g1y * gp[2 * l + 1]);
+      q[l] = (eta[l] + 0.0) * ipp - eta[l] * fric * igpgp * wet;
     }
-    qel += (dt / 3) * m.vsource[e];
+    out[4 * id1 + 0] = make_double2(ipp_ii + c * igpgp_ii, ipp_ij + c * igpgp_ij);
+    out[4 * id1 + 1] = make_double2(ipp_ik + c * igpgp_ik, adv);
+    out[4 * id1 + 2] = make_double2(q[0], q[1]);
+    out[4 * id1 + 3] = make_double2(q[2], src);
+  }
+}
+
+// Row accumulation for k_assemble: in LDS for rows of up to kRowLds entries, in the CSR values
+// themselves for longer rows; both add the same terms in the same order.
+constexpr int kRowLds = 16;
+
+template <class Row>
+__device__ __forceinline__ double assemble_row(const DMesh& m, const StepArgs& a, int i, int len, int diag, Row rv) {
+  for (int k = 0; k < len; ++k) rv(k) = 0.0;
+  double qel = 0.0;
+  const double2* con = reinterpret_cast<const double2*>(m.econ);
+  for (int j = m.ball_ptr[i]; j < m.ball_ptr[i + 1]; ++j) {
+    const int64_t base = 4 * (3 * (int64_t)m.ball_elem[j] + m.ball_loc[j]);
+    const double2 c0 = con[base], c1 = con[base + 1], c2 = con[base + 2], c3 = con[base + 3];
+    qel += c1.y;
+    rv(diag) += c0.x;
+    rv(m.ball_slot[2 * j]) += c0.y;
+    rv(m.ball_slot[2 * j + 1]) += c1.x;
+    qel += c2.x;
+    qel += c2.y;
+    qel += c3.x;
+    qel += c3.y;
   }
   const bool ess = a.iettype != 0;
   if (ess && m.open_bnd[i]) {
-    for (int k = 0; k < len; ++k) rv[k] = 0.0;
-    rv[diag] = 1.0;
+    for (int k = 0; k < len; ++k) rv(k) = 0.0;
+    rv(diag) = 1.0;
     qel = m.elbc[i];
   } else if (ess) {
     for (int j = m.nbr_ptr[i]; j < m.nbr_ptr[i + 1]; ++j) {
       const int nb = m.nbr_node[j];
       if (m.open_bnd[nb]) {
         const int sl = m.csr_slot[j];
-        qel -= rv[sl] * m.elbc[nb];
-        rv[sl] = 0.0;
+        qel -= rv(sl) * m.elbc[nb];
+        rv(sl) = 0.0;
       }
     }
+  }
+  return qel;
+}
+
+__global__ void __launch_bounds__(256) k_assemble(DMesh m, StepArgs a, double* __restrict__ val,
+                                                  double* __restrict__ b) {
+  __shared__ double rows[kRowLds][256];  // entry k of thread t's row at rows[k][t]
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m.n) return;
+  const int row = m.indptr[i];
+  const int len = m.indptr[i + 1] - row;
+  const int diag = m.diag_off[i];
+  double qel;
+  if (len <= kRowLds) {
+    const int t = threadIdx.x;
+    qel = assemble_row(m, a, i, len, diag, [&](int k) -> double& { return rows[k][t]; });
+    for (int k = 0; k < len; ++k) val[row + k] = rows[k][t];
+  } else {
+    double* rv = val + row;
+    qel = assemble_row(m, a, i, len, diag, [&](int k) -> double& { return rv[k]; });
   }
   b[i] = qel;
 }
 
 void launch_assemble(const DMesh& m, const StepArgs& a, double* val, double* b, hipStream_t st) {
+  if (m.ne > 0) hipLaunchKernelGGL(k_elem_assemble, dim3((m.ne + kTpb - 1) / kTpb), dim3(kTpb), 0, st, m, a);
   if (m.n > 0) hipLaunchKernelGGL(k_assemble, dim3((m.n + kTpb - 1) / kTpb), dim3(kTpb), 0, st, m, a, val, b);
 }
 
