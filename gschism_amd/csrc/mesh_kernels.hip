@@ -100,27 +100,34 @@ __device__ __forceinline__ ElemGeom elem_geom(const DMesh& m, int e) {
   return {r.x, r.y, r.z, p0.x, p0.y, p1.x, p1.y, p2.x, p2.y};
 }
 
-// Element::interpolateVelocities for triangles, ics=1 (meshdata.cu:1209-1291) with
-// computeShapeCoords(fiAnywhere) (meshdata.cu:990-1043); g is element e's geometry.
+// A node's elevation, depth, total depth H = eta + h, bottom level and dry flag, in registers.
 struct NodeZ {
-  double eta, h;
+  double eta, h, H;
   int kb;
   bool dry;
 };
 
 __device__ __forceinline__ NodeZ node_z(const DMesh& m, double h0, int node) {
   const double2 q = m.nrec[2 * node + 1];
-  return {q.x, q.y, m.kb[node], q.x + q.y <= h0};
+  const double H = q.x + q.y;
+  return {q.x, q.y, H, m.kb[node], H <= h0};
 }
 
-// zcoord from a node's values held in registers
-__device__ __forceinline__ double zcoord_q(const NodeZ& q, const double* sig, int nv, int k) {
-  if (q.dry) return 0.0;
-  if (k <= q.kb) return -q.h;
-  if (k == nv - 1) return q.eta;
-  return (q.eta + q.h) * sig[k] + q.eta;
+// zcoord(node, max(M, kb)) with sM = sigma[M]: for M <= kb that is the bed -h, above kb level M.
+__device__ __forceinline__ double zlev(const NodeZ& q, double sM, int M, int nv) {
+  const double zs = (M == nv - 1) ? q.eta : q.H * sM + q.eta;
+  return q.dry ? 0.0 : ((M <= q.kb) ? -q.h : zs);
 }
 
+// z of level M at the point with shape coordinates s0..s2 (the vertical bracket search's probe)
+__device__ __forceinline__ double zprobe(const NodeZ& q0, const NodeZ& q1, const NodeZ& q2, double s0, double s1,
+                                         double s2, const double* sig, int M, int nv) {
+  const double sM = sig[max(M, 0)];
+  return s0 * zlev(q0, sM, M, nv) + s1 * zlev(q1, sM, M, nv) + s2 * zlev(q2, sM, M, nv) + 0.0;
+}
+
+// Element::interpolateVelocities for triangles, ics=1 (meshdata.cu:1209-1291) with
+// computeShapeCoords(fiAnywhere) (meshdata.cu:990-1043); g is element e's geometry.
 __device__ void interp_velocity(const DMesh& m, const double* sig, double h0, int e, const ElemGeom& g, double px,
                                 double py, double pz, double& uo, double& vo, double& wo) {
   const int n0 = g.n0, n1 = g.n1, n2 = g.n2;
@@ -141,10 +148,8 @@ __device__ void interp_velocity(const DMesh& m, const double* sig, double h0, in
   const NodeZ q0 = node_z(m, h0, n0), q1 = node_z(m, h0, n1), q2 = node_z(m, h0, n2);
   const int kb0 = q0.kb, kb1 = q1.kb, kb2 = q2.kb;
   int L = m.elem_kb[e] - 1, R = nv - 1;
-  double zL = s0 * zcoord_q(q0, sig, nv, max(L, kb0)) + s1 * zcoord_q(q1, sig, nv, max(L, kb1)) +
-              s2 * zcoord_q(q2, sig, nv, max(L, kb2)) + 0.0;
-  double zR = s0 * zcoord_q(q0, sig, nv, max(R, kb0)) + s1 * zcoord_q(q1, sig, nv, max(R, kb1)) +
-              s2 * zcoord_q(q2, sig, nv, max(R, kb2)) + 0.0;
+  double zL = zprobe(q0, q1, q2, s0, s1, s2, sig, L, nv);
+  double zR = zprobe(q0, q1, q2, s0, s1, s2, sig, R, nv);
   double w1, w2;
   if (pz <= zL) {
     w1 = 1.0;
@@ -155,8 +160,7 @@ __device__ void interp_velocity(const DMesh& m, const double* sig, double h0, in
   } else {
     while (R - L > 1) {
       const int M = (L + R) / 2;
-      const double zM = s0 * zcoord_q(q0, sig, nv, max(M, kb0)) + s1 * zcoord_q(q1, sig, nv, max(M, kb1)) +
-                        s2 * zcoord_q(q2, sig, nv, max(M, kb2)) + 0.0;
+      const double zM = zprobe(q0, q1, q2, s0, s1, s2, sig, M, nv);
       if (zM < pz) {
         L = M;
         zL = zM;

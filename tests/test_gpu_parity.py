@@ -101,6 +101,49 @@ def test_rows_per_thread(ctx, rpt):
     _full_chain(ctx, mesh, om, p, rows_per_thread=rpt)
 
 
+def _wheel(k=24, r1=1500.0, r2=3000.0):
+    """A centre node with k spokes (CSR row of k + 1 entries) inside two rings of k nodes."""
+    a1 = 2 * np.pi * np.arange(k) / k
+    a2 = a1 + np.pi / k
+    xy = np.concatenate([[[0.0, 0.0]], np.c_[r1 * np.cos(a1), r1 * np.sin(a1)],
+                         np.c_[r2 * np.cos(a2), r2 * np.sin(a2)]])
+    tris = []
+    for i in range(k):
+        j = (i + 1) % k
+        tris += [(0, 1 + i, 1 + j), (1 + i, 1 + k + i, 1 + j), (1 + j, 1 + k + i, 1 + k + j)]
+    el = np.array(tris, dtype=np.int32)
+    p0, p1, p2 = xy[el[:, 0]], xy[el[:, 1]], xy[el[:, 2]]
+    cw = (p1[:, 0] - p0[:, 0]) * (p2[:, 1] - p0[:, 1]) - (p2[:, 0] - p0[:, 0]) * (p1[:, 1] - p0[:, 1]) < 0
+    el[cw] = el[cw][:, [0, 2, 1]]
+    open_bnd = np.zeros(xy.shape[0], dtype=np.int8)
+    open_bnd[1 + k:1 + k + 5] = 1
+    return xy, el, open_bnd
+
+
+def test_chain_high_degree_node(ctx):
+    """Rows longer than the assembly's LDS row buffer (16 entries) take the in-place path."""
+    spec = cases.CaseSpec("wheel-24", 13, 13, 500.0, nvrt=11, depth=10.0, substeps=5)
+    xy, el, ob = _wheel()
+    sigma = cases.sigma_levels(spec.nvrt)
+    depth = 10.0 + 0.001 * xy[:, 0]
+    mesh = core.Mesh(ctx, xy, el, depth, sigma, open_bnd=ob)
+    ip = mesh.topology(L.T_CSR_INDPTR)
+    assert np.diff(ip).max() == 25
+    xym = mesh.topology(L.T_XY)
+    eta = 0.1 * np.sin(2 * np.pi * xym[:, 0] / 6000.0)
+    uvw = cases.velocity(xym, sigma, 6000.0, 6000.0, 500.0, 3)
+    su2 = cases.velocity(cases.side_centres(mesh), sigma, 6000.0, 6000.0, 500.0, 2)
+    p = cases.step_params(spec)
+    om = oracle_mesh_for(mesh, sigma)
+    for m in (mesh, om):
+        m.set(L.F_ETA2, eta)
+        m.set(L.F_NODE_UVW, uvw)
+        m.set(L.F_SIDE_U2, su2)
+    mesh.update_levels(p)
+    om.levels(p)
+    _full_chain(ctx, mesh, om, p, rows_per_thread=1)
+
+
 def test_three_steps_with_history(ctx):
     """Consecutive steps: extrapolated initial guess 3x0-3x1+x2, eta update, levels update."""
     spec = cases.CaseSpec("channel-60x30", 60, 30, 500.0, nvrt=8, depth=None, substeps=5)
