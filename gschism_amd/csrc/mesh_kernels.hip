@@ -405,15 +405,30 @@ void launch_backtrack(const DMesh& m, const StepArgs& a, const int32_t* ls, cons
 // ---------------------------------------------------------------- a8-a10, elevation BC
 // computeVerticallyIntegratedVelocity (meshdata.cu:4459-4476) and
 // computeVegetationAndDragParameters for iveg=0, nchi=0 (meshdata.cu:4615-4725).
-// A workgroup takes g consecutive sides: its four waves copy the sides' zc and velocity rows
-// (contiguous in HBM) into LDS with coalesced loads, one thread per side then sums its levels in
-// the reference's order from LDS, and the velocity rows are replaced by the btu rows for the
-// second integral. Row strides in LDS are odd so the per-side reads spread over the banks.
+// A workgroup takes g consecutive sides: the sides' zc and velocity rows are one contiguous
+// span of HBM each, which the workgroup copies into LDS with flat coalesced loads; one thread
+// per side then sums its levels in the reference's order from LDS, and the velocity rows are
+// replaced by the btu rows for the second integral. Row strides in LDS are odd so the
+// per-side reads spread over the banks.
 __device__ __forceinline__ void stage_rows(double* __restrict__ dst, int stride, const double* __restrict__ src,
                                            int row_len, int rows) {
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  for (int g = wv; g < rows; g += kTpb / 64)
-    for (int j = lane; j < row_len; j += 64) dst[g * stride + j] = src[(int64_t)g * row_len + j];
+  const int total = rows * row_len;
+  const float inv = 1.0f / (float)row_len;
+  for (int i = threadIdx.x; i < total; i += blockDim.x) {
+    // row g = i / row_len from a float reciprocal (exact here: i < 2^20, row_len <= 2048);
+    // the two corrections keep it exact whatever the rounding
+    int g = (int)(((float)i + 0.5f) * inv);
+    int j = i - g * row_len;
+    if (j < 0) {
+      --g;
+      j += row_len;
+    }
+    if (j >= row_len) {
+      ++g;
+      j -= row_len;
+    }
+    dst[g * stride + j] = src[i];
+  }
 }
 
 __global__ void __launch_bounds__(256) k_side_rhs(DMesh m, StepArgs a, int g) {
@@ -550,9 +565,10 @@ __global__ void k_elbc(DMesh m, StepArgs a) {
 void launch_rhs(const DMesh& m, const StepArgs& a, hipStream_t st) {
   if (m.n > 0) hipLaunchKernelGGL(k_elbc, dim3((m.n + kTpb - 1) / kTpb), dim3(kTpb), 0, st, m, a);
   if (m.ns > 0) {
-    // sides per workgroup: as many as fit 32 KB of LDS, at most one per thread
+    // sides per workgroup: as many as fit 16 KB of LDS (measured best against 4-64 KB at 1M
+    // sides x 31 levels), at most one per thread
     const int row = 8 * ((m.nvrt | 1) + ((2 * m.nvrt) | 1));
-    const int g = max(1, min(kTpb, (32 * 1024) / row));
+    const int g = max(1, min(kTpb, (16 * 1024) / row));
     hipLaunchKernelGGL(k_side_rhs, dim3((unsigned)((m.ns + g - 1) / g)), dim3(kTpb), (size_t)g * row, st, m, a, g);
   }
   if (m.ne > 0) hipLaunchKernelGGL(k_elem_ghat, dim3((m.ne + kTpb - 1) / kTpb), dim3(kTpb), 0, st, m, a);
