@@ -129,7 +129,7 @@ __global__ void __launch_bounds__(256) k_pcg_init(const PcgSys<VT>* __restrict__
                                                   const TileRef* __restrict__ tiles) {
   __shared__ double red[4];
   __shared__ VT lds[kSpmvLdsCap];
-  const TileRef tr = tiles[blockIdx.x];
+  const TileRef tr = tiles[xcd_block(blockIdx.x, gridDim.x)];
   const SysView<VT> S = view(sys, tr.sys);
   const int n = S.n;
   const gptr<const double> h0 = S.h0, h1 = S.h1, h2 = S.h2;
@@ -217,7 +217,7 @@ __global__ void __launch_bounds__(256) k_pcg_phase_a(const PcgSys<VT>* __restric
                                                      const TileRef* __restrict__ tiles) {
   __shared__ double red[4];
   __shared__ VT lds[4 * kWaveCap];
-  const TileRef tr = tiles[blockIdx.x];
+  const TileRef tr = tiles[xcd_block(blockIdx.x, gridDim.x)];
   const SysView<VT> S = view(sys, tr.sys);
   gptr<PcgState> st = S.st;
   if (load_done(st) >= 0) return;
@@ -403,7 +403,7 @@ __global__ void __launch_bounds__(256) k_true_residual(const PcgSys<double>* __r
                                                        const double* __restrict__ xv_, double* __restrict__ rvec_) {
   __shared__ double red[4];
   __shared__ double lds[kSpmvLdsCap];
-  const TileRef tr = tiles[blockIdx.x];
+  const TileRef tr = tiles[xcd_block(blockIdx.x, gridDim.x)];
   const SysView<double> S = view(sys, tr.sys);
   const int n = S.n;
   const gptr<const double> xv = G(xv_);
@@ -438,7 +438,7 @@ __global__ void __launch_bounds__(256) k_residual_batch(const PcgSys<double>* __
                                                         const TileRef* __restrict__ tiles) {
   __shared__ double red[4];
   __shared__ double lds[kSpmvLdsCap];
-  const TileRef tr = tiles[blockIdx.x];
+  const TileRef tr = tiles[xcd_block(blockIdx.x, gridDim.x)];
   const SysView<double> S = view(sys, tr.sys);
   const int n = S.n;
   const gptr<const double> xv = S.x;

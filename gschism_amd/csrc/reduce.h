@@ -18,6 +18,14 @@ __device__ __forceinline__ gptr<T> G(T* p) {
   return (gptr<T>)p;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (block b runs on XCD b % 8). This maps
+// the blocks of each XCD onto one contiguous range of the grid's nb work items, so neighbouring
+// tiles, whose gathers overlap, share that XCD's L2. It is a bijection of [0, nb).
+__device__ __forceinline__ int xcd_block(int b, int nb) {
+  const int q = nb >> 3, r = nb & 7, x = b & 7, i = b >> 3;
+  return x * q + min(x, r) + i;
+}
+
 __device__ __forceinline__ int32_t load_done(gptr<PcgState> st) {
   return *(volatile __attribute__((address_space(1))) int32_t*)&st->done;
 }

@@ -205,7 +205,7 @@ template <int R>
 __global__ void __launch_bounds__(256) k_ssor_r0(const SsorSys* __restrict__ sys, int tile0, int c) {
   const SsorView S = sview(sys);
   const ColourInfo ci = colour_info(S, c);
-  const int tile = tile0 + blockIdx.x;
+  const int tile = tile0 + xcd_block(blockIdx.x, gridDim.x);
   const gptr<const double> r = C(S.r);
   for (int j = 0; j < R; ++j) {
     const int v = tile_row<R>(tile, j);
@@ -229,7 +229,7 @@ __global__ void __launch_bounds__(256) k_ssor_bwd(const SsorSys* __restrict__ sy
   if (load_done(st) >= 0) return;
   const bool init = flags & kInit, mid = flags & kMid;
   const ColourInfo ci = colour_info(S, c);
-  const int tile = tile0 + blockIdx.x;
+  const int tile = tile0 + xcd_block(blockIdx.x, gridDim.x);
   const gptr<const double> t = C(S.t);
   double usum[R], dtv[R], rv[R], pv[R], qv[R], sv[R], yv[R];
 #pragma unroll
@@ -335,7 +335,7 @@ __global__ void __launch_bounds__(256) k_ssor_fwd(const SsorSys* __restrict__ sy
   const SsorView S = sview(sys);
   if (load_done(S.st) >= 0) return;
   const ColourInfo ci = colour_info(S, c);
-  const int tile = tile0 + blockIdx.x;
+  const int tile = tile0 + xcd_block(blockIdx.x, gridDim.x);
   const int wslot = (flags & kInit) ? 0 : ((S.st->kk + 1) & 1);
   const gptr<const double> u = C(S.u);
   double lsum[R], dtv[R], tv[R], rv[R], kdv[R];
@@ -372,16 +372,17 @@ __global__ void __launch_bounds__(256) k_ssor_true_rr(const SsorSys* __restrict_
   __shared__ double red[4];
   const SsorView S = sview(sys);
   if (load_done(S.st) >= 0) return;
-  const ColourInfo ci = colour_info(S, S.tile_colour[blockIdx.x]);
+  const int tile = xcd_block(blockIdx.x, gridDim.x);
+  const ColourInfo ci = colour_info(S, S.tile_colour[tile]);
   const gptr<const double> r = C(S.r);
   double rr = 0.0;
   for (int j = 0; j < R; ++j) {
-    const int v = tile_row<R>(blockIdx.x, j);
+    const int v = tile_row<R>(tile, j);
     const double wv = S.dt[v] * r[v] + lower_sum(S, ci, v, r);
     rr += wv * wv;
   }
   rr = block_tree(rr, red);
-  if (threadIdx.x == 0) S.part_rr[blockIdx.x] = rr;
+  if (threadIdx.x == 0) S.part_rr[tile] = rr;
 }
 
 __global__ void __launch_bounds__(256) k_ssor_check(const SsorSys* __restrict__ sys) {
@@ -408,7 +409,7 @@ __global__ void __launch_bounds__(256) k_ssor_xsolve(const SsorSys* __restrict__
   const SsorView S = sview(sys);
   if (S.st->zero) return;
   const ColourInfo ci = colour_info(S, c);
-  const int tile = tile0 + blockIdx.x;
+  const int tile = tile0 + xcd_block(blockIdx.x, gridDim.x);
   const gptr<const double> x = C(S.w);
   for (int j = 0; j < R; ++j) {
     const int v = tile_row<R>(tile, j);
