@@ -56,30 +56,63 @@ __global__ void k_node_wet(DMesh m) {
 }
 
 // ul0UpdateSides / ul0UpdateSides2 (meshdata.cu:5679-5760): side kb, zc, wet flag. One thread per
-// (side, level), so consecutive threads write consecutive zc entries.
-__global__ void k_side_levels(DMesh m, double h0) {
-  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t nv = (uint32_t)m.nvrt;
-  if (t >= (uint64_t)m.ns * nv) return;
-  const int s = (t < 0xffffffffull) ? (int)((uint32_t)t / nv) : (int)(t / nv);
-  const int k = (int)(t - (uint64_t)s * nv);
-  const int4 r = m.side[s];
-  const bool wet = m.elem_wet[r.z] || (r.w >= 0 && m.elem_wet[r.w]);
-  const int kb1 = m.kb[r.x], kb2 = m.kb[r.y];
-  const int kb = wet ? min(kb1, kb2) : 0;
-  if (wet)
-    m.zc[t] = (k < kb) ? 0.0 : 0.5 * (zcoord(m, h0, r.x, max(k, kb1)) + zcoord(m, h0, r.y, max(k, kb2)));
-  if (k == 0) {
+// side (g sides per workgroup) computes its levels into LDS (the node values are read once per side); the workgroup then
+// stores its sides' zc rows, one contiguous span of HBM, with flat coalesced stores. Dry sides
+// keep their zc, as in the reference.
+constexpr int kLevelThreads = 128;
+
+__global__ void __launch_bounds__(kLevelThreads) k_side_levels(DMesh m, double h0, int g) {
+  extern __shared__ double lz[];  // [g][nv | 1]
+  __shared__ int8_t lwet[kLevelThreads];
+  const int nv = m.nvrt, zs = nv | 1;
+  const int s0 = blockIdx.x * g;
+  const int gc = min(g, m.ns - s0);
+  const int t = threadIdx.x;
+  if (t < gc) {
+    const int s = s0 + t;
+    const int4 r = m.side[s];
+    const bool wet = m.elem_wet[r.z] || (r.w >= 0 && m.elem_wet[r.w]);
+    const int kb1 = m.kb[r.x], kb2 = m.kb[r.y];
+    const int kb = wet ? min(kb1, kb2) : 0;
+    if (wet) {
+      double* z = lz + t * zs;
+      for (int k = 0; k < kb; ++k) z[k] = 0.0;
+      for (int k = kb; k < nv; ++k)
+        z[k] = 0.5 * (zcoord(m, h0, r.x, max(k, kb1)) + zcoord(m, h0, r.y, max(k, kb2)));
+    }
+    lwet[t] = wet ? 1 : 0;
     m.side_kb[s] = kb;
     m.side_wet[s] = wet ? 1 : 0;
+  }
+  __syncthreads();
+  const int total = gc * nv;
+  const float inv = 1.0f / (float)nv;
+  double* zc = m.zc + (int64_t)s0 * nv;
+  for (int i = t; i < total; i += kLevelThreads) {
+    // side j = i / nv from a float reciprocal (exact for i < 2^17), corrected either way
+    int j = (int)(((float)i + 0.5f) * inv);
+    int k = i - j * nv;
+    if (k < 0) {
+      --j;
+      k += nv;
+    }
+    if (k >= nv) {
+      ++j;
+      k -= nv;
+    }
+    if (lwet[j]) zc[i] = lz[j * zs + k];
   }
 }
 
 void launch_levels(const DMesh& m, const StepArgs& a, hipStream_t st) {
   if (m.ne > 0) hipLaunchKernelGGL(k_elem_levels, dim3((m.ne + kTpb - 1) / kTpb), dim3(kTpb), 0, st, m, a.h0);
   if (m.n > 0) hipLaunchKernelGGL(k_node_wet, dim3((m.n + kTpb - 1) / kTpb), dim3(kTpb), 0, st, m);
-  const int64_t sl = (int64_t)m.ns * m.nvrt;
-  if (sl > 0) hipLaunchKernelGGL(k_side_levels, dim3((unsigned)((sl + kTpb - 1) / kTpb)), dim3(kTpb), 0, st, m, a.h0);
+  if (m.ns > 0) {
+    // sides per workgroup: one per thread, at most 32 KB of LDS
+    const int g = max(1, min(kLevelThreads, 32 * 1024 / (8 * (m.nvrt | 1))));
+    hipLaunchKernelGGL(k_side_levels, dim3((m.ns + g - 1) / g), dim3(kLevelThreads), (size_t)8 * g * (m.nvrt | 1), st,
+                       m, a.h0, g);
+  }
 }
 
 // ---------------------------------------------------------------- a4-a7: ELM backtracking
