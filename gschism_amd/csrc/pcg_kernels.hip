@@ -471,8 +471,8 @@ __global__ void __launch_bounds__(256) k_reduce_systems(const PcgSys<double>* __
                                                         double* __restrict__ out) {
   __shared__ double red[4];
   const PcgSys<double>& S = sys[blockIdx.x];
-  const double a = final_reduce(G((const double*)S.part_rr), S.ntiles, red);
-  const double b = final_reduce(G((const double*)S.part_bb), S.ntiles, red);
+  double a, b;
+  final_reduce2(G((const double*)S.part_rr), G((const double*)S.part_bb), S.ntiles, red, a, b);
   if (threadIdx.x == 0) {
     out[2 * blockIdx.x] = a;
     out[2 * blockIdx.x + 1] = b;
@@ -498,8 +498,8 @@ __global__ void __launch_bounds__(256) k_refine_batch(const PcgSys<double>* __re
 __global__ void k_reduce_pair(const double* __restrict__ pa, const double* __restrict__ pb, int T,
                               double* __restrict__ out) {
   __shared__ double red[4];
-  const double a = final_reduce(G(pa), T, red);
-  const double b = final_reduce(G(pb), T, red);
+  double a, b;
+  final_reduce2(G(pa), G(pb), T, red, a, b);
   if (threadIdx.x == 0) {
     out[0] = a;
     out[1] = b;

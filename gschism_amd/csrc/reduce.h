@@ -60,10 +60,42 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 // Sum of the tile partials P[0..T): thread t sums P[t], P[t+256], ... then block_tree.
+// Thread t adds P[t], P[t + 256], ... in that order; the loads of eight consecutive terms are
+// issued together before their adds (a loop of single loads waits one memory latency per term).
 __device__ __forceinline__ double final_reduce(gptr<const double> P, int T, double* red) {
   double u = 0.0;
-  for (int m = threadIdx.x; m < T; m += kTpb) u += P[m];
+  for (int m = threadIdx.x; m < T; m += 8 * kTpb) {
+    double v[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = (m + j * kTpb < T) ? P[m + j * kTpb] : 0.0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if (m + j * kTpb < T) u += v[j];
+  }
   return block_tree(u, red);
+}
+
+// final_reduce of two partial arrays of T entries each, with the loads of both issued together.
+__device__ __forceinline__ void final_reduce2(gptr<const double> P, gptr<const double> Q, int T, double* red,
+                                              double& a, double& b) {
+  double u = 0.0, w = 0.0;
+  for (int m = threadIdx.x; m < T; m += 8 * kTpb) {
+    double vp[8], vq[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const bool in = m + j * kTpb < T;
+      vp[j] = in ? P[m + j * kTpb] : 0.0;
+      vq[j] = in ? Q[m + j * kTpb] : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j)
+      if (m + j * kTpb < T) {
+        u += vp[j];
+        w += vq[j];
+      }
+  }
+  a = block_tree(u, red);
+  b = block_tree(w, red);
 }
 
 }  // namespace gsh

@@ -252,8 +252,9 @@ __global__ void __launch_bounds__(256) k_ssor_bwd(const SsorSys* __restrict__ sy
     if (flags & kFirst) {
       const int k = st->k;
       wslot = (k + 1) & 1;
-      const double gamma = final_reduce(C(S.part_rz) + (k & 1) * S.ntiles, S.ntiles, red);
-      const double delta = final_reduce(C(S.part_pq) + (k & 1) * S.ntiles, S.ntiles, red);
+      double gamma, delta;
+      final_reduce2(C(S.part_rz) + (k & 1) * S.ntiles, C(S.part_pq) + (k & 1) * S.ntiles, S.ntiles, red, gamma,
+                    delta);
       if (k == 0) {
         alpha = (delta != 0.0) ? gamma / delta : 0.0;
         if (blockIdx.x == 0) {
