@@ -26,8 +26,10 @@ __device__ __forceinline__ int xcd_block(int b, int nb) {
   return x * q + min(x, r) + i;
 }
 
+// The done flag is written by earlier launches only; a plain (cached) load sees it after the
+// launch boundary.
 __device__ __forceinline__ int32_t load_done(gptr<PcgState> st) {
-  return *(volatile __attribute__((address_space(1))) int32_t*)&st->done;
+  return st->done;
 }
 
 __device__ __forceinline__ double wave_tree(double v) {

@@ -226,12 +226,12 @@ __global__ void __launch_bounds__(256) k_ssor_bwd(const SsorSys* __restrict__ sy
   __shared__ double red[4];
   const SsorView S = sview(sys);
   gptr<PcgState> st = S.st;
-  if (load_done(st) >= 0) return;
   const bool init = flags & kInit, mid = flags & kMid;
   const ColourInfo ci = colour_info(S, c);
   const int tile = tile0 + xcd_block(blockIdx.x, gridDim.x);
   const gptr<const double> t = C(S.t);
   double usum[R], dtv[R], rv[R], pv[R], qv[R], sv[R], yv[R];
+  // the row loads are issued before the done test so that their latency overlaps it
 #pragma unroll
   for (int j = 0; j < R; ++j) {
     const int v = tile_row<R>(tile, j);
@@ -244,6 +244,7 @@ __global__ void __launch_bounds__(256) k_ssor_bwd(const SsorSys* __restrict__ sy
       yv[j] = S.y[v];
     }
   }
+  if (load_done(st) >= 0) return;
   // the gamma / delta partials of iteration k live in slot k & 1, so the first sweep can read
   // slot k & 1 while the sweeps write the partials of iteration k + 1 into the other slot
   double alpha = 0.0, beta = 0.0;
@@ -334,12 +335,11 @@ template <int R>
 __global__ void __launch_bounds__(256) k_ssor_fwd(const SsorSys* __restrict__ sys, int tile0, int c, int flags) {
   __shared__ double red[4];
   const SsorView S = sview(sys);
-  if (load_done(S.st) >= 0) return;
   const ColourInfo ci = colour_info(S, c);
   const int tile = tile0 + xcd_block(blockIdx.x, gridDim.x);
-  const int wslot = (flags & kInit) ? 0 : ((S.st->kk + 1) & 1);
   const gptr<const double> u = C(S.u);
   double lsum[R], dtv[R], tv[R], rv[R], kdv[R];
+  // the row loads are issued before the done test so that their latency overlaps it
 #pragma unroll
   for (int j = 0; j < R; ++j) {
     const int v = tile_row<R>(tile, j);
@@ -347,8 +347,11 @@ __global__ void __launch_bounds__(256) k_ssor_fwd(const SsorSys* __restrict__ sy
     tv[j] = S.t[v];
     rv[j] = S.r[v];
     kdv[j] = S.unit_omega ? dtv[j] : S.kd[v];
-    lsum[j] = lower_sum(S, ci, v, u);
   }
+  if (load_done(S.st) >= 0) return;
+  const int wslot = (flags & kInit) ? 0 : ((S.st->kk + 1) & 1);
+#pragma unroll
+  for (int j = 0; j < R; ++j) lsum[j] = lower_sum(S, ci, tile_row<R>(tile, j), u);
   double dl = 0.0;
 #pragma unroll
   for (int j = 0; j < R; ++j) {
