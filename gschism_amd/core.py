@@ -71,7 +71,7 @@ _FIELD_SPEC = {
     L.F_SIDE_U: (np.float64, "ns", (2,)), L.F_SIDE_HHAT: (np.float64, "ns", ()),
     L.F_SIDE_CHI: (np.float64, "ns", ()), L.F_GHAT1: (np.float64, "ne", (2,)),
     L.F_ZC: (np.float64, "ns", ("nvrt",)), L.F_ELBC: (np.float64, "n", ()),
-    L.F_DEPTH: (np.float64, "n", ()),
+    L.F_DEPTH: (np.float64, "n", ()), L.F_ELEM_W: (np.float64, "ne", ("nvrt",)),
 }
 
 
@@ -184,6 +184,10 @@ class Mesh:
 
     def rhs_producers(self, params) -> None:
         check(L.lib().gsh_rhs_producers(self.h, C.byref(params)))
+
+    def nodal_velocity(self, params) -> None:
+        """Nodal u2/w2 (GSH_F_NODE_UVW) from side u2 and element w (GSH_F_ELEM_W)."""
+        check(L.lib().gsh_nodal_velocity(self.h, C.byref(params)))
 
 
 class Solver:

@@ -822,6 +822,7 @@ void gsh_step_params_default(gsh_step_params* p) {
   p->velmin_btrack = 1e-4;
   p->ihhat = 1;
   p->iettype = 2;
+  p->rmaxvel = 5.0;  // readNamelist.py:192
 }
 
 static StepArgs step_args(const gsh_step_params* p) {
@@ -1063,8 +1064,16 @@ static FieldRef field_ref(const gsh_mesh* m, int field) {
     case GSH_F_ELBC: return {d.elbc, n, 8};
     case GSH_F_WET: return {nullptr, n + ne + ns, 1};
     case GSH_F_DEPTH: return {const_cast<double*>(d.depth), n, 8};
+    case GSH_F_ELEM_W: return {d.ew, ne * nv, 8};
     default: throw Error(GSH_E_ARG, "unknown field");
   }
+}
+
+// The element vertical velocity is only read by gsh_nodal_velocity; its buffer (zero) is taken
+// from the mesh's pool when it is first set, exported or used.
+static void ensure_elem_w(gsh_mesh* m) {
+  GSH_REQUIRE(m->ctx, GSH_E_STATE, "host-only mesh (created without a context)");
+  if (!m->d.ew) m->d.ew = m->pool.alloc<double>((int64_t)m->d.ne * m->d.nvrt);
 }
 
 int gsh_mesh_set_field(gsh_mesh* m, int field, const void* host, int64_t count) {
@@ -1073,6 +1082,7 @@ int gsh_mesh_set_field(gsh_mesh* m, int field, const void* host, int64_t count) 
     GSH_REQUIRE(m->ctx, GSH_E_STATE, "host-only mesh (created without a context)");
     m->ctx->set();
     GSH_REQUIRE(field != GSH_F_WET, GSH_E_ARG, "wet flags are derived by gsh_update_levels");
+    if (field == GSH_F_ELEM_W) ensure_elem_w(m);
     FieldRef f = field_ref(m, field);
     GSH_REQUIRE(count == f.count, GSH_E_ARG, "field size mismatch");
     hipStream_t st = m->ctx->stream;
@@ -1102,6 +1112,10 @@ int gsh_mesh_get_field(const gsh_mesh* m, int field, void* host, int64_t count) 
     FieldRef f = field_ref(m, field);
     GSH_REQUIRE(count == f.count, GSH_E_ARG, "field size mismatch");
     hipStream_t st = m->ctx->stream;
+    if (field == GSH_F_ELEM_W && !f.ptr) {  // never set: zero
+      std::memset(host, 0, (size_t)count * f.elem);
+      return GSH_OK;
+    }
     if (field == GSH_F_NODE_UVW) {
       double* tmp = nullptr;
       HIPCHK(hipMallocAsync((void**)&tmp, sizeof(double) * count, st));
@@ -1126,6 +1140,7 @@ int gsh_mesh_field_ptr(gsh_mesh* m, int field, void** dev_ptr, int64_t* count) {
     GSH_REQUIRE(m && dev_ptr, GSH_E_ARG, "NULL argument");
     GSH_REQUIRE(field != GSH_F_WET && field != GSH_F_NODE_UVW, GSH_E_ARG,
                 "this field has a device layout of its own; use gsh_mesh_get_field");
+    if (field == GSH_F_ELEM_W) ensure_elem_w(m);
     FieldRef f = field_ref(m, field);
     *dev_ptr = f.ptr;
     if (count) *count = f.count;
@@ -1175,6 +1190,18 @@ int gsh_backtrack_list(gsh_mesh* m, const gsh_step_params* p, const int32_t* sid
     launch_backtrack(m->d, step_args(p), buf, buf + count, count, st);
     HIPCHK(hipFreeAsync(buf, st));
     HIPCHK(hipStreamSynchronize(st));
+  })
+}
+
+int gsh_nodal_velocity(gsh_mesh* m, const gsh_step_params* p) {
+  GSH_API_BODY({
+    GSH_REQUIRE(m && p, GSH_E_ARG, "NULL argument");
+    GSH_REQUIRE(m->ctx, GSH_E_STATE, "host-only mesh (created without a context)");
+    GSH_REQUIRE(p->rmaxvel >= 0.0, GSH_E_ARG, "rmaxvel must be >= 0");
+    m->ctx->set();
+    ensure_elem_w(m);
+    launch_nodal_velocity(m->d, p->rmaxvel, m->ctx->stream);
+    HIPCHK(hipGetLastError());
   })
 }
 

@@ -40,6 +40,7 @@ struct DMesh {
   double* ghat1;            // [ne][2]
   double* vsource;
   double* econ;             // [ne][3][8]: assembly terms of each (element, corner)
+  double* ew;               // [ne][nvrt] element vertical velocity (allocated at first use)
   // sides
   const int4* side;         // [ns]: (n1, n2, e0, e1)
   const int8_t* side_type;  // 0 internal, 1 land, 2 open
@@ -155,6 +156,7 @@ void launch_levels(const DMesh& m, const StepArgs& a, hipStream_t s);
 void launch_backtrack(const DMesh& m, const StepArgs& a, const int32_t* list_side, const int32_t* list_lev,
                       int32_t count, hipStream_t s);
 void launch_rhs(const DMesh& m, const StepArgs& a, hipStream_t s);
+void launch_nodal_velocity(const DMesh& m, double rmaxvel, hipStream_t s);
 void launch_assemble(const DMesh& m, const StepArgs& a, double* val, double* b, hipStream_t s);
 void launch_update_elevation(const DMesh& m, const double* x, hipStream_t s);
 

@@ -435,6 +435,66 @@ void launch_backtrack(const DMesh& m, const StepArgs& a, const int32_t* ls, cons
   hipLaunchKernelGGL(k_backtrack, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, m, a, ls, ll, count);
 }
 
+// ---------------------------------------------------------------- nodal velocities
+// zeroNodalVelocities, computeNodalVerticalVelocities1/2 and computeNodalHorizontalVelocities1/2
+// (meshdata.cu:5765-5921) for triangles, ics=1, indvel <= 0. The reference scatters each
+// element's contributions to its nodes with atomicAdd; here one thread per (node, level) gathers
+// them over the node's ball in ball order, so the sums are deterministic:
+//   w2[k] = (1 / sum area_e) * sum w_e[k] area_e                 over all ball elements, k >= kb
+//   u2[k] = (sum clamp(u_s2 + u_s3 - u_s1)[k]) * (1 / count)     over the wet ball elements, k >= kb
+// (s1 the element side opposite the node, s2 and s3 the following ones; count = wet elements;
+// zero below kb and where no wet element touches the node). The level index is the fastest, so a
+// wave reads consecutive levels of the same side and element rows.
+__device__ __forceinline__ double clamp_vel(double x, double hi) { return fmin(fmax(x, -hi), hi); }
+
+__global__ void __launch_bounds__(256) k_nodal_velocity(DMesh m, double rx, double ry) {
+  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t nv = (uint32_t)m.nvrt;
+  if (t >= (uint64_t)m.n * nv) return;
+  const int i = (t < 0xffffffffull) ? (int)((uint32_t)t / nv) : (int)(t / nv);
+  const int k = (int)(t - (uint64_t)i * nv);
+  const int j0 = m.ball_ptr[i], j1 = m.ball_ptr[i + 1];
+  const bool node_wet = m.node_wet[i] != 0;
+  double asum = 0.0;
+  int count = 0;
+  for (int j = j0; j < j1; ++j) {
+    const int e = m.ball_elem[j];
+    asum += m.area[e];
+    if (m.elem_wet[e] && node_wet) ++count;
+  }
+  double ux = 0.0, uy = 0.0, w = 0.0;
+  if (k >= m.kb[i]) {
+    const double2* su = reinterpret_cast<const double2*>(m.su2);
+    double ws = 0.0, sx = 0.0, sy = 0.0;
+    for (int j = j0; j < j1; ++j) {
+      const int e = m.ball_elem[j];
+      ws += m.ew[(int64_t)e * nv + k] * m.area[e];
+      if (!m.elem_wet[e] || !node_wet) continue;
+      const int l = m.ball_loc[j];
+      const int4 r0 = m.elem[2 * e], r1 = m.elem[2 * e + 1];
+      const int sd[3] = {r0.w, r1.x, r1.y};
+      const double2 u1 = su[(int64_t)sd[l] * nv + k];
+      const double2 u2 = su[(int64_t)sd[(l + 1) % 3] * nv + k];
+      const double2 u3 = su[(int64_t)sd[(l + 2) % 3] * nv + k];
+      sx += clamp_vel(u2.x + u3.x - u1.x, rx);
+      sy += clamp_vel(u2.y + u3.y - u1.y, ry);
+    }
+    w = (1.0 / asum) * ws;
+    if (count > 0) {
+      ux = sx * (1.0 / count);
+      uy = sy * (1.0 / count);
+    }
+  }
+  reinterpret_cast<double4*>(m.uvw)[t] = make_double4(ux, uy, w, 0.0);
+}
+
+void launch_nodal_velocity(const DMesh& m, double rmaxvel, hipStream_t st) {
+  const int64_t total = (int64_t)m.n * m.nvrt;
+  if (total > 0)
+    hipLaunchKernelGGL(k_nodal_velocity, dim3((unsigned)((total + kTpb - 1) / kTpb)), dim3(kTpb), 0, st, m, rmaxvel,
+                       rmaxvel * 1.013);
+}
+
 // ---------------------------------------------------------------- a8-a10, elevation BC
 // computeVerticallyIntegratedVelocity (meshdata.cu:4459-4476) and
 // computeVegetationAndDragParameters for iveg=0, nchi=0 (meshdata.cu:4615-4725).

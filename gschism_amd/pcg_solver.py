@@ -160,6 +160,12 @@ def backtrackAllNodes(mesh: Mesh, params) -> None:
     mesh.backtrack(params)
 
 
+def computeNodalVelocities(mesh: Mesh, params) -> None:
+    """zeroNodalVelocities + computeNodalVerticalVelocities1/2 + computeNodalHorizontalVelocities1/2
+    (schism_step.py:327-331, meshdata.cu:5765-5921): nodal u2/w2 from side u2 and element w."""
+    mesh.nodal_velocity(params)
+
+
 def computeRHSProducers(mesh: Mesh, params) -> None:
     """computeElevationBoundaryConditions + computeVerticallyIntegratedVelocity +
     computeVegetationAndDragParameters + computeElemAverageGhat (schism_step.py:214-232)."""

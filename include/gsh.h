@@ -139,6 +139,8 @@ int gsh_mesh_get_topology(const gsh_mesh* mesh, int what, void* host_out);
 #define GSH_F_WET        18   /* int8   [n]/[ne]/[ns] packed: nodes, then elements, then sides (1 = wet) */
 #define GSH_F_DEPTH      19   /* double [n]          still-water depth h (dp); levels and wet flags follow at
                                  the next gsh_update_levels */
+#define GSH_F_ELEM_W     20   /* double [ne][nvrt]   element vertical velocity w (computeVerticalVelocity's
+                                 output, meshdata.cu:5967-6177); allocated at first use, zero until set */
 int gsh_mesh_set_field(gsh_mesh* mesh, int field, const void* host, int64_t count);
 int gsh_mesh_get_field(const gsh_mesh* mesh, int field, void* host, int64_t count);
 int gsh_mesh_field_ptr(gsh_mesh* mesh, int field, void** dev_ptr, int64_t* count);
@@ -161,6 +163,8 @@ typedef struct gsh_step_params {
   double velmin_btrack; /* ELM minimum velocity [m/s]                  */
   int32_t ihhat;      /* 1 = clip H^ >= 0 in 3D                        */
   int32_t iettype;    /* elevation BC type of the open boundary: 1,2,4 = eth; 0 = none */
+  double rmaxvel;     /* velocity clamp of the nodal reconstruction: |u| <= rmaxvel,
+                         |v| <= 1.013 rmaxvel (constants_template.h:142-144)     */
 } gsh_step_params;
 void gsh_step_params_default(gsh_step_params* p);
 
@@ -181,6 +185,13 @@ int gsh_backtrack_list(gsh_mesh* mesh, const gsh_step_params* p, const int32_t* 
  * computeElemAverageGhat and computeElevationBoundaryConditions (meshdata.cu:4459-4476, 4615-4725,
  * 6183-6424, 4276-4339). */
 int gsh_rhs_producers(gsh_mesh* mesh, const gsh_step_params* p);
+
+/* The step before backtracking: nodal velocities from side and element velocities.
+ * zeroNodalVelocities, computeNodalVerticalVelocities1/2 and computeNodalHorizontalVelocities1/2
+ * (meshdata.cu:5765-5921) for triangles, ics=1, indvel <= 0: GSH_F_NODE_UVW from GSH_F_SIDE_U2 and
+ * GSH_F_ELEM_W, with the wet flags and levels of the last gsh_update_levels. The reference adds the
+ * element contributions with atomics; here each node sums its ball elements in ball order. */
+int gsh_nodal_velocity(gsh_mesh* mesh, const gsh_step_params* p);
 
 /* ---------------------------------------------------------------- solver */
 /* PCGSolverMultiGPU(configCuda, N, NNZ) equivalent bound to a mesh's CSR pattern

@@ -74,7 +74,7 @@ struct or_mesh {
   /* geometry */
   double *area, *gradphi, *ssign, *scx, *scy, *slen, *snx, *sny;
   /* state */
-  double *eta2, *eta1, *uvw, *su2, *btu, *cd, *grav2, *eth, *vsource, *elbc;
+  double *eta2, *eta1, *uvw, *su2, *btu, *cd, *grav2, *eth, *vsource, *elbc, *ew;
   int32_t *foot, *bstatus;
   double *sU, *hhat, *chi, *ghat1, *zc, *nz;
   int8_t *node_dry, *elem_dry, *side_dry;
@@ -99,7 +99,7 @@ void or_mesh_free(or_mesh* m) {
                   m->indices, m->csr_slot, m->diag_slot, m->area, m->gradphi, m->ssign, m->scx, m->scy,
                   m->slen, m->snx, m->sny, m->eta2, m->eta1, m->uvw, m->su2, m->btu, m->cd, m->grav2,
                   m->eth, m->vsource, m->elbc, m->foot, m->bstatus, m->sU, m->hhat, m->chi, m->ghat1,
-                  m->zc, m->nz, m->node_dry, m->elem_dry, m->side_dry, m->elem_kb, m->side_kb};
+                  m->zc, m->nz, m->node_dry, m->elem_dry, m->side_dry, m->elem_kb, m->side_kb, m->ew};
   for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; ++i) free(ptrs[i]);
   free(m);
 }
@@ -412,6 +412,7 @@ or_mesh* or_mesh_new(int32_t n, int32_t ne, int32_t nvrt, const double* xy, cons
   m->eta1 = xcalloc(n, sizeof(double));
   m->uvw = xcalloc(3 * (size_t)n * nv, sizeof(double));
   m->su2 = xcalloc(2 * ns * nv, sizeof(double));
+  m->ew = xcalloc((size_t)m->ne * nv, sizeof(double));
   m->btu = xcalloc(2 * ns * nv, sizeof(double));
   m->foot = xcalloc(ns * nv, sizeof(int32_t));
   m->bstatus = xcalloc(ns * nv, sizeof(int32_t));
@@ -488,6 +489,7 @@ static int field_info(const or_mesh* m, int f, void** ptr, size_t* bytes) {
     case 16: *ptr = m->zc; *bytes = 8 * ns * nv; return 0;
     case 17: *ptr = m->elbc; *bytes = 8 * n; return 0;
     case 19: *ptr = m->depth; *bytes = 8 * n; return 0;
+    case 20: *ptr = m->ew; *bytes = 8 * ne * nv; return 0;
     default: set_err("unknown field %d", f); return -1;
   }
 }
@@ -567,6 +569,63 @@ void or_levels(or_mesh* m, const or_params* p) {
       }
     }
     m->side_dry[s] = !wet;
+  }
+}
+
+/* ------------------------------------------------------------------ nodal velocities */
+/* clampVelocity (meshdata.cu:192-194) with clamp = fmin(fmax(x, lo), hi) (meshdata.h:155-158) */
+static double clamp_v(double x, double hi) { return fmin(fmax(x, -hi), hi); }
+
+/* zeroNodalVelocities, computeNodalVerticalVelocities1/2 and computeNodalHorizontalVelocities1/2
+ * (meshdata.cu:5765-5921) for triangles, ics=1, indvel <= 0. The reference adds each element's
+ * contribution to its nodes with atomicAdd; here node i adds its ball elements in ball order:
+ *   w2[k]  = (1 / sum_e area_e) * sum_e w_e[k] area_e            (k >= kb_i; all ball elements)
+ *   u2[k]  = (sum_e clamp(u_s2 + u_s3 - u_s1)[k]) * (1 / count)  (k >= kb_i; wet elements, s1 the
+ *            side opposite node i, s2 and s3 the next sides of the element; count = their number)
+ * and zero below kb_i or when no wet element touches the node. */
+void or_nodal_velocity(or_mesh* m, const or_params* p, int nthreads) {
+  const int n = m->n, nv = m->nvrt;
+  const double rx = p->rmaxvel, ry = p->rmaxvel * 1.013;
+  (void)nthreads;
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+  for (int i = 0; i < n; ++i) {
+    const int kbn = m->kb[i];
+    double asum = 0.0;
+    int count = 0;
+    for (int j = m->nne_ptr[i]; j < m->nne_ptr[i + 1]; ++j) {
+      const int e = m->nne[j];
+      asum += m->area[e];
+      if (!m->elem_dry[e] && !m->node_dry[i]) ++count;
+    }
+    const double inv_area = 1.0 / asum;
+    for (int k = 0; k < nv; ++k) {
+      double w = 0.0, ux = 0.0, uy = 0.0;
+      if (k >= kbn) {
+        double ws = 0.0, sx = 0.0, sy = 0.0;
+        for (int j = m->nne_ptr[i]; j < m->nne_ptr[i + 1]; ++j) {
+          const int e = m->nne[j];
+          ws += m->ew[(size_t)e * nv + k] * m->area[e];
+          if (m->elem_dry[e] || m->node_dry[i]) continue;
+          const int l = local_index(m, e, i);
+          const size_t s1 = (size_t)m->elside[3 * e + l], s2 = (size_t)m->elside[3 * e + nxq(1, l, 3)],
+                       s3 = (size_t)m->elside[3 * e + nxq(2, l, 3)];
+          const double* u1 = m->su2 + 2 * (s1 * nv + k);
+          const double* u2 = m->su2 + 2 * (s2 * nv + k);
+          const double* u3 = m->su2 + 2 * (s3 * nv + k);
+          sx += clamp_v(u2[0] + u3[0] - u1[0], rx);
+          sy += clamp_v(u2[1] + u3[1] - u1[1], ry);
+        }
+        w = inv_area * ws;
+        if (count > 0) {
+          ux = sx * (1.0 / count);
+          uy = sy * (1.0 / count);
+        }
+      }
+      double* o = m->uvw + 3 * ((size_t)i * nv + k);
+      o[0] = ux;
+      o[1] = uy;
+      o[2] = w;
+    }
   }
 }
 

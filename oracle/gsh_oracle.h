@@ -18,6 +18,7 @@ typedef struct or_mesh or_mesh;
 typedef struct or_params {
   double dt, theta, h0, ramp, dtb_min, dtb_max, velmin_btrack;
   int32_t ihhat, iettype;
+  double rmaxvel;
 } or_params;
 
 const char* or_error(void);
@@ -39,6 +40,8 @@ void or_levels(or_mesh* m, const or_params* p);
 void or_backtrack(or_mesh* m, const or_params* p, const int32_t* list_side, const int32_t* list_lev,
                   int64_t count, int nthreads);
 void or_rhs(or_mesh* m, const or_params* p);
+/* Nodal velocities (field 2) from side velocities (field 3) and element w (field 20), ball order. */
+void or_nodal_velocity(or_mesh* m, const or_params* p, int nthreads);
 /* Assembly + essential BC into val (CSR of the mesh) and b. */
 void or_assemble(or_mesh* m, const or_params* p, double* val, double* b, int nthreads);
 

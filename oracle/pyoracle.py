@@ -16,7 +16,7 @@ LIB_PATH = HERE / "_build" / "liboracle.so"
 class OrParams(C.Structure):
     _fields_ = [("dt", C.c_double), ("theta", C.c_double), ("h0", C.c_double), ("ramp", C.c_double),
                 ("dtb_min", C.c_double), ("dtb_max", C.c_double), ("velmin_btrack", C.c_double),
-                ("ihhat", C.c_int32), ("iettype", C.c_int32)]
+                ("ihhat", C.c_int32), ("iettype", C.c_int32), ("rmaxvel", C.c_double)]
 
 
 _P = C.c_void_p
@@ -35,6 +35,7 @@ _SIG = {
     "or_levels": (None, [_P, C.POINTER(OrParams)]),
     "or_backtrack": (None, [_P, C.POINTER(OrParams), _PI32, _PI32, C.c_int64, C.c_int]),
     "or_rhs": (None, [_P, C.POINTER(OrParams)]),
+    "or_nodal_velocity": (None, [_P, C.POINTER(OrParams), C.c_int]),
     "or_assemble": (None, [_P, C.POINTER(OrParams), _PD, _PD, C.c_int]),
     "or_pcg": (C.c_int32, [C.c_int32, _PI32, _PI32, _PD, _PD, _PD, _PD, _PD, C.c_double, C.c_int32, C.c_int32,
                            C.c_int32, _PD, _PD, _PD, C.c_int]),
@@ -96,7 +97,7 @@ _FIELDS = {
     12: (np.float64, lambda m: (m.ns,)), 13: (np.float64, lambda m: (m.ns,)),
     14: (np.float64, lambda m: (m.ne, 2)), 16: (np.float64, lambda m: (m.ns, m.nvrt)),
     17: (np.float64, lambda m: (m.n,)), 18: (np.int8, lambda m: (m.n + m.ne + m.ns,)),
-    19: (np.float64, lambda m: (m.n,)),
+    19: (np.float64, lambda m: (m.n,)), 20: (np.float64, lambda m: (m.ne, m.nvrt)),
 }
 
 _TOPO = {
@@ -180,6 +181,9 @@ class OracleMesh:
 
     def rhs(self, p) -> None:
         lib().or_rhs(self.h, C.byref(params(p)))
+
+    def nodal_velocity(self, p, nthreads: int = 1) -> None:
+        lib().or_nodal_velocity(self.h, C.byref(params(p)), nthreads)
 
     def assemble(self, p, nthreads: int = 1):
         val = np.zeros(self.nnz, dtype=np.float64)

@@ -77,6 +77,90 @@ def test_dry_elements_and_kb():
     assert np.array_equal(wet[n:n + ne] == 0, elem_dry_ref)
 
 
+# ---------------------------------------------------------------- nodal velocities
+def _nodal_brute(m, su2, ew, rmaxvel, wet_elem, node_dry, kb):
+    """Nodal velocities element by element, the opposite side found from the side's end nodes."""
+    en, es, sn = m.topology(L.T_ELNODE), m.topology(L.T_ELSIDE), m.topology(L.T_ISIDENODE)
+    area = m.geometry(0)
+    n, nv = m.n, m.nvrt
+    usum = np.zeros((n, nv, 2))
+    cnt = np.zeros(n)
+    wsum = np.zeros((n, nv))
+    asum = np.zeros(n)
+    lim = np.array([rmaxvel, rmaxvel * 1.013])
+    for e in range(m.ne):
+        for i in en[e]:
+            asum[i] += area[e]
+            wsum[i] += ew[e] * area[e]
+            if not wet_elem[e] or node_dry[i]:
+                continue
+            opp = [s for s in es[e] if i not in sn[s]]
+            adj = [s for s in es[e] if i in sn[s]]
+            assert len(opp) == 1 and len(adj) == 2
+            usum[i] += np.clip(su2[adj[0]] + su2[adj[1]] - su2[opp[0]], -lim, lim)
+            cnt[i] += 1
+    out = np.zeros((n, nv, 3))
+    for i in range(n):
+        k0 = kb[i]
+        out[i, k0:, 2] = wsum[i, k0:] / asum[i]
+        if cnt[i] > 0:
+            out[i, k0:, :2] = usum[i, k0:] / cnt[i]
+    return out
+
+
+def test_nodal_velocity_reproduces_linear_fields():
+    """A triangle's side-centre values extrapolated to a node as u_s2 + u_s3 - u_s1 are exact for
+    linear fields (m2 + m3 - m1 is the node), and equal element values of w average to w."""
+    m, xy, en, _ = _mesh(nvrt=5)
+    n, ne, nv = xy.shape[0], en.shape[0], 5
+    p = _params(rmaxvel=100.0)
+    m.set(L.F_ETA2, np.zeros(n))
+    m.levels(p)
+    sc = m.geometry(2)
+    lev = np.arange(nv)
+    u = 0.3 + 1e-4 * sc[:, :1] - 2e-4 * sc[:, 1:] + 0.01 * lev
+    v = -0.2 + 5e-5 * sc[:, :1] + 1e-4 * sc[:, 1:] - 0.02 * lev
+    m.set(L.F_SIDE_U2, np.stack([u, v], axis=-1))
+    m.set(L.F_ELEM_W, np.tile(1e-3 * (lev + 1), (ne, 1)))
+    m.nodal_velocity(p)
+    got = m.get(L.F_NODE_UVW)
+    xyo = m.topology(L.T_XY)
+    eu = 0.3 + 1e-4 * xyo[:, :1] - 2e-4 * xyo[:, 1:] + 0.01 * lev
+    ev = -0.2 + 5e-5 * xyo[:, :1] + 1e-4 * xyo[:, 1:] - 0.02 * lev
+    assert np.allclose(got[..., 0], eu, rtol=0, atol=1e-12)
+    assert np.allclose(got[..., 1], ev, rtol=0, atol=1e-12)
+    assert np.allclose(got[..., 2], 1e-3 * (lev + 1), rtol=1e-14, atol=0)
+
+
+def test_nodal_velocity_matches_element_by_element_sums():
+    """Random side velocities and element w on a mesh with dry elements and deep bottom levels:
+    the oracle's ball-order sums against element-by-element sums (clamping active)."""
+    xy, en, ob = cases.channel(20, 14, 300.0)
+    depth = np.where(np.hypot(xy[:, 0] - 3000, xy[:, 1] - 2000) < 900, -2.0, 8.0)
+    n, nv = xy.shape[0], 6
+    kb = (np.arange(n) % 3).astype(np.int32)
+    m = O.OracleMesh(xy, en, depth, ob, cases.sigma_levels(nv), kb=kb)
+    p = _params(rmaxvel=1.5)
+    m.set(L.F_ETA2, np.zeros(n))
+    m.levels(p)
+    rng = np.random.default_rng(5)
+    su2 = rng.normal(0.0, 1.0, (m.ns, nv, 2))
+    ew = rng.normal(0.0, 1e-3, (m.ne, nv))
+    m.set(L.F_SIDE_U2, su2)
+    m.set(L.F_ELEM_W, ew)
+    m.nodal_velocity(p, nthreads=2)
+    got = m.get(L.F_NODE_UVW)
+    wet = m.get(L.F_WET)
+    ref = _nodal_brute(m, su2, ew, 1.5, wet[n:n + m.ne] != 0, wet[:n] == 0, m.topology(L.T_KB))
+    assert np.allclose(got, ref, rtol=1e-12, atol=1e-15)
+    dry = wet[:n] == 0
+    assert dry.any() and np.all(got[dry, :, :2] == 0.0)
+    for i in range(n):
+        assert np.all(got[i, :kb[i]] == 0.0)
+    assert np.abs(got[..., 0]).max() <= 1.5 and np.abs(got[..., 1]).max() <= 1.5 * 1.013
+    assert np.abs(got[..., 0]).max() == 1.5  # some node averages only clamped values
+
+
 # ---------------------------------------------------------------- interpolation
 def test_interpolation_reproduces_linear_fields():
     m, xy, en, _ = _mesh(depth=10.0, nvrt=6)

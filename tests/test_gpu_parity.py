@@ -94,6 +94,30 @@ def test_chain_island_dry_and_kb(ctx):
     _full_chain(ctx, mesh, om, p)
 
 
+def test_nodal_velocity_then_backtrack(ctx):
+    """Nodal velocities from random side velocities and element w (dry island, node-varying
+    bottom levels, clamping active) bit for bit, then backtracking through them."""
+    spec = cases.CaseSpec("island-48x32", 48, 32, 400.0, nvrt=12, depth=10.0, substeps=6)
+    mesh, om, p = _setup(ctx, spec, kb_fn=_lsc_kb, depth_fn=_island_depth)
+    rng = np.random.default_rng(21)
+    su2 = rng.normal(0.0, 0.5, (mesh.ns, mesh.nvrt, 2))
+    ew = rng.normal(0.0, 1e-3, (mesh.ne, mesh.nvrt))
+    p.rmaxvel = 0.9
+    for m in (mesh, om):
+        m.set(L.F_SIDE_U2, su2)
+        m.set(L.F_ELEM_W, ew)
+    print(f"[gpu] nodal velocity n={mesh.n} nvrt={mesh.nvrt}", flush=True)
+    mesh.nodal_velocity(p)
+    om.nodal_velocity(p, nthreads=4)
+    uvw = mesh.get(L.F_NODE_UVW)
+    assert_bitwise(uvw, om.get(L.F_NODE_UVW), "nodal velocities")
+    assert np.abs(uvw[..., 0]).max() == 0.9
+    mesh.backtrack(p)
+    om.backtrack(p, nthreads=8)
+    assert_bitwise(mesh.get(L.F_FOOT), om.get(L.F_FOOT), "foot elements")
+    assert_bitwise(mesh.get(L.F_BTU), om.get(L.F_BTU), "backtracked velocity")
+
+
 @pytest.mark.parametrize("rpt", [1, 2, 8])
 def test_rows_per_thread(ctx, rpt):
     spec = cases.CaseSpec("channel-64x40", 64, 40, 500.0, nvrt=6, depth=None, substeps=4)
@@ -329,7 +353,17 @@ def test_full_size_1m_channel(ctx):
     om.set(L.F_NODE_UVW, uvw)
     om.set(L.F_SIDE_U2, su2)
     om.levels(p)
-    del uvw
+    print("[gpu] 1M nodal velocity", flush=True)
+    # element w from the nodal field's levels (any smooth values do); the reconstruction from the
+    # side velocities then replaces the synthetic nodal field, on both sides
+    en = mesh.topology(L.T_ELNODE)
+    ew = uvw[en, :, 0].mean(axis=1) * 1e-3
+    mesh.set(L.F_ELEM_W, ew)
+    om.set(L.F_ELEM_W, ew)
+    del uvw, en, ew
+    mesh.nodal_velocity(p)
+    om.nodal_velocity(p, nthreads=16)
+    assert_bitwise(mesh.get(L.F_NODE_UVW), om.get(L.F_NODE_UVW), "1M nodal velocities")
     print("[gpu] 1M backtrack", flush=True)
     mesh.backtrack(p)
     rng = np.random.default_rng(11)
