@@ -455,30 +455,33 @@ __global__ void __launch_bounds__(256) k_nodal_velocity(DMesh m, double rx, doub
   const int k = (int)(t - (uint64_t)i * nv);
   const int j0 = m.ball_ptr[i], j1 = m.ball_ptr[i + 1];
   const bool node_wet = m.node_wet[i] != 0;
-  double asum = 0.0;
+  const bool above = k >= m.kb[i];
+  const double2* su = reinterpret_cast<const double2*>(m.su2);
+  // one pass over the ball; each sum keeps its own ball-order sequence of additions
+  double asum = 0.0, ws = 0.0, sx = 0.0, sy = 0.0;
   int count = 0;
   for (int j = j0; j < j1; ++j) {
     const int e = m.ball_elem[j];
-    asum += m.area[e];
-    if (m.elem_wet[e] && node_wet) ++count;
+    const double a = m.area[e];
+    asum += a;
+    if (above) ws += m.ew[(int64_t)e * nv + k] * a;
+    if (!m.elem_wet[e] || !node_wet) continue;
+    ++count;
+    if (!above) continue;
+    const int l = m.ball_loc[j];
+    const int4 r0 = m.elem[2 * e], r1 = m.elem[2 * e + 1];
+    // sides (r0.w, r1.x, r1.y) are opposite corners 0, 1, 2
+    const int s1 = (l == 0) ? r0.w : ((l == 1) ? r1.x : r1.y);
+    const int s2 = (l == 0) ? r1.x : ((l == 1) ? r1.y : r0.w);
+    const int s3 = (l == 0) ? r1.y : ((l == 1) ? r0.w : r1.x);
+    const double2 u1 = su[(int64_t)s1 * nv + k];
+    const double2 u2 = su[(int64_t)s2 * nv + k];
+    const double2 u3 = su[(int64_t)s3 * nv + k];
+    sx += clamp_vel(u2.x + u3.x - u1.x, rx);
+    sy += clamp_vel(u2.y + u3.y - u1.y, ry);
   }
   double ux = 0.0, uy = 0.0, w = 0.0;
-  if (k >= m.kb[i]) {
-    const double2* su = reinterpret_cast<const double2*>(m.su2);
-    double ws = 0.0, sx = 0.0, sy = 0.0;
-    for (int j = j0; j < j1; ++j) {
-      const int e = m.ball_elem[j];
-      ws += m.ew[(int64_t)e * nv + k] * m.area[e];
-      if (!m.elem_wet[e] || !node_wet) continue;
-      const int l = m.ball_loc[j];
-      const int4 r0 = m.elem[2 * e], r1 = m.elem[2 * e + 1];
-      const int sd[3] = {r0.w, r1.x, r1.y};
-      const double2 u1 = su[(int64_t)sd[l] * nv + k];
-      const double2 u2 = su[(int64_t)sd[(l + 1) % 3] * nv + k];
-      const double2 u3 = su[(int64_t)sd[(l + 2) % 3] * nv + k];
-      sx += clamp_vel(u2.x + u3.x - u1.x, rx);
-      sy += clamp_vel(u2.y + u3.y - u1.y, ry);
-    }
+  if (above) {
     w = (1.0 / asum) * ws;
     if (count > 0) {
       ux = sx * (1.0 / count);
