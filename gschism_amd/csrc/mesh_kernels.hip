@@ -437,7 +437,7 @@ void launch_backtrack(const DMesh& m, const StepArgs& a, const int32_t* ls, cons
 
 // ---------------------------------------------------------------- nodal velocities
 // zeroNodalVelocities, computeNodalVerticalVelocities1/2 and computeNodalHorizontalVelocities1/2
-// (meshdata.cu:5765-5921) for triangles, ics=1, indvel <= 0. The reference scatters each
+// (meshdata.cu:5765-5929) for triangles, ics=1, indvel <= 0. The reference scatters each
 // element's contributions to its nodes with atomicAdd; here one thread per (node, level) gathers
 // them over the node's ball in ball order, so the sums are deterministic:
 //   w2[k] = (1 / sum area_e) * sum w_e[k] area_e                 over all ball elements, k >= kb

@@ -162,7 +162,7 @@ def backtrackAllNodes(mesh: Mesh, params) -> None:
 
 def computeNodalVelocities(mesh: Mesh, params) -> None:
     """zeroNodalVelocities + computeNodalVerticalVelocities1/2 + computeNodalHorizontalVelocities1/2
-    (schism_step.py:327-331, meshdata.cu:5765-5921): nodal u2/w2 from side u2 and element w."""
+    (schism_step.py:327-331, meshdata.cu:5765-5929): nodal u2/w2 from side u2 and element w."""
     mesh.nodal_velocity(params)
 
 

@@ -188,7 +188,7 @@ int gsh_rhs_producers(gsh_mesh* mesh, const gsh_step_params* p);
 
 /* The step before backtracking: nodal velocities from side and element velocities.
  * zeroNodalVelocities, computeNodalVerticalVelocities1/2 and computeNodalHorizontalVelocities1/2
- * (meshdata.cu:5765-5921) for triangles, ics=1, indvel <= 0: GSH_F_NODE_UVW from GSH_F_SIDE_U2 and
+ * (meshdata.cu:5765-5929) for triangles, ics=1, indvel <= 0: GSH_F_NODE_UVW from GSH_F_SIDE_U2 and
  * GSH_F_ELEM_W, with the wet flags and levels of the last gsh_update_levels. The reference adds the
  * element contributions with atomics; here each node sums its ball elements in ball order. */
 int gsh_nodal_velocity(gsh_mesh* mesh, const gsh_step_params* p);

@@ -577,7 +577,7 @@ void or_levels(or_mesh* m, const or_params* p) {
 static double clamp_v(double x, double hi) { return fmin(fmax(x, -hi), hi); }
 
 /* zeroNodalVelocities, computeNodalVerticalVelocities1/2 and computeNodalHorizontalVelocities1/2
- * (meshdata.cu:5765-5921) for triangles, ics=1, indvel <= 0. The reference adds each element's
+ * (meshdata.cu:5765-5929) for triangles, ics=1, indvel <= 0. The reference adds each element's
  * contribution to its nodes with atomicAdd; here node i adds its ball elements in ball order:
  *   w2[k]  = (1 / sum_e area_e) * sum_e w_e[k] area_e            (k >= kb_i; all ball elements)
  *   u2[k]  = (sum_e clamp(u_s2 + u_s3 - u_s1)[k]) * (1 / count)  (k >= kb_i; wet elements, s1 the
