@@ -22,6 +22,7 @@ GSH_E_ADMISSION = 5
 GSH_E_TOPOLOGY = 6
 GSH_E_STATE = 7
 GSH_E_NOTFOUND = 8
+GSH_E_IO = 9
 GSH_W_MAXITER = 64
 PREC_FP64 = 0
 PREC_MIXED = 1
@@ -105,6 +106,11 @@ SIGNATURES = {
     "gsh_backtrack_list": (_I, [_P, C.POINTER(StepParams), _PI32, _PI32, _I32]),
     "gsh_rhs_producers": (_I, [_P, C.POINTER(StepParams)]),
     "gsh_nodal_velocity": (_I, [_P, C.POINTER(StepParams)]),
+    "gsh_grid_read": (_I, [C.c_char_p, C.c_char_p, _PP]),
+    "gsh_grid_info": (_I, [_P, _PI32, _PI32, _PI32, _PI32, _PI32]),
+    "gsh_grid_get": (_I, [_P, _PD, _PD, _PI32, C.POINTER(C.c_int8), _PI32, _PD]),
+    "gsh_grid_segment": (_I, [_P, _I, _I32, _PI32, _I32, _PI32, _PI32]),
+    "gsh_grid_destroy": (_I, [_P]),
     "gsh_solver_create": (_I, [_P, _I, _I, _PP]),
     "gsh_solver_create_csr": (_I, [_P, _I32, _I64, _PI32, _PI32, _I, _I, _PP]),
     "gsh_solver_destroy": (_I, [_P]),

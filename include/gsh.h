@@ -39,6 +39,8 @@ extern "C" {
 #define GSH_E_TOPOLOGY    6   /* broken ball / hanging node / orientation (read_and_process_grid.cpp:47-58, 154-159, 334-341) */
 #define GSH_E_STATE       7   /* call out of order (e.g. solve before a matrix exists) */
 #define GSH_E_NOTFOUND    8   /* serving: unknown ticket */
+#define GSH_E_IO          9   /* a grid file cannot be read or breaks its format (reference: runtime_error
+                                 in readHGrid / readVGrid, read_and_process_grid.cpp:864, 963-966, 1050-1053) */
 #define GSH_W_MAXITER     64  /* flag OR-ed into *status_out of a solve that stopped at maxiter
                                  (the reference returns maxiter silently, pcg_solver.py:1068) */
 
@@ -192,6 +194,25 @@ int gsh_rhs_producers(gsh_mesh* mesh, const gsh_step_params* p);
  * GSH_F_ELEM_W, with the wet flags and levels of the last gsh_update_levels. The reference adds the
  * element contributions with atomics; here each node sums its ball elements in ball order. */
 int gsh_nodal_velocity(gsh_mesh* mesh, const gsh_step_params* p);
+
+/* ---------------------------------------------------------------- grid files (host side)
+ * hgrid.gr3 as readHGrid reads it (read_and_process_grid.cpp:855-1070): node coordinates and
+ * depth, triangles (1-based node ids in the file, 0-based here), open and land boundary segments,
+ * with its checks (neta and nvel totals, looped open boundaries); quadrilaterals are rejected.
+ * vgrid.in with ivcor=1 as readVGrid reads it (:1337-1385): kbp per node and sigma rows, accepted
+ * when the nodes agree on sigma at every level they use (the library holds one sigma per level
+ * and a bottom level per node); ivcor=2 is rejected. vgrid_path may be NULL. */
+typedef struct gsh_grid gsh_grid;
+int gsh_grid_read(const char* hgrid_path, const char* vgrid_path, gsh_grid** out);
+int gsh_grid_info(const gsh_grid* g, int32_t* n, int32_t* ne, int32_t* nvrt, int32_t* n_open_seg,
+                  int32_t* n_land_seg);
+/* Any output may be NULL; kb [n] and sigma [nvrt] need a vgrid. open_bnd[i] = 1 on open segments. */
+int gsh_grid_get(const gsh_grid* g, double* xy, double* depth, int32_t* elnode, int8_t* open_bnd, int32_t* kb,
+                 double* sigma);
+/* Nodes (0-based) of open (land = 0) or land (land = 1) segment seg; *island = the land flag. */
+int gsh_grid_segment(const gsh_grid* g, int land, int32_t seg, int32_t* nodes_out, int32_t cap, int32_t* count,
+                     int32_t* island);
+int gsh_grid_destroy(gsh_grid* g);
 
 /* ---------------------------------------------------------------- solver */
 /* PCGSolverMultiGPU(configCuda, N, NNZ) equivalent bound to a mesh's CSR pattern
