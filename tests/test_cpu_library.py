@@ -34,6 +34,15 @@ def test_every_declared_symbol_is_exported():
 def test_lib_loads_with_signatures():
     lib = L.lib()
     assert lib.gsh_version() == 100
+    # the message is per thread and kept until the next entry point runs; a successful call clears it
+    assert lib.gsh_grid_get(None, None, None, None, None, None, None) == L.GSH_E_ARG
+    assert b"NULL" in lib.gsh_last_error()
+    ncol = C.c_int32()
+    ip = np.array([0, 1], dtype=np.int32)
+    ix = np.array([0], dtype=np.int32)
+    col = np.empty(1, dtype=np.int32)
+    assert lib.gsh_colour_csr(1, ip.ctypes.data_as(C.POINTER(C.c_int32)), ix.ctypes.data_as(C.POINTER(C.c_int32)),
+                              col.ctypes.data_as(C.POINTER(C.c_int32)), C.byref(ncol)) == L.GSH_OK
     assert lib.gsh_last_error() == b""
 
 
