@@ -138,9 +138,9 @@ def batched_leg(ctx, L, cases, core, members, nx, ny, rtol, mixed_rtol, check_ev
 
 def mixed_4m_leg(ctx, L, cases, core, steps, rtol, outer_rtol, check_every):
     """Configuration 3: the 2048 x 2048 log-normal channel (4.2M nodes). FP64 PCG (multicolour
-    SSOR and the reference's Jacobi) to rtol 1e-12 against mixed precision (FP32 inner Jacobi
-    PCG, FP64 dot products, true residual and refinement) to the outer tolerance; relative L2
-    error against the FP64 solution and iteration counts."""
+    SSOR and the reference's Jacobi) to rtol 1e-12 against mixed precision (FP32 inner SSOR or
+    Jacobi PCG, FP64 dot products, true residual and refinement) to the outer tolerance; relative
+    L2 error against the FP64 SSOR solution and iteration counts."""
     spec = cases.spec_mixed_4m()
     data = cases.make_case(spec)
     mesh, p, _ = cases.load_mesh(ctx, data)
@@ -150,6 +150,7 @@ def mixed_4m_leg(ctx, L, cases, core, steps, rtol, outer_rtol, check_every):
     ref = None
     for name, prec, pc, tol in (("fp64_ssor", L.PREC_FP64, L.PRECOND_SSOR, rtol),
                                 ("fp64_jacobi", L.PREC_FP64, L.PRECOND_JACOBI, rtol),
+                                ("mixed_ssor", L.PREC_MIXED, L.PRECOND_SSOR, outer_rtol),
                                 ("mixed_jacobi", L.PREC_MIXED, L.PRECOND_JACOBI, outer_rtol)):
         s = core.Solver(mesh, prec, precond=pc)
         s.set_tuning(2 if pc == L.PRECOND_SSOR else 4)
@@ -458,7 +459,7 @@ def main() -> int:
                "rel_l2_ssor_vs_jacobi": float(np.linalg.norm(x_ssor - x_jac) / np.linalg.norm(x_jac))}
         log(f"[bench] Jacobi: {jac}")
 
-    # ---- mixed precision (FP32 inner Jacobi PCG, FP64 residual and refinement) on the same system
+    # ---- mixed precision (FP32 inner Jacobi / SSOR PCG, FP64 residual and refinement) on the same system
     mixed = None
     if not args.no_mixed and x_jac is not None:
         ms_solver = core.Solver(mesh, L.PREC_MIXED)
@@ -477,6 +478,26 @@ def main() -> int:
         mixed = {"solves_per_s": round(len(tm) / sum(tm), 3), "inner_iters": int(st_m.iters),
                  "outer": int(st_m.outer),
                  "rel_l2_vs_fp64": float(np.linalg.norm(xmix - x_jac) / np.linalg.norm(x_jac))}
+        ms_solver.close()
+        # the same with FP32 multicolour SSOR inner solves
+        ms_solver = core.Solver(mesh, L.PREC_MIXED, precond=L.PRECOND_SSOR)
+        ms_solver.set_tuning(args.ssor_rows_per_thread, 1e-5)
+        ms_solver.assemble(p)
+        ms_solver.solve(None, rtol, maxiter, check_every, want_x=False)  # warm
+        tm = []
+        for i in range(max(2, args.steps // 2)):
+            a = time.perf_counter()
+            ms_solver.reset_history()
+            xm, itm, stm = ms_solver.solve(None, rtol, maxiter, check_every, want_x=(i == 0))
+            tm.append(time.perf_counter() - a)
+            if i == 0:
+                xmix = xm
+                st_m = ms_solver.stats()
+        mixed["ssor"] = {"solves_per_s": round(len(tm) / sum(tm), 3), "inner_iters": int(st_m.iters),
+                         "outer": int(st_m.outer),
+                         "rel_l2_vs_fp64_ssor": float(np.linalg.norm(xmix - x_ssor) / np.linalg.norm(x_ssor)),
+                         "vs_fp64_ssor": round(len(tm) / sum(tm) / solves_per_s, 3)}
+        ms_solver.close()
         log(f"[bench] mixed: {mixed}")
 
     # ---- CPU baseline: the oracle's FP64 SSOR PCG on the same system, all host cores

@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
@@ -16,6 +17,7 @@
 #include <map>
 #include <memory>
 #include <thread>
+#include <type_traits>
 #include <vector>
 
 #include "abi_util.h"
@@ -133,6 +135,10 @@ using namespace gsh;
 struct gsh_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
+  // meshes, solvers and servers on this context; gsh_ctx_destroy frees the context once the
+  // last of them is destroyed, so objects destroyed after the context never see freed state
+  std::atomic<int> refs{0};
+  std::atomic<bool> closed{false};
   PcgState* pinned = nullptr;   // host-visible readback slots
   int pinned_cap = 0;
   hipEvent_t ev[2] = {nullptr, nullptr};
@@ -218,8 +224,10 @@ static bool launch_window_graph(WindowGraph& g, hipStream_t st, const void* key,
 struct SsorState {
   DevPool pool;
   SsorLayout L;
-  SsorSys hs{};
+  SsorSys hs{};                 // FP64 solver
   SsorSys* ds = nullptr;
+  SsorSysT<float> hs32{};       // mixed precision: the FP32 inner solver
+  SsorSysT<float>* ds32 = nullptr;
   int rpt = 1, ntiles = 0;
   std::vector<int> tile0, ntc;  // tiles of each colour
 };
@@ -364,7 +372,9 @@ static void solver_sync_desc(gsh_solver* s) {
 }
 
 // Colour-major layout, split matrix and work vectors of the SSOR solver for the current
-// rows-per-thread (the colour blocks are padded to whole tiles of 256 * rpt rows).
+// rows-per-thread (the colour blocks are padded to whole tiles of 256 * rpt rows). An FP64
+// solver gets FP64 values and vectors; a mixed-precision solver gets FP32 ones for its inner
+// solves (the FP64 residual and refinement use the solver's CSR in the original order).
 static void ssor_build(gsh_solver* s) {
   auto S = std::make_unique<SsorState>();
   S->rpt = s->rpt;
@@ -372,32 +382,40 @@ static void ssor_build(gsh_solver* s) {
   build_ssor_layout(s->n, s->h_indptr.data(), s->h_indices.data(), T, S->L);
   const SsorLayout& L = S->L;
   auto& P = S->pool;
-  SsorSys& h = S->hs;
-  h.n = s->n;
-  h.np = L.np;
-  h.ntiles = L.np / T;
-  h.ncol = L.ncol;
-  h.omega = s->omega;
   std::vector<SsorColour> cols(L.ncol);
   for (int c = 0; c < L.ncol; ++c)
     cols[c] = {L.cstart[c], L.cstart[c + 1] - L.cstart[c], L.wl[c], L.wu[c], (int32_t)L.offl[c], (int32_t)L.offu[c], 0, 0};
-  h.colours = P.upload(cols);
-  h.tile_colour = P.upload(L.tile_colour);
-  h.lcol = P.upload(L.lcol);
-  h.lsrc = P.upload(L.lsrc);
-  h.ucol = P.upload(L.ucol);
-  h.usrc = P.upload(L.usrc);
-  h.dsrc = P.upload(L.dsrc);
-  h.perm = P.upload(L.perm);
-  h.lval = P.alloc<double>((int64_t)L.lcol.size());
-  h.uval = P.alloc<double>((int64_t)L.ucol.size());
-  for (double** v : {&h.d, &h.dt, &h.kd, &h.bp, &h.y, &h.r, &h.p, &h.q, &h.t, &h.u, &h.s, &h.w})
-    *v = P.alloc<double>(L.np);
-  for (double** v : {&h.part_rr, &h.part_bb}) *v = P.alloc<double>(h.ntiles);
-  for (double** v : {&h.part_rz, &h.part_pq}) *v = P.alloc<double>(2 * (int64_t)h.ntiles);  // two iteration slots
-  h.st = P.alloc<PcgState>(1);
-  S->ds = P.alloc<SsorSys>(1);
-  S->ntiles = h.ntiles;
+  auto fill = [&](auto& h) {
+    using VT = std::remove_pointer_t<decltype(h.lval)>;
+    h.n = s->n;
+    h.np = L.np;
+    h.ntiles = L.np / T;
+    h.ncol = L.ncol;
+    h.omega = s->omega;
+    h.colours = P.upload(cols);
+    h.tile_colour = P.upload(L.tile_colour);
+    h.lcol = P.upload(L.lcol);
+    h.lsrc = P.upload(L.lsrc);
+    h.ucol = P.upload(L.ucol);
+    h.usrc = P.upload(L.usrc);
+    h.dsrc = P.upload(L.dsrc);
+    h.perm = P.upload(L.perm);
+    h.lval = P.alloc<VT>((int64_t)L.lcol.size());
+    h.uval = P.alloc<VT>((int64_t)L.ucol.size());
+    for (VT** v : {&h.d, &h.dt, &h.kd, &h.bp, &h.y, &h.r, &h.p, &h.q, &h.t, &h.u, &h.s, &h.w}) *v = P.alloc<VT>(L.np);
+    for (double** v : {&h.part_rr, &h.part_bb}) *v = P.alloc<double>(h.ntiles);
+    for (double** v : {&h.part_rz, &h.part_pq}) *v = P.alloc<double>(2 * (int64_t)h.ntiles);  // two iteration slots
+    h.st = P.alloc<PcgState>(1);
+  };
+  if (s->precision == GSH_PREC_MIXED) {
+    fill(S->hs32);
+    S->ds32 = P.alloc<SsorSysT<float>>(1);
+    S->ntiles = S->hs32.ntiles;
+  } else {
+    fill(S->hs);
+    S->ds = P.alloc<SsorSys>(1);
+    S->ntiles = S->hs.ntiles;
+  }
   for (int c = 0; c < L.ncol; ++c) {
     S->tile0.push_back(L.cstart[c] / T);
     S->ntc.push_back((L.cstart[c + 1] - L.cstart[c]) / T);
@@ -405,6 +423,30 @@ static void ssor_build(gsh_solver* s) {
   s->ssor = std::move(S);
 }
 
+template <class VT>
+static SsorSysT<VT>& ssor_host(SsorState& S);
+template <>
+SsorSysT<double>& ssor_host<double>(SsorState& S) {
+  return S.hs;
+}
+template <>
+SsorSysT<float>& ssor_host<float>(SsorState& S) {
+  return S.hs32;
+}
+template <class VT>
+static SsorSysT<VT>* ssor_dev(SsorState& S);
+template <>
+SsorSysT<double>* ssor_dev<double>(SsorState& S) {
+  return S.ds;
+}
+template <>
+SsorSysT<float>* ssor_dev<float>(SsorState& S) {
+  return S.ds32;
+}
+
+// FP64: the solver's values, b, x and history. FP32 (inner solve of mixed precision): the FP32
+// copy of the values, the given FP64 right-hand side (the outer residual), the FP32 correction
+// as solution and a zero initial guess.
 static void ssor_sync_desc(gsh_solver* s) {
   SsorSys& h = s->ssor->hs;
   h.aval = s->val;
@@ -416,6 +458,18 @@ static void ssor_sync_desc(gsh_solver* s) {
   h.omega = s->omega;
   h.unit_omega = s->omega == 1.0;
   HIPCHK(hipMemcpyAsync(s->ssor->ds, &h, sizeof(h), hipMemcpyHostToDevice, s->ctx->stream));
+  HIPCHK(hipStreamSynchronize(s->ctx->stream));
+}
+
+static void ssor_sync_desc32(gsh_solver* s, const double* rhs) {
+  SsorSysT<float>& h = s->ssor->hs32;
+  h.aval = s->val32;
+  h.b = rhs;
+  h.xout = s->hs32.x;
+  h.h0 = h.h1 = h.h2 = nullptr;
+  h.omega = s->omega;
+  h.unit_omega = s->omega == 1.0;
+  HIPCHK(hipMemcpyAsync(s->ssor->ds32, &h, sizeof(h), hipMemcpyHostToDevice, s->ctx->stream));
   HIPCHK(hipStreamSynchronize(s->ctx->stream));
 }
 
@@ -533,16 +587,17 @@ struct SsorTimes {
 };
 
 // Multicolour SSOR PCG of one system: set-up, windows of check_every iterations (2 * ncolours
-// launches each) closed by the true-residual test, then x = (D~ + U)^-1 y.
+// launches each) closed by the true-residual test, then x = (D~ + U)^-1 y. The descriptor of
+// the VT system must be synchronised first (ssor_sync_desc / ssor_sync_desc32).
+template <class VT>
 static PcgState run_ssor(gsh_solver* s, double rtol, int maxiter, int check_every, double timeout_s,
                          SsorTimes* prof = nullptr, bool use_graph = false) {
   gsh_ctx* ctx = s->ctx;
   hipStream_t st = ctx->stream;
   if (check_every < 1) check_every = 1;
   if (timeout_s <= 0.0) timeout_s = 60.0;
-  ssor_sync_desc(s);
   SsorState& S = *s->ssor;
-  const SsorSys* ds = S.ds;
+  const SsorSysT<VT>* ds = ssor_dev<VT>(S);
   const int rpt = S.rpt, C = S.L.ncol, NT = S.ntiles;
   PcgState* slots = ctx->pinned_slots(1);
   launch_ssor_reset(ds, rtol, maxiter, check_every, st);
@@ -612,7 +667,7 @@ static PcgState run_ssor(gsh_solver* s, double rtol, int maxiter, int check_ever
       inflight.pop_front();
       if (check_slot(old)) done = true;
     }
-    HIPCHK(hipMemcpyAsync(slots + slot, S.hs.st, sizeof(PcgState), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(slots + slot, ssor_host<VT>(S).st, sizeof(PcgState), hipMemcpyDeviceToHost, st));
     HIPCHK(hipEventRecord(ctx->ev[slot], st));
     inflight.push_back(slot);
     ++window;
@@ -622,7 +677,7 @@ static PcgState run_ssor(gsh_solver* s, double rtol, int maxiter, int check_ever
   }
   for (int c = C - 1; c >= 0; --c) launch_ssor_xsolve(ds, S.tile0[c], S.ntc[c], c, rpt, st);
   launch_ssor_scatter(ds, NT, rpt, st);
-  HIPCHK(hipMemcpyAsync(slots, S.hs.st, sizeof(PcgState), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(slots, ssor_host<VT>(S).st, sizeof(PcgState), hipMemcpyDeviceToHost, st));
   HIPCHK(hipEventRecord(ctx->ev[0], st));
   wait_event(ctx->ev[0], timeout_s, "SSOR PCG finish");
   HIPCHK(hipGetLastError());
@@ -724,8 +779,15 @@ static void run_mixed(gsh_ctx* ctx, gsh_solver* const* S, int count, double rtol
     HIPCHK(hipMemcpyAsync(dtl, stl.data(), sizeof(TileRef) * stl.size(), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(drt, rts.data(), sizeof(double) * ns, hipMemcpyHostToDevice, st));
     PcgOutcome o;
-    run_pcg<float>(ctx, d32s, ns, dtl, (int)stl.size(), rpt, rtol, maxiter, check_every, timeout_s, o, nullptr, -1,
-                   nullptr, drt);
+    if (S[0]->precond == GSH_PRECOND_SSOR) {
+      // one system (batches take Jacobi solvers): FP32 SSOR PCG of A d = r, b = the FP64 residual
+      gsh_solver* s = S[sub[0]];
+      ssor_sync_desc32(s, s->r64);
+      o.st.assign(1, run_ssor<float>(s, rts[0], maxiter, check_every, timeout_s));
+    } else {
+      run_pcg<float>(ctx, d32s, ns, dtl, (int)stl.size(), rpt, rtol, maxiter, check_every, timeout_s, o, nullptr, -1,
+                     nullptr, drt);
+    }
     launch_refine_batch(d64s, d32s, dtl, (int)stl.size(), rpt, st);
     for (int j = 0; j < ns; ++j) {
       const int it = iters_of(o.st[j], maxiter);
@@ -779,8 +841,7 @@ int gsh_ctx_create(int device, gsh_ctx** out) {
   })
 }
 
-int gsh_ctx_destroy(gsh_ctx* ctx) {
-  if (!ctx) return GSH_OK;
+static void ctx_free(gsh_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   if (ctx->pinned) (void)hipHostFree(ctx->pinned);
@@ -788,6 +849,25 @@ int gsh_ctx_destroy(gsh_ctx* ctx) {
     if (e) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
+}
+
+static void ctx_retain(gsh_ctx* ctx) {
+  if (ctx) ctx->refs.fetch_add(1);
+}
+
+static void ctx_release(gsh_ctx* ctx) {
+  if (ctx && ctx->refs.fetch_sub(1) == 1 && ctx->closed.load()) ctx_free(ctx);
+}
+
+int gsh_ctx_destroy(gsh_ctx* ctx) {
+  if (!ctx) return GSH_OK;
+  ctx->closed.store(true);
+  if (ctx->refs.load() == 0) {
+    ctx_free(ctx);
+  } else {
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+  }
   return GSH_OK;
 }
 
@@ -978,17 +1058,20 @@ int gsh_mesh_create(gsh_ctx* ctx, const gsh_mesh_desc* desc, gsh_mesh** out, int
     gsh_step_params_default(&p);
     launch_levels(d, step_args(&p), ctx->stream);
     HIPCHK(hipStreamSynchronize(ctx->stream));
+    ctx_retain(ctx);
     *out = m.release();
   })
 }
 
 int gsh_mesh_destroy(gsh_mesh* mesh) {
   if (!mesh) return GSH_OK;
-  if (mesh->ctx) {
-    (void)hipSetDevice(mesh->ctx->device);
-    (void)hipStreamSynchronize(mesh->ctx->stream);
+  gsh_ctx* ctx = mesh->ctx;
+  if (ctx) {
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
   }
   delete mesh;
+  ctx_release(ctx);
   return GSH_OK;
 }
 
@@ -1232,8 +1315,6 @@ int gsh_solver_create(gsh_mesh* mesh, int precision, int precond, gsh_solver** o
     GSH_REQUIRE(mesh && out, GSH_E_ARG, "NULL argument");
     GSH_REQUIRE(precision == GSH_PREC_FP64 || precision == GSH_PREC_MIXED, GSH_E_ARG, "unknown precision");
     GSH_REQUIRE(precond == GSH_PRECOND_JACOBI || precond == GSH_PRECOND_SSOR, GSH_E_ARG, "unknown preconditioner");
-    GSH_REQUIRE(precond == GSH_PRECOND_JACOBI || precision == GSH_PREC_FP64, GSH_E_ARG,
-                "the SSOR preconditioner runs in FP64");
     GSH_REQUIRE(mesh->ctx, GSH_E_STATE, "host-only mesh (created without a context)");
     mesh->ctx->set();
     auto s = std::make_unique<gsh_solver>();
@@ -1253,6 +1334,7 @@ int gsh_solver_create(gsh_mesh* mesh, int precision, int precond, gsh_solver** o
       s->h_indices = t.indices;
       ssor_build(s.get());
     }
+    ctx_retain(s->ctx);
     *out = s.release();
   })
 }
@@ -1263,8 +1345,6 @@ int gsh_solver_create_csr(gsh_ctx* ctx, int32_t n, int64_t nnz, const int32_t* i
     GSH_REQUIRE(ctx && indptr && indices && out && n > 0 && nnz > 0, GSH_E_ARG, "bad arguments");
     GSH_REQUIRE(precision == GSH_PREC_FP64 || precision == GSH_PREC_MIXED, GSH_E_ARG, "unknown precision");
     GSH_REQUIRE(precond == GSH_PRECOND_JACOBI || precond == GSH_PRECOND_SSOR, GSH_E_ARG, "unknown preconditioner");
-    GSH_REQUIRE(precond == GSH_PRECOND_JACOBI || precision == GSH_PREC_FP64, GSH_E_ARG,
-                "the SSOR preconditioner runs in FP64");
     GSH_REQUIRE(nnz < (int64_t)1 << 31, GSH_E_ARG, "nnz exceeds int32 indexing");
     GSH_REQUIRE(indptr[0] == 0 && indptr[n] == nnz, GSH_E_ARG, "indptr must start at 0 and end at nnz");
     for (int32_t i = 0; i < n; ++i) GSH_REQUIRE(indptr[i + 1] >= indptr[i], GSH_E_ARG, "indptr must be non-decreasing");
@@ -1284,15 +1364,18 @@ int gsh_solver_create_csr(gsh_ctx* ctx, int32_t n, int64_t nnz, const int32_t* i
       s->h_indices = std::move(ix);
       ssor_build(s.get());
     }
+    ctx_retain(s->ctx);
     *out = s.release();
   })
 }
 
 int gsh_solver_destroy(gsh_solver* s) {
   if (!s) return GSH_OK;
-  (void)hipSetDevice(s->ctx->device);
-  (void)hipStreamSynchronize(s->ctx->stream);
+  gsh_ctx* ctx = s->ctx;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
   delete s;
+  ctx_release(ctx);
   return GSH_OK;
 }
 
@@ -1382,7 +1465,13 @@ int gsh_solver_autotune(gsh_solver* s, int iters, int* best_rpt, double* us_per_
           ssor_build(s);
         }
         SsorTimes tt;
-        run_ssor(s, 1e-300, iters, iters, 60.0, &tt);
+        if (s->precision == GSH_PREC_MIXED) {
+          ssor_sync_desc32(s, s->b);
+          run_ssor<float>(s, 1e-300, iters, iters, 60.0, &tt);
+        } else {
+          ssor_sync_desc(s);
+          run_ssor<double>(s, 1e-300, iters, iters, 60.0, &tt);
+        }
         us = tt.iters ? 1e3 * (tt.ms_bwd + tt.ms_fwd + tt.ms_upd) / tt.iters : 1e300;
       } else {
         solver_sync_desc(s);
@@ -1479,7 +1568,8 @@ int gsh_solver_solve(gsh_solver* s, const double* b_host, double* x_host, double
     if (s->precision == GSH_PREC_MIXED) {
       solve_mixed(s, rtol, maxiter, check_every, timeout_s, &iters, &status);
     } else if (s->precond == GSH_PRECOND_SSOR) {
-      const PcgState r = run_ssor(s, rtol, maxiter, check_every, timeout_s, nullptr, true);
+      ssor_sync_desc(s);
+      const PcgState r = run_ssor<double>(s, rtol, maxiter, check_every, timeout_s, nullptr, true);
       iters = iters_of(r, maxiter);
       status = (r.done >= 0) ? 0 : GSH_W_MAXITER;
       s->last.iters = iters;
@@ -1875,16 +1965,19 @@ int gsh_server_create(gsh_ctx* ctx, const gsh_server_config* cfg, gsh_server** o
     HIPCHK(hipMemGetInfo(&f, &t));
     s->budget = cfg->hbm_budget_bytes > 0 ? cfg->hbm_budget_bytes : (int64_t)f;
     HIPCHK(hipStreamCreateWithFlags(&s->stage, hipStreamNonBlocking));
+    ctx_retain(ctx);
     *out = s.release();
   })
 }
 
 int gsh_server_destroy(gsh_server* srv) {
   if (!srv) return GSH_OK;
-  (void)hipSetDevice(srv->ctx->device);
-  (void)hipStreamSynchronize(srv->ctx->stream);
+  gsh_ctx* ctx = srv->ctx;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
   if (srv->stage) (void)hipStreamSynchronize(srv->stage);
   delete srv;
+  ctx_release(ctx);
   return GSH_OK;
 }
 
@@ -2140,7 +2233,8 @@ int gsh_solver_profile(gsh_solver* s, double rtol, int maxiter, int check_every,
     for (int i = 0; i < 4; ++i) ms[i] = 0.0;
     if (s->precond == GSH_PRECOND_SSOR) {
       SsorTimes tt;
-      const PcgState r = run_ssor(s, rtol, maxiter, check_every, 60.0, &tt);
+      ssor_sync_desc(s);
+      const PcgState r = run_ssor<double>(s, rtol, maxiter, check_every, 60.0, &tt);
       HIPCHK(hipStreamSynchronize(s->ctx->stream));
       solver_roll(s);
       if (tt.iters) {

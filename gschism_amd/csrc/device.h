@@ -125,7 +125,10 @@ struct SsorColour {
   int32_t offl, offu, pad0, pad1;  // ELL offsets of the lower / upper part
 };
 
-struct SsorSys {
+// VT is the type of the matrix values and iteration vectors (double; float for the inner solves
+// of mixed precision); b, the history, the dot-product partials and the scalars stay FP64.
+template <class VT>
+struct SsorSysT {
   int32_t n, np, ntiles, ncol;
   double omega;
   int32_t unit_omega, pad;                   // omega == 1: 2 D~ - D equals D~ exactly
@@ -134,17 +137,18 @@ struct SsorSys {
   const int32_t *lcol, *ucol;                // ELL columns (-1 unused)
   const int32_t *lsrc, *usrc, *dsrc;         // slots of each entry in the solver's CSR values (-1 unused)
   const int32_t* perm;                       // [np] new row -> original row, -1 for padding
-  const double* aval;                        // the solver's CSR values (original order)
+  const VT* aval;                            // the solver's CSR values (original order)
   const double* b;                           // right-hand side (original order)
   const double *h0, *h1, *h2;                // extrapolation history (original order), NULL = zero guess
-  double* xout;                              // solution (original order)
-  double *lval, *uval;                       // gathered ELL values (0 where unused)
-  double *d, *dt, *kd, *bp;                  // diagonal, D/omega, 2 D/omega - D, permuted b
-  double *y, *r, *p, *q, *t, *u, *s, *w;     // iteration vectors [np]: CG y r p q, sweeps t u, s = A^ z
+  VT* xout;                                  // solution (original order)
+  VT *lval, *uval;                           // gathered ELL values (0 where unused)
+  VT *d, *dt, *kd, *bp;                      // diagonal, D/omega, 2 D/omega - D, permuted b
+  VT *y, *r, *p, *q, *t, *u, *s, *w;         // iteration vectors [np]: CG y r p q, sweeps t u, s = A^ z
   double *part_rr, *part_bb;                 // [ntiles]
   double *part_rz, *part_pq;                 // [2][ntiles]: gamma and delta partials, slot = iteration & 1
   PcgState* st;
 };
+using SsorSys = SsorSysT<double>;
 
 // Launch table entry: which system and which tile a workgroup works on.
 struct TileRef {
@@ -180,20 +184,30 @@ void launch_true_residual(const PcgSys<double>* sys, const TileRef* tiles, int n
 void launch_reduce_pair(const double* pa, const double* pb, int ntiles, double* out2, hipStream_t s);
 
 // Multicolour SSOR PCG (one system). Tiles of colour c are [tile0, tile0 + ntc).
-void launch_ssor_reset(const SsorSys* sys, double rtol, int maxiter, int check_every, hipStream_t s);
-void launch_ssor_setup(const SsorSys* sys, int ntiles, int rpt, hipStream_t s);
-void launch_ssor_residual(const SsorSys* sys, int ntiles, int rpt, hipStream_t s);
-void launch_ssor_r0(const SsorSys* sys, int tile0, int ntc, int c, int rpt, hipStream_t s);
+template <class VT>
+void launch_ssor_reset(const SsorSysT<VT>* sys, double rtol, int maxiter, int check_every, hipStream_t s);
+template <class VT>
+void launch_ssor_setup(const SsorSysT<VT>* sys, int ntiles, int rpt, hipStream_t s);
+template <class VT>
+void launch_ssor_residual(const SsorSysT<VT>* sys, int ntiles, int rpt, hipStream_t s);
+template <class VT>
+void launch_ssor_r0(const SsorSysT<VT>* sys, int tile0, int ntc, int c, int rpt, hipStream_t s);
 // flags: 1 first sweep of the iteration, 2 colour 0 (starts the forward sweep), 4 last sweep
 // of the iteration, 8 the set-up pass (no vector update)
-void launch_ssor_bwd(const SsorSys* sys, int tile0, int ntc, int c, int flags, int rpt, hipStream_t s,
+template <class VT>
+void launch_ssor_bwd(const SsorSysT<VT>* sys, int tile0, int ntc, int c, int flags, int rpt, hipStream_t s,
                      hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
-void launch_ssor_fwd(const SsorSys* sys, int tile0, int ntc, int c, int flags, int rpt, hipStream_t s,
+template <class VT>
+void launch_ssor_fwd(const SsorSysT<VT>* sys, int tile0, int ntc, int c, int flags, int rpt, hipStream_t s,
                      hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
-void launch_ssor_true_rr(const SsorSys* sys, int ntiles, int rpt, hipStream_t s);
-void launch_ssor_check(const SsorSys* sys, hipStream_t s);
-void launch_ssor_xsolve(const SsorSys* sys, int tile0, int ntc, int c, int rpt, hipStream_t s);
-void launch_ssor_scatter(const SsorSys* sys, int ntiles, int rpt, hipStream_t s);
+template <class VT>
+void launch_ssor_true_rr(const SsorSysT<VT>* sys, int ntiles, int rpt, hipStream_t s);
+template <class VT>
+void launch_ssor_check(const SsorSysT<VT>* sys, hipStream_t s);
+template <class VT>
+void launch_ssor_xsolve(const SsorSysT<VT>* sys, int tile0, int ntc, int c, int rpt, hipStream_t s);
+template <class VT>
+void launch_ssor_scatter(const SsorSysT<VT>* sys, int ntiles, int rpt, hipStream_t s);
 
 // Mixed precision helpers.
 void launch_cast_values(const double* in, float* out, int64_t count, hipStream_t s);

@@ -34,21 +34,24 @@
 
 namespace gsh {
 
+template <class VT>
 struct SsorView {
   int np, ntiles;
   double omega;
   gptr<const SsorColour> colours;
   gptr<const int32_t> tile_colour, lcol, ucol, lsrc, usrc, dsrc, perm;
-  gptr<const double> aval, b, h0, h1, h2;
-  gptr<double> xout, lval, uval, d, dt, kd, bp, y, r, p, q, t, u, s, w;
+  gptr<const VT> aval;
+  gptr<const double> b, h0, h1, h2;
+  gptr<VT> xout, lval, uval, d, dt, kd, bp, y, r, p, q, t, u, s, w;
   gptr<double> part_rr, part_rz, part_pq, part_bb;
   gptr<PcgState> st;
   bool has_hist, unit_omega;
 };
 
-__device__ __forceinline__ SsorView sview(const SsorSys* __restrict__ sys) {
-  const SsorSys& S = *sys;
-  SsorView v;
+template <class VT>
+__device__ __forceinline__ SsorView<VT> sview(const SsorSysT<VT>* __restrict__ sys) {
+  const SsorSysT<VT>& S = *sys;
+  SsorView<VT> v;
   v.np = S.np;
   v.ntiles = S.ntiles;
   v.omega = S.omega;
@@ -105,7 +108,8 @@ struct ColourInfo {
   int start, rows, wl, wu, offl, offu;
 };
 
-__device__ __forceinline__ ColourInfo colour_info(const SsorView& S, int c) {
+template <class VT>
+__device__ __forceinline__ ColourInfo colour_info(const SsorView<VT>& S, int c) {
   const gptr<const int32_t> k = (gptr<const int32_t>)(S.colours + c);
   return {k[0], k[1], k[2], k[3], k[4], k[5]};
 }
@@ -114,12 +118,13 @@ __device__ __forceinline__ ColourInfo colour_info(const SsorView& S, int c) {
 // columns of 8 entries are loaded together, then their values and gathers together
 constexpr int kEllChunk = 8;
 
-__device__ __forceinline__ double ell_sum(gptr<const int32_t> ec, gptr<const double> ev, gptr<const double> x,
-                                          int base, int stride, int width) {
-  double acc = 0.0;
+template <class VT>
+__device__ __forceinline__ VT ell_sum(gptr<const int32_t> ec, gptr<const VT> ev, gptr<const VT> x, int base,
+                                      int stride, int width) {
+  VT acc = 0;
   for (int m0 = 0; m0 < width; m0 += kEllChunk) {
     int c[kEllChunk];
-    double vv[kEllChunk], xx[kEllChunk];
+    VT vv[kEllChunk], xx[kEllChunk];
 #pragma unroll
     for (int m = 0; m < kEllChunk; ++m) c[m] = (m0 + m < width) ? ec[base + (m0 + m) * stride] : -1;
 #pragma unroll
@@ -135,47 +140,49 @@ __device__ __forceinline__ double ell_sum(gptr<const int32_t> ec, gptr<const dou
   return acc;
 }
 
-__device__ __forceinline__ double lower_sum(const SsorView& S, const ColourInfo& ci, int v, gptr<const double> x) {
+template <class VT>
+__device__ __forceinline__ VT lower_sum(const SsorView<VT>& S, const ColourInfo& ci, int v, gptr<const VT> x) {
   return ell_sum(S.lcol, C(S.lval), x, ci.offl + (v - ci.start), ci.rows, ci.wl);
 }
 
-__device__ __forceinline__ double upper_sum(const SsorView& S, const ColourInfo& ci, int v, gptr<const double> x) {
+template <class VT>
+__device__ __forceinline__ VT upper_sum(const SsorView<VT>& S, const ColourInfo& ci, int v, gptr<const VT> x) {
   return ell_sum(S.ucol, C(S.uval), x, ci.offu + (v - ci.start), ci.rows, ci.wu);
 }
 
 // ------------------------------------------------------------------ setup
 // Gathers the ELL values and the diagonal from the CSR values, b and the extrapolated guess
 // x0 = 3 x0 - 3 x1 + x2 (pcg_solver.py:881-886) into the colour-major order; b.b partials.
-template <int R>
-__global__ void __launch_bounds__(256) k_ssor_setup(const SsorSys* __restrict__ sys) {
+template <class VT, int R>
+__global__ void __launch_bounds__(256) k_ssor_setup(const SsorSysT<VT>* __restrict__ sys) {
   __shared__ double red[4];
-  const SsorView S = sview(sys);
+  const SsorView<VT> S = sview(sys);
   const int tile = blockIdx.x;
   const ColourInfo ci = colour_info(S, S.tile_colour[tile]);
   double bb = 0.0;
   for (int j = 0; j < R; ++j) {
     const int v = tile_row<R>(tile, j);
     const int ds = S.dsrc[v];
-    const double dv = ds >= 0 ? S.aval[ds] : 1.0;
-    const double dtv = dv / S.omega;
+    const VT dv = ds >= 0 ? S.aval[ds] : (VT)1;
+    const VT dtv = dv / (VT)S.omega;
     S.d[v] = dv;
     S.dt[v] = dtv;
-    S.kd[v] = 2.0 * dtv - dv;
+    S.kd[v] = (VT)2 * dtv - dv;
     const int ro = v - ci.start;
     for (int m = 0; m < ci.wl; ++m) {
       const int k = ci.offl + m * ci.rows + ro;
       const int src = S.lsrc[k];
-      S.lval[k] = src >= 0 ? S.aval[src] : 0.0;
+      S.lval[k] = src >= 0 ? S.aval[src] : (VT)0;
     }
     for (int m = 0; m < ci.wu; ++m) {
       const int k = ci.offu + m * ci.rows + ro;
       const int src = S.usrc[k];
-      S.uval[k] = src >= 0 ? S.aval[src] : 0.0;
+      S.uval[k] = src >= 0 ? S.aval[src] : (VT)0;
     }
     const int i = S.perm[v];
     const double bv = i >= 0 ? S.b[i] : 0.0;
-    S.bp[v] = bv;
-    S.w[v] = (i >= 0 && S.has_hist) ? (3.0 * S.h0[i] - 3.0 * S.h1[i]) + S.h2[i] : 0.0;
+    S.bp[v] = (VT)bv;
+    S.w[v] = (VT)((i >= 0 && S.has_hist) ? (3.0 * S.h0[i] - 3.0 * S.h1[i]) + S.h2[i] : 0.0);
     bb += bv * bv;
   }
   bb = block_tree(bb, red);
@@ -183,30 +190,30 @@ __global__ void __launch_bounds__(256) k_ssor_setup(const SsorSys* __restrict__ 
 }
 
 // r_true = b - A x0 (computeResiduals, pcg_solver.py:889-895) into t; y0 = (D~ + U) x0; p = 0.
-template <int R>
-__global__ void __launch_bounds__(256) k_ssor_residual(const SsorSys* __restrict__ sys) {
-  const SsorView S = sview(sys);
+template <class VT, int R>
+__global__ void __launch_bounds__(256) k_ssor_residual(const SsorSysT<VT>* __restrict__ sys) {
+  const SsorView<VT> S = sview(sys);
   const ColourInfo ci = colour_info(S, S.tile_colour[blockIdx.x]);
-  const gptr<const double> x0 = C(S.w);
+  const gptr<const VT> x0 = C(S.w);
   for (int j = 0; j < R; ++j) {
     const int v = tile_row<R>(blockIdx.x, j);
-    const double xv = x0[v];
-    const double au = upper_sum(S, ci, v, x0);
-    const double ax = (lower_sum(S, ci, v, x0) + S.d[v] * xv) + au;
+    const VT xv = x0[v];
+    const VT au = upper_sum(S, ci, v, x0);
+    const VT ax = (lower_sum(S, ci, v, x0) + S.d[v] * xv) + au;
     S.t[v] = S.bp[v] - ax;
     S.y[v] = S.dt[v] * xv + au;
-    S.p[v] = 0.0;
-    S.q[v] = 0.0;
+    S.p[v] = 0;
+    S.q[v] = 0;
   }
 }
 
 // r0 = (D~ + L)^-1 r_true for the rows of one colour.
-template <int R>
-__global__ void __launch_bounds__(256) k_ssor_r0(const SsorSys* __restrict__ sys, int tile0, int c) {
-  const SsorView S = sview(sys);
+template <class VT, int R>
+__global__ void __launch_bounds__(256) k_ssor_r0(const SsorSysT<VT>* __restrict__ sys, int tile0, int c) {
+  const SsorView<VT> S = sview(sys);
   const ColourInfo ci = colour_info(S, c);
   const int tile = tile0 + xcd_block(blockIdx.x, gridDim.x);
-  const gptr<const double> r = C(S.r);
+  const gptr<const VT> r = C(S.r);
   for (int j = 0; j < R; ++j) {
     const int v = tile_row<R>(tile, j);
     S.r[v] = (S.t[v] - lower_sum(S, ci, v, r)) / S.dt[v];
@@ -221,16 +228,16 @@ constexpr int kFirst = 1, kMid = 2, kLast = 4, kInit = 8;
 // gamma and the iteration index; every sweep then applies p = z + beta p, q = s + beta q,
 // y += alpha p, r -= alpha q to its rows before t = (z - U t) / D~ with the new z = D~ r. The
 // sweep of colour 0 (no lower colours) also forms u and s = t + u for its rows.
-template <int R>
-__global__ void __launch_bounds__(256) k_ssor_bwd(const SsorSys* __restrict__ sys, int tile0, int c, int flags) {
+template <class VT, int R>
+__global__ void __launch_bounds__(256) k_ssor_bwd(const SsorSysT<VT>* __restrict__ sys, int tile0, int c, int flags) {
   __shared__ double red[4];
-  const SsorView S = sview(sys);
+  const SsorView<VT> S = sview(sys);
   gptr<PcgState> st = S.st;
   const bool init = flags & kInit, mid = flags & kMid;
   const ColourInfo ci = colour_info(S, c);
   const int tile = tile0 + xcd_block(blockIdx.x, gridDim.x);
-  const gptr<const double> t = C(S.t);
-  double usum[R], dtv[R], rv[R], pv[R], qv[R], sv[R], yv[R];
+  const gptr<const VT> t = C(S.t);
+  VT usum[R], dtv[R], rv[R], pv[R], qv[R], sv[R], yv[R];
   // the row loads are issued before the done test so that their latency overlaps it
 #pragma unroll
   for (int j = 0; j < R; ++j) {
@@ -291,32 +298,33 @@ __global__ void __launch_bounds__(256) k_ssor_bwd(const SsorSys* __restrict__ sy
   }
 #pragma unroll
   for (int j = 0; j < R; ++j) usum[j] = upper_sum(S, ci, tile_row<R>(tile, j), t);
+  const VT av = (VT)alpha, bv = (VT)beta;
   double g = 0.0, dl = 0.0;
 #pragma unroll
   for (int j = 0; j < R; ++j) {
     const int v = tile_row<R>(tile, j);
-    double r1 = rv[j];
+    VT r1 = rv[j];
     if (!init) {
-      const double zr = dtv[j] * rv[j];
-      const double pn = zr + beta * pv[j];
-      const double qn = sv[j] + beta * qv[j];
+      const VT zr = dtv[j] * rv[j];
+      const VT pn = zr + bv * pv[j];
+      const VT qn = sv[j] + bv * qv[j];
       S.p[v] = pn;
       S.q[v] = qn;
-      S.y[v] = yv[j] + alpha * pn;
-      r1 = rv[j] - alpha * qn;
+      S.y[v] = yv[j] + av * pn;
+      r1 = rv[j] - av * qn;
       S.r[v] = r1;
     }
-    const double zn = dtv[j] * r1;
-    g += r1 * zn;
-    const double tv = (zn - usum[j]) / dtv[j];
+    const VT zn = dtv[j] * r1;
+    g += (double)r1 * (double)zn;
+    const VT tv = (zn - usum[j]) / dtv[j];
     S.t[v] = tv;
     if (mid) {
-      const double kdv = S.unit_omega ? dtv[j] : S.kd[v];
-      const double uv = ((zn - kdv * tv) - 0.0) / dtv[j];
+      const VT kdv = S.unit_omega ? dtv[j] : S.kd[v];
+      const VT uv = ((zn - kdv * tv) - (VT)0) / dtv[j];
       S.u[v] = uv;
-      const double sn = tv + uv;
+      const VT sn = tv + uv;
       S.s[v] = sn;
-      dl += zn * sn;
+      dl += (double)zn * (double)sn;
     }
   }
   g = block_tree(g, red);
@@ -331,14 +339,14 @@ __global__ void __launch_bounds__(256) k_ssor_bwd(const SsorSys* __restrict__ sy
 // ------------------------------------------------------------------ forward sweep
 // u = (D~ + L)^-1 (z - K t) for one colour, s = t + u = A^ z, partials of z.s. The last sweep of
 // an iteration advances the iteration counter.
-template <int R>
-__global__ void __launch_bounds__(256) k_ssor_fwd(const SsorSys* __restrict__ sys, int tile0, int c, int flags) {
+template <class VT, int R>
+__global__ void __launch_bounds__(256) k_ssor_fwd(const SsorSysT<VT>* __restrict__ sys, int tile0, int c, int flags) {
   __shared__ double red[4];
-  const SsorView S = sview(sys);
+  const SsorView<VT> S = sview(sys);
   const ColourInfo ci = colour_info(S, c);
   const int tile = tile0 + xcd_block(blockIdx.x, gridDim.x);
-  const gptr<const double> u = C(S.u);
-  double lsum[R], dtv[R], tv[R], rv[R], kdv[R];
+  const gptr<const VT> u = C(S.u);
+  VT lsum[R], dtv[R], tv[R], rv[R], kdv[R];
   // the row loads are issued before the done test so that their latency overlaps it
 #pragma unroll
   for (int j = 0; j < R; ++j) {
@@ -356,12 +364,12 @@ __global__ void __launch_bounds__(256) k_ssor_fwd(const SsorSys* __restrict__ sy
 #pragma unroll
   for (int j = 0; j < R; ++j) {
     const int v = tile_row<R>(tile, j);
-    const double zn = dtv[j] * rv[j];
-    const double uv = ((zn - kdv[j] * tv[j]) - lsum[j]) / dtv[j];
+    const VT zn = dtv[j] * rv[j];
+    const VT uv = ((zn - kdv[j] * tv[j]) - lsum[j]) / dtv[j];
     S.u[v] = uv;
-    const double sn = tv[j] + uv;
+    const VT sn = tv[j] + uv;
     S.s[v] = sn;
-    dl += zn * sn;
+    dl += (double)zn * (double)sn;
   }
   dl = block_tree(dl, red);
   if (threadIdx.x == 0) S.part_pq[wslot * S.ntiles + tile] = dl;
@@ -371,27 +379,28 @@ __global__ void __launch_bounds__(256) k_ssor_fwd(const SsorSys* __restrict__ sy
 // ------------------------------------------------------------------ stopping test
 // ||b - A x||^2 = ||(D~ + L) r||^2 partials, then the test of iteration k-1 (pcg_loop.cpp:405-412)
 // in one workgroup. Issued after the last iteration of every window of check_every.
-template <int R>
-__global__ void __launch_bounds__(256) k_ssor_true_rr(const SsorSys* __restrict__ sys) {
+template <class VT, int R>
+__global__ void __launch_bounds__(256) k_ssor_true_rr(const SsorSysT<VT>* __restrict__ sys) {
   __shared__ double red[4];
-  const SsorView S = sview(sys);
+  const SsorView<VT> S = sview(sys);
   if (load_done(S.st) >= 0) return;
   const int tile = xcd_block(blockIdx.x, gridDim.x);
   const ColourInfo ci = colour_info(S, S.tile_colour[tile]);
-  const gptr<const double> r = C(S.r);
+  const gptr<const VT> r = C(S.r);
   double rr = 0.0;
   for (int j = 0; j < R; ++j) {
     const int v = tile_row<R>(tile, j);
-    const double wv = S.dt[v] * r[v] + lower_sum(S, ci, v, r);
-    rr += wv * wv;
+    const VT wv = S.dt[v] * r[v] + lower_sum(S, ci, v, r);
+    rr += (double)wv * (double)wv;
   }
   rr = block_tree(rr, red);
   if (threadIdx.x == 0) S.part_rr[tile] = rr;
 }
 
-__global__ void __launch_bounds__(256) k_ssor_check(const SsorSys* __restrict__ sys) {
+template <class VT>
+__global__ void __launch_bounds__(256) k_ssor_check(const SsorSysT<VT>* __restrict__ sys) {
   __shared__ double red[4];
-  const SsorView S = sview(sys);
+  const SsorView<VT> S = sview(sys);
   gptr<PcgState> st = S.st;
   if (load_done(st) >= 0) return;
   const int k = st->k;
@@ -408,31 +417,32 @@ __global__ void __launch_bounds__(256) k_ssor_check(const SsorSys* __restrict__ 
 // ------------------------------------------------------------------ back to x
 // x = (D~ + U)^-1 y for one colour (into w), then scattered to the original order; x = 0 when
 // atol == 0 (pcg_solver.py:1089-1095).
-template <int R>
-__global__ void __launch_bounds__(256) k_ssor_xsolve(const SsorSys* __restrict__ sys, int tile0, int c) {
-  const SsorView S = sview(sys);
+template <class VT, int R>
+__global__ void __launch_bounds__(256) k_ssor_xsolve(const SsorSysT<VT>* __restrict__ sys, int tile0, int c) {
+  const SsorView<VT> S = sview(sys);
   if (S.st->zero) return;
   const ColourInfo ci = colour_info(S, c);
   const int tile = tile0 + xcd_block(blockIdx.x, gridDim.x);
-  const gptr<const double> x = C(S.w);
+  const gptr<const VT> x = C(S.w);
   for (int j = 0; j < R; ++j) {
     const int v = tile_row<R>(tile, j);
     S.w[v] = (S.y[v] - upper_sum(S, ci, v, x)) / S.dt[v];
   }
 }
 
-template <int R>
-__global__ void __launch_bounds__(256) k_ssor_scatter(const SsorSys* __restrict__ sys) {
-  const SsorView S = sview(sys);
+template <class VT, int R>
+__global__ void __launch_bounds__(256) k_ssor_scatter(const SsorSysT<VT>* __restrict__ sys) {
+  const SsorView<VT> S = sview(sys);
   const bool zero = S.st->zero != 0;
   for (int j = 0; j < R; ++j) {
     const int v = tile_row<R>(blockIdx.x, j);
     const int i = S.perm[v];
-    if (i >= 0) S.xout[i] = zero ? 0.0 : S.w[v];
+    if (i >= 0) S.xout[i] = zero ? (VT)0 : S.w[v];
   }
 }
 
-__global__ void k_ssor_reset(const SsorSys* __restrict__ sys, double rtol, int maxiter, int check_every) {
+template <class VT>
+__global__ void k_ssor_reset(const SsorSysT<VT>* __restrict__ sys, double rtol, int maxiter, int check_every) {
   if (threadIdx.x != 0) return;
   gptr<PcgState> st = G(sys->st);
   st->k = 0;
@@ -459,9 +469,9 @@ __global__ void k_ssor_reset(const SsorSys* __restrict__ sys, double rtol, int m
 #define GSH_SSOR_EXT(R, KER, GRID, E0, E1, STREAM, ...)                                               \
   do {                                                                                               \
     if (E0 || E1)                                                                                    \
-      hipExtLaunchKernelGGL(KER<R>, dim3(GRID), dim3(kTpb), 0, STREAM, E0, E1, 0, __VA_ARGS__);       \
+      hipExtLaunchKernelGGL((KER<VT, R>), dim3(GRID), dim3(kTpb), 0, STREAM, E0, E1, 0, __VA_ARGS__);       \
     else                                                                                             \
-      hipLaunchKernelGGL(KER<R>, dim3(GRID), dim3(kTpb), 0, STREAM, __VA_ARGS__);                     \
+      hipLaunchKernelGGL((KER<VT, R>), dim3(GRID), dim3(kTpb), 0, STREAM, __VA_ARGS__);                     \
   } while (0)
 #define GSH_SSOR_RX(rpt, KER, GRID, E0, E1, STREAM, ...)                         \
   switch (rpt) {                                                               \
@@ -472,43 +482,70 @@ __global__ void k_ssor_reset(const SsorSys* __restrict__ sys, double rtol, int m
   }
 #define GSH_SSOR_R(rpt, KER, GRID, STREAM, ...)                                          \
   switch (rpt) {                                                                       \
-    case 1: hipLaunchKernelGGL(KER<1>, dim3(GRID), dim3(kTpb), 0, STREAM, __VA_ARGS__); break;   \
-    case 2: hipLaunchKernelGGL(KER<2>, dim3(GRID), dim3(kTpb), 0, STREAM, __VA_ARGS__); break;   \
-    case 8: hipLaunchKernelGGL(KER<8>, dim3(GRID), dim3(kTpb), 0, STREAM, __VA_ARGS__); break;   \
-    default: hipLaunchKernelGGL(KER<4>, dim3(GRID), dim3(kTpb), 0, STREAM, __VA_ARGS__); break;  \
+    case 1: hipLaunchKernelGGL((KER<VT, 1>), dim3(GRID), dim3(kTpb), 0, STREAM, __VA_ARGS__); break;   \
+    case 2: hipLaunchKernelGGL((KER<VT, 2>), dim3(GRID), dim3(kTpb), 0, STREAM, __VA_ARGS__); break;   \
+    case 8: hipLaunchKernelGGL((KER<VT, 8>), dim3(GRID), dim3(kTpb), 0, STREAM, __VA_ARGS__); break;   \
+    default: hipLaunchKernelGGL((KER<VT, 4>), dim3(GRID), dim3(kTpb), 0, STREAM, __VA_ARGS__); break;  \
   }
 
-void launch_ssor_reset(const SsorSys* sys, double rtol, int maxiter, int check_every, hipStream_t s) {
-  hipLaunchKernelGGL(k_ssor_reset, dim3(1), dim3(64), 0, s, sys, rtol, maxiter, check_every);
+template <class VT>
+void launch_ssor_reset(const SsorSysT<VT>* sys, double rtol, int maxiter, int check_every, hipStream_t s) {
+  hipLaunchKernelGGL(k_ssor_reset<VT>, dim3(1), dim3(64), 0, s, sys, rtol, maxiter, check_every);
 }
-void launch_ssor_setup(const SsorSys* sys, int ntiles, int rpt, hipStream_t s) {
+template <class VT>
+void launch_ssor_setup(const SsorSysT<VT>* sys, int ntiles, int rpt, hipStream_t s) {
   if (ntiles > 0) GSH_SSOR_R(rpt, k_ssor_setup, ntiles, s, sys);
 }
-void launch_ssor_residual(const SsorSys* sys, int ntiles, int rpt, hipStream_t s) {
+template <class VT>
+void launch_ssor_residual(const SsorSysT<VT>* sys, int ntiles, int rpt, hipStream_t s) {
   if (ntiles > 0) GSH_SSOR_R(rpt, k_ssor_residual, ntiles, s, sys);
 }
-void launch_ssor_r0(const SsorSys* sys, int tile0, int ntc, int c, int rpt, hipStream_t s) {
+template <class VT>
+void launch_ssor_r0(const SsorSysT<VT>* sys, int tile0, int ntc, int c, int rpt, hipStream_t s) {
   if (ntc > 0) GSH_SSOR_R(rpt, k_ssor_r0, ntc, s, sys, tile0, c);
 }
-void launch_ssor_bwd(const SsorSys* sys, int tile0, int ntc, int c, int flags, int rpt, hipStream_t s,
+template <class VT>
+void launch_ssor_bwd(const SsorSysT<VT>* sys, int tile0, int ntc, int c, int flags, int rpt, hipStream_t s,
                      hipEvent_t e0, hipEvent_t e1) {
   if (ntc > 0) GSH_SSOR_RX(rpt, k_ssor_bwd, ntc, e0, e1, s, sys, tile0, c, flags);
 }
-void launch_ssor_fwd(const SsorSys* sys, int tile0, int ntc, int c, int flags, int rpt, hipStream_t s,
+template <class VT>
+void launch_ssor_fwd(const SsorSysT<VT>* sys, int tile0, int ntc, int c, int flags, int rpt, hipStream_t s,
                      hipEvent_t e0, hipEvent_t e1) {
   if (ntc > 0) GSH_SSOR_RX(rpt, k_ssor_fwd, ntc, e0, e1, s, sys, tile0, c, flags);
 }
-void launch_ssor_true_rr(const SsorSys* sys, int ntiles, int rpt, hipStream_t s) {
+template <class VT>
+void launch_ssor_true_rr(const SsorSysT<VT>* sys, int ntiles, int rpt, hipStream_t s) {
   if (ntiles > 0) GSH_SSOR_R(rpt, k_ssor_true_rr, ntiles, s, sys);
 }
-void launch_ssor_check(const SsorSys* sys, hipStream_t s) {
-  hipLaunchKernelGGL(k_ssor_check, dim3(1), dim3(kTpb), 0, s, sys);
+template <class VT>
+void launch_ssor_check(const SsorSysT<VT>* sys, hipStream_t s) {
+  hipLaunchKernelGGL(k_ssor_check<VT>, dim3(1), dim3(kTpb), 0, s, sys);
 }
-void launch_ssor_xsolve(const SsorSys* sys, int tile0, int ntc, int c, int rpt, hipStream_t s) {
+template <class VT>
+void launch_ssor_xsolve(const SsorSysT<VT>* sys, int tile0, int ntc, int c, int rpt, hipStream_t s) {
   if (ntc > 0) GSH_SSOR_R(rpt, k_ssor_xsolve, ntc, s, sys, tile0, c);
 }
-void launch_ssor_scatter(const SsorSys* sys, int ntiles, int rpt, hipStream_t s) {
+template <class VT>
+void launch_ssor_scatter(const SsorSysT<VT>* sys, int ntiles, int rpt, hipStream_t s) {
   if (ntiles > 0) GSH_SSOR_R(rpt, k_ssor_scatter, ntiles, s, sys);
 }
+
+
+#define GSH_SSOR_INSTANTIATE(VT)                                                                       \
+  template void launch_ssor_reset<VT>(const SsorSysT<VT>*, double, int, int, hipStream_t);             \
+  template void launch_ssor_setup<VT>(const SsorSysT<VT>*, int, int, hipStream_t);                     \
+  template void launch_ssor_residual<VT>(const SsorSysT<VT>*, int, int, hipStream_t);                  \
+  template void launch_ssor_r0<VT>(const SsorSysT<VT>*, int, int, int, int, hipStream_t);              \
+  template void launch_ssor_bwd<VT>(const SsorSysT<VT>*, int, int, int, int, int, hipStream_t, hipEvent_t, \
+                                    hipEvent_t);                                                       \
+  template void launch_ssor_fwd<VT>(const SsorSysT<VT>*, int, int, int, int, int, hipStream_t, hipEvent_t, \
+                                    hipEvent_t);                                                       \
+  template void launch_ssor_true_rr<VT>(const SsorSysT<VT>*, int, int, hipStream_t);                   \
+  template void launch_ssor_check<VT>(const SsorSysT<VT>*, hipStream_t);                               \
+  template void launch_ssor_xsolve<VT>(const SsorSysT<VT>*, int, int, int, int, hipStream_t);          \
+  template void launch_ssor_scatter<VT>(const SsorSysT<VT>*, int, int, hipStream_t);
+GSH_SSOR_INSTANTIATE(double)
+GSH_SSOR_INSTANTIATE(float)
 
 }  // namespace gsh

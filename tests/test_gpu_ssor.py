@@ -113,8 +113,11 @@ def test_ssor_standalone_csr_and_special_cases(ctx):
     assert_bitwise(x, xo, "SSOR at maxiter")
     with pytest.raises(L.GshError):
         s.set_omega(2.5)
-    with pytest.raises(L.GshError):
-        core.Solver(None, L.PREC_MIXED, ctx=ctx, indptr=A.indptr, indices=A.indices, precond=L.PRECOND_SSOR)
+    sm = core.Solver(None, L.PREC_MIXED, ctx=ctx, indptr=A.indptr, indices=A.indices, precond=L.PRECOND_SSOR)
+    sm.set_values(A.data)
+    xm, itm, stm = sm.solve(b, rtol=1e-10, maxiter=2000)
+    assert stm == 0 and np.linalg.norm(b - A @ xm) / np.linalg.norm(b) <= 1e-10 * (1 + 1e-6)
+    sm.close()
 
 
 def test_autotune_keeps_results_exact(ctx):
@@ -186,3 +189,32 @@ def test_window_graphs_replay_exactly(ctx):
             assert_bitwise(x, xo, f"graph replay {rep} (precond {precond})")
             assert s.check() < 1e-10
         s.close()
+
+
+def test_mixed_precision_ssor(ctx):
+    """FP32 SSOR inner solves with FP64 residual and refinement: the outer test is met on the
+    FP64 residual, the solution is within the tolerance of the FP64 SSOR solution, repeated
+    solves are bitwise equal, and the FP64 solver of the same mesh is unaffected."""
+    spec = cases.CaseSpec("channel-64x40", 64, 40, 500.0, nvrt=6, depth=None, substeps=4)
+    mesh, om, p, val, b = _assembled(ctx, spec)
+    print(f"[gpu] mixed SSOR n={mesh.n}", flush=True)
+    s64 = core.Solver(mesh, L.PREC_FP64, precond=L.PRECOND_SSOR)
+    s64.assemble(p)
+    x64, it64, st64 = s64.solve(None, rtol=1e-12, maxiter=5000, check_every=10)
+    assert st64 == 0
+    sm = core.Solver(mesh, L.PREC_MIXED, precond=L.PRECOND_SSOR)
+    sm.assemble(p)
+    x1, it1, st1 = sm.solve(None, rtol=1e-10, maxiter=5000, check_every=10)
+    stt = sm.stats()
+    assert st1 == 0 and stt.outer >= 2 and it1 > 0, (st1, stt.outer, it1)
+    assert stt.rr <= stt.atol2
+    assert np.linalg.norm(x1 - x64) / np.linalg.norm(x64) < 1e-8
+    assert sm.check() < 1e-9
+    sm.reset_history()
+    x2, it2, _ = sm.solve(None, rtol=1e-10, maxiter=5000, check_every=10)
+    assert it2 == it1
+    assert_bitwise(x2, x1, "repeated mixed SSOR solve")
+    # the mixed solver needs no more inner iterations than twice the FP64 SSOR count
+    assert it1 <= 2 * it64 + 50, (it1, it64)
+    sm.close()
+    s64.close()
