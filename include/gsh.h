@@ -48,7 +48,8 @@ extern "C" {
 #define GSH_PREC_MIXED    1   /* FP32 SpMV/precond/axpy, FP64 dots + true residual + refinement */
 
 #define GSH_PRECOND_JACOBI 0  /* reference preconditioner (pcg_solver.py:933-938) */
-#define GSH_PRECOND_SSOR   1  /* multicolour symmetric SOR, FP64 only (no counterpart in the reference,
+#define GSH_PRECOND_SSOR   1  /* multicolour symmetric SOR; FP64, or FP32 inner solves under GSH_PREC_MIXED
+                                 (single systems; no counterpart in the reference,
                                  whose only preconditioner is Jacobi; same driver and stopping rule) */
 
 /* ---------------------------------------------------------------- context */
