@@ -62,8 +62,8 @@ class PCGSolverMultiGPU:
 
     Constructed either on a mesh (the wave-equation pattern, as main.py:281 does after the
     grid is read) or on a caller CSR pattern (indptr, indices). `precision` is "fp64" (the
-    reference arithmetic) or "mixed"; `precond` is "jacobi" (the reference's) or "ssor"
-    (multicolour symmetric SOR, FP64).
+    reference arithmetic) or "mixed" (FP32 inner solves, FP64 residual and refinement);
+    `precond` is "jacobi" (the reference's) or "ssor" (multicolour symmetric SOR).
     """
 
     TPB = 256
