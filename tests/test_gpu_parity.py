@@ -397,3 +397,21 @@ def test_full_size_1m_channel(ctx):
     assert solver.check() < 1e-10
     solver.close()
     mesh.close()
+
+
+def test_objects_outlive_their_context():
+    """A context closed while a mesh and a solver still use it stays alive until the last of
+    them is closed (they keep working in between)."""
+    spec = cases.CaseSpec("channel-24x16", 24, 16, 500.0, nvrt=4, depth=10.0, substeps=4)
+    c2 = core.Context(0)
+    mesh, p, _ = cases.load_mesh(c2, cases.make_case(spec))
+    s = core.Solver(mesh)
+    print("[gpu] closing the context before its mesh and solver", flush=True)
+    c2.close()
+    mesh.backtrack(p)
+    mesh.rhs_producers(p)
+    s.assemble(p)
+    x, it, st = s.solve(None, rtol=1e-12, maxiter=500, check_every=10)
+    assert st == 0 and it > 0 and np.isfinite(x).all()
+    s.close()
+    mesh.close()
