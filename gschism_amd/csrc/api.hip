@@ -591,7 +591,7 @@ struct SsorTimes {
 // the VT system must be synchronised first (ssor_sync_desc / ssor_sync_desc32).
 template <class VT>
 static PcgState run_ssor(gsh_solver* s, double rtol, int maxiter, int check_every, double timeout_s,
-                         SsorTimes* prof = nullptr, bool use_graph = false) {
+                         SsorTimes* prof = nullptr, bool use_graph = false, bool gather = true) {
   gsh_ctx* ctx = s->ctx;
   hipStream_t st = ctx->stream;
   if (check_every < 1) check_every = 1;
@@ -601,7 +601,7 @@ static PcgState run_ssor(gsh_solver* s, double rtol, int maxiter, int check_ever
   const int rpt = S.rpt, C = S.L.ncol, NT = S.ntiles;
   PcgState* slots = ctx->pinned_slots(1);
   launch_ssor_reset(ds, rtol, maxiter, check_every, st);
-  launch_ssor_setup(ds, NT, rpt, st);
+  launch_ssor_setup(ds, NT, rpt, st, gather);
   launch_ssor_residual(ds, NT, rpt, st);
   for (int c = 0; c < C; ++c) launch_ssor_r0(ds, S.tile0[c], S.ntc[c], c, rpt, st);
   // flags of the sweeps (see ssor_kernels.hip): 1 first, 2 colour 0, 4 last of the iteration, 8 set-up
@@ -725,6 +725,7 @@ static void run_mixed(gsh_ctx* ctx, gsh_solver* const* S, int count, double rtol
   std::vector<double> atol2(count, -1.0), rr(count, 0.0), red(2 * count);
   std::vector<int> outer(count, 0), inner(count, 0), state(count, 0);  // 0 running, 1 converged, 2 stopped
   const int max_outer = 30;
+  bool ssor_gathered = false;  // SSOR inner solves: the ELL values are gathered by the first one
   for (int round = 0; round < max_outer; ++round) {
     // FP64 true residual of every running system
     std::vector<TileRef> tl;
@@ -783,7 +784,8 @@ static void run_mixed(gsh_ctx* ctx, gsh_solver* const* S, int count, double rtol
       // one system (batches take Jacobi solvers): FP32 SSOR PCG of A d = r, b = the FP64 residual
       gsh_solver* s = S[sub[0]];
       ssor_sync_desc32(s, s->r64);
-      o.st.assign(1, run_ssor<float>(s, rts[0], maxiter, check_every, timeout_s));
+      o.st.assign(1, run_ssor<float>(s, rts[0], maxiter, check_every, timeout_s, nullptr, false, !ssor_gathered));
+      ssor_gathered = true;
     } else {
       run_pcg<float>(ctx, d32s, ns, dtl, (int)stl.size(), rpt, rtol, maxiter, check_every, timeout_s, o, nullptr, -1,
                      nullptr, drt);

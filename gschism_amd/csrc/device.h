@@ -187,7 +187,7 @@ void launch_reduce_pair(const double* pa, const double* pb, int ntiles, double* 
 template <class VT>
 void launch_ssor_reset(const SsorSysT<VT>* sys, double rtol, int maxiter, int check_every, hipStream_t s);
 template <class VT>
-void launch_ssor_setup(const SsorSysT<VT>* sys, int ntiles, int rpt, hipStream_t s);
+void launch_ssor_setup(const SsorSysT<VT>* sys, int ntiles, int rpt, hipStream_t s, bool gather = true);
 template <class VT>
 void launch_ssor_residual(const SsorSysT<VT>* sys, int ntiles, int rpt, hipStream_t s);
 template <class VT>

@@ -153,8 +153,31 @@ __device__ __forceinline__ VT upper_sum(const SsorView<VT>& S, const ColourInfo&
 // ------------------------------------------------------------------ setup
 // Gathers the ELL values and the diagonal from the CSR values, b and the extrapolated guess
 // x0 = 3 x0 - 3 x1 + x2 (pcg_solver.py:881-886) into the colour-major order; b.b partials.
+// ELL values of one row part from the CSR values: the slots of 8 entries are loaded together,
+// then their values, then the stores (a store between the loads would order every load after it).
+template <class VT>
+__device__ __forceinline__ void ell_gather(gptr<const int32_t> src, gptr<const VT> aval, gptr<VT> out, int base,
+                                           int stride, int width) {
+  for (int m0 = 0; m0 < width; m0 += kEllChunk) {
+    int k[kEllChunk], sl[kEllChunk];
+    VT v[kEllChunk];
+#pragma unroll
+    for (int m = 0; m < kEllChunk; ++m) {
+      k[m] = base + (m0 + m) * stride;
+      sl[m] = (m0 + m < width) ? src[k[m]] : -1;
+    }
+#pragma unroll
+    for (int m = 0; m < kEllChunk; ++m) v[m] = sl[m] >= 0 ? aval[sl[m]] : (VT)0;
+#pragma unroll
+    for (int m = 0; m < kEllChunk; ++m)
+      if (m0 + m < width) out[k[m]] = v[m];
+  }
+}
+
+// gather = 0 keeps the ELL values and diagonal of the previous set-up (the inner solves of one
+// mixed-precision solve share their matrix) and only loads b and the guess.
 template <class VT, int R>
-__global__ void __launch_bounds__(256) k_ssor_setup(const SsorSysT<VT>* __restrict__ sys) {
+__global__ void __launch_bounds__(256) k_ssor_setup(const SsorSysT<VT>* __restrict__ sys, int gather) {
   __shared__ double red[4];
   const SsorView<VT> S = sview(sys);
   const int tile = blockIdx.x;
@@ -162,22 +185,18 @@ __global__ void __launch_bounds__(256) k_ssor_setup(const SsorSysT<VT>* __restri
   double bb = 0.0;
   for (int j = 0; j < R; ++j) {
     const int v = tile_row<R>(tile, j);
-    const int ds = S.dsrc[v];
-    const VT dv = ds >= 0 ? S.aval[ds] : (VT)1;
-    const VT dtv = dv / (VT)S.omega;
-    S.d[v] = dv;
-    S.dt[v] = dtv;
-    S.kd[v] = (VT)2 * dtv - dv;
-    const int ro = v - ci.start;
-    for (int m = 0; m < ci.wl; ++m) {
-      const int k = ci.offl + m * ci.rows + ro;
-      const int src = S.lsrc[k];
-      S.lval[k] = src >= 0 ? S.aval[src] : (VT)0;
+    if (gather) {
+      const int ds = S.dsrc[v];
+      const VT dv = ds >= 0 ? S.aval[ds] : (VT)1;
+      const VT dtv = dv / (VT)S.omega;
+      S.d[v] = dv;
+      S.dt[v] = dtv;
+      S.kd[v] = (VT)2 * dtv - dv;
     }
-    for (int m = 0; m < ci.wu; ++m) {
-      const int k = ci.offu + m * ci.rows + ro;
-      const int src = S.usrc[k];
-      S.uval[k] = src >= 0 ? S.aval[src] : (VT)0;
+    const int ro = v - ci.start;
+    if (gather) {
+      ell_gather(S.lsrc, S.aval, S.lval, ci.offl + ro, ci.rows, ci.wl);
+      ell_gather(S.usrc, S.aval, S.uval, ci.offu + ro, ci.rows, ci.wu);
     }
     const int i = S.perm[v];
     const double bv = i >= 0 ? S.b[i] : 0.0;
@@ -493,8 +512,8 @@ void launch_ssor_reset(const SsorSysT<VT>* sys, double rtol, int maxiter, int ch
   hipLaunchKernelGGL(k_ssor_reset<VT>, dim3(1), dim3(64), 0, s, sys, rtol, maxiter, check_every);
 }
 template <class VT>
-void launch_ssor_setup(const SsorSysT<VT>* sys, int ntiles, int rpt, hipStream_t s) {
-  if (ntiles > 0) GSH_SSOR_R(rpt, k_ssor_setup, ntiles, s, sys);
+void launch_ssor_setup(const SsorSysT<VT>* sys, int ntiles, int rpt, hipStream_t s, bool gather) {
+  if (ntiles > 0) GSH_SSOR_R(rpt, k_ssor_setup, ntiles, s, sys, gather ? 1 : 0);
 }
 template <class VT>
 void launch_ssor_residual(const SsorSysT<VT>* sys, int ntiles, int rpt, hipStream_t s) {
@@ -534,7 +553,7 @@ void launch_ssor_scatter(const SsorSysT<VT>* sys, int ntiles, int rpt, hipStream
 
 #define GSH_SSOR_INSTANTIATE(VT)                                                                       \
   template void launch_ssor_reset<VT>(const SsorSysT<VT>*, double, int, int, hipStream_t);             \
-  template void launch_ssor_setup<VT>(const SsorSysT<VT>*, int, int, hipStream_t);                     \
+  template void launch_ssor_setup<VT>(const SsorSysT<VT>*, int, int, hipStream_t, bool);               \
   template void launch_ssor_residual<VT>(const SsorSysT<VT>*, int, int, hipStream_t);                  \
   template void launch_ssor_r0<VT>(const SsorSysT<VT>*, int, int, int, int, hipStream_t);              \
   template void launch_ssor_bwd<VT>(const SsorSysT<VT>*, int, int, int, int, int, hipStream_t, hipEvent_t, \
