@@ -124,13 +124,14 @@ __device__ __forceinline__ double signed_area(double p1x, double p1y, double p2x
 // Node ids and coordinates of one element, as the walk leaves them for the interpolation.
 struct ElemGeom {
   int n0, n1, n2;
+  int s0, s1, s2;  // sides opposite the corners
   double x0, y0, x1, y1, x2, y2;
 };
 
 __device__ __forceinline__ ElemGeom elem_geom(const DMesh& m, int e) {
-  const int4 r = m.elem[2 * e];
+  const int4 r = m.elem[2 * e], q = m.elem[2 * e + 1];
   const double2 p0 = m.nrec[2 * r.x], p1 = m.nrec[2 * r.y], p2 = m.nrec[2 * r.z];
-  return {r.x, r.y, r.z, p0.x, p0.y, p1.x, p1.y, p2.x, p2.y};
+  return {r.x, r.y, r.z, r.w, q.x, q.y, p0.x, p0.y, p1.x, p1.y, p2.x, p2.y};
 }
 
 // A node's elevation, depth, total depth H = eta + h, bottom level and dry flag, in registers.
@@ -220,17 +221,16 @@ __device__ void interp_velocity(const DMesh& m, const double* sig, double h0, in
 
 // Leaded celestial walk (meshdata.cu:85-127) with Element::computePredicates (meshdata.cu:651-746).
 // Corner j of a triangle is getNodeInOrder(j) = nodes[(j+1)%3]; walk side j is element side j.
-// On return, geo holds the geometry of the final element cur.
+// geo_valid: geo already holds element cur (the previous walk's final element), so the first
+// step reads nothing. On return, geo holds the geometry of the final element cur.
 __device__ bool celestial_walk(const DMesh& m, int& cur, double x0x, double x0y, double xtx, double xty,
-                               int& last_cross, int& status, ElemGeom& geo) {
+                               int& last_cross, int& status, ElemGeom& geo, bool geo_valid) {
   double fac = 0.00001;
   for (int iter = 0; iter < 40; ++iter) {
-    const int4 r0 = m.elem[2 * cur], r1 = m.elem[2 * cur + 1];
-    const double2 P0 = m.nrec[2 * r0.y], P1 = m.nrec[2 * r0.z], P2 = m.nrec[2 * r0.x];
-    const double C0x = P0.x, C0y = P0.y;
-    const double C1x = P1.x, C1y = P1.y;
-    const double C2x = P2.x, C2y = P2.y;
-    geo = {r0.x, r0.y, r0.z, C2x, C2y, C0x, C0y, C1x, C1y};
+    if (iter > 0 || !geo_valid) geo = elem_geom(m, cur);
+    const double C0x = geo.x1, C0y = geo.y1;  // corner j is node (j + 1) % 3
+    const double C1x = geo.x2, C1y = geo.y2;
+    const double C2x = geo.x0, C2y = geo.y0;
     const double vx = xtx - x0x, vy = xty - x0y;
     const double d0 = vx * (C0x - x0x) + vy * (C0y - x0y);
     const double d1 = vx * (C1x - x0x) + vy * (C1y - x0y);
@@ -255,7 +255,7 @@ __device__ bool celestial_walk(const DMesh& m, int& cur, double x0x, double x0y,
     const bool v1 = !left1 && (!obt1 || bis1);
     const bool v2 = !left2 && (!obt2 || bis2);
     if (!(v0 || v1 || v2)) return true;
-    const int side = v0 ? r0.w : (v1 ? r1.x : r1.y);
+    const int side = v0 ? geo.s0 : (v1 ? geo.s1 : geo.s2);
     last_cross = side;
     const int4 sr = m.side[side];
     const int nxt = (sr.z == cur) ? sr.w : sr.z;
@@ -280,13 +280,15 @@ __device__ void btrack(const DMesh& m, const double* sig, const StepArgs& a, int
   double xtx = -1.0e25, xty = -1.0e25, xtz = -1.0e25;
   int iter = 0;
   ElemGeom geo;  // set by every walk before its first use
+  bool geo_valid = false;
   while (true) {
     const double dtb = fmin(dtbk, time_rm);
     xtx = x0x - dtb * uix;
     xty = x0y - dtb * uiy;
     xtz = x0z - dtb * uiz;
     int last = -1;
-    const bool found = celestial_walk(m, cur, x0x, x0y, xtx, xty, last, status, geo);
+    const bool found = celestial_walk(m, cur, x0x, x0y, xtx, xty, last, status, geo, geo_valid);
+    geo_valid = true;
     if (t3 <= time_rm) {
       xtx = x0x;
       xty = x0y;
