@@ -132,6 +132,26 @@ __global__ void k_unpack_uvw(const double* __restrict__ in, double* __restrict__
 using namespace gsh;
 
 // ====================================================================== context
+// Stream-ordered temporaries of one call, freed on the stream when the scope ends, on the error
+// path as well.
+struct StreamBuffers {
+  hipStream_t st;
+  std::vector<void*> ptrs;
+  explicit StreamBuffers(hipStream_t s) : st(s) {}
+  template <class T>
+  T* alloc(size_t count) {
+    void* p = nullptr;
+    HIPCHK(hipMallocAsync(&p, sizeof(T) * std::max<size_t>(1, count), st));
+    ptrs.push_back(p);
+    return static_cast<T*>(p);
+  }
+  ~StreamBuffers() {
+    for (void* p : ptrs) (void)hipFreeAsync(p, st);
+  }
+  StreamBuffers(const StreamBuffers&) = delete;
+  StreamBuffers& operator=(const StreamBuffers&) = delete;
+};
+
 struct gsh_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -504,8 +524,8 @@ static void run_pcg(gsh_ctx* ctx, const PcgSys<VT>* dsys, int nsys, const TileRe
   if (check_every < 1) check_every = 1;
   if (timeout_s <= 0.0) timeout_s = 60.0;
   PcgState* slots = ctx->pinned_slots(nsys);
-  PcgState* dgather = nullptr;
-  HIPCHK(hipMallocAsync((void**)&dgather, sizeof(PcgState) * 2 * nsys, st));
+  StreamBuffers tmp(st);
+  PcgState* dgather = tmp.alloc<PcgState>(2 * (size_t)nsys);
   launch_pcg_reset<VT>(dsys, nsys, rtol, maxiter, check_every, st, rtols_dev);
   if (dinit_tiles)
     launch_pcg_init<VT>(dsys, dinit_tiles, ninit_tiles, rpt, st);
@@ -574,7 +594,6 @@ static void run_pcg(gsh_ctx* ctx, const PcgSys<VT>* dsys, int nsys, const TileRe
   HIPCHK(hipMemcpyAsync(slots, dgather, sizeof(PcgState) * nsys, hipMemcpyDeviceToHost, st));
   HIPCHK(hipEventRecord(ctx->ev[0], st));
   wait_event(ctx->ev[0], timeout_s, "PCG finish");
-  HIPCHK(hipFreeAsync(dgather, st));
   out.st.assign(slots, slots + nsys);
 }
 
@@ -707,20 +726,15 @@ static void run_mixed(gsh_ctx* ctx, gsh_solver* const* S, int count, double rtol
     h32[i] = s->hs32;
     h32[i].ntiles = s->ntiles;
   }
-  PcgSys<double>* d64 = nullptr;
-  PcgSys<double>* d64s = nullptr;
-  PcgSys<float>* d32s = nullptr;
-  TileRef* dtl = nullptr;
-  double* dred = nullptr;
-  double* drt = nullptr;
   int64_t all_tiles = 0;
   for (int i = 0; i < count; ++i) all_tiles += S[i]->ntiles;
-  HIPCHK(hipMallocAsync((void**)&d64, sizeof(PcgSys<double>) * count, st));
-  HIPCHK(hipMallocAsync((void**)&d64s, sizeof(PcgSys<double>) * count, st));
-  HIPCHK(hipMallocAsync((void**)&d32s, sizeof(PcgSys<float>) * count, st));
-  HIPCHK(hipMallocAsync((void**)&dtl, sizeof(TileRef) * std::max<int64_t>(1, all_tiles), st));
-  HIPCHK(hipMallocAsync((void**)&dred, sizeof(double) * 2 * count, st));
-  HIPCHK(hipMallocAsync((void**)&drt, sizeof(double) * count, st));
+  StreamBuffers tmp(st);
+  PcgSys<double>* d64 = tmp.alloc<PcgSys<double>>(count);
+  PcgSys<double>* d64s = tmp.alloc<PcgSys<double>>(count);
+  PcgSys<float>* d32s = tmp.alloc<PcgSys<float>>(count);
+  TileRef* dtl = tmp.alloc<TileRef>((size_t)all_tiles);
+  double* dred = tmp.alloc<double>(2 * (size_t)count);
+  double* drt = tmp.alloc<double>(count);
   HIPCHK(hipMemcpyAsync(d64, h64.data(), sizeof(PcgSys<double>) * count, hipMemcpyHostToDevice, st));
   std::vector<double> atol2(count, -1.0), rr(count, 0.0), red(2 * count);
   std::vector<int> outer(count, 0), inner(count, 0), state(count, 0);  // 0 running, 1 converged, 2 stopped
@@ -798,7 +812,6 @@ static void run_mixed(gsh_ctx* ctx, gsh_solver* const* S, int count, double rtol
     }
   }
   HIPCHK(hipStreamSynchronize(st));
-  for (void* p : {(void*)d64, (void*)d64s, (void*)d32s, (void*)dtl, (void*)dred, (void*)drt}) (void)hipFree(p);
   iters.assign(count, 0);
   status.assign(count, 0);
   for (int i = 0; i < count; ++i) {
@@ -1172,12 +1185,11 @@ int gsh_mesh_set_field(gsh_mesh* m, int field, const void* host, int64_t count) 
     GSH_REQUIRE(count == f.count, GSH_E_ARG, "field size mismatch");
     hipStream_t st = m->ctx->stream;
     if (field == GSH_F_NODE_UVW) {
-      double* tmp = nullptr;
-      HIPCHK(hipMallocAsync((void**)&tmp, sizeof(double) * count, st));
+      StreamBuffers bufs(st);
+      double* tmp = bufs.alloc<double>((size_t)count);
       HIPCHK(hipMemcpyAsync(tmp, host, sizeof(double) * count, hipMemcpyHostToDevice, st));
       const int64_t recs = count / 3;
       hipLaunchKernelGGL(k_pack_uvw, dim3((unsigned)((recs + 255) / 256)), dim3(256), 0, st, tmp, m->d.uvw, recs);
-      HIPCHK(hipFreeAsync(tmp, st));
     } else {
       HIPCHK(hipMemcpyAsync(f.ptr, host, (size_t)count * f.elem, hipMemcpyHostToDevice, st));
     }
@@ -1202,12 +1214,11 @@ int gsh_mesh_get_field(const gsh_mesh* m, int field, void* host, int64_t count) 
       return GSH_OK;
     }
     if (field == GSH_F_NODE_UVW) {
-      double* tmp = nullptr;
-      HIPCHK(hipMallocAsync((void**)&tmp, sizeof(double) * count, st));
+      StreamBuffers bufs(st);
+      double* tmp = bufs.alloc<double>((size_t)count);
       const int64_t recs = count / 3;
       hipLaunchKernelGGL(k_unpack_uvw, dim3((unsigned)((recs + 255) / 256)), dim3(256), 0, st, m->d.uvw, tmp, recs);
       HIPCHK(hipMemcpyAsync(host, tmp, sizeof(double) * count, hipMemcpyDeviceToHost, st));
-      HIPCHK(hipFreeAsync(tmp, st));
     } else if (field == GSH_F_WET) {
       int8_t* h = static_cast<int8_t*>(host);
       HIPCHK(hipMemcpyAsync(h, m->d.node_wet, m->d.n, hipMemcpyDeviceToHost, st));
@@ -1268,12 +1279,11 @@ int gsh_backtrack_list(gsh_mesh* m, const gsh_step_params* p, const int32_t* sid
                   "start point out of range");
     if (count == 0) return GSH_OK;
     hipStream_t st = m->ctx->stream;
-    int32_t* buf = nullptr;
-    HIPCHK(hipMallocAsync((void**)&buf, 8 * (size_t)count, st));
+    StreamBuffers bufs(st);
+    int32_t* buf = bufs.alloc<int32_t>(2 * (size_t)count);
     HIPCHK(hipMemcpyAsync(buf, sides, 4 * (size_t)count, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(buf + count, levels, 4 * (size_t)count, hipMemcpyHostToDevice, st));
     launch_backtrack(m->d, step_args(p), buf, buf + count, count, st);
-    HIPCHK(hipFreeAsync(buf, st));
     HIPCHK(hipStreamSynchronize(st));
   })
 }
@@ -1683,16 +1693,13 @@ int gsh_batch_solve(gsh_solver* const* solvers, int count, double rtol, int maxi
       }
       return GSH_OK;
     }
-    PcgSys<double>* dsys = nullptr;
-    TileRef* dt = nullptr;
-    HIPCHK(hipMallocAsync((void**)&dsys, sizeof(PcgSys<double>) * count, st));
-    HIPCHK(hipMallocAsync((void**)&dt, sizeof(TileRef) * tiles.size(), st));
+    StreamBuffers tmp(st);
+    PcgSys<double>* dsys = tmp.alloc<PcgSys<double>>(count);
+    TileRef* dt = tmp.alloc<TileRef>(tiles.size());
     HIPCHK(hipMemcpyAsync(dsys, hs.data(), sizeof(PcgSys<double>) * count, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(dt, tiles.data(), sizeof(TileRef) * tiles.size(), hipMemcpyHostToDevice, st));
     PcgOutcome o;
     run_pcg<double>(ctx, dsys, count, dt, (int)tiles.size(), rpt, rtol, maxiter, check_every, timeout_s, o);
-    HIPCHK(hipFreeAsync(dsys, st));
-    HIPCHK(hipFreeAsync(dt, st));
     HIPCHK(hipStreamSynchronize(st));
     for (int i = 0; i < count; ++i) {
       gsh_solver* s = solvers[i];
