@@ -517,12 +517,18 @@ def main() -> int:
                                       nthreads=threads)
             spent += time.perf_counter() - a
             reps += 1
+        # the same solve on one core (one solve)
+        a = time.perf_counter()
+        x1, it1, _, _ = osr.solve(val, bvec, rtol=rtol, maxiter=maxiter, check_every=check_every, nthreads=1)
+        one_core = time.perf_counter() - a
         osr.close()
         cpu = {"value": round(reps / spent, 4), "unit": "solves/s", "cores": threads, "kind": "port",
                "sample": f"{reps} full FP64 SSOR PCG solves of the same {n}-node system ({ito} iterations each), "
                          f"oracle C restatement, OpenMP over rows, {cpu_model()}",
                "seconds": round(spent, 3),
-               "gpu_bitwise_equal": bool(np.array_equal(xo, x_ssor)), "iters_equal": bool(ito == it_ssor)}
+               "gpu_bitwise_equal": bool(np.array_equal(xo, x_ssor)), "iters_equal": bool(ito == it_ssor),
+               "single_core": {"value": round(1.0 / one_core, 4), "unit": "solves/s", "cores": 1,
+                               "sample": "1 solve", "bitwise_equal": bool(np.array_equal(x1, x_ssor))}}
         log(f"[bench] cpu baseline: {cpu}")
 
     mixed4m = None
