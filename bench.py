@@ -121,12 +121,29 @@ def batched_leg(ctx, L, cases, core, members, nx, ny, rtol, mixed_rtol, check_ev
     t_bmx, it_bmx = bat(smx, mixed_rtol)
     err = max(float(np.linalg.norm(s.get_solution() - x) / np.linalg.norm(x)) for s, x in zip(smx, x64))
     t_smx, _ = seq(smx, mixed_rtol)
+    # plain FP64 multicolour SSOR (the 1M headline's method), one member after another, on the
+    # first 8 members (the per-solve rate does not depend on how many are timed)
+    k_ssor = min(8, members)
+    sss = []
+    for a in s64[:k_ssor]:
+        d = core.Solver(mesh, L.PREC_FP64, precond=L.PRECOND_SSOR)
+        d.set_tuning(2)
+        d.set_values(a.get_values())
+        d.set_rhs(a.get_rhs())
+        sss.append(d)
+    seq(sss[:1], rtol)  # warm
+    t_ssor, _ = seq(sss, rtol)
+    ssor_rate = k_ssor / t_ssor
+    for d in sss:
+        d.close()
     out = {"workload": f"{members} members on {spec.name} (n={mesh.n}), per-member bathymetry and RHS",
            "plain_fp64_solves_per_s": round(members / t_plain, 3),
+           "plain_fp64_ssor_solves_per_s": round(ssor_rate, 3),
            "batched_fp64_solves_per_s": round(members / t_b64, 3),
            "mixed_solves_per_s": round(members / t_smx, 3),
            "batched_mixed_solves_per_s": round(members / t_bmx, 3),
            "batched_mixed_vs_plain": round(t_plain / t_bmx, 3),
+           "batched_mixed_vs_plain_ssor": round(members / t_bmx / ssor_rate, 3),
            "fp64_iters_mean": float(np.mean(it_plain)), "mixed_inner_iters_mean": float(np.mean(it_bmx)),
            "batched_fp64_bitwise_equal_to_plain": bool(same and it_b64 == it_plain),
            "mixed_rtol": mixed_rtol, "mixed_max_rel_l2_vs_fp64": err}
@@ -227,9 +244,24 @@ def many_meshes_leg(ctx, L, cases, core, count, rtol, mixed_rtol, check_every):
     run(smx[:4], mixed_rtol, True)
     t_bmx = run(smx, mixed_rtol, True)
     err = max(float(np.linalg.norm(s.get_solution() - x) / np.linalg.norm(x)) for s, x in zip(smx, x64))
+    # plain FP64 multicolour SSOR (the 1M headline's method), one mesh after another
+    sss = []
+    for m, a in zip(meshes, s64):
+        d = core.Solver(m, L.PREC_FP64, precond=L.PRECOND_SSOR)
+        d.set_tuning(2)
+        d.set_values(a.get_values())
+        d.set_rhs(a.get_rhs())
+        sss.append(d)
+    run(sss[:4], rtol, False)
+    t_ssor = run(sss, rtol, False)
+    for d in sss:
+        d.close()
     out = {"workload": f"{count} meshes, n log-uniform in [5e3, 2e5] (total {sum(m.n for m in meshes)} nodes)",
-           "plain_fp64_solves_per_s": round(count / t_plain, 3), "batched_fp64_solves_per_s": round(count / t_b64, 3),
+           "plain_fp64_solves_per_s": round(count / t_plain, 3),
+           "plain_fp64_ssor_solves_per_s": round(count / t_ssor, 3),
+           "batched_fp64_solves_per_s": round(count / t_b64, 3),
            "batched_mixed_solves_per_s": round(count / t_bmx, 3), "batched_mixed_vs_plain": round(t_plain / t_bmx, 3),
+           "batched_mixed_vs_plain_ssor": round(t_ssor / t_bmx, 3),
            "batched_fp64_bitwise_equal_to_plain": bool(same), "mixed_rtol": mixed_rtol,
            "mixed_max_rel_l2_vs_fp64": err}
     for s in s64 + smx:
@@ -283,7 +315,27 @@ def served_leg(ctx, L, cases, core, n_req, rates, rtol, mixed_rtol, check_every)
     plain_rate = n_req / t_plain
     for s in plain:
         s.close()
+    # the same requests with plain FP64 multicolour SSOR solves (the 1M headline's method)
+    plain = [core.Solver(m, L.PREC_FP64, precond=L.PRECOND_SSOR) for m in pool]
+    for s in plain:
+        s.set_tuning(2)
+    for rep_ in range(2):  # the first pass warms the solvers
+        ctx.synchronize()
+        t = time.perf_counter()
+        for j, k in enumerate(which):
+            m = pool[k]
+            m.backtrack(pd)
+            m.rhs_producers(pd)
+            plain[k].assemble(pd)
+            plain[k].reset_history()
+            plain[k].solve(rhs[j], rtol, 20000, check_every, want_x=False)
+        ctx.synchronize()
+        t_plain_ssor = time.perf_counter() - t
+    plain_ssor_rate = n_req / t_plain_ssor
+    for s in plain:
+        s.close()
     res = {"workload": f"{n_req} requests over 6 meshes, n = {sizes}", "plain_fp64_solves_per_s": round(plain_rate, 3),
+           "plain_fp64_ssor_solves_per_s": round(plain_ssor_rate, 3),
            "mixed_rtol": mixed_rtol, "rates": []}
     srv, ids, warm = None, None, True
     for mult in (rates[-1],) + tuple(rates):  # the first pass warms the server's solver pools
