@@ -121,20 +121,29 @@ def batched_leg(ctx, L, cases, core, members, nx, ny, rtol, mixed_rtol, check_ev
     t_bmx, it_bmx = bat(smx, mixed_rtol)
     err = max(float(np.linalg.norm(s.get_solution() - x) / np.linalg.norm(x)) for s, x in zip(smx, x64))
     t_smx, _ = seq(smx, mixed_rtol)
-    # plain FP64 multicolour SSOR (the 1M headline's method), one member after another, on the
-    # first 8 members (the per-solve rate does not depend on how many are timed)
+    # multicolour SSOR (the 1M headline's method): plain FP64 one member after another on the
+    # first 8 members (the per-solve rate does not depend on how many are timed), then all members
+    # batched in FP64 and in mixed precision (FP32 SSOR inner solves)
     k_ssor = min(8, members)
-    sss = []
-    for a in s64[:k_ssor]:
-        d = core.Solver(mesh, L.PREC_FP64, precond=L.PRECOND_SSOR)
-        d.set_tuning(2)
-        d.set_values(a.get_values())
-        d.set_rhs(a.get_rhs())
-        sss.append(d)
+    sss, ssm = [], []
+    for a in s64:
+        for prec, lst in ((L.PREC_FP64, sss), (L.PREC_MIXED, ssm)):
+            d = core.Solver(mesh, prec, precond=L.PRECOND_SSOR)
+            d.set_tuning(2)
+            d.set_values(a.get_values())
+            d.set_rhs(a.get_rhs())
+            lst.append(d)
     seq(sss[:1], rtol)  # warm
-    t_ssor, _ = seq(sss, rtol)
+    t_ssor, it_ssor = seq(sss[:k_ssor], rtol)
     ssor_rate = k_ssor / t_ssor
-    for d in sss:
+    xs = [d.get_solution() for d in sss[:k_ssor]]
+    bat(sss[:2], rtol)  # warm
+    t_bss, it_bss = bat(sss, rtol)
+    same_ssor = all(np.array_equal(d.get_solution(), x) for d, x in zip(sss, xs)) and it_bss[:k_ssor] == it_ssor
+    bat(ssm[:2], mixed_rtol)  # warm
+    t_bsm, it_bsm = bat(ssm, mixed_rtol)
+    err_ssm = max(float(np.linalg.norm(d.get_solution() - x) / np.linalg.norm(x)) for d, x in zip(ssm, x64))
+    for d in sss + ssm:
         d.close()
     out = {"workload": f"{members} members on {spec.name} (n={mesh.n}), per-member bathymetry and RHS",
            "plain_fp64_solves_per_s": round(members / t_plain, 3),
@@ -144,9 +153,14 @@ def batched_leg(ctx, L, cases, core, members, nx, ny, rtol, mixed_rtol, check_ev
            "batched_mixed_solves_per_s": round(members / t_bmx, 3),
            "batched_mixed_vs_plain": round(t_plain / t_bmx, 3),
            "batched_mixed_vs_plain_ssor": round(members / t_bmx / ssor_rate, 3),
+           "batched_fp64_ssor_solves_per_s": round(members / t_bss, 3),
+           "batched_mixed_ssor_solves_per_s": round(members / t_bsm, 3),
+           "batched_mixed_ssor_vs_plain_ssor": round(members / t_bsm / ssor_rate, 3),
            "fp64_iters_mean": float(np.mean(it_plain)), "mixed_inner_iters_mean": float(np.mean(it_bmx)),
+           "ssor_iters_mean": float(np.mean(it_bss)), "mixed_ssor_inner_iters_mean": float(np.mean(it_bsm)),
            "batched_fp64_bitwise_equal_to_plain": bool(same and it_b64 == it_plain),
-           "mixed_rtol": mixed_rtol, "mixed_max_rel_l2_vs_fp64": err}
+           "batched_fp64_ssor_bitwise_equal_to_plain": bool(same_ssor),
+           "mixed_rtol": mixed_rtol, "mixed_max_rel_l2_vs_fp64": err, "mixed_ssor_max_rel_l2_vs_fp64": err_ssm}
     for s in s64 + smx:
         s.close()
     mesh.close()
@@ -244,17 +258,26 @@ def many_meshes_leg(ctx, L, cases, core, count, rtol, mixed_rtol, check_every):
     run(smx[:4], mixed_rtol, True)
     t_bmx = run(smx, mixed_rtol, True)
     err = max(float(np.linalg.norm(s.get_solution() - x) / np.linalg.norm(x)) for s, x in zip(smx, x64))
-    # plain FP64 multicolour SSOR (the 1M headline's method), one mesh after another
-    sss = []
+    # multicolour SSOR (the 1M headline's method): plain FP64 one mesh after another, then all
+    # meshes batched in FP64 and in mixed precision (FP32 SSOR inner solves)
+    sss, ssm = [], []
     for m, a in zip(meshes, s64):
-        d = core.Solver(m, L.PREC_FP64, precond=L.PRECOND_SSOR)
-        d.set_tuning(2)
-        d.set_values(a.get_values())
-        d.set_rhs(a.get_rhs())
-        sss.append(d)
+        for prec, lst in ((L.PREC_FP64, sss), (L.PREC_MIXED, ssm)):
+            d = core.Solver(m, prec, precond=L.PRECOND_SSOR)
+            d.set_tuning(2)
+            d.set_values(a.get_values())
+            d.set_rhs(a.get_rhs())
+            lst.append(d)
     run(sss[:4], rtol, False)
     t_ssor = run(sss, rtol, False)
-    for d in sss:
+    xs = [d.get_solution() for d in sss]
+    run(sss[:4], rtol, True)
+    t_bss = run(sss, rtol, True)
+    same_ssor = all(np.array_equal(d.get_solution(), x) for d, x in zip(sss, xs))
+    run(ssm[:4], mixed_rtol, True)
+    t_bsm = run(ssm, mixed_rtol, True)
+    err_ssm = max(float(np.linalg.norm(d.get_solution() - x) / np.linalg.norm(x)) for d, x in zip(ssm, x64))
+    for d in sss + ssm:
         d.close()
     out = {"workload": f"{count} meshes, n log-uniform in [5e3, 2e5] (total {sum(m.n for m in meshes)} nodes)",
            "plain_fp64_solves_per_s": round(count / t_plain, 3),
@@ -262,8 +285,12 @@ def many_meshes_leg(ctx, L, cases, core, count, rtol, mixed_rtol, check_every):
            "batched_fp64_solves_per_s": round(count / t_b64, 3),
            "batched_mixed_solves_per_s": round(count / t_bmx, 3), "batched_mixed_vs_plain": round(t_plain / t_bmx, 3),
            "batched_mixed_vs_plain_ssor": round(t_ssor / t_bmx, 3),
-           "batched_fp64_bitwise_equal_to_plain": bool(same), "mixed_rtol": mixed_rtol,
-           "mixed_max_rel_l2_vs_fp64": err}
+           "batched_fp64_ssor_solves_per_s": round(count / t_bss, 3),
+           "batched_mixed_ssor_solves_per_s": round(count / t_bsm, 3),
+           "batched_mixed_ssor_vs_plain_ssor": round(t_ssor / t_bsm, 3),
+           "batched_fp64_bitwise_equal_to_plain": bool(same),
+           "batched_fp64_ssor_bitwise_equal_to_plain": bool(same_ssor), "mixed_rtol": mixed_rtol,
+           "mixed_max_rel_l2_vs_fp64": err, "mixed_ssor_max_rel_l2_vs_fp64": err_ssm}
     for s in s64 + smx:
         s.close()
     for m in meshes:

@@ -703,6 +703,134 @@ static PcgState run_ssor(gsh_solver* s, double rtol, int maxiter, int check_ever
   return slots[0];
 }
 
+// Multicolour SSOR PCG of `count` systems in lock-step: the per-system set-up and the final
+// x = (D~ + U)^-1 y as in run_ssor, the sweeps of all systems batched into one launch per colour
+// step over a table of (system, tile, colour, flags) entries (a system with fewer colours sits
+// out the steps beyond its own). Every system stops at its own first passing test; its arithmetic
+// does not depend on the others, so each result equals its solo solve. The VT descriptors of the
+// solvers must be synchronised first; rtols are per system; gather[i] = 0 keeps system i's ELL
+// values from its previous set-up.
+constexpr int kSsorFirst = 1, kSsorMid = 2, kSsorLast = 4, kSsorInit = 8, kSsorLead = 16;
+
+template <class VT>
+static void run_ssor_batch(gsh_solver* const* S, int count, const std::vector<double>& rtols, int maxiter,
+                           int check_every, double timeout_s, const std::vector<char>& gather,
+                           std::vector<PcgState>& out) {
+  gsh_ctx* ctx = S[0]->ctx;
+  hipStream_t st = ctx->stream;
+  if (check_every < 1) check_every = 1;
+  if (timeout_s <= 0.0) timeout_s = 60.0;
+  const int rpt = S[0]->ssor->rpt;
+  int maxC = 0;
+  std::vector<SsorSysT<VT>> hsys(count);
+  for (int i = 0; i < count; ++i) {
+    SsorState& T = *S[i]->ssor;
+    GSH_REQUIRE(T.rpt == rpt, GSH_E_ARG, "batched SSOR solvers need the same rows per thread");
+    hsys[i] = ssor_host<VT>(T);
+    maxC = std::max(maxC, T.L.ncol);
+  }
+  // launch tables: backward steps 0 .. maxC-1, forward steps 0 .. maxC-2, then every tile
+  std::vector<SsorRef> refs;
+  std::vector<int> boff(maxC + 1), foff(std::max(1, maxC));
+  auto add_colour = [&](int i, int c, int flags) {
+    const SsorState& T = *S[i]->ssor;
+    for (int t = 0; t < T.ntc[c]; ++t) refs.push_back({i, T.tile0[c] + t, c, flags | (t == 0 ? kSsorLead : 0)});
+  };
+  for (int step = 0; step < maxC; ++step) {
+    boff[step] = (int)refs.size();
+    for (int i = 0; i < count; ++i) {
+      const int C = S[i]->ssor->L.ncol;
+      if (step >= C) continue;
+      const int c = C - 1 - step;
+      add_colour(i, c, (step == 0 ? kSsorFirst : 0) | (c == 0 ? kSsorMid : 0) | (C == 1 ? kSsorLast : 0));
+    }
+  }
+  boff[maxC] = (int)refs.size();
+  for (int step = 0; step + 1 < maxC; ++step) {
+    foff[step] = (int)refs.size();
+    for (int i = 0; i < count; ++i) {
+      const int C = S[i]->ssor->L.ncol, c = step + 1;
+      if (c < C) add_colour(i, c, c == C - 1 ? kSsorLast : 0);
+    }
+  }
+  foff[std::max(0, maxC - 1)] = (int)refs.size();
+  const int aoff = (int)refs.size();
+  for (int i = 0; i < count; ++i)
+    for (int t = 0; t < S[i]->ssor->ntiles; ++t) refs.push_back({i, t, 0, 0});
+  const int nall = (int)refs.size() - aoff;
+  for (const SsorRef& r : refs)  // every entry names a tile of its own system
+    GSH_REQUIRE(r.sys >= 0 && r.sys < count && r.tile >= 0 && r.tile < S[r.sys]->ssor->ntiles, GSH_E_STATE,
+                "bad batched SSOR table");
+  StreamBuffers tmp(st);
+  SsorSysT<VT>* dsys = tmp.alloc<SsorSysT<VT>>(count);
+  SsorRef* drefs = tmp.alloc<SsorRef>(refs.size());
+  PcgState* dgather = tmp.alloc<PcgState>(2 * (size_t)count);
+  HIPCHK(hipMemcpyAsync(dsys, hsys.data(), sizeof(SsorSysT<VT>) * count, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(drefs, refs.data(), sizeof(SsorRef) * refs.size(), hipMemcpyHostToDevice, st));
+  // per-system set-up: state, ELL values, r0 = (D~ + L)^-1 (b - A x0), y0
+  for (int i = 0; i < count; ++i) {
+    SsorState& T = *S[i]->ssor;
+    const SsorSysT<VT>* ds = ssor_dev<VT>(T);
+    launch_ssor_reset(ds, rtols[i], maxiter, check_every, st);
+    launch_ssor_setup(ds, T.ntiles, rpt, st, gather[i] != 0);
+    launch_ssor_residual(ds, T.ntiles, rpt, st);
+    for (int c = 0; c < T.L.ncol; ++c) launch_ssor_r0(ds, T.tile0[c], T.ntc[c], c, rpt, st);
+  }
+  auto sweeps = [&](int or_flags) {
+    for (int step = 0; step < maxC; ++step)
+      launch_ssor_bwd_batch(dsys, drefs + boff[step], boff[step + 1] - boff[step], or_flags, rpt, st);
+    for (int step = 0; step + 1 < maxC; ++step)
+      launch_ssor_fwd_batch(dsys, drefs + foff[step], foff[step + 1] - foff[step], or_flags, rpt, st);
+  };
+  sweeps(kSsorInit);  // z0, s0 and the partials of iteration 0
+  HIPCHK(hipGetLastError());
+  PcgState* slots = ctx->pinned_slots(count);
+  int issued = 0, window = 0;
+  std::deque<int> inflight;
+  bool all_done = false;
+  const auto t0 = Clock::now();
+  auto check_slot = [&](int slot) {
+    wait_event(ctx->ev[slot], timeout_s, "batched SSOR PCG window");
+    bool done = true;
+    for (int i = 0; i < count; ++i) done = done && (slots[slot * count + i].done >= 0);
+    return done;
+  };
+  while (issued < maxiter && !all_done) {
+    const int w = std::min(check_every, maxiter - issued);
+    for (int it = 0; it < w; ++it) sweeps(0);
+    launch_ssor_true_rr_batch(dsys, drefs + aoff, nall, rpt, st);
+    launch_ssor_check_batch(dsys, count, st);
+    issued += w;
+    const int slot = window & 1;
+    if (inflight.size() == 2) {
+      const int old = inflight.front();
+      inflight.pop_front();
+      if (check_slot(old)) all_done = true;
+    }
+    launch_ssor_gather_states(dsys, count, dgather + slot * count, st);
+    HIPCHK(hipMemcpyAsync(slots + slot * count, dgather + slot * count, sizeof(PcgState) * count,
+                          hipMemcpyDeviceToHost, st));
+    HIPCHK(hipEventRecord(ctx->ev[slot], st));
+    inflight.push_back(slot);
+    ++window;
+    if (std::chrono::duration<double>(Clock::now() - t0).count() > timeout_s)
+      throw Error(GSH_E_TIMEOUT, "batched SSOR PCG: time limit exceeded while issuing iterations");
+    HIPCHK(hipGetLastError());
+  }
+  for (int i = 0; i < count; ++i) {
+    SsorState& T = *S[i]->ssor;
+    const SsorSysT<VT>* ds = ssor_dev<VT>(T);
+    for (int c = T.L.ncol - 1; c >= 0; --c) launch_ssor_xsolve(ds, T.tile0[c], T.ntc[c], c, rpt, st);
+    launch_ssor_scatter(ds, T.ntiles, rpt, st);
+  }
+  launch_ssor_gather_states(dsys, count, dgather, st);
+  HIPCHK(hipMemcpyAsync(slots, dgather, sizeof(PcgState) * count, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipEventRecord(ctx->ev[0], st));
+  wait_event(ctx->ev[0], timeout_s, "batched SSOR PCG finish");
+  HIPCHK(hipGetLastError());
+  out.assign(slots, slots + count);
+}
+
 // Mixed precision for `count` systems in lock-step: FP64 true residual and refinement around FP32
 // PCG inner solves (FP64 dot products). Every system keeps its own outer state: its inner
 // tolerance is max(inner_rtol, 0.5 sqrt(atol^2 / r.r)) from its own residual, each inner solve is
@@ -739,7 +867,7 @@ static void run_mixed(gsh_ctx* ctx, gsh_solver* const* S, int count, double rtol
   std::vector<double> atol2(count, -1.0), rr(count, 0.0), red(2 * count);
   std::vector<int> outer(count, 0), inner(count, 0), state(count, 0);  // 0 running, 1 converged, 2 stopped
   const int max_outer = 30;
-  bool ssor_gathered = false;  // SSOR inner solves: the ELL values are gathered by the first one
+  std::vector<char> ssor_gathered(count, 0);  // SSOR inner solves: gathered by each system's first
   for (int round = 0; round < max_outer; ++round) {
     // FP64 true residual of every running system
     std::vector<TileRef> tl;
@@ -795,11 +923,17 @@ static void run_mixed(gsh_ctx* ctx, gsh_solver* const* S, int count, double rtol
     HIPCHK(hipMemcpyAsync(drt, rts.data(), sizeof(double) * ns, hipMemcpyHostToDevice, st));
     PcgOutcome o;
     if (S[0]->precond == GSH_PRECOND_SSOR) {
-      // one system (batches take Jacobi solvers): FP32 SSOR PCG of A d = r, b = the FP64 residual
-      gsh_solver* s = S[sub[0]];
-      ssor_sync_desc32(s, s->r64);
-      o.st.assign(1, run_ssor<float>(s, rts[0], maxiter, check_every, timeout_s, nullptr, false, !ssor_gathered));
-      ssor_gathered = true;
+      // FP32 SSOR PCG of A d = r for the subset, b = each system's FP64 residual; the first inner
+      // solve of a system gathers its FP32 matrix, the later ones reuse it
+      std::vector<gsh_solver*> ss(ns);
+      std::vector<char> gath(ns);
+      for (int j = 0; j < ns; ++j) {
+        ss[j] = S[sub[j]];
+        ssor_sync_desc32(ss[j], ss[j]->r64);
+        gath[j] = ssor_gathered[sub[j]] ? 0 : 1;
+        ssor_gathered[sub[j]] = 1;
+      }
+      run_ssor_batch<float>(ss.data(), ns, rts, maxiter, check_every, timeout_s, gath, o.st);
     } else {
       run_pcg<float>(ctx, d32s, ns, dtl, (int)stl.size(), rpt, rtol, maxiter, check_every, timeout_s, o, nullptr, -1,
                      nullptr, drt);
@@ -1656,7 +1790,10 @@ int gsh_batch_solve(gsh_solver* const* solvers, int count, double rtol, int maxi
       GSH_REQUIRE(solvers[i] && solvers[i]->ctx == ctx, GSH_E_ARG, "all solvers must share one context");
       GSH_REQUIRE(solvers[i]->precision == solvers[0]->precision, GSH_E_ARG,
                   "all solvers of a batch need the same precision");
-      GSH_REQUIRE(solvers[i]->precond == GSH_PRECOND_JACOBI, GSH_E_ARG, "batched solve supports Jacobi solvers");
+      GSH_REQUIRE(solvers[i]->precond == solvers[0]->precond, GSH_E_ARG,
+                  "all solvers of a batch need the same preconditioner");
+      GSH_REQUIRE(solvers[i]->precond != GSH_PRECOND_SSOR || solvers[i]->rpt == solvers[0]->rpt, GSH_E_ARG,
+                  "batched SSOR solvers need the same rows per thread (gsh_solver_set_tuning)");
       for (int j = 0; j < i; ++j) GSH_REQUIRE(solvers[j] != solvers[i], GSH_E_ARG, "a solver appears twice");
     }
     ctx->set();
@@ -1690,6 +1827,24 @@ int gsh_batch_solve(gsh_solver* const* solvers, int count, double rtol, int maxi
         if (iters_out) iters_out[i] = its[i];
         solvers[i]->last.true_rel = -1.0;
         solver_roll(solvers[i]);
+      }
+      return GSH_OK;
+    }
+    if (solvers[0]->precond == GSH_PRECOND_SSOR) {
+      for (int i = 0; i < count; ++i) ssor_sync_desc(solvers[i]);
+      std::vector<PcgState> out;
+      run_ssor_batch<double>(solvers, count, std::vector<double>(count, rtol), maxiter, check_every, timeout_s,
+                             std::vector<char>(count, 1), out);
+      for (int i = 0; i < count; ++i) {
+        gsh_solver* s = solvers[i];
+        const int it = iters_of(out[i], maxiter);
+        if (iters_out) iters_out[i] = it;
+        s->last.iters = it;
+        s->last.outer = 1;
+        s->last.rr = out[i].rr_last;
+        s->last.atol2 = out[i].atol2;
+        s->last.true_rel = -1.0;
+        solver_roll(s);
       }
       return GSH_OK;
     }

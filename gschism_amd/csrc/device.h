@@ -150,6 +150,13 @@ struct SsorSysT {
 };
 using SsorSys = SsorSysT<double>;
 
+// Batched SSOR launch table entry: the workgroup's system, tile, colour and sweep flags (1 first
+// sweep of the iteration, 2 colour 0, 4 last sweep, 8 set-up pass, 16 the system's lead
+// workgroup, which writes its scalar state).
+struct SsorRef {
+  int32_t sys, tile, c, flags;
+};
+
 // Launch table entry: which system and which tile a workgroup works on.
 struct TileRef {
   int32_t sys, tile;
@@ -208,6 +215,20 @@ template <class VT>
 void launch_ssor_xsolve(const SsorSysT<VT>* sys, int tile0, int ntc, int c, int rpt, hipStream_t s);
 template <class VT>
 void launch_ssor_scatter(const SsorSysT<VT>* sys, int ntiles, int rpt, hipStream_t s);
+// Batched sweeps over a table of nrefs entries (sys is an array of descriptors), the batched
+// stopping test (one workgroup per system) and the states of count systems gathered into out.
+template <class VT>
+void launch_ssor_bwd_batch(const SsorSysT<VT>* sys, const SsorRef* refs, int nrefs, int or_flags, int rpt,
+                           hipStream_t s);
+template <class VT>
+void launch_ssor_fwd_batch(const SsorSysT<VT>* sys, const SsorRef* refs, int nrefs, int or_flags, int rpt,
+                           hipStream_t s);
+template <class VT>
+void launch_ssor_true_rr_batch(const SsorSysT<VT>* sys, const SsorRef* refs, int nrefs, int rpt, hipStream_t s);
+template <class VT>
+void launch_ssor_check_batch(const SsorSysT<VT>* sys, int count, hipStream_t s);
+template <class VT>
+void launch_ssor_gather_states(const SsorSysT<VT>* sys, int count, PcgState* out, hipStream_t s);
 
 // Mixed precision helpers.
 void launch_cast_values(const double* in, float* out, int64_t count, hipStream_t s);

@@ -240,21 +240,20 @@ __global__ void __launch_bounds__(256) k_ssor_r0(const SsorSysT<VT>* __restrict_
 }
 
 // ------------------------------------------------------------------ backward sweep
-constexpr int kFirst = 1, kMid = 2, kLast = 4, kInit = 8;
+constexpr int kFirst = 1, kMid = 2, kLast = 4, kInit = 8, kLead = 16;
 
 // One colour of the backward sweep. Outside the set-up pass the first sweep of an iteration
 // (workgroup 0) turns the gamma and delta partials into alpha and beta and keeps them with
 // gamma and the iteration index; every sweep then applies p = z + beta p, q = s + beta q,
 // y += alpha p, r -= alpha q to its rows before t = (z - U t) / D~ with the new z = D~ r. The
 // sweep of colour 0 (no lower colours) also forms u and s = t + u for its rows.
+// lead: the one workgroup of the launch (per system) that writes the system's scalar state.
 template <class VT, int R>
-__global__ void __launch_bounds__(256) k_ssor_bwd(const SsorSysT<VT>* __restrict__ sys, int tile0, int c, int flags) {
-  __shared__ double red[4];
-  const SsorView<VT> S = sview(sys);
+__device__ __forceinline__ void ssor_bwd_body(const SsorView<VT>& S, int tile, int c, int flags, bool lead,
+                                              double* red) {
   gptr<PcgState> st = S.st;
   const bool init = flags & kInit, mid = flags & kMid;
   const ColourInfo ci = colour_info(S, c);
-  const int tile = tile0 + xcd_block(blockIdx.x, gridDim.x);
   const gptr<const VT> t = C(S.t);
   VT usum[R], dtv[R], rv[R], pv[R], qv[R], sv[R], yv[R];
   // the row loads are issued before the done test so that their latency overlaps it
@@ -284,7 +283,7 @@ __global__ void __launch_bounds__(256) k_ssor_bwd(const SsorSysT<VT>* __restrict
                     delta);
       if (k == 0) {
         alpha = (delta != 0.0) ? gamma / delta : 0.0;
-        if (blockIdx.x == 0) {
+        if (lead) {
           const double bb = final_reduce(C(S.part_bb), S.ntiles, red);
           if (threadIdx.x == 0) {
             const double atol = st->rtol * sqrt(bb);
@@ -302,7 +301,7 @@ __global__ void __launch_bounds__(256) k_ssor_bwd(const SsorSysT<VT>* __restrict
         const double den = delta - beta * gamma / aprev;
         alpha = (aprev != 0.0 && den != 0.0) ? gamma / den : 0.0;
       }
-      if (blockIdx.x == 0 && threadIdx.x == 0) {
+      if (lead && threadIdx.x == 0) {
         st->rho[k & 1] = gamma;
         st->alpha[k & 1] = alpha;
         st->beta = beta;
@@ -351,19 +350,34 @@ __global__ void __launch_bounds__(256) k_ssor_bwd(const SsorSysT<VT>* __restrict
   if (mid) {
     dl = block_tree(dl, red);
     if (threadIdx.x == 0) S.part_pq[wslot * S.ntiles + tile] = dl;
-    if ((flags & kLast) && !init && blockIdx.x == 0 && threadIdx.x == 0) st->k = st->kk + 1;
+    if ((flags & kLast) && !init && lead && threadIdx.x == 0) st->k = st->kk + 1;
   }
+}
+
+template <class VT, int R>
+__global__ void __launch_bounds__(256) k_ssor_bwd(const SsorSysT<VT>* __restrict__ sys, int tile0, int c, int flags) {
+  __shared__ double red[4];
+  ssor_bwd_body<VT, R>(sview(sys), tile0 + xcd_block(blockIdx.x, gridDim.x), c, flags, blockIdx.x == 0, red);
+}
+
+// Several systems in one launch: entry blockIdx of the table names the system, tile, colour and
+// flags (kLead marks each system's lead workgroup); or_flags is added to every entry.
+template <class VT, int R>
+__global__ void __launch_bounds__(256) k_ssor_bwd_batch(const SsorSysT<VT>* __restrict__ sys,
+                                                        const SsorRef* __restrict__ refs, int or_flags) {
+  __shared__ double red[4];
+  const SsorRef ref = refs[xcd_block(blockIdx.x, gridDim.x)];
+  const int f = ref.flags | or_flags;
+  ssor_bwd_body<VT, R>(sview(sys + ref.sys), ref.tile, ref.c, f, (f & kLead) != 0, red);
 }
 
 // ------------------------------------------------------------------ forward sweep
 // u = (D~ + L)^-1 (z - K t) for one colour, s = t + u = A^ z, partials of z.s. The last sweep of
 // an iteration advances the iteration counter.
 template <class VT, int R>
-__global__ void __launch_bounds__(256) k_ssor_fwd(const SsorSysT<VT>* __restrict__ sys, int tile0, int c, int flags) {
-  __shared__ double red[4];
-  const SsorView<VT> S = sview(sys);
+__device__ __forceinline__ void ssor_fwd_body(const SsorView<VT>& S, int tile, int c, int flags, bool lead,
+                                              double* red) {
   const ColourInfo ci = colour_info(S, c);
-  const int tile = tile0 + xcd_block(blockIdx.x, gridDim.x);
   const gptr<const VT> u = C(S.u);
   VT lsum[R], dtv[R], tv[R], rv[R], kdv[R];
   // the row loads are issued before the done test so that their latency overlaps it
@@ -392,18 +406,30 @@ __global__ void __launch_bounds__(256) k_ssor_fwd(const SsorSysT<VT>* __restrict
   }
   dl = block_tree(dl, red);
   if (threadIdx.x == 0) S.part_pq[wslot * S.ntiles + tile] = dl;
-  if ((flags & kLast) && !(flags & kInit) && blockIdx.x == 0 && threadIdx.x == 0) S.st->k = S.st->kk + 1;
+  if ((flags & kLast) && !(flags & kInit) && lead && threadIdx.x == 0) S.st->k = S.st->kk + 1;
+}
+
+template <class VT, int R>
+__global__ void __launch_bounds__(256) k_ssor_fwd(const SsorSysT<VT>* __restrict__ sys, int tile0, int c, int flags) {
+  __shared__ double red[4];
+  ssor_fwd_body<VT, R>(sview(sys), tile0 + xcd_block(blockIdx.x, gridDim.x), c, flags, blockIdx.x == 0, red);
+}
+
+template <class VT, int R>
+__global__ void __launch_bounds__(256) k_ssor_fwd_batch(const SsorSysT<VT>* __restrict__ sys,
+                                                        const SsorRef* __restrict__ refs, int or_flags) {
+  __shared__ double red[4];
+  const SsorRef ref = refs[xcd_block(blockIdx.x, gridDim.x)];
+  const int f = ref.flags | or_flags;
+  ssor_fwd_body<VT, R>(sview(sys + ref.sys), ref.tile, ref.c, f, (f & kLead) != 0, red);
 }
 
 // ------------------------------------------------------------------ stopping test
 // ||b - A x||^2 = ||(D~ + L) r||^2 partials, then the test of iteration k-1 (pcg_loop.cpp:405-412)
 // in one workgroup. Issued after the last iteration of every window of check_every.
 template <class VT, int R>
-__global__ void __launch_bounds__(256) k_ssor_true_rr(const SsorSysT<VT>* __restrict__ sys) {
-  __shared__ double red[4];
-  const SsorView<VT> S = sview(sys);
+__device__ __forceinline__ void ssor_true_rr_body(const SsorView<VT>& S, int tile, double* red) {
   if (load_done(S.st) >= 0) return;
-  const int tile = xcd_block(blockIdx.x, gridDim.x);
   const ColourInfo ci = colour_info(S, S.tile_colour[tile]);
   const gptr<const VT> r = C(S.r);
   double rr = 0.0;
@@ -416,10 +442,26 @@ __global__ void __launch_bounds__(256) k_ssor_true_rr(const SsorSysT<VT>* __rest
   if (threadIdx.x == 0) S.part_rr[tile] = rr;
 }
 
+template <class VT, int R>
+__global__ void __launch_bounds__(256) k_ssor_true_rr(const SsorSysT<VT>* __restrict__ sys) {
+  __shared__ double red[4];
+  ssor_true_rr_body<VT, R>(sview(sys), xcd_block(blockIdx.x, gridDim.x), red);
+}
+
+// table entries: system and tile (the colour and flags are unused)
+template <class VT, int R>
+__global__ void __launch_bounds__(256) k_ssor_true_rr_batch(const SsorSysT<VT>* __restrict__ sys,
+                                                            const SsorRef* __restrict__ refs) {
+  __shared__ double red[4];
+  const SsorRef ref = refs[xcd_block(blockIdx.x, gridDim.x)];
+  ssor_true_rr_body<VT, R>(sview(sys + ref.sys), ref.tile, red);
+}
+
+// workgroup b tests system b (one system: b = 0)
 template <class VT>
 __global__ void __launch_bounds__(256) k_ssor_check(const SsorSysT<VT>* __restrict__ sys) {
   __shared__ double red[4];
-  const SsorView<VT> S = sview(sys);
+  const SsorView<VT> S = sview(sys + blockIdx.x);
   gptr<PcgState> st = S.st;
   if (load_done(st) >= 0) return;
   const int k = st->k;
@@ -551,6 +593,35 @@ void launch_ssor_scatter(const SsorSysT<VT>* sys, int ntiles, int rpt, hipStream
 }
 
 
+template <class VT>
+__global__ void k_ssor_gather_states(const SsorSysT<VT>* __restrict__ sys, int count, PcgState* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) out[i] = *sys[i].st;
+}
+
+template <class VT>
+void launch_ssor_bwd_batch(const SsorSysT<VT>* sys, const SsorRef* refs, int nrefs, int or_flags, int rpt,
+                           hipStream_t s) {
+  if (nrefs > 0) GSH_SSOR_R(rpt, k_ssor_bwd_batch, nrefs, s, sys, refs, or_flags);
+}
+template <class VT>
+void launch_ssor_fwd_batch(const SsorSysT<VT>* sys, const SsorRef* refs, int nrefs, int or_flags, int rpt,
+                           hipStream_t s) {
+  if (nrefs > 0) GSH_SSOR_R(rpt, k_ssor_fwd_batch, nrefs, s, sys, refs, or_flags);
+}
+template <class VT>
+void launch_ssor_true_rr_batch(const SsorSysT<VT>* sys, const SsorRef* refs, int nrefs, int rpt, hipStream_t s) {
+  if (nrefs > 0) GSH_SSOR_R(rpt, k_ssor_true_rr_batch, nrefs, s, sys, refs);
+}
+template <class VT>
+void launch_ssor_check_batch(const SsorSysT<VT>* sys, int count, hipStream_t s) {
+  if (count > 0) hipLaunchKernelGGL(k_ssor_check<VT>, dim3(count), dim3(kTpb), 0, s, sys);
+}
+template <class VT>
+void launch_ssor_gather_states(const SsorSysT<VT>* sys, int count, PcgState* out, hipStream_t s) {
+  if (count > 0) hipLaunchKernelGGL(k_ssor_gather_states<VT>, dim3((count + 63) / 64), dim3(64), 0, s, sys, count, out);
+}
+
 #define GSH_SSOR_INSTANTIATE(VT)                                                                       \
   template void launch_ssor_reset<VT>(const SsorSysT<VT>*, double, int, int, hipStream_t);             \
   template void launch_ssor_setup<VT>(const SsorSysT<VT>*, int, int, hipStream_t, bool);               \
@@ -563,7 +634,12 @@ void launch_ssor_scatter(const SsorSysT<VT>* sys, int ntiles, int rpt, hipStream
   template void launch_ssor_true_rr<VT>(const SsorSysT<VT>*, int, int, hipStream_t);                   \
   template void launch_ssor_check<VT>(const SsorSysT<VT>*, hipStream_t);                               \
   template void launch_ssor_xsolve<VT>(const SsorSysT<VT>*, int, int, int, int, hipStream_t);          \
-  template void launch_ssor_scatter<VT>(const SsorSysT<VT>*, int, int, hipStream_t);
+  template void launch_ssor_scatter<VT>(const SsorSysT<VT>*, int, int, hipStream_t);                   \
+  template void launch_ssor_bwd_batch<VT>(const SsorSysT<VT>*, const SsorRef*, int, int, int, hipStream_t); \
+  template void launch_ssor_fwd_batch<VT>(const SsorSysT<VT>*, const SsorRef*, int, int, int, hipStream_t); \
+  template void launch_ssor_true_rr_batch<VT>(const SsorSysT<VT>*, const SsorRef*, int, int, hipStream_t);  \
+  template void launch_ssor_check_batch<VT>(const SsorSysT<VT>*, int, hipStream_t);                          \
+  template void launch_ssor_gather_states<VT>(const SsorSysT<VT>*, int, PcgState*, hipStream_t);
 GSH_SSOR_INSTANTIATE(double)
 GSH_SSOR_INSTANTIATE(float)
 

@@ -288,9 +288,11 @@ int gsh_colour_csr(int32_t n, const int32_t* indptr, const int32_t* indices, int
 int gsh_update_elevation(gsh_mesh* mesh, gsh_solver* s);
 
 /* ---------------------------------------------------------------- batching */
-/* Solves several independent systems in lock-step launches: every PCG phase is one launch over
- * the tiles of all systems that have not converged. Each system keeps the single-solve
- * stopping rule, so results equal solving them one by one. */
+/* Solves several independent systems in lock-step launches: every PCG phase (Jacobi) or colour
+ * step of the SSOR sweeps is one launch over the tiles of all systems that have not converged.
+ * Each system keeps the single-solve stopping rule, so results equal solving them one by one.
+ * All solvers share one context, precision and preconditioner; SSOR solvers also share the rows
+ * per thread. Systems with fewer colours sit out the colour steps beyond their own. */
 int gsh_batch_solve(gsh_solver* const* solvers, int count, double rtol, int maxiter,
                     int check_every, double timeout_s, int* iters_out);
 

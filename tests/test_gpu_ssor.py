@@ -218,3 +218,123 @@ def test_mixed_precision_ssor(ctx):
     assert it1 <= 2 * it64 + 50, (it1, it64)
     sm.close()
     s64.close()
+
+
+def _csr_system(ctx, kind, n, seed, precision=L.PREC_FP64):
+    """A standalone SSOR solver on an SPD matrix: 'random' (several colours), 'tridiag' (two),
+    'diag' (one colour)."""
+    if kind == "random":
+        A = sp.random(n, n, density=5.0 / n, random_state=seed, format="csr")
+        A = A + A.T
+        A = A + sp.diags(np.abs(A).sum(1).A1 + 0.5)
+    elif kind == "tridiag":
+        A = sp.diags([-np.ones(n - 1), 2.5 * np.ones(n), -np.ones(n - 1)], [-1, 0, 1])
+    else:
+        A = sp.diags(1.0 + np.arange(n) % 7)
+    A = A.tocsr()
+    A.sort_indices()
+    s = core.Solver(None, precision, ctx=ctx, indptr=A.indptr, indices=A.indices, precond=L.PRECOND_SSOR)
+    s.set_values(A.data)
+    return s, A
+
+
+def test_batched_ssor_equals_solo(ctx):
+    """Batched FP64 SSOR: systems with different sizes and colour counts (one, two, several) and a
+    zero right-hand side advance in lock-step; each gets its solo iteration count and solution
+    bit for bit, and the solo results match the oracle."""
+    rng = np.random.default_rng(11)
+    systems = []
+    for k, (nx, ny) in enumerate(((40, 30), (64, 20))):
+        spec = cases.CaseSpec(f"sb{k}", nx, ny, 500.0, nvrt=5, depth=None, substeps=4)
+        mesh, om, p, val, b = _assembled(ctx, spec)
+        s = core.Solver(mesh, precond=L.PRECOND_SSOR)
+        s.assemble(p)
+        ip, ix = mesh.topology(L.T_CSR_INDPTR), mesh.topology(L.T_CSR_INDICES)
+        systems.append((s, ip, ix, val, None))
+    for kind, n, seed in (("random", 3000, 21), ("tridiag", 2000, 0), ("diag", 700, 0), ("random", 900, 22)):
+        s, A = _csr_system(ctx, kind, n, seed)
+        b = np.zeros(n) if seed == 22 else rng.normal(size=n)
+        systems.append((s, A.indptr, A.indices, A.data, b))
+    solo = []
+    for s, ip, ix, val, b in systems:
+        if b is not None:
+            s.set_rhs(b)
+        x, it, st = s.solve(None, rtol=1e-12, maxiter=3000)
+        solo.append((x, it))
+        s.reset_history()
+    ncols = [s.ssor_info()[0] for s, *_ in systems]
+    print(f"[gpu] batched SSOR of {len(systems)}, colours {ncols}", flush=True)
+    assert 1 in ncols and 2 in ncols and max(ncols) > 2
+    its = core.batch_solve([s for s, *_ in systems], rtol=1e-12, maxiter=3000)
+    for (s, ip, ix, val, b), (x, it), itb in zip(systems, solo, its):
+        assert itb == it
+        assert_bitwise(s.get_solution(), x, "batched SSOR solution")
+    assert its[-1] == 0 and not solo[-1][0].any()
+    s, ip, ix, val, b = systems[2]
+    xo, ito, _, _, _ = O.pcg_ssor(ip, ix, val, b, rtol=1e-12, maxiter=3000, rows_per_thread=1)
+    assert ito == its[2]
+    assert_bitwise(solo[2][0], xo, "solo SSOR vs oracle")
+    # every system at maxiter: the same count and the oracle's iterate
+    for s, *_ in systems:
+        s.reset_history()
+    its = core.batch_solve([s for s, *_ in systems[:3]], rtol=1e-30, maxiter=25)
+    assert its == [25, 25, 25]
+    xo, _, _, _, _ = O.pcg_ssor(ip, ix, val, b, rtol=1e-30, maxiter=25, rows_per_thread=1)
+    assert_bitwise(systems[2][0].get_solution(), xo, "batched SSOR at maxiter")
+    # tile size 4 for all: still the solo results; mixed tile sizes or preconditioners are errors
+    for s, *_ in systems:
+        s.set_tuning(4)
+        s.reset_history()
+    its4 = core.batch_solve([s for s, *_ in systems[:3]], rtol=1e-12, maxiter=3000)
+    xo4, ito4, _, _, _ = O.pcg_ssor(ip, ix, val, b, rtol=1e-12, maxiter=3000, rows_per_thread=4)
+    assert its4[2] == ito4
+    assert_bitwise(systems[2][0].get_solution(), xo4, "batched SSOR, 4 rows per thread")
+    systems[1][0].set_tuning(1)
+    with pytest.raises(L.GshError):
+        core.batch_solve([s for s, *_ in systems[:2]], rtol=1e-12, maxiter=100)
+    jac = core.Solver(None, ctx=ctx, indptr=systems[2][1], indices=systems[2][2])
+    jac.set_values(systems[2][3])
+    with pytest.raises(L.GshError):
+        core.batch_solve([systems[2][0], jac], rtol=1e-12, maxiter=100)
+    for s, *_ in systems:
+        s.close()
+    jac.close()
+
+
+def test_mixed_batched_ssor_equals_solo(ctx):
+    """Mixed precision with FP32 SSOR inner solves in lock-step: each system's result is its solo
+    result bit for bit and within 1e-8 of its FP64 SSOR solution."""
+    rng = np.random.default_rng(12)
+    systems = []
+    for k, (nx, ny) in enumerate(((50, 30), (28, 61))):
+        spec = cases.CaseSpec(f"ms{k}", nx, ny, 800.0, nvrt=4, depth=None, substeps=4)
+        mesh, om, p, val, b = _assembled(ctx, spec)
+        f64 = core.Solver(mesh, precond=L.PRECOND_SSOR)
+        f64.assemble(p)
+        x64, _, _ = f64.solve(None, rtol=1e-12, maxiter=3000)
+        f64.close()
+        mx = core.Solver(mesh, L.PREC_MIXED, precond=L.PRECOND_SSOR)
+        mx.assemble(p)
+        systems.append((mx, x64))
+    for kind, n, seed in (("random", 4000, 31), ("tridiag", 1500, 0)):
+        s64, A = _csr_system(ctx, kind, n, seed)
+        b = rng.normal(size=n)
+        x64, _, _ = s64.solve(b, rtol=1e-12, maxiter=3000)
+        s64.close()
+        mx, _ = _csr_system(ctx, kind, n, seed, L.PREC_MIXED)
+        mx.set_rhs(b)
+        systems.append((mx, x64))
+    solo = []
+    for mx, x64 in systems:
+        xm, itm, stm = mx.solve(None, rtol=1e-12, maxiter=3000)
+        assert stm == 0
+        assert np.linalg.norm(xm - x64) / np.linalg.norm(x64) < 1e-8
+        solo.append((xm, itm, mx.stats().outer))
+        mx.reset_history()
+    print(f"[gpu] mixed batched SSOR of {len(systems)}", flush=True)
+    its = core.batch_solve([m for m, _ in systems], rtol=1e-12, maxiter=3000)
+    for (mx, _), (xm, itm, outer), itb in zip(systems, solo, its):
+        assert itb == itm and mx.stats().outer == outer
+        assert_bitwise(mx.get_solution(), xm, "mixed batched SSOR solution")
+    for mx, _ in systems:
+        mx.close()
