@@ -302,8 +302,9 @@ def served_leg(ctx, L, cases, core, n_req, rates, rtol, mixed_rtol, check_every)
     """Served solves: a pool of 6 meshes with n log-uniform in [1e4, 2e6] (random bathymetry,
     seeds 3 + k), requests drawn uniformly from the pool with Poisson arrivals (seed 7) and
     right-hand sides N(0,1) (seeds 1000 + j); each request backtracks on its mesh and solves.
-    Plain = the same requests one after another in FP64; served = the continuous-batching
-    server in mixed precision at each arrival rate."""
+    Plain = the same requests one after another in FP64 (Jacobi, then multicolour SSOR); served =
+    the continuous-batching server in mixed precision at each arrival rate, with FP32 Jacobi
+    ("rates") and FP32 multicolour SSOR ("ssor_rates") inner solves."""
     from math import erfc, log, sqrt
 
     sizes = [int(round(1e4 * 200 ** (k / 5))) for k in range(6)]
@@ -364,44 +365,52 @@ def served_leg(ctx, L, cases, core, n_req, rates, rtol, mixed_rtol, check_every)
     res = {"workload": f"{n_req} requests over 6 meshes, n = {sizes}", "plain_fp64_solves_per_s": round(plain_rate, 3),
            "plain_fp64_ssor_solves_per_s": round(plain_ssor_rate, 3),
            "mixed_rtol": mixed_rtol, "rates": []}
-    srv, ids, warm = None, None, True
-    for mult in (rates[-1],) + tuple(rates):  # the first pass warms the server's solver pools
-        lam = mult * plain_rate
-        arrive = np.cumsum(expo) / lam
-        if srv is None:
-            srv = core.Server(ctx, rtol=mixed_rtol, maxiter=20000, check_every=check_every, precision=L.PREC_MIXED)
-            ids = [srv.add_mesh(m) for m in pool]
-        tickets, by_ticket = {}, {}
-        done = {}
-        t0 = time.perf_counter()
-        nxt = 0
-        while len(done) < n_req:
-            now = time.perf_counter() - t0
-            while nxt < n_req and arrive[nxt] <= now:
-                tickets[nxt] = srv.submit(ids[which[nxt]], rhs[nxt], with_backtrack=True)
-                by_ticket[tickets[nxt]] = nxt
-                nxt += 1
-            a, q = srv.step()
-            for tk in srv.finished():
-                j = by_ticket[tk]
-                st, x, it, lat = srv.poll(tk, pool[which[j]].n)
-                if st != 0:
-                    raise RuntimeError(f"request {j} failed with status {st}")
-                done[j] = (x, it, lat, time.perf_counter() - t0)
-            if a == 0 and q == 0 and nxt < n_req:
-                time.sleep(max(0.0, min(0.01, arrive[nxt] - (time.perf_counter() - t0))))
-        t_end = max(v[3] for v in done.values())
-        lats = np.array([done[j][2] for j in range(n_req)])
-        errs = [float(np.linalg.norm(done[j][0] - x64[j]) / np.linalg.norm(x64[j])) for j in range(n_req)]
-        if warm:
-            warm = False
-            continue
-        res["rates"].append({"arrival_rate_per_s": round(lam, 3), "solves_per_s": round(n_req / (t_end - arrive[0]), 3),
-                             "p50_ms": round(float(np.percentile(lats, 50)), 3),
-                             "p99_ms": round(float(np.percentile(lats, 99)), 3),
-                             "max_rel_l2_vs_fp64": max(errs),
-                             "mean_inner_iters": float(np.mean([done[j][1] for j in range(n_req)]))})
-    srv.close()
+    def serve(precond):
+        out = []
+        srv, ids, warm = None, None, True
+        for mult in (rates[-1],) + tuple(rates):  # the first pass warms the server's solver pools
+            lam = mult * plain_rate
+            arrive = np.cumsum(expo) / lam
+            if srv is None:
+                srv = core.Server(ctx, rtol=mixed_rtol, maxiter=20000, check_every=check_every,
+                                  precision=L.PREC_MIXED, precond=precond)
+                ids = [srv.add_mesh(m) for m in pool]
+            tickets, by_ticket = {}, {}
+            done = {}
+            t0 = time.perf_counter()
+            nxt = 0
+            while len(done) < n_req:
+                now = time.perf_counter() - t0
+                while nxt < n_req and arrive[nxt] <= now:
+                    tickets[nxt] = srv.submit(ids[which[nxt]], rhs[nxt], with_backtrack=True)
+                    by_ticket[tickets[nxt]] = nxt
+                    nxt += 1
+                a, q = srv.step()
+                for tk in srv.finished():
+                    j = by_ticket[tk]
+                    st, x, it, lat = srv.poll(tk, pool[which[j]].n)
+                    if st != 0:
+                        raise RuntimeError(f"request {j} failed with status {st}")
+                    done[j] = (x, it, lat, time.perf_counter() - t0)
+                if a == 0 and q == 0 and nxt < n_req:
+                    time.sleep(max(0.0, min(0.01, arrive[nxt] - (time.perf_counter() - t0))))
+            t_end = max(v[3] for v in done.values())
+            lats = np.array([done[j][2] for j in range(n_req)])
+            errs = [float(np.linalg.norm(done[j][0] - x64[j]) / np.linalg.norm(x64[j])) for j in range(n_req)]
+            if warm:
+                warm = False
+                continue
+            out.append({"arrival_rate_per_s": round(lam, 3),
+                        "solves_per_s": round(n_req / (t_end - arrive[0]), 3),
+                        "p50_ms": round(float(np.percentile(lats, 50)), 3),
+                        "p99_ms": round(float(np.percentile(lats, 99)), 3),
+                        "max_rel_l2_vs_fp64": max(errs),
+                        "mean_inner_iters": float(np.mean([done[j][1] for j in range(n_req)]))})
+        srv.close()
+        return out
+
+    res["rates"] = serve(L.PRECOND_JACOBI)
+    res["ssor_rates"] = serve(L.PRECOND_SSOR)
     for m in pool:
         m.close()
     return res

@@ -72,7 +72,8 @@ class SolveStats(C.Structure):
 
 class ServerConfig(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_int64), ("max_active", C.c_int32), ("precision", C.c_int32),
-                ("rtol", C.c_double), ("maxiter", C.c_int32), ("check_every", C.c_int32)]
+                ("rtol", C.c_double), ("maxiter", C.c_int32), ("check_every", C.c_int32),
+                ("precond", C.c_int32)]
 
 
 _P = C.c_void_p

@@ -341,8 +341,10 @@ class Server:
     PCG launches at check-window boundaries."""
 
     def __init__(self, ctx: Context, rtol: float = 1e-12, maxiter: int = 3000, check_every: int = 10,
-                 max_active: int = 0, hbm_budget_bytes: int = 0, precision: int = L.PREC_FP64):
+                 max_active: int = 0, hbm_budget_bytes: int = 0, precision: int = L.PREC_FP64,
+                 precond: int = L.PRECOND_JACOBI):
         cfg = L.ServerConfig()
+        cfg.precond = precond
         cfg.hbm_budget_bytes = hbm_budget_bytes
         cfg.max_active = max_active
         cfg.precision = precision

@@ -467,7 +467,8 @@ SsorSysT<float>* ssor_dev<float>(SsorState& S) {
 // FP64: the solver's values, b, x and history. FP32 (inner solve of mixed precision): the FP32
 // copy of the values, the given FP64 right-hand side (the outer residual), the FP32 correction
 // as solution and a zero initial guess.
-static void ssor_sync_desc(gsh_solver* s) {
+// With a stream given the copy is only queued there (the host descriptor lives in the solver).
+static void ssor_sync_desc(gsh_solver* s, hipStream_t on = nullptr) {
   SsorSys& h = s->ssor->hs;
   h.aval = s->val;
   h.b = s->b;
@@ -477,8 +478,8 @@ static void ssor_sync_desc(gsh_solver* s) {
   h.h2 = s->h[2];
   h.omega = s->omega;
   h.unit_omega = s->omega == 1.0;
-  HIPCHK(hipMemcpyAsync(s->ssor->ds, &h, sizeof(h), hipMemcpyHostToDevice, s->ctx->stream));
-  HIPCHK(hipStreamSynchronize(s->ctx->stream));
+  HIPCHK(hipMemcpyAsync(s->ssor->ds, &h, sizeof(h), hipMemcpyHostToDevice, on ? on : s->ctx->stream));
+  if (!on) HIPCHK(hipStreamSynchronize(s->ctx->stream));
 }
 
 static void ssor_sync_desc32(gsh_solver* s, const double* rhs) {
@@ -605,6 +606,98 @@ struct SsorTimes {
   int64_t iters = 0;
 };
 
+// Flags of an SSOR sweep launch (the kernels' kFirst .. kLead): first sweep of an iteration,
+// colour 0 (forms u and s), last sweep of an iteration, set-up pass, workgroup that writes the
+// system's scalars (batched launches).
+constexpr int kSsorFirst = 1, kSsorMid = 2, kSsorLast = 4, kSsorInit = 8, kSsorLead = 16;
+
+// Launch table of a batch of SSOR systems: backward colour steps 0 .. maxC-1 at boff, forward
+// steps 0 .. maxC-2 at foff, then every tile of every system at aoff (true residual).
+struct SsorTable {
+  std::vector<SsorRef> refs;
+  std::vector<int> boff, foff;
+  int aoff = 0, nall = 0, maxC = 0;
+};
+
+static SsorTable ssor_table(gsh_solver* const* S, int count) {
+  SsorTable tb;
+  for (int i = 0; i < count; ++i) tb.maxC = std::max(tb.maxC, S[i]->ssor->L.ncol);
+  const int maxC = tb.maxC;
+  auto& refs = tb.refs;
+  tb.boff.assign(maxC + 1, 0);
+  tb.foff.assign(std::max(1, maxC), 0);
+  auto add_colour = [&](int i, int c, int flags) {
+    const SsorState& T = *S[i]->ssor;
+    for (int t = 0; t < T.ntc[c]; ++t) refs.push_back({i, T.tile0[c] + t, c, flags | (t == 0 ? kSsorLead : 0)});
+  };
+  for (int step = 0; step < maxC; ++step) {
+    tb.boff[step] = (int)refs.size();
+    for (int i = 0; i < count; ++i) {
+      const int C = S[i]->ssor->L.ncol;
+      if (step >= C) continue;
+      const int c = C - 1 - step;
+      add_colour(i, c, (step == 0 ? kSsorFirst : 0) | (c == 0 ? kSsorMid : 0) | (C == 1 ? kSsorLast : 0));
+    }
+  }
+  tb.boff[maxC] = (int)refs.size();
+  for (int step = 0; step + 1 < maxC; ++step) {
+    tb.foff[step] = (int)refs.size();
+    for (int i = 0; i < count; ++i) {
+      const int C = S[i]->ssor->L.ncol, c = step + 1;
+      if (c < C) add_colour(i, c, c == C - 1 ? kSsorLast : 0);
+    }
+  }
+  tb.foff[std::max(0, maxC - 1)] = (int)refs.size();
+  tb.aoff = (int)refs.size();
+  for (int i = 0; i < count; ++i)
+    for (int t = 0; t < S[i]->ssor->ntiles; ++t) refs.push_back({i, t, 0, 0});
+  tb.nall = (int)refs.size() - tb.aoff;
+  for (const SsorRef& r : refs)  // every entry names a tile of its own system
+    GSH_REQUIRE(r.sys >= 0 && r.sys < count && r.tile >= 0 && r.tile < S[r.sys]->ssor->ntiles, GSH_E_STATE,
+                "bad batched SSOR table");
+  return tb;
+}
+
+// One SSOR iteration (or, with kSsorInit, the set-up pass) of every system of a table.
+template <class VT>
+static void ssor_table_sweeps(const SsorSysT<VT>* dsys, const SsorRef* drefs, const SsorTable& tb, int or_flags,
+                              int rpt, hipStream_t st) {
+  for (int step = 0; step < tb.maxC; ++step)
+    launch_ssor_bwd_batch(dsys, drefs + tb.boff[step], tb.boff[step + 1] - tb.boff[step], or_flags, rpt, st);
+  for (int step = 0; step + 1 < tb.maxC; ++step)
+    launch_ssor_fwd_batch(dsys, drefs + tb.foff[step], tb.foff[step + 1] - tb.foff[step], or_flags, rpt, st);
+}
+
+// Per-system start of an SSOR solve: state, ELL values (unless kept), r0 = (D~ + L)^-1 (b - A x0)
+// and, with init_pass, the set-up sweeps (z0, s0, partials of iteration 0) by the one-system
+// kernels, whose bodies are the batched ones.
+template <class VT>
+static void ssor_begin(gsh_solver* s, double rtol, int maxiter, int check_every, bool gather, bool init_pass,
+                       hipStream_t st) {
+  SsorState& T = *s->ssor;
+  const SsorSysT<VT>* ds = ssor_dev<VT>(T);
+  const int rpt = T.rpt, C = T.L.ncol;
+  launch_ssor_reset(ds, rtol, maxiter, check_every, st);
+  launch_ssor_setup(ds, T.ntiles, rpt, st, gather);
+  launch_ssor_residual(ds, T.ntiles, rpt, st);
+  for (int c = 0; c < C; ++c) launch_ssor_r0(ds, T.tile0[c], T.ntc[c], c, rpt, st);
+  if (!init_pass) return;
+  for (int c = C - 1; c >= 0; --c)
+    launch_ssor_bwd(ds, T.tile0[c], T.ntc[c], c,
+                    (c == C - 1 ? kSsorFirst : 0) | (c == 0 ? kSsorMid : 0) | (C == 1 ? kSsorLast : 0) | kSsorInit,
+                    rpt, st);
+  for (int c = 1; c < C; ++c) launch_ssor_fwd(ds, T.tile0[c], T.ntc[c], c, (c == C - 1 ? kSsorLast : 0) | kSsorInit, rpt, st);
+}
+
+// End of an SSOR solve: x = (D~ + U)^-1 y, back to the original row order.
+template <class VT>
+static void ssor_finish(gsh_solver* s, hipStream_t st) {
+  SsorState& T = *s->ssor;
+  const SsorSysT<VT>* ds = ssor_dev<VT>(T);
+  for (int c = T.L.ncol - 1; c >= 0; --c) launch_ssor_xsolve(ds, T.tile0[c], T.ntc[c], c, T.rpt, st);
+  launch_ssor_scatter(ds, T.ntiles, T.rpt, st);
+}
+
 // Multicolour SSOR PCG of one system: set-up, windows of check_every iterations (2 * ncolours
 // launches each) closed by the true-residual test, then x = (D~ + U)^-1 y. The descriptor of
 // the VT system must be synchronised first (ssor_sync_desc / ssor_sync_desc32).
@@ -619,22 +712,19 @@ static PcgState run_ssor(gsh_solver* s, double rtol, int maxiter, int check_ever
   const SsorSysT<VT>* ds = ssor_dev<VT>(S);
   const int rpt = S.rpt, C = S.L.ncol, NT = S.ntiles;
   PcgState* slots = ctx->pinned_slots(1);
-  launch_ssor_reset(ds, rtol, maxiter, check_every, st);
-  launch_ssor_setup(ds, NT, rpt, st, gather);
-  launch_ssor_residual(ds, NT, rpt, st);
-  for (int c = 0; c < C; ++c) launch_ssor_r0(ds, S.tile0[c], S.ntc[c], c, rpt, st);
-  // flags of the sweeps (see ssor_kernels.hip): 1 first, 2 colour 0, 4 last of the iteration, 8 set-up
-  auto bflags = [&](int c) { return (c == C - 1 ? 1 : 0) | (c == 0 ? 2 : 0) | (C == 1 ? 4 : 0); };
-  auto fflags = [&](int c) { return c == C - 1 ? 4 : 0; };
+  // set-up, then the set-up pass: z0 = D~ r0, s0 = A^ z0 and the gamma, delta partials of iteration 0
+  ssor_begin<VT>(s, rtol, maxiter, check_every, gather, true, st);
+  // flags of the sweeps (see ssor_kernels.hip): first sweep, colour 0, last of the iteration
+  auto bflags = [&](int c) {
+    return (c == C - 1 ? kSsorFirst : 0) | (c == 0 ? kSsorMid : 0) | (C == 1 ? kSsorLast : 0);
+  };
+  auto fflags = [&](int c) { return c == C - 1 ? kSsorLast : 0; };
   auto bwd = [&]() {
     for (int c = C - 1; c >= 0; --c) launch_ssor_bwd(ds, S.tile0[c], S.ntc[c], c, bflags(c), rpt, st);
   };
   auto fwd = [&]() {
     for (int c = 1; c < C; ++c) launch_ssor_fwd(ds, S.tile0[c], S.ntc[c], c, fflags(c), rpt, st);
   };
-  // set-up pass: z0 = D~ r0, s0 = A^ z0 and the gamma, delta partials of iteration 0
-  for (int c = C - 1; c >= 0; --c) launch_ssor_bwd(ds, S.tile0[c], S.ntc[c], c, bflags(c) | 8, rpt, st);
-  for (int c = 1; c < C; ++c) launch_ssor_fwd(ds, S.tile0[c], S.ntc[c], c, fflags(c) | 8, rpt, st);
   HIPCHK(hipGetLastError());
   int issued = 0, window = 0;
   std::deque<int> inflight;
@@ -694,8 +784,7 @@ static PcgState run_ssor(gsh_solver* s, double rtol, int maxiter, int check_ever
       throw Error(GSH_E_TIMEOUT, "SSOR PCG: time limit exceeded while issuing iterations");
     HIPCHK(hipGetLastError());
   }
-  for (int c = C - 1; c >= 0; --c) launch_ssor_xsolve(ds, S.tile0[c], S.ntc[c], c, rpt, st);
-  launch_ssor_scatter(ds, NT, rpt, st);
+  ssor_finish<VT>(s, st);
   HIPCHK(hipMemcpyAsync(slots, ssor_host<VT>(S).st, sizeof(PcgState), hipMemcpyDeviceToHost, st));
   HIPCHK(hipEventRecord(ctx->ev[0], st));
   wait_event(ctx->ev[0], timeout_s, "SSOR PCG finish");
@@ -710,8 +799,6 @@ static PcgState run_ssor(gsh_solver* s, double rtol, int maxiter, int check_ever
 // does not depend on the others, so each result equals its solo solve. The VT descriptors of the
 // solvers must be synchronised first; rtols are per system; gather[i] = 0 keeps system i's ELL
 // values from its previous set-up.
-constexpr int kSsorFirst = 1, kSsorMid = 2, kSsorLast = 4, kSsorInit = 8, kSsorLead = 16;
-
 template <class VT>
 static void run_ssor_batch(gsh_solver* const* S, int count, const std::vector<double>& rtols, int maxiter,
                            int check_every, double timeout_s, const std::vector<char>& gather,
@@ -721,68 +808,21 @@ static void run_ssor_batch(gsh_solver* const* S, int count, const std::vector<do
   if (check_every < 1) check_every = 1;
   if (timeout_s <= 0.0) timeout_s = 60.0;
   const int rpt = S[0]->ssor->rpt;
-  int maxC = 0;
   std::vector<SsorSysT<VT>> hsys(count);
   for (int i = 0; i < count; ++i) {
     SsorState& T = *S[i]->ssor;
     GSH_REQUIRE(T.rpt == rpt, GSH_E_ARG, "batched SSOR solvers need the same rows per thread");
     hsys[i] = ssor_host<VT>(T);
-    maxC = std::max(maxC, T.L.ncol);
   }
-  // launch tables: backward steps 0 .. maxC-1, forward steps 0 .. maxC-2, then every tile
-  std::vector<SsorRef> refs;
-  std::vector<int> boff(maxC + 1), foff(std::max(1, maxC));
-  auto add_colour = [&](int i, int c, int flags) {
-    const SsorState& T = *S[i]->ssor;
-    for (int t = 0; t < T.ntc[c]; ++t) refs.push_back({i, T.tile0[c] + t, c, flags | (t == 0 ? kSsorLead : 0)});
-  };
-  for (int step = 0; step < maxC; ++step) {
-    boff[step] = (int)refs.size();
-    for (int i = 0; i < count; ++i) {
-      const int C = S[i]->ssor->L.ncol;
-      if (step >= C) continue;
-      const int c = C - 1 - step;
-      add_colour(i, c, (step == 0 ? kSsorFirst : 0) | (c == 0 ? kSsorMid : 0) | (C == 1 ? kSsorLast : 0));
-    }
-  }
-  boff[maxC] = (int)refs.size();
-  for (int step = 0; step + 1 < maxC; ++step) {
-    foff[step] = (int)refs.size();
-    for (int i = 0; i < count; ++i) {
-      const int C = S[i]->ssor->L.ncol, c = step + 1;
-      if (c < C) add_colour(i, c, c == C - 1 ? kSsorLast : 0);
-    }
-  }
-  foff[std::max(0, maxC - 1)] = (int)refs.size();
-  const int aoff = (int)refs.size();
-  for (int i = 0; i < count; ++i)
-    for (int t = 0; t < S[i]->ssor->ntiles; ++t) refs.push_back({i, t, 0, 0});
-  const int nall = (int)refs.size() - aoff;
-  for (const SsorRef& r : refs)  // every entry names a tile of its own system
-    GSH_REQUIRE(r.sys >= 0 && r.sys < count && r.tile >= 0 && r.tile < S[r.sys]->ssor->ntiles, GSH_E_STATE,
-                "bad batched SSOR table");
+  const SsorTable tb = ssor_table(S, count);
   StreamBuffers tmp(st);
   SsorSysT<VT>* dsys = tmp.alloc<SsorSysT<VT>>(count);
-  SsorRef* drefs = tmp.alloc<SsorRef>(refs.size());
+  SsorRef* drefs = tmp.alloc<SsorRef>(tb.refs.size());
   PcgState* dgather = tmp.alloc<PcgState>(2 * (size_t)count);
   HIPCHK(hipMemcpyAsync(dsys, hsys.data(), sizeof(SsorSysT<VT>) * count, hipMemcpyHostToDevice, st));
-  HIPCHK(hipMemcpyAsync(drefs, refs.data(), sizeof(SsorRef) * refs.size(), hipMemcpyHostToDevice, st));
-  // per-system set-up: state, ELL values, r0 = (D~ + L)^-1 (b - A x0), y0
-  for (int i = 0; i < count; ++i) {
-    SsorState& T = *S[i]->ssor;
-    const SsorSysT<VT>* ds = ssor_dev<VT>(T);
-    launch_ssor_reset(ds, rtols[i], maxiter, check_every, st);
-    launch_ssor_setup(ds, T.ntiles, rpt, st, gather[i] != 0);
-    launch_ssor_residual(ds, T.ntiles, rpt, st);
-    for (int c = 0; c < T.L.ncol; ++c) launch_ssor_r0(ds, T.tile0[c], T.ntc[c], c, rpt, st);
-  }
-  auto sweeps = [&](int or_flags) {
-    for (int step = 0; step < maxC; ++step)
-      launch_ssor_bwd_batch(dsys, drefs + boff[step], boff[step + 1] - boff[step], or_flags, rpt, st);
-    for (int step = 0; step + 1 < maxC; ++step)
-      launch_ssor_fwd_batch(dsys, drefs + foff[step], foff[step + 1] - foff[step], or_flags, rpt, st);
-  };
-  sweeps(kSsorInit);  // z0, s0 and the partials of iteration 0
+  HIPCHK(hipMemcpyAsync(drefs, tb.refs.data(), sizeof(SsorRef) * tb.refs.size(), hipMemcpyHostToDevice, st));
+  for (int i = 0; i < count; ++i) ssor_begin<VT>(S[i], rtols[i], maxiter, check_every, gather[i] != 0, false, st);
+  ssor_table_sweeps(dsys, drefs, tb, kSsorInit, rpt, st);  // z0, s0 and the partials of iteration 0
   HIPCHK(hipGetLastError());
   PcgState* slots = ctx->pinned_slots(count);
   int issued = 0, window = 0;
@@ -797,8 +837,8 @@ static void run_ssor_batch(gsh_solver* const* S, int count, const std::vector<do
   };
   while (issued < maxiter && !all_done) {
     const int w = std::min(check_every, maxiter - issued);
-    for (int it = 0; it < w; ++it) sweeps(0);
-    launch_ssor_true_rr_batch(dsys, drefs + aoff, nall, rpt, st);
+    for (int it = 0; it < w; ++it) ssor_table_sweeps(dsys, drefs, tb, 0, rpt, st);
+    launch_ssor_true_rr_batch(dsys, drefs + tb.aoff, tb.nall, rpt, st);
     launch_ssor_check_batch(dsys, count, st);
     issued += w;
     const int slot = window & 1;
@@ -817,12 +857,7 @@ static void run_ssor_batch(gsh_solver* const* S, int count, const std::vector<do
       throw Error(GSH_E_TIMEOUT, "batched SSOR PCG: time limit exceeded while issuing iterations");
     HIPCHK(hipGetLastError());
   }
-  for (int i = 0; i < count; ++i) {
-    SsorState& T = *S[i]->ssor;
-    const SsorSysT<VT>* ds = ssor_dev<VT>(T);
-    for (int c = T.L.ncol - 1; c >= 0; --c) launch_ssor_xsolve(ds, T.tile0[c], T.ntc[c], c, rpt, st);
-    launch_ssor_scatter(ds, T.ntiles, rpt, st);
-  }
+  for (int i = 0; i < count; ++i) ssor_finish<VT>(S[i], st);
   launch_ssor_gather_states(dsys, count, dgather, st);
   HIPCHK(hipMemcpyAsync(slots, dgather, sizeof(PcgState) * count, hipMemcpyDeviceToHost, st));
   HIPCHK(hipEventRecord(ctx->ev[0], st));
@@ -1456,32 +1491,37 @@ static std::vector<int32_t> diag_abs_of(const std::vector<int32_t>& indptr, cons
   return d;
 }
 
+// A solver on the mesh's pattern; an SSOR solver gets its layout for `ssor_rpt` rows per thread.
+static gsh_solver* mesh_solver_new(gsh_mesh* mesh, int precision, int precond, int ssor_rpt) {
+  GSH_REQUIRE(precision == GSH_PREC_FP64 || precision == GSH_PREC_MIXED, GSH_E_ARG, "unknown precision");
+  GSH_REQUIRE(precond == GSH_PRECOND_JACOBI || precond == GSH_PRECOND_SSOR, GSH_E_ARG, "unknown preconditioner");
+  GSH_REQUIRE(mesh->ctx, GSH_E_STATE, "host-only mesh (created without a context)");
+  mesh->ctx->set();
+  auto s = std::make_unique<gsh_solver>();
+  s->ctx = mesh->ctx;
+  s->mesh = mesh;
+  s->precision = precision;
+  s->precond = precond;
+  const Topology& t = mesh->topo;
+  std::vector<int32_t> dabs(mesh->hm.n);
+  for (int32_t i = 0; i < mesh->hm.n; ++i) dabs[i] = t.indptr[i] + t.diag_off[i];
+  solver_alloc(s.get(), mesh->hm.n, t.indptr.back(), t.indptr, t.indices, dabs);
+  if (precond == GSH_PRECOND_SSOR) {
+    s->rpt = ssor_rpt;
+    s->ntiles = s->tiles_for(s->n);
+    s->hs.ntiles = s->ntiles;
+    s->h_indptr = t.indptr;
+    s->h_indices = t.indices;
+    ssor_build(s.get());
+  }
+  ctx_retain(s->ctx);
+  return s.release();
+}
+
 int gsh_solver_create(gsh_mesh* mesh, int precision, int precond, gsh_solver** out) {
   GSH_API_BODY({
     GSH_REQUIRE(mesh && out, GSH_E_ARG, "NULL argument");
-    GSH_REQUIRE(precision == GSH_PREC_FP64 || precision == GSH_PREC_MIXED, GSH_E_ARG, "unknown precision");
-    GSH_REQUIRE(precond == GSH_PRECOND_JACOBI || precond == GSH_PRECOND_SSOR, GSH_E_ARG, "unknown preconditioner");
-    GSH_REQUIRE(mesh->ctx, GSH_E_STATE, "host-only mesh (created without a context)");
-    mesh->ctx->set();
-    auto s = std::make_unique<gsh_solver>();
-    s->ctx = mesh->ctx;
-    s->mesh = mesh;
-    s->precision = precision;
-    s->precond = precond;
-    const Topology& t = mesh->topo;
-    std::vector<int32_t> dabs(mesh->hm.n);
-    for (int32_t i = 0; i < mesh->hm.n; ++i) dabs[i] = t.indptr[i] + t.diag_off[i];
-    solver_alloc(s.get(), mesh->hm.n, t.indptr.back(), t.indptr, t.indices, dabs);
-    if (precond == GSH_PRECOND_SSOR) {
-      s->rpt = 1;
-      s->ntiles = s->tiles_for(s->n);
-      s->hs.ntiles = s->ntiles;
-      s->h_indptr = t.indptr;
-      s->h_indices = t.indices;
-      ssor_build(s.get());
-    }
-    ctx_retain(s->ctx);
-    *out = s.release();
+    *out = mesh_solver_new(mesh, precision, precond, 1);
   })
 }
 
@@ -1895,6 +1935,7 @@ struct Request {
   // mixed precision: outer refinement state
   int phase = 0;          // 0: FP64 residual check due, 1: FP32 inner solve running
   bool inner_fresh = false;
+  bool ssor_gathered = false;  // SSOR inner solves: FP32 ELL values gathered by the first one
   int outer = 0, inner_total = 0;
   double atol2 = -1.0, in_rtol = 0.0;
   hipEvent_t ready = nullptr;  // end of the request's set-up on the staging stream
@@ -1926,6 +1967,12 @@ struct gsh_server {
   int dsys32_cap = 0;
   double* dred = nullptr;
   int dred_cap = 0;
+  SsorSysT<double>* dssor = nullptr;
+  int dssor_cap = 0;
+  SsorSysT<float>* dssor32 = nullptr;
+  int dssor32_cap = 0;
+  SsorRef* drefs = nullptr;
+  int drefs_cap = 0;
   // admission set-up (backtracking, assembly, casts) runs on its own stream and overlaps the
   // solve window of the step that admitted the request; the request joins the next step
   hipStream_t stage = nullptr;
@@ -1942,6 +1989,9 @@ struct gsh_server {
     (void)hipFree(dgather);
     (void)hipFree(dsys32);
     (void)hipFree(dred);
+    (void)hipFree(dssor);
+    (void)hipFree(dssor32);
+    (void)hipFree(drefs);
     if (stage) (void)hipStreamDestroy(stage);
   }
 };
@@ -1956,10 +2006,19 @@ static void ensure_cap(T*& p, int& cap, int need) {
   cap = c;
 }
 
-static int64_t solver_bytes(const gsh_mesh* m, int precision) {
+// Rows per thread of the server's solvers.
+static int server_rpt(const gsh_server_config& cfg) { return cfg.precond == GSH_PRECOND_SSOR ? 2 : 4; }
+
+static int64_t solver_bytes(const gsh_mesh* m, int precision, int precond) {
   const int64_t n = m->hm.n, nnz = m->topo.indptr.back();
   int64_t b = 14 * nnz + 4 * (n + 1) + 4 * n + 8 * 14 * n + 4 * 8 * ((n + kTpb - 1) / kTpb) + 4096;
   if (precision == GSH_PREC_MIXED) b += 4 * nnz + 8 * n + 4 * 7 * n + 4 * 8 * ((n + kTpb - 1) / kTpb);
+  if (precond == GSH_PRECOND_SSOR) {
+    // colour-major layout: ELL entries (column, source, value; at most 2.5x the off-diagonal
+    // count with padding), 12 vectors and 2 index arrays over rows padded to whole tiles
+    const int64_t vt = precision == GSH_PREC_MIXED ? 4 : 8, np = n + 16 * kTpb * 2;
+    b += (int64_t)(2.5 * (double)(nnz - n)) * (8 + vt) + np * (12 * vt + 8) + 64 * ((np + kTpb - 1) / kTpb);
+  }
   return b;
 }
 
@@ -1978,6 +2037,51 @@ static void server_retire(gsh_server* srv, Request* r, int32_t iters) {
   srv->pool[r->mesh_id].push_back(sol);
   r->solver = nullptr;
   srv->used -= r->bytes;
+}
+
+// One step of an FP64 SSOR server: one window of check_every iterations for every active
+// request (set up on the staging stream at admission) over a launch table, the window-end
+// test, then the finished requests leave with x = (D~ + U)^-1 y.
+static void server_step_ssor(gsh_server* srv) {
+  hipStream_t st = srv->ctx->stream;
+  const auto& cfg = srv->cfg;
+  const int na = (int)srv->active.size();
+  if (na == 0) return;
+  const int rpt = server_rpt(cfg);
+  std::vector<gsh_solver*> ss(na);
+  std::vector<SsorSys> hsys(na);
+  for (int i = 0; i < na; ++i) {
+    ss[i] = srv->active[i]->solver;
+    hsys[i] = ss[i]->ssor->hs;
+  }
+  const SsorTable tb = ssor_table(ss.data(), na);
+  ensure_cap(srv->dssor, srv->dssor_cap, na);
+  ensure_cap(srv->drefs, srv->drefs_cap, (int)tb.refs.size());
+  ensure_cap(srv->dgather, srv->dgather_cap, na);
+  HIPCHK(hipMemcpyAsync(srv->dssor, hsys.data(), sizeof(SsorSys) * na, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(srv->drefs, tb.refs.data(), sizeof(SsorRef) * tb.refs.size(), hipMemcpyHostToDevice, st));
+  for (int i = 0; i < cfg.check_every; ++i) ssor_table_sweeps<double>(srv->dssor, srv->drefs, tb, 0, rpt, st);
+  launch_ssor_true_rr_batch(srv->dssor, srv->drefs + tb.aoff, tb.nall, rpt, st);
+  launch_ssor_check_batch(srv->dssor, na, st);
+  launch_ssor_gather_states(srv->dssor, na, srv->dgather, st);
+  std::vector<PcgState> sts(na);
+  HIPCHK(hipMemcpyAsync(sts.data(), srv->dgather, sizeof(PcgState) * na, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipGetLastError());
+  std::vector<Request*> keep;
+  for (int i = 0; i < na; ++i) {
+    Request* r = srv->active[i];
+    r->run += cfg.check_every;
+    const PcgState& q = sts[i];
+    const bool done = q.done >= 0, maxed = r->run >= cfg.maxiter;
+    if (!done && !maxed) {
+      keep.push_back(r);
+      continue;
+    }
+    ssor_finish<double>(r->solver, st);
+    server_retire(srv, r, done ? q.done : cfg.maxiter);
+  }
+  srv->active.swap(keep);
 }
 
 // One step of a mixed-precision server: FP64 residual checks of the requests due one, then one
@@ -2017,7 +2121,7 @@ static void server_step_mixed(gsh_server* srv, const std::vector<Request*>& fres
     ensure_cap(srv->dred, srv->dred_cap, 2 * nc);
     HIPCHK(hipMemcpyAsync(srv->dsys, h64.data(), sizeof(PcgSys<double>) * nc, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(srv->dtiles, tl.data(), sizeof(TileRef) * tl.size(), hipMemcpyHostToDevice, st));
-    launch_residual_batch(srv->dsys, srv->dtiles, (int)tl.size(), 4, st);
+    launch_residual_batch(srv->dsys, srv->dtiles, (int)tl.size(), server_rpt(cfg), st);
     launch_reduce_systems(srv->dsys, nc, srv->dred, st);
     std::vector<double> red(2 * nc);
     HIPCHK(hipMemcpyAsync(red.data(), srv->dred, sizeof(double) * 2 * nc, hipMemcpyDeviceToHost, st));
@@ -2052,7 +2156,51 @@ static void server_step_mixed(gsh_server* srv, const std::vector<Request*>& fres
   const int n_old = (int)inr.size();
   for (Request* r : srv->active)
     if (r->phase == 1 && r->inner_fresh) inr.push_back(r);
-  if (!inr.empty()) {
+  if (!inr.empty() && cfg.precond == GSH_PRECOND_SSOR) {
+    // FP32 SSOR inner solves: the fresh ones start (their first one gathers the FP32 ELL
+    // values), then one window of sweeps for all over a launch table
+    const int ni = (int)inr.size();
+    std::vector<gsh_solver*> ss(ni);
+    std::vector<SsorSysT<float>> h32(ni);
+    for (int j = 0; j < ni; ++j) {
+      Request* r = inr[j];
+      gsh_solver* s = r->solver;
+      ss[j] = s;
+      if (j >= n_old) {
+        ssor_sync_desc32(s, s->r64);
+        ssor_begin<float>(s, r->in_rtol, cfg.maxiter, cfg.check_every, !r->ssor_gathered, true, st);
+        r->ssor_gathered = true;
+      }
+      h32[j] = s->ssor->hs32;
+    }
+    const SsorTable tb = ssor_table(ss.data(), ni);
+    const int rpt = server_rpt(cfg);
+    ensure_cap(srv->dssor32, srv->dssor32_cap, ni);
+    ensure_cap(srv->drefs, srv->drefs_cap, (int)tb.refs.size());
+    ensure_cap(srv->dgather, srv->dgather_cap, ni);
+    HIPCHK(hipMemcpyAsync(srv->dssor32, h32.data(), sizeof(SsorSysT<float>) * ni, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(srv->drefs, tb.refs.data(), sizeof(SsorRef) * tb.refs.size(), hipMemcpyHostToDevice, st));
+    for (int i = 0; i < cfg.check_every; ++i) ssor_table_sweeps<float>(srv->dssor32, srv->drefs, tb, 0, rpt, st);
+    launch_ssor_true_rr_batch(srv->dssor32, srv->drefs + tb.aoff, tb.nall, rpt, st);
+    launch_ssor_check_batch(srv->dssor32, ni, st);
+    launch_ssor_gather_states(srv->dssor32, ni, srv->dgather, st);
+    std::vector<PcgState> sts(ni);
+    HIPCHK(hipMemcpyAsync(sts.data(), srv->dgather, sizeof(PcgState) * ni, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    for (int j = 0; j < ni; ++j) {
+      Request* r = inr[j];
+      r->inner_fresh = false;
+      r->run += cfg.check_every;
+      const bool done = sts[j].done >= 0, maxed = r->run >= cfg.maxiter;
+      if (!done && !maxed) continue;
+      const int it = done ? sts[j].done : cfg.maxiter;
+      ssor_finish<float>(r->solver, st);  // the FP32 correction in the original row order
+      launch_refine_update(r->solver->x, r->solver->hs32.x, r->solver->n, st);
+      r->inner_total += it;
+      r->phase = 0;
+      if (it == 0) done_now.push_back(r);
+    }
+  } else if (!inr.empty()) {
     const int ni = (int)inr.size();
     std::vector<PcgSys<float>> h32(ni);
     std::vector<TileRef> tl, ftl;
@@ -2077,10 +2225,10 @@ static void server_step_mixed(gsh_server* srv, const std::vector<Request*>& fres
       HIPCHK(hipMemcpyAsync(srv->dfresh, ftl.data(), sizeof(TileRef) * ftl.size(), hipMemcpyHostToDevice, st));
       HIPCHK(hipMemcpyAsync(srv->dred, rts.data(), sizeof(double) * rts.size(), hipMemcpyHostToDevice, st));
       launch_pcg_reset<float>(srv->dsys32 + n_old, ni - n_old, cfg.rtol, cfg.maxiter, cfg.check_every, st, srv->dred);
-      launch_pcg_init<float>(srv->dsys32, srv->dfresh, (int)ftl.size(), 4, st);
+      launch_pcg_init<float>(srv->dsys32, srv->dfresh, (int)ftl.size(), server_rpt(cfg), st);
     }
     for (int i = 0; i < cfg.check_every; ++i)
-      launch_pcg_iter<float>(srv->dsys32, srv->dtiles, (int)tl.size(), 4, st);
+      launch_pcg_iter<float>(srv->dsys32, srv->dtiles, (int)tl.size(), server_rpt(cfg), st);
     launch_pcg_check<float>(srv->dsys32, ni, st);
     hipLaunchKernelGGL(k_gather_states<float>, dim3((ni + 63) / 64), dim3(64), 0, st, srv->dsys32, ni, srv->dgather);
     std::vector<PcgState> sts(ni);
@@ -2118,6 +2266,8 @@ int gsh_server_create(gsh_ctx* ctx, const gsh_server_config* cfg, gsh_server** o
     GSH_REQUIRE(ctx && cfg && out, GSH_E_ARG, "NULL argument");
     GSH_REQUIRE(cfg->precision == GSH_PREC_FP64 || cfg->precision == GSH_PREC_MIXED, GSH_E_ARG,
                 "unknown precision");
+    GSH_REQUIRE(cfg->precond == GSH_PRECOND_JACOBI || cfg->precond == GSH_PRECOND_SSOR, GSH_E_ARG,
+                "unknown preconditioner");
     GSH_REQUIRE(cfg->check_every >= 1 && cfg->maxiter >= 1 && cfg->maxiter % cfg->check_every == 0, GSH_E_ARG,
                 "maxiter must be a positive multiple of check_every");
     GSH_REQUIRE(cfg->rtol >= 0.0, GSH_E_ARG, "rtol must be >= 0");
@@ -2168,7 +2318,7 @@ int gsh_server_submit(gsh_server* srv, int32_t mesh_id, const double* b, int wit
       r->b.assign(b, b + srv->meshes[mesh_id]->hm.n);
     }
     r->t_submit = Clock::now();
-    r->bytes = solver_bytes(srv->meshes[mesh_id], srv->cfg.precision);
+    r->bytes = solver_bytes(srv->meshes[mesh_id], srv->cfg.precision, srv->cfg.precond);
     *ticket = r->ticket;
     srv->queue.push_back(r.get());
     srv->reqs[r->ticket] = std::move(r);
@@ -2222,8 +2372,11 @@ int gsh_server_step(gsh_server* srv, int32_t* active_out, int32_t* queued_out) {
         s = idle.back();
         idle.pop_back();
       } else {
-        if (gsh_solver_create(m, cfg.precision, GSH_PRECOND_JACOBI, &s) != GSH_OK)
-          throw Error(GSH_E_OOM, std::string("server admission: ") + g_last_error);
+        try {
+          s = mesh_solver_new(m, cfg.precision, cfg.precond, server_rpt(cfg));
+        } catch (const Error& e) {
+          throw Error(GSH_E_OOM, std::string("server admission: ") + e.what());
+        }
       }
       for (auto* hh : s->h) launch_fill_f64(hh, 0.0, s->n, sg);  // each request is a cold start
       if (r->with_bt) {
@@ -2242,8 +2395,12 @@ int gsh_server_step(gsh_server* srv, int32_t* active_out, int32_t* queued_out) {
       if (cfg.precision == GSH_PREC_MIXED) {
         launch_cast_values(s->val, s->val32, s->nnz, sg);
         launch_fill_f64(s->x, 0.0, s->n, sg);
+      } else if (cfg.precond == GSH_PRECOND_SSOR) {
+        // the whole SSOR set-up (values into the colour-major layout, r0, the set-up sweeps)
+        ssor_sync_desc(s, sg);
+        ssor_begin<double>(s, cfg.rtol, cfg.maxiter, cfg.check_every, true, true, sg);
       }
-      s->rpt = 4;
+      s->rpt = server_rpt(cfg);
       s->ntiles = s->tiles_for(s->n);
       s->hs.x = s->x;
       s->hs.h0 = s->h[0];
@@ -2259,8 +2416,13 @@ int gsh_server_step(gsh_server* srv, int32_t* active_out, int32_t* queued_out) {
       srv->staged.push_back(r);
     }
     for (auto* r : fresh) srv->active.push_back(r);
-    const int na = cfg.precision == GSH_PREC_MIXED ? 0 : (int)srv->active.size();
-    if (cfg.precision == GSH_PREC_MIXED) server_step_mixed(srv, fresh);
+    const bool jacobi64 = cfg.precision == GSH_PREC_FP64 && cfg.precond == GSH_PRECOND_JACOBI;
+    const int na = jacobi64 ? (int)srv->active.size() : 0;
+    if (cfg.precision == GSH_PREC_MIXED) {
+      server_step_mixed(srv, fresh);
+    } else if (cfg.precond == GSH_PRECOND_SSOR) {
+      server_step_ssor(srv);
+    }
     if (na > 0) {
       // ---- descriptors and tile tables (fresh systems are the last ones)
       std::vector<PcgSys<double>> hs(na);
@@ -2285,10 +2447,10 @@ int gsh_server_step(gsh_server* srv, int32_t* active_out, int32_t* queued_out) {
                               st));
         const int f0 = na - (int)fresh.size();
         launch_pcg_reset<double>(srv->dsys + f0, (int)fresh.size(), cfg.rtol, cfg.maxiter, cfg.check_every, st);
-        launch_pcg_init<double>(srv->dsys, srv->dfresh, (int)ftiles.size(), 4, st);
+        launch_pcg_init<double>(srv->dsys, srv->dfresh, (int)ftiles.size(), server_rpt(cfg), st);
       }
       // ---- one check window for every active system
-      for (int i = 0; i < cfg.check_every; ++i) launch_pcg_iter<double>(srv->dsys, srv->dtiles, (int)tiles.size(), 4, st);
+      for (int i = 0; i < cfg.check_every; ++i) launch_pcg_iter<double>(srv->dsys, srv->dtiles, (int)tiles.size(), server_rpt(cfg), st);
       launch_pcg_check<double>(srv->dsys, na, st);
       for (auto* r : srv->active) r->run += cfg.check_every;
       hipLaunchKernelGGL(k_gather_states<double>, dim3((na + 63) / 64), dim3(64), 0, st, srv->dsys, na, srv->dgather);

@@ -305,6 +305,7 @@ typedef struct gsh_server_config {
   double  rtol;               /* solve tolerance                                    */
   int32_t maxiter;
   int32_t check_every;
+  int32_t precond;            /* GSH_PRECOND_* (Jacobi: 4 rows per thread, SSOR: 2)   */
 } gsh_server_config;
 int gsh_server_create(gsh_ctx* ctx, const gsh_server_config* cfg, gsh_server** out);
 int gsh_server_destroy(gsh_server* srv);

@@ -97,3 +97,75 @@ def test_mixed_alone_vs_mix(ctx):
         assert st == 0, st
         assert it == alone[k][1], (k, it, alone[k][1])
         assert np.array_equal(x, alone[k][0]), f"mixed request on mesh {k} differs from the solo solve"
+
+
+def _ssor_alone(ctx, meshes, rhs, precision):
+    """Solo SSOR solves of each request (2 rows per thread, as the server's solvers)."""
+    alone = []
+    for (m, s), b in zip(meshes, rhs):
+        d = core.Solver(m, precision, precond=L.PRECOND_SSOR)
+        d.set_tuning(2)
+        d.set_values(s.get_values())
+        x, it, st = d.solve(b, rtol=1e-12, maxiter=3000, check_every=10)
+        assert st == 0
+        alone.append((x, it))
+        d.close()
+    return alone
+
+
+def _serve(srv, ids, rhs, order, every):
+    tickets = []
+    for j, k in enumerate(order):
+        tickets.append((k, srv.submit(ids[k], rhs[k])))
+        if j % every == every - 1:
+            srv.step()
+    for _ in range(5000):
+        a, q = srv.step()
+        if a == 0 and q == 0:
+            break
+    return tickets
+
+
+def test_ssor_alone_vs_mix(ctx):
+    """FP64 SSOR serving: each request, set up on the staging stream and advanced in the batched
+    colour-step launches with the others, equals its solo SSOR solve bit for bit; a zero
+    right-hand side returns x = 0 at once."""
+    meshes = [_mesh(ctx, 40, 30, 0), _mesh(ctx, 90, 25, 1), _mesh(ctx, 31, 57, 2)]
+    rhs = [cases.normal(500 + k, m.n) for k, (m, _) in enumerate(meshes)]
+    rhs.append(np.zeros(meshes[1][0].n))
+    alone = _ssor_alone(ctx, meshes + [meshes[1]], rhs, L.PREC_FP64)
+    assert alone[3][1] == 0
+    srv = core.Server(ctx, rtol=1e-12, maxiter=3000, check_every=10, max_active=3, precond=L.PRECOND_SSOR)
+    ids = [srv.add_mesh(m) for m, _ in meshes]
+    ids.append(ids[1])
+    print("[gpu] SSOR server, 7 requests", flush=True)
+    tickets = _serve(srv, ids, rhs, [0, 1, 3, 2, 1, 0, 2], 2)
+    for k, t in tickets:
+        st, x, it, lat = srv.poll(t, meshes[1 if k == 3 else k][0].n)
+        assert st == 0, st
+        assert it == alone[k][1], (k, it, alone[k][1])
+        assert np.array_equal(x, alone[k][0]), f"SSOR request {k} differs from the solo solve"
+    srv.close()
+
+
+def test_mixed_ssor_alone_vs_mix(ctx):
+    """Mixed-precision SSOR serving (FP32 SSOR inner windows, FP64 residual checks): every result
+    equals the request's solo mixed SSOR solve bit for bit and is within 1e-8 of FP64."""
+    meshes = [_mesh(ctx, 44, 30, 0, L.PREC_MIXED), _mesh(ctx, 80, 26, 1, L.PREC_MIXED),
+              _mesh(ctx, 35, 52, 2, L.PREC_MIXED)]
+    rhs = [cases.normal(600 + k, m.n) for k, (m, _) in enumerate(meshes)]
+    alone = _ssor_alone(ctx, meshes, rhs, L.PREC_MIXED)
+    ref = _ssor_alone(ctx, meshes, rhs, L.PREC_FP64)
+    for (x, _), (x64, _) in zip(alone, ref):
+        assert np.linalg.norm(x - x64) / np.linalg.norm(x64) < 1e-8
+    srv = core.Server(ctx, rtol=1e-12, maxiter=3000, check_every=10, max_active=2, precision=L.PREC_MIXED,
+                      precond=L.PRECOND_SSOR)
+    ids = [srv.add_mesh(m) for m, _ in meshes]
+    print("[gpu] mixed SSOR server, 7 requests", flush=True)
+    tickets = _serve(srv, ids, rhs, [2, 0, 1, 0, 2, 1, 1], 3)
+    for k, t in tickets:
+        st, x, it, lat = srv.poll(t, meshes[k][0].n)
+        assert st == 0, st
+        assert it == alone[k][1], (k, it, alone[k][1])
+        assert np.array_equal(x, alone[k][0]), f"mixed SSOR request on mesh {k} differs from the solo solve"
+    srv.close()
