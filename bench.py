@@ -476,10 +476,15 @@ def main() -> int:
     # (zero here); the system above is already assembled, so replacing the nodal field is harmless
     ms_nv = min(timed(lambda: mesh.nodal_velocity(p)) for _ in range(3))
     nv_bytes = (16 * mesh.ns + 8 * mesh.ne + 32 * n) * mesh.nvrt
+    # the step after the solve: side velocities from the momentum solve (zc, btu in, u2 out per
+    # side-level; about 300 B of element and node gathers per side for the elevation gradients)
+    ms_mom = min(timed(lambda: mesh.momentum(p)) for _ in range(3))
+    mom_bytes = 40 * mesh.ns * mesh.nvrt + 300 * mesh.ns
     ndelt = int(round(p.dt / p.dtb_max))
     bt_bytes = walks * (36 + 216 * ndelt)
     log(f"[bench] backtrack {ms_bt:.2f} ms ({walks} side-levels), rhs {ms_rhs:.2f} ms, assembly {ms_asm:.2f} ms, "
-        f"nodal velocity {ms_nv:.2f} ms ({nv_bytes / (ms_nv * 1e-3) / 1e9:.0f} GB/s)")
+        f"nodal velocity {ms_nv:.2f} ms ({nv_bytes / (ms_nv * 1e-3) / 1e9:.0f} GB/s), "
+        f"momentum {ms_mom:.2f} ms ({mom_bytes / (ms_mom * 1e-3) / 1e9:.0f} GB/s)")
 
     # ---- headline: FP64 multicolour SSOR PCG
     rtol, maxiter, check_every = 1e-12, 20000, 10
@@ -664,6 +669,7 @@ def main() -> int:
         "backtrack": {"ms": round(ms_bt, 3), "side_levels": walks, "per_s": round(walks / (ms_bt * 1e-3), 1),
                       "substeps": ndelt, "effective_gbs": round(bt_bytes / (ms_bt * 1e-3) / 1e9, 1)},
         "nodal_velocity": {"ms": round(ms_nv, 3), "bytes": nv_bytes, "gbs": round(nv_bytes / (ms_nv * 1e-3) / 1e9, 1)},
+        "momentum": {"ms": round(ms_mom, 3), "bytes": mom_bytes, "gbs": round(mom_bytes / (ms_mom * 1e-3) / 1e9, 1)},
         "assembly_ms": round(ms_asm, 3),
         "rhs_ms": round(ms_rhs, 3),
         "mixed": mixed,

@@ -62,7 +62,8 @@ class MeshInfo(C.Structure):
 class StepParams(C.Structure):
     _fields_ = [("dt", C.c_double), ("theta", C.c_double), ("h0", C.c_double), ("ramp", C.c_double),
                 ("dtb_min", C.c_double), ("dtb_max", C.c_double), ("velmin_btrack", C.c_double),
-                ("ihhat", C.c_int32), ("iettype", C.c_int32), ("rmaxvel", C.c_double)]
+                ("ihhat", C.c_int32), ("iettype", C.c_int32), ("rmaxvel", C.c_double),
+                ("dfv0", C.c_double)]
 
 
 class SolveStats(C.Structure):
@@ -107,6 +108,7 @@ SIGNATURES = {
     "gsh_backtrack_list": (_I, [_P, C.POINTER(StepParams), _PI32, _PI32, _I32]),
     "gsh_rhs_producers": (_I, [_P, C.POINTER(StepParams)]),
     "gsh_nodal_velocity": (_I, [_P, C.POINTER(StepParams)]),
+    "gsh_momentum": (_I, [_P, C.POINTER(StepParams)]),
     "gsh_grid_read": (_I, [C.c_char_p, C.c_char_p, _PP]),
     "gsh_grid_info": (_I, [_P, _PI32, _PI32, _PI32, _PI32, _PI32]),
     "gsh_grid_get": (_I, [_P, _PD, _PD, _PI32, C.POINTER(C.c_int8), _PI32, _PD]),

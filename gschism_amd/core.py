@@ -189,6 +189,11 @@ class Mesh:
         """Nodal u2/w2 (GSH_F_NODE_UVW) from side u2 and element w (GSH_F_ELEM_W)."""
         check(L.lib().gsh_nodal_velocity(self.h, C.byref(params)))
 
+    def momentum(self, params) -> None:
+        """Side velocities u2 of the next step (GSH_F_SIDE_U2): elevation gradients, the momentum
+        solve and the land-boundary slip condition."""
+        check(L.lib().gsh_momentum(self.h, C.byref(params)))
+
 
 class Solver:
     """PCG on the device (gsh_solver), bound to a mesh or to a caller CSR pattern; Jacobi

@@ -1087,6 +1087,7 @@ void gsh_step_params_default(gsh_step_params* p) {
   p->ihhat = 1;
   p->iettype = 2;
   p->rmaxvel = 5.0;  // readNamelist.py:192
+  p->dfv0 = 0.01;    // readNamelist.py:130
 }
 
 static StepArgs step_args(const gsh_step_params* p) {
@@ -1101,6 +1102,8 @@ static StepArgs step_args(const gsh_step_params* p) {
   a.dtb_min = p->dtb_min;
   a.dtb_max = p->dtb_max;
   a.velmin2 = p->velmin_btrack * p->velmin_btrack;
+  a.dfv0 = p->dfv0;
+  a.rmaxvel = p->rmaxvel;
   a.ihhat = p->ihhat;
   a.iettype = p->iettype;
   // backtrackAllNodes (meshdata.cu:3550-3551)
@@ -1465,6 +1468,19 @@ int gsh_nodal_velocity(gsh_mesh* m, const gsh_step_params* p) {
     m->ctx->set();
     ensure_elem_w(m);
     launch_nodal_velocity(m->d, p->rmaxvel, m->ctx->stream);
+    HIPCHK(hipGetLastError());
+  })
+}
+
+int gsh_momentum(gsh_mesh* m, const gsh_step_params* p) {
+  GSH_API_BODY({
+    GSH_REQUIRE(m && p, GSH_E_ARG, "NULL argument");
+    GSH_REQUIRE(m->ctx, GSH_E_STATE, "host-only mesh (created without a context)");
+    GSH_REQUIRE(p->rmaxvel >= 0.0, GSH_E_ARG, "rmaxvel must be >= 0");
+    GSH_REQUIRE(p->dfv0 >= 0.0, GSH_E_ARG, "dfv0 must be >= 0");
+    const StepArgs a = step_args(p);
+    m->ctx->set();
+    launch_momentum(m->d, a, m->ctx->stream);
     HIPCHK(hipGetLastError());
   })
 }

@@ -71,7 +71,7 @@ struct DMesh {
 };
 
 struct StepArgs {
-  double dt, theta, h0, ramp, dtb_min, dtb_max, velmin2;
+  double dt, theta, h0, ramp, dtb_min, dtb_max, velmin2, dfv0, rmaxvel;
   int32_t ihhat, iettype, ndelt_min, ndelt_max;
 };
 
@@ -168,6 +168,7 @@ void launch_backtrack(const DMesh& m, const StepArgs& a, const int32_t* list_sid
                       int32_t count, hipStream_t s);
 void launch_rhs(const DMesh& m, const StepArgs& a, hipStream_t s);
 void launch_nodal_velocity(const DMesh& m, double rmaxvel, hipStream_t s);
+void launch_momentum(const DMesh& m, const StepArgs& a, hipStream_t s);
 void launch_assemble(const DMesh& m, const StepArgs& a, double* val, double* b, hipStream_t s);
 void launch_update_elevation(const DMesh& m, const double* x, hipStream_t s);
 

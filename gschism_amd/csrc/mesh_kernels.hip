@@ -672,6 +672,223 @@ void launch_rhs(const DMesh& m, const StepArgs& a, hipStream_t st) {
   if (m.ne > 0) hipLaunchKernelGGL(k_elem_ghat, dim3((m.ne + kTpb - 1) / kTpb), dim3(kTpb), 0, st, m, a);
 }
 
+// ---------------------------------------------------------------- momentum (the step after the solve)
+// The inverse of stage_rows: `rows` LDS rows of `row_len` values (stride `stride`) to one
+// contiguous span of HBM, with flat coalesced stores.
+__device__ __forceinline__ void unstage_rows(double* __restrict__ dst, const double* __restrict__ src, int stride,
+                                             int row_len, int rows) {
+  const int total = rows * row_len;
+  const float inv = 1.0f / (float)row_len;
+  for (int i = threadIdx.x; i < total; i += blockDim.x) {
+    int g = (int)(((float)i + 0.5f) * inv);
+    int j = i - g * row_len;
+    if (j < 0) {
+      --g;
+      j += row_len;
+    }
+    if (j >= row_len) {
+      ++g;
+      j -= row_len;
+    }
+    dst[i] = src[g * stride + j];
+  }
+}
+
+// Side::grad2 (meshdata.cu:1363-1398) of eta1 over the side's elements that are wet (isDry of
+// the last level update) and of eta2 over those wet by eta2 (Element::isDry2, meshdata.cu:774-778:
+// a node with eta2 + depth <= h0 makes its element dry), each the mean of the element gradients
+// (Element::grad2, meshdata.h:1105-1114).
+__device__ __forceinline__ void side_eta_grads(const DMesh& m, const int4 sr, double h0, double& g1x, double& g1y,
+                                               double& g2x, double& g2y) {
+  g1x = g1y = g2x = g2y = 0.0;
+  int c1 = 0, c2 = 0;
+  const int es[2] = {sr.z, sr.w};
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    const int e = es[j];
+    if (e < 0) continue;
+    const int4 r0 = m.elem[2 * e];
+    const double* gp = m.gradphi + 6 * (int64_t)e;
+    const double p0x = gp[0], p0y = gp[1], p1x = gp[2], p1y = gp[3], p2x = gp[4], p2y = gp[5];
+    if (m.elem_wet[e]) {
+      const double f0 = m.eta1[r0.x], f1 = m.eta1[r0.y], f2 = m.eta1[r0.z];
+      g1x += f0 * p0x + f1 * p1x + f2 * p2x;
+      g1y += f0 * p0y + f1 * p1y + f2 * p2y;
+      ++c1;
+    }
+    const double f0 = m.eta2[r0.x], f1 = m.eta2[r0.y], f2 = m.eta2[r0.z];
+    const bool dry2 = f0 + m.depth[r0.x] <= h0 || f1 + m.depth[r0.y] <= h0 || f2 + m.depth[r0.z] <= h0;
+    if (!dry2) {
+      g2x += f0 * p0x + f1 * p1x + f2 * p2x;
+      g2y += f0 * p0y + f1 * p1y + f2 * p2y;
+      ++c2;
+    }
+  }
+  if (c1 > 0) {
+    g1x = g1x * (1.0 / c1);
+    g1y = g1y * (1.0 / c1);
+  }
+  if (c2 > 0) {
+    g2x = g2x * (1.0 / c2);
+    g2y = g2y * (1.0 / c2);
+  }
+}
+
+// solveMomentumEquations (meshdata.cu:5074-5352) for one side, barotropic, itur=0 (dfv = dfv0),
+// iveg=0, ics=1, imm=0 and no Coriolis, wind, horizontal viscosity, pressure or tidal potential
+// (zero terms omitted), then applyMomentumBC for land sides (meshdata.cu:5356-5446, islip=0).
+// zc and u are the side's rows in LDS; u holds btu on entry and u2 on exit. The 3D case is
+// tridag_sch<2> (meshdata.cu:3665-3711) with the coefficients formed on the fly: the forward
+// sweep keeps u_j in the btu slot k-1 and gam_j in the zc slot k-1 (both dead once row j is
+// formed), the back substitution runs over those slots, and the result moves up one slot.
+__device__ void momentum_side(const DMesh& m, const StepArgs& a, int s, double* zc, double* u) {
+  const int nv = m.nvrt;
+  const int kb = m.side_kb[s];
+  const int4 sr = m.side[s];
+  const double rx = a.rmaxvel, ry = a.rmaxvel * 1.013;
+  if (!m.side_wet[s]) {
+    for (int k = 0; k < 2 * nv; ++k) u[k] = 0.0;
+  } else {
+    double g1x, g1y, g2x, g2y;
+    side_eta_grads(m, sr, a.h0, g1x, g1y, g2x, g2y);
+    const double grav3 = 0.5 * (m.grav2[sr.x] + m.grav2[sr.y]);
+    const double Cd = 0.5 * (m.cd[sr.x] + m.cd[sr.y]);
+    const double dt = a.dt, th = a.theta;
+    if (kb == nv - 2) {
+      const double htot = zc[nv - 1] - zc[kb];
+      const double hhat = m.hhat[s];
+      const double hx = u[2 * (nv - 1)] + dt * (0.0 - grav3 * ((1.0 - th) * g1x + th * g2x));
+      const double hy = u[2 * (nv - 1) + 1] + dt * (0.0 - grav3 * ((1.0 - th) * g1y + th * g2y));
+      const double vx = fmin(fmax(hx * hhat / htot, -rx), rx), vy = fmin(fmax(hy * hhat / htot, -ry), ry);
+      for (int k = 0; k < nv; ++k) {
+        u[2 * k] = vx;
+        u[2 * k + 1] = vy;
+      }
+    } else {
+      const double Gx = grav3 * (th * g2x + (1.0 - th) * g1x), Gy = grav3 * (th * g2y + (1.0 - th) * g1y);
+      const double chi2 = m.chi[s], dfv0 = a.dfv0;
+      double zm = zc[kb], z0 = zc[kb + 1];                                          // zc[k-1], zc[k]
+      double bmx = u[2 * kb], bmy = u[2 * kb + 1], b0x = u[2 * kb + 2], b0y = u[2 * kb + 3];  // btu[k-1], btu[k]
+      double bet = 1.0, cprev = 0.0, upx = 0.0, upy = 0.0;
+      for (int k = kb + 1; k < nv; ++k) {
+        const bool top = k < nv - 1;
+        const double zp = top ? zc[k + 1] : 0.0;
+        const double bpx = top ? u[2 * (k + 1)] : 0.0, bpy = top ? u[2 * (k + 1) + 1] : 0.0;
+        const double dz_0 = z0 - zm;
+        const double dz_1 = top ? zp - z0 : 0.0;
+        double aj = 0.0, bj = 0.0, cj = 0.0;
+        if (top) {
+          const double tmp = dt * dfv0 / dz_1;
+          cj += 1.0 * dz_1 / 6.0 - tmp;
+          bj += 1.0 * dz_1 / 3.0 + tmp;
+        }
+        if (k > kb + 1) {
+          const double tmp = dt * dfv0 / dz_0;
+          aj += 1.0 * dz_0 / 6.0 - tmp;
+          bj += 1.0 * dz_0 / 3.0 + tmp;
+        } else {
+          bj += dt * chi2;
+        }
+        double dz = 0.0;
+        dz += dz_1;
+        if (k > kb + 1) dz += dz_0;
+        double qx = -(0.5 * dz) * dt * Gx, qy = -(0.5 * dz) * dt * Gy;
+        if (top) {
+          qx += (dz_1 / 6.0) * (2.0 * b0x + bpx);
+          qy += (dz_1 / 6.0) * (2.0 * b0y + bpy);
+        }
+        if (k > kb + 1) {
+          qx += (dz_0 / 6.0) * (2.0 * b0x + bmx);
+          qy += (dz_0 / 6.0) * (2.0 * b0y + bmy);
+        }
+        double ux, uy;
+        if (k == kb + 1) {
+          bet = bj;
+          ux = qx / bet;
+          uy = qy / bet;
+        } else {
+          const double gam = cprev / bet;
+          bet = bj - aj * gam;
+          ux = (qx - aj * upx) / bet;
+          uy = (qy - aj * upy) / bet;
+          zc[k - 1] = gam;
+        }
+        u[2 * (k - 1)] = ux;
+        u[2 * (k - 1) + 1] = uy;
+        upx = ux;
+        upy = uy;
+        cprev = cj;
+        zm = z0;
+        z0 = zp;
+        bmx = b0x;
+        bmy = b0y;
+        b0x = bpx;
+        b0y = bpy;
+      }
+      // back substitution: u_j (slot kb + j) -= gam_{j+1} (zc slot kb + j + 1) u_{j+1}
+      for (int k = nv - 3; k >= kb; --k) {
+        const double gam = zc[k + 1];
+        upx = u[2 * k] - gam * upx;
+        upy = u[2 * k + 1] - gam * upy;
+        u[2 * k] = upx;
+        u[2 * k + 1] = upy;
+      }
+      for (int k = nv - 1; k > kb; --k) {
+        u[2 * k] = fmin(fmax(u[2 * (k - 1)], -rx), rx);
+        u[2 * k + 1] = fmin(fmax(u[2 * (k - 1) + 1], -ry), ry);
+      }
+    }
+    // bottom condition, imm = 0
+    if (Cd == 0.0) {
+      u[2 * kb] = u[2 * (kb + 1)];
+      u[2 * kb + 1] = u[2 * (kb + 1) + 1];
+    } else if (kb < nv - 2) {
+      u[2 * kb] = 0.0;
+      u[2 * kb + 1] = 0.0;
+    }
+    for (int k = 0; k < kb; ++k) u[2 * k] = u[2 * k + 1] = 0.0;
+  }
+  if (m.side_type[s] == 1) {
+    // free slip: the tangential component, t = (-n.y, n.x), n from populateSides (meshdata.cu:2283-2287)
+    const double vx = -(m.y[sr.x] - m.y[sr.y]), vy = m.x[sr.x] - m.x[sr.y];
+    const double len = sqrt(vx * vx + vy * vy);
+    const double nx = vx / len, ny = vy / len;
+    const double tx = -ny, ty = nx;
+    for (int k = kb; k < nv; ++k) {
+      const double vtan = u[2 * k] * tx + u[2 * k + 1] * ty;
+      u[2 * k] = vtan * tx;
+      u[2 * k + 1] = vtan * ty;
+    }
+  }
+}
+
+// A workgroup of 64 takes g consecutive sides: their zc and btu rows come into LDS with flat
+// coalesced loads, one thread per side solves in LDS, the u2 rows leave with flat stores.
+__global__ void __launch_bounds__(64) k_momentum(DMesh m, StepArgs a, int g) {
+  extern __shared__ double lds[];
+  const int nv = m.nvrt;
+  const int zs = nv | 1, us = (2 * nv) | 1;
+  double* Lz = lds;
+  double* Lu = lds + g * zs;
+  const int s0 = blockIdx.x * g;
+  const int gc = min(g, m.ns - s0);
+  stage_rows(Lz, zs, m.zc + (int64_t)s0 * nv, nv, gc);
+  stage_rows(Lu, us, m.btu + 2 * (int64_t)s0 * nv, 2 * nv, gc);
+  __syncthreads();
+  const int t = threadIdx.x;
+  if (t < gc) momentum_side(m, a, s0 + t, Lz + t * zs, Lu + t * us);
+  __syncthreads();
+  unstage_rows(m.su2 + 2 * (int64_t)s0 * nv, Lu, us, 2 * nv, gc);
+}
+
+void launch_momentum(const DMesh& m, const StepArgs& a, hipStream_t st) {
+  if (m.ns <= 0) return;
+  // sides per workgroup: one per thread, as many as 32 KB of LDS holds
+  const int row = 8 * ((m.nvrt | 1) + ((2 * m.nvrt) | 1));
+  const int g = max(1, min(64, (32 * 1024) / row));
+  hipLaunchKernelGGL(k_momentum, dim3((unsigned)((m.ns + g - 1) / g)), dim3(64), (size_t)g * row, st, m, a, g);
+}
+
 // ---------------------------------------------------------------- a11+a12: assembly + BC
 // setupCoefficientMatrixAndRHS for triangles, imm=0 (meshdata.cu:3182-3311), fused with
 // setEssentialBC (meshdata.cu:4400-4446), in two passes. k_elem_assemble evaluates, once per

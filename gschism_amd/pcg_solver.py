@@ -177,6 +177,12 @@ def setupCoefficientMatrixAndRHS(mesh: Mesh, solver: PCGSolverMultiGPU, params, 
     solver._solver.assemble(params, time)
 
 
+def solveMomentumEquations(mesh: Mesh, params, time: float = 0.0) -> None:
+    """computeHorGradientAtSide + solveMomentumEquations + applyMomentumBC (schism_step.py:280-288,
+    meshdata.cu:5045-5446): the side velocities of the next step from the new elevation."""
+    mesh.momentum(params)
+
+
 def updateElevation(mesh: Mesh, solver: PCGSolverMultiGPU) -> None:
     """rollUpElevation + updateElevation (meshdata.cu:3538-3544, 4449-4456)."""
     solver._solver.update_elevation()
@@ -189,5 +195,6 @@ def hot_path_step(mesh: Mesh, solver: PCGSolverMultiGPU, params, maxIter=1500, t
     setupCoefficientMatrixAndRHS(mesh, solver, params)
     iters = solver.solve(rtol=tol, maxiter=maxIter)
     updateElevation(mesh, solver)
+    solveMomentumEquations(mesh, params)
     mesh.update_levels(params)
     return iters

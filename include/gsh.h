@@ -168,6 +168,7 @@ typedef struct gsh_step_params {
   int32_t iettype;    /* elevation BC type of the open boundary: 1,2,4 = eth; 0 = none */
   double rmaxvel;     /* velocity clamp of the nodal reconstruction: |u| <= rmaxvel,
                          |v| <= 1.013 rmaxvel (constants_template.h:142-144)     */
+  double dfv0;        /* vertical viscosity of the momentum solve, itur=0 [m^2/s]  */
 } gsh_step_params;
 void gsh_step_params_default(gsh_step_params* p);
 
@@ -195,6 +196,15 @@ int gsh_rhs_producers(gsh_mesh* mesh, const gsh_step_params* p);
  * GSH_F_ELEM_W, with the wet flags and levels of the last gsh_update_levels. The reference adds the
  * element contributions with atomics; here each node sums its ball elements in ball order. */
 int gsh_nodal_velocity(gsh_mesh* mesh, const gsh_step_params* p);
+
+/* The step after the solve (call after gsh_update_elevation, before gsh_update_levels):
+ * computeHorGradientAtSide, solveMomentumEquations and applyMomentumBC (meshdata.cu:5045-5446)
+ * for the barotropic, itur=0 (dfv0), iveg=0, ics=1, imm=0, islip=0 subset without Coriolis,
+ * wind, horizontal viscosity, pressure or tidal potential: GSH_F_SIDE_U2 from eta1/eta2, btu,
+ * chi and H^ (gsh_rhs_producers) and the levels and wet flags of the last gsh_update_levels.
+ * 3D sides solve their vertical tridiagonal system, 2D sides the depth-averaged update; land
+ * boundary sides keep the tangential component. */
+int gsh_momentum(gsh_mesh* mesh, const gsh_step_params* p);
 
 /* ---------------------------------------------------------------- grid files (host side)
  * hgrid.gr3 as readHGrid reads it (read_and_process_grid.cpp:855-1070): node coordinates and

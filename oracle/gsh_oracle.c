@@ -629,6 +629,188 @@ void or_nodal_velocity(or_mesh* m, const or_params* p, int nthreads) {
   }
 }
 
+/* ------------------------------------------------------------------ momentum */
+/* Element::grad2 (meshdata.h:1105-1114), triangles: sum_l f_l gradphi_l */
+static void elem_grad(const or_mesh* m, int e, const double* f, double g[2]) {
+  const int32_t* nd = m->elnode + 3 * e;
+  const double* gp = m->gradphi + 6 * e;
+  g[0] = f[nd[0]] * gp[0] + f[nd[1]] * gp[2] + f[nd[2]] * gp[4];
+  g[1] = f[nd[0]] * gp[1] + f[nd[1]] * gp[3] + f[nd[2]] * gp[5];
+}
+
+/* Element::isDry2 (meshdata.cu:774-778) from Node::isDry2 (meshdata.h:1757): eta2 + depth <= h0 */
+static int elem_dry2(const or_mesh* m, int e, double h0) {
+  const int32_t* nd = m->elnode + 3 * e;
+  for (int l = 0; l < 3; ++l)
+    if (m->eta2[nd[l]] + m->depth[nd[l]] <= h0) return 1;
+  return 0;
+}
+
+/* computeHorGradientAtSide (meshdata.cu:5045-5071) with Side::grad2 (meshdata.cu:1363-1398):
+ * gradEta1 over the side's elements that are wet (isDry of the last level update), gradEta2 over
+ * the elements wet by eta2 (isDry2), each the mean of the element gradients; gradETP and gradPr
+ * are zero in the supported subset (no tidal potential, no atmospheric pressure). */
+static void side_grads(const or_mesh* m, int s, double h0, double g1[2], double g2[2]) {
+  g1[0] = g1[1] = g2[0] = g2[1] = 0.0;
+  int c1 = 0, c2 = 0;
+  for (int j = 0; j < 2; ++j) {
+    const int e = m->isdel[2 * s + j];
+    if (e < 0) continue;
+    double g[2];
+    if (!m->elem_dry[e]) {
+      elem_grad(m, e, m->eta1, g);
+      g1[0] += g[0];
+      g1[1] += g[1];
+      ++c1;
+    }
+    if (!elem_dry2(m, e, h0)) {
+      elem_grad(m, e, m->eta2, g);
+      g2[0] += g[0];
+      g2[1] += g[1];
+      ++c2;
+    }
+  }
+  if (c1 > 0) {
+    g1[0] = g1[0] * (1.0 / c1);
+    g1[1] = g1[1] * (1.0 / c1);
+  }
+  if (c2 > 0) {
+    g2[0] = g2[0] * (1.0 / c2);
+    g2[1] = g2[1] * (1.0 / c2);
+  }
+}
+
+/* computeHorGradientAtSide + solveMomentumEquations (meshdata.cu:5045-5352) + applyMomentumBC
+ * (meshdata.cu:5356-5446) for the barotropic (ibc=1), itur=0 (dfv = dfv0 at every node and
+ * level), iveg=0, ics=1, imm=0, islip=0 subset with no Coriolis, wind stress, horizontal
+ * viscosity, atmospheric pressure or tidal potential (terms that are zero there are omitted;
+ * they could only change the sign of a zero). Per side:
+ *   dry: u2 = 0.
+ *   2D (kb = nvrt-2): u2[k] = clamp((btu[nvrt-1] - dt grav3 ((1-th) gradEta1 + th gradEta2)) Hhat / htot1)
+ *       for every k, htot1 = zc[nvrt-1] - zc[kb].
+ *   3D: the Galerkin tridiagonal system over k = kb+1 .. nvrt-1 (mass dz/6, dz/3; viscosity
+ *       dt dfv0 / dz; bottom drag dt chi2 on the first row; right-hand side -(dz/2) dt grad2D +
+ *       the mass-weighted btu) solved by tridag_sch (meshdata.cu:3665-3711), u2[k] = clamp(soln);
+ *       u2[kb] = u2[kb+1] when Cd = 0, else 0. (The reference's loop also assigns k = kb from
+ *       soln[-1], which the bottom condition then overwrites; it is not restated.)
+ *   k < kb: 0. Land boundary sides then keep only the tangential component for k >= kb. */
+void or_momentum(or_mesh* m, const or_params* p) {
+  const int ns = m->ns, nv = m->nvrt;
+  const double dt = p->dt, th = p->theta, rx = p->rmaxvel, ry = p->rmaxvel * 1.013, dfv0 = p->dfv0;
+  double* a = xcalloc(nv, sizeof(double));
+  double* b = xcalloc(nv, sizeof(double));
+  double* c = xcalloc(nv, sizeof(double));
+  double* r = xcalloc(2 * (size_t)nv, sizeof(double));
+  double* u = xcalloc(2 * (size_t)nv, sizeof(double));
+  double* gam = xcalloc(nv, sizeof(double));
+  for (int s = 0; s < ns; ++s) {
+    double* u2 = m->su2 + 2 * (size_t)s * nv;
+    const double* bt = m->btu + 2 * (size_t)s * nv;
+    const double* zc = m->zc + (size_t)s * nv;
+    const int kb = m->side_kb[s];
+    const int n1 = m->isidenode[2 * s], n2 = m->isidenode[2 * s + 1];
+    if (m->side_dry[s]) {
+      for (int k = 0; k < 2 * nv; ++k) u2[k] = 0.0;
+    } else {
+      double g1[2], g2[2];
+      side_grads(m, s, p->h0, g1, g2);
+      const double grav3 = 0.5 * (m->grav2[n1] + m->grav2[n2]);
+      const double Cd = 0.5 * (m->cd[n1] + m->cd[n2]);
+      if (kb == nv - 2) {
+        const double htot = zc[nv - 1] - zc[kb];
+        const double hhat = m->hhat[s];
+        double hg[2];
+        for (int l = 0; l < 2; ++l)
+          hg[l] = bt[2 * (nv - 1) + l] + dt * (0.0 - grav3 * ((1.0 - th) * g1[l] + th * g2[l]));
+        const double vx = clamp_v(hg[0] * hhat / htot, rx), vy = clamp_v(hg[1] * hhat / htot, ry);
+        for (int k = 0; k < nv; ++k) {
+          u2[2 * k] = vx;
+          u2[2 * k + 1] = vy;
+        }
+      } else {
+        const int nr = nv - kb - 1;
+        for (int k = kb + 1; k < nv; ++k) {
+          const int kin = k - (kb + 1);
+          a[kin] = 0.0;
+          c[kin] = 0.0;
+          b[kin] = 0.0;
+          const double cff1 = 1.0;
+          if (k < nv - 1) {
+            const double dzz = zc[k + 1] - zc[k];
+            const double tmp = dt * dfv0 / dzz;
+            c[kin] += cff1 * dzz / 6.0 - tmp;
+            b[kin] += cff1 * dzz / 3.0 + tmp;
+          }
+          if (k > kb + 1) {
+            const double dzz = zc[k] - zc[k - 1];
+            const double tmp = dt * dfv0 / dzz;
+            a[kin] += cff1 * dzz / 6.0 - tmp;
+            b[kin] += cff1 * dzz / 3.0 + tmp;
+          } else {
+            b[kin] += dt * m->chi[s];
+          }
+        }
+        double G[2];
+        for (int l = 0; l < 2; ++l) G[l] = grav3 * (th * g2[l] + (1.0 - th) * g1[l]);
+        for (int k = kb + 1; k < nv; ++k) {
+          const int kin = k - kb - 1;
+          const double dz_0 = zc[k] - zc[k - 1];
+          const double dz_1 = (k < nv - 1) ? zc[k + 1] - zc[k] : 0.0;
+          double dz = 0.0;
+          dz += dz_1;
+          if (k > kb + 1) dz += dz_0;
+          for (int l = 0; l < 2; ++l) {
+            double v = -(0.5 * dz) * dt * G[l];
+            if (k < nv - 1) v += (dz_1 / 6.0) * (2.0 * bt[2 * k + l] + bt[2 * (k + 1) + l]);
+            if (k > kb + 1) v += (dz_0 / 6.0) * (2.0 * bt[2 * k + l] + bt[2 * (k - 1) + l]);
+            r[2 * kin + l] = v;
+          }
+        }
+        /* tridag_sch<2> */
+        double bet = b[0];
+        u[0] = r[0] / bet;
+        u[1] = r[1] / bet;
+        for (int j = 1; j < nr; ++j) {
+          gam[j] = c[j - 1] / bet;
+          bet = b[j] - a[j] * gam[j];
+          for (int l = 0; l < 2; ++l) u[2 * j + l] = (r[2 * j + l] - a[j] * u[2 * (j - 1) + l]) / bet;
+        }
+        for (int j = nr - 2; j >= 0; --j)
+          for (int l = 0; l < 2; ++l) u[2 * j + l] = u[2 * j + l] - gam[j + 1] * u[2 * (j + 1) + l];
+        for (int k = kb + 1; k < nv; ++k) {
+          const int kin = k - kb - 1;
+          u2[2 * k] = clamp_v(u[2 * kin], rx);
+          u2[2 * k + 1] = clamp_v(u[2 * kin + 1], ry);
+        }
+      }
+      /* bottom condition, imm = 0 */
+      if (Cd == 0.0) {
+        u2[2 * kb] = u2[2 * (kb + 1)];
+        u2[2 * kb + 1] = u2[2 * (kb + 1) + 1];
+      } else if (kb < nv - 2) {
+        u2[2 * kb] = 0.0;
+        u2[2 * kb + 1] = 0.0;
+      }
+      for (int k = 0; k < kb; ++k) u2[2 * k] = u2[2 * k + 1] = 0.0;
+    }
+    /* applyMomentumBC: land sides, free slip (islip = 0): u2 = (u2 . t) t, t = (-n.y, n.x) */
+    if (m->side_type[s] == 1) {
+      const double tx = -m->sny[s], ty = m->snx[s];
+      for (int k = kb; k < nv; ++k) {
+        const double vtan = u2[2 * k] * tx + u2[2 * k + 1] * ty;
+        u2[2 * k] = vtan * tx;
+        u2[2 * k + 1] = vtan * ty;
+      }
+    }
+  }
+  free(a);
+  free(b);
+  free(c);
+  free(r);
+  free(u);
+  free(gam);
+}
+
 /* ------------------------------------------------------------------ backtracking */
 typedef struct {
   double x, y;

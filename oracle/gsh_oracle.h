@@ -19,6 +19,7 @@ typedef struct or_params {
   double dt, theta, h0, ramp, dtb_min, dtb_max, velmin_btrack;
   int32_t ihhat, iettype;
   double rmaxvel;
+  double dfv0;
 } or_params;
 
 const char* or_error(void);
@@ -42,6 +43,8 @@ void or_backtrack(or_mesh* m, const or_params* p, const int32_t* list_side, cons
 void or_rhs(or_mesh* m, const or_params* p);
 /* Nodal velocities (field 2) from side velocities (field 3) and element w (field 20), ball order. */
 void or_nodal_velocity(or_mesh* m, const or_params* p, int nthreads);
+/* side velocities u2 of the next step: elevation gradients, momentum solve, land-boundary slip */
+void or_momentum(or_mesh* m, const or_params* p);
 /* Assembly + essential BC into val (CSR of the mesh) and b. */
 void or_assemble(or_mesh* m, const or_params* p, double* val, double* b, int nthreads);
 

@@ -16,7 +16,8 @@ LIB_PATH = HERE / "_build" / "liboracle.so"
 class OrParams(C.Structure):
     _fields_ = [("dt", C.c_double), ("theta", C.c_double), ("h0", C.c_double), ("ramp", C.c_double),
                 ("dtb_min", C.c_double), ("dtb_max", C.c_double), ("velmin_btrack", C.c_double),
-                ("ihhat", C.c_int32), ("iettype", C.c_int32), ("rmaxvel", C.c_double)]
+                ("ihhat", C.c_int32), ("iettype", C.c_int32), ("rmaxvel", C.c_double),
+                ("dfv0", C.c_double)]
 
 
 _P = C.c_void_p
@@ -36,6 +37,7 @@ _SIG = {
     "or_backtrack": (None, [_P, C.POINTER(OrParams), _PI32, _PI32, C.c_int64, C.c_int]),
     "or_rhs": (None, [_P, C.POINTER(OrParams)]),
     "or_nodal_velocity": (None, [_P, C.POINTER(OrParams), C.c_int]),
+    "or_momentum": (None, [_P, C.POINTER(OrParams)]),
     "or_assemble": (None, [_P, C.POINTER(OrParams), _PD, _PD, C.c_int]),
     "or_pcg": (C.c_int32, [C.c_int32, _PI32, _PI32, _PD, _PD, _PD, _PD, _PD, C.c_double, C.c_int32, C.c_int32,
                            C.c_int32, _PD, _PD, _PD, C.c_int]),
@@ -184,6 +186,9 @@ class OracleMesh:
 
     def nodal_velocity(self, p, nthreads: int = 1) -> None:
         lib().or_nodal_velocity(self.h, C.byref(params(p)), nthreads)
+
+    def momentum(self, p) -> None:
+        lib().or_momentum(self.h, C.byref(params(p)))
 
     def assemble(self, p, nthreads: int = 1):
         val = np.zeros(self.nnz, dtype=np.float64)

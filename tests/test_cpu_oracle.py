@@ -561,3 +561,143 @@ def test_bottom_level_range_is_checked():
         core.Mesh(None, xy, en, np.full(xy.shape[0], 5.0), cases.sigma_levels(5), open_bnd=ob, kb=kb)
     kb[3] = 3
     O.OracleMesh(xy, en, np.full(xy.shape[0], 5.0), ob, cases.sigma_levels(5), kb=kb)
+
+
+# ---------------------------------------------------------------- momentum
+def _momentum_state(nvrt, kb=None, depth=10.0, cd=0.0, btu=None, seed=0, eta_old=None, eta_new=None):
+    """Oracle mesh after levels (old elevation), the RHS producers (chi, H^) and the elevation
+    update (eta1 = old, eta2 = new), ready for the momentum step."""
+    m, xy, en, ob = _mesh(nx=18, ny=12, nvrt=nvrt, kb=kb, depth=depth)
+    n, nv = xy.shape[0], nvrt
+    xyo = m.topology(L.T_XY)
+    e_old = eta_old(xyo) if eta_old else np.zeros(n)
+    e_new = eta_new(xyo) if eta_new else e_old
+    p = _params(dfv0=0.05, rmaxvel=100.0)
+    m.set(L.F_ETA2, e_old)
+    m.set(L.F_CD, np.full(n, cd))
+    m.set(L.F_GRAV2, np.full(n, 9.81))
+    m.levels(p)
+    if btu is None:
+        rng = np.random.default_rng(seed)
+        btu = rng.normal(0.0, 0.3, (m.ns, nv, 2))
+    m.set(L.F_BTU, btu)
+    m.set(L.F_SIDE_U2, np.zeros((m.ns, nv, 2)))
+    m.rhs(p)
+    m.set(L.F_ETA1, e_old)
+    m.set(L.F_ETA2, e_new)
+    return m, p, btu
+
+
+def _side_info(m):
+    n, ne = m.n, m.ne
+    wet = m.get(L.F_WET)
+    sn = m.topology(L.T_ISIDENODE)
+    kbn = m.topology(L.T_KB)
+    swet = wet[n + ne:] != 0
+    skb = np.where(swet, np.minimum(kbn[sn[:, 0]], kbn[sn[:, 1]]), 0)
+    return swet, skb, m.topology(L.T_SIDE_TYPE), m.geometry(4)
+
+
+def test_momentum_uniform_flow_linear_elevation():
+    """Cd = 0, uniform btu = u0 at every level, linear old and new elevations: every wet side,
+    3D (any dfv0) or 2D (H^ = H), gets u0 - dt g (theta grad eta2 + (1 - theta) grad eta1) at
+    and above its bottom level (u2[kb] = u2[kb+1] without drag), zero below; land sides keep the
+    tangential component of that value, open and internal sides all of it."""
+    u0 = np.array([0.4, -0.25])
+    a1, a2 = np.array([2e-5, -1e-5]), np.array([-3e-5, 4e-5])
+    for nvrt, kb in ((6, None), (2, None), (7, "mixed")):
+        mm, xy, en, _ = _mesh(nx=18, ny=12, nvrt=nvrt)
+        kbv = None if kb is None else (np.arange(xy.shape[0]) % 4).astype(np.int32)
+        btu = np.broadcast_to(u0, (mm.ns, nvrt, 2)).copy()
+        m, p, _ = _momentum_state(nvrt, kb=kbv, btu=btu, eta_old=lambda q: 0.1 + q @ a1,
+                                  eta_new=lambda q: 0.12 + q @ a2)
+        m.momentum(p)
+        got = m.get(L.F_SIDE_U2)
+        swet, skb, stype, snrm = _side_info(m)
+        ref = u0 - p.dt * 9.81 * (p.theta * a2 + (1 - p.theta) * a1)
+        tang = np.stack([-snrm[:, 1], snrm[:, 0]], axis=1)
+        checked = 0
+        for s in range(m.ns):
+            if not swet[s]:
+                assert np.all(got[s] == 0.0)
+                continue
+            exp = ref if stype[s] != 1 else (ref @ tang[s]) * tang[s]
+            assert np.allclose(got[s, skb[s]:], exp, rtol=1e-12, atol=1e-14), (nvrt, s)
+            assert np.all(got[s, :skb[s]] == 0.0)
+            checked += 1
+        assert checked > 0.9 * m.ns
+        if kb is not None:
+            assert len(set(skb.tolist())) > 2
+
+
+def test_momentum_matches_galerkin_assembly():
+    """3D sides with bottom drag, vertical viscosity and random btu profiles: the oracle's
+    tridiagonal rows against the Galerkin system assembled layer by layer (linear elements on
+    zc[kb+1..nvrt-1]: mass dz [[1/3, 1/6], [1/6, 1/3]], stiffness [[1, -1], [-1, 1]] / dz), drag
+    dt chi on the first unknown, gradient from plane fits of the elevations, solved densely."""
+    nv = 8
+    mm, xy, en, _ = _mesh(nx=18, ny=12, nvrt=nv)
+    kbv = (np.arange(xy.shape[0]) % 3).astype(np.int32)
+    cd = 2.5e-3
+    m, p, btu = _momentum_state(nv, kb=kbv, cd=cd, seed=4, eta_old=lambda q: 0.05 * np.sin(q[:, 0] / 900.0),
+                                eta_new=lambda q: 0.07 * np.cos(q[:, 1] / 700.0))
+    m.momentum(p)
+    got = m.get(L.F_SIDE_U2)
+    swet, skb, stype, _ = _side_info(m)
+    zc = m.get(L.F_ZC)
+    xyo, eno = m.topology(L.T_XY), m.topology(L.T_ELNODE)
+    isdel, sn = m.topology(L.T_ISDEL), m.topology(L.T_ISIDENODE)
+    wet = m.get(L.F_WET)
+    eta1, eta2, dep = m.get(L.F_ETA1), m.get(L.F_ETA2), m.topology(L.T_DEPTH)
+    elem_wet = wet[m.n:m.n + m.ne] != 0
+
+    def plane_grad(e, f):
+        A = np.c_[np.ones(3), xyo[eno[e]]]
+        return np.linalg.solve(A, f[eno[e]])[1:]
+
+    count = 0
+    for s in range(m.ns):
+        if not swet[s] or stype[s] == 1 or skb[s] == nv - 2:
+            continue
+        g1, g2, c1, c2 = np.zeros(2), np.zeros(2), 0, 0
+        for e in isdel[s]:
+            if e < 0:
+                continue
+            if elem_wet[e]:
+                g1 += plane_grad(e, eta1)
+                c1 += 1
+            if np.all(eta2[eno[e]] + dep[eno[e]] > p.h0):
+                g2 += plane_grad(e, eta2)
+                c2 += 1
+        g1, g2 = g1 / max(c1, 1), g2 / max(c2, 1)
+        G = 9.81 * (p.theta * g2 + (1 - p.theta) * g1)
+        k0 = skb[s] + 1
+        z = zc[s, k0:]
+        nr = z.size
+        M, K = np.zeros((nr, nr)), np.zeros((nr, nr))
+        for j in range(nr - 1):
+            dz = z[j + 1] - z[j]
+            M[j:j + 2, j:j + 2] += dz * np.array([[1 / 3, 1 / 6], [1 / 6, 1 / 3]])
+            K[j:j + 2, j:j + 2] += np.array([[1.0, -1.0], [-1.0, 1.0]]) / dz
+        chi = cd * np.hypot(*btu[s, k0])
+        A = M + p.dt * p.dfv0 * K
+        A[0, 0] += p.dt * chi
+        rhs = M @ btu[s, k0:] - p.dt * np.outer(M.sum(axis=1), G)
+        sol = np.linalg.solve(A, rhs)
+        assert np.allclose(got[s, k0:], sol, rtol=1e-10, atol=1e-13), s
+        assert np.all(got[s, skb[s]] == 0.0) and np.all(got[s, :skb[s]] == 0.0)
+        count += 1
+    assert count > 100
+
+
+def test_momentum_land_sides_have_no_normal_flow():
+    m, p, _ = _momentum_state(6, cd=2.5e-3, seed=9, eta_old=lambda q: 1e-5 * q[:, 0],
+                              eta_new=lambda q: -2e-5 * q[:, 1])
+    m.momentum(p)
+    got = m.get(L.F_SIDE_U2)
+    swet, skb, stype, snrm = _side_info(m)
+    land = np.flatnonzero(stype == 1)
+    assert land.size > 10
+    un = np.einsum("skc,sc->sk", got[land], snrm[land])
+    assert np.abs(un).max() < 1e-15
+    assert np.abs(got[land]).max() > 1e-3

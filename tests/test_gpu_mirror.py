@@ -77,6 +77,8 @@ def test_hot_path_step_matches_manual_sequence(ctx):
         sb.assemble(p)
         _, ib, _ = sb.solve(None, rtol=1e-12, maxiter=1500)
         sb.update_elevation()
+        mesh_b.momentum(p)
         mesh_b.update_levels(p)
         assert ia == ib
         assert_bitwise(mesh_a.get(L.F_ETA2), mesh_b.get(L.F_ETA2), f"step {step} eta2")
+        assert_bitwise(mesh_a.get(L.F_SIDE_U2), mesh_b.get(L.F_SIDE_U2), f"step {step} side velocity")

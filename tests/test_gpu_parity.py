@@ -63,6 +63,14 @@ def _full_chain(ctx, mesh, om, p, rows_per_thread=4):
     assert rel < 1e-10
     stt = solver.stats()
     assert stt.atol2 == at and stt.rr == rr
+    # the step after the solve: new elevation, then the momentum solve for the side velocities
+    eta_old = mesh.get(L.F_ETA2)
+    solver.update_elevation()
+    om.set(L.F_ETA1, eta_old)
+    om.set(L.F_ETA2, xo)
+    mesh.momentum(p)
+    om.momentum(p)
+    assert_bitwise(mesh.get(L.F_SIDE_U2), om.get(L.F_SIDE_U2), "side velocity from the momentum step")
     return solver, x, val, b
 
 
@@ -192,7 +200,12 @@ def test_three_steps_with_history(ctx):
         solver.update_elevation()
         assert_bitwise(mesh.get(L.F_ETA1), eta_old, "eta1 roll")
         assert_bitwise(mesh.get(L.F_ETA2), xo, "eta2 update")
+        om.set(L.F_ETA1, eta_old)
         om.set(L.F_ETA2, xo)
+        # side velocities of the next step (they feed its vertically integrated velocity)
+        mesh.momentum(p)
+        om.momentum(p)
+        assert_bitwise(mesh.get(L.F_SIDE_U2), om.get(L.F_SIDE_U2), f"step {step} side velocity")
         mesh.update_levels(p)
         om.levels(p)
         assert_bitwise(mesh.get(L.F_ZC), om.get(L.F_ZC), f"step {step} zc")
@@ -395,7 +408,40 @@ def test_full_size_1m_channel(ctx):
     assert st == 0 and it == ito, (it, ito, st)
     assert_bitwise(x, xo, "1M PCG solution")
     assert solver.check() < 1e-10
+    print("[gpu] 1M momentum", flush=True)
+    solver.update_elevation()
+    om.set(L.F_ETA1, eta)
+    om.set(L.F_ETA2, xo)
+    mesh.momentum(p)
+    om.momentum(p)
+    assert_bitwise(mesh.get(L.F_SIDE_U2), om.get(L.F_SIDE_U2), "1M side velocity from the momentum step")
     solver.close()
+    mesh.close()
+
+
+@pytest.mark.parametrize("nvrt", [200, 1024])
+def test_momentum_many_levels(ctx, nvrt):
+    """Deep columns (a workgroup holds fewer sides, down to one at 1024 levels), random btu,
+    node-varying bottom levels and a new elevation with a dry corner: side velocities bit for bit."""
+    spec = cases.CaseSpec(f"deep-{nvrt}", 14, 9, 500.0, nvrt=nvrt, depth=10.0, substeps=4)
+    mesh, om, p = _setup(ctx, spec, kb_fn=lambda d: (np.arange(d.size) % 5).astype(np.int32))
+    rng = np.random.default_rng(nvrt)
+    btu = rng.normal(0.0, 0.4, (mesh.ns, nvrt, 2))
+    eta_old = mesh.get(L.F_ETA2)
+    xy = mesh.topology(L.T_XY)
+    eta_new = eta_old + 0.02 * np.sin(xy[:, 0] / 800.0)
+    eta_new[np.argmin(xy[:, 0] + xy[:, 1])] = -11.0  # drier than the depth: its elements leave gradEta2
+    print(f"[gpu] momentum nvrt={nvrt} ns={mesh.ns}", flush=True)
+    for m in (mesh, om):
+        m.set(L.F_BTU, btu)
+    mesh.rhs_producers(p)
+    om.rhs(p)
+    for m in (mesh, om):
+        m.set(L.F_ETA1, eta_old)
+        m.set(L.F_ETA2, eta_new)
+    mesh.momentum(p)
+    om.momentum(p)
+    assert_bitwise(mesh.get(L.F_SIDE_U2), om.get(L.F_SIDE_U2), f"side velocity, {nvrt} levels")
     mesh.close()
 
 
