@@ -508,24 +508,45 @@ void launch_nodal_velocity(const DMesh& m, double rmaxvel, hipStream_t st) {
 // per side then sums its levels in the reference's order from LDS, and the velocity rows are
 // replaced by the btu rows for the second integral. Row strides in LDS are odd so the
 // per-side reads spread over the banks.
+// Row g = i / row_len and column j from a float reciprocal (exact here: i < 2^20, row_len <=
+// 2048); the two corrections keep it exact whatever the rounding.
+__device__ __forceinline__ int split_row(int i, float inv, int row_len, int& j) {
+  int g = (int)(((float)i + 0.5f) * inv);
+  j = i - g * row_len;
+  if (j < 0) {
+    --g;
+    j += row_len;
+  }
+  if (j >= row_len) {
+    ++g;
+    j -= row_len;
+  }
+  return g;
+}
+
+// Each thread issues its loads 8 at a time before the LDS writes, so a small workgroup still
+// keeps enough loads in flight.
 __device__ __forceinline__ void stage_rows(double* __restrict__ dst, int stride, const double* __restrict__ src,
                                            int row_len, int rows) {
+  constexpr int B = 8;
   const int total = rows * row_len;
   const float inv = 1.0f / (float)row_len;
-  for (int i = threadIdx.x; i < total; i += blockDim.x) {
-    // row g = i / row_len from a float reciprocal (exact here: i < 2^20, row_len <= 2048);
-    // the two corrections keep it exact whatever the rounding
-    int g = (int)(((float)i + 0.5f) * inv);
-    int j = i - g * row_len;
-    if (j < 0) {
-      --g;
-      j += row_len;
+  for (int base = threadIdx.x; base < total; base += B * blockDim.x) {
+    double v[B];
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      const int i = base + u * blockDim.x;
+      v[u] = i < total ? src[i] : 0.0;
     }
-    if (j >= row_len) {
-      ++g;
-      j -= row_len;
+#pragma unroll
+    for (int u = 0; u < B; ++u) {
+      const int i = base + u * blockDim.x;
+      if (i < total) {
+        int j;
+        const int g = split_row(i, inv, row_len, j);
+        dst[g * stride + j] = v[u];
+      }
     }
-    dst[g * stride + j] = src[i];
   }
 }
 
@@ -680,16 +701,8 @@ __device__ __forceinline__ void unstage_rows(double* __restrict__ dst, const dou
   const int total = rows * row_len;
   const float inv = 1.0f / (float)row_len;
   for (int i = threadIdx.x; i < total; i += blockDim.x) {
-    int g = (int)(((float)i + 0.5f) * inv);
-    int j = i - g * row_len;
-    if (j < 0) {
-      --g;
-      j += row_len;
-    }
-    if (j >= row_len) {
-      ++g;
-      j -= row_len;
-    }
+    int j;
+    const int g = split_row(i, inv, row_len, j);
     dst[i] = src[g * stride + j];
   }
 }
@@ -770,22 +783,26 @@ __device__ void momentum_side(const DMesh& m, const StepArgs& a, int s, double* 
       double zm = zc[kb], z0 = zc[kb + 1];                                          // zc[k-1], zc[k]
       double bmx = u[2 * kb], bmy = u[2 * kb + 1], b0x = u[2 * kb + 2], b0y = u[2 * kb + 3];  // btu[k-1], btu[k]
       double bet = 1.0, cprev = 0.0, upx = 0.0, upy = 0.0;
+      // dz/6, dz/3 and dt dfv0 / dz of layer (k, k+1); the next row reuses them for its layer
+      // (k-1, k), whose dz is the same difference of the same two levels
+      double d6p = 0.0, d3p = 0.0, tp = 0.0;
       for (int k = kb + 1; k < nv; ++k) {
         const bool top = k < nv - 1;
         const double zp = top ? zc[k + 1] : 0.0;
         const double bpx = top ? u[2 * (k + 1)] : 0.0, bpy = top ? u[2 * (k + 1) + 1] : 0.0;
         const double dz_0 = z0 - zm;
         const double dz_1 = top ? zp - z0 : 0.0;
-        double aj = 0.0, bj = 0.0, cj = 0.0;
+        double aj = 0.0, bj = 0.0, cj = 0.0, d6 = 0.0, d3 = 0.0, tq = 0.0;
         if (top) {
-          const double tmp = dt * dfv0 / dz_1;
-          cj += 1.0 * dz_1 / 6.0 - tmp;
-          bj += 1.0 * dz_1 / 3.0 + tmp;
+          tq = dt * dfv0 / dz_1;
+          d3 = dz_1 / 3.0;
+          d6 = d3 * 0.5;  // = dz_1 / 6.0 exactly: halving commutes with rounding for normal values
+          cj += d6 - tq;
+          bj += d3 + tq;
         }
         if (k > kb + 1) {
-          const double tmp = dt * dfv0 / dz_0;
-          aj += 1.0 * dz_0 / 6.0 - tmp;
-          bj += 1.0 * dz_0 / 3.0 + tmp;
+          aj += d6p - tp;
+          bj += d3p + tp;
         } else {
           bj += dt * chi2;
         }
@@ -794,13 +811,16 @@ __device__ void momentum_side(const DMesh& m, const StepArgs& a, int s, double* 
         if (k > kb + 1) dz += dz_0;
         double qx = -(0.5 * dz) * dt * Gx, qy = -(0.5 * dz) * dt * Gy;
         if (top) {
-          qx += (dz_1 / 6.0) * (2.0 * b0x + bpx);
-          qy += (dz_1 / 6.0) * (2.0 * b0y + bpy);
+          qx += d6 * (2.0 * b0x + bpx);
+          qy += d6 * (2.0 * b0y + bpy);
         }
         if (k > kb + 1) {
-          qx += (dz_0 / 6.0) * (2.0 * b0x + bmx);
-          qy += (dz_0 / 6.0) * (2.0 * b0y + bmy);
+          qx += d6p * (2.0 * b0x + bmx);
+          qy += d6p * (2.0 * b0y + bmy);
         }
+        d6p = d6;
+        d3p = d3;
+        tp = tq;
         double ux, uy;
         if (k == kb + 1) {
           bet = bj;
@@ -883,9 +903,10 @@ __global__ void __launch_bounds__(64) k_momentum(DMesh m, StepArgs a, int g) {
 
 void launch_momentum(const DMesh& m, const StepArgs& a, hipStream_t st) {
   if (m.ns <= 0) return;
-  // sides per workgroup: one per thread, as many as 32 KB of LDS holds
+  // sides per workgroup: one per thread, as many as 24 KB of LDS holds (measured best against
+  // 16-64 KB and 32-256 threads at 3.1M sides x 31 levels)
   const int row = 8 * ((m.nvrt | 1) + ((2 * m.nvrt) | 1));
-  const int g = max(1, min(64, (32 * 1024) / row));
+  const int g = max(1, min(64, (24 * 1024) / row));
   hipLaunchKernelGGL(k_momentum, dim3((unsigned)((m.ns + g - 1) / g)), dim3(64), (size_t)g * row, st, m, a, g);
 }
 
