@@ -480,11 +480,15 @@ def main() -> int:
     # side-level; about 300 B of element and node gathers per side for the elevation gradients)
     ms_mom = min(timed(lambda: mesh.momentum(p)) for _ in range(3))
     mom_bytes = 40 * mesh.ns * mesh.nvrt + 300 * mesh.ns
+    # element w from the new side velocities: u2 and zc rows of every side read, w written
+    ms_vv = min(timed(lambda: mesh.vertical_velocity(p)) for _ in range(3))
+    vv_bytes = (24 * mesh.ns + 8 * mesh.ne) * mesh.nvrt + 64 * n
     ndelt = int(round(p.dt / p.dtb_max))
     bt_bytes = walks * (36 + 216 * ndelt)
     log(f"[bench] backtrack {ms_bt:.2f} ms ({walks} side-levels), rhs {ms_rhs:.2f} ms, assembly {ms_asm:.2f} ms, "
         f"nodal velocity {ms_nv:.2f} ms ({nv_bytes / (ms_nv * 1e-3) / 1e9:.0f} GB/s), "
-        f"momentum {ms_mom:.2f} ms ({mom_bytes / (ms_mom * 1e-3) / 1e9:.0f} GB/s)")
+        f"momentum {ms_mom:.2f} ms ({mom_bytes / (ms_mom * 1e-3) / 1e9:.0f} GB/s), "
+        f"vertical velocity {ms_vv:.2f} ms ({vv_bytes / (ms_vv * 1e-3) / 1e9:.0f} GB/s)")
 
     # ---- headline: FP64 multicolour SSOR PCG
     rtol, maxiter, check_every = 1e-12, 20000, 10
@@ -670,6 +674,8 @@ def main() -> int:
                       "substeps": ndelt, "effective_gbs": round(bt_bytes / (ms_bt * 1e-3) / 1e9, 1)},
         "nodal_velocity": {"ms": round(ms_nv, 3), "bytes": nv_bytes, "gbs": round(nv_bytes / (ms_nv * 1e-3) / 1e9, 1)},
         "momentum": {"ms": round(ms_mom, 3), "bytes": mom_bytes, "gbs": round(mom_bytes / (ms_mom * 1e-3) / 1e9, 1)},
+        "vertical_velocity": {"ms": round(ms_vv, 3), "bytes": vv_bytes,
+                              "gbs": round(vv_bytes / (ms_vv * 1e-3) / 1e9, 1)},
         "assembly_ms": round(ms_asm, 3),
         "rhs_ms": round(ms_rhs, 3),
         "mixed": mixed,

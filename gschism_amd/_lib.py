@@ -109,6 +109,7 @@ SIGNATURES = {
     "gsh_rhs_producers": (_I, [_P, C.POINTER(StepParams)]),
     "gsh_nodal_velocity": (_I, [_P, C.POINTER(StepParams)]),
     "gsh_momentum": (_I, [_P, C.POINTER(StepParams)]),
+    "gsh_vertical_velocity": (_I, [_P, C.POINTER(StepParams)]),
     "gsh_grid_read": (_I, [C.c_char_p, C.c_char_p, _PP]),
     "gsh_grid_info": (_I, [_P, _PI32, _PI32, _PI32, _PI32, _PI32]),
     "gsh_grid_get": (_I, [_P, _PD, _PD, _PI32, C.POINTER(C.c_int8), _PI32, _PD]),

@@ -194,6 +194,11 @@ class Mesh:
         solve and the land-boundary slip condition."""
         check(L.lib().gsh_momentum(self.h, C.byref(params)))
 
+    def vertical_velocity(self, params) -> None:
+        """Element vertical velocity w (GSH_F_ELEM_W) from the side velocities and the levels of
+        the last level update."""
+        check(L.lib().gsh_vertical_velocity(self.h, C.byref(params)))
+
 
 class Solver:
     """PCG on the device (gsh_solver), bound to a mesh or to a caller CSR pattern; Jacobi

@@ -1206,6 +1206,7 @@ int gsh_mesh_create(gsh_ctx* ctx, const gsh_mesh_desc* desc, gsh_mesh** out, int
     d.elbc = P.alloc<double>(n);
     d.uvw = P.alloc<double>(4 * (int64_t)n * nv);
     d.nrec = P.alloc<double2>(2 * (int64_t)n);
+    d.eta_lev = P.alloc<double>(n);
     d.node_wet = P.alloc<int8_t>(n);
     d.elem = P.upload(erec);
     d.area = P.upload(m->geo.area);
@@ -1481,6 +1482,17 @@ int gsh_momentum(gsh_mesh* m, const gsh_step_params* p) {
     const StepArgs a = step_args(p);
     m->ctx->set();
     launch_momentum(m->d, a, m->ctx->stream);
+    HIPCHK(hipGetLastError());
+  })
+}
+
+int gsh_vertical_velocity(gsh_mesh* m, const gsh_step_params* p) {
+  GSH_API_BODY({
+    GSH_REQUIRE(m && p, GSH_E_ARG, "NULL argument");
+    GSH_REQUIRE(m->ctx, GSH_E_STATE, "host-only mesh (created without a context)");
+    m->ctx->set();
+    ensure_elem_w(m);
+    launch_vertical_velocity(m->d, p->h0, m->ctx->stream);
     HIPCHK(hipGetLastError());
   })
 }

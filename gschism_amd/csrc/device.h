@@ -30,6 +30,7 @@ struct DMesh {
   double* elbc;
   double* uvw;              // [n][nvrt][4]: u, v, w, 0
   double2* nrec;            // [n][2]: (x, y), (eta2, depth), packed for the backtracking gathers
+  double* eta_lev;          // [n] eta2 of the last level update (the node z levels follow it)
   int8_t* node_wet;
   // elements
   const int4* elem;         // [ne][2]: (n0, n1, n2, s0), (s1, s2, -, -)
@@ -169,6 +170,7 @@ void launch_backtrack(const DMesh& m, const StepArgs& a, const int32_t* list_sid
 void launch_rhs(const DMesh& m, const StepArgs& a, hipStream_t s);
 void launch_nodal_velocity(const DMesh& m, double rmaxvel, hipStream_t s);
 void launch_momentum(const DMesh& m, const StepArgs& a, hipStream_t s);
+void launch_vertical_velocity(const DMesh& m, double h0, hipStream_t s);
 void launch_assemble(const DMesh& m, const StepArgs& a, double* val, double* b, hipStream_t s);
 void launch_update_elevation(const DMesh& m, const double* x, hipStream_t s);
 

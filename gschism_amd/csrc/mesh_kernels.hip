@@ -53,6 +53,7 @@ __global__ void k_node_wet(DMesh m) {
       break;
     }
   m.node_wet[i] = wet;
+  m.eta_lev[i] = m.eta2[i];
 }
 
 // ul0UpdateSides / ul0UpdateSides2 (meshdata.cu:5679-5760): side kb, zc, wet flag. One thread per
@@ -908,6 +909,141 @@ void launch_momentum(const DMesh& m, const StepArgs& a, hipStream_t st) {
   const int row = 8 * ((m.nvrt | 1) + ((2 * m.nvrt) | 1));
   const int g = max(1, min(64, (24 * 1024) / row));
   hipLaunchKernelGGL(k_momentum, dim3((unsigned)((m.ns + g - 1) / g)), dim3(64), (size_t)g * row, st, m, a, g);
+}
+
+// ---------------------------------------------------------------- vertical velocity
+// computeVerticalVelocity (meshdata.cu:5978-6102), triangles, ics=1, imm=0: element w from the
+// flux balance of its prisms, bottom up, with the levels of the last level update (node z from
+// the elevation of that update, side zc and kb) and the side velocities u2. A workgroup takes E
+// elements. Phase 1, one thread per (element, level l): the side fluxes of layer (l, l+1), the
+// horizontal flux through the face at level l+1 and that face's n_z A (Element::n,
+// meshdata.cu:559-631); phase 2, one thread per element: the running top flux
+// top_l = -side_l - bottom_l, bottom_l+1 = -top_l, in the reference's order; phase 3, one thread
+// per (element, level): w = (top - horizontal) / (n_z A), stored with coalesced writes.
+__device__ __forceinline__ double zcoord_lev(const DMesh& m, double h0, int node, int k) {
+  const double eta = m.eta_lev[node], h = m.depth[node];
+  if (eta + h <= h0) return 0.0;  // isDry2
+  const int kb = m.kb[node];
+  if (k <= kb) return -h;
+  if (k == m.nvrt - 1) return eta;
+  return (eta + h) * m.sigma[k] + eta;
+}
+
+__global__ void __launch_bounds__(256) k_vertical_velocity(DMesh m, double h0, int E) {
+  extern __shared__ double lds[];
+  const int nv = m.nvrt, nvp = nv - 1;
+  double* Lsf = lds;            // [E][nvp]: side flux of layer (l, l+1), then the top flux
+  double* Lth = Lsf + E * nvp;  // [E][nvp]: horizontal flux through the face at level l+1
+  double* Lden = Lth + E * nvp; // [E][nvp]: n_z A of that face
+  double* Lbw = Lden + E * nvp; // [E]: w at the element's bottom level
+  const int e0 = blockIdx.x * E;
+  const int ec = min(E, m.ne - e0);
+  for (int idx = threadIdx.x; idx < ec * nvp; idx += blockDim.x) {
+    const int el = idx / nvp, l = idx - el * nvp;
+    const int e = e0 + el;
+    if (!m.elem_wet[e] || l < m.elem_kb[e]) continue;
+    const int4 r0 = m.elem[2 * e], r1 = m.elem[2 * e + 1];
+    const int nd[3] = {r0.x, r0.y, r0.z};
+    const int sd[3] = {r0.w, r1.x, r1.y};
+    // the face at level l+1 (node levels clamped to each node's bed)
+    double px[3], py[3], pz[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const int i = nd[j];
+      px[j] = m.x[i];
+      py[j] = m.y[i];
+      pz[j] = zcoord_lev(m, h0, i, max(l + 1, m.kb[i]));
+    }
+    const double n20x = px[2] - px[0], n20y = py[2] - py[0], n20z = pz[2] - pz[0];
+    const double n10x = px[1] - px[0], n10y = py[1] - py[0], n10z = pz[1] - pz[0];
+    const double nx = (n10y) * (n20z) - (n20y) * (n10z);
+    const double ny = (n20x) * (n10z) - (n10x) * (n20z);
+    const double nz = 2.0 * m.area[e];
+    const double len = sqrt(nx * nx + ny * ny + nz * nz);
+    const double ta = 0.5 * len;
+    const double inv_len = 1.0 / len;
+    const double tnx = nx * inv_len, tny = ny * inv_len, tnz = nz * inv_len;
+    double sf = 0.0, vx[3], vy[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const int s = sd[j];
+      const int4 sr = m.side[s];
+      const int kbs = m.side_kb[s];
+      const int kt = max(l + 1, kbs), kl = max(l, kbs);
+      const double2* u2 = reinterpret_cast<const double2*>(m.su2) + (int64_t)s * nv;
+      const double* zc = m.zc + (int64_t)s * nv;
+      const double2 ut = u2[kt], ul = u2[kl];
+      const double ucx = 0.5 * (ut.x + ul.x), ucy = 0.5 * (ut.y + ul.y);
+      const double sideArea = m.side_len[s] * (zc[kt] - zc[kl]);
+      const double sign = (sr.z == e) ? 1.0 : -1.0;
+      // side normal from populateSides (meshdata.cu:2283-2287)
+      const double qx = -(m.y[sr.x] - m.y[sr.y]), qy = m.x[sr.x] - m.x[sr.y];
+      const double ql = sqrt(qx * qx + qy * qy);
+      const double snx = qx / ql, sny = qy / ql;
+      sf += sign * (ucx * snx + ucy * sny) * sideArea;
+      vx[j] = ut.x;
+      vy[j] = ut.y;
+    }
+    const double utx = (vx[0] + vx[1] + vx[2] + 0.0) / 3, uty = (vy[0] + vy[1] + vy[2] + 0.0) / 3;
+    Lsf[idx] = sf;
+    Lth[idx] = (utx * tnx + uty * tny) * ta;
+    Lden[idx] = tnz * ta;
+  }
+  __syncthreads();
+  if (threadIdx.x < ec) {
+    const int el = threadIdx.x, e = e0 + el;
+    if (m.elem_wet[e]) {
+      const int kb = m.elem_kb[e];
+      const int4 r0 = m.elem[2 * e], r1 = m.elem[2 * e + 1];
+      const int sd[3] = {r0.w, r1.x, r1.y};
+      const double* gp = m.gradphi + 6 * (int64_t)e;
+      const double d0 = m.depth[r0.x], d1 = m.depth[r0.y], d2 = m.depth[r0.z];
+      const double ghx = d0 * gp[0] + d1 * gp[2] + d2 * gp[4];
+      const double ghy = d0 * gp[1] + d1 * gp[3] + d2 * gp[5];
+      double bx[3], by[3];
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        const int s = sd[j];
+        const double2 u = reinterpret_cast<const double2*>(m.su2)[(int64_t)s * nv + m.side_kb[s]];
+        bx[j] = u.x;
+        by[j] = u.y;
+      }
+      const double ubx = (bx[0] + bx[1] + bx[2] + 0.0) / 3, uby = (by[0] + by[1] + by[2] + 0.0) / 3;
+      const double bottomW = -(ghx * ubx + ghy * uby);
+      Lbw[el] = 0.0 + bottomW;
+      double bottom = 0.0;
+      for (int l = kb; l < nv - 1; ++l) {
+        const double top = -Lsf[el * nvp + l] - bottom;
+        Lsf[el * nvp + l] = top;
+        bottom = -top;
+      }
+    }
+  }
+  __syncthreads();
+  double* w = m.ew + (int64_t)e0 * nv;
+  for (int idx = threadIdx.x; idx < ec * nv; idx += blockDim.x) {
+    const int el = idx / nv, k = idx - el * nv;
+    const int e = e0 + el;
+    double v = 0.0;
+    if (m.elem_wet[e]) {
+      const int kb = m.elem_kb[e];
+      if (k == kb) {
+        v = Lbw[el];
+      } else if (k > kb) {
+        const int j = el * nvp + k - 1;
+        v = (Lsf[j] - Lth[j]) / Lden[j];
+      }
+    }
+    w[idx] = v;
+  }
+}
+
+void launch_vertical_velocity(const DMesh& m, double h0, hipStream_t st) {
+  if (m.ne <= 0 || m.nvrt < 2) return;
+  const int nvp = m.nvrt - 1;
+  const int E = max(1, 256 / nvp);  // elements per workgroup: about one thread per (element, level)
+  const size_t lds = 8 * ((size_t)3 * E * nvp + E);
+  hipLaunchKernelGGL(k_vertical_velocity, dim3((unsigned)((m.ne + E - 1) / E)), dim3(256), lds, st, m, h0, E);
 }
 
 // ---------------------------------------------------------------- a11+a12: assembly + BC

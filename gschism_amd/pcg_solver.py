@@ -183,6 +183,12 @@ def solveMomentumEquations(mesh: Mesh, params, time: float = 0.0) -> None:
     mesh.momentum(params)
 
 
+def computeVerticalVelocity(mesh: Mesh, params) -> None:
+    """computeVerticalVelocity (schism_step.py:294, meshdata.cu:5978-6102): element w from the
+    side velocities of the momentum step."""
+    mesh.vertical_velocity(params)
+
+
 def updateElevation(mesh: Mesh, solver: PCGSolverMultiGPU) -> None:
     """rollUpElevation + updateElevation (meshdata.cu:3538-3544, 4449-4456)."""
     solver._solver.update_elevation()
@@ -196,5 +202,7 @@ def hot_path_step(mesh: Mesh, solver: PCGSolverMultiGPU, params, maxIter=1500, t
     iters = solver.solve(rtol=tol, maxiter=maxIter)
     updateElevation(mesh, solver)
     solveMomentumEquations(mesh, params)
+    computeVerticalVelocity(mesh, params)
     mesh.update_levels(params)
+    computeNodalVelocities(mesh, params)
     return iters

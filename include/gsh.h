@@ -206,6 +206,12 @@ int gsh_nodal_velocity(gsh_mesh* mesh, const gsh_step_params* p);
  * boundary sides keep the tangential component. */
 int gsh_momentum(gsh_mesh* mesh, const gsh_step_params* p);
 
+/* computeVerticalVelocity (meshdata.cu:5978-6102), triangles, ics=1, imm=0: GSH_F_ELEM_W from
+ * GSH_F_SIDE_U2 (after gsh_momentum) and the levels of the last gsh_update_levels (node z from
+ * the elevation of that update, side zc and kb): the flux balance of each prism bottom up, with
+ * w = -grad(depth) . u at the element's bottom level. */
+int gsh_vertical_velocity(gsh_mesh* mesh, const gsh_step_params* p);
+
 /* ---------------------------------------------------------------- grid files (host side)
  * hgrid.gr3 as readHGrid reads it (read_and_process_grid.cpp:855-1070): node coordinates and
  * depth, triangles (1-based node ids in the file, 0-based here), open and land boundary segments,

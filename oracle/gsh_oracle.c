@@ -811,6 +811,91 @@ void or_momentum(or_mesh* m, const or_params* p) {
   free(gam);
 }
 
+/* ------------------------------------------------------------------ vertical velocity */
+/* Element::n (meshdata.cu:559-631), triangles, ics=1: the unit normal of the element's face at
+ * level k (node levels max(k, kb) of its nodes) and that face's area */
+static void elem_face_normal(const or_mesh* m, int e, int k, double n[3], double* area3d) {
+  const int nv = m->nvrt;
+  const int32_t* nd = m->elnode + 3 * e;
+  double px[3], py[3], pz[3];
+  for (int l = 0; l < 3; ++l) {
+    const int kk = k > m->kb[nd[l]] ? k : m->kb[nd[l]];
+    px[l] = m->x[nd[l]];
+    py[l] = m->y[nd[l]];
+    pz[l] = m->nz[(size_t)nd[l] * nv + kk];
+  }
+  const double n20[3] = {px[2] - px[0], py[2] - py[0], pz[2] - pz[0]};
+  const double n10[3] = {px[1] - px[0], py[1] - py[0], pz[1] - pz[0]};
+  const double nx = (n10[1]) * (n20[2]) - (n20[1]) * (n10[2]);
+  const double ny = (n20[0]) * (n10[2]) - (n10[0]) * (n20[2]);
+  const double nzz = 2.0 * m->area[e];
+  const double len = sqrt(nx * nx + ny * ny + nzz * nzz);
+  *area3d = 0.5 * len;
+  const double inv_len = 1.0 / len;
+  n[0] = nx * inv_len;
+  n[1] = ny * inv_len;
+  n[2] = nzz * inv_len;
+}
+
+/* computeVerticalVelocity (meshdata.cu:5978-6102), triangles, ics=1, imm=0: w of each element
+ * from the flux balance of its prisms, bottom up. w = 0 for dry elements and below kb;
+ * w[kb] = -grad(depth) . mean_j u2_j[kb_j]; for l = kb .. nvrt-2, with the side fluxes
+ * F_l = sum_j ssign_j (0.5 (u2_j[max(l+1, kb_j)] + u2_j[max(l, kb_j)]) . n_j) len_j dz_j(l),
+ * top = -F_l - bottom, w[l+1] = (top - (mean_j u2_j[max(l+1, kb_j)] . n_xy) A) / (n_z A)
+ * (n, A the unit normal and area of the face at level l+1) and bottom = -top. */
+void or_vertical_velocity(or_mesh* m) {
+  const int ne = m->ne, nv = m->nvrt;
+  for (int e = 0; e < ne; ++e) {
+    double* w = m->ew + (size_t)e * nv;
+    for (int l = 0; l < nv; ++l) w[l] = 0.0;
+    if (m->elem_dry[e]) continue;
+    const int kb = m->elem_kb[e];
+    const int32_t* es = m->elside + 3 * e;
+    double gh[2];
+    elem_grad(m, e, m->depth, gh);
+    double ub[2] = {0.0, 0.0};
+    {
+      double v[3][2];
+      for (int j = 0; j < 3; ++j) {
+        const int s = es[j];
+        const double* u = m->su2 + 2 * ((size_t)s * nv + m->side_kb[s]);
+        v[j][0] = u[0];
+        v[j][1] = u[1];
+      }
+      for (int c = 0; c < 2; ++c) ub[c] = (v[0][c] + v[1][c] + v[2][c] + 0.0) / 3;
+    }
+    const double bottomW = -(gh[0] * ub[0] + gh[1] * ub[1]);
+    w[kb] = 0.0 + bottomW;
+    double bottomFlux = 0.0;
+    for (int l = kb; l < nv - 1; ++l) {
+      double tn[3], ta;
+      elem_face_normal(m, e, l + 1, tn, &ta);
+      double sideFlux = 0.0;
+      double ut[2] = {0.0, 0.0};
+      double vt[3][2];
+      for (int j = 0; j < 3; ++j) {
+        const int s = es[j];
+        const int kbs = m->side_kb[s];
+        const int kt = l + 1 > kbs ? l + 1 : kbs, kl = l > kbs ? l : kbs;
+        const double* u2 = m->su2 + 2 * (size_t)s * nv;
+        const double* zc = m->zc + (size_t)s * nv;
+        const double ucx = 0.5 * (u2[2 * kt] + u2[2 * kl]), ucy = 0.5 * (u2[2 * kt + 1] + u2[2 * kl + 1]);
+        const double sideArea = m->slen[s] * (zc[kt] - zc[kl]);
+        const double sign = m->ssign[3 * e + j];
+        sideFlux += sign * (ucx * m->snx[s] + ucy * m->sny[s]) * sideArea;
+        vt[j][0] = u2[2 * kt];
+        vt[j][1] = u2[2 * kt + 1];
+      }
+      for (int c = 0; c < 2; ++c) ut[c] = (vt[0][c] + vt[1][c] + vt[2][c] + 0.0) / 3;
+      const double topFlux = -sideFlux - bottomFlux;
+      const double topHorizontalFlux = (ut[0] * tn[0] + ut[1] * tn[1]) * ta;
+      const double topVerticalFlux = topFlux - topHorizontalFlux;
+      w[l + 1] = topVerticalFlux / (tn[2] * ta);
+      bottomFlux = -topFlux;
+    }
+  }
+}
+
 /* ------------------------------------------------------------------ backtracking */
 typedef struct {
   double x, y;

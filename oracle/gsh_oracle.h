@@ -45,6 +45,8 @@ void or_rhs(or_mesh* m, const or_params* p);
 void or_nodal_velocity(or_mesh* m, const or_params* p, int nthreads);
 /* side velocities u2 of the next step: elevation gradients, momentum solve, land-boundary slip */
 void or_momentum(or_mesh* m, const or_params* p);
+/* element vertical velocity w (GSH_F_ELEM_W) from the side velocities and the levels */
+void or_vertical_velocity(or_mesh* m);
 /* Assembly + essential BC into val (CSR of the mesh) and b. */
 void or_assemble(or_mesh* m, const or_params* p, double* val, double* b, int nthreads);
 

@@ -38,6 +38,7 @@ _SIG = {
     "or_rhs": (None, [_P, C.POINTER(OrParams)]),
     "or_nodal_velocity": (None, [_P, C.POINTER(OrParams), C.c_int]),
     "or_momentum": (None, [_P, C.POINTER(OrParams)]),
+    "or_vertical_velocity": (None, [_P]),
     "or_assemble": (None, [_P, C.POINTER(OrParams), _PD, _PD, C.c_int]),
     "or_pcg": (C.c_int32, [C.c_int32, _PI32, _PI32, _PD, _PD, _PD, _PD, _PD, C.c_double, C.c_int32, C.c_int32,
                            C.c_int32, _PD, _PD, _PD, C.c_int]),
@@ -189,6 +190,9 @@ class OracleMesh:
 
     def momentum(self, p) -> None:
         lib().or_momentum(self.h, C.byref(params(p)))
+
+    def vertical_velocity(self) -> None:
+        lib().or_vertical_velocity(self.h)
 
     def assemble(self, p, nthreads: int = 1):
         val = np.zeros(self.nnz, dtype=np.float64)

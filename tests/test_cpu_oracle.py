@@ -701,3 +701,116 @@ def test_momentum_land_sides_have_no_normal_flow():
     un = np.einsum("skc,sc->sk", got[land], snrm[land])
     assert np.abs(un).max() < 1e-15
     assert np.abs(got[land]).max() > 1e-3
+
+
+# ---------------------------------------------------------------- vertical velocity
+def _vv_mesh(nvrt=6, depth=10.0, kb=None, eta=None):
+    m, xy, en, ob = _mesh(nx=16, ny=10, nvrt=nvrt, kb=kb, depth=depth)
+    xyo = m.topology(L.T_XY)
+    m.set(L.F_ETA2, np.zeros(m.n) if eta is None else eta(xyo))
+    m.levels(_params())
+    return m
+
+
+def test_vertical_velocity_divergence_closed_form():
+    """Flat levels (constant depth and elevation), u = a_k x at level k on every side: the side
+    fluxes of a linear field are exact, so w[k+1] = w[k] - 0.5 (a_k + a_k+1) dz and w[0] = 0."""
+    nv = 6
+    m = _vv_mesh(nvrt=nv)
+    sc = m.geometry(2)
+    a = 1e-5 * (1.0 + np.arange(nv))
+    u2 = np.zeros((m.ns, nv, 2))
+    u2[:, :, 0] = sc[:, :1] * a
+    m.set(L.F_SIDE_U2, u2)
+    m.vertical_velocity()
+    w = m.get(L.F_ELEM_W)
+    zc = m.get(L.F_ZC)[0]
+    exp = np.zeros(nv)
+    for k in range(nv - 1):
+        exp[k + 1] = exp[k] - 0.5 * (a[k] + a[k + 1]) * (zc[k + 1] - zc[k])
+    assert np.allclose(w, exp, rtol=1e-9, atol=1e-15)
+    assert np.abs(exp).max() > 1e-5
+
+
+def test_vertical_velocity_bottom_condition():
+    """Uniform u0 over a bed sloping in x: w at each element's bottom level is -u0 . grad(h)."""
+    nv = 5
+    xy, en, ob = cases.channel(16, 10, 400.0, 0.2)
+    b = 2e-3
+    m = O.OracleMesh(xy, en, 8.0 + b * xy[:, 0], ob, cases.sigma_levels(nv))
+    m.set(L.F_ETA2, np.zeros(m.n))
+    m.levels(_params())
+    u0 = np.array([0.3, -0.2])
+    m.set(L.F_SIDE_U2, np.broadcast_to(u0, (m.ns, nv, 2)).copy())
+    m.vertical_velocity()
+    w = m.get(L.F_ELEM_W)
+    assert np.allclose(w[:, 0], -u0[0] * b, rtol=1e-9, atol=1e-15)
+
+
+def test_vertical_velocity_matches_prism_flux_balance():
+    """Random side velocities, node-varying bottom levels and a non-flat elevation and bed: the
+    oracle against the prism flux balance assembled from geometry (outward side normals from the
+    element centroid, side faces len x clamped dz, top-face normals from cross products, upward
+    flux Q[l+1] = Q[l] - outward side flux, Q[kb] = 0, w = (Q - u_top . n_xy A) / (n_z A))."""
+    nv = 7
+    xy, en, ob = cases.channel(14, 9, 500.0, 0.25)
+    depth = 9.0 + 3e-3 * xy[:, 0] - 2e-3 * xy[:, 1]
+    kb = (np.arange(xy.shape[0]) % 3).astype(np.int32)
+    m = O.OracleMesh(xy, en, depth, ob, cases.sigma_levels(nv), kb=kb)
+    xyo = m.topology(L.T_XY)
+    m.set(L.F_ETA2, 0.3 * np.sin(xyo[:, 0] / 1500.0))
+    m.levels(_params())
+    rng = np.random.default_rng(3)
+    u2 = rng.normal(0.0, 0.5, (m.ns, nv, 2))
+    m.set(L.F_SIDE_U2, u2)
+    m.vertical_velocity()
+    w = m.get(L.F_ELEM_W)
+    eno, es, sn = m.topology(L.T_ELNODE), m.topology(L.T_ELSIDE), m.topology(L.T_ISIDENODE)
+    kbn = m.topology(L.T_KB)
+    zc = m.get(L.F_ZC)
+    wet = m.get(L.F_WET)
+    swet = wet[m.n + m.ne:] != 0
+    skb = np.where(swet, np.minimum(kbn[sn[:, 0]], kbn[sn[:, 1]]), 0)
+    eta, dep = m.get(L.F_ETA2), m.topology(L.T_DEPTH)
+    sig = cases.sigma_levels(nv)
+
+    def node_z(i, k):
+        k = max(k, kbn[i])
+        return -dep[i] if k <= kbn[i] else (eta[i] if k == nv - 1 else (eta[i] + dep[i]) * sig[k] + eta[i])
+
+    checked = 0
+    for e in range(m.ne):
+        if not wet[m.n + e]:
+            assert np.all(w[e] == 0.0)
+            continue
+        kbe = kbn[eno[e]].min()
+        ce = xyo[eno[e]].mean(axis=0)
+        A = np.c_[np.ones(3), xyo[eno[e]]]
+        gh = np.linalg.solve(A, dep[eno[e]])[1:]
+        ub = np.mean([u2[s, skb[s]] for s in es[e]], axis=0)
+        exp = np.zeros(nv)
+        exp[kbe] = -gh @ ub
+        Q = 0.0
+        for l in range(kbe, nv - 1):
+            out = 0.0
+            for s in es[e]:
+                p1, p2 = xyo[sn[s, 0]], xyo[sn[s, 1]]
+                t = p2 - p1
+                nrm = np.array([t[1], -t[0]]) / np.hypot(*t)
+                if nrm @ (0.5 * (p1 + p2) - ce) < 0:
+                    nrm = -nrm
+                kt, kl = max(l + 1, skb[s]), max(l, skb[s])
+                uc = 0.5 * (u2[s, kt] + u2[s, kl])
+                out += (uc @ nrm) * np.hypot(*t) * (zc[s, kt] - zc[s, kl])
+            Q = Q - out
+            pts = np.array([[*xyo[i], node_z(i, l + 1)] for i in eno[e]])
+            nvec = np.cross(pts[1] - pts[0], pts[2] - pts[0])
+            if nvec[2] < 0:
+                nvec = -nvec
+            area3 = 0.5 * np.linalg.norm(nvec)
+            nu = nvec / np.linalg.norm(nvec)
+            ut = np.mean([u2[s, max(l + 1, skb[s])] for s in es[e]], axis=0)
+            exp[l + 1] = (Q - (ut @ nu[:2]) * area3) / (nu[2] * area3)
+        assert np.allclose(w[e], exp, rtol=1e-9, atol=1e-12), e
+        checked += 1
+    assert checked > 0.9 * m.ne

@@ -78,7 +78,10 @@ def test_hot_path_step_matches_manual_sequence(ctx):
         _, ib, _ = sb.solve(None, rtol=1e-12, maxiter=1500)
         sb.update_elevation()
         mesh_b.momentum(p)
+        mesh_b.vertical_velocity(p)
         mesh_b.update_levels(p)
+        mesh_b.nodal_velocity(p)
         assert ia == ib
         assert_bitwise(mesh_a.get(L.F_ETA2), mesh_b.get(L.F_ETA2), f"step {step} eta2")
         assert_bitwise(mesh_a.get(L.F_SIDE_U2), mesh_b.get(L.F_SIDE_U2), f"step {step} side velocity")
+        assert_bitwise(mesh_a.get(L.F_NODE_UVW), mesh_b.get(L.F_NODE_UVW), f"step {step} nodal velocities")

@@ -71,6 +71,16 @@ def _full_chain(ctx, mesh, om, p, rows_per_thread=4):
     mesh.momentum(p)
     om.momentum(p)
     assert_bitwise(mesh.get(L.F_SIDE_U2), om.get(L.F_SIDE_U2), "side velocity from the momentum step")
+    mesh.vertical_velocity(p)
+    om.vertical_velocity()
+    assert_bitwise(mesh.get(L.F_ELEM_W), om.get(L.F_ELEM_W), "element vertical velocity")
+    # the levels of the new elevation, then the nodal velocities the next step backtracks through
+    mesh.update_levels(p)
+    om.levels(p)
+    assert_bitwise(mesh.get(L.F_ZC), om.get(L.F_ZC), "zc of the new elevation")
+    mesh.nodal_velocity(p)
+    om.nodal_velocity(p, nthreads=4)
+    assert_bitwise(mesh.get(L.F_NODE_UVW), om.get(L.F_NODE_UVW), "nodal velocities of the next step")
     return solver, x, val, b
 
 
@@ -206,9 +216,16 @@ def test_three_steps_with_history(ctx):
         mesh.momentum(p)
         om.momentum(p)
         assert_bitwise(mesh.get(L.F_SIDE_U2), om.get(L.F_SIDE_U2), f"step {step} side velocity")
+        mesh.vertical_velocity(p)
+        om.vertical_velocity()
+        assert_bitwise(mesh.get(L.F_ELEM_W), om.get(L.F_ELEM_W), f"step {step} element w")
         mesh.update_levels(p)
         om.levels(p)
         assert_bitwise(mesh.get(L.F_ZC), om.get(L.F_ZC), f"step {step} zc")
+        # the next step backtracks through the velocities this step produced
+        mesh.nodal_velocity(p)
+        om.nodal_velocity(p, nthreads=4)
+        assert_bitwise(mesh.get(L.F_NODE_UVW), om.get(L.F_NODE_UVW), f"step {step} nodal velocities")
 
 
 def test_backtrack_list_matches_full(ctx):
@@ -415,6 +432,9 @@ def test_full_size_1m_channel(ctx):
     mesh.momentum(p)
     om.momentum(p)
     assert_bitwise(mesh.get(L.F_SIDE_U2), om.get(L.F_SIDE_U2), "1M side velocity from the momentum step")
+    mesh.vertical_velocity(p)
+    om.vertical_velocity()
+    assert_bitwise(mesh.get(L.F_ELEM_W), om.get(L.F_ELEM_W), "1M element vertical velocity")
     solver.close()
     mesh.close()
 
@@ -442,6 +462,9 @@ def test_momentum_many_levels(ctx, nvrt):
     mesh.momentum(p)
     om.momentum(p)
     assert_bitwise(mesh.get(L.F_SIDE_U2), om.get(L.F_SIDE_U2), f"side velocity, {nvrt} levels")
+    mesh.vertical_velocity(p)
+    om.vertical_velocity()
+    assert_bitwise(mesh.get(L.F_ELEM_W), om.get(L.F_ELEM_W), f"element w, {nvrt} levels")
     mesh.close()
 
 
