@@ -920,14 +920,14 @@ void launch_momentum(const DMesh& m, const StepArgs& a, hipStream_t st) {
 // meshdata.cu:559-631); phase 2, one thread per element: the running top flux
 // top_l = -side_l - bottom_l, bottom_l+1 = -top_l, in the reference's order; phase 3, one thread
 // per (element, level): w = (top - horizontal) / (n_z A), stored with coalesced writes.
-__device__ __forceinline__ double zcoord_lev(const DMesh& m, double h0, int node, int k) {
-  const double eta = m.eta_lev[node], h = m.depth[node];
-  if (eta + h <= h0) return 0.0;  // isDry2
-  const int kb = m.kb[node];
-  if (k <= kb) return -h;
-  if (k == m.nvrt - 1) return eta;
-  return (eta + h) * m.sigma[k] + eta;
-}
+
+// Per-element values that every level of phase 1 reads: node positions and level data, side
+// normals (populateSides, meshdata.cu:2283-2287), orientation signs, lengths and bottom levels.
+struct VvElem {
+  double px[3], py[3], eta[3], h[3];
+  double snx[3], sny[3], sign[3], slen[3];
+  int nkb[3], s[3], skb[3], dry[3];
+};
 
 __global__ void __launch_bounds__(256) k_vertical_velocity(DMesh m, double h0, int E) {
   extern __shared__ double lds[];
@@ -936,26 +936,51 @@ __global__ void __launch_bounds__(256) k_vertical_velocity(DMesh m, double h0, i
   double* Lth = Lsf + E * nvp;  // [E][nvp]: horizontal flux through the face at level l+1
   double* Lden = Lth + E * nvp; // [E][nvp]: n_z A of that face
   double* Lbw = Lden + E * nvp; // [E]: w at the element's bottom level
+  VvElem* Lel = reinterpret_cast<VvElem*>(Lbw + E);  // [E]
   const int e0 = blockIdx.x * E;
   const int ec = min(E, m.ne - e0);
+  if (threadIdx.x < ec) {
+    const int e = e0 + threadIdx.x;
+    VvElem& q = Lel[threadIdx.x];
+    const int4 r0 = m.elem[2 * e], r1 = m.elem[2 * e + 1];
+    const int nd[3] = {r0.x, r0.y, r0.z};
+    const int sd[3] = {r0.w, r1.x, r1.y};
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const int i = nd[j];
+      q.px[j] = m.x[i];
+      q.py[j] = m.y[i];
+      q.eta[j] = m.eta_lev[i];
+      q.h[j] = m.depth[i];
+      q.nkb[j] = m.kb[i];
+      q.dry[j] = q.eta[j] + q.h[j] <= h0;  // isDry2 at the level update
+      const int s = sd[j];
+      const int4 sr = m.side[s];
+      const double vx = -(m.y[sr.x] - m.y[sr.y]), vy = m.x[sr.x] - m.x[sr.y];
+      const double ql = sqrt(vx * vx + vy * vy);
+      q.snx[j] = vx / ql;
+      q.sny[j] = vy / ql;
+      q.sign[j] = (sr.z == e) ? 1.0 : -1.0;
+      q.slen[j] = m.side_len[s];
+      q.s[j] = s;
+      q.skb[j] = m.side_kb[s];
+    }
+  }
+  __syncthreads();
   for (int idx = threadIdx.x; idx < ec * nvp; idx += blockDim.x) {
     const int el = idx / nvp, l = idx - el * nvp;
     const int e = e0 + el;
     if (!m.elem_wet[e] || l < m.elem_kb[e]) continue;
-    const int4 r0 = m.elem[2 * e], r1 = m.elem[2 * e + 1];
-    const int nd[3] = {r0.x, r0.y, r0.z};
-    const int sd[3] = {r0.w, r1.x, r1.y};
-    // the face at level l+1 (node levels clamped to each node's bed)
-    double px[3], py[3], pz[3];
+    const VvElem& q = Lel[el];
+    // the face at level l+1 (node levels clamped to each node's bed), zcoord of the level update
+    double pz[3];
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-      const int i = nd[j];
-      px[j] = m.x[i];
-      py[j] = m.y[i];
-      pz[j] = zcoord_lev(m, h0, i, max(l + 1, m.kb[i]));
+      const int k = max(l + 1, q.nkb[j]);
+      pz[j] = q.dry[j] ? 0.0 : (k <= q.nkb[j] ? -q.h[j] : (k == nv - 1 ? q.eta[j] : (q.eta[j] + q.h[j]) * m.sigma[k] + q.eta[j]));
     }
-    const double n20x = px[2] - px[0], n20y = py[2] - py[0], n20z = pz[2] - pz[0];
-    const double n10x = px[1] - px[0], n10y = py[1] - py[0], n10z = pz[1] - pz[0];
+    const double n20x = q.px[2] - q.px[0], n20y = q.py[2] - q.py[0], n20z = pz[2] - pz[0];
+    const double n10x = q.px[1] - q.px[0], n10y = q.py[1] - q.py[0], n10z = pz[1] - pz[0];
     const double nx = (n10y) * (n20z) - (n20y) * (n10z);
     const double ny = (n20x) * (n10z) - (n10x) * (n20z);
     const double nz = 2.0 * m.area[e];
@@ -966,21 +991,14 @@ __global__ void __launch_bounds__(256) k_vertical_velocity(DMesh m, double h0, i
     double sf = 0.0, vx[3], vy[3];
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-      const int s = sd[j];
-      const int4 sr = m.side[s];
-      const int kbs = m.side_kb[s];
+      const int s = q.s[j], kbs = q.skb[j];
       const int kt = max(l + 1, kbs), kl = max(l, kbs);
       const double2* u2 = reinterpret_cast<const double2*>(m.su2) + (int64_t)s * nv;
       const double* zc = m.zc + (int64_t)s * nv;
       const double2 ut = u2[kt], ul = u2[kl];
       const double ucx = 0.5 * (ut.x + ul.x), ucy = 0.5 * (ut.y + ul.y);
-      const double sideArea = m.side_len[s] * (zc[kt] - zc[kl]);
-      const double sign = (sr.z == e) ? 1.0 : -1.0;
-      // side normal from populateSides (meshdata.cu:2283-2287)
-      const double qx = -(m.y[sr.x] - m.y[sr.y]), qy = m.x[sr.x] - m.x[sr.y];
-      const double ql = sqrt(qx * qx + qy * qy);
-      const double snx = qx / ql, sny = qy / ql;
-      sf += sign * (ucx * snx + ucy * sny) * sideArea;
+      const double sideArea = q.slen[j] * (zc[kt] - zc[kl]);
+      sf += q.sign[j] * (ucx * q.snx[j] + ucy * q.sny[j]) * sideArea;
       vx[j] = ut.x;
       vy[j] = ut.y;
     }
@@ -993,18 +1011,15 @@ __global__ void __launch_bounds__(256) k_vertical_velocity(DMesh m, double h0, i
   if (threadIdx.x < ec) {
     const int el = threadIdx.x, e = e0 + el;
     if (m.elem_wet[e]) {
+      const VvElem& q = Lel[el];
       const int kb = m.elem_kb[e];
-      const int4 r0 = m.elem[2 * e], r1 = m.elem[2 * e + 1];
-      const int sd[3] = {r0.w, r1.x, r1.y};
       const double* gp = m.gradphi + 6 * (int64_t)e;
-      const double d0 = m.depth[r0.x], d1 = m.depth[r0.y], d2 = m.depth[r0.z];
-      const double ghx = d0 * gp[0] + d1 * gp[2] + d2 * gp[4];
-      const double ghy = d0 * gp[1] + d1 * gp[3] + d2 * gp[5];
+      const double ghx = q.h[0] * gp[0] + q.h[1] * gp[2] + q.h[2] * gp[4];
+      const double ghy = q.h[0] * gp[1] + q.h[1] * gp[3] + q.h[2] * gp[5];
       double bx[3], by[3];
 #pragma unroll
       for (int j = 0; j < 3; ++j) {
-        const int s = sd[j];
-        const double2 u = reinterpret_cast<const double2*>(m.su2)[(int64_t)s * nv + m.side_kb[s]];
+        const double2 u = reinterpret_cast<const double2*>(m.su2)[(int64_t)q.s[j] * nv + q.skb[j]];
         bx[j] = u.x;
         by[j] = u.y;
       }
@@ -1041,8 +1056,10 @@ __global__ void __launch_bounds__(256) k_vertical_velocity(DMesh m, double h0, i
 void launch_vertical_velocity(const DMesh& m, double h0, hipStream_t st) {
   if (m.ne <= 0 || m.nvrt < 2) return;
   const int nvp = m.nvrt - 1;
-  const int E = max(1, 256 / nvp);  // elements per workgroup: about one thread per (element, level)
-  const size_t lds = 8 * ((size_t)3 * E * nvp + E);
+  // elements per workgroup: about one thread per (element, level), within 32 KB of LDS
+  const size_t per_elem = 8 * (3 * (size_t)nvp + 1) + sizeof(VvElem);
+  const int E = max(1, min(256 / nvp, (int)((32 * 1024) / per_elem)));
+  const size_t lds = per_elem * E;
   hipLaunchKernelGGL(k_vertical_velocity, dim3((unsigned)((m.ne + E - 1) / E)), dim3(256), lds, st, m, h0, E);
 }
 

@@ -439,12 +439,13 @@ def test_full_size_1m_channel(ctx):
     mesh.close()
 
 
-@pytest.mark.parametrize("nvrt", [200, 1024])
+@pytest.mark.parametrize("nvrt", [2, 3, 200, 1024])
 def test_momentum_many_levels(ctx, nvrt):
-    """Deep columns (a workgroup holds fewer sides, down to one at 1024 levels), random btu,
-    node-varying bottom levels and a new elevation with a dry corner: side velocities bit for bit."""
+    """Two levels (every side 2D) up to deep columns (a workgroup holds fewer sides, down to one at
+    1024 levels), random btu, node-varying bottom levels and a new elevation with a dry corner:
+    side velocities and element w bit for bit."""
     spec = cases.CaseSpec(f"deep-{nvrt}", 14, 9, 500.0, nvrt=nvrt, depth=10.0, substeps=4)
-    mesh, om, p = _setup(ctx, spec, kb_fn=lambda d: (np.arange(d.size) % 5).astype(np.int32))
+    mesh, om, p = _setup(ctx, spec, kb_fn=lambda d: (np.arange(d.size) % min(5, nvrt - 1)).astype(np.int32))
     rng = np.random.default_rng(nvrt)
     btu = rng.normal(0.0, 0.4, (mesh.ns, nvrt, 2))
     eta_old = mesh.get(L.F_ETA2)
