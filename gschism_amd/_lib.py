@@ -34,7 +34,7 @@ T_XY, T_ELNODE, T_ELSIDE, T_ISDEL, T_ISIDENODE, T_BALL_PTR, T_BALL_ELEM, T_NBR_N
 T_CSR_INDPTR, T_CSR_INDICES, T_CSR_SLOT, T_DEPTH, T_OPEN_BND, T_KB, T_SIDE_TYPE = range(8, 15)
 # field codes
 (F_ETA2, F_ETA1, F_NODE_UVW, F_SIDE_U2, F_BTU, F_FOOT, F_BSTATUS, F_CD, F_GRAV2, F_ETH, F_VSOURCE,
- F_SIDE_U, F_SIDE_HHAT, F_SIDE_CHI, F_GHAT1, F_QEL, F_ZC, F_ELBC, F_WET, F_DEPTH, F_ELEM_W) = range(21)
+ F_SIDE_U, F_SIDE_HHAT, F_SIDE_CHI, F_GHAT1, F_QEL, F_ZC, F_ELBC, F_WET, F_DEPTH, F_ELEM_W, F_ETYPE) = range(22)
 
 BT_WALK_CAP, BT_ITER38, BT_STUCK, BT_OPEN_EXIT, BT_SLIDE = 1, 2, 4, 8, 16
 
@@ -63,7 +63,7 @@ class StepParams(C.Structure):
     _fields_ = [("dt", C.c_double), ("theta", C.c_double), ("h0", C.c_double), ("ramp", C.c_double),
                 ("dtb_min", C.c_double), ("dtb_max", C.c_double), ("velmin_btrack", C.c_double),
                 ("ihhat", C.c_int32), ("iettype", C.c_int32), ("rmaxvel", C.c_double),
-                ("dfv0", C.c_double)]
+                ("dfv0", C.c_double), ("time", C.c_double)]
 
 
 class SolveStats(C.Structure):
@@ -110,6 +110,7 @@ SIGNATURES = {
     "gsh_nodal_velocity": (_I, [_P, C.POINTER(StepParams)]),
     "gsh_momentum": (_I, [_P, C.POINTER(StepParams)]),
     "gsh_vertical_velocity": (_I, [_P, C.POINTER(StepParams)]),
+    "gsh_set_tides": (_I, [_P, _I32, _PD, _PD, _PD, _PD, _PD]),
     "gsh_grid_read": (_I, [C.c_char_p, C.c_char_p, _PP]),
     "gsh_grid_info": (_I, [_P, _PI32, _PI32, _PI32, _PI32, _PI32]),
     "gsh_grid_get": (_I, [_P, _PD, _PD, _PI32, C.POINTER(C.c_int8), _PI32, _PD]),

@@ -72,6 +72,7 @@ _FIELD_SPEC = {
     L.F_SIDE_CHI: (np.float64, "ns", ()), L.F_GHAT1: (np.float64, "ne", (2,)),
     L.F_ZC: (np.float64, "ns", ("nvrt",)), L.F_ELBC: (np.float64, "n", ()),
     L.F_DEPTH: (np.float64, "n", ()), L.F_ELEM_W: (np.float64, "ne", ("nvrt",)),
+    L.F_ETYPE: (np.int8, "n", ()),
 }
 
 
@@ -188,6 +189,16 @@ class Mesh:
     def nodal_velocity(self, params) -> None:
         """Nodal u2/w2 (GSH_F_NODE_UVW) from side u2 and element w (GSH_F_ELEM_W)."""
         check(L.lib().gsh_nodal_velocity(self.h, C.byref(params)))
+
+    def set_tides(self, amig, ff, face, emo, efa) -> None:
+        """Tidal constituents of the open boundary (iettype 3 and 5): angular frequency amig,
+        nodal factor ff, equilibrium argument face [nbfr]; amplitude emo and phase efa [n][nbfr]
+        (phases and arguments in radians)."""
+        a = [np.ascontiguousarray(v, dtype=np.float64) for v in (amig, ff, face)]
+        nb = a[0].size
+        e = [np.ascontiguousarray(v, dtype=np.float64).reshape(self.n, nb) for v in (emo, efa)]
+        pd = C.POINTER(C.c_double)
+        check(L.lib().gsh_set_tides(self.h, nb, *[v.ctypes.data_as(pd) for v in a + e]))
 
     def momentum(self, params) -> None:
         """Side velocities u2 of the next step (GSH_F_SIDE_U2): elevation gradients, the momentum

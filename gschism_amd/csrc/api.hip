@@ -184,6 +184,10 @@ struct gsh_mesh {
   DMesh d{};
   DevPool pool;
   int32_t n_open = 0;
+  // tidal forcing tables (gsh_set_tides): capacities of the pool buffers
+  double *tide_glob = nullptr, *tide_emo = nullptr, *tide_efa = nullptr;
+  int64_t tide_cap = 0;
+  int32_t tide_nb = 0;
 };
 
 // ====================================================================== window graphs
@@ -1088,6 +1092,7 @@ void gsh_step_params_default(gsh_step_params* p) {
   p->iettype = 2;
   p->rmaxvel = 5.0;  // readNamelist.py:192
   p->dfv0 = 0.01;    // readNamelist.py:130
+  p->time = 0.0;
 }
 
 static StepArgs step_args(const gsh_step_params* p) {
@@ -1104,6 +1109,7 @@ static StepArgs step_args(const gsh_step_params* p) {
   a.velmin2 = p->velmin_btrack * p->velmin_btrack;
   a.dfv0 = p->dfv0;
   a.rmaxvel = p->rmaxvel;
+  a.time = p->time;
   a.ihhat = p->ihhat;
   a.iettype = p->iettype;
   // backtrackAllNodes (meshdata.cu:3550-3551)
@@ -1208,6 +1214,7 @@ int gsh_mesh_create(gsh_ctx* ctx, const gsh_mesh_desc* desc, gsh_mesh** out, int
     d.nrec = P.alloc<double2>(2 * (int64_t)n);
     d.eta_lev = P.alloc<double>(n);
     d.node_wet = P.alloc<int8_t>(n);
+    d.etype = P.upload(std::vector<int8_t>(n, -1));
     d.elem = P.upload(erec);
     d.area = P.upload(m->geo.area);
     d.gradphi = P.upload(m->geo.gradphi);
@@ -1336,6 +1343,7 @@ static FieldRef field_ref(const gsh_mesh* m, int field) {
     case GSH_F_WET: return {nullptr, n + ne + ns, 1};
     case GSH_F_DEPTH: return {const_cast<double*>(d.depth), n, 8};
     case GSH_F_ELEM_W: return {d.ew, ne * nv, 8};
+    case GSH_F_ETYPE: return {d.etype, n, 1};
     default: throw Error(GSH_E_ARG, "unknown field");
   }
 }
@@ -1483,6 +1491,47 @@ int gsh_momentum(gsh_mesh* m, const gsh_step_params* p) {
     m->ctx->set();
     launch_momentum(m->d, a, m->ctx->stream);
     HIPCHK(hipGetLastError());
+  })
+}
+
+int gsh_set_tides(gsh_mesh* m, int32_t nbfr, const double* amig, const double* ff, const double* face,
+                  const double* emo, const double* efa) {
+  GSH_API_BODY({
+    GSH_REQUIRE(m, GSH_E_ARG, "mesh is NULL");
+    GSH_REQUIRE(m->ctx, GSH_E_STATE, "host-only mesh (created without a context)");
+    GSH_REQUIRE(nbfr >= 0 && nbfr <= 256, GSH_E_ARG, "nbfr must be in [0, 256]");
+    GSH_REQUIRE(nbfr == 0 || (amig && ff && face && emo && efa), GSH_E_ARG, "NULL tidal array");
+    m->ctx->set();
+    DMesh& d = m->d;
+    if (nbfr == 0) {
+      d.nbfr = 0;
+      return GSH_OK;
+    }
+    const int64_t per = (int64_t)d.n * nbfr;
+    if (per > m->tide_cap) {  // the pool keeps its chunks; a larger table takes a new one
+      m->tide_emo = m->pool.alloc<double>(per);
+      m->tide_efa = m->pool.alloc<double>(per);
+      m->tide_cap = per;
+    }
+    if (nbfr > m->tide_nb) {
+      m->tide_glob = m->pool.alloc<double>(3 * (int64_t)nbfr);
+      m->tide_nb = nbfr;
+    }
+    std::vector<double> g(3 * (size_t)nbfr);
+    for (int j = 0; j < nbfr; ++j) {
+      g[j] = amig[j];
+      g[nbfr + j] = ff[j];
+      g[2 * nbfr + j] = face[j];
+    }
+    hipStream_t st = m->ctx->stream;
+    HIPCHK(hipMemcpyAsync(m->tide_glob, g.data(), sizeof(double) * g.size(), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(m->tide_emo, emo, sizeof(double) * per, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(m->tide_efa, efa, sizeof(double) * per, hipMemcpyHostToDevice, st));
+    HIPCHK(hipStreamSynchronize(st));
+    d.nbfr = nbfr;
+    d.tide = m->tide_glob;
+    d.emo = m->tide_emo;
+    d.efa = m->tide_efa;
   })
 }
 

@@ -32,6 +32,11 @@ struct DMesh {
   double2* nrec;            // [n][2]: (x, y), (eta2, depth), packed for the backtracking gathers
   double* eta_lev;          // [n] eta2 of the last level update (the node z levels follow it)
   int8_t* node_wet;
+  int8_t* etype;            // [n] elevation BC type of each open node (-1: the step parameter iettype)
+  int32_t nbfr;             // tidal constituents of the open boundary (0: none)
+  const double* tide;       // [3][nbfr]: angular frequency, nodal factor, equilibrium argument
+  const double* emo;        // [n][nbfr] amplitude at each open node
+  const double* efa;        // [n][nbfr] phase at each open node
   // elements
   const int4* elem;         // [ne][2]: (n0, n1, n2, s0), (s1, s2, -, -)
   const double* area;
@@ -72,7 +77,7 @@ struct DMesh {
 };
 
 struct StepArgs {
-  double dt, theta, h0, ramp, dtb_min, dtb_max, velmin2, dfv0, rmaxvel;
+  double dt, theta, h0, ramp, dtb_min, dtb_max, velmin2, dfv0, rmaxvel, time;
   int32_t ihhat, iettype, ndelt_min, ndelt_max;
 };
 

@@ -670,16 +670,42 @@ __global__ void k_elem_ghat(DMesh m, StepArgs a) {
   m.ghat1[2 * e + 1] = gy;
 }
 
-// computeElevationBoundaryConditions for iettype 1, 2, 4 without ramp-down of an initial
-// elevation (nramp_elev = 0) (meshdata.cu:4276-4339).
+// Elevation BC type of node i: its own (GSH_F_ETYPE) or, where never set, the step parameter.
+__device__ __forceinline__ int node_etype(const DMesh& m, const StepArgs& a, int i) {
+  const int t = m.etype[i];
+  return t < 0 ? a.iettype : t;
+}
+
+// computeElevationBoundaryConditions (meshdata.cu:4276-4339) for node types 1-5: eth for 1, 2, 4,
+// the tidal constituents for 3, both for 5; no ramp-down of an initial elevation
+// (nramp_elev = 0), no loading tide, no inverse barometric term. Type 0 leaves elbc as it is.
 __global__ void k_elbc(DMesh m, StepArgs a) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= m.n) return;
   if (!m.open_bnd[i]) return;
+  const int iettype = node_etype(m, a, i);
   const double eic = 0.0;
   const double r_eic = (1.0 - a.ramp) * eic;
   const double r_eth = a.ramp * m.eth[i];
-  m.elbc[i] = r_eic + r_eth;
+  double r_eta1_bar = 0.0;
+  if (iettype == 3 || iettype == 5) {
+    const double pi = 3.14159265358979323846;
+    for (int j = 0; j < m.nbfr; ++j) {
+      const double amig = m.tide[j], ff = m.tide[m.nbfr + j], face = m.tide[2 * m.nbfr + j];
+      const double efa = m.efa[(int64_t)i * m.nbfr + j], emo = m.emo[(int64_t)i * m.nbfr + j];
+      const int ncyc = (int)(amig * a.time / (2.0 * pi));
+      const double arg = amig * a.time - (2.0 * pi) * ncyc + face - efa;
+      r_eta1_bar += a.ramp * ff * emo * cos(arg);
+    }
+  }
+  switch (iettype) {
+    case 1:
+    case 2:
+    case 4: m.elbc[i] = r_eic + r_eth; break;
+    case 3: m.elbc[i] = r_eic + r_eta1_bar; break;
+    case 5: m.elbc[i] = r_eic + r_eta1_bar + r_eth; break;
+    default: break;
+  }
 }
 
 void launch_rhs(const DMesh& m, const StepArgs& a, hipStream_t st) {
@@ -1144,15 +1170,14 @@ __device__ __forceinline__ double assemble_row(const DMesh& m, const StepArgs& a
     qel += c3.x;
     qel += c3.y;
   }
-  const bool ess = a.iettype != 0;
-  if (ess && m.open_bnd[i]) {
+  if (m.open_bnd[i] && node_etype(m, a, i) != 0) {
     for (int k = 0; k < len; ++k) rv(k) = 0.0;
     rv(diag) = 1.0;
     qel = m.elbc[i];
-  } else if (ess) {
+  } else {
     for (int j = m.nbr_ptr[i]; j < m.nbr_ptr[i + 1]; ++j) {
       const int nb = m.nbr_node[j];
-      if (m.open_bnd[nb]) {
+      if (m.open_bnd[nb] && node_etype(m, a, nb) != 0) {
         const int sl = m.csr_slot[j];
         qel -= rv(sl) * m.elbc[nb];
         rv(sl) = 0.0;

@@ -142,8 +142,11 @@ int gsh_mesh_get_topology(const gsh_mesh* mesh, int what, void* host_out);
 #define GSH_F_WET        18   /* int8   [n]/[ne]/[ns] packed: nodes, then elements, then sides (1 = wet) */
 #define GSH_F_DEPTH      19   /* double [n]          still-water depth h (dp); levels and wet flags follow at
                                  the next gsh_update_levels */
-#define GSH_F_ELEM_W     20   /* double [ne][nvrt]   element vertical velocity w (computeVerticalVelocity's
-                                 output, meshdata.cu:5967-6177); allocated at first use, zero until set */
+#define GSH_F_ELEM_W     20   /* double [ne][nvrt]   element vertical velocity w (gsh_vertical_velocity, or set
+                                 by the caller); allocated at first use, zero until written */
+#define GSH_F_ETYPE      21   /* int8   [n]          elevation BC type of each open node: 0 none, 1-5 as
+                                                       iettype of bctides.in (readTides.py:161-185);
+                                                       -1 (the initial value) = gsh_step_params.iettype */
 int gsh_mesh_set_field(gsh_mesh* mesh, int field, const void* host, int64_t count);
 int gsh_mesh_get_field(const gsh_mesh* mesh, int field, void* host, int64_t count);
 int gsh_mesh_field_ptr(gsh_mesh* mesh, int field, void** dev_ptr, int64_t* count);
@@ -169,6 +172,7 @@ typedef struct gsh_step_params {
   double rmaxvel;     /* velocity clamp of the nodal reconstruction: |u| <= rmaxvel,
                          |v| <= 1.013 rmaxvel (constants_template.h:142-144)     */
   double dfv0;        /* vertical viscosity of the momentum solve, itur=0 [m^2/s]  */
+  double time;        /* model time [s] of the harmonic elevation BC (iettype 3, 5) */
 } gsh_step_params;
 void gsh_step_params_default(gsh_step_params* p);
 
@@ -189,6 +193,14 @@ int gsh_backtrack_list(gsh_mesh* mesh, const gsh_step_params* p, const int32_t* 
  * computeElemAverageGhat and computeElevationBoundaryConditions (meshdata.cu:4459-4476, 4615-4725,
  * 6183-6424, 4276-4339). */
 int gsh_rhs_producers(gsh_mesh* mesh, const gsh_step_params* p);
+
+/* Tidal forcing of the open boundary for iettype 3 and 5 (bctides.in, readTides.py:96-118,
+ * 170-185; used by computeElevationBoundaryConditions, meshdata.cu:4298-4316): nbfr
+ * constituents with angular frequency amig, nodal factor ff and equilibrium argument face
+ * [rad], and per node amplitude emo [n][nbfr] and phase efa [n][nbfr] [rad] (read at open
+ * nodes only). nbfr = 0 removes the forcing; nbfr <= 256. */
+int gsh_set_tides(gsh_mesh* mesh, int32_t nbfr, const double* amig, const double* ff, const double* face,
+                  const double* emo, const double* efa);
 
 /* The step before backtracking: nodal velocities from side and element velocities.
  * zeroNodalVelocities, computeNodalVerticalVelocities1/2 and computeNodalHorizontalVelocities1/2

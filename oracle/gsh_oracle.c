@@ -79,6 +79,10 @@ struct or_mesh {
   double *sU, *hhat, *chi, *ghat1, *zc, *nz;
   int8_t *node_dry, *elem_dry, *side_dry;
   int32_t *elem_kb, *side_kb;
+  /* elevation BC type per node (-1: the parameter iettype) and tidal constituents */
+  int8_t* etype;
+  int32_t nbfr;
+  double *amig, *ff, *face, *emo, *efa;
 };
 
 static int nxq(int offset, int index, int size) { return (index + offset + 16 * size) % size; }
@@ -99,7 +103,8 @@ void or_mesh_free(or_mesh* m) {
                   m->indices, m->csr_slot, m->diag_slot, m->area, m->gradphi, m->ssign, m->scx, m->scy,
                   m->slen, m->snx, m->sny, m->eta2, m->eta1, m->uvw, m->su2, m->btu, m->cd, m->grav2,
                   m->eth, m->vsource, m->elbc, m->foot, m->bstatus, m->sU, m->hhat, m->chi, m->ghat1,
-                  m->zc, m->nz, m->node_dry, m->elem_dry, m->side_dry, m->elem_kb, m->side_kb, m->ew};
+                  m->zc, m->nz, m->node_dry, m->elem_dry, m->side_dry, m->elem_kb, m->side_kb, m->ew,
+                  m->etype, m->amig, m->ff, m->face, m->emo, m->efa};
   for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; ++i) free(ptrs[i]);
   free(m);
 }
@@ -420,6 +425,8 @@ or_mesh* or_mesh_new(int32_t n, int32_t ne, int32_t nvrt, const double* xy, cons
   m->grav2 = xcalloc(n, sizeof(double));
   for (int i = 0; i < n; ++i) m->cd[i] = 2.5e-3, m->grav2[i] = 9.81;
   m->eth = xcalloc(n, sizeof(double));
+  m->etype = xcalloc(n, sizeof(int8_t));
+  for (int i = 0; i < n; ++i) m->etype[i] = -1;
   m->elbc = xcalloc(n, sizeof(double));
   m->vsource = xcalloc(ne, sizeof(double));
   m->sU = xcalloc(2 * ns, sizeof(double));
@@ -490,6 +497,7 @@ static int field_info(const or_mesh* m, int f, void** ptr, size_t* bytes) {
     case 17: *ptr = m->elbc; *bytes = 8 * n; return 0;
     case 19: *ptr = m->depth; *bytes = 8 * n; return 0;
     case 20: *ptr = m->ew; *bytes = 8 * ne * nv; return 0;
+    case 21: *ptr = m->etype; *bytes = n; return 0;
     default: set_err("unknown field %d", f); return -1;
   }
 }
@@ -1232,16 +1240,67 @@ void or_backtrack(or_mesh* m, const or_params* p, const int32_t* ls, const int32
 }
 
 /* ------------------------------------------------------------------ RHS producers */
+/* a node's elevation BC type: its own, or the parameter iettype where never set */
+static int node_etype(const or_mesh* m, const or_params* p, int i) {
+  return m->etype[i] < 0 ? p->iettype : m->etype[i];
+}
+
+int or_set_tides(or_mesh* m, int32_t nbfr, const double* amig, const double* ff, const double* face,
+                 const double* emo, const double* efa) {
+  if (nbfr < 0) {
+    set_err("nbfr < 0");
+    return -1;
+  }
+  free(m->amig);
+  free(m->ff);
+  free(m->face);
+  free(m->emo);
+  free(m->efa);
+  m->nbfr = nbfr;
+  m->amig = xcalloc(nbfr, sizeof(double));
+  m->ff = xcalloc(nbfr, sizeof(double));
+  m->face = xcalloc(nbfr, sizeof(double));
+  m->emo = xcalloc((size_t)m->n * nbfr, sizeof(double));
+  m->efa = xcalloc((size_t)m->n * nbfr, sizeof(double));
+  if (nbfr) {
+    memcpy(m->amig, amig, 8 * (size_t)nbfr);
+    memcpy(m->ff, ff, 8 * (size_t)nbfr);
+    memcpy(m->face, face, 8 * (size_t)nbfr);
+    memcpy(m->emo, emo, 8 * (size_t)m->n * nbfr);
+    memcpy(m->efa, efa, 8 * (size_t)m->n * nbfr);
+  }
+  return 0;
+}
+
 void or_rhs(or_mesh* m, const or_params* p) {
   const int n = m->n, ne = m->ne, ns = m->ns, nv = m->nvrt;
   const double dt = p->dt;
-  /* computeElevationBoundaryConditions, iettype 1/2/4, nramp_elev=0 (meshdata.cu:4276-4339) */
+  /* computeElevationBoundaryConditions, node types 1-5, nramp_elev=0, no loading tide or inverse
+   * barometric term (meshdata.cu:4276-4339) */
   for (int i = 0; i < n; ++i) {
     if (!m->open[i]) continue;
+    const int iettype = node_etype(m, p, i);
     const double eic = 0.0;
     const double r_eic = (1.0 - p->ramp) * eic;
     const double r_eth = p->ramp * m->eth[i];
-    m->elbc[i] = r_eic + r_eth;
+    double r_eta1_bar = 0.0;
+    if (iettype == 3 || iettype == 5) {
+      const double pi = 3.14159265358979323846; /* meshdata.h:12 */
+      for (int jfr = 0; jfr < m->nbfr; ++jfr) {
+        const double efa = m->efa[(size_t)i * m->nbfr + jfr], emo = m->emo[(size_t)i * m->nbfr + jfr];
+        const int ncyc = (int)(m->amig[jfr] * p->time / (2.0 * pi));
+        const double arg = m->amig[jfr] * p->time - (2.0 * pi) * ncyc + m->face[jfr] - efa;
+        r_eta1_bar += p->ramp * m->ff[jfr] * emo * cos(arg);
+      }
+    }
+    switch (iettype) {
+      case 1:
+      case 2:
+      case 4: m->elbc[i] = r_eic + r_eth; break;
+      case 3: m->elbc[i] = r_eic + r_eta1_bar; break;
+      case 5: m->elbc[i] = r_eic + r_eta1_bar + r_eth; break;
+      default: break;
+    }
   }
   for (int s = 0; s < ns; ++s) {
     const double* u2 = m->su2 + 2 * (size_t)s * nv;
@@ -1351,7 +1410,6 @@ static void add_to_row(const or_mesh* m, int node1, int node, double value, doub
 void or_assemble(or_mesh* m, const or_params* p, double* val, double* b, int nthreads) {
   const int n = m->n;
   const double dt = p->dt, thetai = p->theta;
-  const int ess = p->iettype != 0;
 #ifdef _OPENMP
   if (nthreads < 1) nthreads = 1;
 #pragma omp parallel for schedule(static) num_threads(nthreads)
@@ -1405,16 +1463,17 @@ void or_assemble(or_mesh* m, const or_params* p, double* val, double* b, int nth
       }
       qel += (dt / 3) * m->vsource[e];
     }
-    /* setEssentialBC (meshdata.cu:4400-4446) */
-    if (ess && m->open[node1]) {
+    /* setEssentialBC (meshdata.cu:4400-4446): essential at open nodes of type != 0
+     * (isEssentialBC = iettype != 0, meshdata.cu:2506) */
+    if (m->open[node1] && node_etype(m, p, node1) != 0) {
       for (int j = 0; j < len; ++j) row[j] = 0.0;
       row[m->diag_slot[node1]] = 1.0;
       qel = m->elbc[node1];
-    } else if (ess) {
+    } else {
       const int bb = m->nnp_ptr[node1], nn = m->nnp_ptr[node1 + 1] - bb;
       for (int j = 0; j < nn; ++j) {
         const int nb = m->nnp[bb + j];
-        if (m->open[nb]) {
+        if (m->open[nb] && node_etype(m, p, nb) != 0) {
           qel -= row[m->csr_slot[bb + j]] * m->elbc[nb];
           row[m->csr_slot[bb + j]] = 0.0;
         }

@@ -20,6 +20,7 @@ typedef struct or_params {
   int32_t ihhat, iettype;
   double rmaxvel;
   double dfv0;
+  double time;
 } or_params;
 
 const char* or_error(void);
@@ -41,6 +42,9 @@ void or_levels(or_mesh* m, const or_params* p);
 void or_backtrack(or_mesh* m, const or_params* p, const int32_t* list_side, const int32_t* list_lev,
                   int64_t count, int nthreads);
 void or_rhs(or_mesh* m, const or_params* p);
+/* tidal constituents of the open boundary (amig, ff, face [nbfr]; emo, efa [n][nbfr]) */
+int or_set_tides(or_mesh* m, int32_t nbfr, const double* amig, const double* ff, const double* face,
+                 const double* emo, const double* efa);
 /* Nodal velocities (field 2) from side velocities (field 3) and element w (field 20), ball order. */
 void or_nodal_velocity(or_mesh* m, const or_params* p, int nthreads);
 /* side velocities u2 of the next step: elevation gradients, momentum solve, land-boundary slip */

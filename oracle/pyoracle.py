@@ -17,7 +17,7 @@ class OrParams(C.Structure):
     _fields_ = [("dt", C.c_double), ("theta", C.c_double), ("h0", C.c_double), ("ramp", C.c_double),
                 ("dtb_min", C.c_double), ("dtb_max", C.c_double), ("velmin_btrack", C.c_double),
                 ("ihhat", C.c_int32), ("iettype", C.c_int32), ("rmaxvel", C.c_double),
-                ("dfv0", C.c_double)]
+                ("dfv0", C.c_double), ("time", C.c_double)]
 
 
 _P = C.c_void_p
@@ -38,6 +38,7 @@ _SIG = {
     "or_rhs": (None, [_P, C.POINTER(OrParams)]),
     "or_nodal_velocity": (None, [_P, C.POINTER(OrParams), C.c_int]),
     "or_momentum": (None, [_P, C.POINTER(OrParams)]),
+    "or_set_tides": (C.c_int, [_P, C.c_int32, _PD, _PD, _PD, _PD, _PD]),
     "or_vertical_velocity": (None, [_P]),
     "or_assemble": (None, [_P, C.POINTER(OrParams), _PD, _PD, C.c_int]),
     "or_pcg": (C.c_int32, [C.c_int32, _PI32, _PI32, _PD, _PD, _PD, _PD, _PD, C.c_double, C.c_int32, C.c_int32,
@@ -101,6 +102,7 @@ _FIELDS = {
     14: (np.float64, lambda m: (m.ne, 2)), 16: (np.float64, lambda m: (m.ns, m.nvrt)),
     17: (np.float64, lambda m: (m.n,)), 18: (np.int8, lambda m: (m.n + m.ne + m.ns,)),
     19: (np.float64, lambda m: (m.n,)), 20: (np.float64, lambda m: (m.ne, m.nvrt)),
+    21: (np.int8, lambda m: (m.n,)),
 }
 
 _TOPO = {
@@ -187,6 +189,14 @@ class OracleMesh:
 
     def nodal_velocity(self, p, nthreads: int = 1) -> None:
         lib().or_nodal_velocity(self.h, C.byref(params(p)), nthreads)
+
+    def set_tides(self, amig, ff, face, emo, efa) -> None:
+        """Tidal constituents (amig, ff, face [nbfr]) and open-node amplitudes and phases [n][nbfr]."""
+        a = [np.ascontiguousarray(v, dtype=np.float64) for v in (amig, ff, face)]
+        nb = a[0].size
+        e = [np.ascontiguousarray(v, dtype=np.float64).reshape(self.n, nb) for v in (emo, efa)]
+        if lib().or_set_tides(self.h, nb, *[v.ctypes.data_as(_PD) for v in a + e]):
+            raise RuntimeError(lib().or_error().decode())
 
     def momentum(self, p) -> None:
         lib().or_momentum(self.h, C.byref(params(p)))

@@ -485,3 +485,63 @@ def test_objects_outlive_their_context():
     assert st == 0 and it > 0 and np.isfinite(x).all()
     s.close()
     mesh.close()
+
+
+def test_elevation_bc_types_and_tides(ctx, tmp_path):
+    """A bctides.in with an elevation-plus-tides boundary (type 5, two constituents) applied through
+    the reader, then per-node types mixed over the open nodes (0, 2, 3, 5) at model time t: elbc
+    within 1e-14 of the oracle where cosines enter (the device and host cosines may differ in the
+    last bit), bit for bit elsewhere; then assembly with per-node essential rows and the solve bit
+    for bit from the same elbc."""
+    from gschism_amd.bctides import apply_bctides, elevation_forcing, read_bctides
+
+    spec = cases.CaseSpec("tides-60x30", 60, 30, 500.0, nvrt=5, depth=None, substeps=4)
+    data = cases.make_case(spec)
+    mesh, om, p = _setup(ctx, spec)
+    seg_file = np.flatnonzero(data.open_bnd)
+    seg_file = seg_file[np.argsort(data.xy[seg_file, 1])]
+    k = seg_file.size
+    lines = ["01/01/2020 00:00:00", "0 40.", "2", "M2", "1.405189D-04 0.97 90.0", "S2", "1.454441D-04 1.0 0.0",
+             "1", f"{k} 5 0 0 0", "M2"]
+    lines += [f"{0.4 + 0.01 * j:.3f} {10.0 + j:.1f}" for j in range(k)] + ["S2"]
+    lines += [f"{0.15 + 0.005 * j:.4f} {40.0 + 2 * j:.1f}" for j in range(k)]
+    path = tmp_path / "bctides.in"
+    path.write_text("\n".join(lines) + "\n")
+    tide = read_bctides(str(path), 1, k)
+    apply_bctides(mesh, tide, [seg_file])
+    inv = np.empty_like(mesh.node_perm)
+    inv[mesh.node_perm] = np.arange(mesh.n)
+    seg = inv[seg_file]
+    etype, eth, emo, efa = elevation_forcing(tide, [seg], mesh.n)
+    assert np.array_equal(mesh.get(L.F_ETYPE), etype)
+    # mix the types along the boundary; a constant elevation for the types that use one
+    etype[seg] = np.array([0, 2, 3, 5], dtype=np.int8)[np.arange(k) % 4]
+    eth[seg] = 0.05 + 0.01 * np.arange(k)
+    for m in (mesh, om):
+        m.set(L.F_ETYPE, etype)
+        m.set(L.F_ETH, eth)
+    om.set_tides(tide.amig, tide.ff, tide.face, emo, efa)
+    p.time, p.ramp = 2.5e5, 0.9
+    print(f"[gpu] elevation BC types, {k} open nodes", flush=True)
+    mesh.backtrack(p)
+    om.backtrack(p, nthreads=8)
+    mesh.rhs_producers(p)
+    om.rhs(p)
+    got, ref = mesh.get(L.F_ELBC), om.get(L.F_ELBC)
+    harm = np.isin(np.arange(mesh.n), seg[np.isin(etype[seg], (3, 5))])
+    assert np.abs(got[harm] - ref[harm]).max() <= 1e-14
+    assert_bitwise(got[~harm], ref[~harm], "elbc of the non-tidal nodes")
+    assert np.abs(ref[harm]).max() > 0.1
+    om.set(L.F_ELBC, got)
+    solver = core.Solver(mesh)
+    solver.assemble(p)
+    val, b = om.assemble(p)
+    assert_bitwise(solver.get_values(), val, "CSR values with per-node essential rows")
+    assert_bitwise(solver.get_rhs(), b, "RHS with per-node essential rows")
+    x, it, st = solver.solve(None, rtol=1e-12, maxiter=2000)
+    xo, ito, _, _ = O.pcg(mesh.topology(L.T_CSR_INDPTR), mesh.topology(L.T_CSR_INDICES), val, b, rtol=1e-12,
+                          maxiter=2000)
+    assert st == 0 and it == ito
+    assert_bitwise(x, xo, "solution with per-node elevation BCs")
+    ess = seg[etype[seg] != 0]
+    assert np.abs(x[ess] - got[ess]).max() < 1e-10  # identity rows: x = elbc up to the solve tolerance
