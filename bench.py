@@ -628,6 +628,31 @@ def main() -> int:
                                "sample": "1 solve", "bitwise_equal": bool(np.array_equal(x1, x_ssor))}}
         log(f"[bench] cpu baseline: {cpu}")
 
+    # ---- whole time steps of the rebuilt path on the same mesh (after every measurement above
+    # that needs the original state): backtracking, RHS producers, assembly, FP64 SSOR solve,
+    # elevation update, momentum, element w, levels, nodal velocities
+    def one_step():
+        mesh.backtrack(p)
+        mesh.rhs_producers(p)
+        solver.assemble(p)
+        _, it_s, st_s = solver.solve(None, rtol, maxiter, check_every, want_x=False)
+        solver.update_elevation()
+        mesh.momentum(p)
+        mesh.vertical_velocity(p)
+        mesh.update_levels(p)
+        mesh.nodal_velocity(p)
+        return it_s, st_s
+
+    one_step()  # warm
+    ctx.synchronize()
+    t_st = time.perf_counter()
+    step_its = [one_step() for _ in range(3)]
+    ctx.synchronize()
+    ms_step = (time.perf_counter() - t_st) * 1e3 / 3
+    full_step = {"ms": round(ms_step, 3), "steps_per_s": round(1e3 / ms_step, 3),
+                 "solver_iters": [int(i) for i, _ in step_its], "status": [int(s) for _, s in step_its]}
+    log(f"[bench] full step: {full_step}")
+
     mixed4m = None
     if not args.no_mixed_4m:
         mixed4m = mixed_4m_leg(ctx, L, cases, core, 3, rtol, 1e-10, check_every)
@@ -674,6 +699,7 @@ def main() -> int:
                       "substeps": ndelt, "effective_gbs": round(bt_bytes / (ms_bt * 1e-3) / 1e9, 1)},
         "nodal_velocity": {"ms": round(ms_nv, 3), "bytes": nv_bytes, "gbs": round(nv_bytes / (ms_nv * 1e-3) / 1e9, 1)},
         "momentum": {"ms": round(ms_mom, 3), "bytes": mom_bytes, "gbs": round(mom_bytes / (ms_mom * 1e-3) / 1e9, 1)},
+        "full_step": full_step,
         "vertical_velocity": {"ms": round(ms_vv, 3), "bytes": vv_bytes,
                               "gbs": round(vv_bytes / (ms_vv * 1e-3) / 1e9, 1)},
         "assembly_ms": round(ms_asm, 3),

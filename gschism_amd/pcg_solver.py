@@ -195,7 +195,10 @@ def updateElevation(mesh: Mesh, solver: PCGSolverMultiGPU) -> None:
 
 
 def hot_path_step(mesh: Mesh, solver: PCGSolverMultiGPU, params, maxIter=1500, tol=1e-12) -> int:
-    """One time step of the rebuilt path in schism_step order (schism_step.py:183-275)."""
+    """One time step of the rebuilt path in schism_step order (schism_step.py:183-331): ELM
+    backtracking, RHS producers, assembly, the wave-equation solve, the elevation update, the
+    momentum solve, the element vertical velocity, the level update and the nodal velocities the
+    next step backtracks through."""
     backtrackAllNodes(mesh, params)
     computeRHSProducers(mesh, params)
     setupCoefficientMatrixAndRHS(mesh, solver, params)
