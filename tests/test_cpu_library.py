@@ -176,3 +176,19 @@ def test_mirror_constructor_checks():
         ps.PCGSolverMultiGPU(ps.CudaDeviceConfig([0, 1]), [10], [20])
     with pytest.raises(ValueError):
         ps.PCGSolverMultiGPU(ps.CudaDeviceConfig([0, 1]), [10, 10], [20, 20])
+
+
+def test_c_harness_builds_and_links():
+    """The plain-C caller of the C ABI compiles with -Werror against include/gsh.h, links to the
+    in-tree library and runs its device-free query."""
+    import importlib.util
+    import subprocess
+    from pathlib import Path
+
+    here = Path(__file__).resolve().parent / "c_abi"
+    spec = importlib.util.spec_from_file_location("c_abi_build", here / "build.py")
+    hb = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(hb)
+    exe = hb.build()
+    r = subprocess.run([str(exe), "--version"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0 and r.stdout.startswith("gsh "), (r.stdout, r.stderr)

@@ -85,3 +85,18 @@ def test_hot_path_step_matches_manual_sequence(ctx):
         assert_bitwise(mesh_a.get(L.F_ETA2), mesh_b.get(L.F_ETA2), f"step {step} eta2")
         assert_bitwise(mesh_a.get(L.F_SIDE_U2), mesh_b.get(L.F_SIDE_U2), f"step {step} side velocity")
         assert_bitwise(mesh_a.get(L.F_NODE_UVW), mesh_b.get(L.F_NODE_UVW), f"step {step} nodal velocities")
+
+
+def test_native_c_caller_runs_a_step(ctx):
+    """tests/c_abi/step_harness (plain C, built by build()) runs one whole step through the C ABI
+    in a process of its own: every call succeeds, the solve converges, the state stays finite."""
+    import subprocess
+    from pathlib import Path
+
+    exe = Path(__file__).resolve().parent / "c_abi" / "_build" / "step_harness"
+    assert exe.exists(), "run __graft_entry__.build() first"
+    print("[gpu] native C caller: step_harness 40 24 6", flush=True)
+    r = subprocess.run([str(exe), "40", "24", "6"], capture_output=True, text=True, timeout=120)
+    print(r.stdout.strip(), r.stderr.strip(), flush=True)
+    assert r.returncode == 0, (r.stdout, r.stderr)
+    assert "status=0" in r.stdout and "finite=1" in r.stdout
