@@ -1,9 +1,11 @@
 """Writes the golden fixtures of the wave-equation step from the oracle.
 
 Each fixture holds a small mesh, its initial state and step parameters, and the oracle's
-outputs of two consecutive steps (levels, backtracking, RHS producers, assembly with
-essential BC, PCG with the extrapolated guess, elevation update). They pin the oracle
-against regression and give the device library fixed vectors to reproduce bit for bit.
+outputs of two consecutive whole steps (levels, backtracking, RHS producers, assembly with
+essential BC, PCG with the extrapolated guess, elevation update, momentum, element vertical
+velocity, levels, nodal velocities; the second step backtracks through the velocities the
+first produced). They pin the oracle against regression and give the device library fixed
+vectors to reproduce bit for bit.
 
     python tests/golden/make_golden.py      # rewrites tests/golden/*.npz
 """
@@ -24,7 +26,8 @@ from gschism_amd import cases  # noqa: E402
 from oracle import pyoracle as O  # noqa: E402
 
 STEPS = 2
-PARAM_KEYS = ("dt", "theta", "h0", "ramp", "dtb_min", "dtb_max", "velmin_btrack", "ihhat", "iettype")
+PARAM_KEYS = ("dt", "theta", "h0", "ramp", "dtb_min", "dtb_max", "velmin_btrack", "ihhat", "iettype", "rmaxvel",
+              "dfv0")
 
 
 def _island(nx, ny, dx):
@@ -89,8 +92,14 @@ def run_oracle(inp, p, steps=STEPS, rtol=1e-12, maxiter=2000):
         if s == 0:  # the bulkier per-level and matrix outputs of the first step only
             out.update({"btu_0": om.get(L.F_BTU), "hhat_0": om.get(L.F_SIDE_HHAT), "val_0": val})
         hist = [x, hist[0], hist[1]]
+        om.set(L.F_ETA1, om.get(L.F_ETA2))
         om.set(L.F_ETA2, x)
+        om.momentum(p)
+        om.vertical_velocity()
+        out.update({f"su2_{s}": om.get(L.F_SIDE_U2), f"ew_{s}": om.get(L.F_ELEM_W)})
         om.levels(p)
+        om.nodal_velocity(p)
+        out[f"uvw_{s}"] = om.get(L.F_NODE_UVW)
     out["zc_end"] = om.get(L.F_ZC)
     return out
 

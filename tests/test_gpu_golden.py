@@ -41,7 +41,13 @@ def test_library_reproduces_fixture(ctx, name):
             assert_bitwise(mesh.get(L.F_SIDE_HHAT), fx["hhat_0"], f"{name} hhat")
             assert_bitwise(solver.get_values(), fx["val_0"], f"{name} matrix")
         solver.update_elevation()
+        mesh.momentum(p)
+        mesh.vertical_velocity(p)
+        assert_bitwise(mesh.get(L.F_SIDE_U2), fx[f"su2_{s}"], f"{name} step {s} side velocity")
+        assert_bitwise(mesh.get(L.F_ELEM_W), fx[f"ew_{s}"], f"{name} step {s} element w")
         mesh.update_levels(p)
+        mesh.nodal_velocity(p)
+        assert_bitwise(mesh.get(L.F_NODE_UVW), fx[f"uvw_{s}"], f"{name} step {s} nodal velocities")
     assert_bitwise(mesh.get(L.F_ZC), fx["zc_end"], f"{name} zc")
     solver.close()
     mesh.close()
