@@ -545,3 +545,35 @@ def test_elevation_bc_types_and_tides(ctx, tmp_path):
     assert_bitwise(x, xo, "solution with per-node elevation BCs")
     ess = seg[etype[seg] != 0]
     assert np.abs(x[ess] - got[ess]).max() < 1e-10  # identity rows: x = elbc up to the solve tolerance
+
+
+def test_momentum_branches(ctx):
+    """Momentum branches the default case does not reach: no vertical viscosity (the pure mass
+    matrix), sides without drag (u2[kb] = u2[kb+1]) next to sides with drag, clamping active, dry
+    sides of an island: side velocities and element w bit for bit."""
+    spec = cases.CaseSpec("island-48x32", 48, 32, 400.0, nvrt=9, depth=10.0, substeps=4)
+    mesh, om, p = _setup(ctx, spec, kb_fn=_lsc_kb, depth_fn=_island_depth)
+    rng = np.random.default_rng(7)
+    cd = np.where(np.arange(mesh.n) % 2 == 0, 0.0, 3e-3)
+    btu = rng.normal(0.0, 0.4, (mesh.ns, mesh.nvrt, 2))
+    eta_old = mesh.get(L.F_ETA2)
+    eta_new = eta_old + 0.05 * rng.normal(size=mesh.n)
+    p.dfv0, p.rmaxvel = 0.0, 0.3
+    for m in (mesh, om):
+        m.set(L.F_CD, cd)
+        m.set(L.F_BTU, btu)
+    mesh.rhs_producers(p)
+    om.rhs(p)
+    for m in (mesh, om):
+        m.set(L.F_ETA1, eta_old)
+        m.set(L.F_ETA2, eta_new)
+    print(f"[gpu] momentum branches ns={mesh.ns}", flush=True)
+    mesh.momentum(p)
+    om.momentum(p)
+    u2 = mesh.get(L.F_SIDE_U2)
+    assert_bitwise(u2, om.get(L.F_SIDE_U2), "side velocity, no viscosity, mixed drag, clamping")
+    assert np.abs(u2[..., 0]).max() == 0.3
+    mesh.vertical_velocity(p)
+    om.vertical_velocity()
+    assert_bitwise(mesh.get(L.F_ELEM_W), om.get(L.F_ELEM_W), "element w")
+    mesh.close()
