@@ -63,7 +63,8 @@ class StepParams(C.Structure):
     _fields_ = [("dt", C.c_double), ("theta", C.c_double), ("h0", C.c_double), ("ramp", C.c_double),
                 ("dtb_min", C.c_double), ("dtb_max", C.c_double), ("velmin_btrack", C.c_double),
                 ("ihhat", C.c_int32), ("iettype", C.c_int32), ("rmaxvel", C.c_double),
-                ("dfv0", C.c_double), ("time", C.c_double)]
+                ("dfv0", C.c_double), ("time", C.c_double), ("shapiro0", C.c_double),
+                ("ishapiro", C.c_int32)]
 
 
 class SolveStats(C.Structure):

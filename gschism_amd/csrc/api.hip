@@ -57,6 +57,9 @@ struct DevPool {
       void* p = nullptr;
       HIPCHK(hipMalloc(&p, cb));
       HIPCHK(hipMemset(p, 0, cb));
+      // the fill runs on the null stream, which the library's non-blocking streams do not wait
+      // for: it must be complete before a kernel on those streams writes the new buffer
+      HIPCHK(hipStreamSynchronize(nullptr));
       ptrs.push_back(p);
       cur = static_cast<char*>(p);
       left = cb;
@@ -188,6 +191,7 @@ struct gsh_mesh {
   double *tide_glob = nullptr, *tide_emo = nullptr, *tide_efa = nullptr;
   int64_t tide_cap = 0;
   int32_t tide_nb = 0;
+  double* shap_tmp = nullptr;  // Shapiro filter output before it replaces u2 (first use)
 };
 
 // ====================================================================== window graphs
@@ -1093,6 +1097,8 @@ void gsh_step_params_default(gsh_step_params* p) {
   p->rmaxvel = 5.0;  // readNamelist.py:192
   p->dfv0 = 0.01;    // readNamelist.py:130
   p->time = 0.0;
+  p->shapiro0 = 0.5;  // readNamelist.py:95
+  p->ishapiro = 1;    // readNamelist.py:94
 }
 
 static StepArgs step_args(const gsh_step_params* p) {
@@ -1110,6 +1116,8 @@ static StepArgs step_args(const gsh_step_params* p) {
   a.dfv0 = p->dfv0;
   a.rmaxvel = p->rmaxvel;
   a.time = p->time;
+  a.shapiro0 = p->shapiro0;
+  a.ishapiro = p->ishapiro;
   a.ihhat = p->ihhat;
   a.iettype = p->iettype;
   // backtrackAllNodes (meshdata.cu:3550-3551)
@@ -1487,9 +1495,16 @@ int gsh_momentum(gsh_mesh* m, const gsh_step_params* p) {
     GSH_REQUIRE(m->ctx, GSH_E_STATE, "host-only mesh (created without a context)");
     GSH_REQUIRE(p->rmaxvel >= 0.0, GSH_E_ARG, "rmaxvel must be >= 0");
     GSH_REQUIRE(p->dfv0 >= 0.0, GSH_E_ARG, "dfv0 must be >= 0");
+    GSH_REQUIRE(p->ishapiro == 0 || p->ishapiro == 1, GSH_E_ARG, "ishapiro must be 0 or 1");
     const StepArgs a = step_args(p);
     m->ctx->set();
     launch_momentum(m->d, a, m->ctx->stream);
+    if (a.ishapiro == 1) {
+      // the filtered values go through a scratch [ns][nvrt][2] of the mesh's pool (the
+      // reference reuses its baroclinic array bcc for them)
+      if (!m->shap_tmp) m->shap_tmp = m->pool.alloc<double>(2 * (int64_t)m->d.ns * m->d.nvrt);
+      launch_shapiro(m->d, a.shapiro0, m->shap_tmp, m->ctx->stream);
+    }
     HIPCHK(hipGetLastError());
   })
 }

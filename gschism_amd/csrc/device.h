@@ -77,8 +77,8 @@ struct DMesh {
 };
 
 struct StepArgs {
-  double dt, theta, h0, ramp, dtb_min, dtb_max, velmin2, dfv0, rmaxvel, time;
-  int32_t ihhat, iettype, ndelt_min, ndelt_max;
+  double dt, theta, h0, ramp, dtb_min, dtb_max, velmin2, dfv0, rmaxvel, time, shapiro0;
+  int32_t ihhat, iettype, ndelt_min, ndelt_max, ishapiro;
 };
 
 // ---------------------------------------------------------------- PCG
@@ -175,6 +175,7 @@ void launch_backtrack(const DMesh& m, const StepArgs& a, const int32_t* list_sid
 void launch_rhs(const DMesh& m, const StepArgs& a, hipStream_t s);
 void launch_nodal_velocity(const DMesh& m, double rmaxvel, hipStream_t s);
 void launch_momentum(const DMesh& m, const StepArgs& a, hipStream_t s);
+void launch_shapiro(const DMesh& m, double shapiro0, double* tmp, hipStream_t s);
 void launch_vertical_velocity(const DMesh& m, double h0, hipStream_t s);
 void launch_assemble(const DMesh& m, const StepArgs& a, double* val, double* b, hipStream_t s);
 void launch_update_elevation(const DMesh& m, const double* x, hipStream_t s);

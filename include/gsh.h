@@ -173,6 +173,8 @@ typedef struct gsh_step_params {
                          |v| <= 1.013 rmaxvel (constants_template.h:142-144)     */
   double dfv0;        /* vertical viscosity of the momentum solve, itur=0 [m^2/s]  */
   double time;        /* model time [s] of the harmonic elevation BC (iettype 3, 5) */
+  double shapiro0;    /* Shapiro filter strength of the side velocities (readNamelist.py:95) */
+  int32_t ishapiro;   /* 1 = gsh_momentum applies the Shapiro filter (readNamelist.py:94)  */
 } gsh_step_params;
 void gsh_step_params_default(gsh_step_params* p);
 
@@ -217,6 +219,10 @@ int gsh_nodal_velocity(gsh_mesh* mesh, const gsh_step_params* p);
  * 3D sides solve their vertical tridiagonal system, 2D sides the depth-averaged update; land
  * boundary sides keep the tangential component. */
 int gsh_momentum(gsh_mesh* mesh, const gsh_step_params* p);
+/* With p->ishapiro == 1, gsh_momentum ends with shapiroFilter1/2 (meshdata.cu:5449-5514, called at
+ * schism_step.py:289-292): every wet internal side's u2[k], k > kb, becomes
+ * u2 + shapiro0 (sum of its four neighbour sides' u2 / 4 - u2), the neighbours' levels clamped to
+ * max(k, kb + 1) where wet; the 2D bottom slot follows the surface value. */
 
 /* computeVerticalVelocity (meshdata.cu:5978-6102), triangles, ics=1, imm=0: GSH_F_ELEM_W from
  * GSH_F_SIDE_U2 (after gsh_momentum) and the levels of the last gsh_update_levels (node z from

@@ -702,6 +702,60 @@ static void side_grads(const or_mesh* m, int s, double h0, double g1[2], double 
  *       u2[kb] = u2[kb+1] when Cd = 0, else 0. (The reference's loop also assigns k = kb from
  *       soln[-1], which the bottom condition then overwrites; it is not restated.)
  *   k < kb: 0. Land boundary sides then keep only the tangential component for k >= kb. */
+/* shapiroFilter1 + shapiroFilter2 (meshdata.cu:5449-5514), niter_shap = 1: for each wet internal
+ * side, bcc[k] = u2[k] + shapiro0 (0.25 sum_j u2_j[kin_j] - u2[k]) for k > kb, over its four
+ * neighbours Side::neigh(j) (meshdata.h:1409-1422: element j & 1, the side before (j < 2) or after
+ * it there), kin_j = max(k, kb_j + 1) for a wet neighbour; then u2 <- bcc above kb (2D: u2[kb+1]
+ * and u2[kb] both take bcc[kb+1]) and u2[k] = 0 for k < kb - 1. */
+static void shapiro_filter(or_mesh* m, double shapiro0) {
+  const int ns = m->ns, nv = m->nvrt;
+  double* bcc = xcalloc(2 * (size_t)ns * nv, sizeof(double));
+  for (int s = 0; s < ns; ++s) {
+    const int e0 = m->isdel[2 * s], e1 = m->isdel[2 * s + 1];
+    if (m->side_dry[s] || e1 < 0) continue;
+    const int kb = m->side_kb[s];
+    const int es[2] = {e0, e1};
+    int nb[4];
+    for (int j = 0; j < 4; ++j) {
+      const int e = es[j & 1], off = 2 * (j >> 1) - 1;
+      int loc = 0;
+      while (m->elside[3 * e + loc] != s) ++loc;
+      nb[j] = m->elside[3 * e + (loc + off + 48) % 3];
+    }
+    const double* u2 = m->su2 + 2 * (size_t)s * nv;
+    for (int k = kb + 1; k < nv; ++k) {
+      double sx = 0.0, sy = 0.0;
+      for (int j = 0; j < 4; ++j) {
+        const int n = nb[j];
+        const int kin = m->side_dry[n] ? k : (k > m->side_kb[n] + 1 ? k : m->side_kb[n] + 1);
+        sx += m->su2[2 * ((size_t)n * nv + kin)];
+        sy += m->su2[2 * ((size_t)n * nv + kin) + 1];
+      }
+      bcc[2 * ((size_t)s * nv + k)] = u2[2 * k] + shapiro0 * (0.25 * sx - u2[2 * k]);
+      bcc[2 * ((size_t)s * nv + k) + 1] = u2[2 * k + 1] + shapiro0 * (0.25 * sy - u2[2 * k + 1]);
+    }
+  }
+  for (int s = 0; s < ns; ++s) {
+    if (m->side_dry[s] || m->isdel[2 * s + 1] < 0) continue;
+    const int kb = m->side_kb[s];
+    double* u2 = m->su2 + 2 * (size_t)s * nv;
+    const double* f = bcc + 2 * (size_t)s * nv;
+    for (int k = 0; k < kb - 1; ++k) u2[2 * k] = u2[2 * k + 1] = 0.0;
+    if (kb == nv - 2) {
+      u2[2 * (kb + 1)] = f[2 * (kb + 1)];
+      u2[2 * (kb + 1) + 1] = f[2 * (kb + 1) + 1];
+      u2[2 * kb] = u2[2 * (nv - 1)];
+      u2[2 * kb + 1] = u2[2 * (nv - 1) + 1];
+    } else {
+      for (int k = kb + 1; k < nv; ++k) {
+        u2[2 * k] = f[2 * k];
+        u2[2 * k + 1] = f[2 * k + 1];
+      }
+    }
+  }
+  free(bcc);
+}
+
 void or_momentum(or_mesh* m, const or_params* p) {
   const int ns = m->ns, nv = m->nvrt;
   const double dt = p->dt, th = p->theta, rx = p->rmaxvel, ry = p->rmaxvel * 1.013, dfv0 = p->dfv0;
@@ -817,6 +871,7 @@ void or_momentum(or_mesh* m, const or_params* p) {
   free(r);
   free(u);
   free(gam);
+  if (p->ishapiro == 1) shapiro_filter(m, p->shapiro0);
 }
 
 /* ------------------------------------------------------------------ vertical velocity */

@@ -21,6 +21,8 @@ typedef struct or_params {
   double rmaxvel;
   double dfv0;
   double time;
+  double shapiro0;
+  int32_t ishapiro;
 } or_params;
 
 const char* or_error(void);

@@ -27,7 +27,7 @@ from oracle import pyoracle as O  # noqa: E402
 
 STEPS = 2
 PARAM_KEYS = ("dt", "theta", "h0", "ramp", "dtb_min", "dtb_max", "velmin_btrack", "ihhat", "iettype", "rmaxvel",
-              "dfv0")
+              "dfv0", "shapiro0", "ishapiro")
 
 
 def _island(nx, ny, dx):
@@ -68,7 +68,7 @@ def case_inputs(name):
 
 
 def params_from(arr):
-    kw = {k: (int(v) if k in ("ihhat", "iettype") else float(v)) for k, v in zip(PARAM_KEYS, arr)}
+    kw = {k: (int(v) if k in ("ihhat", "iettype", "ishapiro") else float(v)) for k, v in zip(PARAM_KEYS, arr)}
     return L.default_step_params(**kw)
 
 

@@ -572,7 +572,7 @@ def _momentum_state(nvrt, kb=None, depth=10.0, cd=0.0, btu=None, seed=0, eta_old
     xyo = m.topology(L.T_XY)
     e_old = eta_old(xyo) if eta_old else np.zeros(n)
     e_new = eta_new(xyo) if eta_new else e_old
-    p = _params(dfv0=0.05, rmaxvel=100.0)
+    p = _params(dfv0=0.05, rmaxvel=100.0, ishapiro=0)
     m.set(L.F_ETA2, e_old)
     m.set(L.F_CD, np.full(n, cd))
     m.set(L.F_GRAV2, np.full(n, 9.81))
@@ -814,3 +814,40 @@ def test_vertical_velocity_matches_prism_flux_balance():
         assert np.allclose(w[e], exp, rtol=1e-9, atol=1e-12), e
         checked += 1
     assert checked > 0.9 * m.ne
+
+
+def test_shapiro_filter_against_neighbour_average():
+    """With ishapiro = 1 the momentum output of every wet internal side becomes
+    u + shapiro0 (mean of its four neighbours - u) above its bottom level, the neighbours being
+    the other two sides of each of its elements (found here from the element side lists), each
+    read at max(k, its kb + 1) when wet; the 2D bottom slot takes the surface value, levels below
+    kb - 1 are cleared, boundary and dry sides keep the unfiltered values."""
+    nv = 6
+    mm, xy, en, _ = _mesh(nx=18, ny=12, nvrt=nv)
+    kbv = (np.arange(xy.shape[0]) % 3).astype(np.int32)
+    m, p, _ = _momentum_state(nv, kb=kbv, cd=2.5e-3, seed=8, eta_old=lambda q: 1e-5 * q[:, 0],
+                              eta_new=lambda q: 0.2 * np.sin(q[:, 1] / 1500.0))
+    m.momentum(p)
+    raw = m.get(L.F_SIDE_U2)
+    m2, p2, _ = _momentum_state(nv, kb=kbv, cd=2.5e-3, seed=8, eta_old=lambda q: 1e-5 * q[:, 0],
+                                eta_new=lambda q: 0.2 * np.sin(q[:, 1] / 1500.0))
+    p2.ishapiro, p2.shapiro0 = 1, 0.5
+    m2.momentum(p2)
+    got = m2.get(L.F_SIDE_U2)
+    swet, skb, stype, _ = _side_info(m2)
+    isdel, es = m2.topology(L.T_ISDEL), m2.topology(L.T_ELSIDE)
+    exp = raw.copy()
+    for s in range(m2.ns):
+        if not swet[s] or isdel[s, 1] < 0:
+            continue
+        nbs = [t for e in isdel[s] for t in es[e] if t != s]
+        assert len(nbs) == 4
+        kb = skb[s]
+        for k in range(kb + 1, nv):
+            acc = sum(raw[n, max(k, skb[n] + 1) if swet[n] else k] for n in nbs)
+            exp[s, k] = raw[s, k] + 0.5 * (0.25 * acc - raw[s, k])
+        exp[s, :max(kb - 1, 0)] = 0.0
+        if kb == nv - 2:
+            exp[s, kb] = exp[s, nv - 1]
+    assert np.allclose(got, exp, rtol=1e-13, atol=1e-15)
+    assert np.abs(got - raw).max() > 1e-3
