@@ -928,7 +928,7 @@ __global__ void __launch_bounds__(64) k_momentum(DMesh m, StepArgs a, int g) {
   unstage_rows(m.su2 + 2 * (int64_t)s0 * nv, Lu, us, 2 * nv, gc);
 }
 
-// shapiroFilter1 (meshdata.cu:5449-5487): one thread per (side, level k > kb) of each wet internal
+// shapiroFilter1 (meshdata.cu:5456-5493): one thread per (side, level k > kb) of each wet internal
 // side. Its four neighbours are Side::neigh(j) (meshdata.h:1409-1422): the sides before (j = 0, 1)
 // and after (j = 2, 3) it in elements 0 and 1. The sum runs in that order.
 __global__ void k_shapiro1(DMesh m, double shapiro0, double* __restrict__ tmp) {
@@ -968,7 +968,7 @@ __global__ void k_shapiro1(DMesh m, double shapiro0, double* __restrict__ tmp) {
   tmp[2 * t + 1] = u.y + shapiro0 * (0.25 * sy - u.y);
 }
 
-// shapiroFilter2 (meshdata.cu:5490-5514): the filtered values replace u2 above kb (the 2D bottom
+// shapiroFilter2 (meshdata.cu:5496-5518): the filtered values replace u2 above kb (the 2D bottom
 // slot takes the surface value), and the levels below kb - 1 are cleared.
 __global__ void k_shapiro2(DMesh m, const double* __restrict__ tmp) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;

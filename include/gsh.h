@@ -219,7 +219,7 @@ int gsh_nodal_velocity(gsh_mesh* mesh, const gsh_step_params* p);
  * 3D sides solve their vertical tridiagonal system, 2D sides the depth-averaged update; land
  * boundary sides keep the tangential component. */
 int gsh_momentum(gsh_mesh* mesh, const gsh_step_params* p);
-/* With p->ishapiro == 1, gsh_momentum ends with shapiroFilter1/2 (meshdata.cu:5449-5514, called at
+/* With p->ishapiro == 1, gsh_momentum ends with shapiroFilter1/2 (meshdata.cu:5456-5518, called at
  * schism_step.py:289-292): every wet internal side's u2[k], k > kb, becomes
  * u2 + shapiro0 (sum of its four neighbour sides' u2 / 4 - u2), the neighbours' levels clamped to
  * max(k, kb + 1) where wet; the 2D bottom slot follows the surface value. */

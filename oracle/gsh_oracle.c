@@ -702,7 +702,7 @@ static void side_grads(const or_mesh* m, int s, double h0, double g1[2], double 
  *       u2[kb] = u2[kb+1] when Cd = 0, else 0. (The reference's loop also assigns k = kb from
  *       soln[-1], which the bottom condition then overwrites; it is not restated.)
  *   k < kb: 0. Land boundary sides then keep only the tangential component for k >= kb. */
-/* shapiroFilter1 + shapiroFilter2 (meshdata.cu:5449-5514), niter_shap = 1: for each wet internal
+/* shapiroFilter1 + shapiroFilter2 (meshdata.cu:5456-5518), niter_shap = 1: for each wet internal
  * side, bcc[k] = u2[k] + shapiro0 (0.25 sum_j u2_j[kin_j] - u2[k]) for k > kb, over its four
  * neighbours Side::neigh(j) (meshdata.h:1409-1422: element j & 1, the side before (j < 2) or after
  * it there), kin_j = max(k, kb_j + 1) for a wet neighbour; then u2 <- bcc above kb (2D: u2[kb+1]
