@@ -478,8 +478,9 @@ def main() -> int:
     nv_bytes = (16 * mesh.ns + 8 * mesh.ne + 32 * n) * mesh.nvrt
     # the step after the solve: side velocities from the momentum solve (zc, btu in, u2 out per
     # side-level; about 300 B of element and node gathers per side for the elevation gradients)
+    # and the Shapiro filter (u2 read, the filtered u2 written per side-level)
     ms_mom = min(timed(lambda: mesh.momentum(p)) for _ in range(3))
-    mom_bytes = 40 * mesh.ns * mesh.nvrt + 300 * mesh.ns
+    mom_bytes = 40 * mesh.ns * mesh.nvrt + 300 * mesh.ns + (32 * mesh.ns * mesh.nvrt if p.ishapiro == 1 else 0)
     # element w from the new side velocities: u2 and zc rows of every side read, w written
     ms_vv = min(timed(lambda: mesh.vertical_velocity(p)) for _ in range(3))
     vv_bytes = (24 * mesh.ns + 8 * mesh.ne) * mesh.nvrt + 64 * n

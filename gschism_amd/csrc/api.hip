@@ -1500,10 +1500,12 @@ int gsh_momentum(gsh_mesh* m, const gsh_step_params* p) {
     m->ctx->set();
     launch_momentum(m->d, a, m->ctx->stream);
     if (a.ishapiro == 1) {
-      // the filtered values go through a scratch [ns][nvrt][2] of the mesh's pool (the
-      // reference reuses its baroclinic array bcc for them)
+      // the filtered field is written whole into a second [ns][nvrt][2] buffer of the mesh's
+      // pool, which then becomes u2 (the reference filters through its baroclinic array bcc and
+      // copies back); device pointers to GSH_F_SIDE_U2 taken before this call go stale
       if (!m->shap_tmp) m->shap_tmp = m->pool.alloc<double>(2 * (int64_t)m->d.ns * m->d.nvrt);
       launch_shapiro(m->d, a.shapiro0, m->shap_tmp, m->ctx->stream);
+      std::swap(m->d.su2, m->shap_tmp);
     }
     HIPCHK(hipGetLastError());
   })

@@ -928,73 +928,74 @@ __global__ void __launch_bounds__(64) k_momentum(DMesh m, StepArgs a, int g) {
   unstage_rows(m.su2 + 2 * (int64_t)s0 * nv, Lu, us, 2 * nv, gc);
 }
 
-// shapiroFilter1 (meshdata.cu:5456-5493): one thread per (side, level k > kb) of each wet internal
-// side. Its four neighbours are Side::neigh(j) (meshdata.h:1409-1422): the sides before (j = 0, 1)
-// and after (j = 2, 3) it in elements 0 and 1. The sum runs in that order.
-__global__ void k_shapiro1(DMesh m, double shapiro0, double* __restrict__ tmp) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// shapiroFilter1 + shapiroFilter2 (meshdata.cu:5456-5518) in one pass that writes every side's
+// final u2 into `out` (the caller then swaps it with u2). For a wet internal side, level k > kb
+// takes bcc[k] = u2[k] + shapiro0 (0.25 sum_j u2_j[kin_j] - u2[k]) over its four neighbours
+// Side::neigh(j) (meshdata.h:1409-1422: the sides before (j = 0, 1) and after (j = 2, 3) it in
+// elements 0 and 1, summed in that order; kin_j = max(k, kb_j + 1) for a wet neighbour); a 2D
+// side's bottom slot takes bcc[kb+1] (recomputed by its own thread, the same bits); levels below
+// kb - 1 become 0; every other value, and boundary and dry sides, pass through unchanged.
+__device__ __forceinline__ double2 shapiro_value(const DMesh& m, const int (&nb)[4], const double2* u2, int64_t row,
+                                                 int k, double shapiro0) {
   const int nv = m.nvrt;
-  if (t >= (int64_t)m.ns * nv) return;
-  const int s = (int)(t / nv), k = (int)(t - (int64_t)s * nv);
-  const int4 sr = m.side[s];
-  if (!m.side_wet[s] || sr.w < 0) return;
-  const int kb = m.side_kb[s];
-  if (k < kb + 1) return;
-  const double2* u2 = reinterpret_cast<const double2*>(m.su2);
-  int es[2][3];
-  int loc[2];
-#pragma unroll
-  for (int i = 0; i < 2; ++i) {
-    const int e = i == 0 ? sr.z : sr.w;
-    const int4 r0 = m.elem[2 * e], r1 = m.elem[2 * e + 1];
-    es[i][0] = r0.w;
-    es[i][1] = r1.x;
-    es[i][2] = r1.y;
-    loc[i] = es[i][0] == s ? 0 : (es[i][1] == s ? 1 : 2);
-  }
   double sx = 0.0, sy = 0.0;
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
-    const int i = j & 1, off = 2 * (j >> 1) - 1;
-    const int l = (loc[i] + off + 48) % 3;
-    const int n = es[i][0] * (l == 0) + es[i][1] * (l == 1) + es[i][2] * (l == 2);
+    const int n = nb[j];
     const int kin = m.side_wet[n] ? max(k, m.side_kb[n] + 1) : k;
     const double2 v = u2[(int64_t)n * nv + kin];
     sx += v.x;
     sy += v.y;
   }
-  const double2 u = u2[t];
-  tmp[2 * t] = u.x + shapiro0 * (0.25 * sx - u.x);
-  tmp[2 * t + 1] = u.y + shapiro0 * (0.25 * sy - u.y);
+  const double2 u = u2[row + k];
+  return make_double2(u.x + shapiro0 * (0.25 * sx - u.x), u.y + shapiro0 * (0.25 * sy - u.y));
 }
 
-// shapiroFilter2 (meshdata.cu:5496-5518): the filtered values replace u2 above kb (the 2D bottom
-// slot takes the surface value), and the levels below kb - 1 are cleared.
-__global__ void k_shapiro2(DMesh m, const double* __restrict__ tmp) {
+__global__ void k_shapiro(DMesh m, double shapiro0, double* __restrict__ out) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int nv = m.nvrt;
   if (t >= (int64_t)m.ns * nv) return;
   const int s = (int)(t / nv), k = (int)(t - (int64_t)s * nv);
+  const double2* u2 = reinterpret_cast<const double2*>(m.su2);
+  double2* o = reinterpret_cast<double2*>(out);
   const int4 sr = m.side[s];
-  if (!m.side_wet[s] || sr.w < 0) return;
   const int kb = m.side_kb[s];
-  double2* u2 = reinterpret_cast<double2*>(m.su2);
-  const double2* f = reinterpret_cast<const double2*>(tmp);
-  if (k < kb - 1) {
-    u2[t] = make_double2(0.0, 0.0);
-  } else if (kb == nv - 2) {
-    if (k >= kb) u2[t] = f[(int64_t)s * nv + kb + 1];  // u2[kb+1] = bcc[kb+1], u2[kb] = u2[nvrt-1]
-  } else if (k > kb) {
-    u2[t] = f[t];
+  const bool filtered = m.side_wet[s] && sr.w >= 0;
+  const int kf = (kb == nv - 2) ? kb + 1 : k;  // the level whose filtered value this slot takes
+  if (!filtered || (k >= kb - 1 && (k < kb || (kb < nv - 2 && k == kb)))) {
+    o[t] = u2[t];
+    return;
   }
+  if (k < kb - 1) {
+    o[t] = make_double2(0.0, 0.0);
+    return;
+  }
+  int nb[4];
+  {
+    int es[2][3], loc[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const int e = i == 0 ? sr.z : sr.w;
+      const int4 r0 = m.elem[2 * e], r1 = m.elem[2 * e + 1];
+      es[i][0] = r0.w;
+      es[i][1] = r1.x;
+      es[i][2] = r1.y;
+      loc[i] = es[i][0] == s ? 0 : (es[i][1] == s ? 1 : 2);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int i = j & 1, off = 2 * (j >> 1) - 1;
+      const int l = (loc[i] + off + 48) % 3;
+      nb[j] = es[i][0] * (l == 0) + es[i][1] * (l == 1) + es[i][2] * (l == 2);
+    }
+  }
+  o[t] = shapiro_value(m, nb, u2, (int64_t)s * nv, kf, shapiro0);
 }
 
-void launch_shapiro(const DMesh& m, double shapiro0, double* tmp, hipStream_t st) {
+void launch_shapiro(const DMesh& m, double shapiro0, double* out, hipStream_t st) {
   const int64_t total = (int64_t)m.ns * m.nvrt;
   if (total <= 0) return;
-  const unsigned grid = (unsigned)((total + kTpb - 1) / kTpb);
-  hipLaunchKernelGGL(k_shapiro1, dim3(grid), dim3(kTpb), 0, st, m, shapiro0, tmp);
-  hipLaunchKernelGGL(k_shapiro2, dim3(grid), dim3(kTpb), 0, st, m, tmp);
+  hipLaunchKernelGGL(k_shapiro, dim3((unsigned)((total + kTpb - 1) / kTpb)), dim3(kTpb), 0, st, m, shapiro0, out);
 }
 
 void launch_momentum(const DMesh& m, const StepArgs& a, hipStream_t st) {
