@@ -222,7 +222,9 @@ int gsh_momentum(gsh_mesh* mesh, const gsh_step_params* p);
 /* With p->ishapiro == 1, gsh_momentum ends with shapiroFilter1/2 (meshdata.cu:5456-5518, called at
  * schism_step.py:289-292): every wet internal side's u2[k], k > kb, becomes
  * u2 + shapiro0 (sum of its four neighbour sides' u2 / 4 - u2), the neighbours' levels clamped to
- * max(k, kb + 1) where wet; the 2D bottom slot follows the surface value. */
+ * max(k, kb + 1) where wet; the 2D bottom slot follows the surface value. The filtered field is
+ * written into a second buffer that then becomes GSH_F_SIDE_U2, so a device pointer to that field
+ * (gsh_mesh_field_ptr) must be fetched again after the call. */
 
 /* computeVerticalVelocity (meshdata.cu:5978-6102), triangles, ics=1, imm=0: GSH_F_ELEM_W from
  * GSH_F_SIDE_U2 (after gsh_momentum) and the levels of the last gsh_update_levels (node z from
