@@ -577,3 +577,27 @@ def test_momentum_branches(ctx):
     om.vertical_velocity()
     assert_bitwise(mesh.get(L.F_ELEM_W), om.get(L.F_ELEM_W), "element w")
     mesh.close()
+
+
+def test_step_call_argument_errors(ctx):
+    """Bad arguments of the step calls are errors (GSH_E_ARG) raised before any launch, and the
+    mesh keeps working afterwards."""
+    spec = cases.CaseSpec("args-24x16", 24, 16, 500.0, nvrt=4, depth=10.0, substeps=4)
+    mesh, om, p = _setup(ctx, spec)
+    n = mesh.n
+    bad = [dict(ishapiro=2), dict(dfv0=-1.0), dict(rmaxvel=-1.0), dict(theta=1.5)]
+    for kw in bad:
+        q = L.default_step_params(**kw)
+        with pytest.raises(L.GshError) as ei:
+            mesh.momentum(q)
+        assert ei.value.code == L.GSH_E_ARG, kw
+    with pytest.raises(L.GshError) as ei:
+        mesh.set_tides(np.ones(300), np.ones(300), np.zeros(300), np.zeros((n, 300)), np.zeros((n, 300)))
+    assert ei.value.code == L.GSH_E_ARG
+    with pytest.raises(L.GshError):
+        core.Server(ctx, precond=7)
+    mesh.backtrack(p)
+    mesh.rhs_producers(p)
+    mesh.momentum(p)
+    assert np.isfinite(mesh.get(L.F_SIDE_U2)).all()
+    mesh.close()
