@@ -1233,6 +1233,7 @@ int gsh_mesh_create(gsh_ctx* ctx, const gsh_mesh_desc* desc, gsh_mesh** out, int
     d.vsource = P.alloc<double>(ne);
     d.side = P.upload(srec);
     d.side_type = P.upload(t.side_type);
+    d.side_nbr = P.upload(t.side_nbr);
     d.side_cx = P.upload(m->geo.side_cx);
     d.side_cy = P.upload(m->geo.side_cy);
     d.side_len = P.upload(m->geo.side_len);

@@ -50,6 +50,7 @@ struct DMesh {
   // sides
   const int4* side;         // [ns]: (n1, n2, e0, e1)
   const int8_t* side_type;  // 0 internal, 1 land, 2 open
+  const int32_t* side_nbr;  // [ns][4] the Shapiro filter's neighbour sides (internal sides)
   const double* side_cx;
   const double* side_cy;
   const double* side_len;

@@ -970,25 +970,8 @@ __global__ void k_shapiro(DMesh m, double shapiro0, double* __restrict__ out) {
     o[t] = make_double2(0.0, 0.0);
     return;
   }
-  int nb[4];
-  {
-    int es[2][3], loc[2];
-#pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      const int e = i == 0 ? sr.z : sr.w;
-      const int4 r0 = m.elem[2 * e], r1 = m.elem[2 * e + 1];
-      es[i][0] = r0.w;
-      es[i][1] = r1.x;
-      es[i][2] = r1.y;
-      loc[i] = es[i][0] == s ? 0 : (es[i][1] == s ? 1 : 2);
-    }
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int i = j & 1, off = 2 * (j >> 1) - 1;
-      const int l = (loc[i] + off + 48) % 3;
-      nb[j] = es[i][0] * (l == 0) + es[i][1] * (l == 1) + es[i][2] * (l == 2);
-    }
-  }
+  const int4 q = reinterpret_cast<const int4*>(m.side_nbr)[s];
+  const int nb[4] = {q.x, q.y, q.z, q.w};
   o[t] = shapiro_value(m, nb, u2, (int64_t)s * nv, kf, shapiro0);
 }
 

@@ -178,6 +178,19 @@ void build_topology(const HostMesh& m, Topology& t) {
         t.isidenode[2 * s + 1] = n1;
       }
     }
+  // Side::neigh(j) (meshdata.h:1409-1422), the Shapiro filter's neighbours: in element j & 1 of
+  // the side, the side before it (j < 2) or after it (j >= 2) in that element's side order
+  t.side_nbr.assign(4 * (size_t)ns, -1);
+  for (int32_t s = 0; s < ns; ++s) {
+    if (t.isdel[2 * s + 1] < 0) continue;
+    for (int j = 0; j < 4; ++j) {
+      const int32_t e = t.isdel[2 * s + (j & 1)];
+      const int off = 2 * (j >> 1) - 1;
+      int loc = 0;
+      while (t.elside[3 * e + loc] != s) ++loc;
+      t.side_nbr[4 * (size_t)s + j] = t.elside[3 * e + (loc + off + 48) % 3];
+    }
+  }
   t.side_type.assign(ns, 0);
   for (int32_t s = 0; s < ns; ++s)
     if (t.isdel[2 * s + 1] < 0) {
