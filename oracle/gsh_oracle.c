@@ -15,8 +15,11 @@
  *   src/kernels/meshdata.cu:4459-4476, 4615-4725, 6183-6424, 4276-4339   RHS producers
  *   src/kernels/meshdata.cu:3182-3311, 4400-4446                    assembly + essential BC
  *   src/solver/pcg_solver.py:172-276, 881-947, 1075-1124; pcg_loop.cpp:361-413   Jacobi PCG
+ *   src/kernels/meshdata.cu:5765-5929                               nodal velocities
+ *   src/kernels/meshdata.cu:5045-5446, 3665-3711, 5456-5518         momentum, Shapiro filter
+ *   src/kernels/meshdata.cu:5978-6102, 559-631                      element vertical velocity
  * for the option subset ics=1, ivcor=1 (global sigma), imm=0, iveg=0, ibc=1 (barotropic),
- * nws=0, ncor=0, ihorcon=0, nchi=0, triangles.
+ * itur=0, islip=0, nws=0, ncor=0, ihorcon=0, nchi=0, triangles; elevation BC types 0-5 per node.
  *
  * Deliberate deviations, shared with the device code:
  *  - Dot products are summed in the device kernels' fixed tile / wave64 tree (see
@@ -24,9 +27,11 @@
  *    is not reproducible; with identical order the CPU and device iterates agree bit for bit.
  *  - The celestial walk's assert on exceeding 40 steps (meshdata.cu:126) and the ELM asserts
  *    become status flags; the walk then keeps the current element.
- *  - Terms of computeElemAverageGhat that are identically zero in this subset (Coriolis,
- *    wind stress, viscosity, pressure, tidal potential, baroclinic) are omitted; they could
- *    only change the sign of a zero.
+ *  - Terms of computeElemAverageGhat and of the momentum solve that are identically zero in
+ *    this subset (Coriolis, wind stress, viscosity, pressure, tidal potential, baroclinic) are
+ *    omitted; they could only change the sign of a zero.
+ *  - The nodal velocities sum each node's ball elements in ball order instead of the
+ *    reference's per-element atomics.
  * Compile with -ffp-contract=off and without fast-math.
  */
 #include "gsh_oracle.h"
