@@ -606,7 +606,11 @@ def main() -> int:
         ip_h, ix_h = mesh.topology(L.T_CSR_INDPTR), mesh.topology(L.T_CSR_INDICES)
         val = solver.get_values()
         bvec = solver.get_rhs()
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        # the host cores this process may use: the CPU share the GPU box gives one GPU
+        # (OMP_NUM_THREADS there), else every core of the affinity mask
+        affinity = len(os.sched_getaffinity(0))
+        share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+        threads = args.cpu_threads or (min(share, affinity) if share > 0 else affinity)
         osr = O.OracleSsor(ip_h, ix_h, args.ssor_rows_per_thread)
         reps, spent, xo, ito = 0, 0.0, None, 0
         while reps < 40 and (spent < 10.0 or reps < 2):
@@ -623,6 +627,8 @@ def main() -> int:
         cpu = {"value": round(reps / spent, 4), "unit": "solves/s", "cores": threads, "kind": "port",
                "sample": f"{reps} full FP64 SSOR PCG solves of the same {n}-node system ({ito} iterations each), "
                          f"oracle C restatement, OpenMP over rows, {cpu_model()}",
+               "cpu_model": cpu_model(), "nproc_affinity": affinity, "os_cpu_count": os.cpu_count(),
+               "omp_num_threads_env": share,
                "seconds": round(spent, 3),
                "gpu_bitwise_equal": bool(np.array_equal(xo, x_ssor)), "iters_equal": bool(ito == it_ssor),
                "single_core": {"value": round(1.0 / one_core, 4), "unit": "solves/s", "cores": 1,

@@ -138,6 +138,7 @@ SIGNATURES = {
     "gsh_server_create": (_I, [_P, C.POINTER(ServerConfig), _PP]),
     "gsh_server_destroy": (_I, [_P]),
     "gsh_server_add_mesh": (_I, [_P, _P, _PI32]),
+    "gsh_server_set_mesh_params": (_I, [_P, _I32, C.POINTER(StepParams)]),
     "gsh_server_submit": (_I, [_P, _I32, _PD, _I, _PI64]),
     "gsh_server_step": (_I, [_P, _PI32, _PI32]),
     "gsh_server_finished": (_I, [_P, C.POINTER(C.c_int64), _I32, _PI32]),

@@ -216,6 +216,11 @@ def elevation_forcing(tide: TideData, open_segments, n: int):
     values (Mesh::populateBoundaryNodes, meshdata.cu:2499-2527)."""
     if len(open_segments) != tide.nope:
         raise ValueError(f"{len(open_segments)} open boundaries in the grid, {tide.nope} in bctides.in")
+    flow = [k for k in range(tide.nope) if tide.ifltype[k] != 0]
+    if flow:
+        # flow forcing needs the I_3 integral, the Flather / discharge matrix terms and the
+        # momentum boundary velocities (meshdata.cu:3384-3454, 5357-5446), which are not built
+        raise NotImplementedError(f"open boundaries {flow} have flow forcing (ifltype != 0), which is not supported")
     etype = np.zeros(n, dtype=np.int8)
     eth = np.zeros(n)
     emo = np.zeros((n, tide.nbfr))

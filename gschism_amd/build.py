@@ -28,7 +28,7 @@ HIP_FLAGS = ["-x", "hip", f"--offload-arch={ARCH}", "-Xclang", "-target-feature"
              "-packed-fp32-ops", "-fno-gpu-rdc"]
 HOST_SOURCES = ["gen.cpp", "topology.cpp", "ssor_layout.cpp", "grid_io.cpp"]
 HIP_SOURCES = ["mesh_kernels.hip", "pcg_kernels.hip", "ssor_kernels.hip", "api.hip"]
-HEADERS = ["host_mesh.h", "abi_util.h", "device.h", "reduce.h", "ssor_layout.h"]
+HEADERS = ["host_mesh.h", "abi_util.h", "context.h", "device.h", "reduce.h", "ssor_layout.h"]
 
 
 def _stale(obj: Path, src: Path) -> bool:

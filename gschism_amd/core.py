@@ -387,11 +387,18 @@ class Server:
     def __del__(self):
         self.close()
 
-    def add_mesh(self, mesh: Mesh) -> int:
+    def add_mesh(self, mesh: Mesh, params=None) -> int:
+        """Registers a mesh; `params` (StepParams) are the step parameters of its requests'
+        set-up (default: the library defaults)."""
         i = C.c_int32()
         check(L.lib().gsh_server_add_mesh(self.h, mesh.h, C.byref(i)))
         self.meshes.append(mesh)
+        if params is not None:
+            self.set_mesh_params(i.value, params)
         return i.value
+
+    def set_mesh_params(self, mesh_id: int, params) -> None:
+        check(L.lib().gsh_server_set_mesh_params(self.h, mesh_id, C.byref(params)))
 
     def submit(self, mesh_id: int, b=None, with_backtrack: bool = False) -> int:
         bb = None if b is None else np.ascontiguousarray(b, dtype=np.float64)

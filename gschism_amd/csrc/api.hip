@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "abi_util.h"
+#include "context.h"
 #include "device.h"
 #include "ssor_layout.h"
 
@@ -30,14 +31,6 @@ static thread_local std::string g_last_error;
 void set_last_error(const std::string& msg) { g_last_error = msg; }
 void clear_last_error() { g_last_error.clear(); }
 
-static void hip_check(hipError_t e, const char* what) {
-  if (e == hipSuccess) return;
-  const int code = (e == hipErrorOutOfMemory) ? GSH_E_OOM : GSH_E_DEVICE;
-  throw Error(code, std::string(what) + ": " + hipGetErrorString(e));
-}
-#define HIPCHK(x) hip_check((x), #x)
-
-using Clock = std::chrono::steady_clock;
 
 // Device allocations owned by one object, released together.
 struct DevPool {
@@ -86,20 +79,6 @@ struct DevPool {
   }
   ~DevPool() { release(); }
 };
-
-// Waits for an event with a time limit (pcg_loop.cpp:444-456 raises instead of hanging).
-static void wait_event(hipEvent_t ev, double timeout_s, const char* what) {
-  const auto t0 = Clock::now();
-  int spins = 0;
-  for (;;) {
-    const hipError_t r = hipEventQuery(ev);
-    if (r == hipSuccess) return;
-    if (r != hipErrorNotReady) hip_check(r, what);
-    if (std::chrono::duration<double>(Clock::now() - t0).count() > timeout_s)
-      throw Error(GSH_E_TIMEOUT, std::string(what) + ": the GPU did not finish within the time limit");
-    if (++spins > 64) std::this_thread::sleep_for(std::chrono::microseconds(20));
-  }
-}
 
 template <class VT>
 __global__ void k_gather_states(const PcgSys<VT>* __restrict__ sys, int nsys, PcgState* __restrict__ out) {
@@ -153,29 +132,6 @@ struct StreamBuffers {
   }
   StreamBuffers(const StreamBuffers&) = delete;
   StreamBuffers& operator=(const StreamBuffers&) = delete;
-};
-
-struct gsh_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  // meshes, solvers and servers on this context; gsh_ctx_destroy frees the context once the
-  // last of them is destroyed, so objects destroyed after the context never see freed state
-  std::atomic<int> refs{0};
-  std::atomic<bool> closed{false};
-  PcgState* pinned = nullptr;   // host-visible readback slots
-  int pinned_cap = 0;
-  hipEvent_t ev[2] = {nullptr, nullptr};
-  void set() const { HIPCHK(hipSetDevice(device)); }
-  PcgState* pinned_slots(int per_slot) {
-    if (2 * per_slot > pinned_cap) {
-      if (pinned) (void)hipHostFree(pinned);
-      pinned = nullptr;
-      const int cap = std::max(64, 2 * per_slot);
-      HIPCHK(hipHostMalloc((void**)&pinned, sizeof(PcgState) * cap, hipHostMallocDefault));
-      pinned_cap = cap;
-    }
-    return pinned;
-  }
 };
 
 // ====================================================================== mesh
@@ -1105,6 +1061,10 @@ static StepArgs step_args(const gsh_step_params* p) {
   GSH_REQUIRE(p, GSH_E_ARG, "step params are NULL");
   GSH_REQUIRE(p->dt > 0.0 && p->dtb_min > 0.0 && p->dtb_max > 0.0, GSH_E_ARG, "dt, dtb_min, dtb_max must be > 0");
   GSH_REQUIRE(p->theta >= 0.0 && p->theta <= 1.0, GSH_E_ARG, "theta must be in [0, 1]");
+  GSH_REQUIRE(p->iettype >= 0 && p->iettype <= 5, GSH_E_ARG, "iettype must be in [0, 5]");
+  GSH_REQUIRE(p->ishapiro == 0 || p->ishapiro == 1, GSH_E_ARG, "ishapiro must be 0 or 1");
+  GSH_REQUIRE(!p->ishapiro || (p->shapiro0 >= 0.0 && p->shapiro0 <= 0.5), GSH_E_ARG,
+              "shapiro0 must be in [0, 0.5]");
   StepArgs a{};
   a.dt = p->dt;
   a.theta = p->theta;
@@ -1212,6 +1172,7 @@ int gsh_mesh_create(gsh_ctx* ctx, const gsh_mesh_desc* desc, gsh_mesh** out, int
     d.kb = P.upload(hm.kb);
     d.sigma = P.upload(hm.sigma);
     d.open_bnd = P.upload(hm.open_bnd);
+    d.node_bside = reinterpret_cast<const int2*>(P.upload(t.node_bside));
     d.eta2 = P.alloc<double>(n);
     d.eta1 = P.alloc<double>(n);
     d.grav2 = P.upload(std::vector<double>(n, 9.81));
@@ -1373,6 +1334,11 @@ int gsh_mesh_set_field(gsh_mesh* m, int field, const void* host, int64_t count) 
     if (field == GSH_F_ELEM_W) ensure_elem_w(m);
     FieldRef f = field_ref(m, field);
     GSH_REQUIRE(count == f.count, GSH_E_ARG, "field size mismatch");
+    if (field == GSH_F_ETYPE) {  // -1 (the step parameter) or an elevation BC type 0-5
+      const int8_t* t = static_cast<const int8_t*>(host);
+      for (int64_t i = 0; i < count; ++i)
+        GSH_REQUIRE(t[i] >= -1 && t[i] <= 5, GSH_E_ARG, "elevation BC types must be in [-1, 5]");
+    }
     hipStream_t st = m->ctx->stream;
     if (field == GSH_F_NODE_UVW) {
       StreamBuffers bufs(st);
@@ -2045,6 +2011,7 @@ struct gsh_server {
   gsh_server_config cfg{};
   int64_t budget = 0, used = 0;
   std::vector<gsh_mesh*> meshes;
+  std::vector<gsh_step_params> params;           // step parameters of each mesh's admission set-up
   std::vector<std::vector<gsh_solver*>> pool;  // idle solvers per mesh
   std::deque<Request*> queue;
   std::vector<Request*> active;
@@ -2394,9 +2361,21 @@ int gsh_server_add_mesh(gsh_server* srv, gsh_mesh* mesh, int32_t* id) {
   GSH_API_BODY({
     GSH_REQUIRE(srv && mesh && id, GSH_E_ARG, "NULL argument");
     GSH_REQUIRE(mesh->ctx == srv->ctx, GSH_E_ARG, "mesh belongs to another context");
+    gsh_step_params p;
+    gsh_step_params_default(&p);
     srv->meshes.push_back(mesh);
+    srv->params.push_back(p);
     srv->pool.emplace_back();
     *id = (int32_t)srv->meshes.size() - 1;
+  })
+}
+
+int gsh_server_set_mesh_params(gsh_server* srv, int32_t mesh_id, const gsh_step_params* p) {
+  GSH_API_BODY({
+    GSH_REQUIRE(srv && p, GSH_E_ARG, "NULL argument");
+    GSH_REQUIRE(mesh_id >= 0 && mesh_id < (int32_t)srv->meshes.size(), GSH_E_ARG, "unknown mesh id");
+    (void)step_args(p);  // validated now, not at admission
+    srv->params[mesh_id] = *p;
   })
 }
 
@@ -2474,18 +2453,14 @@ int gsh_server_step(gsh_server* srv, int32_t* active_out, int32_t* queued_out) {
         }
       }
       for (auto* hh : s->h) launch_fill_f64(hh, 0.0, s->n, sg);  // each request is a cold start
+      // the mesh's step parameters (gsh_server_set_mesh_params; default gsh_step_params_default)
+      const StepArgs a = step_args(&srv->params[r->mesh_id]);
       if (r->with_bt) {
-        gsh_step_params p;
-        gsh_step_params_default(&p);
-        const StepArgs a = step_args(&p);
         launch_backtrack(m->d, a, nullptr, nullptr, 0, sg);
         launch_rhs(m->d, a, sg);
       }
-      {  // the request's matrix (and right-hand side unless given) from the mesh state
-        gsh_step_params p;
-        gsh_step_params_default(&p);
-        launch_assemble(m->d, step_args(&p), s->val, s->b, sg);
-      }
+      // the request's matrix (and right-hand side unless given) from the mesh state
+      launch_assemble(m->d, a, s->val, s->b, sg);
       if (r->has_b) HIPCHK(hipMemcpyAsync(s->b, r->b.data(), sizeof(double) * s->n, hipMemcpyHostToDevice, sg));
       if (cfg.precision == GSH_PREC_MIXED) {
         launch_cast_values(s->val, s->val32, s->nnz, sg);
@@ -2610,6 +2585,31 @@ __global__ void k_copy_probe(const double4* __restrict__ src, double4* __restric
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride) dst[i] = src[i];
 }
+
+// U records per thread per round (U loads in flight before the stores); NT: stores bypass the
+// caches (nontemporal). Every index is below n4.
+template <int U, bool NT>
+__global__ void k_copy_probe_u(const double4* __restrict__ src, double4* __restrict__ dst, int64_t n4) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i + (U - 1) * stride < n4; i += U * stride) {
+    double4 v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = src[i + u * stride];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (NT) {
+        __builtin_nontemporal_store(v[u].x, &dst[i + u * stride].x);
+        __builtin_nontemporal_store(v[u].y, &dst[i + u * stride].y);
+        __builtin_nontemporal_store(v[u].z, &dst[i + u * stride].z);
+        __builtin_nontemporal_store(v[u].w, &dst[i + u * stride].w);
+      } else {
+        dst[i + u * stride] = v[u];
+      }
+    }
+  }
+  for (; i < n4; i += stride) dst[i] = src[i];
+}
 }  // namespace gsh
 
 extern "C" {
@@ -2629,15 +2629,31 @@ int gsh_copy_probe(gsh_ctx* ctx, int64_t bytes, int reps, double* ms_avg) {
     HIPCHK(hipEventCreate(&e1));
     int dev_cus = 0;
     HIPCHK(hipDeviceGetAttribute(&dev_cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
-    const int grid = std::max(1, dev_cus) * 16;
-    hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(256), 0, st, a, b, n4);  // warm-up
-    HIPCHK(hipEventRecord(e0, st));
-    for (int r = 0; r < reps; ++r) hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(256), 0, st, a, b, n4);
-    HIPCHK(hipEventRecord(e1, st));
-    HIPCHK(hipEventSynchronize(e1));
-    float ms = 0.f;
-    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-    *ms_avg = ms / reps;
+    const int cus = std::max(1, dev_cus);
+    // the fastest of a few launch shapes: the achievable-bandwidth reference
+    double best = 1e30;
+    for (int variant = 0; variant < 7; ++variant) {
+      const int mult[7] = {16, 8, 16, 32, 8, 16, 32};
+      const int grid = cus * mult[variant];
+      auto launch = [&]() {
+        switch (variant) {
+          case 0: hipLaunchKernelGGL(k_copy_probe, dim3(grid), dim3(256), 0, st, a, b, n4); break;
+          case 1:
+          case 2:
+          case 3: hipLaunchKernelGGL((k_copy_probe_u<4, false>), dim3(grid), dim3(256), 0, st, a, b, n4); break;
+          default: hipLaunchKernelGGL((k_copy_probe_u<4, true>), dim3(grid), dim3(256), 0, st, a, b, n4); break;
+        }
+      };
+      launch();  // warm-up
+      HIPCHK(hipEventRecord(e0, st));
+      for (int r = 0; r < reps; ++r) launch();
+      HIPCHK(hipEventRecord(e1, st));
+      HIPCHK(hipEventSynchronize(e1));
+      float ms = 0.f;
+      HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+      best = std::min(best, (double)ms / reps);
+    }
+    *ms_avg = best;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     (void)hipFree(a);

@@ -22,6 +22,7 @@ struct DMesh {
   const int32_t* kb;
   const double* sigma;      // [nvrt]
   const int8_t* open_bnd;
+  const int2* node_bside;   // [n] boundary sides ending at the node, ascending side index, -1 = none
   double* eta2;
   double* eta1;
   double* grav2;

@@ -47,6 +47,7 @@ struct Topology {
   std::vector<int8_t> side_type;   // [ns] 0 internal, 1 land, 2 open
   std::vector<int32_t> side_nbr;   // [ns][4] Side::neigh(j) of internal sides (meshdata.h:1409-1422), -1 else
   std::vector<int8_t> isbnd;       // [n] 1 = node on the mesh boundary
+  std::vector<int32_t> node_bside; // [n][2] boundary sides ending at the node, ascending, -1 = none
   std::vector<int32_t> ball_ptr;   // [n+1]
   std::vector<int32_t> ball_elem;  // CCW ball
   std::vector<int8_t> ball_loc;    // local index of the node in each ball element

@@ -1150,8 +1150,8 @@ void launch_vertical_velocity(const DMesh& m, double h0, hipStream_t st) {
 // increments in their order (advective/ghat term, eta terms of corners 0-2, volume source).
 // k_assemble then gives each node its CSR row (node-gather, no atomics) by adding those terms
 // over its ball in ball order, so every sum has the reference's operands and order.
-// Open boundaries carry an essential BC here, so the natural-BC integrals I3/I5 of open nodes
-// (meshdata.cu:3380-3470) are overwritten and not evaluated.
+// Open nodes of elevation type 0 (no essential BC) add the boundary integral I_5; I_3 is zero
+// because every open boundary here has flow type ifltype = 0 (meshdata.cu:3373-3470).
 __global__ void __launch_bounds__(256) k_elem_assemble(DMesh m, StepArgs a) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= m.ne) return;
@@ -1222,6 +1222,21 @@ __device__ __forceinline__ double assemble_row(const DMesh& m, const StepArgs& a
     qel += c2.y;
     qel += c3.x;
     qel += c3.y;
+  }
+  // I_5 of an open node without an essential BC (meshdata.cu:3373-3380, 3457-3463): its two
+  // boundary sides (open or land, ascending side index), the non-dry ones.
+  if (m.open_bnd[i] && node_etype(m, a, i) == 0) {
+    const int2 bs = m.node_bside[i];
+    const int sl[2] = {bs.x, bs.y};
+    for (int l = 0; l < 2; ++l) {
+      const int s = sl[l];
+      if (s < 0 || !m.side_wet[s]) continue;
+      const int4 sr = m.side[s];
+      const double vx = -(m.y[sr.x] - m.y[sr.y]), vy = m.x[sr.x] - m.x[sr.y];
+      const double ql = sqrt(vx * vx + vy * vy);
+      const double Unbar = m.sU[2 * s] * (vx / ql) + m.sU[2 * s + 1] * (vy / ql);
+      qel -= 0.5 * (1.0 - a.theta) * a.dt * m.side_len[s] * Unbar;
+    }
   }
   if (m.open_bnd[i] && node_etype(m, a, i) != 0) {
     for (int k = 0; k < len; ++k) rv(k) = 0.0;

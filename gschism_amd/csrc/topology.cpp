@@ -197,6 +197,23 @@ void build_topology(const HostMesh& m, Topology& t) {
       const bool open = m.open_bnd[t.isidenode[2 * s]] && m.open_bnd[t.isidenode[2 * s + 1]];
       t.side_type[s] = open ? 2 : 1;
     }
+  // The boundary sides ending at each boundary node, ascending side index (-1 where fewer than
+  // two): the sides of the open-boundary integral I_5 (meshdata.cu:3373-3380). An open node
+  // links at most two (the assert of NodeBoundary::linkSideBoundary, meshdata.h:1544-1555).
+  t.node_bside.assign(2 * (size_t)n, -1);
+  for (int32_t s = 0; s < ns; ++s)
+    if (t.isdel[2 * s + 1] < 0)
+      for (int j = 0; j < 2; ++j) {
+        const int32_t i = t.isidenode[2 * s + j];
+        int32_t* b = &t.node_bside[2 * (size_t)i];
+        if (b[0] < 0) {
+          b[0] = s;
+        } else if (b[1] < 0) {
+          b[1] = s;
+        } else if (m.open_bnd[i]) {
+          throw Error(GSH_E_TOPOLOGY, "open boundary node " + std::to_string(i) + " ends more than two boundary sides");
+        }
+      }
 
   // ---- CSR pattern (populateNodes2): row i = sorted {i} U neighbours; indptr[i] = nbrStart + i
   t.indptr.assign(n + 1, 0);

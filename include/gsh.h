@@ -345,8 +345,14 @@ typedef struct gsh_server_config {
 } gsh_server_config;
 int gsh_server_create(gsh_ctx* ctx, const gsh_server_config* cfg, gsh_server** out);
 int gsh_server_destroy(gsh_server* srv);
-/* Registers a mesh so that requests can reference it; the mesh stays device resident. */
+/* Registers a mesh so that requests can reference it; the mesh stays device resident. Its
+ * requests' set-up (backtracking, RHS producers, assembly) uses gsh_step_params_default until
+ * gsh_server_set_mesh_params gives the mesh other step parameters. */
 int gsh_server_add_mesh(gsh_server* srv, gsh_mesh* mesh, int32_t* mesh_id_out);
+/* Step parameters (dt, theta, iettype, ...) of a registered mesh, used by every request on it
+ * admitted after the call; validated here (GSH_E_ARG). The reference runs one step parameter
+ * set per model (schism_step.py), which a caller passes once per mesh. */
+int gsh_server_set_mesh_params(gsh_server* srv, int32_t mesh_id, const gsh_step_params* p);
 /* Submits one solve (+ optional backtrack) request: eta2/b host vectors are copied at submit.
  * with_backtrack != 0 runs gsh_backtrack on the mesh's current velocities before the solve.
  * b_host = NULL assembles the RHS from the mesh state. */
@@ -367,7 +373,9 @@ int gsh_server_poll(gsh_server* srv, int64_t ticket, double* x_host, int32_t* it
 /* ---------------------------------------------------------------- measurement */
 /* No counterpart in the reference: measurement helpers of this library.
  * gsh_copy_probe: average time of one device-to-device copy of `bytes` (read + write) by a
- * streaming kernel, the achievable-bandwidth reference. gsh_solver_profile: one FP64 solve (same
+ * streaming kernel, the fastest of seven launch shapes (grid-stride double4 copies at 8-32
+ * workgroups per CU, 1 or 4 records in flight per thread, cached or nontemporal stores): the
+ * achievable-bandwidth reference. gsh_solver_profile: one FP64 solve (same
  * arithmetic as gsh_solver_solve) with HIP events around the launches of every iteration;
  * ms[4] receives the mean time per iteration of each launch kind:
  *   Jacobi: ms[0] phase A (p update + SpMV + p.q), ms[1] phase B (x, r, z updates + r.r, r.z);

@@ -34,5 +34,20 @@ def build(force: bool = False) -> Path:
     return LIB
 
 
+def build_planted(k: int) -> Path:
+    """The oracle with deliberate mistake k (OR_PLANT in gsh_oracle.c) built into its own library,
+    for the tests that show a pin fails on it."""
+    OUT.mkdir(exist_ok=True)
+    lib = OUT / f"liboracle_plant{k}.so"
+    if not lib.exists() or max(SRC.stat().st_mtime, HDR.stat().st_mtime) > lib.stat().st_mtime:
+        tmp = lib.with_suffix(".so.tmp")
+        r = subprocess.run([CC, *FLAGS, f"-DOR_PLANT={int(k)}", str(SRC), "-o", str(tmp), "-lm"], capture_output=True,
+                           text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"oracle build failed:\n{r.stdout}\n{r.stderr}")
+        os.replace(tmp, lib)
+    return lib
+
+
 if __name__ == "__main__":
     print(build(force="--force" in sys.argv))

@@ -13,7 +13,7 @@
  *   src/kernels/meshdata.cu:3547-3660, 1449-1629, 85-127, 635-746,
  *                           990-1043, 1209-1291                     ELM backtracking
  *   src/kernels/meshdata.cu:4459-4476, 4615-4725, 6183-6424, 4276-4339   RHS producers
- *   src/kernels/meshdata.cu:3182-3311, 4400-4446                    assembly + essential BC
+ *   src/kernels/meshdata.cu:3182-3311, 3373-3380, 3457-3463, 4400-4446   assembly, I_5, essential BC
  *   src/solver/pcg_solver.py:172-276, 881-947, 1075-1124; pcg_loop.cpp:361-413   Jacobi PCG
  *   src/kernels/meshdata.cu:5765-5929                               nodal velocities
  *   src/kernels/meshdata.cu:5045-5446, 3665-3711, 5456-5518         momentum, Shapiro filter
@@ -35,6 +35,12 @@
  * Compile with -ffp-contract=off and without fast-math.
  */
 #include "gsh_oracle.h"
+
+/* OR_PLANT = k (k > 0) builds the oracle with one deliberate mistake, so that the CPU tests can
+ * show that their pins catch it (oracle/build.py: build_planted). The default build has none. */
+#ifndef OR_PLANT
+#define OR_PLANT 0
+#endif
 
 #include <math.h>
 #include <stdarg.h>
@@ -304,6 +310,22 @@ static int build_topology(or_mesh* m) {
   for (int s = 0; s < ns; ++s)
     if (m->isdel[2 * s + 1] == -1)
       m->side_type[s] = (m->open[m->isidenode[2 * s]] && m->open[m->isidenode[2 * s + 1]]) ? 2 : 1;
+  /* an open node links at most two boundary sides (the assert of NodeBoundary::linkSideBoundary,
+   * meshdata.h:1544-1555) */
+  {
+    int8_t* cnt = xcalloc(n, 1);
+    for (int s = 0; s < ns; ++s)
+      if (m->isdel[2 * s + 1] == -1)
+        for (int j = 0; j < 2; ++j) {
+          const int i = m->isidenode[2 * s + j];
+          if (m->open[i] && ++cnt[i] > 2) {
+            free(cnt);
+            set_err("open boundary node %d ends more than two boundary sides", i);
+            return -1;
+          }
+        }
+    free(cnt);
+  }
   /* populateNodes / populateNodes2 (meshdata.cu:1931-1935, 2094-2141) */
   m->indptr = xcalloc(n + 1, sizeof(int32_t));
   for (int i = 0; i <= n; ++i) m->indptr[i] = m->nnp_ptr[i] + i;
@@ -1372,7 +1394,11 @@ void or_rhs(or_mesh* m, const or_params* p) {
     double U[2] = {0.0, 0.0};
     if (!m->side_dry[s]) {
       for (int k = kb; k < nv - 1; ++k) {
+#if OR_PLANT == 6
+        const double uc[2] = {u2[2 * k], u2[2 * k + 1]};
+#else
         const double uc[2] = {0.5 * (u2[2 * k] + u2[2 * (k + 1)]), 0.5 * (u2[2 * k + 1] + u2[2 * (k + 1) + 1])};
+#endif
         const double Dz = zc[k + 1] - zc[k];
         U[0] += uc[0] * Dz;
         U[1] += uc[1] * Dz;
@@ -1385,7 +1411,14 @@ void or_rhs(or_mesh* m, const or_params* p) {
     m->chi[s] = 0.0;
     if (m->side_dry[s]) continue;
     const double htot = 0.5 * (m->eta2[n1] + m->eta2[n2]) + 0.5 * (m->depth[n1] + m->depth[n2]);
-    const double vmag1 = sqrt(bt[2 * (kb + 1)] * bt[2 * (kb + 1)] + bt[2 * (kb + 1) + 1] * bt[2 * (kb + 1) + 1]);
+#if OR_PLANT == 4
+    const int kc = kb;
+#elif OR_PLANT == 5
+    const int kc = kb + 2 < nv ? kb + 2 : kb + 1;
+#else
+    const int kc = kb + 1;
+#endif
+    const double vmag1 = sqrt(bt[2 * kc] * bt[2 * kc] + bt[2 * kc + 1] * bt[2 * kc + 1]);
     const double Cd = 0.5 * (m->cd[n1] + m->cd[n2]);
     const double chi2 = Cd * vmag1;
     const double chi = chi2;
@@ -1395,7 +1428,9 @@ void or_rhs(or_mesh* m, const or_params* p) {
       m->hhat[s] = htot * htot / fricH;
     } else {
       m->hhat[s] = htot - chi * dt;
+#if OR_PLANT != 7
       if (p->ihhat == 1) m->hhat[s] = fmax(0.0, m->hhat[s]);
+#endif
     }
   }
   /* computeElemAverageGhat (meshdata.cu:6183-6424), barotropic subset */
@@ -1430,14 +1465,23 @@ void or_rhs(or_mesh* m, const or_params* p) {
         } else {
           for (int k = kb + 1; k < nv; ++k) {
             const double Dz = zc[k] - zc[k - 1];
+#if OR_PLANT == 9
+            Ustar[0] += bt[2 * k] * Dz;
+            Ustar[1] += bt[2 * k + 1] * Dz;
+#else
             Ustar[0] += (bt[2 * k] + bt[2 * (k - 1)]) / 2.0 * Dz;
             Ustar[1] += (bt[2 * k + 1] + bt[2 * (k - 1) + 1]) / 2.0 * Dz;
+#endif
           }
         }
         g[0] += Ustar[0] * cff1 - Ustar_alpha[0] * cff3;
         g[1] += Ustar[1] * cff1 - Ustar_alpha[1] * cff3;
         const double tau[2] = {0.0, 0.0};
+#if OR_PLANT == 8
+        const double ub[2] = {bt[2 * kb], bt[2 * kb + 1]};
+#else
         const double ub[2] = {bt[2 * (kb + 1)], bt[2 * (kb + 1) + 1]};
+#endif
         g[0] -= ub[0] * (chi * dt * cff2) + tau[0] * (dt * cff1);
         g[1] -= ub[1] * (chi * dt * cff2) + tau[1] * (dt * cff1);
       }
@@ -1465,6 +1509,26 @@ static void add_to_row(const or_mesh* m, int node1, int node, double value, doub
     abort();
   }
   row[m->csr_slot[b + i]] += value;
+}
+
+/* The boundary sides (one element) that end at node i, in ascending side index; -1 where
+ * fewer than two. */
+static void node_bsides(const or_mesh* m, int i, int out[2]) {
+  out[0] = out[1] = -1;
+  for (int j = m->nne_ptr[i]; j < m->nne_ptr[i + 1]; ++j) {
+    const int e = m->nne[j];
+    for (int k = 0; k < 3; ++k) {
+      const int s = m->elside[3 * e + k];
+      if (m->isdel[2 * s + 1] >= 0 || (m->isidenode[2 * s] != i && m->isidenode[2 * s + 1] != i)) continue;
+      if (s == out[0] || s == out[1]) continue;
+      if (out[0] < 0 || s < out[0]) {
+        out[1] = out[0];
+        out[0] = s;
+      } else if (out[1] < 0 || s < out[1]) {
+        out[1] = s;
+      }
+    }
+  }
 }
 
 void or_assemble(or_mesh* m, const or_params* p, double* val, double* b, int nthreads) {
@@ -1495,7 +1559,11 @@ void or_assemble(or_mesh* m, const or_params* p, double* val, double* b, int nth
       const double gradPhi_1[2] = {gp[2 * id1], gp[2 * id1 + 1]};
       const double I_4_1 = gradPhi_1[0] * U[0] + gradPhi_1[1] * U[1];
       const double I_4_2 = gradPhi_1[0] * m->ghat1[2 * e] + gradPhi_1[1] * m->ghat1[2 * e + 1];
+#if OR_PLANT == 1
+      qel += dt * area * (thetai * I_4_1 + (1 - thetai) * I_4_2);
+#else
       qel += dt * area * ((1 - thetai) * I_4_1 + thetai * I_4_2);
+#endif
       const double fricCoeff = grav3 * thetai * (1.0 - thetai) * dt * dt * hhatb;
       const int id2 = nxq(1 + id1, 0, 3), id3 = nxq(2 + id1, 0, 3);
       const int node2 = nd[id2], node3 = nd[id3];
@@ -1518,10 +1586,33 @@ void or_assemble(or_mesh* m, const or_params* p, double* val, double* b, int nth
         const double extraEta = 0.0;
         const double ipp = ((id1 == l) ? (2.0 / 12.0) : (1.0 / 12.0)) * area;      /* intPhiPhiTri */
         const double igpgp = area * (gp[2 * id1] * gp[2 * l] + gp[2 * id1 + 1] * gp[2 * l + 1]); /* intGradPhiGradPhiTri */
+#if OR_PLANT == 2
+        const double isElemWet = 1.0;
+#else
         const double isElemWet = m->elem_dry[e] ? 0.0 : 1.0;
+#endif
         qel += (m->eta2[node] + extraEta) * ipp - m->eta2[node] * fricCoeff * igpgp * isElemWet;
       }
       qel += (dt / 3) * m->vsource[e];
+    }
+    /* Boundary integral I_5 at open nodes (meshdata.cu:3373-3380, 3457-3463): over the node's two
+     * boundary sides (open or land: NodeBoundary::linkSideBoundary links both kinds,
+     * meshdata.cu:2198-2221, meshdata.h:1544-1555), taken in ascending side index, the non-dry
+     * ones subtract 0.5 (1 - theta) dt len (U . n). Essential rows overwrite it below. I_3 is
+     * zero: every flow type here is ifltype = 0. */
+    if (m->open[node1] && node_etype(m, p, node1) == 0) {
+      int bs[2];
+      node_bsides(m, node1, bs);
+      for (int l = 0; l < 2; l++) {
+        const int s = bs[l];
+        if (s < 0 || m->side_dry[s]) continue;
+        const double Unbar = m->sU[2 * s] * m->snx[s] + m->sU[2 * s + 1] * m->sny[s];
+#if OR_PLANT == 3
+        qel += 0.5 * (1.0 - thetai) * dt * m->slen[s] * Unbar;
+#else
+        qel -= 0.5 * (1.0 - thetai) * dt * m->slen[s] * Unbar;
+#endif
+      }
     }
     /* setEssentialBC (meshdata.cu:4400-4446): essential at open nodes of type != 0
      * (isEssentialBC = iettype != 0, meshdata.cu:2506) */

@@ -58,21 +58,26 @@ _SIG = {
     "or_true_rel": (C.c_double, [C.c_int32, _PI32, _PI32, _PD, _PD, _PD, C.c_int32]),
 }
 
-_lib = None
+_libs: dict = {}
 
 
-def lib() -> C.CDLL:
-    global _lib
-    if _lib is None:
-        if not LIB_PATH.exists():
-            raise RuntimeError(f"{LIB_PATH} is not built: run python oracle/build.py")
-        L = C.CDLL(str(LIB_PATH))
+def load(path) -> C.CDLL:
+    """Loads an oracle library (the default build, or one with a planted mistake) once per path."""
+    key = str(path)
+    if key not in _libs:
+        if not Path(path).exists():
+            raise RuntimeError(f"{path} is not built: run python oracle/build.py")
+        L = C.CDLL(key)
         for k, (r, a) in _SIG.items():
             f = getattr(L, k)
             f.restype = r
             f.argtypes = a
-        _lib = L
-    return _lib
+        _libs[key] = L
+    return _libs[key]
+
+
+def lib() -> C.CDLL:
+    return load(LIB_PATH)
 
 
 def params(p) -> OrParams:
@@ -118,8 +123,8 @@ _TOPO = {
 class OracleMesh:
     """FP64 CPU restatement of one mesh and its per-step state."""
 
-    def __init__(self, xy, elnode, depth, open_bnd, sigma, kb=None):
-        L = lib()
+    def __init__(self, xy, elnode, depth, open_bnd, sigma, kb=None, library=None):
+        L = self.L = lib() if library is None else library
         self.xy = np.ascontiguousarray(xy, dtype=np.float64)
         self.elnode = np.ascontiguousarray(elnode, dtype=np.int32)
         self.depth = np.ascontiguousarray(depth, dtype=np.float64)
@@ -139,76 +144,76 @@ class OracleMesh:
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().or_mesh_free(self.h)
+            self.L.or_mesh_free(self.h)
             self.h = None
 
     def topology(self, code: int) -> np.ndarray:
         dt, shp = _TOPO[code]
         out = np.empty(shp(self), dtype=dt)
-        if lib().or_mesh_get(self.h, code, out.ctypes.data_as(C.c_void_p)):
-            raise RuntimeError(lib().or_error().decode())
+        if self.L.or_mesh_get(self.h, code, out.ctypes.data_as(C.c_void_p)):
+            raise RuntimeError(self.L.or_error().decode())
         return out
 
     def set(self, code: int, arr) -> None:
         dt, shp = _FIELDS[code]
         a = np.ascontiguousarray(arr, dtype=dt).reshape(shp(self))
-        if lib().or_set_field(self.h, code, a.ctypes.data_as(C.c_void_p)):
-            raise RuntimeError(lib().or_error().decode())
+        if self.L.or_set_field(self.h, code, a.ctypes.data_as(C.c_void_p)):
+            raise RuntimeError(self.L.or_error().decode())
 
     def get(self, code: int) -> np.ndarray:
         dt, shp = _FIELDS[code]
         out = np.empty(shp(self), dtype=dt)
-        if lib().or_get_field(self.h, code, out.ctypes.data_as(C.c_void_p)):
-            raise RuntimeError(lib().or_error().decode())
+        if self.L.or_get_field(self.h, code, out.ctypes.data_as(C.c_void_p)):
+            raise RuntimeError(self.L.or_error().decode())
         return out
 
     def levels(self, p) -> None:
-        lib().or_levels(self.h, C.byref(params(p)))
+        self.L.or_levels(self.h, C.byref(params(p)))
 
     def backtrack(self, p, sides=None, levels=None, nthreads: int = 1) -> None:
         if sides is None:
-            lib().or_backtrack(self.h, C.byref(params(p)), None, None, -1, nthreads)
+            self.L.or_backtrack(self.h, C.byref(params(p)), None, None, -1, nthreads)
         else:
             s = np.ascontiguousarray(sides, dtype=np.int32)
             k = np.ascontiguousarray(levels, dtype=np.int32)
-            lib().or_backtrack(self.h, C.byref(params(p)), _i(s), _i(k), s.size, nthreads)
+            self.L.or_backtrack(self.h, C.byref(params(p)), _i(s), _i(k), s.size, nthreads)
 
     def geometry(self, what: int) -> np.ndarray:
         shp = {0: (self.ne,), 1: (self.ne, 3, 2), 2: (self.ns, 2), 3: (self.ns,), 4: (self.ns, 2)}[what]
         out = np.empty(shp, dtype=np.float64)
-        if lib().or_geometry(self.h, what, _d(out)):
-            raise RuntimeError(lib().or_error().decode())
+        if self.L.or_geometry(self.h, what, _d(out)):
+            raise RuntimeError(self.L.or_error().decode())
         return out
 
     def interp(self, e: int, x: float, y: float, z: float, h0: float = 0.01) -> np.ndarray:
         out = np.empty(3, dtype=np.float64)
-        lib().or_interp(self.h, e, x, y, z, h0, _d(out))
+        self.L.or_interp(self.h, e, x, y, z, h0, _d(out))
         return out
 
     def rhs(self, p) -> None:
-        lib().or_rhs(self.h, C.byref(params(p)))
+        self.L.or_rhs(self.h, C.byref(params(p)))
 
     def nodal_velocity(self, p, nthreads: int = 1) -> None:
-        lib().or_nodal_velocity(self.h, C.byref(params(p)), nthreads)
+        self.L.or_nodal_velocity(self.h, C.byref(params(p)), nthreads)
 
     def set_tides(self, amig, ff, face, emo, efa) -> None:
         """Tidal constituents (amig, ff, face [nbfr]) and open-node amplitudes and phases [n][nbfr]."""
         a = [np.ascontiguousarray(v, dtype=np.float64) for v in (amig, ff, face)]
         nb = a[0].size
         e = [np.ascontiguousarray(v, dtype=np.float64).reshape(self.n, nb) for v in (emo, efa)]
-        if lib().or_set_tides(self.h, nb, *[v.ctypes.data_as(_PD) for v in a + e]):
-            raise RuntimeError(lib().or_error().decode())
+        if self.L.or_set_tides(self.h, nb, *[v.ctypes.data_as(_PD) for v in a + e]):
+            raise RuntimeError(self.L.or_error().decode())
 
     def momentum(self, p) -> None:
-        lib().or_momentum(self.h, C.byref(params(p)))
+        self.L.or_momentum(self.h, C.byref(params(p)))
 
     def vertical_velocity(self) -> None:
-        lib().or_vertical_velocity(self.h)
+        self.L.or_vertical_velocity(self.h)
 
     def assemble(self, p, nthreads: int = 1):
         val = np.zeros(self.nnz, dtype=np.float64)
         b = np.zeros(self.n, dtype=np.float64)
-        lib().or_assemble(self.h, C.byref(params(p)), _d(val), _d(b), nthreads)
+        self.L.or_assemble(self.h, C.byref(params(p)), _d(val), _d(b), nthreads)
         return val, b
 
 

@@ -92,6 +92,7 @@ def test_reference_checks_are_errors(tmp_path):
 
 def test_elevation_forcing_last_boundary_wins(tmp_path):
     t = read_bctides(_write(tmp_path, HAND), 3, 3)
+    t.ifltype[:] = 0  # flow forcing is refused (test_flow_forcing_is_refused)
     segs = [np.array([0, 1, 2]), np.array([2, 3]), np.array([3, 4])]
     etype, eth, emo, efa = elevation_forcing(t, segs, 6)
     assert etype.tolist() == [3, 3, 2, 5, 5, 0]
@@ -169,3 +170,17 @@ def test_oracle_essential_rows_follow_node_types():
             assert np.all(val[lo:hi][~keep] == 0.0)
             exp = ref_b[i] - np.sum(ref_val[lo:hi][~keep] * elbc[ix[lo:hi][~keep]])
             assert abs(b[i] - exp) <= 1e-12 * max(1.0, abs(exp))
+
+
+def test_flow_forcing_is_refused(tmp_path):
+    """Flow types (ifltype != 0) are parsed but not applied: the per-node forcing refuses them
+    instead of running as if the boundary had none."""
+    t = read_bctides(_write(tmp_path, HAND), 3, 3)
+    segs = [np.array([0, 1, 2]), np.array([2, 3]), np.array([3, 4])]
+    assert t.ifltype.tolist() == [0, 2, 0]
+    with pytest.raises(NotImplementedError, match="flow forcing"):
+        elevation_forcing(t, segs, 6)
+    t.ifltype[1] = 0
+    etype, _, _, _ = elevation_forcing(t, segs, 6)
+    assert etype.tolist() == [3, 3, 2, 5, 5, 0]
+

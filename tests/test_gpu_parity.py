@@ -371,9 +371,39 @@ def test_mixed_batch_equals_solo(ctx):
         assert_bitwise(mx.get_solution(), xm, "mixed batched solution")
 
 
+def _timeout_then_same_result(ctx, mesh, solver, val, b, xo, ito):
+    """A 1M solve whose host wait gives up at once (timeout 1 us: GSH_E_TIMEOUT while the
+    launched kernels still run to their normal end, pcg_loop.cpp:444-456); the next solve of the
+    same solver, with the history reset, reproduces the oracle's solution and iteration count.
+    The same for the SSOR solver against the oracle's SSOR PCG."""
+    ip, ix = mesh.topology(L.T_CSR_INDPTR), mesh.topology(L.T_CSR_INDICES)
+    print("[gpu] 1M solve with a 1 us time limit", flush=True)
+    solver.reset_history()
+    with pytest.raises(L.GshError) as ei:
+        solver.solve(None, rtol=1e-12, maxiter=3000, timeout_s=1e-6)
+    assert ei.value.code == L.GSH_E_TIMEOUT
+    solver.reset_history()
+    x, it, st = solver.solve(None, rtol=1e-12, maxiter=3000)
+    assert st == 0 and it == ito
+    assert_bitwise(x, xo, "1M PCG solution after a timed-out solve")
+    ss = core.Solver(mesh, L.PREC_FP64, precond=L.PRECOND_SSOR)
+    ss.set_tuning(2)
+    ss.set_values(val)
+    print("[gpu] 1M SSOR solve with a 1 us time limit", flush=True)
+    with pytest.raises(L.GshError) as ei:
+        ss.solve(b, rtol=1e-12, maxiter=3000, timeout_s=1e-6)
+    assert ei.value.code == L.GSH_E_TIMEOUT
+    ss.reset_history()
+    x, it, st = ss.solve(b, rtol=1e-12, maxiter=3000)
+    xs, its, _, _, _ = O.pcg_ssor(ip, ix, val, b, rtol=1e-12, maxiter=3000, rows_per_thread=2, nthreads=16)
+    assert st == 0 and it == its, (it, its)
+    assert_bitwise(x, xs, "1M SSOR solution after a timed-out solve")
+    ss.close()
+
+
 def test_full_size_1m_channel(ctx):
-    """The 1024 x 1024 log-normal channel (1.05M nodes, 31 levels): a sample of backtracks
-    against the oracle's list mode; RHS, assembly and the whole FP64 solve bit for bit."""
+    """The 1024 x 1024 log-normal channel (1.05M nodes, 31 levels): every backtrack (all sides x
+    levels, 10 substeps) against the oracle; RHS, assembly and the whole FP64 solve bit for bit."""
     spec = cases.spec_gpu_1m()
     data = cases.make_case(spec)
     print(f"[gpu] 1M case {spec.name}: building", flush=True)
@@ -396,20 +426,14 @@ def test_full_size_1m_channel(ctx):
     assert_bitwise(mesh.get(L.F_NODE_UVW), om.get(L.F_NODE_UVW), "1M nodal velocities")
     print("[gpu] 1M backtrack", flush=True)
     mesh.backtrack(p)
-    rng = np.random.default_rng(11)
-    cnt = 1 << 15
-    pick = np.unique(rng.integers(0, mesh.ns * mesh.nvrt, cnt))
-    s = (pick // mesh.nvrt).astype(np.int32)
-    k = (pick % mesh.nvrt).astype(np.int32)
-    om.backtrack(p, sides=s, levels=k, nthreads=16)
-    foot, btu, bst = mesh.get(L.F_FOOT), mesh.get(L.F_BTU), mesh.get(L.F_BSTATUS)
-    assert_bitwise(foot[s, k], om.get(L.F_FOOT)[s, k], "1M sample foot elements")
-    assert_bitwise(btu[s, k], om.get(L.F_BTU)[s, k], "1M sample backtracked velocity")
-    assert_bitwise(bst[s, k], om.get(L.F_BSTATUS)[s, k], "1M sample status")
+    print(f"[gpu] 1M backtrack on the oracle: {mesh.ns} sides x {mesh.nvrt} levels", flush=True)
+    om.backtrack(p, nthreads=16)
+    foot = mesh.get(L.F_FOOT)
+    assert_bitwise(foot, om.get(L.F_FOOT), "1M foot elements (every side and level)")
     assert (foot >= 0).mean() > 0.9
-    # the oracle continues from the device's (sample-checked) backtracked velocities
-    om.set(L.F_BTU, btu)
-    del foot, btu, bst
+    del foot
+    assert_bitwise(mesh.get(L.F_BTU), om.get(L.F_BTU), "1M backtracked velocity")
+    assert_bitwise(mesh.get(L.F_BSTATUS), om.get(L.F_BSTATUS), "1M backtracking status")
     print("[gpu] 1M rhs + assembly", flush=True)
     mesh.rhs_producers(p)
     om.rhs(p)
@@ -425,6 +449,7 @@ def test_full_size_1m_channel(ctx):
     assert st == 0 and it == ito, (it, ito, st)
     assert_bitwise(x, xo, "1M PCG solution")
     assert solver.check() < 1e-10
+    _timeout_then_same_result(ctx, mesh, solver, val, b, xo, ito)
     print("[gpu] 1M momentum", flush=True)
     solver.update_elevation()
     om.set(L.F_ETA1, eta)
@@ -596,6 +621,15 @@ def test_step_call_argument_errors(ctx):
     assert ei.value.code == L.GSH_E_ARG
     with pytest.raises(L.GshError):
         core.Server(ctx, precond=7)
+    for kw in (dict(iettype=6), dict(iettype=-1), dict(shapiro0=0.7)):
+        with pytest.raises(L.GshError) as ei:
+            mesh.rhs_producers(L.default_step_params(**kw))
+        assert ei.value.code == L.GSH_E_ARG, kw
+    bad_types = np.full(n, -1, dtype=np.int8)
+    bad_types[0] = 6
+    with pytest.raises(L.GshError) as ei:
+        mesh.set(L.F_ETYPE, bad_types)
+    assert ei.value.code == L.GSH_E_ARG
     mesh.backtrack(p)
     mesh.rhs_producers(p)
     mesh.momentum(p)

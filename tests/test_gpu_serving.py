@@ -169,3 +169,148 @@ def test_mixed_ssor_alone_vs_mix(ctx):
         assert it == alone[k][1], (k, it, alone[k][1])
         assert np.array_equal(x, alone[k][0]), f"mixed SSOR request on mesh {k} differs from the solo solve"
     srv.close()
+
+
+def test_served_request_uses_mesh_step_params(ctx):
+    """A served request (backtracking, RHS producers and assembly at admission, no RHS given) with
+    non-default step parameters set for its mesh equals the solo sequence with the same
+    parameters bit for bit, and both equal the oracle; open nodes of type 0 bring in the
+    open-boundary integral."""
+    from conftest import assert_bitwise, oracle_mesh_for
+    from oracle import pyoracle as O
+
+    spec = cases.CaseSpec("srvp-36x24", 36, 24, 600.0, nvrt=5, depth=None, substeps=4)
+    data = cases.make_case(spec)
+    mesh, _, (eta, uvw, su2) = cases.load_mesh(ctx, data)
+    p = L.default_step_params(dt=90.0, theta=0.7, dtb_min=15.0, dtb_max=30.0, iettype=2)
+    opn = np.flatnonzero(mesh.topology(L.T_OPEN_BND))
+    etype = np.full(mesh.n, -1, dtype=np.int8)
+    etype[opn[::2]] = 0
+    eth = np.where(mesh.topology(L.T_OPEN_BND) != 0, 0.07, 0.0)
+    om = oracle_mesh_for(mesh, data.sigma)
+    for m in (mesh, om):
+        m.set(L.F_ETYPE, etype)
+        m.set(L.F_ETH, eth)
+    om.set(L.F_ETA2, eta)
+    om.set(L.F_NODE_UVW, uvw)
+    om.set(L.F_SIDE_U2, su2)
+    om.levels(p)
+    # solo
+    solo = core.Solver(mesh)
+    mesh.backtrack(p)
+    mesh.rhs_producers(p)
+    solo.assemble(p)
+    x_solo, it_solo, st = solo.solve(None, rtol=1e-12, maxiter=3000, check_every=10)
+    assert st == 0
+    # oracle
+    om.backtrack(p, nthreads=8)
+    om.rhs(p)
+    val, b = om.assemble(p)
+    assert_bitwise(solo.get_values(), val, "matrix with the request's step parameters")
+    assert_bitwise(solo.get_rhs(), b, "RHS with the request's step parameters")
+    xo, ito, _, _ = O.pcg(mesh.topology(L.T_CSR_INDPTR), mesh.topology(L.T_CSR_INDICES), val, b, rtol=1e-12,
+                          maxiter=3000, check_every=10)
+    assert ito == it_solo
+    assert_bitwise(x_solo, xo, "solo solution")
+    # served, twice (the second on a pooled solver)
+    srv = core.Server(ctx, rtol=1e-12, maxiter=3000, check_every=10, max_active=2)
+    mid = srv.add_mesh(mesh, params=p)
+    print("[gpu] served request with mesh step parameters", flush=True)
+    for _ in range(2):
+        t = srv.submit(mid, None, with_backtrack=True)
+        for _ in range(2000):
+            a, q = srv.step()
+            if a == 0 and q == 0:
+                break
+        st, x, it, lat = srv.poll(t, mesh.n)
+        assert st == 0 and it == it_solo
+        assert_bitwise(x, x_solo, "served solution with the mesh's step parameters")
+    bad = L.default_step_params(iettype=9)
+    with pytest.raises(L.GshError) as ei:
+        srv.set_mesh_params(mid, bad)
+    assert ei.value.code == L.GSH_E_ARG
+    srv.close()
+    solo.close()
+    mesh.close()
+
+
+def _pool_mesh(ctx, k):
+    """Mesh k of the served configuration's pool (n log-uniform from 1e4; seeds 3 + k)."""
+    from math import sqrt
+
+    n = int(round(1e4 * 200 ** (k / 5)))
+    nx = int(round(sqrt(2 * n)))
+    ny = max(3, n // nx)
+    spec = cases.CaseSpec(f"pool{k}-{nx}x{ny}", nx, ny, 1000.0, nvrt=11, depth=None, substeps=10, corr_cells=20.0)
+    data = cases.make_case(spec)
+    data.depth = cases.bathymetry(nx, ny, 20.0, 5.0, 4000.0, seed=3 + k)
+    mesh, _, state = cases.load_mesh(ctx, data)
+    return mesh, data, state
+
+
+@pytest.mark.parametrize("precond", [L.PRECOND_JACOBI, L.PRECOND_SSOR])
+def test_served_pool_against_oracle(ctx, precond):
+    """The served configuration's two smallest pool meshes (10k and 29k nodes, 11 levels, 10
+    backtracking substeps): requests with backtracking, served in a mix, in FP64 equal the
+    oracle's backtracking + assembly + PCG bit for bit (Jacobi at 4 rows per thread, SSOR at 2),
+    and in mixed precision are within 1e-8 of it."""
+    from conftest import oracle_mesh_for
+    from oracle import pyoracle as O
+
+    pd = L.default_step_params()
+    pool, ref = [], []
+    for k in (0, 1):
+        mesh, data, (eta, uvw, su2) = _pool_mesh(ctx, k)
+        om = oracle_mesh_for(mesh, data.sigma)
+        om.set(L.F_ETA2, eta)
+        om.set(L.F_NODE_UVW, uvw)
+        om.set(L.F_SIDE_U2, su2)
+        om.levels(pd)
+        print(f"[gpu] pool mesh {k}: n={mesh.n}, oracle backtracking", flush=True)
+        om.backtrack(pd, nthreads=16)
+        om.rhs(pd)
+        val, _ = om.assemble(pd)
+        ip, ix = mesh.topology(L.T_CSR_INDPTR), mesh.topology(L.T_CSR_INDICES)
+        pool.append(mesh)
+        ref.append((ip, ix, val))
+    which = [0, 1, 1, 0, 1]
+    rhs = [cases.normal(1000 + j, pool[k].n) for j, k in enumerate(which)]
+    xref = []
+    for j, k in enumerate(which):
+        ip, ix, val = ref[k]
+        if precond == L.PRECOND_JACOBI:
+            x, it, _, _ = O.pcg(ip, ix, val, rhs[j], rtol=1e-12, maxiter=20000, check_every=10, rows_per_thread=4,
+                                nthreads=16)
+        else:
+            x, it, _, _, _ = O.pcg_ssor(ip, ix, val, rhs[j], rtol=1e-12, maxiter=20000, check_every=10,
+                                        rows_per_thread=2, nthreads=16)
+        xref.append((x, it))
+    for precision in (L.PREC_FP64, L.PREC_MIXED):
+        srv = core.Server(ctx, rtol=1e-12 if precision == L.PREC_FP64 else 1e-10, maxiter=20000, check_every=10,
+                          max_active=3, precision=precision, precond=precond)
+        ids = [srv.add_mesh(m) for m in pool]
+        print(f"[gpu] served pool, precision {precision}, precond {precond}", flush=True)
+        tickets = []
+        for j, k in enumerate(which):
+            tickets.append(srv.submit(ids[k], rhs[j], with_backtrack=True))
+            if j % 2 == 1:
+                srv.step()
+        for _ in range(20000):
+            a, q = srv.step()
+            if a == 0 and q == 0:
+                break
+        for j, t in enumerate(tickets):
+            st, x, it, lat = srv.poll(t, pool[which[j]].n)
+            assert st == 0, (j, st)
+            x64, it64 = xref[j]
+            rel = np.linalg.norm(x - x64) / np.linalg.norm(x64)
+            print(f"[gpu]   request {j} (mesh {which[j]}): iterations {it} (oracle FP64 {it64}), rel L2 {rel:.2e}",
+                  flush=True)
+            if precision == L.PREC_FP64:
+                assert it == it64
+                assert np.array_equal(x, x64), f"served FP64 request {j} differs from the oracle"
+            else:
+                assert rel <= 1e-8
+        srv.close()
+    for m in pool:
+        m.close()
