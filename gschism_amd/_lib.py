@@ -28,6 +28,7 @@ PREC_FP64 = 0
 PREC_MIXED = 1
 PRECOND_JACOBI = 0
 PRECOND_SSOR = 1
+STATIC_MAX_ROWS = 1048576  # GSH_STATIC_MAX_ROWS: rows of gsh_pcg_solve_static's static workspace
 
 # topology array codes
 T_XY, T_ELNODE, T_ELSIDE, T_ISDEL, T_ISIDENODE, T_BALL_PTR, T_BALL_ELEM, T_NBR_NODE = range(8)
@@ -129,6 +130,7 @@ SIGNATURES = {
     "gsh_solver_get_solution": (_I, [_P, _PD]),
     "gsh_assemble": (_I, [_P, _P, C.POINTER(StepParams), _D]),
     "gsh_solver_solve": (_I, [_P, _PD, _PD, _D, _I, _I, _D, C.POINTER(_I), C.POINTER(_I)]),
+    "gsh_pcg_solve_static": (_I, [_P, _I32, _I64, _P, _P, _P, _P, _P, _D, _I, _I, _D, C.POINTER(_I), C.POINTER(_I)]),
     "gsh_solver_check": (_I, [_P, _PD]),
     "gsh_solver_reset_history": (_I, [_P]),
     "gsh_solver_last_stats": (_I, [_P, C.POINTER(SolveStats)]),

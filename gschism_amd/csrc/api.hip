@@ -16,6 +16,7 @@
 #include <deque>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <thread>
 #include <type_traits>
 #include <vector>
@@ -480,17 +481,18 @@ struct PhaseTimes {
   int64_t launches = 0;
 };
 
+// The iteration loop itself: dgather (device, 2 * nsys states) stages the window readbacks and
+// the final states land in the context's pinned slots 0..nsys-1. It allocates nothing when the
+// pinned slots already hold 2 * nsys states and no profile is asked for.
 template <class VT>
-static void run_pcg(gsh_ctx* ctx, const PcgSys<VT>* dsys, int nsys, const TileRef* dtiles, int ntiles, int rpt,
-                    double rtol, int maxiter, int check_every, double timeout_s, PcgOutcome& out,
-                    const TileRef* dinit_tiles = nullptr, int ninit_tiles = -1, PhaseTimes* prof = nullptr,
-                    const double* rtols_dev = nullptr, WindowGraph* graph = nullptr) {
+static void run_pcg_core(gsh_ctx* ctx, const PcgSys<VT>* dsys, int nsys, const TileRef* dtiles, int ntiles, int rpt,
+                         double rtol, int maxiter, int check_every, double timeout_s, PcgState* dgather,
+                         const TileRef* dinit_tiles, int ninit_tiles, PhaseTimes* prof, const double* rtols_dev,
+                         WindowGraph* graph) {
   hipStream_t st = ctx->stream;
   if (check_every < 1) check_every = 1;
   if (timeout_s <= 0.0) timeout_s = 60.0;
   PcgState* slots = ctx->pinned_slots(nsys);
-  StreamBuffers tmp(st);
-  PcgState* dgather = tmp.alloc<PcgState>(2 * (size_t)nsys);
   launch_pcg_reset<VT>(dsys, nsys, rtol, maxiter, check_every, st, rtols_dev);
   if (dinit_tiles)
     launch_pcg_init<VT>(dsys, dinit_tiles, ninit_tiles, rpt, st);
@@ -498,7 +500,7 @@ static void run_pcg(gsh_ctx* ctx, const PcgSys<VT>* dsys, int nsys, const TileRe
     launch_pcg_init<VT>(dsys, dtiles, ntiles, rpt, st);
   HIPCHK(hipGetLastError());
   int issued = 0, window = 0;
-  std::deque<int> inflight;  // window slots pending readback
+  int inflight[2], nflight = 0;  // window slots pending readback, oldest first
   bool all_done = false;
   const auto t0 = Clock::now();
   auto check_slot = [&](int slot) {
@@ -538,9 +540,10 @@ static void run_pcg(gsh_ctx* ctx, const PcgSys<VT>* dsys, int nsys, const TileRe
     }
     issued += w;
     const int slot = window & 1;
-    if (inflight.size() == 2) {
-      const int old = inflight.front();
-      inflight.pop_front();
+    if (nflight == 2) {
+      const int old = inflight[0];
+      inflight[0] = inflight[1];
+      nflight = 1;
       if (check_slot(old)) all_done = true;
     }
     hipLaunchKernelGGL(k_gather_states<VT>, dim3((nsys + 63) / 64), dim3(64), 0, st, dsys, nsys,
@@ -548,7 +551,7 @@ static void run_pcg(gsh_ctx* ctx, const PcgSys<VT>* dsys, int nsys, const TileRe
     HIPCHK(hipMemcpyAsync(slots + slot * nsys, dgather + slot * nsys, sizeof(PcgState) * nsys,
                           hipMemcpyDeviceToHost, st));
     HIPCHK(hipEventRecord(ctx->ev[slot], st));
-    inflight.push_back(slot);
+    inflight[nflight++] = slot;
     ++window;
     if (std::chrono::duration<double>(Clock::now() - t0).count() > timeout_s)
       throw Error(GSH_E_TIMEOUT, "PCG: time limit exceeded while issuing iterations");
@@ -559,6 +562,18 @@ static void run_pcg(gsh_ctx* ctx, const PcgSys<VT>* dsys, int nsys, const TileRe
   HIPCHK(hipMemcpyAsync(slots, dgather, sizeof(PcgState) * nsys, hipMemcpyDeviceToHost, st));
   HIPCHK(hipEventRecord(ctx->ev[0], st));
   wait_event(ctx->ev[0], timeout_s, "PCG finish");
+}
+
+template <class VT>
+static void run_pcg(gsh_ctx* ctx, const PcgSys<VT>* dsys, int nsys, const TileRef* dtiles, int ntiles, int rpt,
+                    double rtol, int maxiter, int check_every, double timeout_s, PcgOutcome& out,
+                    const TileRef* dinit_tiles = nullptr, int ninit_tiles = -1, PhaseTimes* prof = nullptr,
+                    const double* rtols_dev = nullptr, WindowGraph* graph = nullptr) {
+  StreamBuffers tmp(ctx->stream);
+  PcgState* dgather = tmp.alloc<PcgState>(2 * (size_t)nsys);
+  run_pcg_core<VT>(ctx, dsys, nsys, dtiles, ntiles, rpt, rtol, maxiter, check_every, timeout_s, dgather, dinit_tiles,
+                   ninit_tiles, prof, rtols_dev, graph);
+  const PcgState* slots = ctx->pinned;
   out.st.assign(slots, slots + nsys);
 }
 
@@ -691,7 +706,7 @@ static PcgState run_ssor(gsh_solver* s, double rtol, int maxiter, int check_ever
   };
   HIPCHK(hipGetLastError());
   int issued = 0, window = 0;
-  std::deque<int> inflight;
+  int inflight[2], nflight = 0;  // window slots pending readback, oldest first
   bool done = false;
   const auto t0 = Clock::now();
   auto check_slot = [&](int slot) {
@@ -735,14 +750,15 @@ static PcgState run_ssor(gsh_solver* s, double rtol, int maxiter, int check_ever
     }
     issued += w;
     const int slot = window & 1;
-    if (inflight.size() == 2) {
-      const int old = inflight.front();
-      inflight.pop_front();
+    if (nflight == 2) {
+      const int old = inflight[0];
+      inflight[0] = inflight[1];
+      nflight = 1;
       if (check_slot(old)) done = true;
     }
     HIPCHK(hipMemcpyAsync(slots + slot, ssor_host<VT>(S).st, sizeof(PcgState), hipMemcpyDeviceToHost, st));
     HIPCHK(hipEventRecord(ctx->ev[slot], st));
-    inflight.push_back(slot);
+    inflight[nflight++] = slot;
     ++window;
     if (std::chrono::duration<double>(Clock::now() - t0).count() > timeout_s)
       throw Error(GSH_E_TIMEOUT, "SSOR PCG: time limit exceeded while issuing iterations");
@@ -790,7 +806,7 @@ static void run_ssor_batch(gsh_solver* const* S, int count, const std::vector<do
   HIPCHK(hipGetLastError());
   PcgState* slots = ctx->pinned_slots(count);
   int issued = 0, window = 0;
-  std::deque<int> inflight;
+  int inflight[2], nflight = 0;  // window slots pending readback, oldest first
   bool all_done = false;
   const auto t0 = Clock::now();
   auto check_slot = [&](int slot) {
@@ -806,16 +822,17 @@ static void run_ssor_batch(gsh_solver* const* S, int count, const std::vector<do
     launch_ssor_check_batch(dsys, count, st);
     issued += w;
     const int slot = window & 1;
-    if (inflight.size() == 2) {
-      const int old = inflight.front();
-      inflight.pop_front();
+    if (nflight == 2) {
+      const int old = inflight[0];
+      inflight[0] = inflight[1];
+      nflight = 1;
       if (check_slot(old)) all_done = true;
     }
     launch_ssor_gather_states(dsys, count, dgather + slot * count, st);
     HIPCHK(hipMemcpyAsync(slots + slot * count, dgather + slot * count, sizeof(PcgState) * count,
                           hipMemcpyDeviceToHost, st));
     HIPCHK(hipEventRecord(ctx->ev[slot], st));
-    inflight.push_back(slot);
+    inflight[nflight++] = slot;
     ++window;
     if (std::chrono::duration<double>(Clock::now() - t0).count() > timeout_s)
       throw Error(GSH_E_TIMEOUT, "batched SSOR PCG: time limit exceeded while issuing iterations");
@@ -968,6 +985,79 @@ static void solve_mixed(gsh_solver* s, double rtol, int maxiter, int check_every
 }
 
 // ====================================================================== C ABI
+// ====================================================================== solve without allocation
+// gsh_pcg_solve_static keeps every buffer of its solve beyond the caller's inputs and output in
+// one file-scope device array, `scratch`, laid out as below for up to GSH_STATIC_MAX_ROWS rows
+// (about 53 MB, held by every device the library runs on until the process ends). Host code
+// never passes `scratch` itself to the device: gsh_ctx_create looks up its address on each
+// device once (hipGetSymbolAddress) and the entry point uses that address.
+namespace gsh {
+constexpr int kStaticRpt = 4;  // rows per thread of a fresh Jacobi solver (gsh_solver.rpt)
+constexpr int64_t kStaticTiles = ((int64_t)GSH_STATIC_MAX_ROWS + kTpb - 1) / kTpb;
+constexpr size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+constexpr size_t kScrSys = 0;                                                      // PcgSys<double>
+constexpr size_t kScrState = kScrSys + align256(sizeof(PcgSys<double>));            // PcgState
+constexpr size_t kScrGather = kScrState + align256(sizeof(PcgState));               // 2 PcgState
+constexpr size_t kScrFlag = kScrGather + align256(2 * sizeof(PcgState));            // int32 check flag
+constexpr size_t kScrTiles = kScrFlag + 256;                                        // TileRef per tile
+constexpr size_t kScrPart = kScrTiles + align256(kStaticTiles * sizeof(TileRef));   // 4 x double per tile
+constexpr size_t kScrDiag = kScrPart + 4 * align256(kStaticTiles * sizeof(double));  // int32 per row
+constexpr size_t kScrVec = kScrDiag + align256((size_t)GSH_STATIC_MAX_ROWS * 4);     // r, z, pa, pb, q, invd
+constexpr size_t kScrVecBytes = align256((size_t)GSH_STATIC_MAX_ROWS * sizeof(double));
+constexpr size_t kScratchBytes = kScrVec + 6 * kScrVecBytes;
+constexpr int kStaticMaxDevices = 64;
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+static std::mutex g_scratch_mu;
+static char* g_scratch_dev[kStaticMaxDevices] = {};  // address of `scratch` on each device
+
+// With `device` current: finds the device address of `scratch` once per device.
+static void static_scratch_setup(int device) {
+  if (device < 0 || device >= kStaticMaxDevices) return;  // gsh_pcg_solve_static then refuses
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_dev[device]) return;
+  void* p = nullptr;
+  HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  if (!p) throw Error(GSH_E_DEVICE, "no device address for the static scratch array");
+  g_scratch_dev[device] = static_cast<char*>(p);
+}
+
+__global__ void k_static_clear(int32_t* __restrict__ flag) { *flag = 0; }
+
+// The caller's CSR, read on the device: flag bits 1 (indptr is not a non-decreasing sequence from
+// 0 to nnz), 2 (a column outside 0..n-1), 4 (a row without a diagonal entry); diag[i] = index of
+// row i's diagonal entry (the last one if repeated, as the solver's own set-up). Also writes the
+// single-system tile table.
+__global__ void k_static_csr(const int32_t* __restrict__ indptr, const int32_t* __restrict__ indices, int32_t n,
+                             int64_t nnz, int32_t* __restrict__ diag, int32_t* __restrict__ flag,
+                             TileRef* __restrict__ tiles, int32_t ntiles) {
+  const int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < ntiles) tiles[i] = {0, i};
+  if (i >= n) return;
+  const int64_t a = indptr[i], b = indptr[i + 1];
+  int f = 0;
+  if ((i == 0 && a != 0) || (i == n - 1 && b != nnz)) f |= 1;
+  int32_t d = -1;
+  if (a < 0 || b < a || b > nnz) {
+    f |= 1;
+  } else {
+    for (int64_t k = a; k < b; ++k) {
+      const int32_t c = indices[k];
+      if (c < 0 || c >= n)
+        f |= 2;
+      else if (c == i)
+        d = (int32_t)k;
+    }
+    if (d < 0) f |= 4;
+  }
+  diag[i] = d;
+  if (f) atomicOr(flag, f);
+}
+
+__global__ void k_static_desc(PcgSys<double> v, PcgSys<double>* __restrict__ out) { *out = v; }
+}  // namespace gsh
+
 extern "C" {
 
 const char* gsh_last_error(void) { return g_last_error.c_str(); }
@@ -985,6 +1075,11 @@ int gsh_ctx_create(int device, gsh_ctx** out) {
     HIPCHK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     HIPCHK(hipEventCreateWithFlags(&c->ev[0], hipEventDisableTiming));
     HIPCHK(hipEventCreateWithFlags(&c->ev[1], hipEventDisableTiming));
+    // readback memory of the solve that allocates nothing (gsh_pcg_solve_static), and the
+    // device address of its scratch array on this device
+    c->pinned_slots(1);
+    HIPCHK(hipHostMalloc((void**)&c->pinned_flag, 256, hipHostMallocDefault));
+    static_scratch_setup(device);
     *out = c.release();
   })
 }
@@ -993,6 +1088,7 @@ static void ctx_free(gsh_ctx* ctx) {
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
   if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+  if (ctx->pinned_flag) (void)hipHostFree(ctx->pinned_flag);
   for (auto& e : ctx->ev)
     if (e) (void)hipEventDestroy(e);
   (void)hipStreamDestroy(ctx->stream);
@@ -1842,6 +1938,78 @@ int gsh_solver_solve(gsh_solver* s, const double* b_host, double* x_host, double
     s->last.ms = std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
     if (iters_out) *iters_out = iters;
     if (status_out) *status_out = status;
+  })
+}
+
+int gsh_pcg_solve_static(gsh_ctx* ctx, int32_t n, int64_t nnz, const int32_t* indptr, const int32_t* indices,
+                         const double* data, const double* b, double* x, double rtol, int maxiter, int check_every,
+                         double timeout_s, int* iters_out, int* status_out) {
+  GSH_API_BODY({
+    GSH_REQUIRE(ctx && indptr && indices && data && b && x, GSH_E_ARG, "NULL argument");
+    GSH_REQUIRE(n >= 1 && nnz >= 1, GSH_E_ARG, "n and nnz must be positive");
+    GSH_REQUIRE(n <= GSH_STATIC_MAX_ROWS, GSH_E_ARG,
+                "n = " + std::to_string(n) + " exceeds the static workspace (GSH_STATIC_MAX_ROWS = " +
+                    std::to_string(GSH_STATIC_MAX_ROWS) + " rows)");
+    GSH_REQUIRE(nnz < (int64_t)1 << 31, GSH_E_ARG, "nnz exceeds int32 indexing");
+    GSH_REQUIRE(rtol >= 0.0, GSH_E_ARG, "rtol must be >= 0");
+    GSH_REQUIRE(ctx->device >= 0 && ctx->device < kStaticMaxDevices && g_scratch_dev[ctx->device], GSH_E_STATE,
+                "no static workspace on this device");
+    GSH_REQUIRE(ctx->pinned && ctx->pinned_cap >= 2 && ctx->pinned_flag, GSH_E_STATE, "context without readback memory");
+    if (maxiter <= 0) maxiter = 10 * n;  // pcg_solver.py:1082-1083
+    if (timeout_s <= 0.0) timeout_s = 60.0;
+    ctx->set();
+    hipStream_t st = ctx->stream;
+    char* base = g_scratch_dev[ctx->device];
+    const int ntiles = (n + kTpb * kStaticRpt - 1) / (kTpb * kStaticRpt);
+    PcgSys<double>* dsys = reinterpret_cast<PcgSys<double>*>(base + kScrSys);
+    PcgState* dgather = reinterpret_cast<PcgState*>(base + kScrGather);
+    int32_t* dflag = reinterpret_cast<int32_t*>(base + kScrFlag);
+    TileRef* dtiles = reinterpret_cast<TileRef*>(base + kScrTiles);
+    int32_t* ddiag = reinterpret_cast<int32_t*>(base + kScrDiag);
+    // the pattern check and the diagonal positions, read back before any row is used
+    hipLaunchKernelGGL(k_static_clear, dim3(1), dim3(1), 0, st, dflag);
+    hipLaunchKernelGGL(k_static_csr, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, indptr, indices, n, nnz, ddiag,
+                       dflag, dtiles, ntiles);
+    HIPCHK(hipMemcpyAsync(ctx->pinned_flag, dflag, sizeof(int32_t), hipMemcpyDeviceToHost, st));
+    HIPCHK(hipEventRecord(ctx->ev[0], st));
+    wait_event(ctx->ev[0], timeout_s, "CSR check");
+    const int32_t f = *ctx->pinned_flag;
+    GSH_REQUIRE(!(f & 1), GSH_E_ARG, "indptr must start at 0, be non-decreasing and end at nnz");
+    GSH_REQUIRE(!(f & 2), GSH_E_ARG, "column out of range");
+    GSH_REQUIRE(!(f & 4), GSH_E_ARG, "a CSR row has no diagonal entry");
+    PcgSys<double> h{};
+    h.n = n;
+    h.ntiles = ntiles;
+    h.rowptr = indptr;
+    h.col = indices;
+    h.val = data;
+    h.diag = ddiag;
+    h.cdel = nullptr;
+    h.b = b;
+    h.x = x;
+    double* v = reinterpret_cast<double*>(base + kScrVec);
+    const size_t vs = kScrVecBytes / sizeof(double);
+    h.r = v;
+    h.z = v + vs;
+    h.pa = v + 2 * vs;
+    h.pb = v + 3 * vs;
+    h.q = v + 4 * vs;
+    h.invd = v + 5 * vs;
+    h.h0 = h.h1 = h.h2 = nullptr;  // cold start: zero initial guess
+    double* part = reinterpret_cast<double*>(base + kScrPart);
+    const size_t ps = align256(kStaticTiles * sizeof(double)) / sizeof(double);
+    h.part_rr = part;
+    h.part_rz = part + ps;
+    h.part_pq = part + 2 * ps;
+    h.part_bb = part + 3 * ps;
+    h.st = reinterpret_cast<PcgState*>(base + kScrState);
+    hipLaunchKernelGGL(k_static_desc, dim3(1), dim3(1), 0, st, h, dsys);
+    HIPCHK(hipGetLastError());
+    run_pcg_core<double>(ctx, dsys, 1, dtiles, ntiles, kStaticRpt, rtol, maxiter, check_every, timeout_s, dgather,
+                         nullptr, -1, nullptr, nullptr, nullptr);
+    const PcgState r = ctx->pinned[0];
+    if (iters_out) *iters_out = iters_of(r, maxiter);
+    if (status_out) *status_out = (r.done >= 0) ? 0 : GSH_W_MAXITER;
   })
 }
 

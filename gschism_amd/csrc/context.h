@@ -49,6 +49,7 @@ struct gsh_ctx {
   std::atomic<bool> closed{false};
   gsh::PcgState* pinned = nullptr;   // host-visible readback slots
   int pinned_cap = 0;
+  int32_t* pinned_flag = nullptr;    // host-visible word for small device-to-host readbacks
   hipEvent_t ev[2] = {nullptr, nullptr};
   void set() const { HIPCHK(hipSetDevice(device)); }
   gsh::PcgState* pinned_slots(int per_slot) {

@@ -284,6 +284,20 @@ int gsh_assemble(gsh_mesh* mesh, gsh_solver* s, const gsh_step_params* p, double
  * timeout_s <= 0 means 60 s. */
 int gsh_solver_solve(gsh_solver* s, const double* b_host, double* x_host, double rtol,
                      int maxiter, int check_every, double timeout_s, int* iters_out, int* status_out);
+/* No counterpart in the reference: a solve that allocates nothing, for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc). It gives the same iteration count, status and solution, bit for bit, as
+ * gsh_solver_solve of an FP64 Jacobi solver made by gsh_solver_create_csr on this pattern with
+ * its history cleared (zero initial guess) and these values and b. indptr (n+1), indices (nnz),
+ * data (nnz), b (n) and x (n, the output) are DEVICE pointers on ctx's device; the pattern is
+ * checked on the device (GSH_E_ARG). Every other buffer of the solve lives in one static device
+ * array of the library (about 53 MB per device, held until the process ends) sized for
+ * n <= GSH_STATIC_MAX_ROWS (GSH_E_ARG beyond). All calls on one device share that array: calls
+ * that overlap on one device (several contexts or threads) race; consecutive calls do not. */
+#define GSH_STATIC_MAX_ROWS 1048576
+int gsh_pcg_solve_static(gsh_ctx* ctx, int32_t n, int64_t nnz, const int32_t* indptr_dev, const int32_t* indices_dev,
+                         const double* data_dev, const double* b_dev, double* x_dev, double rtol, int maxiter,
+                         int check_every, double timeout_s, int* iters_out, int* status_out);
 /* checkSolution (pcg_solver.py:1127-1143): ||b - A x|| / ||b||. */
 int gsh_solver_check(gsh_solver* s, double* rel_out);
 /* Clears the extrapolation history x0=x1=x2=0 (cold start, main.py:352). */
