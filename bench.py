@@ -40,6 +40,23 @@ def cpu_model() -> str:
     return "unknown"
 
 
+def counter_traffic(kernel: str, rows_per_thread: int):
+    """HBM bytes per launch of `kernel` at this rows-per-thread setting, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload (profiles/), or None. The summary
+    holds one line 'traffic <kernel> rows_per_thread=<R> bytes_per_launch=<B>' per setting,
+    already scaled by the copy-kernel factors it states."""
+    path = os.path.join(ROOT, "profiles", "ssor_counters_channel-1024x1024.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                w = line.strip().strip("`").split()
+                if len(w) == 4 and w[0] == "traffic" and w[1] == kernel and w[2] == f"rows_per_thread={rows_per_thread}":
+                    return int(w[3].split("=", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
 def timed_solves(ctx, solver, steps, rtol, maxiter, check_every):
     iters = []
     ctx.synchronize()
@@ -169,9 +186,10 @@ def batched_leg(ctx, L, cases, core, members, nx, ny, rtol, mixed_rtol, check_ev
 
 def mixed_4m_leg(ctx, L, cases, core, steps, rtol, outer_rtol, check_every):
     """Configuration 3: the 2048 x 2048 log-normal channel (4.2M nodes). FP64 PCG (multicolour
-    SSOR and the reference's Jacobi) to rtol 1e-12 against mixed precision (FP32 inner SSOR or
-    Jacobi PCG, FP64 dot products, true residual and refinement) to the outer tolerance; relative
-    L2 error against the FP64 SSOR solution and iteration counts."""
+    SSOR and the reference's Jacobi) to rtol 1e-12, FP64 SSOR also to the outer tolerance, against
+    mixed precision (FP32 inner SSOR or Jacobi PCG, FP64 dot products, true residual and
+    refinement) to the outer tolerance; relative L2 error against the FP64 SSOR solution (rtol
+    1e-12) and iteration counts, and mixed SSOR over FP64 SSOR at the same tolerance."""
     spec = cases.spec_mixed_4m()
     data = cases.make_case(spec)
     mesh, p, _ = cases.load_mesh(ctx, data)
@@ -181,6 +199,7 @@ def mixed_4m_leg(ctx, L, cases, core, steps, rtol, outer_rtol, check_every):
     ref = None
     for name, prec, pc, tol in (("fp64_ssor", L.PREC_FP64, L.PRECOND_SSOR, rtol),
                                 ("fp64_jacobi", L.PREC_FP64, L.PRECOND_JACOBI, rtol),
+                                ("fp64_ssor_outer_rtol", L.PREC_FP64, L.PRECOND_SSOR, outer_rtol),
                                 ("mixed_ssor", L.PREC_MIXED, L.PRECOND_SSOR, outer_rtol),
                                 ("mixed_jacobi", L.PREC_MIXED, L.PRECOND_JACOBI, outer_rtol)):
         s = core.Solver(mesh, prec, precond=pc)
@@ -205,6 +224,10 @@ def mixed_4m_leg(ctx, L, cases, core, steps, rtol, outer_rtol, check_every):
             rec["rel_l2_vs_fp64_ssor"] = float(np.linalg.norm(x - ref) / np.linalg.norm(ref))
         out[name] = rec
         s.close()
+    # mixed precision against FP64 SSOR stopped at the same (outer) tolerance
+    if out.get("fp64_ssor_outer_rtol", {}).get("solves_per_s"):
+        out["mixed_ssor_vs_fp64_ssor_same_rtol"] = round(
+            out["mixed_ssor"]["solves_per_s"] / out["fp64_ssor_outer_rtol"]["solves_per_s"], 3)
     mesh.close()
     return out
 
@@ -423,7 +446,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--nx", type=int, default=1024)
     ap.add_argument("--ny", type=int, default=1024)
-    ap.add_argument("--ssor-rows-per-thread", type=int, default=2)
+    ap.add_argument("--ssor-rows-per-thread", type=int, default=1)
     ap.add_argument("--jacobi-rows-per-thread", type=int, default=4)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-mixed", action="store_true")
@@ -696,7 +719,8 @@ def main() -> int:
                    "rows_per_thread": args.ssor_rows_per_thread, "dt": p.dt, "theta": p.theta,
                    "true_rel_residual": rel_ssor},
         "roofline": {"bound": "hbm", "kernel": d["kernel"], "achieved": d["gbs"], "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": round(d["gbs"] / PEAK_HBM_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(d["gbs"] / PEAK_HBM_GBS, 4),
+                     "traffic": counter_traffic(d["kernel"], args.ssor_rows_per_thread),
                      "bytes_per_launch": round(d["bytes_per_iter"] / d["launches_per_iter"]),
                      "us_per_launch": round(us_launch, 2)},
         "ssor_kernels": per,
