@@ -48,11 +48,18 @@ class GshError(RuntimeError):
         self.code = code
 
 
+class VGrid(C.Structure):
+    _fields_ = [("ivcor", C.c_int32), ("sigma_node", C.POINTER(C.c_double)), ("nz", C.c_int32),
+                ("h_s", C.c_double), ("h_c", C.c_double), ("theta_b", C.c_double), ("theta_f", C.c_double),
+                ("ztot", C.POINTER(C.c_double)), ("sigma_s", C.POINTER(C.c_double))]
+
+
 class MeshDesc(C.Structure):
     _fields_ = [("n", C.c_int32), ("ne", C.c_int32), ("nvrt", C.c_int32),
                 ("xy", C.POINTER(C.c_double)), ("elnode", C.POINTER(C.c_int32)),
                 ("depth", C.POINTER(C.c_double)), ("open_bnd", C.POINTER(C.c_int8)),
-                ("sigma", C.POINTER(C.c_double)), ("kb", C.POINTER(C.c_int32)), ("renumber", C.c_int32)]
+                ("sigma", C.POINTER(C.c_double)), ("kb", C.POINTER(C.c_int32)), ("renumber", C.c_int32),
+                ("vgrid", C.POINTER(VGrid))]
 
 
 class MeshInfo(C.Structure):
@@ -117,6 +124,7 @@ SIGNATURES = {
     "gsh_grid_info": (_I, [_P, _PI32, _PI32, _PI32, _PI32, _PI32]),
     "gsh_grid_get": (_I, [_P, _PD, _PD, _PI32, C.POINTER(C.c_int8), _PI32, _PD]),
     "gsh_grid_segment": (_I, [_P, _I, _I32, _PI32, _I32, _PI32, _PI32]),
+    "gsh_grid_vgrid": (_I, [_P, _PI32, _PI32, _PI32, _PD, _PD, _PD, _PD]),
     "gsh_grid_destroy": (_I, [_P]),
     "gsh_solver_create": (_I, [_P, _I, _I, _PP]),
     "gsh_solver_create_csr": (_I, [_P, _I32, _I64, _PI32, _PI32, _I, _I, _PP]),

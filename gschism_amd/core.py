@@ -76,6 +76,46 @@ _FIELD_SPEC = {
 }
 
 
+class VGrid:
+    """Vertical grid beyond one sigma per level (gsh_vgrid; readVGrid,
+    read_and_process_grid.cpp:1313-1520).
+
+    ivcor=1: `sigma_node` [n][nvrt], a sigma per node and level (LSC2), in the node order given to
+    Mesh; each node's bottom level comes from Mesh's kb.
+    ivcor=2 (SZ): `nz` Z levels `ztot` (the last one at -h_s), S levels `sigma_s` (-1 .. 0,
+    nvrt - nz + 1 of them) and h_s, h_c, theta_b, theta_f; every node's kb follows from its depth.
+    """
+
+    def __init__(self, ivcor: int, sigma_node=None, nz: int = 0, h_s: float = 0.0, h_c: float = 0.0,
+                 theta_b: float = 0.0, theta_f: float = 0.0, ztot=None, sigma_s=None):
+        self.ivcor = int(ivcor)
+        self.sigma_node = None if sigma_node is None else np.ascontiguousarray(sigma_node, dtype=np.float64)
+        self.nz = int(nz)
+        self.h_s, self.h_c, self.theta_b, self.theta_f = float(h_s), float(h_c), float(theta_b), float(theta_f)
+        self.ztot = None if ztot is None else np.ascontiguousarray(ztot, dtype=np.float64)
+        self.sigma_s = None if sigma_s is None else np.ascontiguousarray(sigma_s, dtype=np.float64)
+
+    @property
+    def nvrt(self) -> int:
+        if self.ivcor == 1:
+            return int(self.sigma_node.shape[1])
+        return self.nz - 1 + int(self.sigma_s.size)
+
+    def permuted(self, node_perm) -> "VGrid":
+        """The same grid in the node order node_perm[new] = old (per-node sigma rows follow)."""
+        sn = None if self.sigma_node is None else self.sigma_node[np.asarray(node_perm)]
+        return VGrid(self.ivcor, sn, self.nz, self.h_s, self.h_c, self.theta_b, self.theta_f, self.ztot, self.sigma_s)
+
+    def c_struct(self) -> L.VGrid:
+        v = L.VGrid()
+        v.ivcor, v.nz = self.ivcor, self.nz
+        v.h_s, v.h_c, v.theta_b, v.theta_f = self.h_s, self.h_c, self.theta_b, self.theta_f
+        v.sigma_node = dptr(self.sigma_node) if self.sigma_node is not None else None
+        v.ztot = dptr(self.ztot) if self.ztot is not None else None
+        v.sigma_s = dptr(self.sigma_s) if self.sigma_s is not None else None
+        return v
+
+
 class Mesh:
     """A triangular mesh with its device-resident state (gsh_mesh).
 
@@ -84,23 +124,27 @@ class Mesh:
     ctx=None builds a host-only mesh (topology and geometry, no device state).
     """
 
-    def __init__(self, ctx: Context | None, xy, elnode, depth, sigma, open_bnd=None, kb=None, renumber=True):
+    def __init__(self, ctx: Context | None, xy, elnode, depth, sigma, open_bnd=None, kb=None, renumber=True,
+                 vgrid: VGrid | None = None):
         xy = np.ascontiguousarray(xy, dtype=np.float64).reshape(-1, 2)
         elnode = np.ascontiguousarray(elnode, dtype=np.int32).reshape(-1, 3)
         depth = np.ascontiguousarray(depth, dtype=np.float64)
-        sigma = np.ascontiguousarray(sigma, dtype=np.float64)
+        sigma = None if sigma is None else np.ascontiguousarray(sigma, dtype=np.float64)
         n, ne = xy.shape[0], elnode.shape[0]
         ob = None if open_bnd is None else np.ascontiguousarray(open_bnd, dtype=np.int8)
         kbv = None if kb is None else np.ascontiguousarray(kb, dtype=np.int32)
         d = L.MeshDesc()
-        d.n, d.ne, d.nvrt = n, ne, sigma.size
+        d.n, d.ne = n, ne
+        d.nvrt = vgrid.nvrt if vgrid is not None else sigma.size
         d.xy = dptr(xy)
         d.elnode = dptr(elnode, C.c_int32)
         d.depth = dptr(depth)
         d.open_bnd = ob.ctypes.data_as(C.POINTER(C.c_int8)) if ob is not None else None
-        d.sigma = dptr(sigma)
+        d.sigma = dptr(sigma) if sigma is not None else None
         d.kb = dptr(kbv, C.c_int32) if kbv is not None else None
         d.renumber = 1 if renumber else 0
+        self._vg_struct = vgrid.c_struct() if vgrid is not None else None
+        d.vgrid = C.pointer(self._vg_struct) if vgrid is not None else None
         self.node_perm = np.empty(n, dtype=np.int32)
         self.elem_perm = np.empty(ne, dtype=np.int32)
         h = C.c_void_p()
@@ -113,6 +157,8 @@ class Mesh:
         check(L.lib().gsh_mesh_get_info(self.h, C.byref(info)))
         self.n, self.ne, self.ns, self.nvrt = info.n, info.ne, info.ns, info.nvrt
         self.nnz, self.max_row, self.n_open = info.nnz, info.max_row, info.n_open
+        # the vertical grid in the mesh's own node order (for the oracle of the tests)
+        self.vgrid = vgrid.permuted(self.node_perm) if vgrid is not None else None
 
     def close(self):
         if getattr(self, "h", None):

@@ -1188,7 +1188,8 @@ int gsh_mesh_create(gsh_ctx* ctx, const gsh_mesh_desc* desc, gsh_mesh** out, int
     GSH_REQUIRE(desc && out, GSH_E_ARG, "NULL argument");
     GSH_REQUIRE(desc->n >= 3 && desc->ne >= 1, GSH_E_ARG, "mesh needs at least 3 nodes and 1 element");
     GSH_REQUIRE(desc->nvrt >= 2 && desc->nvrt <= 1024, GSH_E_ARG, "nvrt must be in [2, 1024]");
-    GSH_REQUIRE(desc->xy && desc->elnode && desc->depth && desc->sigma, GSH_E_ARG, "xy, elnode, depth, sigma required");
+    GSH_REQUIRE(desc->xy && desc->elnode && desc->depth && (desc->sigma || desc->vgrid), GSH_E_ARG,
+                "xy, elnode, depth and sigma (or a vertical grid) required");
     if (ctx) ctx->set();
     auto m = std::make_unique<gsh_mesh>();
     m->ctx = ctx;
@@ -1200,7 +1201,10 @@ int gsh_mesh_create(gsh_ctx* ctx, const gsh_mesh_desc* desc, gsh_mesh** out, int
     hm.y.resize(hm.n);
     for (int32_t i = 0; i < hm.n; ++i) hm.x[i] = desc->xy[2 * i], hm.y[i] = desc->xy[2 * i + 1];
     hm.depth.assign(desc->depth, desc->depth + hm.n);
-    hm.sigma.assign(desc->sigma, desc->sigma + hm.nvrt);
+    if (desc->sigma)
+      hm.sigma.assign(desc->sigma, desc->sigma + hm.nvrt);
+    else
+      hm.sigma.assign(hm.nvrt, 0.0);  // unused: the vertical grid gives the levels
     hm.elnode.assign(desc->elnode, desc->elnode + 3 * (size_t)hm.ne);
     hm.open_bnd.assign(hm.n, 0);
     if (desc->open_bnd)
@@ -1209,7 +1213,12 @@ int gsh_mesh_create(gsh_ctx* ctx, const gsh_mesh_desc* desc, gsh_mesh** out, int
     if (desc->kb) hm.kb.assign(desc->kb, desc->kb + hm.n);
     for (int32_t i = 0; i < hm.n; ++i)
       GSH_REQUIRE(hm.kb[i] >= 0 && hm.kb[i] <= hm.nvrt - 2, GSH_E_ARG, "kb must be in [0, nvrt-2]");
-    for (int k = 1; k < hm.nvrt; ++k) GSH_REQUIRE(hm.sigma[k] > hm.sigma[k - 1], GSH_E_ARG, "sigma must increase");
+    if (desc->vgrid) {
+      const gsh_vgrid& v = *desc->vgrid;
+      setup_vertical(hm, VGridIn{v.ivcor, v.sigma_node, v.nz, v.h_s, v.h_c, v.theta_b, v.theta_f, v.ztot, v.sigma_s});
+    } else {
+      for (int k = 1; k < hm.nvrt; ++k) GSH_REQUIRE(hm.sigma[k] > hm.sigma[k - 1], GSH_E_ARG, "sigma must increase");
+    }
     std::vector<int32_t> nperm(hm.n), eperm(hm.ne);
     for (int32_t i = 0; i < hm.n; ++i) nperm[i] = i;
     for (int32_t i = 0; i < hm.ne; ++i) eperm[i] = i;
@@ -1267,6 +1276,16 @@ int gsh_mesh_create(gsh_ctx* ctx, const gsh_mesh_desc* desc, gsh_mesh** out, int
     d.depth = P.upload(hm.depth);
     d.kb = P.upload(hm.kb);
     d.sigma = P.upload(hm.sigma);
+    d.vmode = hm.vmode;
+    if (hm.vmode == 1) d.sig_node = P.upload(hm.sig_node);
+    if (hm.vmode == 2) {
+      d.ztot = P.upload(hm.ztot);
+      d.ssig = P.upload(hm.ssig);
+      d.cs = P.upload(hm.cs);
+      d.izlev = hm.nz - 1;
+      d.h_s = hm.h_s;
+      d.h_c = hm.h_c;
+    }
     d.open_bnd = P.upload(hm.open_bnd);
     d.node_bside = reinterpret_cast<const int2*>(P.upload(t.node_bside));
     d.eta2 = P.alloc<double>(n);
@@ -1435,6 +1454,15 @@ int gsh_mesh_set_field(gsh_mesh* m, int field, const void* host, int64_t count) 
       for (int64_t i = 0; i < count; ++i)
         GSH_REQUIRE(t[i] >= -1 && t[i] <= 5, GSH_E_ARG, "elevation BC types must be in [-1, 5]");
     }
+    std::vector<int32_t> sz_kb;
+    if (field == GSH_F_DEPTH && m->hm.vmode == 2) {
+      // SZ: the bottom levels follow the depth (Node::updateKb); a depth below the deepest Z
+      // level is refused before anything changes
+      HostMesh tmp = m->hm;
+      tmp.depth.assign(static_cast<const double*>(host), static_cast<const double*>(host) + count);
+      sz_bottom_levels(tmp);
+      sz_kb = std::move(tmp.kb);
+    }
     hipStream_t st = m->ctx->stream;
     if (field == GSH_F_NODE_UVW) {
       StreamBuffers bufs(st);
@@ -1447,6 +1475,11 @@ int gsh_mesh_set_field(gsh_mesh* m, int field, const void* host, int64_t count) 
     }
     if (field == GSH_F_DEPTH) {  // the host copy answers GSH_T_DEPTH
       const double* h = static_cast<const double*>(host);
+      if (m->hm.vmode == 2) {
+        m->hm.kb = sz_kb;
+        HIPCHK(hipMemcpyAsync(const_cast<int32_t*>(m->d.kb), m->hm.kb.data(), sizeof(int32_t) * m->hm.n,
+                              hipMemcpyHostToDevice, st));
+      }
       m->hm.depth.assign(h, h + count);
     }
     HIPCHK(hipStreamSynchronize(st));

@@ -20,7 +20,16 @@ struct DMesh {
   const double* y;
   const double* depth;
   const int32_t* kb;
-  const double* sigma;      // [nvrt]
+  const double* sigma;      // [nvrt] (vmode 0)
+  // vertical coordinates of Node::compZCoord (meshdata.cu:1744-1807): vmode 0 = ivcor 1 with one
+  // sigma per level, 1 = ivcor 1 with a sigma per node and level, 2 = ivcor 2 (SZ)
+  int32_t vmode;
+  int32_t izlev;            // nz - 1: the level at -min(h, h_s) (vmode 2)
+  double h_s, h_c;          // (vmode 2)
+  const double* sig_node;   // [n][nvrt] (vmode 1)
+  const double* ztot;       // [nz] Z levels (vmode 2)
+  const double* ssig;       // [nvrt - nz + 1] S levels (vmode 2)
+  const double* cs;         // [nvrt - nz + 1] C(sigma) of the S levels (vmode 2)
   const int8_t* open_bnd;
   const int2* node_bside;   // [n] boundary sides ending at the node, ascending side index, -1 = none
   double* eta2;

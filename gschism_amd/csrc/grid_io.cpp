@@ -1,7 +1,7 @@
 // grid_io.cpp -- SCHISM grid files on the host: hgrid.gr3 (nodes, triangles, open and land
-// boundary segments) and vgrid.in with ivcor=1, parsed with the rules of the reference's readers
-// (readHGrid, read_and_process_grid.cpp:855-1070; readVGrid, :1322-1385) into the arrays that
-// gsh_mesh_create takes.
+// boundary segments) and vgrid.in (ivcor=1 with one or per-node sigma columns, ivcor=2 SZ),
+// parsed with the rules of the reference's readers (readHGrid, read_and_process_grid.cpp:855-1070;
+// readVGrid, :1313-1520) into the arrays that gsh_mesh_create takes.
 #include <algorithm>
 #include <cerrno>
 #include <cstdio>
@@ -21,8 +21,13 @@ struct gsh_grid {
   std::vector<std::vector<int32_t>> open_seg;      // 0-based node ids per open segment
   std::vector<std::vector<int32_t>> land_seg;      // 0-based node ids per land segment
   std::vector<int8_t> land_island;                 // island flag of each land segment
-  std::vector<int32_t> kb;                         // vgrid: bottom level (0-based) per node
-  std::vector<double> sigma;                       // vgrid: one sigma per level
+  std::vector<int32_t> kb;                         // vgrid: bottom level (0-based) per node (ivcor 1)
+  std::vector<double> sigma;                       // vgrid: one sigma per level (ivcor 1, nodes agree)
+  int32_t ivcor = 0, per_node = 0;                 // vgrid: 1 or 2; ivcor 1 with differing node sigmas
+  std::vector<double> sigma_node;                  // vgrid: [n][nvrt] sigma of every node (ivcor 1)
+  int32_t nz = 0;                                  // vgrid, ivcor 2: kz (Z levels incl. -h_s)
+  double h_s = 0, h_c = 0, theta_b = 0, theta_f = 0;
+  std::vector<double> ztot, sigma_s;               // vgrid, ivcor 2: [nz], [nvrt - nz + 1]
 };
 
 namespace gsh {
@@ -167,16 +172,74 @@ void read_hgrid(gsh_grid& g, const std::string& path) {
     r.fail("nvel (" + std::to_string(nvel) + ") != total # of land boundary nodes (" + std::to_string(total) + ")");
 }
 
-// vgrid.in, ivcor=1 (read_and_process_grid.cpp:1337-1385): ivcor; nvrt; kbp of every node; then
-// nvrt rows 'k sigma(node 1) ... sigma(node np)'. The library holds one sigma per level, so the
-// file is accepted when the nodes agree on sigma at every level above their bottom level (a
-// node's own bottom level is its bed, z = -h, whatever its sigma there; level 0 is taken from the
-// nodes whose bottom it is).
+// vgrid.in (readVGrid, read_and_process_grid.cpp:1313-1520).
+//
+// ivcor=1 (:1341-1395): ivcor; nvrt; kbp of every node; then nvrt rows 'k sigma(node 1) ...
+// sigma(node np)'. Every node's column is kept; when the nodes agree on sigma at every level
+// above their bottom level (a node's own bottom level is its bed, z = -h, whatever its sigma
+// there) the grid also has one sigma per level.
+//
+// ivcor=2 (:1397-1503): 'nvrt kz h_s'; a comment line; the Z levels 'k z(k)' for k = 1..kz, of
+// which only 1..kz-1 are read (z(kz) is -h_s); a comment line; 'h_c theta_b theta_f'; the S levels
+// 'k sigma(k)' for k = 1..nvrt-kz+1, of which the first (-1) is not read; with readVGrid's checks.
+// readVGrid takes one line for the kz-th Z level only when kz = 1 (for kz > 1 its line skips
+// leave that line to be read as the S-levels comment); here every kz reads that format.
 void read_vgrid(gsh_grid& g, const std::string& path) {
   LineReader r(path);
   const int64_t ivcor = r.first_int("ivcor");
-  if (ivcor == 2) r.fail("ivcor=2 (SZ levels) is not supported");
-  if (ivcor != 1) r.fail("ivcor must be 1 or 2");
+  if (ivcor != 1 && ivcor != 2) r.fail("ivcor must be 1 or 2");
+  g.ivcor = (int32_t)ivcor;
+  if (ivcor == 2) {
+    const char *b, *e;
+    if (!r.next(b, e)) r.fail("end of file before 'nvrt kz h_s'");
+    const std::vector<double> head = line_numbers(b, e);
+    if (head.size() < 3) r.fail("expected 'nvrt kz h_s'");
+    const int64_t nvrt = (int64_t)head[0], kz = (int64_t)head[1];
+    const double h_s = head[2];
+    if (nvrt < 2 || nvrt > 1024) r.fail("nvrt must be in [2, 1024]");
+    if (kz < 1 || kz > nvrt - 1) r.fail("kz must be in [1, nvrt-1]");
+    if (h_s < 6.0) r.fail("h_s needs to be larger: " + std::to_string(h_s));
+    if (!r.next(b, e)) r.fail("end of file before the Z levels");  // "Z levels"
+    g.ztot.assign((size_t)kz, 0.0);
+    for (int64_t k = 0; k < kz; ++k) {
+      if (!r.next(b, e)) r.fail("end of file in the Z levels");
+      const std::vector<double> v = line_numbers(b, e);
+      if (v.size() < 2 || (int64_t)v[0] != k + 1) r.fail("invalid level number in the Z levels");
+      if (k == kz - 1) break;  // z(kz) is -h_s
+      g.ztot[k] = v[1];
+      if (g.ztot[k] >= -h_s) r.fail("illegal Z level " + std::to_string(k + 1));
+      if (k > 0 && g.ztot[k] <= g.ztot[k - 1]) r.fail("Z levels inverted at " + std::to_string(k + 1));
+    }
+    g.ztot[kz - 1] = -h_s;
+    if (!r.next(b, e)) r.fail("end of file before the S levels");  // "S levels"
+    if (!r.next(b, e)) r.fail("end of file before 'h_c theta_b theta_f'");
+    const std::vector<double> sp = line_numbers(b, e);
+    if (sp.size() < 3) r.fail("expected 'h_c theta_b theta_f'");
+    const double h_c = sp[0], theta_b = sp[1], theta_f = sp[2];
+    if (h_c < 5.0 || h_c >= h_s) r.fail("h_c needs to be larger to avoid 2nd type abnormality");
+    if (theta_b < 0.0 || theta_b > 1.0) r.fail("wrong theta_b");
+    if (theta_f <= 0.0) r.fail("wrong theta_f");
+    const int64_t nsig = nvrt - kz + 1;
+    g.sigma_s.assign((size_t)nsig, 0.0);
+    g.sigma_s[0] = -1.0;
+    g.sigma_s[nsig - 1] = 0.0;
+    for (int64_t k = 0; k < nsig; ++k) {
+      if (!r.next(b, e)) r.fail("end of file in the S levels");
+      const std::vector<double> v = line_numbers(b, e);
+      if (v.size() < 2 || (int64_t)v[0] != k + 1) r.fail("invalid level number in the S levels");
+      if (k == 0) continue;  // sigma(1) = -1
+      g.sigma_s[k] = v[1];
+      if (g.sigma_s[k] <= g.sigma_s[k - 1] || g.sigma_s[k] > 0.0 || g.sigma_s[k] < -1.0)
+        r.fail("invalid sigma levels at " + std::to_string(k + 1));
+    }
+    g.nvrt = (int32_t)nvrt;
+    g.nz = (int32_t)kz;
+    g.h_s = h_s;
+    g.h_c = h_c;
+    g.theta_b = theta_b;
+    g.theta_f = theta_f;
+    return;
+  }
   const int64_t nvrt = r.first_int("nvrt");
   if (nvrt < 2 || nvrt > 1024) r.fail("nvrt must be in [2, 1024]");
   // the rest is a stream of numbers
@@ -190,7 +253,6 @@ void read_vgrid(gsh_grid& g, const std::string& path) {
   };
   g.nvrt = (int32_t)nvrt;
   g.kb.resize(g.n);
-  int32_t kmin = (int32_t)nvrt;
   for (int32_t i = 0; i < g.n; ++i) {
     const double v = num("kbp");
     const int32_t kb = (int32_t)v - 1;
@@ -198,28 +260,50 @@ void read_vgrid(gsh_grid& g, const std::string& path) {
       throw Error(GSH_E_IO, path + ": node " + std::to_string(i + 1) + " has kbp " + std::to_string((int)v) +
                                 " outside 1.." + std::to_string(nvrt - 1));
     g.kb[i] = kb;
-    kmin = std::min(kmin, kb);
   }
-  if (kmin != 0) throw Error(GSH_E_IO, path + ": no node uses the bottom level (every kbp > 1)");
+  g.sigma_node.assign((size_t)g.n * nvrt, 0.0);
   g.sigma.assign((size_t)nvrt, 0.0);
+  g.sigma[0] = -1.0;
+  bool shared = true;
   for (int32_t k = 0; k < nvrt; ++k) {
     const double row = num("level number");
     if ((int64_t)row != k + 1) throw Error(GSH_E_IO, path + ": level rows out of order at level " + std::to_string(k + 1));
     bool have = false;
     for (int32_t i = 0; i < g.n; ++i) {
       const double s = num("sigma");
+      g.sigma_node[(size_t)i * nvrt + k] = s;
       if (k == 0 ? g.kb[i] != 0 : k <= g.kb[i]) continue;  // at or below the node's bed
       if (!have) {
         g.sigma[k] = s;
         have = true;
       } else if (s != g.sigma[k]) {
-        throw Error(GSH_E_IO, path + ": nodes differ in sigma at level " + std::to_string(k + 1) +
-                                  " (per-node sigma levels are not supported)");
+        shared = false;
       }
     }
   }
-  for (int32_t k = 1; k < nvrt; ++k)
-    if (!(g.sigma[k] > g.sigma[k - 1])) throw Error(GSH_E_IO, path + ": sigma must increase with the level");
+  for (int32_t i = 0; i < g.n && shared; ++i)
+    for (int32_t k = std::max(1, g.kb[i] + 1); k < nvrt; ++k)
+      if (g.sigma_node[(size_t)i * nvrt + k] != g.sigma[k]) shared = false;
+  if (shared) {
+    // levels 1..min(kb) lie at or below every node's bed, so no node gives their sigma and z never
+    // reads it: spread them evenly between -1 and the first level some node uses
+    const int32_t m = *std::min_element(g.kb.begin(), g.kb.end());
+    for (int32_t k = 1; k <= m; ++k) g.sigma[k] = -1.0 + (g.sigma[m + 1] + 1.0) * k / (m + 1);
+  }
+  if (shared)
+    for (int32_t k = 1; k < nvrt; ++k)
+      if (!(g.sigma[k] > g.sigma[k - 1])) throw Error(GSH_E_IO, path + ": sigma must increase with the level");
+  g.per_node = shared ? 0 : 1;
+  if (shared) {
+    g.sigma_node.clear();
+  } else {
+    g.sigma.clear();
+    for (int32_t i = 0; i < g.n; ++i)
+      for (int32_t k = g.kb[i] + 2; k < nvrt; ++k)
+        if (!(g.sigma_node[(size_t)i * nvrt + k] > g.sigma_node[(size_t)i * nvrt + k - 1]))
+          throw Error(GSH_E_IO, path + ": sigma must increase with the level above the bed of node " +
+                                    std::to_string(i + 1));
+  }
 }
 
 }  // namespace
@@ -255,6 +339,9 @@ int gsh_grid_get(const gsh_grid* g, double* xy, double* depth, int32_t* elnode, 
   GSH_API_BODY({
     GSH_REQUIRE(g, GSH_E_ARG, "grid is NULL");
     GSH_REQUIRE(!(kb || sigma) || g->nvrt > 0, GSH_E_STATE, "no vgrid.in was read");
+    GSH_REQUIRE(!kb || g->ivcor == 1, GSH_E_STATE, "an SZ grid (ivcor 2) has no bottom levels in the file");
+    GSH_REQUIRE(!sigma || (g->ivcor == 1 && !g->per_node), GSH_E_STATE,
+                "the vertical grid has no single sigma column (gsh_grid_vgrid)");
     if (xy) std::memcpy(xy, g->xy.data(), 8 * g->xy.size());
     if (depth) std::memcpy(depth, g->depth.data(), 8 * g->depth.size());
     if (elnode) std::memcpy(elnode, g->elnode.data(), 4 * g->elnode.size());
@@ -265,6 +352,26 @@ int gsh_grid_get(const gsh_grid* g, double* xy, double* depth, int32_t* elnode, 
     }
     if (kb) std::memcpy(kb, g->kb.data(), 4 * g->kb.size());
     if (sigma) std::memcpy(sigma, g->sigma.data(), 8 * g->sigma.size());
+  })
+}
+
+int gsh_grid_vgrid(const gsh_grid* g, int32_t* ivcor, int32_t* per_node, int32_t* nz, double* params,
+                   double* ztot, double* sigma_s, double* sigma_node) {
+  GSH_API_BODY({
+    GSH_REQUIRE(g, GSH_E_ARG, "grid is NULL");
+    GSH_REQUIRE(g->nvrt > 0, GSH_E_STATE, "no vgrid.in was read");
+    if (ivcor) *ivcor = g->ivcor;
+    if (per_node) *per_node = g->per_node;
+    if (nz) *nz = g->nz;
+    if (g->ivcor == 2) {
+      if (params) params[0] = g->h_s, params[1] = g->h_c, params[2] = g->theta_b, params[3] = g->theta_f;
+      if (ztot) std::memcpy(ztot, g->ztot.data(), 8 * g->ztot.size());
+      if (sigma_s) std::memcpy(sigma_s, g->sigma_s.data(), 8 * g->sigma_s.size());
+    }
+    if (sigma_node) {
+      GSH_REQUIRE(g->ivcor == 1 && g->per_node, GSH_E_STATE, "sigma_node: the grid has no per-node sigma");
+      std::memcpy(sigma_node, g->sigma_node.data(), 8 * g->sigma_node.size());
+    }
   })
 }
 

@@ -35,7 +35,31 @@ struct HostMesh {
   std::vector<int32_t> elnode;     // [ne][3]
   std::vector<int8_t> open_bnd;    // [n]
   std::vector<int32_t> kb;         // [n]
+  // vertical coordinates: vmode 0 = ivcor 1 with `sigma` at every node, 1 = ivcor 1 with a sigma
+  // per node and level, 2 = ivcor 2 (SZ)
+  int32_t vmode = 0;
+  std::vector<double> sig_node;    // [n][nvrt] (vmode 1)
+  int32_t nz = 0;                  // Z levels incl. the one at -h_s (vmode 2)
+  double h_s = 0, h_c = 0, theta_b = 0, theta_f = 0;
+  std::vector<double> ztot;        // [nz] (vmode 2)
+  std::vector<double> ssig, cs;    // [nvrt - nz + 1] S levels and their stretching C(sigma) (vmode 2)
 };
+
+// Vertical grid of a gsh_vgrid (validated as readVGrid does, read_and_process_grid.cpp:1397-1503)
+// into m (input node order; before renumbering). Throws Error(GSH_E_ARG) on a bad grid.
+struct VGridIn {
+  int32_t ivcor;
+  const double* sigma_node;
+  int32_t nz;
+  double h_s, h_c, theta_b, theta_f;
+  const double* ztot;
+  const double* sigma_s;
+};
+void setup_vertical(HostMesh& m, const VGridIn& vg);
+// SZ bottom level of every node from its depth (Node::updateKb, meshdata.cu:1676-1718): the Z
+// level below the bed, or nz-1 where the depth is within h_s. Throws where the bed lies below
+// the deepest Z level.
+void sz_bottom_levels(HostMesh& m);
 
 // Topology of HostMesh following read_and_process_grid.cpp and meshdata.cu:2094-2141.
 struct Topology {

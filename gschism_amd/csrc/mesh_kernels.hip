@@ -17,14 +17,33 @@ namespace gsh {
 
 constexpr double kSmall = 1.0e-7;  // meshdata.h:10
 
-// compZCoord for ivcor=1 (meshdata.cu:1744-1807): z of level k at a node.
+// Node::compZCoord (meshdata.cu:1744-1807) at level k of a wet node with elevation eta, depth h
+// and bottom level kb; sigmaTransitionDepth (:1647) is min(h, h_s). ivcor=2 (SZ: Z levels, the
+// level at -min(h, h_s), S levels) and ivcor=1 (one sigma per level, or per node and level).
+__device__ __forceinline__ double z_wet(const DMesh& m, int node, double eta, double h, int kb, int k) {
+  const int nv = m.nvrt;
+  if (m.vmode == 2) {
+    const double hmod2 = fmin(h, m.h_s);
+    if (k == nv - 1) return eta;
+    if (k == m.izlev) return -hmod2;
+    if (k == kb) return -h;
+    if (k < m.izlev) return m.ztot[k];
+    const int ks = k - m.izlev;
+    const double sg = m.ssig[ks];
+    if (hmod2 <= m.h_c) return sg * (hmod2 + eta) + eta;
+    return eta * (1.0 + sg) + m.h_c * sg + (hmod2 - m.h_c) * m.cs[ks];
+  }
+  if (k <= kb) return -h;
+  if (k == nv - 1) return eta;
+  const double sg = m.vmode == 1 ? m.sig_node[(int64_t)node * nv + k] : m.sigma[k];
+  return (eta + h) * sg + eta;
+}
+
+// compZCoord (meshdata.cu:1744-1807): z of level k at a node (0 when the node is dry).
 __device__ __forceinline__ double zcoord(const DMesh& m, double h0, int node, int k) {
   const double eta = m.eta2[node], h = m.depth[node];
   if (eta + h <= h0) return 0.0;  // isDry2
-  const int kb = m.kb[node];
-  if (k <= kb) return -h;
-  if (k == m.nvrt - 1) return eta;
-  return (eta + h) * m.sigma[k] + eta;
+  return z_wet(m, node, eta, h, m.kb[node], k);
 }
 
 // ---------------------------------------------------------------- a3: levels and wet/dry
@@ -135,34 +154,46 @@ __device__ __forceinline__ ElemGeom elem_geom(const DMesh& m, int e) {
   return {r.x, r.y, r.z, r.w, q.x, q.y, p0.x, p0.y, p1.x, p1.y, p2.x, p2.y};
 }
 
-// A node's elevation, depth, total depth H = eta + h, bottom level and dry flag, in registers.
+// A node's elevation, depth, total depth H = eta + h, bottom level, dry flag and id, in registers.
 struct NodeZ {
   double eta, h, H;
   int kb;
   bool dry;
+  int node;
 };
 
 __device__ __forceinline__ NodeZ node_z(const DMesh& m, double h0, int node) {
   const double2 q = m.nrec[2 * node + 1];
   const double H = q.x + q.y;
-  return {q.x, q.y, H, m.kb[node], H <= h0};
+  return {q.x, q.y, H, m.kb[node], H <= h0, node};
 }
 
-// zcoord(node, max(M, kb)) with sM = sigma[M]: for M <= kb that is the bed -h, above kb level M.
+// zcoord(node, max(M, kb)) with sM = sigma[M] (one sigma per level): for M <= kb that is the bed
+// -h, above kb level M.
 __device__ __forceinline__ double zlev(const NodeZ& q, double sM, int M, int nv) {
   const double zs = (M == nv - 1) ? q.eta : q.H * sM + q.eta;
   return q.dry ? 0.0 : ((M <= q.kb) ? -q.h : zs);
 }
 
-// z of level M at the point with shape coordinates s0..s2 (the vertical bracket search's probe)
-__device__ __forceinline__ double zprobe(const NodeZ& q0, const NodeZ& q1, const NodeZ& q2, double s0, double s1,
-                                         double s2, const double* sig, int M, int nv) {
-  const double sM = sig[max(M, 0)];
-  return s0 * zlev(q0, sM, M, nv) + s1 * zlev(q1, sM, M, nv) + s2 * zlev(q2, sM, M, nv) + 0.0;
+// z of level M at the point with shape coordinates s0..s2 (the vertical bracket search's probe).
+// VM = 0: one sigma per level, read once per probe from sig (LDS); otherwise each node's level
+// max(M, kb) from z_wet (per-node sigma or SZ), whose level kb is the bed -h in every system.
+template <int VM>
+__device__ __forceinline__ double zprobe(const DMesh& m, const NodeZ& q0, const NodeZ& q1, const NodeZ& q2,
+                                         double s0, double s1, double s2, const double* sig, int M, int nv) {
+  if (VM == 0) {
+    const double sM = sig[max(M, 0)];
+    return s0 * zlev(q0, sM, M, nv) + s1 * zlev(q1, sM, M, nv) + s2 * zlev(q2, sM, M, nv) + 0.0;
+  }
+  auto zl = [&](const NodeZ& q) {
+    return q.dry ? 0.0 : ((M <= q.kb) ? -q.h : z_wet(m, q.node, q.eta, q.h, q.kb, M));
+  };
+  return s0 * zl(q0) + s1 * zl(q1) + s2 * zl(q2) + 0.0;
 }
 
 // Element::interpolateVelocities for triangles, ics=1 (meshdata.cu:1209-1291) with
 // computeShapeCoords(fiAnywhere) (meshdata.cu:990-1043); g is element e's geometry.
+template <int VM>
 __device__ void interp_velocity(const DMesh& m, const double* sig, double h0, int e, const ElemGeom& g, double px,
                                 double py, double pz, double& uo, double& vo, double& wo) {
   const int n0 = g.n0, n1 = g.n1, n2 = g.n2;
@@ -183,8 +214,8 @@ __device__ void interp_velocity(const DMesh& m, const double* sig, double h0, in
   const NodeZ q0 = node_z(m, h0, n0), q1 = node_z(m, h0, n1), q2 = node_z(m, h0, n2);
   const int kb0 = q0.kb, kb1 = q1.kb, kb2 = q2.kb;
   int L = m.elem_kb[e] - 1, R = nv - 1;
-  double zL = zprobe(q0, q1, q2, s0, s1, s2, sig, L, nv);
-  double zR = zprobe(q0, q1, q2, s0, s1, s2, sig, R, nv);
+  double zL = zprobe<VM>(m, q0, q1, q2, s0, s1, s2, sig, L, nv);
+  double zR = zprobe<VM>(m, q0, q1, q2, s0, s1, s2, sig, R, nv);
   double w1, w2;
   if (pz <= zL) {
     w1 = 1.0;
@@ -195,7 +226,7 @@ __device__ void interp_velocity(const DMesh& m, const double* sig, double h0, in
   } else {
     while (R - L > 1) {
       const int M = (L + R) / 2;
-      const double zM = zprobe(q0, q1, q2, s0, s1, s2, sig, M, nv);
+      const double zM = zprobe<VM>(m, q0, q1, q2, s0, s1, s2, sig, M, nv);
       if (zM < pz) {
         L = M;
         zL = zM;
@@ -269,6 +300,7 @@ __device__ bool celestial_walk(const DMesh& m, int& cur, double x0x, double x0y,
 }
 
 // Side::btrack (meshdata.cu:1449-1629). Returns the final velocity and the foot element.
+template <int VM>
 __device__ void btrack(const DMesh& m, const double* sig, const StepArgs& a, int s, int k, double dtbk, int cur, double& ux,
                        double& uy, int& foot, int& status) {
   const double h0 = a.h0;
@@ -331,7 +363,7 @@ __device__ void btrack(const DMesh& m, const double* sig, const StepArgs& a, int
       const double len = sqrt(v2x * v2x + v2y * v2y);
       const double tx = v2x / len, ty = v2y / len;
       double iu, iv, iw;
-      interp_velocity(m, sig, h0, cur, geo, xtx, xty, xtz, iu, iv, iw);
+      interp_velocity<VM>(m, sig, h0, cur, geo, xtx, xty, xtz, iu, iv, iw);
       const double d = iu * tx + iv * ty;
       uix = tx * d;
       uiy = ty * d;
@@ -339,7 +371,7 @@ __device__ void btrack(const DMesh& m, const double* sig, const StepArgs& a, int
       status |= GSH_BT_SLIDE;
     } else {
       time_rm -= dtb;
-      interp_velocity(m, sig, h0, cur, geo, xtx, xty, xtz, uix, uiy, uiz);
+      interp_velocity<VM>(m, sig, h0, cur, geo, xtx, xty, xtz, uix, uiy, uiz);
     }
     if (time_rm <= 0.0) break;
     if (uix * uix + uiy * uiy <= a.velmin2) {
@@ -364,18 +396,22 @@ __device__ void btrack(const DMesh& m, const double* sig, const StepArgs& a, int
     t1 = time_rm;
   }
   double iw;
-  interp_velocity(m, sig, h0, cur, geo, xtx, xty, xtz, ux, uy, iw);
+  interp_velocity<VM>(m, sig, h0, cur, geo, xtx, xty, xtz, ux, uy, iw);
   foot = cur;
 }
 
 // backtrackAllNodes (meshdata.cu:3547-3660), one thread per (side, level): consecutive
 // lanes are consecutive levels of one side, so the nodal [node][level] gathers of a wave
 // fall on a few contiguous cache lines.
+// VM: the vertical coordinate system (DMesh::vmode); 0, one sigma per level, keeps it in LDS.
+template <int VM>
 __global__ void __launch_bounds__(256) k_backtrack(DMesh m, StepArgs a, const int32_t* list_side,
                                                    const int32_t* list_lev, int32_t count) {
-  __shared__ double sig[1024];
-  for (int j = threadIdx.x; j < m.nvrt; j += blockDim.x) sig[j] = m.sigma[j];
-  __syncthreads();
+  __shared__ double sig[VM == 0 ? 1024 : 1];
+  if (VM == 0) {
+    for (int j = threadIdx.x; j < m.nvrt; j += blockDim.x) sig[j] = m.sigma[j];
+    __syncthreads();
+  }
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int s, k;
   if (list_side) {
@@ -414,7 +450,7 @@ __global__ void __launch_bounds__(256) k_backtrack(DMesh m, StepArgs a, const in
     const int ndelt = max(a.ndelt_min, min(nd, a.ndelt_max));
     const double dtbk = a.dt / ndelt;
     const int start = m.elem_wet[sr.z] ? sr.z : sr.w;
-    btrack(m, sig, a, s, k, dtbk, start, bx, by, foot, status);
+    btrack<VM>(m, sig, a, s, k, dtbk, start, bx, by, foot, status);
   }
   m.btu[2 * idx] = bx;
   m.btu[2 * idx + 1] = by;
@@ -435,7 +471,11 @@ void launch_backtrack(const DMesh& m, const StepArgs& a, const int32_t* ls, cons
   if (m.n > 0) hipLaunchKernelGGL(k_pack_nodes, dim3((m.n + kTpb - 1) / kTpb), dim3(kTpb), 0, st, m);
   const int64_t total = ls ? (int64_t)count : (int64_t)m.ns * m.nvrt;
   if (total <= 0) return;
-  hipLaunchKernelGGL(k_backtrack, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, st, m, a, ls, ll, count);
+  const dim3 grid((unsigned)((total + 255) / 256));
+  if (m.vmode == 0)
+    hipLaunchKernelGGL(k_backtrack<0>, grid, dim3(256), 0, st, m, a, ls, ll, count);
+  else
+    hipLaunchKernelGGL(k_backtrack<1>, grid, dim3(256), 0, st, m, a, ls, ll, count);
 }
 
 // ---------------------------------------------------------------- nodal velocities
@@ -1005,7 +1045,7 @@ void launch_momentum(const DMesh& m, const StepArgs& a, hipStream_t st) {
 struct VvElem {
   double px[3], py[3], eta[3], h[3];
   double snx[3], sny[3], sign[3], slen[3];
-  int nkb[3], s[3], skb[3], dry[3];
+  int nkb[3], s[3], skb[3], dry[3], nd[3];
 };
 
 __global__ void __launch_bounds__(256) k_vertical_velocity(DMesh m, double h0, int E) {
@@ -1032,6 +1072,7 @@ __global__ void __launch_bounds__(256) k_vertical_velocity(DMesh m, double h0, i
       q.eta[j] = m.eta_lev[i];
       q.h[j] = m.depth[i];
       q.nkb[j] = m.kb[i];
+      q.nd[j] = i;
       q.dry[j] = q.eta[j] + q.h[j] <= h0;  // isDry2 at the level update
       const int s = sd[j];
       const int4 sr = m.side[s];
@@ -1056,7 +1097,7 @@ __global__ void __launch_bounds__(256) k_vertical_velocity(DMesh m, double h0, i
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
       const int k = max(l + 1, q.nkb[j]);
-      pz[j] = q.dry[j] ? 0.0 : (k <= q.nkb[j] ? -q.h[j] : (k == nv - 1 ? q.eta[j] : (q.eta[j] + q.h[j]) * m.sigma[k] + q.eta[j]));
+      pz[j] = q.dry[j] ? 0.0 : z_wet(m, q.nd[j], q.eta[j], q.h[j], q.nkb[j], k);
     }
     const double n20x = q.px[2] - q.px[0], n20y = q.py[2] - q.py[0], n20z = pz[2] - pz[0];
     const double n10x = q.px[1] - q.px[0], n10y = q.py[1] - q.py[0], n10z = pz[1] - pz[0];

@@ -381,6 +381,12 @@ void renumber(HostMesh& m, const std::vector<int32_t>& perm, std::vector<int32_t
   apply(m.depth);
   apply(m.open_bnd);
   apply(m.kb);
+  if (!m.sig_node.empty()) {  // rows of the per-node sigma follow their nodes
+    const size_t nv = (size_t)m.nvrt;
+    std::vector<double> old = m.sig_node;
+    for (int32_t k = 0; k < n; ++k)
+      std::copy(old.begin() + perm[k] * nv, old.begin() + (perm[k] + 1) * nv, m.sig_node.begin() + k * nv);
+  }
   std::vector<int32_t> en(3 * (size_t)ne);
   for (size_t k = 0; k < en.size(); ++k) en[k] = newid[m.elnode[k]];
   elem_perm.resize(ne);
@@ -396,6 +402,75 @@ void renumber(HostMesh& m, const std::vector<int32_t>& perm, std::vector<int32_t
   m.elnode.resize(3 * (size_t)ne);
   for (int32_t k = 0; k < ne; ++k)
     for (int j = 0; j < 3; ++j) m.elnode[3 * k + j] = en[3 * elem_perm[k] + j];
+}
+
+void setup_vertical(HostMesh& m, const VGridIn& vg) {
+  const int32_t nv = m.nvrt;
+  auto require = [](bool ok, const std::string& what) {
+    if (!ok) throw Error(1 /* GSH_E_ARG */, "vertical grid: " + what);
+  };
+  if (vg.ivcor == 1) {
+    require(vg.sigma_node != nullptr, "ivcor 1 needs sigma_node");
+    m.sig_node.assign(vg.sigma_node, vg.sigma_node + (size_t)m.n * nv);
+    for (int32_t i = 0; i < m.n; ++i) {
+      const double* s = m.sig_node.data() + (size_t)i * nv;
+      // the levels above a node's bed must increase towards the surface (its kb level is the bed)
+      for (int k = m.kb[i] + 2; k < nv; ++k)
+        require(s[k] > s[k - 1], "sigma must increase above the bottom level of node " + std::to_string(i));
+    }
+    m.vmode = 1;
+    return;
+  }
+  require(vg.ivcor == 2, "ivcor must be 1 or 2");
+  // readVGrid's checks (read_and_process_grid.cpp:1412-1486)
+  require(vg.nz >= 1 && vg.nz <= nv - 1, "nz must be in [1, nvrt-1]");
+  require(vg.h_s >= 6.0, "h_s needs to be larger");
+  require(vg.h_c >= 5.0 && vg.h_c < vg.h_s, "h_c needs to be in [5, h_s)");
+  require(vg.theta_b >= 0.0 && vg.theta_b <= 1.0, "wrong theta_b");
+  require(vg.theta_f > 0.0, "wrong theta_f");
+  require(vg.ztot != nullptr && vg.sigma_s != nullptr, "ivcor 2 needs ztot and sigma_s");
+  const int32_t nz = vg.nz, nsig = nv - nz + 1;
+  for (int k = 0; k < nz - 1; ++k) {
+    require(vg.ztot[k] < -vg.h_s, "illegal Z level " + std::to_string(k + 1));
+    require(k == 0 || vg.ztot[k] > vg.ztot[k - 1], "Z levels inverted at " + std::to_string(k + 1));
+  }
+  require(vg.ztot[nz - 1] == -vg.h_s, "the last Z level must be -h_s");
+  // the first and last S levels are the bed (-1) and the surface (0); compZCoord never reads them
+  for (int k = 1; k < nsig; ++k)
+    require(vg.sigma_s[k] > vg.sigma_s[k - 1] && vg.sigma_s[k] >= -1.0 && vg.sigma_s[k] <= 0.0,
+            "S levels must increase within [-1, 0]");
+  m.nz = nz;
+  m.h_s = vg.h_s;
+  m.h_c = vg.h_c;
+  m.theta_b = vg.theta_b;
+  m.theta_f = vg.theta_f;
+  m.ztot.assign(vg.ztot, vg.ztot + nz);
+  m.ssig.assign(vg.sigma_s, vg.sigma_s + nsig);
+  // the stretching function of readVGrid (read_and_process_grid.cpp:1497-1502)
+  m.cs.resize(nsig);
+  const double tb = vg.theta_b, tf = vg.theta_f;
+  for (int k = 0; k < nsig; ++k)
+    m.cs[k] = (1.0 - tb) * std::sinh(tf * m.ssig[k]) / std::sinh(tf) +
+              tb * (std::tanh(tf * (m.ssig[k] + 0.5)) - std::tanh(tf * 0.5)) / 2.0 / std::tanh(tf * 0.5);
+  m.vmode = 2;
+  sz_bottom_levels(m);
+}
+
+void sz_bottom_levels(HostMesh& m) {
+  const int32_t izlev = m.nz - 1;
+  for (int32_t i = 0; i < m.n; ++i) {
+    const double d = m.depth[i];
+    int32_t kb = izlev;
+    if (d > m.h_s) {
+      // the Z level k < nz - 1 with ztot[k] <= -d < ztot[k + 1]
+      if (!(-d >= m.ztot[0]))
+        throw Error(1, "vertical grid: depth " + std::to_string(d) + " at node " + std::to_string(i) +
+                           " lies below the deepest Z level");
+      kb = 0;
+      while (kb + 1 < izlev && m.ztot[kb + 1] <= -d) ++kb;
+    }
+    m.kb[i] = kb;
+  }
 }
 
 }  // namespace gsh

@@ -72,6 +72,22 @@ int  gsh_ctx_synchronize(gsh_ctx* ctx);
 /* Host description of a triangular mesh. Replaces the hgrid/vgrid arrays that
  * readHGrid/readVGrid hand to populateNodes/populateSides/populateElements
  * (read_and_process_grid.cpp:829-1249, meshdata.cu:1903-2462). */
+/* Vertical grid beyond one sigma per level, as readVGrid returns it
+ * (read_and_process_grid.cpp:1313-1520) and Node::compZCoord uses it (meshdata.cu:1744-1807). */
+typedef struct gsh_vgrid {
+  int32_t ivcor;             /* 1 = localized sigma (LSC2, a sigma per node and level), 2 = SZ */
+  /* ivcor = 1 */
+  const double* sigma_node;  /* [n][nvrt] sigma of every node and level (desc node order);
+                                levels at or below a node's kb are its bed */
+  /* ivcor = 2: levels 0 .. nz-2 are Z levels ztot, level nz-1 is at -min(h, h_s), levels
+   * nz .. nvrt-1 are S levels 1 .. nvrt-nz of sigma_s; each node's kb follows from its depth
+   * (Node::updateKb, meshdata.cu:1676-1718) and desc->kb is ignored */
+  int32_t nz;                /* kz of vgrid.in: Z levels including the one at -h_s (>= 1) */
+  double h_s, h_c, theta_b, theta_f;
+  const double* ztot;        /* [nz] increasing, ztot[k] < -h_s for k < nz-1; ztot[nz-1] = -h_s */
+  const double* sigma_s;     /* [nvrt-nz+1] S levels, sigma_s[0] = -1 ... sigma_s[nvrt-nz] = 0 */
+} gsh_vgrid;
+
 typedef struct gsh_mesh_desc {
   int32_t n;              /* nodes */
   int32_t ne;             /* triangles */
@@ -80,9 +96,11 @@ typedef struct gsh_mesh_desc {
   const int32_t* elnode;  /* [ne][3] counter-clockwise node ids */
   const double*  depth;   /* [n] depth h (positive down, metres) */
   const int8_t*  open_bnd;/* [n] 1 = node on an open boundary with an elevation (essential) BC; NULL = none */
-  const double*  sigma;   /* [nvrt] sigma levels, sigma[0]=-1 ... sigma[nvrt-1]=0 (ivcor=1, meshdata.cu:1798-1803) */
+  const double*  sigma;   /* [nvrt] sigma levels, sigma[0]=-1 ... sigma[nvrt-1]=0 (ivcor=1, meshdata.cu:1798-1803);
+                             may be NULL when vgrid is given */
   const int32_t* kb;      /* [n] bottom level index per node; NULL = 0 everywhere */
   int32_t renumber;       /* 1 = reverse Cuthill-McKee renumbering of nodes (elements follow) */
+  const gsh_vgrid* vgrid; /* NULL = ivcor 1 with `sigma` at every node */
 } gsh_mesh_desc;
 
 typedef struct gsh_mesh_info {
@@ -244,9 +262,18 @@ int gsh_grid_read(const char* hgrid_path, const char* vgrid_path, gsh_grid** out
 int gsh_grid_info(const gsh_grid* g, int32_t* n, int32_t* ne, int32_t* nvrt, int32_t* n_open_seg,
                   int32_t* n_land_seg);
 /* Any output may be NULL; kb [n] and sigma [nvrt] need a vgrid. open_bnd[i] = 1 on open segments. */
+/* kb is in the file for ivcor 1 only; sigma (one per level) exists for ivcor 1 when every node has
+ * the same column above its bed (GSH_E_STATE otherwise). */
 int gsh_grid_get(const gsh_grid* g, double* xy, double* depth, int32_t* elnode, int8_t* open_bnd, int32_t* kb,
                  double* sigma);
 /* Nodes (0-based) of open (land = 0) or land (land = 1) segment seg; *island = the land flag. */
+/* The vertical grid of the vgrid.in read (readVGrid, read_and_process_grid.cpp:1313-1520): ivcor
+ * (1 or 2); ivcor 1: per_node = 1 when the nodes' sigma differ above their beds, and then
+ * sigma_node [n][nvrt] receives every node's column (gsh_grid_get's kb and, with per_node = 0,
+ * sigma give the rest); ivcor 2: nz (kz), params = {h_s, h_c, theta_b, theta_f}, ztot [nz],
+ * sigma_s [nvrt-nz+1]. Outputs may be NULL. */
+int gsh_grid_vgrid(const gsh_grid* g, int32_t* ivcor, int32_t* per_node, int32_t* nz, double* params,
+                   double* ztot, double* sigma_s, double* sigma_node);
 int gsh_grid_segment(const gsh_grid* g, int land, int32_t seg, int32_t* nodes_out, int32_t cap, int32_t* count,
                      int32_t* island);
 int gsh_grid_destroy(gsh_grid* g);

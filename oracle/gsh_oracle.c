@@ -9,7 +9,8 @@
  *   src/grid/read_and_process_grid.cpp:27-174, 272-437, 496-574   topology
  *   src/kernels/meshdata.cu:1903-1958, 2094-2141                   CSR pattern
  *   src/kernels/meshdata.cu:2274-2312, 2350-2440, 749-771           geometry
- *   src/kernels/meshdata.cu:5521-5612, 5679-5760, 1676-1807         levels, wet/dry
+ *   src/kernels/meshdata.cu:5521-5612, 5679-5760, 1676-1807, 1647    levels (SZ, LSC2), wet/dry
+ *   src/grid/read_and_process_grid.cpp:1497-1502                    SZ stretching C(sigma)
  *   src/kernels/meshdata.cu:3547-3660, 1449-1629, 85-127, 635-746,
  *                           990-1043, 1209-1291                     ELM backtracking
  *   src/kernels/meshdata.cu:4459-4476, 4615-4725, 6183-6424, 4276-4339   RHS producers
@@ -18,7 +19,8 @@
  *   src/kernels/meshdata.cu:5765-5929                               nodal velocities
  *   src/kernels/meshdata.cu:5045-5446, 3665-3711, 5456-5518         momentum, Shapiro filter
  *   src/kernels/meshdata.cu:5978-6102, 559-631                      element vertical velocity
- * for the option subset ics=1, ivcor=1 (global sigma), imm=0, iveg=0, ibc=1 (barotropic),
+ * for the option subset ics=1, ivcor=1 (one sigma per level, or per node and level: LSC2) and
+ * ivcor=2 (SZ), imm=0, iveg=0, ibc=1 (barotropic),
  * itur=0, islip=0, nws=0, ncor=0, ihorcon=0, nchi=0, triangles; elevation BC types 0-5 per node.
  *
  * Deliberate deviations, shared with the device code:
@@ -94,6 +96,11 @@ struct or_mesh {
   int8_t* etype;
   int32_t nbfr;
   double *amig, *ff, *face, *emo, *efa;
+  /* vertical coordinates: vmode 0 = ivcor 1 with sigma per level, 1 = ivcor 1 with a sigma per
+   * node and level (sig_node [n][nvrt]), 2 = ivcor 2 (SZ: ztot [nz], ssig and cs [nvrt-nz+1]) */
+  int32_t vmode, izlev;
+  double h_s, h_c;
+  double *sig_node, *ztot, *ssig, *cs;
 };
 
 static int nxq(int offset, int index, int size) { return (index + offset + 16 * size) % size; }
@@ -115,7 +122,7 @@ void or_mesh_free(or_mesh* m) {
                   m->slen, m->snx, m->sny, m->eta2, m->eta1, m->uvw, m->su2, m->btu, m->cd, m->grav2,
                   m->eth, m->vsource, m->elbc, m->foot, m->bstatus, m->sU, m->hhat, m->chi, m->ghat1,
                   m->zc, m->nz, m->node_dry, m->elem_dry, m->side_dry, m->elem_kb, m->side_kb, m->ew,
-                  m->etype, m->amig, m->ff, m->face, m->emo, m->efa};
+                  m->etype, m->amig, m->ff, m->face, m->emo, m->efa, m->sig_node, m->ztot, m->ssig, m->cs};
   for (size_t i = 0; i < sizeof ptrs / sizeof ptrs[0]; ++i) free(ptrs[i]);
   free(m);
 }
@@ -555,18 +562,116 @@ int or_get_field(const or_mesh* m, int field, void* out) {
 /* ------------------------------------------------------------------ levels */
 static int is_dry2(const or_mesh* m, int i, double h0) { return m->eta2[i] + m->depth[i] <= h0; }
 
-/* Node::compZCoord, ivcor=1 (meshdata.cu:1744-1807) */
+/* Node::compZCoord (meshdata.cu:1744-1807) with sigmaTransitionDepth (:1647): ivcor=2 (SZ) and
+ * ivcor=1 (localized sigma: one sigma per level, or one per node and level) */
 static double comp_z(const or_mesh* m, int i, int k, double h0) {
   if (is_dry2(m, i, h0)) return 0.0;
   const double eta2 = m->eta2[i], depth = m->depth[i];
+  if (m->vmode == 2) {
+    const double hmod2 = fmin(depth, m->h_s);
+    const double h_c = m->h_c;
+    if (k == m->nvrt - 1) return eta2;
+    else if (k == m->izlev) return -hmod2;
+    else if (k == m->kb[i]) return -depth;
+    else if (k < m->izlev) return m->ztot[k];
+#if OR_PLANT == 10
+    const int ksig = k - m->izlev - 1;
+#else
+    const int ksig = k - m->izlev;
+#endif
+    if (hmod2 <= h_c) return m->ssig[ksig] * (hmod2 + eta2) + eta2;
+#if OR_PLANT == 11
+    return eta2 * (1.0 + m->ssig[ksig]) + hmod2 * m->ssig[ksig] + (h_c - hmod2) * m->cs[ksig];
+#else
+    return eta2 * (1.0 + m->ssig[ksig]) + h_c * m->ssig[ksig] + (hmod2 - h_c) * m->cs[ksig];
+#endif
+  }
+#if OR_PLANT == 12
+  const double sg = m->vmode == 1 ? m->sig_node[(size_t)((i + 1) % m->n) * m->nvrt + k] : m->sigma[k];
+#else
+  const double sg = m->vmode == 1 ? m->sig_node[(size_t)i * m->nvrt + k] : m->sigma[k];
+#endif
   if (k <= m->kb[i]) return -depth;
   else if (k == m->nvrt - 1) return eta2;
-  else return (eta2 + depth) * m->sigma[k] + eta2;
+  else return (eta2 + depth) * sg + eta2;
+}
+
+/* Node::updateKb for ivcor=2 (meshdata.cu:1676-1718): the bisection over the Z levels;
+ * returns -1 where the reference's assert fails (bed below the deepest Z level). */
+static int sz_kb(const or_mesh* m, double depth) {
+  if (depth <= m->h_s) return m->izlev;
+  int kb = -1, L = 0, R = m->izlev + 1; /* R = nzlev */
+  const double* ztot = m->ztot;
+  if (!(-depth >= ztot[L] && -depth < (R <= m->izlev ? ztot[R] : 0.0))) return -1;
+  while (L <= R) {
+    const int M = (R + L) / 2;
+    const double zM = M <= m->izlev ? ztot[M] : 0.0; /* readVGrid leaves ztot[nz] = 0 */
+    if (-depth >= ztot[L] && -depth < zM)
+      R = M;
+    else
+      L = M;
+    if (R - L == 1) {
+      kb = L;
+      break;
+    }
+  }
+  return kb;
+}
+
+static int update_sz_kb(or_mesh* m) {
+  for (int i = 0; i < m->n; ++i) {
+    const int kb = sz_kb(m, m->depth[i]);
+    if (kb < 0) {
+      set_err("ZCOOR: invalid depth %g at node %d (below the deepest Z level)", m->depth[i], i);
+      return -1;
+    }
+    m->kb[i] = kb;
+  }
+  return 0;
+}
+
+int or_set_vertical(or_mesh* m, int32_t ivcor, const double* sigma_node, int32_t nz, double h_s, double h_c,
+                    double theta_b, double theta_f, const double* ztot, const double* sigma_s) {
+  const int nv = m->nvrt;
+  if (ivcor == 1) {
+    if (!sigma_node) {
+      set_err("ivcor 1 needs sigma_node");
+      return -1;
+    }
+    free(m->sig_node);
+    m->sig_node = xcalloc((size_t)m->n * nv, sizeof(double));
+    memcpy(m->sig_node, sigma_node, sizeof(double) * (size_t)m->n * nv);
+    m->vmode = 1;
+    return 0;
+  }
+  if (ivcor != 2 || nz < 1 || nz > nv - 1 || !ztot || !sigma_s) {
+    set_err("ivcor 2 needs 1 <= nz <= nvrt-1, ztot and sigma_s");
+    return -1;
+  }
+  const int nsig = nv - nz + 1;
+  free(m->ztot);
+  free(m->ssig);
+  free(m->cs);
+  m->ztot = xcalloc(nz, sizeof(double));
+  memcpy(m->ztot, ztot, sizeof(double) * nz);
+  m->ssig = xcalloc(nsig, sizeof(double));
+  memcpy(m->ssig, sigma_s, sizeof(double) * nsig);
+  /* stretching function C(sigma) of readVGrid (read_and_process_grid.cpp:1497-1502) */
+  m->cs = xcalloc(nsig, sizeof(double));
+  for (int k = 0; k < nsig; ++k)
+    m->cs[k] = (1.0 - theta_b) * sinh(theta_f * m->ssig[k]) / sinh(theta_f) +
+               theta_b * (tanh(theta_f * (m->ssig[k] + 0.5)) - tanh(theta_f * 0.5)) / 2.0 / tanh(theta_f * 0.5);
+  m->h_s = h_s;
+  m->h_c = h_c;
+  m->izlev = nz - 1;
+  m->vmode = 2;
+  return update_sz_kb(m);
 }
 
 void or_levels(or_mesh* m, const or_params* p) {
   const int n = m->n, ne = m->ne, ns = m->ns, nv = m->nvrt;
-  /* ul0UpdateZCoords (meshdata.cu:5521-5548) */
+  /* ul0UpdateZCoords (meshdata.cu:5521-5548): Node::updateKb (ivcor=2 only), then the levels */
+  if (m->vmode == 2) update_sz_kb(m);
   for (int i = 0; i < n; ++i)
     for (int k = m->kb[i]; k < nv; ++k) m->nz[(size_t)i * nv + k] = comp_z(m, i, k, p->h0);
   /* ul0ComputeDryElements (meshdata.cu:5550-5580) */

@@ -30,6 +30,12 @@ const char* or_error(void);
 or_mesh* or_mesh_new(int32_t n, int32_t ne, int32_t nvrt, const double* xy, const int32_t* elnode,
                      const double* depth, const int8_t* open_bnd, const int32_t* kb, const double* sigma);
 void or_mesh_free(or_mesh* m);
+/* Vertical coordinates beyond one sigma per level (readVGrid, read_and_process_grid.cpp:1313-1520):
+ * ivcor 1 with sigma_node [n][nvrt] (mesh node order), or ivcor 2 (SZ) with nz Z levels ztot [nz],
+ * S levels sigma_s [nvrt-nz+1] and h_s, h_c, theta_b, theta_f; for ivcor 2 every node's kb is
+ * recomputed from its depth. Returns 0, or -1 with or_error(). */
+int or_set_vertical(or_mesh* m, int32_t ivcor, const double* sigma_node, int32_t nz, double h_s, double h_c,
+                    double theta_b, double theta_f, const double* ztot, const double* sigma_s);
 int32_t or_mesh_ns(const or_mesh* m);
 int64_t or_mesh_nnz(const or_mesh* m);
 int32_t or_mesh_nbr_count(const or_mesh* m);

@@ -40,6 +40,8 @@ _SIG = {
     "or_nodal_velocity": (None, [_P, C.POINTER(OrParams), C.c_int]),
     "or_momentum": (None, [_P, C.POINTER(OrParams)]),
     "or_set_tides": (C.c_int, [_P, C.c_int32, _PD, _PD, _PD, _PD, _PD]),
+    "or_set_vertical": (C.c_int, [_P, C.c_int32, _PD, C.c_int32, C.c_double, C.c_double, C.c_double, C.c_double,
+                                  _PD, _PD]),
     "or_vertical_velocity": (None, [_P]),
     "or_assemble": (None, [_P, C.POINTER(OrParams), _PD, _PD, C.c_int]),
     "or_pcg": (C.c_int32, [C.c_int32, _PI32, _PI32, _PD, _PD, _PD, _PD, _PD, C.c_double, C.c_int32, C.c_int32,
@@ -195,6 +197,18 @@ class OracleMesh:
 
     def nodal_velocity(self, p, nthreads: int = 1) -> None:
         self.L.or_nodal_velocity(self.h, C.byref(params(p)), nthreads)
+
+    def set_vertical(self, vg) -> None:
+        """Vertical grid beyond one sigma per level: `vg` has the fields of gsh_vgrid (ivcor,
+        sigma_node [n][nvrt] in this mesh's node order, nz, h_s, h_c, theta_b, theta_f, ztot,
+        sigma_s), e.g. gschism_amd.core.VGrid."""
+        sn = None if vg.sigma_node is None else np.ascontiguousarray(vg.sigma_node, dtype=np.float64)
+        zt = None if vg.ztot is None else np.ascontiguousarray(vg.ztot, dtype=np.float64)
+        ss = None if vg.sigma_s is None else np.ascontiguousarray(vg.sigma_s, dtype=np.float64)
+        self._vg = (sn, zt, ss)
+        if self.L.or_set_vertical(self.h, int(vg.ivcor), _d(sn), int(vg.nz), float(vg.h_s), float(vg.h_c),
+                                  float(vg.theta_b), float(vg.theta_f), _d(zt), _d(ss)):
+            raise RuntimeError("oracle vertical grid: " + self.L.or_error().decode())
 
     def set_tides(self, amig, ff, face, emo, efa) -> None:
         """Tidal constituents (amig, ff, face [nbfr]) and open-node amplitudes and phases [n][nbfr]."""
