@@ -54,6 +54,34 @@ __device__ __forceinline__ double block_tree(double v, double* red) {
   return t;
 }
 
+// block_tree over the first 256 threads of a larger workgroup (waves 4 and up add nothing and
+// write nothing; every thread passes the barriers). The total is returned to every thread.
+__device__ __forceinline__ double block_tree_lo(double v, double* red) {
+  v = wave_tree(v);
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0 && w < 4) red[w] = v;
+  __syncthreads();
+  const double t = (red[0] + red[1]) + (red[2] + red[3]);
+  __syncthreads();
+  return t;
+}
+
+// The partial of a tile of 256 * R rows whose workgroup has 256 * R threads, one row each, in the
+// order of a 256-thread workgroup with R rows per thread: lane position t adds the values of rows
+// t, t + 256, ..., t + 256 (R - 1) to 0.0 in that order, then block_tree. rows holds 256 * R
+// doubles of LDS; v is the thread's own row value.
+template <int R>
+__device__ __forceinline__ double tile_tree_wide(double v, double* rows, double* red) {
+  rows[threadIdx.x] = v;
+  __syncthreads();
+  double acc = 0.0;
+  if (threadIdx.x < kTpb) {
+#pragma unroll
+    for (int j = 0; j < R; ++j) acc += rows[j * kTpb + threadIdx.x];
+  }
+  return block_tree_lo(acc, red);
+}
+
 // Orders LDS writes before reads among the lanes of one wave.
 __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -98,6 +126,46 @@ __device__ __forceinline__ void final_reduce2(gptr<const double> P, gptr<const d
   }
   a = block_tree(u, red);
   b = block_tree(w, red);
+}
+
+// final_reduce / final_reduce2 by the first 256 threads of a larger workgroup, in the same order.
+__device__ __forceinline__ double final_reduce_lo(gptr<const double> P, int T, double* red) {
+  double u = 0.0;
+  if (threadIdx.x < kTpb) {
+    for (int m = threadIdx.x; m < T; m += 8 * kTpb) {
+      double v[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) v[j] = (m + j * kTpb < T) ? P[m + j * kTpb] : 0.0;
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        if (m + j * kTpb < T) u += v[j];
+    }
+  }
+  return block_tree_lo(u, red);
+}
+
+__device__ __forceinline__ void final_reduce2_lo(gptr<const double> P, gptr<const double> Q, int T, double* red,
+                                                 double& a, double& b) {
+  double u = 0.0, w = 0.0;
+  if (threadIdx.x < kTpb) {
+    for (int m = threadIdx.x; m < T; m += 8 * kTpb) {
+      double vp[8], vq[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const bool in = m + j * kTpb < T;
+        vp[j] = in ? P[m + j * kTpb] : 0.0;
+        vq[j] = in ? Q[m + j * kTpb] : 0.0;
+      }
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        if (m + j * kTpb < T) {
+          u += vp[j];
+          w += vq[j];
+        }
+    }
+  }
+  a = block_tree_lo(u, red);
+  b = block_tree_lo(w, red);
 }
 
 }  // namespace gsh

@@ -104,6 +104,39 @@ __device__ __forceinline__ int tile_row(int tile, int j) {
   return tile * kTpb * R + j * kTpb + threadIdx.x;
 }
 
+// Wide workgroups (W): 256 * R threads with one row each, thread i taking row i of the tile, i.e.
+// row tile_row<R>(tile, i / 256) of lane i % 256; every partial is summed in the R-rows-per-
+// thread order (tile_tree_wide), so results equal the 256-thread kernels' bit for bit.
+template <int R, bool W>
+__device__ __forceinline__ int srow(int tile, int j) {
+  return W ? tile * kTpb * R + (int)threadIdx.x : tile_row<R>(tile, j);
+}
+// a thread's share of a partial: its one row (W) or the running sum over its R rows
+template <bool W>
+__device__ __forceinline__ void sacc(double& a, double x) {
+  if (W)
+    a = x;
+  else
+    a += x;
+}
+template <int R, bool W>
+__device__ __forceinline__ double stree(double v, double* lrows, double* red) {
+  if (W) return tile_tree_wide<R>(v, lrows, red);
+  return block_tree(v, red);
+}
+
+// Row-vector stores and loads of the sweeps. Plain (cached) accesses: nontemporal stores, with or
+// without nontemporal loads of the streamed rows, measured slower on the 1M-node system (83.6 and
+// 83.2 against 88.5 solves/s at one row per thread).
+template <class T>
+__device__ __forceinline__ void vst(gptr<T> p, int i, T v) {
+  p[i] = v;
+}
+template <class T>
+__device__ __forceinline__ T vld(gptr<T> p, int i) {
+  return p[i];
+}
+
 struct ColourInfo {
   int start, rows, wl, wu, offl, offu;
 };
@@ -248,25 +281,26 @@ constexpr int kFirst = 1, kMid = 2, kLast = 4, kInit = 8, kLead = 16;
 // y += alpha p, r -= alpha q to its rows before t = (z - U t) / D~ with the new z = D~ r. The
 // sweep of colour 0 (no lower colours) also forms u and s = t + u for its rows.
 // lead: the one workgroup of the launch (per system) that writes the system's scalar state.
-template <class VT, int R>
+template <class VT, int R, bool W = false>
 __device__ __forceinline__ void ssor_bwd_body(const SsorView<VT>& S, int tile, int c, int flags, bool lead,
-                                              double* red) {
+                                              double* red, double* lrows) {
+  constexpr int J = W ? 1 : R;  // rows of this thread
   gptr<PcgState> st = S.st;
   const bool init = flags & kInit, mid = flags & kMid;
   const ColourInfo ci = colour_info(S, c);
   const gptr<const VT> t = C(S.t);
-  VT usum[R], dtv[R], rv[R], pv[R], qv[R], sv[R], yv[R];
+  VT usum[J], dtv[J], rv[J], pv[J], qv[J], sv[J], yv[J];
   // the row loads are issued before the done test so that their latency overlaps it
 #pragma unroll
-  for (int j = 0; j < R; ++j) {
-    const int v = tile_row<R>(tile, j);
-    dtv[j] = S.dt[v];
-    rv[j] = S.r[v];
+  for (int j = 0; j < J; ++j) {
+    const int v = srow<R, W>(tile, j);
+    dtv[j] = vld(S.dt, v);
+    rv[j] = vld(S.r, v);
     if (!init) {
-      pv[j] = S.p[v];
-      qv[j] = S.q[v];
-      sv[j] = S.s[v];
-      yv[j] = S.y[v];
+      pv[j] = vld(S.p, v);
+      qv[j] = vld(S.q, v);
+      sv[j] = vld(S.s, v);
+      yv[j] = vld(S.y, v);
     }
   }
   if (load_done(st) >= 0) return;
@@ -279,12 +313,16 @@ __device__ __forceinline__ void ssor_bwd_body(const SsorView<VT>& S, int tile, i
       const int k = st->k;
       wslot = (k + 1) & 1;
       double gamma, delta;
-      final_reduce2(C(S.part_rz) + (k & 1) * S.ntiles, C(S.part_pq) + (k & 1) * S.ntiles, S.ntiles, red, gamma,
-                    delta);
+      if (W)
+        final_reduce2_lo(C(S.part_rz) + (k & 1) * S.ntiles, C(S.part_pq) + (k & 1) * S.ntiles, S.ntiles, red, gamma,
+                         delta);
+      else
+        final_reduce2(C(S.part_rz) + (k & 1) * S.ntiles, C(S.part_pq) + (k & 1) * S.ntiles, S.ntiles, red, gamma,
+                      delta);
       if (k == 0) {
         alpha = (delta != 0.0) ? gamma / delta : 0.0;
         if (lead) {
-          const double bb = final_reduce(C(S.part_bb), S.ntiles, red);
+          const double bb = W ? final_reduce_lo(C(S.part_bb), S.ntiles, red) : final_reduce(C(S.part_bb), S.ntiles, red);
           if (threadIdx.x == 0) {
             const double atol = st->rtol * sqrt(bb);
             st->bb = bb;
@@ -315,49 +353,52 @@ __device__ __forceinline__ void ssor_bwd_body(const SsorView<VT>& S, int tile, i
     }
   }
 #pragma unroll
-  for (int j = 0; j < R; ++j) usum[j] = upper_sum(S, ci, tile_row<R>(tile, j), t);
+  for (int j = 0; j < J; ++j) usum[j] = upper_sum(S, ci, srow<R, W>(tile, j), t);
   const VT av = (VT)alpha, bv = (VT)beta;
   double g = 0.0, dl = 0.0;
 #pragma unroll
-  for (int j = 0; j < R; ++j) {
-    const int v = tile_row<R>(tile, j);
+  for (int j = 0; j < J; ++j) {
+    const int v = srow<R, W>(tile, j);
     VT r1 = rv[j];
     if (!init) {
       const VT zr = dtv[j] * rv[j];
       const VT pn = zr + bv * pv[j];
       const VT qn = sv[j] + bv * qv[j];
-      S.p[v] = pn;
-      S.q[v] = qn;
-      S.y[v] = yv[j] + av * pn;
+      vst(S.p, v, pn);
+      vst(S.q, v, qn);
+      vst(S.y, v, (VT)(yv[j] + av * pn));
       r1 = rv[j] - av * qn;
-      S.r[v] = r1;
+      vst(S.r, v, r1);
     }
     const VT zn = dtv[j] * r1;
-    g += (double)r1 * (double)zn;
+    sacc<W>(g, (double)r1 * (double)zn);
     const VT tv = (zn - usum[j]) / dtv[j];
-    S.t[v] = tv;
+    vst(S.t, v, tv);
     if (mid) {
       const VT kdv = S.unit_omega ? dtv[j] : S.kd[v];
       const VT uv = ((zn - kdv * tv) - (VT)0) / dtv[j];
-      S.u[v] = uv;
+      vst(S.u, v, uv);
       const VT sn = tv + uv;
-      S.s[v] = sn;
-      dl += (double)zn * (double)sn;
+      vst(S.s, v, sn);
+      sacc<W>(dl, (double)zn * (double)sn);
     }
   }
-  g = block_tree(g, red);
+  g = stree<R, W>(g, lrows, red);
   if (threadIdx.x == 0) S.part_rz[wslot * S.ntiles + tile] = g;
   if (mid) {
-    dl = block_tree(dl, red);
+    dl = stree<R, W>(dl, lrows, red);
     if (threadIdx.x == 0) S.part_pq[wslot * S.ntiles + tile] = dl;
     if ((flags & kLast) && !init && lead && threadIdx.x == 0) st->k = st->kk + 1;
   }
 }
 
-template <class VT, int R>
-__global__ void __launch_bounds__(256) k_ssor_bwd(const SsorSysT<VT>* __restrict__ sys, int tile0, int c, int flags) {
+template <class VT, int R, bool W = false>
+__global__ void __launch_bounds__(W ? kTpb * R : kTpb)
+    k_ssor_bwd(const SsorSysT<VT>* __restrict__ sys, int tile0, int c, int flags) {
   __shared__ double red[4];
-  ssor_bwd_body<VT, R>(sview(sys), tile0 + xcd_block(blockIdx.x, gridDim.x), c, flags, blockIdx.x == 0, red);
+  __shared__ double lrows[W ? kTpb * R : 1];
+  ssor_bwd_body<VT, R, W>(sview(sys), tile0 + xcd_block(blockIdx.x, gridDim.x), c, flags, blockIdx.x == 0, red,
+                          lrows);
 }
 
 // Several systems in one launch: entry blockIdx of the table names the system, tile, colour and
@@ -368,51 +409,55 @@ __global__ void __launch_bounds__(256) k_ssor_bwd_batch(const SsorSysT<VT>* __re
   __shared__ double red[4];
   const SsorRef ref = refs[xcd_block(blockIdx.x, gridDim.x)];
   const int f = ref.flags | or_flags;
-  ssor_bwd_body<VT, R>(sview(sys + ref.sys), ref.tile, ref.c, f, (f & kLead) != 0, red);
+  ssor_bwd_body<VT, R>(sview(sys + ref.sys), ref.tile, ref.c, f, (f & kLead) != 0, red, nullptr);
 }
 
 // ------------------------------------------------------------------ forward sweep
 // u = (D~ + L)^-1 (z - K t) for one colour, s = t + u = A^ z, partials of z.s. The last sweep of
 // an iteration advances the iteration counter.
-template <class VT, int R>
+template <class VT, int R, bool W = false>
 __device__ __forceinline__ void ssor_fwd_body(const SsorView<VT>& S, int tile, int c, int flags, bool lead,
-                                              double* red) {
+                                              double* red, double* lrows) {
+  constexpr int J = W ? 1 : R;  // rows of this thread
   const ColourInfo ci = colour_info(S, c);
   const gptr<const VT> u = C(S.u);
-  VT lsum[R], dtv[R], tv[R], rv[R], kdv[R];
+  VT lsum[J], dtv[J], tv[J], rv[J], kdv[J];
   // the row loads are issued before the done test so that their latency overlaps it
 #pragma unroll
-  for (int j = 0; j < R; ++j) {
-    const int v = tile_row<R>(tile, j);
-    dtv[j] = S.dt[v];
-    tv[j] = S.t[v];
-    rv[j] = S.r[v];
-    kdv[j] = S.unit_omega ? dtv[j] : S.kd[v];
+  for (int j = 0; j < J; ++j) {
+    const int v = srow<R, W>(tile, j);
+    dtv[j] = vld(S.dt, v);
+    tv[j] = vld(S.t, v);
+    rv[j] = vld(S.r, v);
+    kdv[j] = S.unit_omega ? dtv[j] : vld(S.kd, v);
   }
   if (load_done(S.st) >= 0) return;
   const int wslot = (flags & kInit) ? 0 : ((S.st->kk + 1) & 1);
 #pragma unroll
-  for (int j = 0; j < R; ++j) lsum[j] = lower_sum(S, ci, tile_row<R>(tile, j), u);
+  for (int j = 0; j < J; ++j) lsum[j] = lower_sum(S, ci, srow<R, W>(tile, j), u);
   double dl = 0.0;
 #pragma unroll
-  for (int j = 0; j < R; ++j) {
-    const int v = tile_row<R>(tile, j);
+  for (int j = 0; j < J; ++j) {
+    const int v = srow<R, W>(tile, j);
     const VT zn = dtv[j] * rv[j];
     const VT uv = ((zn - kdv[j] * tv[j]) - lsum[j]) / dtv[j];
-    S.u[v] = uv;
+    vst(S.u, v, uv);
     const VT sn = tv[j] + uv;
-    S.s[v] = sn;
-    dl += (double)zn * (double)sn;
+    vst(S.s, v, sn);
+    sacc<W>(dl, (double)zn * (double)sn);
   }
-  dl = block_tree(dl, red);
+  dl = stree<R, W>(dl, lrows, red);
   if (threadIdx.x == 0) S.part_pq[wslot * S.ntiles + tile] = dl;
   if ((flags & kLast) && !(flags & kInit) && lead && threadIdx.x == 0) S.st->k = S.st->kk + 1;
 }
 
-template <class VT, int R>
-__global__ void __launch_bounds__(256) k_ssor_fwd(const SsorSysT<VT>* __restrict__ sys, int tile0, int c, int flags) {
+template <class VT, int R, bool W = false>
+__global__ void __launch_bounds__(W ? kTpb * R : kTpb)
+    k_ssor_fwd(const SsorSysT<VT>* __restrict__ sys, int tile0, int c, int flags) {
   __shared__ double red[4];
-  ssor_fwd_body<VT, R>(sview(sys), tile0 + xcd_block(blockIdx.x, gridDim.x), c, flags, blockIdx.x == 0, red);
+  __shared__ double lrows[W ? kTpb * R : 1];
+  ssor_fwd_body<VT, R, W>(sview(sys), tile0 + xcd_block(blockIdx.x, gridDim.x), c, flags, blockIdx.x == 0, red,
+                          lrows);
 }
 
 template <class VT, int R>
@@ -421,7 +466,7 @@ __global__ void __launch_bounds__(256) k_ssor_fwd_batch(const SsorSysT<VT>* __re
   __shared__ double red[4];
   const SsorRef ref = refs[xcd_block(blockIdx.x, gridDim.x)];
   const int f = ref.flags | or_flags;
-  ssor_fwd_body<VT, R>(sview(sys + ref.sys), ref.tile, ref.c, f, (f & kLead) != 0, red);
+  ssor_fwd_body<VT, R>(sview(sys + ref.sys), ref.tile, ref.c, f, (f & kLead) != 0, red, nullptr);
 }
 
 // ------------------------------------------------------------------ stopping test
@@ -524,23 +569,6 @@ __global__ void k_ssor_reset(const SsorSysT<VT>* __restrict__ sys, double rtol, 
 }
 
 // ------------------------------------------------------------------ launch wrappers
-// Launches KER<rpt>; with events e0/e1 the runtime stamps the kernel's own start and end
-// (hipExtLaunchKernelGGL), which the profiling path reads; without them a plain launch, which
-// stream capture records into window graphs.
-#define GSH_SSOR_EXT(R, KER, GRID, E0, E1, STREAM, ...)                                               \
-  do {                                                                                               \
-    if (E0 || E1)                                                                                    \
-      hipExtLaunchKernelGGL((KER<VT, R>), dim3(GRID), dim3(kTpb), 0, STREAM, E0, E1, 0, __VA_ARGS__);       \
-    else                                                                                             \
-      hipLaunchKernelGGL((KER<VT, R>), dim3(GRID), dim3(kTpb), 0, STREAM, __VA_ARGS__);                     \
-  } while (0)
-#define GSH_SSOR_RX(rpt, KER, GRID, E0, E1, STREAM, ...)                         \
-  switch (rpt) {                                                               \
-    case 1: GSH_SSOR_EXT(1, KER, GRID, E0, E1, STREAM, __VA_ARGS__); break;    \
-    case 2: GSH_SSOR_EXT(2, KER, GRID, E0, E1, STREAM, __VA_ARGS__); break;    \
-    case 8: GSH_SSOR_EXT(8, KER, GRID, E0, E1, STREAM, __VA_ARGS__); break;    \
-    default: GSH_SSOR_EXT(4, KER, GRID, E0, E1, STREAM, __VA_ARGS__); break;   \
-  }
 #define GSH_SSOR_R(rpt, KER, GRID, STREAM, ...)                                          \
   switch (rpt) {                                                                       \
     case 1: hipLaunchKernelGGL((KER<VT, 1>), dim3(GRID), dim3(kTpb), 0, STREAM, __VA_ARGS__); break;   \
@@ -565,15 +593,40 @@ template <class VT>
 void launch_ssor_r0(const SsorSysT<VT>* sys, int tile0, int ntc, int c, int rpt, hipStream_t s) {
   if (ntc > 0) GSH_SSOR_R(rpt, k_ssor_r0, ntc, s, sys, tile0, c);
 }
+// The single-system sweeps run in wide workgroups (256 * rpt threads, one row each) for rpt 2 and
+// 4 (GSH_SSOR_WIDE = 0 builds the 256-thread form everywhere; both give the same bits). With
+// events e0/e1 the runtime stamps the kernel's own start and end (hipExtLaunchKernelGGL), which
+// the profiling path reads; without them a plain launch, which stream capture records into
+// window graphs. At 1M nodes: 86.4 against 84.1 solves/s at rpt 2, 72.2 against 69.7 at rpt 4.
+#ifndef GSH_SSOR_WIDE
+#define GSH_SSOR_WIDE 1
+#endif
+#define GSH_SSOR_WX(R, KER, GRID, E0, E1, STREAM, ...)                                                       \
+  do {                                                                                                      \
+    constexpr bool W_ = GSH_SSOR_WIDE && (R == 2 || R == 4);                                                \
+    const dim3 blk(W_ ? kTpb * R : kTpb);                                                                   \
+    if (E0 || E1)                                                                                           \
+      hipExtLaunchKernelGGL((KER<VT, R, W_>), dim3(GRID), blk, 0, STREAM, E0, E1, 0, __VA_ARGS__);          \
+    else                                                                                                    \
+      hipLaunchKernelGGL((KER<VT, R, W_>), dim3(GRID), blk, 0, STREAM, __VA_ARGS__);                        \
+  } while (0)
+#define GSH_SSOR_WRX(rpt, KER, GRID, E0, E1, STREAM, ...)                        \
+  switch (rpt) {                                                               \
+    case 1: GSH_SSOR_WX(1, KER, GRID, E0, E1, STREAM, __VA_ARGS__); break;     \
+    case 2: GSH_SSOR_WX(2, KER, GRID, E0, E1, STREAM, __VA_ARGS__); break;     \
+    case 8: GSH_SSOR_WX(8, KER, GRID, E0, E1, STREAM, __VA_ARGS__); break;     \
+    default: GSH_SSOR_WX(4, KER, GRID, E0, E1, STREAM, __VA_ARGS__); break;    \
+  }
+
 template <class VT>
 void launch_ssor_bwd(const SsorSysT<VT>* sys, int tile0, int ntc, int c, int flags, int rpt, hipStream_t s,
                      hipEvent_t e0, hipEvent_t e1) {
-  if (ntc > 0) GSH_SSOR_RX(rpt, k_ssor_bwd, ntc, e0, e1, s, sys, tile0, c, flags);
+  if (ntc > 0) GSH_SSOR_WRX(rpt, k_ssor_bwd, ntc, e0, e1, s, sys, tile0, c, flags);
 }
 template <class VT>
 void launch_ssor_fwd(const SsorSysT<VT>* sys, int tile0, int ntc, int c, int flags, int rpt, hipStream_t s,
                      hipEvent_t e0, hipEvent_t e1) {
-  if (ntc > 0) GSH_SSOR_RX(rpt, k_ssor_fwd, ntc, e0, e1, s, sys, tile0, c, flags);
+  if (ntc > 0) GSH_SSOR_WRX(rpt, k_ssor_fwd, ntc, e0, e1, s, sys, tile0, c, flags);
 }
 template <class VT>
 void launch_ssor_true_rr(const SsorSysT<VT>* sys, int ntiles, int rpt, hipStream_t s) {

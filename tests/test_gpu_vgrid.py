@@ -92,3 +92,54 @@ def test_chain_lsc2_per_node_sigma(ctx):
     mesh, om, p = _setup_vgrid(ctx, spec, depth, vg, kb=kb)
     assert (mesh.topology(L.T_KB) > 0).any()
     _full_chain(ctx, mesh, om, p)
+
+
+def test_run_like_case_from_files(ctx, tmp_path):
+    """A case shaped like the reference's configured run (constants.h:53, 123-126, 140, 166:
+    ivcor=2 with one Z level, nvrt=23, dt=300 s, theta=0.51, 10-30 s ELM substeps): a 129 x 41
+    channel at 500 m spacing written as hgrid.gr3, vgrid.in (SZ) and bctides.in (a constant
+    elevation on the open boundary), read back through the library's readers and run through a
+    whole step against the oracle, bit for bit."""
+    from gschism_amd.bctides import apply_bctides, read_bctides
+    from gschism_amd.grid_files import read_grid, write_hgrid, write_vgrid_sz
+    from test_cpu_grid_files import _boundary_segments
+
+    xy, en, ob = cases.channel(129, 41, 500.0)
+    x = (xy[:, 0] - xy[:, 0].min()) / np.ptp(xy[:, 0])
+    depth = 4.0 + 46.0 * x + 3.0 * np.sin(xy[:, 1] / 1500.0)
+    opens, lands = _boundary_segments(xy, en, ob)
+    hpath, vpath, bpath = (str(tmp_path / f) for f in ("hgrid.gr3", "vgrid.in", "bctides.in"))
+    write_hgrid(hpath, xy, depth, en, opens, lands, name="run-like channel 129x41")
+    nvrt = 23
+    sigma_s = -1.0 + np.linspace(0.0, 1.0, nvrt) ** 1.2
+    sigma_s[0], sigma_s[-1] = -1.0, 0.0
+    write_vgrid_sz(vpath, nvrt, [-100.0], 100.0, 30.0, 0.7, 5.0, sigma_s)
+    k = opens[0].size
+    (tmp_path / "bctides.in").write_text("\n".join(["01/01/2020 00:00:00", "0 40.", "0", "1",
+                                                    f"{k} 2 0 0 0", "0.25"]) + "\n")
+    g = read_grid(hpath, vpath)
+    assert g.vgrid.ivcor == 2 and g.vgrid.nz == 1 and g.vgrid.nvrt == nvrt
+    mesh = g.mesh(ctx)
+    apply_bctides(mesh, read_bctides(bpath, 1, k), g.open_segments)
+    kb = mesh.topology(L.T_KB)
+    assert (kb == 0).all()
+    spec = cases.CaseSpec("run-like-129x41", 129, 41, 500.0, nvrt=nvrt, dt=300.0, theta=0.51)
+    p = L.default_step_params(dt=300.0, theta=0.51, dtb_min=10.0, dtb_max=30.0)
+    xym = mesh.topology(L.T_XY)
+    eta, uvw, su2 = cases.initial_state(spec, xym, cases.side_centres(mesh), np.linspace(-1.0, 0.0, nvrt))
+    for f, v in ((L.F_ETA2, eta), (L.F_NODE_UVW, uvw), (L.F_SIDE_U2, su2)):
+        mesh.set(f, v)
+    mesh.update_levels(p)
+    om = O.OracleMesh(xym, mesh.topology(L.T_ELNODE), mesh.topology(L.T_DEPTH), mesh.topology(L.T_OPEN_BND),
+                      np.zeros(nvrt), kb=kb)
+    om.set_vertical(mesh.vgrid)
+    for f, v in ((L.F_ETA2, eta), (L.F_NODE_UVW, uvw), (L.F_SIDE_U2, su2), (L.F_ETYPE, mesh.get(L.F_ETYPE)),
+                 (L.F_ETH, mesh.get(L.F_ETH))):
+        om.set(f, v)
+    om.levels(p)
+    h = mesh.topology(L.T_DEPTH)
+    assert (h > 30.0).any() and (h < 30.0).any()  # both branches of the S levels
+    print(f"[gpu] run-like case from files: n={mesh.n} ns={mesh.ns} nvrt={nvrt}", flush=True)
+    _full_chain(ctx, mesh, om, p)
+    st = mesh.get(L.F_BSTATUS)
+    assert (st & L.BT_OPEN_EXIT).any() or (st & L.BT_SLIDE).any()
