@@ -167,6 +167,37 @@ def test_ssor_2m_matches_oracle(ctx):
         s.close()
 
 
+def test_mixed_4m_against_oracle(ctx):
+    """Configuration 3: the 2048 x 2048 log-normal channel (4.2M nodes), mixed precision (FP32 SSOR
+    inner solves, FP64 residual and refinement) to the outer tolerance 1e-10, against the oracle's
+    FP64 SSOR PCG of the same system (rtol 1e-12): relative L2 error <= 1e-8, the FP64 true
+    residual within the outer tolerance; iteration counts printed beside the FP64 solver's."""
+    spec = cases.spec_mixed_4m()
+    data = cases.make_case(spec)
+    print(f"[gpu] mixed 4M: building {spec.name}", flush=True)
+    mesh, p, _ = cases.load_mesh(ctx, data)
+    mesh.backtrack(p)
+    mesh.rhs_producers(p)
+    ip, ix = mesh.topology(L.T_CSR_INDPTR), mesh.topology(L.T_CSR_INDICES)
+    sm = core.Solver(mesh, L.PREC_MIXED, precond=L.PRECOND_SSOR)
+    sm.set_tuning(1)
+    sm.assemble(p)
+    val, b = sm.get_values(), sm.get_rhs()
+    xm, itm, stm = sm.solve(None, rtol=1e-10, maxiter=5000, check_every=10)
+    stt = sm.stats()
+    assert stm == 0 and stt.rr <= stt.atol2
+    print(f"[gpu] mixed 4M: oracle FP64 SSOR solve (n={mesh.n})", flush=True)
+    xo, ito, _, _, _ = O.pcg_ssor(ip, ix, val, b, rtol=1e-12, maxiter=5000, rows_per_thread=1, nthreads=16)
+    err = np.linalg.norm(xm - xo) / np.linalg.norm(xo)
+    A = sp.csr_matrix((val, ix, ip), shape=(mesh.n, mesh.n))
+    rel = np.linalg.norm(b - A @ xm) / np.linalg.norm(b)
+    print(f"[gpu] mixed 4M: {stt.outer} outer, {itm} inner FP32 iterations; oracle FP64 SSOR {ito} iterations; "
+          f"rel L2 vs oracle {err:.3e}, true rel residual {rel:.3e}", flush=True)
+    assert err <= 1e-8, err
+    assert rel <= 1.01e-10, rel
+    sm.close()
+
+
 def test_window_graphs_replay_exactly(ctx):
     """With window graphs on, repeated solves (graph replays with eager work in between) give the
     oracle's solutions, for Jacobi and SSOR."""
