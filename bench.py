@@ -87,7 +87,8 @@ def ssor_bytes(mesh, solver):
 def batched_leg(ctx, L, cases, core, members, nx, ny, rtol, mixed_rtol, check_every):
     """Ensemble: `members` systems on one nx x ny mesh, bathymetry h (1 + 0.05 N(0,1)) per member
     (seeds 100 + m), one random right-hand side each (seeds 200 + m). Plain = one FP64 Jacobi solve
-    after another; batched = all members in one lock-step launch per PCG phase; mixed = FP32 inner
+    after another; batched = all members in one lock-step launch per PCG phase; block = all members
+    in one member-interleaved Jacobi PCG over the shared CSR pattern (gsh_block); mixed = FP32 inner
     iterations with FP64 residual and refinement."""
     spec = cases.CaseSpec(f"ensemble-{nx}x{ny}", nx, ny, 1000.0, nvrt=11, depth=None, substeps=10)
     data = cases.make_case(spec)
@@ -162,7 +163,24 @@ def batched_leg(ctx, L, cases, core, members, nx, ny, rtol, mixed_rtol, check_ev
     err_ssm = max(float(np.linalg.norm(d.get_solution() - x) / np.linalg.norm(x)) for d, x in zip(ssm, x64))
     for d in sss + ssm:
         d.close()
+    # the multi-RHS block: all members in one member-interleaved Jacobi PCG over the shared pattern
+    blk = core.Block(mesh, members)
+    for m, a in enumerate(s64):
+        blk.set_member(m, a.get_values(), a.get_rhs())
+    blk.solve(rtol, 20000, check_every)  # warm
+    ctx.synchronize()
+    t = time.perf_counter()
+    it_blk, st_blk = blk.solve(rtol, 20000, check_every)
+    ctx.synchronize()
+    t_blk = time.perf_counter() - t
+    err_blk = max(float(np.linalg.norm(blk.solution(m) - x) / np.linalg.norm(x)) for m, x in enumerate(x64))
+    blk.close()
     out = {"workload": f"{members} members on {spec.name} (n={mesh.n}), per-member bathymetry and RHS",
+           "block_fp64_solves_per_s": round(members / t_blk, 3),
+           "block_fp64_vs_plain": round(t_plain / t_blk, 3),
+           "block_fp64_vs_batched_fp64": round(t_b64 / t_blk, 3),
+           "block_iters_equal_to_plain": bool(list(map(int, it_blk)) == list(map(int, it_plain))),
+           "block_max_rel_l2_vs_plain": err_blk,
            "plain_fp64_solves_per_s": round(members / t_plain, 3),
            "plain_fp64_ssor_solves_per_s": round(ssor_rate, 3),
            "batched_fp64_solves_per_s": round(members / t_b64, 3),

@@ -145,6 +145,12 @@ SIGNATURES = {
     "gsh_solver_set_tuning": (_I, [_P, _I, _D]),
     "gsh_update_elevation": (_I, [_P, _P]),
     "gsh_batch_solve": (_I, [_PP, _I, _D, _I, _I, _D, C.POINTER(_I)]),
+    "gsh_block_create": (_I, [_P, _I32, _PP]),
+    "gsh_block_destroy": (_I, [_P]),
+    "gsh_block_assemble": (_I, [_P, _I32, C.POINTER(StepParams), _D]),
+    "gsh_block_set_member": (_I, [_P, _I32, _PD, _PD]),
+    "gsh_block_solve": (_I, [_P, _D, _I, _I, _D, _PI32, _PI32]),
+    "gsh_block_get_solution": (_I, [_P, _I32, _PD]),
     "gsh_server_create": (_I, [_P, C.POINTER(ServerConfig), _PP]),
     "gsh_server_destroy": (_I, [_P]),
     "gsh_server_add_mesh": (_I, [_P, _P, _PI32]),

@@ -27,7 +27,7 @@ COMMON = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "
 HIP_FLAGS = ["-x", "hip", f"--offload-arch={ARCH}", "-Xclang", "-target-feature", "-Xclang",
              "-packed-fp32-ops", "-fno-gpu-rdc"]
 HOST_SOURCES = ["gen.cpp", "topology.cpp", "ssor_layout.cpp", "grid_io.cpp"]
-HIP_SOURCES = ["mesh_kernels.hip", "pcg_kernels.hip", "ssor_kernels.hip", "api.hip"]
+HIP_SOURCES = ["mesh_kernels.hip", "pcg_kernels.hip", "ssor_kernels.hip", "block_kernels.hip", "api.hip"]
 HEADERS = ["host_mesh.h", "abi_util.h", "context.h", "device.h", "reduce.h", "ssor_layout.h"]
 
 

@@ -403,6 +403,48 @@ def batch_solve(solvers, rtol=1e-12, maxiter=0, check_every=10, timeout_s=60.0):
     return list(its)
 
 
+class Block:
+    """Ensemble members of one mesh solved together on its shared CSR pattern (gsh_block):
+    Jacobi PCG per member with the reference's recurrences, member-interleaved on the device."""
+
+    def __init__(self, mesh: Mesh, members: int):
+        h = C.c_void_p()
+        check(L.lib().gsh_block_create(mesh.h, members, C.byref(h)))
+        self.h = h
+        self.mesh = mesh
+        self.members = members
+        _register(self)
+
+    def close(self):
+        if getattr(self, "h", None):
+            L.lib().gsh_block_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def assemble(self, member: int, params, time: float = 0.0) -> None:
+        check(L.lib().gsh_block_assemble(self.h, member, C.byref(params), time))
+
+    def set_member(self, member: int, values=None, b=None) -> None:
+        v = None if values is None else np.ascontiguousarray(values, dtype=np.float64)
+        r = None if b is None else np.ascontiguousarray(b, dtype=np.float64)
+        check(L.lib().gsh_block_set_member(self.h, member, dptr(v) if v is not None else None,
+                                           dptr(r) if r is not None else None))
+
+    def solve(self, rtol: float = 1e-12, maxiter: int = 0, check_every: int = 10, timeout_s: float = 60.0):
+        it = np.empty(self.members, dtype=np.int32)
+        st = np.empty(self.members, dtype=np.int32)
+        check(L.lib().gsh_block_solve(self.h, rtol, maxiter, check_every, timeout_s, dptr(it, C.c_int32),
+                                      dptr(st, C.c_int32)))
+        return it, st
+
+    def solution(self, member: int) -> np.ndarray:
+        x = np.empty(self.mesh.n)
+        check(L.lib().gsh_block_get_solution(self.h, member, dptr(x)))
+        return x
+
+
 class Server:
     """Continuous-batching solve service (gsh_server): requests join and leave the lock-step
     PCG launches at check-window boundaries."""

@@ -2780,6 +2780,166 @@ int gsh_server_poll(gsh_server* srv, int64_t ticket, double* x_host, int32_t* it
 
 }  // extern "C"
 
+// ====================================================================== multi-RHS block
+// Ensemble members of one mesh solved together on its shared CSR pattern (member-interleaved
+// layout, block_kernels.hip).
+struct gsh_block {
+  gsh_ctx* ctx = nullptr;
+  gsh_mesh* mesh = nullptr;
+  int32_t members = 0, n = 0;
+  int64_t nnz = 0;
+  DevPool pool;
+  double *tmp_val = nullptr, *tmp_b = nullptr;  // one member's assembled values and RHS
+  double *vblk = nullptr, *bblk = nullptr;      // [nnz][M], [n][M]
+  BlkSys<double> S{};
+  std::vector<BlkState> last;                   // member states of the last solve
+};
+
+static int32_t blk_width(int32_t members) {
+  int32_t M = 1;
+  while (M < members) M *= 2;
+  return M;
+}
+
+extern "C" {
+
+int gsh_block_create(gsh_mesh* mesh, int32_t members, gsh_block** out) {
+  GSH_API_BODY({
+    GSH_REQUIRE(mesh && out, GSH_E_ARG, "NULL argument");
+    GSH_REQUIRE(mesh->ctx, GSH_E_STATE, "host-only mesh (created without a context)");
+    GSH_REQUIRE(members >= 1 && members <= 256, GSH_E_ARG, "members must be in [1, 256]");
+    mesh->ctx->set();
+    auto b = std::make_unique<gsh_block>();
+    b->ctx = mesh->ctx;
+    b->mesh = mesh;
+    b->members = members;
+    const Topology& t = mesh->topo;
+    const int32_t n = mesh->hm.n, M = blk_width(members);
+    const int64_t nnz = (int64_t)t.indices.size();
+    b->n = n;
+    b->nnz = nnz;
+    DevPool& P = b->pool;
+    BlkSys<double>& S = b->S;
+    S.n = n;
+    S.M = M;
+    S.ntiles = (n + kBlkTile - 1) / kBlkTile;
+    S.rowptr = P.upload(t.indptr);
+    S.col = P.upload(t.indices);
+    S.diag = P.upload(diag_abs_of(t.indptr, t.indices, n));
+    b->vblk = P.alloc<double>(nnz * M);
+    b->bblk = P.alloc<double>((int64_t)n * M);
+    S.val = b->vblk;
+    S.b = b->bblk;
+    for (double** v : {&S.x, &S.r, &S.z, &S.pa, &S.pb, &S.q, &S.invd}) *v = P.alloc<double>((int64_t)n * M);
+    S.part_a = P.alloc<double>((int64_t)S.ntiles * M);
+    S.part_b = P.alloc<double>((int64_t)S.ntiles * M);
+    S.st = P.alloc<BlkState>(M);
+    b->tmp_val = P.alloc<double>(nnz);
+    b->tmp_b = P.alloc<double>(n);
+    // members beyond `members` (the block is a power of two wide): unit values, zero RHS, so they
+    // stop at once with x = 0
+    launch_fill_f64(b->vblk, 1.0, nnz * M, b->ctx->stream);
+    HIPCHK(hipStreamSynchronize(b->ctx->stream));
+    ctx_retain(b->ctx);
+    *out = b.release();
+  })
+}
+
+int gsh_block_destroy(gsh_block* b) {
+  if (!b) return GSH_OK;
+  gsh_ctx* ctx = b->ctx;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  delete b;
+  ctx_release(ctx);
+  return GSH_OK;
+}
+
+int gsh_block_assemble(gsh_block* b, int32_t member, const gsh_step_params* p, double time) {
+  (void)time;  // as gsh_assemble
+  GSH_API_BODY({
+    GSH_REQUIRE(b && p, GSH_E_ARG, "NULL argument");
+    GSH_REQUIRE(member >= 0 && member < b->members, GSH_E_ARG, "member out of range");
+    b->ctx->set();
+    hipStream_t st = b->ctx->stream;
+    launch_assemble(b->mesh->d, step_args(p), b->tmp_val, b->tmp_b, st);
+    launch_blk_put(b->vblk, b->bblk, b->tmp_val, b->tmp_b, b->nnz, b->n, b->S.M, member, st);
+    HIPCHK(hipGetLastError());
+  })
+}
+
+int gsh_block_set_member(gsh_block* b, int32_t member, const double* values_host, const double* b_host) {
+  GSH_API_BODY({
+    GSH_REQUIRE(b, GSH_E_ARG, "NULL argument");
+    GSH_REQUIRE(member >= 0 && member < b->members, GSH_E_ARG, "member out of range");
+    b->ctx->set();
+    hipStream_t st = b->ctx->stream;
+    if (values_host) HIPCHK(hipMemcpyAsync(b->tmp_val, values_host, 8 * b->nnz, hipMemcpyHostToDevice, st));
+    if (b_host) HIPCHK(hipMemcpyAsync(b->tmp_b, b_host, 8 * (size_t)b->n, hipMemcpyHostToDevice, st));
+    launch_blk_put(b->vblk, b->bblk, values_host ? b->tmp_val : nullptr, b_host ? b->tmp_b : nullptr, b->nnz, b->n,
+                   b->S.M, member, st);
+    HIPCHK(hipStreamSynchronize(st));
+  })
+}
+
+int gsh_block_solve(gsh_block* b, double rtol, int maxiter, int check_every, double timeout_s, int32_t* iters_out,
+                    int32_t* status_out) {
+  GSH_API_BODY({
+    GSH_REQUIRE(b, GSH_E_ARG, "NULL argument");
+    GSH_REQUIRE(rtol >= 0.0, GSH_E_ARG, "rtol must be >= 0");
+    if (maxiter <= 0) maxiter = 10 * b->n;  // pcg_solver.py:1082-1083
+    if (check_every < 1) check_every = 1;
+    if (timeout_s <= 0.0) timeout_s = 60.0;
+    gsh_ctx* ctx = b->ctx;
+    ctx->set();
+    hipStream_t st = ctx->stream;
+    const BlkSys<double>& S = b->S;
+    const int M = S.M;
+    b->last.assign(M, BlkState{});
+    // one readback slot of M states per check window
+    BlkState* host = reinterpret_cast<BlkState*>(ctx->pinned_slots((M * (int)sizeof(BlkState) + sizeof(PcgState) - 1) /
+                                                                   sizeof(PcgState)));
+    launch_blk_init(S, st);
+    launch_blk_reduce_init(S, rtol, st);
+    HIPCHK(hipGetLastError());
+    const auto t0 = Clock::now();
+    int it = 0;
+    bool all_done = false;
+    while (it < maxiter && !all_done) {
+      const int w = std::min(check_every, maxiter - it);
+      for (int i = 0; i < w; ++i, ++it) launch_blk_iter(S, it, rtol, maxiter, check_every, st);
+      HIPCHK(hipMemcpyAsync(host, S.st, sizeof(BlkState) * M, hipMemcpyDeviceToHost, st));
+      HIPCHK(hipEventRecord(ctx->ev[0], st));
+      wait_event(ctx->ev[0], timeout_s, "block PCG window");
+      all_done = true;
+      for (int m = 0; m < b->members; ++m) all_done = all_done && host[m].done >= 0;
+      if (std::chrono::duration<double>(Clock::now() - t0).count() > timeout_s)
+        throw Error(GSH_E_TIMEOUT, "block PCG: time limit exceeded");
+      HIPCHK(hipGetLastError());
+    }
+    b->last.assign(host, host + M);
+    for (int m = 0; m < b->members; ++m) {
+      const BlkState& s = b->last[m];
+      if (iters_out) iters_out[m] = s.done >= 0 ? s.done : maxiter;
+      if (status_out) status_out[m] = s.done >= 0 ? 0 : GSH_W_MAXITER;
+    }
+  })
+}
+
+int gsh_block_get_solution(gsh_block* b, int32_t member, double* x_host) {
+  GSH_API_BODY({
+    GSH_REQUIRE(b && x_host, GSH_E_ARG, "NULL argument");
+    GSH_REQUIRE(member >= 0 && member < b->members, GSH_E_ARG, "member out of range");
+    b->ctx->set();
+    hipStream_t st = b->ctx->stream;
+    launch_blk_get(b->tmp_b, b->S.x, b->n, b->S.M, member, st);
+    HIPCHK(hipMemcpyAsync(x_host, b->tmp_b, 8 * (size_t)b->n, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+  })
+}
+
+}  // extern "C"
+
 // ====================================================================== measurement helpers
 namespace gsh {
 __global__ void k_copy_probe(const double4* __restrict__ src, double4* __restrict__ dst, int64_t n4) {

@@ -179,6 +179,30 @@ struct TileRef {
   int32_t sys, tile;
 };
 
+// ---------------------------------------------------------------- multi-RHS block (block_kernels.hip)
+constexpr int kBlkTile = 512;  // rows per workgroup tile of the block kernels
+
+// Scalar state of one member of a block solve.
+struct BlkState {
+  int32_t done, zero, pad0, pad1;  // done: iteration count once stopped, -1 before
+  double atol2, rz_cur, rz_old, alpha, beta, rr_last;
+};
+
+// M systems on one CSR pattern, member-interleaved: vectors [n][M], values [nnz][M].
+template <class VT>
+struct BlkSys {
+  int32_t n, M, ntiles, pad;
+  const int32_t* rowptr;
+  const int32_t* col;
+  const int32_t* diag;     // absolute index of each row's diagonal entry
+  const VT* val;           // [nnz][M]
+  const double* b;         // [n][M]
+  VT *x, *r, *z, *pa, *pb, *q, *invd;
+  double* part_a;          // [ntiles][M]
+  double* part_b;          // [ntiles][M]
+  BlkState* st;            // [M]
+};
+
 // ---------------------------------------------------------------- launch wrappers
 void launch_levels(const DMesh& m, const StepArgs& a, hipStream_t s);
 void launch_backtrack(const DMesh& m, const StepArgs& a, const int32_t* list_side, const int32_t* list_lev,
@@ -209,6 +233,18 @@ void launch_pcg_finish(const PcgSys<VT>* sys, const TileRef* tiles, int ntiles, 
 void launch_true_residual(const PcgSys<double>* sys, const TileRef* tiles, int ntiles, int rpt, const double* xv,
                           double* rvec, hipStream_t s);
 void launch_reduce_pair(const double* pa, const double* pb, int ntiles, double* out2, hipStream_t s);
+
+// multi-RHS block (block_kernels.hip)
+template <class VT>
+void launch_blk_init(const BlkSys<VT>& S, hipStream_t st);
+template <class VT>
+void launch_blk_reduce_init(const BlkSys<VT>& S, double rtol, hipStream_t st);
+template <class VT>
+void launch_blk_iter(const BlkSys<VT>& S, int it, double rtol, int maxiter, int check_every, hipStream_t st);
+void launch_blk_put(double* vblk, double* bblk, const double* v, const double* b, int64_t nnz, int n, int M, int m,
+                    hipStream_t st);
+template <class VT>
+void launch_blk_get(double* x, const VT* xblk, int n, int M, int m, hipStream_t st);
 
 // Multicolour SSOR PCG (one system). Tiles of colour c are [tile0, tile0 + ntc).
 template <class VT>

@@ -373,6 +373,29 @@ int gsh_update_elevation(gsh_mesh* mesh, gsh_solver* s);
 int gsh_batch_solve(gsh_solver* const* solvers, int count, double rtol, int maxiter,
                     int check_every, double timeout_s, int* iters_out);
 
+/* ---------------------------------------------------------------- multi-RHS block */
+/* No counterpart in the reference (it solves one system per call): `members` systems (1-256) on
+ * one mesh's CSR pattern, e.g. ensemble members with per-member bathymetry and right-hand side,
+ * solved together by Jacobi PCG with the reference's recurrences, cold start and stopping rule per
+ * member (pcg_solver.py:881-947, 1075-1124), in a member-interleaved layout that reads the pattern
+ * once for all members. Dot products are summed in the block's own fixed order, so each member's
+ * result equals the oracle's block restatement bit for bit and a single solve of that system to
+ * rounding. */
+typedef struct gsh_block gsh_block;
+int gsh_block_create(gsh_mesh* mesh, int32_t members, gsh_block** out);
+int gsh_block_destroy(gsh_block* b);
+/* setupCoefficientMatrixAndRHS + setEssentialBC (meshdata.cu:3182-3474, 4400-4446) of the mesh's
+ * current state into member `member`. */
+int gsh_block_assemble(gsh_block* b, int32_t member, const gsh_step_params* p, double time);
+/* A member's CSR values (nnz, the mesh's pattern) and/or right-hand side (n) from the host; NULL
+ * keeps the current one. */
+int gsh_block_set_member(gsh_block* b, int32_t member, const double* values_host, const double* b_host);
+/* Solves every member from x = 0: iters_out[m] (maxiter if not converged) and status_out[m]
+ * (GSH_W_MAXITER then); either may be NULL. timeout_s <= 0 means 60 s. */
+int gsh_block_solve(gsh_block* b, double rtol, int maxiter, int check_every, double timeout_s,
+                    int32_t* iters_out, int32_t* status_out);
+int gsh_block_get_solution(gsh_block* b, int32_t member, double* x_host);
+
 /* ---------------------------------------------------------------- serving */
 typedef struct gsh_server gsh_server;
 typedef struct gsh_server_config {

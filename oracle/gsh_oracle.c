@@ -16,6 +16,7 @@
  *   src/kernels/meshdata.cu:4459-4476, 4615-4725, 6183-6424, 4276-4339   RHS producers
  *   src/kernels/meshdata.cu:3182-3311, 3373-3380, 3457-3463, 4400-4446   assembly, I_5, essential BC
  *   src/solver/pcg_solver.py:172-276, 881-947, 1075-1124; pcg_loop.cpp:361-413   Jacobi PCG
+ *                                                                   (single, and a multi-RHS block)
  *   src/kernels/meshdata.cu:5765-5929                               nodal velocities
  *   src/kernels/meshdata.cu:5045-5446, 3665-3711, 5456-5518         momentum, Shapiro filter
  *   src/kernels/meshdata.cu:5978-6102, 559-631                      element vertical velocity
@@ -1880,6 +1881,124 @@ done:
   free(p);
   free(q);
   return result;
+}
+
+/* ------------------------------------------------------------------ multi-RHS block
+ * Jacobi PCG of M systems on one CSR pattern (values [nnz][M], b and x [n][M]) with the
+ * reference's recurrences per member (as or_pcg, cold start: x = 0, r = b), every dot product in
+ * the library's block order (block_kernels.hip): a tile of 512 rows, S = 256 / M slot threads per
+ * member, slot s adding rows s, s + S, ... of the tile to 0.0; the tile partial adding the slot
+ * sums in slot order; the total taking RS = 1024 / M slots, slot r adding tile partials r,
+ * r + RS, ... to 0.0, then the slot sums in order. */
+static double blk_dot(int32_t n, int32_t M, int32_t m, const double* a, const double* b, double* part) {
+  const int S = 256 / M, JT = 512 / S, RS = 1024 / M;
+  const int64_t T = ((int64_t)n + 511) / 512;
+  for (int64_t t = 0; t < T; ++t) {
+    double tp = 0.0;
+    for (int s = 0; s < S; ++s) {
+      double acc = 0.0;
+      for (int j = 0; j < JT; ++j) {
+        const int64_t i = 512 * t + s + (int64_t)S * j;
+        if (i >= n) break;
+        acc += a[i * M + m] * b[i * M + m];
+      }
+      tp += acc;
+    }
+    part[t] = tp;
+  }
+  double tot = 0.0;
+  for (int r = 0; r < RS; ++r) {
+    double u = 0.0;
+    for (int64_t t = r; t < T; t += RS) u += part[t];
+    tot += u;
+  }
+  return tot;
+}
+
+int or_pcg_block(int32_t n, const int32_t* indptr, const int32_t* indices, int32_t M, const double* val,
+                 const double* b, double rtol, int32_t maxiter, int32_t checkEvery, double* x, int32_t* iters,
+                 double* rr_last, int nthreads) {
+  if (M < 1 || M > 256 || (M & (M - 1))) {
+    set_err("M must be a power of two in [1, 256]");
+    return -1;
+  }
+#ifdef _OPENMP
+  if (nthreads > 0) omp_set_num_threads(nthreads);
+#endif
+  (void)nthreads;
+  if (maxiter <= 0) maxiter = 10 * n; /* pcg_solver.py:1082-1083 */
+  if (checkEvery < 1) checkEvery = 1;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1)
+#endif
+  for (int32_t m = 0; m < M; ++m) {
+    const int64_t T = ((int64_t)n + 511) / 512;
+    double* part = xcalloc(T, sizeof(double));
+    double *invD = xcalloc(n, sizeof(double)), *r = xcalloc((size_t)n * M, sizeof(double));
+    double *z = xcalloc((size_t)n * M, sizeof(double)), *p = xcalloc((size_t)n * M, sizeof(double));
+    double* q = xcalloc((size_t)n * M, sizeof(double));
+    /* computeInvDiag (pcg_solver.py:933-938) */
+    for (int32_t i = 0; i < n; ++i)
+      for (int32_t k = indptr[i]; k < indptr[i + 1]; ++k)
+        if (indices[k] == i) invD[i] = 1.0 / val[(int64_t)k * M + m];
+    /* computeAtol (pcg_solver.py:928-930) */
+    const double atol = rtol * sqrt(blk_dot(n, M, m, b, b, part));
+    const double atol2 = atol * atol;
+    rr_last[m] = -1.0;
+    for (int32_t i = 0; i < n; ++i) x[(int64_t)i * M + m] = 0.0;
+    if (atol == 0.0) { /* pcg_solver.py:1089-1095 */
+      iters[m] = 0;
+    } else {
+      for (int32_t i = 0; i < n; ++i) {
+        const int64_t e = (int64_t)i * M + m;
+        r[e] = b[e];
+        z[e] = invD[i] * r[e];
+      }
+      double rz_cur = blk_dot(n, M, m, r, z, part), rz_old = 0.0;
+      int32_t result = maxiter;
+      for (int32_t it = 0; it < maxiter; ++it) {
+        const double beta = (it == 0 || rz_old == 0.0) ? 0.0 : rz_cur / rz_old;
+        for (int32_t i = 0; i < n; ++i) {
+          const int64_t e = (int64_t)i * M + m;
+          p[e] = z[e] + beta * p[e];
+        }
+        for (int32_t i = 0; i < n; ++i) {
+          double sq = 0.0;
+          for (int32_t k = indptr[i]; k < indptr[i + 1]; ++k)
+            sq += val[(int64_t)k * M + m] * p[(int64_t)indices[k] * M + m];
+          q[(int64_t)i * M + m] = sq;
+        }
+        const double pq = blk_dot(n, M, m, p, q, part);
+        const double alpha = (pq != 0.0) ? rz_cur / pq : 0.0;
+        for (int32_t i = 0; i < n; ++i) {
+          const int64_t e = (int64_t)i * M + m;
+          x[e] = x[e] + alpha * p[e];
+          const double ri = r[e] - alpha * q[e];
+          r[e] = ri;
+          z[e] = invD[i] * ri;
+        }
+        const double rr = blk_dot(n, M, m, r, r, part);
+        const double rz_new = blk_dot(n, M, m, r, z, part);
+        rz_old = rz_cur;
+        rz_cur = rz_new;
+        if ((it + 1) % checkEvery == 0 || it == maxiter - 1) {
+          rr_last[m] = rr;
+          if (rr <= atol2) {
+            result = it + 1;
+            break;
+          }
+        }
+      }
+      iters[m] = result;
+    }
+    free(part);
+    free(invD);
+    free(r);
+    free(z);
+    free(p);
+    free(q);
+  }
+  return 0;
 }
 
 double or_true_rel(int32_t n, const int32_t* indptr, const int32_t* indices, const double* val, const double* b,

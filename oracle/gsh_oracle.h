@@ -93,6 +93,12 @@ int32_t or_ssor_solve(or_ssor* s, const double* val, const double* b, const doub
                       const double* h2, double omega, double rtol, int32_t maxiter, int32_t check_every,
                       double* x_out, double* rr_last, double* atol2, int nthreads);
 void or_ssor_free(or_ssor* s);
+/* Jacobi PCG of M (a power of two <= 256) systems on one CSR pattern, member-interleaved (values
+ * [nnz][M], b and x [n][M]), cold start, dot products in the library's block order; iters[M],
+ * rr_last[M]. Returns 0, or -1 with or_error(). */
+int or_pcg_block(int32_t n, const int32_t* indptr, const int32_t* indices, int32_t M, const double* val,
+                 const double* b, double rtol, int32_t maxiter, int32_t check_every, double* x, int32_t* iters,
+                 double* rr_last, int nthreads);
 /* Dot product in the device's fixed reduction order. */
 double or_dot_canon(int32_t n, const double* a, const double* b, int32_t rows_per_thread);
 /* ||b - A x|| / ||b|| in the same order as the device check. */

@@ -47,6 +47,8 @@ _SIG = {
     "or_pcg": (C.c_int32, [C.c_int32, _PI32, _PI32, _PD, _PD, _PD, _PD, _PD, C.c_double, C.c_int32, C.c_int32,
                            C.c_int32, _PD, _PD, _PD, C.c_int]),
     "or_colour": (C.c_int32, [C.c_int32, _PI32, _PI32, _PI32]),
+    "or_pcg_block": (C.c_int, [C.c_int32, _PI32, _PI32, C.c_int32, _PD, _PD, C.c_double, C.c_int32, C.c_int32, _PD,
+                               _PI32, _PD, C.c_int]),
     "or_pcg_ssor": (C.c_int32, [C.c_int32, _PI32, _PI32, _PD, _PD, _PD, _PD, _PD, C.c_double, C.c_double,
                                 C.c_int32, C.c_int32, C.c_int32, _PD, _PD, _PD, _PI32, C.c_int]),
     "or_ssor_new": (_P, [C.c_int32, _PI32, _PI32, C.c_int32]),
@@ -256,6 +258,34 @@ def colour(indptr, indices):
     col = np.empty(n, dtype=np.int32)
     nc = lib().or_colour(n, _i(indptr), _i(indices), _i(col))
     return col, int(nc)
+
+
+def pcg_block(indptr, indices, vals, bs, rtol=1e-12, maxiter=1500, check_every=10, nthreads: int = 1):
+    """Multi-RHS block Jacobi PCG: vals [members][nnz], bs [members][n] (members padded to a power
+    of two with unit values and zero right-hand sides, as the library does). Returns
+    (x [members][n], iters [members], rr_last [members])."""
+    vals = np.asarray(vals, dtype=np.float64)
+    bs = np.asarray(bs, dtype=np.float64)
+    k, nnz = vals.shape
+    n = bs.shape[1]
+    M = 1
+    while M < k:
+        M *= 2
+    vb = np.ones((nnz, M))
+    vb[:, :k] = vals.T
+    bb = np.zeros((n, M))
+    bb[:, :k] = bs.T
+    vb = np.ascontiguousarray(vb)
+    bb = np.ascontiguousarray(bb)
+    x = np.zeros((n, M))
+    it = np.zeros(M, dtype=np.int32)
+    rr = np.zeros(M)
+    ip = np.ascontiguousarray(indptr, dtype=np.int32)
+    ix = np.ascontiguousarray(indices, dtype=np.int32)
+    if lib().or_pcg_block(n, _i(ip), _i(ix), M, _d(vb), _d(bb), rtol, maxiter, check_every, _d(x), _i(it), _d(rr),
+                          nthreads):
+        raise RuntimeError(lib().or_error().decode())
+    return np.ascontiguousarray(x[:, :k].T), it[:k], rr[:k]
 
 
 def pcg_ssor(indptr, indices, val, b, hist=None, omega=1.0, rtol=1e-12, maxiter=1500, check_every=10,
