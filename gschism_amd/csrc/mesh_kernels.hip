@@ -193,9 +193,10 @@ __device__ __forceinline__ double zprobe(const DMesh& m, const NodeZ& q0, const 
 
 // Element::interpolateVelocities for triangles, ics=1 (meshdata.cu:1209-1291) with
 // computeShapeCoords(fiAnywhere) (meshdata.cu:990-1043); g is element e's geometry.
+// hint: the lower level of the previous vertical bracket (updated here).
 template <int VM>
 __device__ void interp_velocity(const DMesh& m, const double* sig, double h0, int e, const ElemGeom& g, double px,
-                                double py, double pz, double& uo, double& vo, double& wo) {
+                                double py, double pz, double& uo, double& vo, double& wo, int& hint) {
   const int n0 = g.n0, n1 = g.n1, n2 = g.n2;
   const double x0 = g.x0, y0 = g.y0, x1 = g.x1, y1 = g.y1, x2 = g.x2, y2 = g.y2;
   const double area = m.area[e];
@@ -213,31 +214,59 @@ __device__ void interp_velocity(const DMesh& m, const double* sig, double h0, in
   const int nv = m.nvrt;
   const NodeZ q0 = node_z(m, h0, n0), q1 = node_z(m, h0, n1), q2 = node_z(m, h0, n2);
   const int kb0 = q0.kb, kb1 = q1.kb, kb2 = q2.kb;
-  int L = m.elem_kb[e] - 1, R = nv - 1;
-  double zL = zprobe<VM>(m, q0, q1, q2, s0, s1, s2, sig, L, nv);
-  double zR = zprobe<VM>(m, q0, q1, q2, s0, s1, s2, sig, R, nv);
-  double w1, w2;
-  if (pz <= zL) {
-    w1 = 1.0;
-    w2 = 0.0;
-  } else if (pz >= zR) {
-    w1 = 0.0;
-    w2 = 1.0;
-  } else {
-    while (R - L > 1) {
-      const int M = (L + R) / 2;
-      const double zM = zprobe<VM>(m, q0, q1, q2, s0, s1, s2, sig, M, nv);
-      if (zM < pz) {
-        L = M;
-        zL = zM;
-      } else {
-        R = M;
-        zR = zM;
-      }
+  const int L0 = m.elem_kb[e] - 1, R0 = nv - 1;
+  int L = L0, R = R0;
+  double zL = 0.0, zR = 0.0, w1, w2;
+  bool inside = false;
+  if (VM == 0) {
+    // Start from the bracket of the previous point, checked with the exact level z values. The
+    // interpolated z is non-decreasing in the level (non-negative shape coordinates, each node's
+    // levels non-decreasing), so a bracket z_h < pz < z_(h+1) is the unique one the bisection
+    // below would end in, with the same zL and zR; the column-end cases (pz <= z_L0,
+    // pz >= z_R0) are excluded by it. Otherwise, one step down or up, then the full search.
+    const int h = min(max(hint, L0), R0 - 1);
+    const double zh = zprobe<VM>(m, q0, q1, q2, s0, s1, s2, sig, h, nv);
+    const double zh1 = zprobe<VM>(m, q0, q1, q2, s0, s1, s2, sig, h + 1, nv);
+    if (zh < pz && pz < zh1) {
+      L = h, R = h + 1, zL = zh, zR = zh1, inside = true;
+    } else if (pz < zh && h > L0) {
+      const double zd = zprobe<VM>(m, q0, q1, q2, s0, s1, s2, sig, h - 1, nv);
+      if (zd < pz) L = h - 1, R = h, zL = zd, zR = zh, inside = true;
+    } else if (pz > zh1 && h + 2 <= R0) {
+      const double zu = zprobe<VM>(m, q0, q1, q2, s0, s1, s2, sig, h + 2, nv);
+      if (pz < zu) L = h + 1, R = h + 2, zL = zh1, zR = zu, inside = true;
     }
+  }
+  if (inside) {
     w2 = (pz - zL) / (zR - zL);
     w1 = 1.0 - w2;
+  } else {
+    zL = zprobe<VM>(m, q0, q1, q2, s0, s1, s2, sig, L, nv);
+    zR = zprobe<VM>(m, q0, q1, q2, s0, s1, s2, sig, R, nv);
+    if (pz <= zL) {
+      w1 = 1.0;
+      w2 = 0.0;
+    } else if (pz >= zR) {
+      w1 = 0.0;
+      w2 = 1.0;
+    } else {
+      while (R - L > 1) {
+        const int M = (L + R) / 2;
+        const double zM = zprobe<VM>(m, q0, q1, q2, s0, s1, s2, sig, M, nv);
+        if (zM < pz) {
+          L = M;
+          zL = zM;
+        } else {
+          R = M;
+          zR = zM;
+        }
+      }
+      w2 = (pz - zL) / (zR - zL);
+      w1 = 1.0 - w2;
+      hint = L;
+    }
   }
+  if (inside) hint = L;
   const double4* U = reinterpret_cast<const double4*>(m.uvw);
   const double4 a0 = U[(int64_t)n0 * nv + max(L, kb0)], a1 = U[(int64_t)n1 * nv + max(L, kb1)],
                 a2 = U[(int64_t)n2 * nv + max(L, kb2)];
@@ -287,7 +316,11 @@ __device__ bool celestial_walk(const DMesh& m, int& cur, double x0x, double x0y,
     const bool v1 = !left1 && (!obt1 || bis1);
     const bool v2 = !left2 && (!obt2 || bis2);
     if (!(v0 || v1 || v2)) return true;
-    const int side = v0 ? geo.s0 : (v1 ? geo.s1 : geo.s2);
+    // the three side ids as opaque register values: a select among the struct's fields would
+    // otherwise become an indexed load from a stack copy of it
+    int gs0 = geo.s0, gs1 = geo.s1, gs2 = geo.s2;
+    asm volatile("" : "+v"(gs0), "+v"(gs1), "+v"(gs2));
+    const int side = v0 ? gs0 : (v1 ? gs1 : gs2);
     last_cross = side;
     const int4 sr = m.side[side];
     const int nxt = (sr.z == cur) ? sr.w : sr.z;
@@ -312,6 +345,7 @@ __device__ void btrack(const DMesh& m, const double* sig, const StepArgs& a, int
   double uiz = 0.5 * (U[(int64_t)sr.x * m.nvrt + k].z + U[(int64_t)sr.y * m.nvrt + k].z);
   double xtx = -1.0e25, xty = -1.0e25, xtz = -1.0e25;
   int iter = 0;
+  int hint = k;  // lower level of the last vertical bracket; the start point lies at level k
   ElemGeom geo;  // set by every walk before its first use
   bool geo_valid = false;
   while (true) {
@@ -363,7 +397,7 @@ __device__ void btrack(const DMesh& m, const double* sig, const StepArgs& a, int
       const double len = sqrt(v2x * v2x + v2y * v2y);
       const double tx = v2x / len, ty = v2y / len;
       double iu, iv, iw;
-      interp_velocity<VM>(m, sig, h0, cur, geo, xtx, xty, xtz, iu, iv, iw);
+      interp_velocity<VM>(m, sig, h0, cur, geo, xtx, xty, xtz, iu, iv, iw, hint);
       const double d = iu * tx + iv * ty;
       uix = tx * d;
       uiy = ty * d;
@@ -371,7 +405,7 @@ __device__ void btrack(const DMesh& m, const double* sig, const StepArgs& a, int
       status |= GSH_BT_SLIDE;
     } else {
       time_rm -= dtb;
-      interp_velocity<VM>(m, sig, h0, cur, geo, xtx, xty, xtz, uix, uiy, uiz);
+      interp_velocity<VM>(m, sig, h0, cur, geo, xtx, xty, xtz, uix, uiy, uiz, hint);
     }
     if (time_rm <= 0.0) break;
     if (uix * uix + uiy * uiy <= a.velmin2) {
@@ -396,7 +430,7 @@ __device__ void btrack(const DMesh& m, const double* sig, const StepArgs& a, int
     t1 = time_rm;
   }
   double iw;
-  interp_velocity<VM>(m, sig, h0, cur, geo, xtx, xty, xtz, ux, uy, iw);
+  interp_velocity<VM>(m, sig, h0, cur, geo, xtx, xty, xtz, ux, uy, iw, hint);
   foot = cur;
 }
 
@@ -405,7 +439,7 @@ __device__ void btrack(const DMesh& m, const double* sig, const StepArgs& a, int
 // fall on a few contiguous cache lines.
 // VM: the vertical coordinate system (DMesh::vmode); 0, one sigma per level, keeps it in LDS.
 template <int VM>
-__global__ void __launch_bounds__(256) k_backtrack(DMesh m, StepArgs a, const int32_t* list_side,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) k_backtrack(DMesh m, StepArgs a, const int32_t* list_side,
                                                    const int32_t* list_lev, int32_t count) {
   __shared__ double sig[VM == 0 ? 1024 : 1];
   if (VM == 0) {
