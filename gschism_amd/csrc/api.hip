@@ -473,6 +473,50 @@ struct PcgOutcome {
   std::vector<PcgState> st;
 };
 
+// When the host issues the windows of a solve. The host reads each window's state (the true
+// r.r of its closing check) while the next windows run, so the windows issued behind the one
+// that converges run kernels that return at once: up to two windows of empty launches per
+// solve. From the last two checks read, the pacer extrapolates log(r.r) per window for every
+// system still running; when all of them are expected to pass by the newest window in flight,
+// the host waits for the windows in flight before it issues another (one GPU round trip
+// instead of a window of empty launches). Pacing decides when windows are issued, never which
+// iterations run or where the checks fall, so results do not depend on it.
+struct WindowPacer {
+  struct Sys {
+    double l0 = 0.0, l1 = 0.0, la = 0.0;  // log r.r of the last two checks read, log atol^2
+    int w0 = -1, w1 = -1;                 // their window indices
+    bool fin = false;
+  };
+  std::vector<Sys> sys;
+  explicit WindowPacer(int nsys) : sys(nsys) {}
+  // the states of window `win`, one per system
+  void read(const PcgState* s, int win) {
+    for (size_t i = 0; i < sys.size(); ++i) {
+      Sys& y = sys[i];
+      if (s[i].done >= 0) {
+        y.fin = true;
+      } else if (s[i].rr_last > 0.0 && s[i].atol2 > 0.0 && win > y.w1) {
+        y.l0 = y.l1;
+        y.w0 = y.w1;
+        y.l1 = std::log(s[i].rr_last);
+        y.w1 = win;
+        y.la = std::log(s[i].atol2);
+      }
+    }
+  }
+  // every running system is expected to pass its check by window `newest` (half a window of
+  // margin: a miss costs a round trip, a late stop a window)
+  bool expect_done(int newest) const {
+    for (const Sys& y : sys) {
+      if (y.fin) continue;
+      if (y.w0 < 0) return false;
+      const double rate = (y.l0 - y.l1) / (double)(y.w1 - y.w0);
+      if (!(rate > 0.0) || y.l1 - rate * ((double)(newest - y.w1) + 0.5) > y.la) return false;
+    }
+    return true;
+  }
+};
+
 // Runs the PCG iteration for nsys systems described by dsys (device) over the tile table
 // dtiles. Each system stops at its own first check with r.r <= atol^2 (pcg_loop.cpp:405-412).
 // Per-phase launch timing collected with HIP events on the solver stream.
@@ -500,16 +544,23 @@ static void run_pcg_core(gsh_ctx* ctx, const PcgSys<VT>* dsys, int nsys, const T
     launch_pcg_init<VT>(dsys, dtiles, ntiles, rpt, st);
   HIPCHK(hipGetLastError());
   int issued = 0, window = 0;
-  int inflight[2], nflight = 0;  // window slots pending readback, oldest first
+  int inflight[2], nflight = 0;  // windows pending readback (slot = window & 1), oldest first
   bool all_done = false;
+  WindowPacer pacer(nsys);
   const auto t0 = Clock::now();
-  auto check_slot = [&](int slot) {
+  auto check_oldest = [&]() {
+    const int win = inflight[0], slot = win & 1;
+    inflight[0] = inflight[1];
+    --nflight;
     wait_event(ctx->ev[slot], timeout_s, "PCG window");
     bool done = true;
     for (int i = 0; i < nsys; ++i) done = done && (slots[slot * nsys + i].done >= 0);
+    pacer.read(slots + slot * nsys, win);
     return done;
   };
   while (issued < maxiter && !all_done) {
+    while (!all_done && nflight > 0 && pacer.expect_done(window - 1)) all_done = check_oldest();
+    if (all_done) break;
     const int w = std::min(check_every, maxiter - issued);
     if (prof) {
       // timed window: the runtime stamps each phase kernel's start and end into its own event
@@ -540,18 +591,13 @@ static void run_pcg_core(gsh_ctx* ctx, const PcgSys<VT>* dsys, int nsys, const T
     }
     issued += w;
     const int slot = window & 1;
-    if (nflight == 2) {
-      const int old = inflight[0];
-      inflight[0] = inflight[1];
-      nflight = 1;
-      if (check_slot(old)) all_done = true;
-    }
+    if (nflight == 2 && check_oldest()) all_done = true;
     hipLaunchKernelGGL(k_gather_states<VT>, dim3((nsys + 63) / 64), dim3(64), 0, st, dsys, nsys,
                        dgather + slot * nsys);
     HIPCHK(hipMemcpyAsync(slots + slot * nsys, dgather + slot * nsys, sizeof(PcgState) * nsys,
                           hipMemcpyDeviceToHost, st));
     HIPCHK(hipEventRecord(ctx->ev[slot], st));
-    inflight[nflight++] = slot;
+    inflight[nflight++] = window;
     ++window;
     if (std::chrono::duration<double>(Clock::now() - t0).count() > timeout_s)
       throw Error(GSH_E_TIMEOUT, "PCG: time limit exceeded while issuing iterations");
@@ -706,14 +752,21 @@ static PcgState run_ssor(gsh_solver* s, double rtol, int maxiter, int check_ever
   };
   HIPCHK(hipGetLastError());
   int issued = 0, window = 0;
-  int inflight[2], nflight = 0;  // window slots pending readback, oldest first
+  int inflight[2], nflight = 0;  // windows pending readback (slot = window & 1), oldest first
   bool done = false;
+  WindowPacer pacer(1);
   const auto t0 = Clock::now();
-  auto check_slot = [&](int slot) {
+  auto check_oldest = [&]() {
+    const int win = inflight[0], slot = win & 1;
+    inflight[0] = inflight[1];
+    --nflight;
     wait_event(ctx->ev[slot], timeout_s, "SSOR PCG window");
+    pacer.read(slots + slot, win);
     return slots[slot].done >= 0;
   };
   while (issued < maxiter && !done) {
+    while (!done && nflight > 0 && pacer.expect_done(window - 1)) done = check_oldest();
+    if (done) break;
     const int w = std::min(check_every, maxiter - issued);
     if (prof) {
       // every launch stamps its own kernel start and end (hipExtLaunchKernelGGL events)
@@ -750,15 +803,10 @@ static PcgState run_ssor(gsh_solver* s, double rtol, int maxiter, int check_ever
     }
     issued += w;
     const int slot = window & 1;
-    if (nflight == 2) {
-      const int old = inflight[0];
-      inflight[0] = inflight[1];
-      nflight = 1;
-      if (check_slot(old)) done = true;
-    }
+    if (nflight == 2 && check_oldest()) done = true;
     HIPCHK(hipMemcpyAsync(slots + slot, ssor_host<VT>(S).st, sizeof(PcgState), hipMemcpyDeviceToHost, st));
     HIPCHK(hipEventRecord(ctx->ev[slot], st));
-    inflight[nflight++] = slot;
+    inflight[nflight++] = window;
     ++window;
     if (std::chrono::duration<double>(Clock::now() - t0).count() > timeout_s)
       throw Error(GSH_E_TIMEOUT, "SSOR PCG: time limit exceeded while issuing iterations");
@@ -806,33 +854,35 @@ static void run_ssor_batch(gsh_solver* const* S, int count, const std::vector<do
   HIPCHK(hipGetLastError());
   PcgState* slots = ctx->pinned_slots(count);
   int issued = 0, window = 0;
-  int inflight[2], nflight = 0;  // window slots pending readback, oldest first
+  int inflight[2], nflight = 0;  // windows pending readback (slot = window & 1), oldest first
   bool all_done = false;
+  WindowPacer pacer(count);
   const auto t0 = Clock::now();
-  auto check_slot = [&](int slot) {
+  auto check_oldest = [&]() {
+    const int win = inflight[0], slot = win & 1;
+    inflight[0] = inflight[1];
+    --nflight;
     wait_event(ctx->ev[slot], timeout_s, "batched SSOR PCG window");
     bool done = true;
     for (int i = 0; i < count; ++i) done = done && (slots[slot * count + i].done >= 0);
+    pacer.read(slots + slot * count, win);
     return done;
   };
   while (issued < maxiter && !all_done) {
+    while (!all_done && nflight > 0 && pacer.expect_done(window - 1)) all_done = check_oldest();
+    if (all_done) break;
     const int w = std::min(check_every, maxiter - issued);
     for (int it = 0; it < w; ++it) ssor_table_sweeps(dsys, drefs, tb, 0, rpt, st);
     launch_ssor_true_rr_batch(dsys, drefs + tb.aoff, tb.nall, rpt, st);
     launch_ssor_check_batch(dsys, count, st);
     issued += w;
     const int slot = window & 1;
-    if (nflight == 2) {
-      const int old = inflight[0];
-      inflight[0] = inflight[1];
-      nflight = 1;
-      if (check_slot(old)) all_done = true;
-    }
+    if (nflight == 2 && check_oldest()) all_done = true;
     launch_ssor_gather_states(dsys, count, dgather + slot * count, st);
     HIPCHK(hipMemcpyAsync(slots + slot * count, dgather + slot * count, sizeof(PcgState) * count,
                           hipMemcpyDeviceToHost, st));
     HIPCHK(hipEventRecord(ctx->ev[slot], st));
-    inflight[nflight++] = slot;
+    inflight[nflight++] = window;
     ++window;
     if (std::chrono::duration<double>(Clock::now() - t0).count() > timeout_s)
       throw Error(GSH_E_TIMEOUT, "batched SSOR PCG: time limit exceeded while issuing iterations");
@@ -949,7 +999,10 @@ static void run_mixed(gsh_ctx* ctx, gsh_solver* const* S, int count, double rtol
         gath[j] = ssor_gathered[sub[j]] ? 0 : 1;
         ssor_gathered[sub[j]] = 1;
       }
-      run_ssor_batch<float>(ss.data(), ns, rts, maxiter, check_every, timeout_s, gath, o.st);
+      if (ns == 1)  // the one-system kernels (same bodies, no launch table)
+        o.st.assign(1, run_ssor<float>(ss[0], rts[0], maxiter, check_every, timeout_s, nullptr, false, gath[0] != 0));
+      else
+        run_ssor_batch<float>(ss.data(), ns, rts, maxiter, check_every, timeout_s, gath, o.st);
     } else {
       run_pcg<float>(ctx, d32s, ns, dtl, (int)stl.size(), rpt, rtol, maxiter, check_every, timeout_s, o, nullptr, -1,
                      nullptr, drt);
