@@ -452,6 +452,53 @@ def served_leg(ctx, L, cases, core, n_req, rates, rtol, mixed_rtol, check_every)
 
     res["rates"] = serve(L.PRECOND_JACOBI)
     res["ssor_rates"] = serve(L.PRECOND_SSOR)
+
+    # Solve only: the same requests without backtracking (assembly + solve of the given RHS).
+    # Plain = FP64 SSOR one request after another; served = mixed SSOR with every request queued
+    # at once (saturation). With backtracking on both sides the served gain is bounded by the
+    # backtracking share of a plain request, which runs at the same speed in both forms.
+    def plain_solve_only():
+        sol = [core.Solver(m, L.PREC_FP64, precond=L.PRECOND_SSOR) for m in pool]
+        for s in sol:
+            s.set_tuning(2)
+        for rep_ in range(2):  # the first pass warms the solvers
+            ctx.synchronize()
+            t = time.perf_counter()
+            for j, k in enumerate(which):
+                sol[k].assemble(pd)
+                sol[k].reset_history()
+                sol[k].solve(rhs[j], rtol, 20000, check_every, want_x=False)
+            ctx.synchronize()
+            dt = time.perf_counter() - t
+        for s in sol:
+            s.close()
+        return n_req / dt
+
+    def served_saturated():
+        srv = core.Server(ctx, rtol=mixed_rtol, maxiter=20000, check_every=check_every,
+                          precision=L.PREC_MIXED, precond=L.PRECOND_SSOR)
+        ids = [srv.add_mesh(m) for m in pool]
+        for rep_ in range(2):  # the first pass warms the server's solver pools
+            ctx.synchronize()
+            t = time.perf_counter()
+            left = {srv.submit(ids[k], rhs[j], with_backtrack=False) for j, k in enumerate(which)}
+            while left:
+                srv.step()
+                for tk in srv.finished():
+                    st_, _, _, _ = srv.poll(tk)
+                    if st_ != 0:
+                        raise RuntimeError(f"solve-only request failed with status {st_}")
+                    left.discard(tk)
+            dt = time.perf_counter() - t
+        srv.close()
+        return n_req / dt
+
+    p_so = plain_solve_only()
+    s_so = served_saturated()
+    res["solve_only"] = {"plain_fp64_ssor_solves_per_s": round(p_so, 3),
+                         "served_mixed_ssor_saturated_solves_per_s": round(s_so, 3),
+                         "served_vs_plain": round(s_so / p_so, 3),
+                         "backtrack_share_of_plain_ssor_request": round(1.0 - plain_ssor_rate / p_so, 3)}
     for m in pool:
         m.close()
     return res
