@@ -23,6 +23,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0  # MI355X HBM3E
+PEAK_FP64_TFLOPS = 78.6  # MI355X vector FP64
 
 
 def log(*a):
@@ -52,6 +53,23 @@ def counter_traffic(kernel: str, rows_per_thread: int):
                 w = line.strip().strip("`").split()
                 if len(w) == 4 and w[0] == "traffic" and w[1] == kernel and w[2] == f"rows_per_thread={rows_per_thread}":
                     return int(w[3].split("=", 1)[1])
+    except OSError:
+        pass
+    return None
+
+
+def backtrack_bound():
+    """Per-launch FP64 operations, HBM bytes, VALU busy fraction and occupancy of k_backtrack on
+    this workload, from the committed rocprofv3 counter summary (profiles/), or None. The summary
+    holds one line 'bound k_backtrack fp64_ops_per_launch=<F> hbm_bytes_per_launch=<B>
+    valu_busy=<V> waves_per_simd=<W>'."""
+    path = os.path.join(ROOT, "profiles", "backtrack_counters_channel-1024x1024.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                w = line.strip().strip("`").split()
+                if len(w) == 6 and w[0] == "bound" and w[1] == "k_backtrack":
+                    return {k: float(v) for k, v in (x.split("=", 1) for x in w[2:])}
     except OSError:
         pass
     return None
@@ -555,6 +573,10 @@ def main() -> int:
     # ---- one hot-path step up to the system: backtracking, RHS producers, assembly
     mesh.backtrack(p)  # warm
     ms_bt = min(timed(lambda: mesh.backtrack(p)) for _ in range(3))
+    # the LDS-staged element walk (same results) beside the default cache-only walk
+    mesh.set_backtrack_staging(True)
+    ms_bt_staged = min(timed(lambda: mesh.backtrack(p)) for _ in range(3))
+    mesh.set_backtrack_staging(False)
     walks = mesh.ns * mesh.nvrt
     ms_rhs = min(timed(lambda: mesh.rhs_producers(p)) for _ in range(3))
     solver = core.Solver(mesh, L.PREC_FP64, precond=L.PRECOND_SSOR)
@@ -573,8 +595,19 @@ def main() -> int:
     ms_vv = min(timed(lambda: mesh.vertical_velocity(p)) for _ in range(3))
     vv_bytes = (24 * mesh.ns + 8 * mesh.ne) * mesh.nvrt + 64 * n
     ndelt = int(round(p.dt / p.dtb_max))
-    bt_bytes = walks * (36 + 216 * ndelt)
-    log(f"[bench] backtrack {ms_bt:.2f} ms ({walks} side-levels), rhs {ms_rhs:.2f} ms, assembly {ms_asm:.2f} ms, "
+    # backtracking is VALU-issue bound: its FP64 rate and HBM rate against the peaks, from the
+    # counter summary's per-launch figures over this run's time
+    bt_rec = {"ms": round(ms_bt, 3), "side_levels": walks, "per_s": round(walks / (ms_bt * 1e-3), 1),
+              "substeps": ndelt, "ms_lds_staged_walk": round(ms_bt_staged, 3)}
+    bb = backtrack_bound()
+    if bb is not None:
+        bt_rec["roofline"] = {"bound": "valu", "valu_busy": bb["valu_busy"], "waves_per_simd": bb["waves_per_simd"],
+                              "fp64_tflops": round(bb["fp64_ops_per_launch"] / (ms_bt * 1e-3) / 1e12, 2),
+                              "fp64_peak_tflops": PEAK_FP64_TFLOPS,
+                              "fp64_frac": round(bb["fp64_ops_per_launch"] / (ms_bt * 1e-3) / 1e12 / PEAK_FP64_TFLOPS, 3),
+                              "hbm_gbs": round(bb["hbm_bytes_per_launch"] / (ms_bt * 1e-3) / 1e9, 1),
+                              "hbm_frac": round(bb["hbm_bytes_per_launch"] / (ms_bt * 1e-3) / 1e9 / PEAK_HBM_GBS, 3)}
+    log(f"[bench] backtrack {ms_bt:.2f} ms ({walks} side-levels; LDS-staged walk {ms_bt_staged:.2f} ms), rhs {ms_rhs:.2f} ms, assembly {ms_asm:.2f} ms, "
         f"nodal velocity {ms_nv:.2f} ms ({nv_bytes / (ms_nv * 1e-3) / 1e9:.0f} GB/s), "
         f"momentum {ms_mom:.2f} ms ({mom_bytes / (ms_mom * 1e-3) / 1e9:.0f} GB/s), "
         f"vertical velocity {ms_vv:.2f} ms ({vv_bytes / (ms_vv * 1e-3) / 1e9:.0f} GB/s)")
@@ -791,8 +824,7 @@ def main() -> int:
         "ssor_kernels": per,
         "copy_probe_gbs": round(copy_gbs, 1),
         "jacobi": jac,
-        "backtrack": {"ms": round(ms_bt, 3), "side_levels": walks, "per_s": round(walks / (ms_bt * 1e-3), 1),
-                      "substeps": ndelt, "effective_gbs": round(bt_bytes / (ms_bt * 1e-3) / 1e9, 1)},
+        "backtrack": bt_rec,
         "nodal_velocity": {"ms": round(ms_nv, 3), "bytes": nv_bytes, "gbs": round(nv_bytes / (ms_nv * 1e-3) / 1e9, 1)},
         "momentum": {"ms": round(ms_mom, 3), "bytes": mom_bytes, "gbs": round(mom_bytes / (ms_mom * 1e-3) / 1e9, 1)},
         "full_step": full_step,

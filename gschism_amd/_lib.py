@@ -115,6 +115,7 @@ SIGNATURES = {
     "gsh_update_levels": (_I, [_P, C.POINTER(StepParams)]),
     "gsh_backtrack": (_I, [_P, C.POINTER(StepParams), _PD, _PI32]),
     "gsh_backtrack_list": (_I, [_P, C.POINTER(StepParams), _PI32, _PI32, _I32]),
+    "gsh_mesh_set_backtrack_staging": (_I, [_P, _I32]),
     "gsh_rhs_producers": (_I, [_P, C.POINTER(StepParams)]),
     "gsh_nodal_velocity": (_I, [_P, C.POINTER(StepParams)]),
     "gsh_momentum": (_I, [_P, C.POINTER(StepParams)]),

@@ -224,6 +224,11 @@ class Mesh:
         check(L.lib().gsh_backtrack(self.h, C.byref(params), None, None))
         return None
 
+    def set_backtrack_staging(self, staged: bool) -> None:
+        """Backtracking reads elements through the caches (False, default) or stages each
+        workgroup's element patch in LDS first (True); results are identical."""
+        check(L.lib().gsh_mesh_set_backtrack_staging(self.h, 1 if staged else 0))
+
     def backtrack_list(self, params, sides, levels) -> None:
         s = np.ascontiguousarray(sides, dtype=np.int32)
         k = np.ascontiguousarray(levels, dtype=np.int32)

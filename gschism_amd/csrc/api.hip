@@ -1591,6 +1591,14 @@ int gsh_update_levels(gsh_mesh* m, const gsh_step_params* p) {
   })
 }
 
+int gsh_mesh_set_backtrack_staging(gsh_mesh* m, int32_t staged) {
+  GSH_API_BODY({
+    GSH_REQUIRE(m, GSH_E_ARG, "mesh is NULL");
+    GSH_REQUIRE(staged == 0 || staged == 1, GSH_E_ARG, "staged must be 0 or 1");
+    m->d.bt_stage = staged;
+  })
+}
+
 int gsh_backtrack(gsh_mesh* m, const gsh_step_params* p, double* btu_host, int32_t* foot_host) {
   GSH_API_BODY({
     GSH_REQUIRE(m, GSH_E_ARG, "mesh is NULL");

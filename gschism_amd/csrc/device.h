@@ -24,6 +24,7 @@ struct DMesh {
   // vertical coordinates of Node::compZCoord (meshdata.cu:1744-1807): vmode 0 = ivcor 1 with one
   // sigma per level, 1 = ivcor 1 with a sigma per node and level, 2 = ivcor 2 (SZ)
   int32_t vmode;
+  int32_t bt_stage;         // backtracking stages each workgroup's element patch in LDS (0: off)
   int32_t izlev;            // nz - 1: the level at -min(h, h_s) (vmode 2)
   double h_s, h_c;          // (vmode 2)
   const double* sig_node;   // [n][nvrt] (vmode 1)

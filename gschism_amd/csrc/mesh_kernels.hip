@@ -168,6 +168,95 @@ __device__ __forceinline__ NodeZ node_z(const DMesh& m, double h0, int node) {
   return {q.x, q.y, H, m.kb[node], H <= h0, node};
 }
 
+// LDS-staged element patch (k_backtrack with ST = true): every element the walk and the
+// interpolation read, for the elements on both sides of each of the workgroup's sides and their
+// neighbours, in one record, in an LDS table keyed by element id (open addressing, linear
+// probing). An element outside the patch is read from global memory as before, so the values
+// and results are those of the unstaged kernel.
+struct StElem {
+  double x[3], y[3];    // corners in node order n0, n1, n2
+  double eta[3], h[3];  // eta2 and depth of the corners
+  double area;
+  int n[3], s[3];       // node ids; sides opposite the corners
+  int ekb, kb[3];       // element bottom level; the corners' kb
+};
+constexpr int kStSlots = 128;  // table size (a power of two)
+constexpr int kStCand = 96;    // at most this many candidates, so a free slot always ends a probe
+
+__device__ __forceinline__ unsigned st_hash(int e) { return ((unsigned)e * 2654435761u) >> 25; }
+
+// slot of element e, or -1 when it is not staged
+__device__ __forceinline__ int st_find(const int* key, int e) {
+  unsigned h = st_hash(e);
+  for (int p = 0; p < kStSlots; ++p) {
+    const int k = key[h];
+    if (k == e) return (int)h;
+    if (k < 0) return -1;
+    h = (h + 1) & (kStSlots - 1);
+  }
+  return -1;
+}
+
+__device__ __forceinline__ ElemGeom st_geom(const StElem& r) {
+  return {r.n[0], r.n[1], r.n[2], r.s[0], r.s[1], r.s[2], r.x[0], r.y[0], r.x[1], r.y[1], r.x[2], r.y[2]};
+}
+
+__device__ __forceinline__ NodeZ st_node_z(const StElem& r, int c, double h0) {
+  const double H = r.eta[c] + r.h[c];
+  return {r.eta[c], r.h[c], H, r.kb[c], H <= h0, r.n[c]};
+}
+
+// Fills the table for sides s_first .. s_first + nsides - 1: candidate j of a side is one of
+// its two elements (j = 0, 4) or that element's neighbour across its side j & 3 - 1.
+__device__ void st_build(const DMesh& m, int* key, StElem* rec, int s_first, int nsides) {
+  for (int i = threadIdx.x; i < kStSlots; i += blockDim.x) key[i] = -1;
+  __syncthreads();
+  const int i = threadIdx.x;
+  if (i < min(nsides * 8, kStCand)) {
+    const int4 sr = m.side[s_first + i / 8];
+    const int j = i % 8, q = j & 3;
+    int e = (j & 4) ? sr.w : sr.z;
+    if (e >= 0 && q > 0) {
+      const int4 r0 = m.elem[2 * e], r1 = m.elem[2 * e + 1];
+      const int es = q == 1 ? r0.w : (q == 2 ? r1.x : r1.y);
+      const int4 o = m.side[es];
+      e = (o.z == e) ? o.w : o.z;
+    }
+    if (e >= 0) {
+      unsigned h = st_hash(e);
+      for (int p = 0; p < kStSlots; ++p) {
+        const int old = atomicCAS(&key[h], -1, e);
+        if (old == -1 || old == e) break;
+        h = (h + 1) & (kStSlots - 1);
+      }
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < kStSlots; t += blockDim.x) {
+    const int e = key[t];
+    if (e < 0) continue;
+    const int4 r0 = m.elem[2 * e], r1 = m.elem[2 * e + 1];
+    StElem& R = rec[t];
+    const int nn[3] = {r0.x, r0.y, r0.z};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      const double2 p = m.nrec[2 * nn[c]], z = m.nrec[2 * nn[c] + 1];
+      R.x[c] = p.x;
+      R.y[c] = p.y;
+      R.eta[c] = z.x;
+      R.h[c] = z.y;
+      R.kb[c] = m.kb[nn[c]];
+      R.n[c] = nn[c];
+    }
+    R.s[0] = r0.w;
+    R.s[1] = r1.x;
+    R.s[2] = r1.y;
+    R.area = m.area[e];
+    R.ekb = m.elem_kb[e];
+  }
+  __syncthreads();
+}
+
 // zcoord(node, max(M, kb)) with sM = sigma[M] (one sigma per level): for M <= kb that is the bed
 // -h, above kb level M.
 __device__ __forceinline__ double zlev(const NodeZ& q, double sM, int M, int nv) {
@@ -194,12 +283,14 @@ __device__ __forceinline__ double zprobe(const DMesh& m, const NodeZ& q0, const 
 // Element::interpolateVelocities for triangles, ics=1 (meshdata.cu:1209-1291) with
 // computeShapeCoords(fiAnywhere) (meshdata.cu:990-1043); g is element e's geometry.
 // hint: the lower level of the previous vertical bracket (updated here).
-template <int VM>
-__device__ void interp_velocity(const DMesh& m, const double* sig, double h0, int e, const ElemGeom& g, double px,
-                                double py, double pz, double& uo, double& vo, double& wo, int& hint) {
+// rc: element e's staged record (ST), else NULL.
+template <int VM, bool ST>
+__device__ void interp_velocity(const DMesh& m, const double* sig, double h0, int e, const ElemGeom& g,
+                                const StElem* rc, double px, double py, double pz, double& uo, double& vo,
+                                double& wo, int& hint) {
   const int n0 = g.n0, n1 = g.n1, n2 = g.n2;
   const double x0 = g.x0, y0 = g.y0, x1 = g.x1, y1 = g.y1, x2 = g.x2, y2 = g.y2;
-  const double area = m.area[e];
+  const double area = (ST && rc) ? rc->area : m.area[e];
   double s0 = signed_area(px, py, x1, y1, x2, y2) / area;
   double s1 = signed_area(x0, y0, px, py, x2, y2) / area;
   double s2 = 1.0 - s0 - s1;
@@ -212,9 +303,12 @@ __device__ void interp_velocity(const DMesh& m, const double* sig, double h0, in
     s2 = (s2 + aa) * bb;
   }
   const int nv = m.nvrt;
-  const NodeZ q0 = node_z(m, h0, n0), q1 = node_z(m, h0, n1), q2 = node_z(m, h0, n2);
+  const bool st = ST && rc;
+  const NodeZ q0 = st ? st_node_z(*rc, 0, h0) : node_z(m, h0, n0);
+  const NodeZ q1 = st ? st_node_z(*rc, 1, h0) : node_z(m, h0, n1);
+  const NodeZ q2 = st ? st_node_z(*rc, 2, h0) : node_z(m, h0, n2);
   const int kb0 = q0.kb, kb1 = q1.kb, kb2 = q2.kb;
-  const int L0 = m.elem_kb[e] - 1, R0 = nv - 1;
+  const int L0 = (st ? rc->ekb : m.elem_kb[e]) - 1, R0 = nv - 1;
   int L = L0, R = R0;
   double zL = 0.0, zR = 0.0, w1, w2;
   bool inside = false;
@@ -284,11 +378,25 @@ __device__ void interp_velocity(const DMesh& m, const double* sig, double h0, in
 // Corner j of a triangle is getNodeInOrder(j) = nodes[(j+1)%3]; walk side j is element side j.
 // geo_valid: geo already holds element cur (the previous walk's final element), so the first
 // step reads nothing. On return, geo holds the geometry of the final element cur.
-__device__ bool celestial_walk(const DMesh& m, int& cur, double x0x, double x0y, double xtx, double xty,
-                               int& last_cross, int& status, ElemGeom& geo, bool geo_valid) {
+// ST: slot is element cur's staged slot (-1: not staged), kept with geo.
+template <bool ST>
+__device__ __forceinline__ void walk_geom(const DMesh& m, const int* key, const StElem* rec, int cur, ElemGeom& geo,
+                                          int& slot) {
+  if (ST) {
+    slot = st_find(key, cur);
+    geo = slot >= 0 ? st_geom(rec[slot]) : elem_geom(m, cur);
+  } else {
+    geo = elem_geom(m, cur);
+  }
+}
+
+template <bool ST>
+__device__ bool celestial_walk(const DMesh& m, const int* key, const StElem* rec, int& cur, double x0x, double x0y,
+                               double xtx, double xty, int& last_cross, int& status, ElemGeom& geo, int& slot,
+                               bool geo_valid) {
   double fac = 0.00001;
   for (int iter = 0; iter < 40; ++iter) {
-    if (iter > 0 || !geo_valid) geo = elem_geom(m, cur);
+    if (iter > 0 || !geo_valid) walk_geom<ST>(m, key, rec, cur, geo, slot);
     const double C0x = geo.x1, C0y = geo.y1;  // corner j is node (j + 1) % 3
     const double C1x = geo.x2, C1y = geo.y2;
     const double C2x = geo.x0, C2y = geo.y0;
@@ -328,14 +436,14 @@ __device__ bool celestial_walk(const DMesh& m, int& cur, double x0x, double x0y,
     cur = nxt;
   }
   status |= GSH_BT_WALK_CAP;
-  geo = elem_geom(m, cur);
+  walk_geom<ST>(m, key, rec, cur, geo, slot);
   return true;
 }
 
 // Side::btrack (meshdata.cu:1449-1629). Returns the final velocity and the foot element.
-template <int VM>
-__device__ void btrack(const DMesh& m, const double* sig, const StepArgs& a, int s, int k, double dtbk, int cur, double& ux,
-                       double& uy, int& foot, int& status) {
+template <int VM, bool ST>
+__device__ void btrack(const DMesh& m, const double* sig, const int* key, const StElem* rec, const StepArgs& a, int s,
+                       int k, double dtbk, int cur, double& ux, double& uy, int& foot, int& status) {
   const double h0 = a.h0;
   double x0x = m.side_cx[s], x0y = m.side_cy[s], x0z = m.zc[(int64_t)s * m.nvrt + k];
   double time_rm = a.dt, t1 = 1e10, t2 = 2e10, t3 = 3e10;
@@ -347,6 +455,7 @@ __device__ void btrack(const DMesh& m, const double* sig, const StepArgs& a, int
   int iter = 0;
   int hint = k;  // lower level of the last vertical bracket; the start point lies at level k
   ElemGeom geo;  // set by every walk before its first use
+  int slot = -1;  // geo's staged slot (ST)
   bool geo_valid = false;
   while (true) {
     const double dtb = fmin(dtbk, time_rm);
@@ -354,8 +463,9 @@ __device__ void btrack(const DMesh& m, const double* sig, const StepArgs& a, int
     xty = x0y - dtb * uiy;
     xtz = x0z - dtb * uiz;
     int last = -1;
-    const bool found = celestial_walk(m, cur, x0x, x0y, xtx, xty, last, status, geo, geo_valid);
+    const bool found = celestial_walk<ST>(m, key, rec, cur, x0x, x0y, xtx, xty, last, status, geo, slot, geo_valid);
     geo_valid = true;
+    const StElem* rc = (ST && slot >= 0) ? rec + slot : nullptr;
     if (t3 <= time_rm) {
       xtx = x0x;
       xty = x0y;
@@ -397,7 +507,7 @@ __device__ void btrack(const DMesh& m, const double* sig, const StepArgs& a, int
       const double len = sqrt(v2x * v2x + v2y * v2y);
       const double tx = v2x / len, ty = v2y / len;
       double iu, iv, iw;
-      interp_velocity<VM>(m, sig, h0, cur, geo, xtx, xty, xtz, iu, iv, iw, hint);
+      interp_velocity<VM, ST>(m, sig, h0, cur, geo, rc, xtx, xty, xtz, iu, iv, iw, hint);
       const double d = iu * tx + iv * ty;
       uix = tx * d;
       uiy = ty * d;
@@ -405,7 +515,7 @@ __device__ void btrack(const DMesh& m, const double* sig, const StepArgs& a, int
       status |= GSH_BT_SLIDE;
     } else {
       time_rm -= dtb;
-      interp_velocity<VM>(m, sig, h0, cur, geo, xtx, xty, xtz, uix, uiy, uiz, hint);
+      interp_velocity<VM, ST>(m, sig, h0, cur, geo, rc, xtx, xty, xtz, uix, uiy, uiz, hint);
     }
     if (time_rm <= 0.0) break;
     if (uix * uix + uiy * uiy <= a.velmin2) {
@@ -430,7 +540,8 @@ __device__ void btrack(const DMesh& m, const double* sig, const StepArgs& a, int
     t1 = time_rm;
   }
   double iw;
-  interp_velocity<VM>(m, sig, h0, cur, geo, xtx, xty, xtz, ux, uy, iw, hint);
+  interp_velocity<VM, ST>(m, sig, h0, cur, geo, (ST && slot >= 0) ? rec + slot : nullptr, xtx, xty, xtz, ux, uy, iw,
+                          hint);
   foot = cur;
 }
 
@@ -438,13 +549,22 @@ __device__ void btrack(const DMesh& m, const double* sig, const StepArgs& a, int
 // lanes are consecutive levels of one side, so the nodal [node][level] gathers of a wave
 // fall on a few contiguous cache lines.
 // VM: the vertical coordinate system (DMesh::vmode); 0, one sigma per level, keeps it in LDS.
-template <int VM>
+// ST: the workgroup first stages its element patch in LDS (all sides and levels, no list).
+template <int VM, bool ST>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) k_backtrack(DMesh m, StepArgs a, const int32_t* list_side,
                                                    const int32_t* list_lev, int32_t count) {
   __shared__ double sig[VM == 0 ? 1024 : 1];
+  __shared__ int st_key[ST ? kStSlots : 1];
+  __shared__ StElem st_rec[ST ? kStSlots : 1];
   if (VM == 0) {
     for (int j = threadIdx.x; j < m.nvrt; j += blockDim.x) sig[j] = m.sigma[j];
     __syncthreads();
+  }
+  if (ST) {  // the workgroup's sides: from its first to its last (side, level) pair
+    const int64_t t0 = (int64_t)blockIdx.x * blockDim.x;
+    const int64_t tl = min(t0 + (int64_t)blockDim.x, (int64_t)m.ns * m.nvrt) - 1;
+    const int s_first = (int)(t0 / m.nvrt), s_last = (int)(tl / m.nvrt);
+    st_build(m, st_key, st_rec, s_first, s_last - s_first + 1);
   }
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   int s, k;
@@ -484,7 +604,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))
     const int ndelt = max(a.ndelt_min, min(nd, a.ndelt_max));
     const double dtbk = a.dt / ndelt;
     const int start = m.elem_wet[sr.z] ? sr.z : sr.w;
-    btrack<VM>(m, sig, a, s, k, dtbk, start, bx, by, foot, status);
+    btrack<VM, ST>(m, sig, st_key, st_rec, a, s, k, dtbk, start, bx, by, foot, status);
   }
   m.btu[2 * idx] = bx;
   m.btu[2 * idx + 1] = by;
@@ -506,10 +626,15 @@ void launch_backtrack(const DMesh& m, const StepArgs& a, const int32_t* ls, cons
   const int64_t total = ls ? (int64_t)count : (int64_t)m.ns * m.nvrt;
   if (total <= 0) return;
   const dim3 grid((unsigned)((total + 255) / 256));
-  if (m.vmode == 0)
-    hipLaunchKernelGGL(k_backtrack<0>, grid, dim3(256), 0, st, m, a, ls, ll, count);
+  const bool stage = m.bt_stage != 0 && !ls;
+  if (m.vmode == 0 && stage)
+    hipLaunchKernelGGL((k_backtrack<0, true>), grid, dim3(256), 0, st, m, a, ls, ll, count);
+  else if (m.vmode == 0)
+    hipLaunchKernelGGL((k_backtrack<0, false>), grid, dim3(256), 0, st, m, a, ls, ll, count);
+  else if (stage)
+    hipLaunchKernelGGL((k_backtrack<1, true>), grid, dim3(256), 0, st, m, a, ls, ll, count);
   else
-    hipLaunchKernelGGL(k_backtrack<1>, grid, dim3(256), 0, st, m, a, ls, ll, count);
+    hipLaunchKernelGGL((k_backtrack<1, false>), grid, dim3(256), 0, st, m, a, ls, ll, count);
 }
 
 // ---------------------------------------------------------------- nodal velocities

@@ -204,6 +204,12 @@ int gsh_update_levels(gsh_mesh* mesh, const gsh_step_params* p);
  * the device (GSH_F_BTU, GSH_F_FOOT, GSH_F_BSTATUS); btu_host / foot_host may be NULL. */
 int gsh_backtrack(gsh_mesh* mesh, const gsh_step_params* p, double* btu_host, int32_t* foot_host);
 
+/* How gsh_backtrack reads the elements: 0 (default) through the caches; 1 stages each
+ * workgroup's element patch (the elements on both sides of its sides and their neighbours) in
+ * LDS first and reads the rest from global memory. Results are identical; only speed differs.
+ * No counterpart in the reference (a tuning choice of this implementation). */
+int gsh_mesh_set_backtrack_staging(gsh_mesh* mesh, int32_t staged);
+
 /* Backtracks an explicit list of (side, level) start points (the same walk as gsh_backtrack,
  * results written to the same per-(side, level) slots). */
 int gsh_backtrack_list(gsh_mesh* mesh, const gsh_step_params* p, const int32_t* sides_host,

@@ -136,6 +136,26 @@ def test_nodal_velocity_then_backtrack(ctx):
     assert_bitwise(mesh.get(L.F_BTU), om.get(L.F_BTU), "backtracked velocity")
 
 
+@pytest.mark.parametrize("nvrt", [12, 2])
+def test_backtrack_lds_staging(ctx, nvrt):
+    """The LDS-staged element walk (dry island, node-varying kb, open and land boundaries) gives
+    the oracle's feet, velocities and flags bit for bit; at 2 levels a workgroup spans 128 sides,
+    more than the table stages, so staged and global element reads mix."""
+    spec = cases.CaseSpec(f"island-48x32-{nvrt}", 48, 32, 400.0, nvrt=nvrt, depth=10.0, substeps=6)
+    mesh, om, p = _setup(ctx, spec, kb_fn=(lambda d: np.minimum(_lsc_kb(d), nvrt - 2)), depth_fn=_island_depth)
+    om.backtrack(p, nthreads=8)
+    for staged in (True, False, True):
+        print(f"[gpu] backtrack staged={staged} nvrt={nvrt}", flush=True)
+        mesh.set_backtrack_staging(staged)
+        mesh.backtrack(p)
+        assert_bitwise(mesh.get(L.F_FOOT), om.get(L.F_FOOT), f"foot elements (staged={staged})")
+        assert_bitwise(mesh.get(L.F_BTU), om.get(L.F_BTU), f"backtracked velocity (staged={staged})")
+        assert_bitwise(mesh.get(L.F_BSTATUS), om.get(L.F_BSTATUS), f"backtrack status (staged={staged})")
+    st = mesh.get(L.F_BSTATUS)
+    assert (st & L.BT_SLIDE).any()
+    assert L.lib().gsh_mesh_set_backtrack_staging(mesh.h, 2) == L.GSH_E_ARG
+
+
 @pytest.mark.parametrize("rpt", [1, 2, 8])
 def test_rows_per_thread(ctx, rpt):
     spec = cases.CaseSpec("channel-64x40", 64, 40, 500.0, nvrt=6, depth=None, substeps=4)
@@ -434,6 +454,13 @@ def test_full_size_1m_channel(ctx):
     del foot
     assert_bitwise(mesh.get(L.F_BTU), om.get(L.F_BTU), "1M backtracked velocity")
     assert_bitwise(mesh.get(L.F_BSTATUS), om.get(L.F_BSTATUS), "1M backtracking status")
+    print("[gpu] 1M backtrack with the LDS-staged walk", flush=True)
+    mesh.set_backtrack_staging(True)
+    mesh.backtrack(p)
+    mesh.set_backtrack_staging(False)
+    assert_bitwise(mesh.get(L.F_FOOT), om.get(L.F_FOOT), "1M foot elements, staged walk")
+    assert_bitwise(mesh.get(L.F_BTU), om.get(L.F_BTU), "1M backtracked velocity, staged walk")
+    assert_bitwise(mesh.get(L.F_BSTATUS), om.get(L.F_BSTATUS), "1M backtracking status, staged walk")
     print("[gpu] 1M rhs + assembly", flush=True)
     mesh.rhs_producers(p)
     om.rhs(p)

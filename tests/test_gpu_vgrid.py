@@ -92,6 +92,12 @@ def test_chain_lsc2_per_node_sigma(ctx):
     mesh, om, p = _setup_vgrid(ctx, spec, depth, vg, kb=kb)
     assert (mesh.topology(L.T_KB) > 0).any()
     _full_chain(ctx, mesh, om, p)
+    # the LDS-staged element walk on per-node sigma levels
+    mesh.set_backtrack_staging(True)
+    mesh.backtrack(p)
+    om.backtrack(p, nthreads=8)
+    assert_bitwise(mesh.get(L.F_FOOT), om.get(L.F_FOOT), "foot elements, staged walk")
+    assert_bitwise(mesh.get(L.F_BTU), om.get(L.F_BTU), "backtracked velocity, staged walk")
 
 
 def test_run_like_case_from_files(ctx, tmp_path):
