@@ -390,6 +390,8 @@ static void ssor_build(gsh_solver* s) {
     for (VT** v : {&h.d, &h.dt, &h.kd, &h.bp, &h.y, &h.r, &h.p, &h.q, &h.t, &h.u, &h.s, &h.w}) *v = P.alloc<VT>(L.np);
     for (double** v : {&h.part_rr, &h.part_bb}) *v = P.alloc<double>(h.ntiles);
     for (double** v : {&h.part_rz, &h.part_pq}) *v = P.alloc<double>(2 * (int64_t)h.ntiles);  // two iteration slots
+    h.pre = P.alloc<double>(4 * (int64_t)kTpb);
+    h.pre_t0 = L.cstart[L.ncol - 1] / T;
     h.st = P.alloc<PcgState>(1);
   };
   if (s->precision == GSH_PREC_MIXED) {
@@ -634,7 +636,8 @@ struct SsorTimes {
 // Flags of an SSOR sweep launch (the kernels' kFirst .. kLead): first sweep of an iteration,
 // colour 0 (forms u and s), last sweep of an iteration, set-up pass, workgroup that writes the
 // system's scalars (batched launches).
-constexpr int kSsorFirst = 1, kSsorMid = 2, kSsorLast = 4, kSsorInit = 8, kSsorLead = 16;
+constexpr int kSsorFirst = 1, kSsorMid = 2, kSsorLast = 4, kSsorInit = 8, kSsorLead = 16, kSsorPre = 32,
+              kSsorPreWG = 64;
 
 // Launch table of a batch of SSOR systems: backward colour steps 0 .. maxC-1 at boff, forward
 // steps 0 .. maxC-2 at foff, then every tile of every system at aoff (true residual).
@@ -711,7 +714,9 @@ static void ssor_begin(gsh_solver* s, double rtol, int maxiter, int check_every,
     launch_ssor_bwd(ds, T.tile0[c], T.ntc[c], c,
                     (c == C - 1 ? kSsorFirst : 0) | (c == 0 ? kSsorMid : 0) | (C == 1 ? kSsorLast : 0) | kSsorInit,
                     rpt, st);
-  for (int c = 1; c < C; ++c) launch_ssor_fwd(ds, T.tile0[c], T.ntc[c], c, (c == C - 1 ? kSsorLast : 0) | kSsorInit, rpt, st);
+  // the last forward sweep also forms the sums the first sweep of iteration 0 starts from
+  for (int c = 1; c < C; ++c)
+    launch_ssor_fwd(ds, T.tile0[c], T.ntc[c], c, (c == C - 1 ? kSsorLast | kSsorPreWG : 0) | kSsorInit, rpt, st);
 }
 
 // End of an SSOR solve: x = (D~ + U)^-1 y, back to the original row order.
@@ -739,11 +744,14 @@ static PcgState run_ssor(gsh_solver* s, double rtol, int maxiter, int check_ever
   PcgState* slots = ctx->pinned_slots(1);
   // set-up, then the set-up pass: z0 = D~ r0, s0 = A^ z0 and the gamma, delta partials of iteration 0
   ssor_begin<VT>(s, rtol, maxiter, check_every, gather, true, st);
-  // flags of the sweeps (see ssor_kernels.hip): first sweep, colour 0, last of the iteration
+  // flags of the sweeps (see ssor_kernels.hip): first sweep, colour 0, last of the iteration;
+  // with two colours or more the last sweep (a forward one) forms the per-thread sums of the
+  // gamma and delta partials that the next first sweep starts from
   auto bflags = [&](int c) {
-    return (c == C - 1 ? kSsorFirst : 0) | (c == 0 ? kSsorMid : 0) | (C == 1 ? kSsorLast : 0);
+    return (c == C - 1 ? kSsorFirst | (C >= 2 ? kSsorPre : 0) : 0) | (c == 0 ? kSsorMid : 0) |
+           (C == 1 ? kSsorLast : 0);
   };
-  auto fflags = [&](int c) { return c == C - 1 ? kSsorLast : 0; };
+  auto fflags = [&](int c) { return c == C - 1 ? kSsorLast | kSsorPreWG : 0; };
   auto bwd = [&]() {
     for (int c = C - 1; c >= 0; --c) launch_ssor_bwd(ds, S.tile0[c], S.ntc[c], c, bflags(c), rpt, st);
   };

@@ -149,7 +149,8 @@ template <class VT>
 struct SsorSysT {
   int32_t n, np, ntiles, ncol;
   double omega;
-  int32_t unit_omega, pad;                   // omega == 1: 2 D~ - D equals D~ exactly
+  int32_t unit_omega;                        // omega == 1: 2 D~ - D equals D~ exactly
+  int32_t pre_t0;                            // first tile of the last colour
   const SsorColour* colours;                 // [ncol]
   const int32_t* tile_colour;                // [ntiles]
   const int32_t *lcol, *ucol;                // ELL columns (-1 unused)
@@ -164,6 +165,8 @@ struct SsorSysT {
   VT *y, *r, *p, *q, *t, *u, *s, *w;         // iteration vectors [np]: CG y r p q, sweeps t u, s = A^ z
   double *part_rr, *part_bb;                 // [ntiles]
   double *part_rz, *part_pq;                 // [2][ntiles]: gamma and delta partials, slot = iteration & 1
+  double* pre;                               // [2][2][256]: per-thread gamma sums and delta sums below
+                                             // pre_t0, slot = iteration & 1 (NULL: not kept)
   PcgState* st;
 };
 using SsorSys = SsorSysT<double>;

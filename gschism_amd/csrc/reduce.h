@@ -128,6 +128,58 @@ __device__ __forceinline__ void final_reduce2(gptr<const double> P, gptr<const d
   b = block_tree(w, red);
 }
 
+// final_reduce2 of (P, Q) over T tiles from sums formed in an earlier launch: pre[t] is thread
+// t's sum of P over all T tiles and pre[256 + t] its sum of Q over the tiles below T0, both from
+// 0.0 in ascending order (pre_sums); thread t then adds Q of its tiles from T0 on in ascending
+// order, so a and b equal final_reduce2's bit for bit. LO: the first 256 threads of a larger
+// workgroup (final_reduce2_lo).
+template <bool LO>
+__device__ __forceinline__ void final_reduce2_pre(gptr<const double> pre, gptr<const double> Q, int T0, int T,
+                                                  double* red, double& a, double& b) {
+  double u = 0.0, w = 0.0;
+  if (!LO || threadIdx.x < kTpb) {
+    const int t = threadIdx.x;
+    u = pre[t];
+    w = pre[kTpb + t];
+    int m = t;
+    if (m < T0) m += ((T0 - m + kTpb - 1) / kTpb) * kTpb;
+    for (; m < T; m += 8 * kTpb) {
+      double v[8];
+#pragma unroll
+      for (int j = 0; j < 8; ++j) v[j] = (m + j * kTpb < T) ? Q[m + j * kTpb] : 0.0;
+#pragma unroll
+      for (int j = 0; j < 8; ++j)
+        if (m + j * kTpb < T) w += v[j];
+    }
+  }
+  a = LO ? block_tree_lo(u, red) : block_tree(u, red);
+  b = LO ? block_tree_lo(w, red) : block_tree(w, red);
+}
+
+// The sums final_reduce2_pre starts from, by threads 0..255 of one workgroup: thread t's sum of
+// P over tiles t, t + 256, ... below T and of Q over those below T0, each from 0.0 in that order.
+__device__ __forceinline__ void pre_sums(gptr<const double> P, gptr<const double> Q, int T0, int T,
+                                         gptr<double> pre) {
+  const int t = threadIdx.x;
+  double u = 0.0, w = 0.0;
+  for (int m = t; m < T; m += 8 * kTpb) {
+    double vp[8], vq[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      const int i = m + j * kTpb;
+      vp[j] = i < T ? P[i] : 0.0;
+      vq[j] = i < T0 ? Q[i] : 0.0;
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      if (m + j * kTpb < T) u += vp[j];
+      if (m + j * kTpb < T0) w += vq[j];
+    }
+  }
+  pre[t] = u;
+  pre[kTpb + t] = w;
+}
+
 // final_reduce / final_reduce2 by the first 256 threads of a larger workgroup, in the same order.
 __device__ __forceinline__ double final_reduce_lo(gptr<const double> P, int T, double* red) {
   double u = 0.0;

@@ -43,9 +43,10 @@ struct SsorView {
   gptr<const VT> aval;
   gptr<const double> b, h0, h1, h2;
   gptr<VT> xout, lval, uval, d, dt, kd, bp, y, r, p, q, t, u, s, w;
-  gptr<double> part_rr, part_rz, part_pq, part_bb;
+  gptr<double> part_rr, part_rz, part_pq, part_bb, pre;
   gptr<PcgState> st;
   bool has_hist, unit_omega;
+  int pre_t0;
 };
 
 template <class VT>
@@ -89,6 +90,8 @@ __device__ __forceinline__ SsorView<VT> sview(const SsorSysT<VT>* __restrict__ s
   v.part_rz = G(S.part_rz);
   v.part_pq = G(S.part_pq);
   v.part_bb = G(S.part_bb);
+  v.pre = G(S.pre);
+  v.pre_t0 = S.pre_t0;
   v.st = G(S.st);
   return v;
 }
@@ -273,7 +276,10 @@ __global__ void __launch_bounds__(256) k_ssor_r0(const SsorSysT<VT>* __restrict_
 }
 
 // ------------------------------------------------------------------ backward sweep
-constexpr int kFirst = 1, kMid = 2, kLast = 4, kInit = 8, kLead = 16;
+// kPre: the first sweep starts from the sums of pre_sums (S.pre) instead of all tile partials;
+// kPreWG: workgroup 0 of this launch (the last sweep of an iteration) forms those sums for the
+// next iteration and the sweep's tiles go to workgroups 1 and up.
+constexpr int kFirst = 1, kMid = 2, kLast = 4, kInit = 8, kLead = 16, kPre = 32, kPreWG = 64;
 
 // One colour of the backward sweep. Outside the set-up pass the first sweep of an iteration
 // (workgroup 0) turns the gamma and delta partials into alpha and beta and keeps them with
@@ -313,7 +319,10 @@ __device__ __forceinline__ void ssor_bwd_body(const SsorView<VT>& S, int tile, i
       const int k = st->k;
       wslot = (k + 1) & 1;
       double gamma, delta;
-      if (W)
+      if (flags & kPre)
+        final_reduce2_pre<W>(C(S.pre) + (k & 1) * 2 * kTpb, C(S.part_pq) + (k & 1) * S.ntiles, S.pre_t0, S.ntiles,
+                             red, gamma, delta);
+      else if (W)
         final_reduce2_lo(C(S.part_rz) + (k & 1) * S.ntiles, C(S.part_pq) + (k & 1) * S.ntiles, S.ntiles, red, gamma,
                          delta);
       else
@@ -456,6 +465,22 @@ __global__ void __launch_bounds__(W ? kTpb * R : kTpb)
     k_ssor_fwd(const SsorSysT<VT>* __restrict__ sys, int tile0, int c, int flags) {
   __shared__ double red[4];
   __shared__ double lrows[W ? kTpb * R : 1];
+  if (flags & kPreWG) {
+    // every gamma partial and the delta partials of the colours before this one are final (the
+    // sweeps of earlier launches wrote them): workgroup 0 forms the next first sweep's sums
+    const SsorView<VT> S = sview(sys);
+    if (blockIdx.x == 0) {
+      if (load_done(S.st) >= 0) return;
+      const int slot = (flags & kInit) ? 0 : ((S.st->kk + 1) & 1);
+      if (threadIdx.x < kTpb)
+        pre_sums(C(S.part_rz) + slot * S.ntiles, C(S.part_pq) + slot * S.ntiles, S.pre_t0, S.ntiles,
+                 S.pre + slot * 2 * kTpb);
+      return;
+    }
+    ssor_fwd_body<VT, R, W>(S, tile0 + xcd_block(blockIdx.x - 1, gridDim.x - 1), c, flags, blockIdx.x == 1, red,
+                            lrows);
+    return;
+  }
   ssor_fwd_body<VT, R, W>(sview(sys), tile0 + xcd_block(blockIdx.x, gridDim.x), c, flags, blockIdx.x == 0, red,
                           lrows);
 }
@@ -626,7 +651,7 @@ void launch_ssor_bwd(const SsorSysT<VT>* sys, int tile0, int ntc, int c, int fla
 template <class VT>
 void launch_ssor_fwd(const SsorSysT<VT>* sys, int tile0, int ntc, int c, int flags, int rpt, hipStream_t s,
                      hipEvent_t e0, hipEvent_t e1) {
-  if (ntc > 0) GSH_SSOR_WRX(rpt, k_ssor_fwd, ntc, e0, e1, s, sys, tile0, c, flags);
+  if (ntc > 0) GSH_SSOR_WRX(rpt, k_ssor_fwd, ntc + ((flags & kPreWG) ? 1 : 0), e0, e1, s, sys, tile0, c, flags);
 }
 template <class VT>
 void launch_ssor_true_rr(const SsorSysT<VT>* sys, int ntiles, int rpt, hipStream_t s) {
