@@ -637,7 +637,7 @@ struct SsorTimes {
 // colour 0 (forms u and s), last sweep of an iteration, set-up pass, workgroup that writes the
 // system's scalars (batched launches).
 constexpr int kSsorFirst = 1, kSsorMid = 2, kSsorLast = 4, kSsorInit = 8, kSsorLead = 16, kSsorPre = 32,
-              kSsorPreWG = 64;
+              kSsorPreWG = 64, kSsorChk = 128;
 
 // Launch table of a batch of SSOR systems: backward colour steps 0 .. maxC-1 at boff, forward
 // steps 0 .. maxC-2 at foff, then every tile of every system at aoff (true residual).
@@ -740,7 +740,7 @@ static PcgState run_ssor(gsh_solver* s, double rtol, int maxiter, int check_ever
   if (timeout_s <= 0.0) timeout_s = 60.0;
   SsorState& S = *s->ssor;
   const SsorSysT<VT>* ds = ssor_dev<VT>(S);
-  const int rpt = S.rpt, C = S.L.ncol, NT = S.ntiles;
+  const int rpt = S.rpt, C = S.L.ncol;
   PcgState* slots = ctx->pinned_slots(1);
   // set-up, then the set-up pass: z0 = D~ r0, s0 = A^ z0 and the gamma, delta partials of iteration 0
   ssor_begin<VT>(s, rtol, maxiter, check_every, gather, true, st);
@@ -752,11 +752,12 @@ static PcgState run_ssor(gsh_solver* s, double rtol, int maxiter, int check_ever
            (C == 1 ? kSsorLast : 0);
   };
   auto fflags = [&](int c) { return c == C - 1 ? kSsorLast | kSsorPreWG : 0; };
-  auto bwd = [&]() {
-    for (int c = C - 1; c >= 0; --c) launch_ssor_bwd(ds, S.tile0[c], S.ntc[c], c, bflags(c), rpt, st);
+  // chk: the iteration closes a window, so its sweeps also write the true-residual partials
+  auto bwd = [&](int chk) {
+    for (int c = C - 1; c >= 0; --c) launch_ssor_bwd(ds, S.tile0[c], S.ntc[c], c, bflags(c) | chk, rpt, st);
   };
-  auto fwd = [&]() {
-    for (int c = 1; c < C; ++c) launch_ssor_fwd(ds, S.tile0[c], S.ntc[c], c, fflags(c), rpt, st);
+  auto fwd = [&](int chk) {
+    for (int c = 1; c < C; ++c) launch_ssor_fwd(ds, S.tile0[c], S.ntc[c], c, fflags(c) | chk, rpt, st);
   };
   HIPCHK(hipGetLastError());
   int issued = 0, window = 0;
@@ -783,10 +784,11 @@ static PcgState run_ssor(gsh_solver* s, double rtol, int maxiter, int check_ever
       for (auto& x : e) HIPCHK(hipEventCreate(&x));
       for (int i = 0; i < w; ++i) {
         int l = 0;
+        const int chk = i == w - 1 ? kSsorChk : 0;
         for (int c = C - 1; c >= 0; --c, ++l)
-          launch_ssor_bwd(ds, S.tile0[c], S.ntc[c], c, bflags(c), rpt, st, e[2 * l], e[2 * l + 1]);
+          launch_ssor_bwd(ds, S.tile0[c], S.ntc[c], c, bflags(c) | chk, rpt, st, e[2 * l], e[2 * l + 1]);
         for (int c = 1; c < C; ++c, ++l)
-          launch_ssor_fwd(ds, S.tile0[c], S.ntc[c], c, fflags(c), rpt, st, e[2 * l], e[2 * l + 1]);
+          launch_ssor_fwd(ds, S.tile0[c], S.ntc[c], c, fflags(c) | chk, rpt, st, e[2 * l], e[2 * l + 1]);
         HIPCHK(hipEventSynchronize(e[2 * (l - 1) + 1]));
         for (int m = 0; m < l; ++m) {
           float a = 0.f;
@@ -796,15 +798,14 @@ static PcgState run_ssor(gsh_solver* s, double rtol, int maxiter, int check_ever
         prof->iters += 1;
       }
       for (auto& x : e) (void)hipEventDestroy(x);
-      launch_ssor_true_rr(ds, NT, rpt, st);
       launch_ssor_check(ds, st);
     } else {
       auto issue = [&]() {
         for (int i = 0; i < w; ++i) {
-          bwd();
-          fwd();
+          const int chk = i == w - 1 ? kSsorChk : 0;
+          bwd(chk);
+          fwd(chk);
         }
-        launch_ssor_true_rr(ds, NT, rpt, st);
         launch_ssor_check(ds, st);
       };
       if (!(use_graph && launch_window_graph(s->wg_ssor, st, ds, w, rpt, issue))) issue();

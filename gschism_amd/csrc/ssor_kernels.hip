@@ -176,6 +176,35 @@ __device__ __forceinline__ VT ell_sum(gptr<const int32_t> ec, gptr<const VT> ev,
   return acc;
 }
 
+// ell_sum of one row part against two vectors at once (each sum in entry order, from 0.0): the
+// columns and values are loaded once for both.
+template <class VT>
+__device__ __forceinline__ void ell_sum2(gptr<const int32_t> ec, gptr<const VT> ev, gptr<const VT> x, gptr<const VT> y,
+                                         int base, int stride, int width, VT& ax, VT& ay) {
+  VT sx = 0, sy = 0;
+  for (int m0 = 0; m0 < width; m0 += kEllChunk) {
+    int c[kEllChunk];
+    VT vv[kEllChunk], xx[kEllChunk], yy[kEllChunk];
+#pragma unroll
+    for (int m = 0; m < kEllChunk; ++m) c[m] = (m0 + m < width) ? ec[base + (m0 + m) * stride] : -1;
+#pragma unroll
+    for (int m = 0; m < kEllChunk; ++m)
+      if (c[m] >= 0) {
+        vv[m] = ev[base + (m0 + m) * stride];
+        xx[m] = x[c[m]];
+        yy[m] = y[c[m]];
+      }
+#pragma unroll
+    for (int m = 0; m < kEllChunk; ++m)
+      if (c[m] >= 0) {
+        sx += vv[m] * xx[m];
+        sy += vv[m] * yy[m];
+      }
+  }
+  ax = sx;
+  ay = sy;
+}
+
 template <class VT>
 __device__ __forceinline__ VT lower_sum(const SsorView<VT>& S, const ColourInfo& ci, int v, gptr<const VT> x) {
   return ell_sum(S.lcol, C(S.lval), x, ci.offl + (v - ci.start), ci.rows, ci.wl);
@@ -279,7 +308,10 @@ __global__ void __launch_bounds__(256) k_ssor_r0(const SsorSysT<VT>* __restrict_
 // kPre: the first sweep starts from the sums of pre_sums (S.pre) instead of all tile partials;
 // kPreWG: workgroup 0 of this launch (the last sweep of an iteration) forms those sums for the
 // next iteration and the sweep's tiles go to workgroups 1 and up.
-constexpr int kFirst = 1, kMid = 2, kLast = 4, kInit = 8, kLead = 16, kPre = 32, kPreWG = 64;
+// kChk: the iteration closes a window; its sweeps also write the partials of ||(D~ + L) r||^2
+// of their rows (colour 0 in the backward sweep, the others in the forward sweeps), which
+// k_ssor_true_rr writes otherwise, in the same order per tile.
+constexpr int kFirst = 1, kMid = 2, kLast = 4, kInit = 8, kLead = 16, kPre = 32, kPreWG = 64, kChk = 128;
 
 // One colour of the backward sweep. Outside the set-up pass the first sweep of an iteration
 // (workgroup 0) turns the gamma and delta partials into alpha and beta and keeps them with
@@ -295,7 +327,7 @@ __device__ __forceinline__ void ssor_bwd_body(const SsorView<VT>& S, int tile, i
   const bool init = flags & kInit, mid = flags & kMid;
   const ColourInfo ci = colour_info(S, c);
   const gptr<const VT> t = C(S.t);
-  VT usum[J], dtv[J], rv[J], pv[J], qv[J], sv[J], yv[J];
+  VT usum[J], dtv[J], rv[J], pv[J], qv[J], sv[J], yv[J], rsv[J];
   // the row loads are issued before the done test so that their latency overlaps it
 #pragma unroll
   for (int j = 0; j < J; ++j) {
@@ -379,6 +411,7 @@ __device__ __forceinline__ void ssor_bwd_body(const SsorView<VT>& S, int tile, i
       r1 = rv[j] - av * qn;
       vst(S.r, v, r1);
     }
+    rsv[j] = r1;
     const VT zn = dtv[j] * r1;
     sacc<W>(g, (double)r1 * (double)zn);
     const VT tv = (zn - usum[j]) / dtv[j];
@@ -394,6 +427,17 @@ __device__ __forceinline__ void ssor_bwd_body(const SsorView<VT>& S, int tile, i
   }
   g = stree<R, W>(g, lrows, red);
   if (threadIdx.x == 0) S.part_rz[wslot * S.ntiles + tile] = g;
+  if (mid && (flags & kChk)) {
+    // colour 0 has no lower part: (D~ + L) r is D~ r
+    double rr = 0.0;
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      const VT wv = dtv[j] * rsv[j] + (VT)0;
+      sacc<W>(rr, (double)wv * (double)wv);
+    }
+    rr = stree<R, W>(rr, lrows, red);
+    if (threadIdx.x == 0) S.part_rr[tile] = rr;
+  }
   if (mid) {
     dl = stree<R, W>(dl, lrows, red);
     if (threadIdx.x == 0) S.part_pq[wslot * S.ntiles + tile] = dl;
@@ -424,7 +468,9 @@ __global__ void __launch_bounds__(256) k_ssor_bwd_batch(const SsorSysT<VT>* __re
 // ------------------------------------------------------------------ forward sweep
 // u = (D~ + L)^-1 (z - K t) for one colour, s = t + u = A^ z, partials of z.s. The last sweep of
 // an iteration advances the iteration counter.
-template <class VT, int R, bool W = false>
+// CK: the check iteration's sweep (kChk), a separate instantiation so that the other sweeps keep
+// their register count.
+template <class VT, int R, bool W = false, bool CK = false>
 __device__ __forceinline__ void ssor_fwd_body(const SsorView<VT>& S, int tile, int c, int flags, bool lead,
                                               double* red, double* lrows) {
   constexpr int J = W ? 1 : R;  // rows of this thread
@@ -442,8 +488,15 @@ __device__ __forceinline__ void ssor_fwd_body(const SsorView<VT>& S, int tile, i
   }
   if (load_done(S.st) >= 0) return;
   const int wslot = (flags & kInit) ? 0 : ((S.st->kk + 1) & 1);
+  VT rlo[J];  // (L r) of the rows (CK)
 #pragma unroll
-  for (int j = 0; j < J; ++j) lsum[j] = lower_sum(S, ci, srow<R, W>(tile, j), u);
+  for (int j = 0; j < J; ++j) {
+    const int v = srow<R, W>(tile, j);
+    if (CK)
+      ell_sum2(S.lcol, C(S.lval), u, C(S.r), ci.offl + (v - ci.start), ci.rows, ci.wl, lsum[j], rlo[j]);
+    else
+      lsum[j] = lower_sum(S, ci, v, u);
+  }
   double dl = 0.0;
 #pragma unroll
   for (int j = 0; j < J; ++j) {
@@ -457,14 +510,22 @@ __device__ __forceinline__ void ssor_fwd_body(const SsorView<VT>& S, int tile, i
   }
   dl = stree<R, W>(dl, lrows, red);
   if (threadIdx.x == 0) S.part_pq[wslot * S.ntiles + tile] = dl;
+  if (CK) {
+    double rr = 0.0;
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      const VT wv = dtv[j] * rv[j] + rlo[j];
+      sacc<W>(rr, (double)wv * (double)wv);
+    }
+    rr = stree<R, W>(rr, lrows, red);
+    if (threadIdx.x == 0) S.part_rr[tile] = rr;
+  }
   if ((flags & kLast) && !(flags & kInit) && lead && threadIdx.x == 0) S.st->k = S.st->kk + 1;
 }
 
-template <class VT, int R, bool W = false>
-__global__ void __launch_bounds__(W ? kTpb * R : kTpb)
-    k_ssor_fwd(const SsorSysT<VT>* __restrict__ sys, int tile0, int c, int flags) {
-  __shared__ double red[4];
-  __shared__ double lrows[W ? kTpb * R : 1];
+template <class VT, int R, bool W, bool CK>
+__device__ __forceinline__ void ssor_fwd_kernel(const SsorSysT<VT>* __restrict__ sys, int tile0, int c, int flags,
+                                                double* red, double* lrows) {
   if (flags & kPreWG) {
     // every gamma partial and the delta partials of the colours before this one are final (the
     // sweeps of earlier launches wrote them): workgroup 0 forms the next first sweep's sums
@@ -477,12 +538,29 @@ __global__ void __launch_bounds__(W ? kTpb * R : kTpb)
                  S.pre + slot * 2 * kTpb);
       return;
     }
-    ssor_fwd_body<VT, R, W>(S, tile0 + xcd_block(blockIdx.x - 1, gridDim.x - 1), c, flags, blockIdx.x == 1, red,
-                            lrows);
+    ssor_fwd_body<VT, R, W, CK>(S, tile0 + xcd_block(blockIdx.x - 1, gridDim.x - 1), c, flags, blockIdx.x == 1, red,
+                                lrows);
     return;
   }
-  ssor_fwd_body<VT, R, W>(sview(sys), tile0 + xcd_block(blockIdx.x, gridDim.x), c, flags, blockIdx.x == 0, red,
-                          lrows);
+  ssor_fwd_body<VT, R, W, CK>(sview(sys), tile0 + xcd_block(blockIdx.x, gridDim.x), c, flags, blockIdx.x == 0, red,
+                              lrows);
+}
+
+template <class VT, int R, bool W = false>
+__global__ void __launch_bounds__(W ? kTpb * R : kTpb)
+    k_ssor_fwd(const SsorSysT<VT>* __restrict__ sys, int tile0, int c, int flags) {
+  __shared__ double red[4];
+  __shared__ double lrows[W ? kTpb * R : 1];
+  ssor_fwd_kernel<VT, R, W, false>(sys, tile0, c, flags, red, lrows);
+}
+
+// the check iteration's forward sweeps (kChk)
+template <class VT, int R, bool W = false>
+__global__ void __launch_bounds__(W ? kTpb * R : kTpb)
+    k_ssor_fwd_chk(const SsorSysT<VT>* __restrict__ sys, int tile0, int c, int flags) {
+  __shared__ double red[4];
+  __shared__ double lrows[W ? kTpb * R : 1];
+  ssor_fwd_kernel<VT, R, W, true>(sys, tile0, c, flags, red, lrows);
 }
 
 template <class VT, int R>
@@ -651,7 +729,13 @@ void launch_ssor_bwd(const SsorSysT<VT>* sys, int tile0, int ntc, int c, int fla
 template <class VT>
 void launch_ssor_fwd(const SsorSysT<VT>* sys, int tile0, int ntc, int c, int flags, int rpt, hipStream_t s,
                      hipEvent_t e0, hipEvent_t e1) {
-  if (ntc > 0) GSH_SSOR_WRX(rpt, k_ssor_fwd, ntc + ((flags & kPreWG) ? 1 : 0), e0, e1, s, sys, tile0, c, flags);
+  if (ntc <= 0) return;
+  const int grid = ntc + ((flags & kPreWG) ? 1 : 0);
+  if (flags & kChk) {
+    GSH_SSOR_WRX(rpt, k_ssor_fwd_chk, grid, e0, e1, s, sys, tile0, c, flags);
+  } else {
+    GSH_SSOR_WRX(rpt, k_ssor_fwd, grid, e0, e1, s, sys, tile0, c, flags);
+  }
 }
 template <class VT>
 void launch_ssor_true_rr(const SsorSysT<VT>* sys, int ntiles, int rpt, hipStream_t s) {
