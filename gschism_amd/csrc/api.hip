@@ -213,6 +213,8 @@ struct SsorState {
   SsorSys* ds = nullptr;
   SsorSysT<float> hs32{};       // mixed precision: the FP32 inner solver
   SsorSysT<float>* ds32 = nullptr;
+  SsorSysT<float> up32{};       // the FP32 descriptor as last uploaded to ds32
+  bool up32_valid = false;
   int rpt = 1, ntiles = 0;
   std::vector<int> tile0, ntc;  // tiles of each colour
 };
@@ -457,8 +459,12 @@ static void ssor_sync_desc32(gsh_solver* s, const double* rhs) {
   h.h0 = h.h1 = h.h2 = nullptr;
   h.omega = s->omega;
   h.unit_omega = s->omega == 1.0;
+  // the inner solves of one mixed solve share the descriptor: upload it only when it changed
+  if (s->ssor->up32_valid && std::memcmp(&s->ssor->up32, &h, sizeof(h)) == 0) return;
   HIPCHK(hipMemcpyAsync(s->ssor->ds32, &h, sizeof(h), hipMemcpyHostToDevice, s->ctx->stream));
   HIPCHK(hipStreamSynchronize(s->ctx->stream));
+  s->ssor->up32 = h;
+  s->ssor->up32_valid = true;
 }
 
 // rollHistory (pcg_solver.py:941-947) by pointer rotation: x2 <- x1 <- x0 <- x.
@@ -640,11 +646,12 @@ constexpr int kSsorFirst = 1, kSsorMid = 2, kSsorLast = 4, kSsorInit = 8, kSsorL
               kSsorPreWG = 64, kSsorChk = 128;
 
 // Launch table of a batch of SSOR systems: backward colour steps 0 .. maxC-1 at boff, forward
-// steps 0 .. maxC-2 at foff, then every tile of every system at aoff (true residual).
+// steps 0 .. maxC-2 at foff (with each system's pre-sum entry in the step of its last forward
+// sweep).
 struct SsorTable {
   std::vector<SsorRef> refs;
   std::vector<int> boff, foff;
-  int aoff = 0, nall = 0, maxC = 0;
+  int maxC = 0;
 };
 
 static SsorTable ssor_table(gsh_solver* const* S, int count) {
@@ -658,13 +665,16 @@ static SsorTable ssor_table(gsh_solver* const* S, int count) {
     const SsorState& T = *S[i]->ssor;
     for (int t = 0; t < T.ntc[c]; ++t) refs.push_back({i, T.tile0[c] + t, c, flags | (t == 0 ? kSsorLead : 0)});
   };
+  // a system of two colours or more starts its first sweep from the sums that an extra entry
+  // (tile -1, kSsorPreWG) forms in the step of its last forward sweep
   for (int step = 0; step < maxC; ++step) {
     tb.boff[step] = (int)refs.size();
     for (int i = 0; i < count; ++i) {
       const int C = S[i]->ssor->L.ncol;
       if (step >= C) continue;
       const int c = C - 1 - step;
-      add_colour(i, c, (step == 0 ? kSsorFirst : 0) | (c == 0 ? kSsorMid : 0) | (C == 1 ? kSsorLast : 0));
+      add_colour(i, c, (step == 0 ? kSsorFirst | (C >= 2 ? kSsorPre : 0) : 0) | (c == 0 ? kSsorMid : 0) |
+                           (C == 1 ? kSsorLast : 0));
     }
   }
   tb.boff[maxC] = (int)refs.size();
@@ -673,16 +683,14 @@ static SsorTable ssor_table(gsh_solver* const* S, int count) {
     for (int i = 0; i < count; ++i) {
       const int C = S[i]->ssor->L.ncol, c = step + 1;
       if (c < C) add_colour(i, c, c == C - 1 ? kSsorLast : 0);
+      if (c == C - 1) refs.push_back({i, -1, c, kSsorPreWG});
     }
   }
   tb.foff[std::max(0, maxC - 1)] = (int)refs.size();
-  tb.aoff = (int)refs.size();
-  for (int i = 0; i < count; ++i)
-    for (int t = 0; t < S[i]->ssor->ntiles; ++t) refs.push_back({i, t, 0, 0});
-  tb.nall = (int)refs.size() - tb.aoff;
-  for (const SsorRef& r : refs)  // every entry names a tile of its own system
-    GSH_REQUIRE(r.sys >= 0 && r.sys < count && r.tile >= 0 && r.tile < S[r.sys]->ssor->ntiles, GSH_E_STATE,
-                "bad batched SSOR table");
+  for (const SsorRef& r : refs)  // every entry names a tile of its own system, or is a pre-sum entry
+    GSH_REQUIRE(r.sys >= 0 && r.sys < count &&
+                    ((r.tile >= 0 && r.tile < S[r.sys]->ssor->ntiles) || (r.tile == -1 && r.flags == kSsorPreWG)),
+                GSH_E_STATE, "bad batched SSOR table");
   return tb;
 }
 
@@ -881,8 +889,8 @@ static void run_ssor_batch(gsh_solver* const* S, int count, const std::vector<do
     while (!all_done && nflight > 0 && pacer.expect_done(window - 1)) all_done = check_oldest();
     if (all_done) break;
     const int w = std::min(check_every, maxiter - issued);
-    for (int it = 0; it < w; ++it) ssor_table_sweeps(dsys, drefs, tb, 0, rpt, st);
-    launch_ssor_true_rr_batch(dsys, drefs + tb.aoff, tb.nall, rpt, st);
+    // the window's last iteration also writes the true-residual partials
+    for (int it = 0; it < w; ++it) ssor_table_sweeps(dsys, drefs, tb, it == w - 1 ? kSsorChk : 0, rpt, st);
     launch_ssor_check_batch(dsys, count, st);
     issued += w;
     const int slot = window & 1;
@@ -943,15 +951,22 @@ static void run_mixed(gsh_ctx* ctx, gsh_solver* const* S, int count, double rtol
   std::vector<int> outer(count, 0), inner(count, 0), state(count, 0);  // 0 running, 1 converged, 2 stopped
   const int max_outer = 30;
   std::vector<char> ssor_gathered(count, 0);  // SSOR inner solves: gathered by each system's first
+  const bool ssor = S[0]->precond == GSH_PRECOND_SSOR;
+  TileRef* dres = ssor ? tmp.alloc<TileRef>((size_t)all_tiles) : dtl;  // residual tiles (SSOR: kept)
+  std::vector<TileRef> tl, tl_up;  // residual tiles of the running systems; as last uploaded to dres
   for (int round = 0; round < max_outer; ++round) {
     // FP64 true residual of every running system
-    std::vector<TileRef> tl;
+    tl.clear();
     for (int i = 0; i < count; ++i)
       if (state[i] == 0)
         for (int t = 0; t < S[i]->ntiles; ++t) tl.push_back({i, t});
     if (tl.empty()) break;
-    HIPCHK(hipMemcpyAsync(dtl, tl.data(), sizeof(TileRef) * tl.size(), hipMemcpyHostToDevice, st));
-    launch_residual_batch(d64, dtl, (int)tl.size(), rpt, st);
+    if (!ssor || tl.size() != tl_up.size() ||
+        std::memcmp(tl.data(), tl_up.data(), sizeof(TileRef) * tl.size()) != 0) {
+      HIPCHK(hipMemcpyAsync(dres, tl.data(), sizeof(TileRef) * tl.size(), hipMemcpyHostToDevice, st));
+      tl_up = tl;
+    }
+    launch_residual_batch(d64, dres, (int)tl.size(), rpt, st);
     launch_reduce_systems(d64, count, dred, st);
     HIPCHK(hipMemcpyAsync(red.data(), dred, sizeof(double) * 2 * count, hipMemcpyDeviceToHost, st));
     HIPCHK(hipEventRecord(ctx->ev[0], st));
@@ -984,20 +999,8 @@ static void run_mixed(gsh_ctx* ctx, gsh_solver* const* S, int count, double rtol
     if (sub.empty()) break;
     // inner FP32 solves A d = r of the subset, lock-step, from d = 0
     const int ns = (int)sub.size();
-    std::vector<PcgSys<double>> s64(ns);
-    std::vector<PcgSys<float>> s32(ns);
-    std::vector<TileRef> stl;
-    for (int j = 0; j < ns; ++j) {
-      s64[j] = h64[sub[j]];
-      s32[j] = h32[sub[j]];
-      for (int t = 0; t < S[sub[j]]->ntiles; ++t) stl.push_back({j, t});
-    }
-    HIPCHK(hipMemcpyAsync(d64s, s64.data(), sizeof(PcgSys<double>) * ns, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(d32s, s32.data(), sizeof(PcgSys<float>) * ns, hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(dtl, stl.data(), sizeof(TileRef) * stl.size(), hipMemcpyHostToDevice, st));
-    HIPCHK(hipMemcpyAsync(drt, rts.data(), sizeof(double) * ns, hipMemcpyHostToDevice, st));
     PcgOutcome o;
-    if (S[0]->precond == GSH_PRECOND_SSOR) {
+    if (ssor) {
       // FP32 SSOR PCG of A d = r for the subset, b = each system's FP64 residual; the first inner
       // solve of a system gathers its FP32 matrix, the later ones reuse it
       std::vector<gsh_solver*> ss(ns);
@@ -1012,11 +1015,25 @@ static void run_mixed(gsh_ctx* ctx, gsh_solver* const* S, int count, double rtol
         o.st.assign(1, run_ssor<float>(ss[0], rts[0], maxiter, check_every, timeout_s, nullptr, false, gath[0] != 0));
       else
         run_ssor_batch<float>(ss.data(), ns, rts, maxiter, check_every, timeout_s, gath, o.st);
+      // x += d, d the FP32 correction in the original row order
+      for (int j = 0; j < ns; ++j) launch_refine_update(ss[j]->x, ss[j]->hs32.x, ss[j]->n, st);
     } else {
+      std::vector<PcgSys<double>> s64(ns);
+      std::vector<PcgSys<float>> s32(ns);
+      std::vector<TileRef> stl;
+      for (int j = 0; j < ns; ++j) {
+        s64[j] = h64[sub[j]];
+        s32[j] = h32[sub[j]];
+        for (int t = 0; t < S[sub[j]]->ntiles; ++t) stl.push_back({j, t});
+      }
+      HIPCHK(hipMemcpyAsync(d64s, s64.data(), sizeof(PcgSys<double>) * ns, hipMemcpyHostToDevice, st));
+      HIPCHK(hipMemcpyAsync(d32s, s32.data(), sizeof(PcgSys<float>) * ns, hipMemcpyHostToDevice, st));
+      HIPCHK(hipMemcpyAsync(dtl, stl.data(), sizeof(TileRef) * stl.size(), hipMemcpyHostToDevice, st));
+      HIPCHK(hipMemcpyAsync(drt, rts.data(), sizeof(double) * ns, hipMemcpyHostToDevice, st));
       run_pcg<float>(ctx, d32s, ns, dtl, (int)stl.size(), rpt, rtol, maxiter, check_every, timeout_s, o, nullptr, -1,
                      nullptr, drt);
+      launch_refine_batch(d64s, d32s, dtl, (int)stl.size(), rpt, st);
     }
-    launch_refine_batch(d64s, d32s, dtl, (int)stl.size(), rpt, st);
     for (int j = 0; j < ns; ++j) {
       const int it = iters_of(o.st[j], maxiter);
       inner[sub[j]] += it;
@@ -2393,8 +2410,8 @@ static void server_step_ssor(gsh_server* srv) {
   ensure_cap(srv->dgather, srv->dgather_cap, na);
   HIPCHK(hipMemcpyAsync(srv->dssor, hsys.data(), sizeof(SsorSys) * na, hipMemcpyHostToDevice, st));
   HIPCHK(hipMemcpyAsync(srv->drefs, tb.refs.data(), sizeof(SsorRef) * tb.refs.size(), hipMemcpyHostToDevice, st));
-  for (int i = 0; i < cfg.check_every; ++i) ssor_table_sweeps<double>(srv->dssor, srv->drefs, tb, 0, rpt, st);
-  launch_ssor_true_rr_batch(srv->dssor, srv->drefs + tb.aoff, tb.nall, rpt, st);
+  for (int i = 0; i < cfg.check_every; ++i)
+    ssor_table_sweeps<double>(srv->dssor, srv->drefs, tb, i == cfg.check_every - 1 ? kSsorChk : 0, rpt, st);
   launch_ssor_check_batch(srv->dssor, na, st);
   launch_ssor_gather_states(srv->dssor, na, srv->dgather, st);
   std::vector<PcgState> sts(na);
@@ -2513,8 +2530,8 @@ static void server_step_mixed(gsh_server* srv, const std::vector<Request*>& fres
     ensure_cap(srv->dgather, srv->dgather_cap, ni);
     HIPCHK(hipMemcpyAsync(srv->dssor32, h32.data(), sizeof(SsorSysT<float>) * ni, hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(srv->drefs, tb.refs.data(), sizeof(SsorRef) * tb.refs.size(), hipMemcpyHostToDevice, st));
-    for (int i = 0; i < cfg.check_every; ++i) ssor_table_sweeps<float>(srv->dssor32, srv->drefs, tb, 0, rpt, st);
-    launch_ssor_true_rr_batch(srv->dssor32, srv->drefs + tb.aoff, tb.nall, rpt, st);
+    for (int i = 0; i < cfg.check_every; ++i)
+      ssor_table_sweeps<float>(srv->dssor32, srv->drefs, tb, i == cfg.check_every - 1 ? kSsorChk : 0, rpt, st);
     launch_ssor_check_batch(srv->dssor32, ni, st);
     launch_ssor_gather_states(srv->dssor32, ni, srv->dgather, st);
     std::vector<PcgState> sts(ni);

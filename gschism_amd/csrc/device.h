@@ -268,8 +268,6 @@ template <class VT>
 void launch_ssor_fwd(const SsorSysT<VT>* sys, int tile0, int ntc, int c, int flags, int rpt, hipStream_t s,
                      hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr);
 template <class VT>
-void launch_ssor_true_rr(const SsorSysT<VT>* sys, int ntiles, int rpt, hipStream_t s);
-template <class VT>
 void launch_ssor_check(const SsorSysT<VT>* sys, hipStream_t s);
 template <class VT>
 void launch_ssor_xsolve(const SsorSysT<VT>* sys, int tile0, int ntc, int c, int rpt, hipStream_t s);
@@ -283,8 +281,6 @@ void launch_ssor_bwd_batch(const SsorSysT<VT>* sys, const SsorRef* refs, int nre
 template <class VT>
 void launch_ssor_fwd_batch(const SsorSysT<VT>* sys, const SsorRef* refs, int nrefs, int or_flags, int rpt,
                            hipStream_t s);
-template <class VT>
-void launch_ssor_true_rr_batch(const SsorSysT<VT>* sys, const SsorRef* refs, int nrefs, int rpt, hipStream_t s);
 template <class VT>
 void launch_ssor_check_batch(const SsorSysT<VT>* sys, int count, hipStream_t s);
 template <class VT>

@@ -309,8 +309,8 @@ __global__ void __launch_bounds__(256) k_ssor_r0(const SsorSysT<VT>* __restrict_
 // kPreWG: workgroup 0 of this launch (the last sweep of an iteration) forms those sums for the
 // next iteration and the sweep's tiles go to workgroups 1 and up.
 // kChk: the iteration closes a window; its sweeps also write the partials of ||(D~ + L) r||^2
-// of their rows (colour 0 in the backward sweep, the others in the forward sweeps), which
-// k_ssor_true_rr writes otherwise, in the same order per tile.
+// of their rows (colour 0 in the backward sweep, the others in the forward sweeps): per tile,
+// thread t's rows' (D~ r + L r)^2 summed in row order, then the block tree.
 constexpr int kFirst = 1, kMid = 2, kLast = 4, kInit = 8, kLead = 16, kPre = 32, kPreWG = 64, kChk = 128;
 
 // One colour of the backward sweep. Outside the set-up pass the first sweep of an iteration
@@ -563,48 +563,42 @@ __global__ void __launch_bounds__(W ? kTpb * R : kTpb)
   ssor_fwd_kernel<VT, R, W, true>(sys, tile0, c, flags, red, lrows);
 }
 
+// An entry with tile -1 (kPreWG) forms its system's pre_sums in the step of the system's last
+// forward sweep (whose other partials are final by then).
+template <class VT, int R, bool CK>
+__device__ __forceinline__ void ssor_fwd_batch_entry(const SsorSysT<VT>* __restrict__ sys,
+                                                     const SsorRef* __restrict__ refs, int or_flags, double* red) {
+  const SsorRef ref = refs[xcd_block(blockIdx.x, gridDim.x)];
+  const int f = ref.flags | or_flags;
+  const SsorView<VT> S = sview(sys + ref.sys);
+  if (ref.tile < 0) {
+    if (load_done(S.st) >= 0) return;
+    const int slot = (f & kInit) ? 0 : ((S.st->kk + 1) & 1);
+    pre_sums(C(S.part_rz) + slot * S.ntiles, C(S.part_pq) + slot * S.ntiles, S.pre_t0, S.ntiles,
+             S.pre + slot * 2 * kTpb);
+    return;
+  }
+  ssor_fwd_body<VT, R, false, CK>(S, ref.tile, ref.c, f, (f & kLead) != 0, red, nullptr);
+}
+
 template <class VT, int R>
 __global__ void __launch_bounds__(256) k_ssor_fwd_batch(const SsorSysT<VT>* __restrict__ sys,
                                                         const SsorRef* __restrict__ refs, int or_flags) {
   __shared__ double red[4];
-  const SsorRef ref = refs[xcd_block(blockIdx.x, gridDim.x)];
-  const int f = ref.flags | or_flags;
-  ssor_fwd_body<VT, R>(sview(sys + ref.sys), ref.tile, ref.c, f, (f & kLead) != 0, red, nullptr);
+  ssor_fwd_batch_entry<VT, R, false>(sys, refs, or_flags, red);
+}
+
+// the check iteration's forward steps (kChk in or_flags)
+template <class VT, int R>
+__global__ void __launch_bounds__(256) k_ssor_fwd_batch_chk(const SsorSysT<VT>* __restrict__ sys,
+                                                            const SsorRef* __restrict__ refs, int or_flags) {
+  __shared__ double red[4];
+  ssor_fwd_batch_entry<VT, R, true>(sys, refs, or_flags, red);
 }
 
 // ------------------------------------------------------------------ stopping test
-// ||b - A x||^2 = ||(D~ + L) r||^2 partials, then the test of iteration k-1 (pcg_loop.cpp:405-412)
-// in one workgroup. Issued after the last iteration of every window of check_every.
-template <class VT, int R>
-__device__ __forceinline__ void ssor_true_rr_body(const SsorView<VT>& S, int tile, double* red) {
-  if (load_done(S.st) >= 0) return;
-  const ColourInfo ci = colour_info(S, S.tile_colour[tile]);
-  const gptr<const VT> r = C(S.r);
-  double rr = 0.0;
-  for (int j = 0; j < R; ++j) {
-    const int v = tile_row<R>(tile, j);
-    const VT wv = S.dt[v] * r[v] + lower_sum(S, ci, v, r);
-    rr += (double)wv * (double)wv;
-  }
-  rr = block_tree(rr, red);
-  if (threadIdx.x == 0) S.part_rr[tile] = rr;
-}
-
-template <class VT, int R>
-__global__ void __launch_bounds__(256) k_ssor_true_rr(const SsorSysT<VT>* __restrict__ sys) {
-  __shared__ double red[4];
-  ssor_true_rr_body<VT, R>(sview(sys), xcd_block(blockIdx.x, gridDim.x), red);
-}
-
-// table entries: system and tile (the colour and flags are unused)
-template <class VT, int R>
-__global__ void __launch_bounds__(256) k_ssor_true_rr_batch(const SsorSysT<VT>* __restrict__ sys,
-                                                            const SsorRef* __restrict__ refs) {
-  __shared__ double red[4];
-  const SsorRef ref = refs[xcd_block(blockIdx.x, gridDim.x)];
-  ssor_true_rr_body<VT, R>(sview(sys + ref.sys), ref.tile, red);
-}
-
+// ||b - A x||^2 = ||(D~ + L) r||^2 from the tile partials that the sweeps of a window's last
+// iteration write (kChk), then the test of iteration k-1 (pcg_loop.cpp:405-412) in one workgroup.
 // workgroup b tests system b (one system: b = 0)
 template <class VT>
 __global__ void __launch_bounds__(256) k_ssor_check(const SsorSysT<VT>* __restrict__ sys) {
@@ -738,10 +732,6 @@ void launch_ssor_fwd(const SsorSysT<VT>* sys, int tile0, int ntc, int c, int fla
   }
 }
 template <class VT>
-void launch_ssor_true_rr(const SsorSysT<VT>* sys, int ntiles, int rpt, hipStream_t s) {
-  if (ntiles > 0) GSH_SSOR_R(rpt, k_ssor_true_rr, ntiles, s, sys);
-}
-template <class VT>
 void launch_ssor_check(const SsorSysT<VT>* sys, hipStream_t s) {
   hipLaunchKernelGGL(k_ssor_check<VT>, dim3(1), dim3(kTpb), 0, s, sys);
 }
@@ -769,11 +759,12 @@ void launch_ssor_bwd_batch(const SsorSysT<VT>* sys, const SsorRef* refs, int nre
 template <class VT>
 void launch_ssor_fwd_batch(const SsorSysT<VT>* sys, const SsorRef* refs, int nrefs, int or_flags, int rpt,
                            hipStream_t s) {
-  if (nrefs > 0) GSH_SSOR_R(rpt, k_ssor_fwd_batch, nrefs, s, sys, refs, or_flags);
-}
-template <class VT>
-void launch_ssor_true_rr_batch(const SsorSysT<VT>* sys, const SsorRef* refs, int nrefs, int rpt, hipStream_t s) {
-  if (nrefs > 0) GSH_SSOR_R(rpt, k_ssor_true_rr_batch, nrefs, s, sys, refs);
+  if (nrefs <= 0) return;
+  if (or_flags & kChk) {
+    GSH_SSOR_R(rpt, k_ssor_fwd_batch_chk, nrefs, s, sys, refs, or_flags);
+  } else {
+    GSH_SSOR_R(rpt, k_ssor_fwd_batch, nrefs, s, sys, refs, or_flags);
+  }
 }
 template <class VT>
 void launch_ssor_check_batch(const SsorSysT<VT>* sys, int count, hipStream_t s) {
@@ -793,13 +784,11 @@ void launch_ssor_gather_states(const SsorSysT<VT>* sys, int count, PcgState* out
                                     hipEvent_t);                                                       \
   template void launch_ssor_fwd<VT>(const SsorSysT<VT>*, int, int, int, int, int, hipStream_t, hipEvent_t, \
                                     hipEvent_t);                                                       \
-  template void launch_ssor_true_rr<VT>(const SsorSysT<VT>*, int, int, hipStream_t);                   \
   template void launch_ssor_check<VT>(const SsorSysT<VT>*, hipStream_t);                               \
   template void launch_ssor_xsolve<VT>(const SsorSysT<VT>*, int, int, int, int, hipStream_t);          \
   template void launch_ssor_scatter<VT>(const SsorSysT<VT>*, int, int, hipStream_t);                   \
   template void launch_ssor_bwd_batch<VT>(const SsorSysT<VT>*, const SsorRef*, int, int, int, hipStream_t); \
   template void launch_ssor_fwd_batch<VT>(const SsorSysT<VT>*, const SsorRef*, int, int, int, hipStream_t); \
-  template void launch_ssor_true_rr_batch<VT>(const SsorSysT<VT>*, const SsorRef*, int, int, hipStream_t);  \
   template void launch_ssor_check_batch<VT>(const SsorSysT<VT>*, int, hipStream_t);                          \
   template void launch_ssor_gather_states<VT>(const SsorSysT<VT>*, int, PcgState*, hipStream_t);
 GSH_SSOR_INSTANTIATE(double)
