@@ -120,6 +120,48 @@ def test_ssor_standalone_csr_and_special_cases(ctx):
     sm.close()
 
 
+def _tiny_systems():
+    """(name, CSR matrix) of small and degenerate SPD systems: one row, a diagonal matrix (one
+    colour), a 2 x 2 system, a tridiagonal chain (two colours) and a 3 x 3 dense block (three)."""
+    rng = np.random.default_rng(11)
+    out = [("n1", sp.csr_matrix(np.array([[4.0]])))]
+    out.append(("diagonal-300", sp.diags(rng.uniform(1.0, 5.0, 300)).tocsr()))
+    out.append(("2x2", sp.csr_matrix(np.array([[4.0, -1.0], [-1.0, 3.0]]))))
+    n = 700
+    out.append(("chain-700", sp.diags([-np.ones(n - 1), 2.5 + rng.uniform(0, 1, n), -np.ones(n - 1)],
+                                     [-1, 0, 1]).tocsr()))
+    out.append(("dense-3", sp.csr_matrix(np.array([[5.0, 1.0, 1.0], [1.0, 4.0, 1.0], [1.0, 1.0, 6.0]]))))
+    return out
+
+
+@pytest.mark.parametrize("name", [t[0] for t in _tiny_systems()])
+def test_tiny_and_degenerate_systems(ctx, name):
+    """Tiny systems through both solvers against the oracle, bit for bit, with a cold start, a
+    repeat and a zero right-hand side."""
+    A = dict(_tiny_systems())[name]
+    A.sort_indices()
+    n = A.shape[0]
+    b = np.random.default_rng(3).normal(size=n)
+    for precond in (L.PRECOND_SSOR, L.PRECOND_JACOBI):
+        print(f"[gpu] tiny system {name} n={n} precond={precond}", flush=True)
+        s = core.Solver(None, ctx=ctx, indptr=A.indptr, indices=A.indices, precond=precond)
+        s.set_values(A.data)
+        for rep in range(2):
+            s.reset_history()
+            x, it, st = s.solve(b, rtol=1e-12, maxiter=500)
+            if precond == L.PRECOND_SSOR:
+                xo, ito, _, _, nc = O.pcg_ssor(A.indptr, A.indices, A.data, b, rtol=1e-12, maxiter=500,
+                                               rows_per_thread=1)
+            else:
+                xo, ito, _, _ = O.pcg(A.indptr, A.indices, A.data, b, rtol=1e-12, maxiter=500, rows_per_thread=4)
+            assert st == 0 and it == ito, (name, precond, it, ito, st)
+            assert_bitwise(x, xo, f"{name} precond={precond} solve {rep}")
+        assert np.linalg.norm(A @ x - b) <= 1e-10 * np.linalg.norm(b)
+        x, it, st = s.solve(np.zeros(n), rtol=1e-12, maxiter=100)
+        assert it == 0 and st == 0 and not x.any()
+        s.close()
+
+
 def test_autotune_keeps_results_exact(ctx):
     """The autotuner only picks the tile size; the solve afterwards still matches the oracle run
     with that tile size (Jacobi and SSOR)."""
