@@ -406,7 +406,13 @@ __device__ bool celestial_walk(const DMesh& m, const int* key, const StElem* rec
     const double d2 = vx * (C2x - x0x) + vy * (C2y - x0y);
     double maxp = fmax(d0, d1);
     maxp = fmax(maxp, d2);
-    fac = fmax(1.00001 * fac, fmin(1.0, (maxp / (vx * vx + vy * vy)) * 1.00001));
+    // fmin(1, (maxp / vv) 1.00001) is 1 whenever maxp > 2 vv (the rounded quotient is then at
+    // least 2; vv = 0 gives +inf as well), the usual case of a step shorter than the element:
+    // the division runs only otherwise (NaN and overflow included), with the same result
+    const double vv = vx * vx + vy * vy;
+    double lead = 1.0;
+    if (!(maxp > 2.0 * vv)) lead = fmin(1.0, (maxp / vv) * 1.00001);
+    fac = fmax(1.00001 * fac, lead);
     const double ix = x0x + (xtx - x0x) * fac, iy = x0y + (xty - x0y) * fac;
     const double s0x = C1x - C0x, s0y = C1y - C0y;
     const double s1x = C2x - C1x, s1y = C2y - C1y;
