@@ -495,11 +495,16 @@ struct WindowPacer {
     int w0 = -1, w1 = -1;                 // their window indices
     bool fin = false;
   };
-  std::vector<Sys> sys;
-  explicit WindowPacer(int nsys) : sys(nsys) {}
+  Sys one;               // the state of a single system (no heap allocation)
+  std::vector<Sys> many;  // several systems
+  Sys* sys;
+  int n;
+  explicit WindowPacer(int nsys) : many(nsys > 1 ? nsys : 0), sys(nsys > 1 ? many.data() : &one), n(nsys) {}
+  WindowPacer(const WindowPacer&) = delete;
+  WindowPacer& operator=(const WindowPacer&) = delete;
   // the states of window `win`, one per system
   void read(const PcgState* s, int win) {
-    for (size_t i = 0; i < sys.size(); ++i) {
+    for (int i = 0; i < n; ++i) {
       Sys& y = sys[i];
       if (s[i].done >= 0) {
         y.fin = true;
@@ -515,7 +520,8 @@ struct WindowPacer {
   // every running system is expected to pass its check by window `newest` (half a window of
   // margin: a miss costs a round trip, a late stop a window)
   bool expect_done(int newest) const {
-    for (const Sys& y : sys) {
+    for (int i = 0; i < n; ++i) {
+      const Sys& y = sys[i];
       if (y.fin) continue;
       if (y.w0 < 0) return false;
       const double rate = (y.l0 - y.l1) / (double)(y.w1 - y.w0);

@@ -194,11 +194,12 @@ __device__ __forceinline__ int col_of(const SysView<VT>& S, int kk, int base64) 
   return S.has_cdel ? base64 + (int)S.cdel[kk] : S.col[kk];
 }
 
+// s0, e0: the chunk's first entry and the entry after its last (rowptr at r0w and r0w + 64)
 template <class VT>
-__device__ __forceinline__ void load_chunk(const SysView<VT>& S, gptr<const VT> po, int r0w, int lane, Chunk<VT>& c) {
-  const int rend = min(r0w + 64, S.n);
-  c.s0 = S.rowptr[r0w];
-  c.e0 = S.rowptr[rend];
+__device__ __forceinline__ void load_chunk(const SysView<VT>& S, gptr<const VT> po, int r0w, int lane, int s0, int e0,
+                                           Chunk<VT>& c) {
+  c.s0 = s0;
+  c.e0 = e0;
   if (c.e0 - c.s0 > kWaveCap) return;
 #pragma unroll
   for (int m = 0; m < kWaveM; ++m) {
@@ -227,9 +228,26 @@ __global__ void __launch_bounds__(256) k_pcg_phase_a(const PcgSys<VT>* __restric
   const gptr<VT> pn = (k & 1) ? S.pb : S.pa;
   const gptr<const VT> po = (k & 1) ? S.pa : S.pb;
   const int base = tr.tile * kTpb * R;
+  // the row pointers of every sub-round first: they depend on nothing, so each sub-round's chunk
+  // then waits for two memory steps (entries, then gathers) instead of three
+  int cs0[R], ce0[R], ra[R], rb[R];
+#pragma unroll
+  for (int j = 0; j < R; ++j) {
+    const int r0w = base + j * kTpb + w * 64;
+    cs0[j] = ce0[j] = ra[j] = rb[j] = 0;
+    if (r0w < n) {
+      cs0[j] = S.rowptr[r0w];
+      ce0[j] = S.rowptr[min(r0w + 64, n)];
+      const int i = r0w + lane;
+      if (i < n) {
+        ra[j] = S.rowptr[i];
+        rb[j] = S.rowptr[i + 1];
+      }
+    }
+  }
   Chunk<VT> c;
   const int r00 = base + w * 64;
-  if (r00 < n) load_chunk(S, po, r00, lane, c);
+  if (r00 < n) load_chunk(S, po, r00, lane, cs0[0], ce0[0], c);
   const double rz_cur = final_reduce(S.part_rz, S.ntiles, red);
   double beta = 0.0;
   if (k > 0) {
@@ -269,10 +287,11 @@ __global__ void __launch_bounds__(256) k_pcg_phase_a(const PcgSys<VT>* __restric
   const gptr<const VT> z = S.z;
   VT* my = lds + w * kWaveCap;
   double pq = 0.0;
+#pragma unroll
   for (int j = 0; j < R; ++j) {
     const int r0w = base + j * kTpb + w * 64;
     if (r0w >= n) break;
-    if (j > 0) load_chunk(S, po, r0w, lane, c);
+    if (j > 0) load_chunk(S, po, r0w, lane, cs0[j], ce0[j], c);
     const int i = r0w + lane;
     VT q = (VT)0;
     if (c.e0 - c.s0 <= kWaveCap) {
@@ -283,12 +302,12 @@ __global__ void __launch_bounds__(256) k_pcg_phase_a(const PcgSys<VT>* __restric
       }
       wave_lds_sync();
       if (i < n) {
-        const int a = S.rowptr[i], b = S.rowptr[i + 1];
+        const int a = ra[j], b = rb[j];
         for (int kk = a; kk < b; ++kk) q += my[kk - c.s0];
       }
       wave_lds_sync();
     } else if (i < n) {
-      const int a = S.rowptr[i], b = S.rowptr[i + 1];
+      const int a = ra[j], b = rb[j];
       for (int kk = a; kk < b; ++kk) {
         const int col = col_of(S, kk, r0w);
         q += S.val[kk] * (z[col] + bv * po[col]);
