@@ -148,9 +148,13 @@ struct ElemGeom {
   double x0, y0, x1, y1, x2, y2;
 };
 
+// Ids are non-negative int32 wherever they index (every -1 is tested first), so the hot gathers
+// index with unsigned 32-bit arithmetic (2 id + 1 < 2^32): one address instruction per load.
+__device__ __forceinline__ unsigned ux(int i) { return (unsigned)i; }
+
 __device__ __forceinline__ ElemGeom elem_geom(const DMesh& m, int e) {
-  const int4 r = m.elem[2 * e], q = m.elem[2 * e + 1];
-  const double2 p0 = m.nrec[2 * r.x], p1 = m.nrec[2 * r.y], p2 = m.nrec[2 * r.z];
+  const int4 r = m.elem[2u * ux(e)], q = m.elem[2u * ux(e) + 1u];
+  const double2 p0 = m.nrec[2u * ux(r.x)], p1 = m.nrec[2u * ux(r.y)], p2 = m.nrec[2u * ux(r.z)];
   return {r.x, r.y, r.z, r.w, q.x, q.y, p0.x, p0.y, p1.x, p1.y, p2.x, p2.y};
 }
 
@@ -163,9 +167,9 @@ struct NodeZ {
 };
 
 __device__ __forceinline__ NodeZ node_z(const DMesh& m, double h0, int node) {
-  const double2 q = m.nrec[2 * node + 1];
+  const double2 q = m.nrec[2u * ux(node) + 1u];
   const double H = q.x + q.y;
-  return {q.x, q.y, H, m.kb[node], H <= h0, node};
+  return {q.x, q.y, H, m.kb[ux(node)], H <= h0, node};
 }
 
 // LDS-staged element patch (k_backtrack with ST = true): every element the walk and the
@@ -290,7 +294,7 @@ __device__ void interp_velocity(const DMesh& m, const double* sig, double h0, in
                                 double& wo, int& hint) {
   const int n0 = g.n0, n1 = g.n1, n2 = g.n2;
   const double x0 = g.x0, y0 = g.y0, x1 = g.x1, y1 = g.y1, x2 = g.x2, y2 = g.y2;
-  const double area = (ST && rc) ? rc->area : m.area[e];
+  const double area = (ST && rc) ? rc->area : m.area[ux(e)];
   double s0 = signed_area(px, py, x1, y1, x2, y2) / area;
   double s1 = signed_area(x0, y0, px, py, x2, y2) / area;
   double s2 = 1.0 - s0 - s1;
@@ -308,7 +312,7 @@ __device__ void interp_velocity(const DMesh& m, const double* sig, double h0, in
   const NodeZ q1 = st ? st_node_z(*rc, 1, h0) : node_z(m, h0, n1);
   const NodeZ q2 = st ? st_node_z(*rc, 2, h0) : node_z(m, h0, n2);
   const int kb0 = q0.kb, kb1 = q1.kb, kb2 = q2.kb;
-  const int L0 = (st ? rc->ekb : m.elem_kb[e]) - 1, R0 = nv - 1;
+  const int L0 = (st ? rc->ekb : m.elem_kb[ux(e)]) - 1, R0 = nv - 1;
   int L = L0, R = R0;
   double zL = 0.0, zR = 0.0, w1, w2;
   bool inside = false;
@@ -361,11 +365,13 @@ __device__ void interp_velocity(const DMesh& m, const double* sig, double h0, in
     }
   }
   if (inside) hint = L;
+  // the three nodes' level rows (64-bit row offsets), then two levels of each
   const double4* U = reinterpret_cast<const double4*>(m.uvw);
-  const double4 a0 = U[(int64_t)n0 * nv + max(L, kb0)], a1 = U[(int64_t)n1 * nv + max(L, kb1)],
-                a2 = U[(int64_t)n2 * nv + max(L, kb2)];
-  const double4 b0 = U[(int64_t)n0 * nv + max(R, kb0)], b1 = U[(int64_t)n1 * nv + max(R, kb1)],
-                b2 = U[(int64_t)n2 * nv + max(R, kb2)];
+  const double4* u0 = U + (uint64_t)ux(n0) * ux(nv);
+  const double4* u1 = U + (uint64_t)ux(n1) * ux(nv);
+  const double4* u2 = U + (uint64_t)ux(n2) * ux(nv);
+  const double4 a0 = u0[ux(max(L, kb0))], a1 = u1[ux(max(L, kb1))], a2 = u2[ux(max(L, kb2))];
+  const double4 b0 = u0[ux(max(R, kb0))], b1 = u1[ux(max(R, kb1))], b2 = u2[ux(max(R, kb2))];
   const double uL = s0 * a0.x + s1 * a1.x + s2 * a2.x + 0.0, vL = s0 * a0.y + s1 * a1.y + s2 * a2.y + 0.0;
   const double uR = s0 * b0.x + s1 * b1.x + s2 * b2.x + 0.0, vR = s0 * b0.y + s1 * b1.y + s2 * b2.y + 0.0;
   const double wL = s0 * a0.z + s1 * a1.z + s2 * a2.z + 0.0, wR = s0 * b0.z + s1 * b1.z + s2 * b2.z + 0.0;
@@ -420,6 +426,9 @@ __device__ bool celestial_walk(const DMesh& m, const int* key, const StElem* rec
     const bool left0 = (s0x * (iy - C0y) - s0y * (ix - C0x)) > -kSmall;
     const bool left1 = (s1x * (iy - C1y) - s1y * (ix - C1x)) > -kSmall;
     const bool left2 = (s2x * (iy - C2y) - s2y * (ix - C2x)) > -kSmall;
+    // a side is an exit candidate only if the point is not left of it: with all three left
+    // tests passed the walk ends here (the usual case), without the obtuse and bisector tests
+    if (left0 && left1 && left2) return true;
     const bool obt0 = (s0x * s1x + s0y * s1y) > 0.0;
     const bool obt1 = (s1x * s2x + s1y * s2y) > 0.0;
     const bool obt2 = (s2x * s0x + s2y * s0y) > 0.0;
@@ -436,9 +445,9 @@ __device__ bool celestial_walk(const DMesh& m, const int* key, const StElem* rec
     asm volatile("" : "+v"(gs0), "+v"(gs1), "+v"(gs2));
     const int side = v0 ? gs0 : (v1 ? gs1 : gs2);
     last_cross = side;
-    const int4 sr = m.side[side];
+    const int4 sr = m.side[ux(side)];
     const int nxt = (sr.z == cur) ? sr.w : sr.z;
-    if (nxt < 0 || !m.elem_wet[nxt]) return false;
+    if (nxt < 0 || !m.elem_wet[ux(nxt)]) return false;
     cur = nxt;
   }
   status |= GSH_BT_WALK_CAP;
