@@ -589,7 +589,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))
     k = list_lev[t];
   } else {
     if (t >= (int64_t)m.ns * m.nvrt) return;
-    s = (int)(t / m.nvrt);
+    s = (t < 0xffffffffll) ? (int)((uint32_t)t / (uint32_t)m.nvrt) : (int)(t / m.nvrt);
     k = (int)(t - (int64_t)s * m.nvrt);
   }
   const int64_t idx = (int64_t)s * m.nvrt + k;
@@ -1169,7 +1169,9 @@ __global__ void k_shapiro(DMesh m, double shapiro0, double* __restrict__ out) {
   const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int nv = m.nvrt;
   if (t >= (int64_t)m.ns * nv) return;
-  const int s = (int)(t / nv), k = (int)(t - (int64_t)s * nv);
+  // a 32-bit division below 2^32 (the 64-bit one is a long instruction sequence)
+  const int s = (t < 0xffffffffll) ? (int)((uint32_t)t / (uint32_t)nv) : (int)(t / nv);
+  const int k = (int)(t - (int64_t)s * nv);
   const double2* u2 = reinterpret_cast<const double2*>(m.su2);
   double2* o = reinterpret_cast<double2*>(out);
   const int4 sr = m.side[s];
