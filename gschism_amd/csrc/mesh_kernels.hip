@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include "device.h"
+#include "reduce.h"
 
 namespace gsh {
 
@@ -995,42 +996,45 @@ __device__ __forceinline__ void side_eta_grads(const DMesh& m, const int4 sr, do
 // tridag_sch<2> (meshdata.cu:3665-3711) with the coefficients formed on the fly: the forward
 // sweep keeps u_j in the btu slot k-1 and gam_j in the zc slot k-1 (both dead once row j is
 // formed), the back substitution runs over those slots, and the result moves up one slot.
-__device__ void momentum_side(const DMesh& m, const StepArgs& a, int s, double* zc, double* u) {
+// Component c (0: x, 1: y) of one side's momentum solve, by one of the side's two lanes (adjacent
+// lanes of one wave): both lanes compute the shared coefficients identically (the Thomas pivots,
+// written to the same zc slots with the same values, in lockstep) and each solves its own
+// component. The free-slip projection needs both components and runs in lane 0 after a wave
+// fence. Every value has the operands and order of the one-lane form.
+__device__ void momentum_side(const DMesh& m, const StepArgs& a, int s, int c, double* zc, double* u) {
   const int nv = m.nvrt;
   const int kb = m.side_kb[s];
   const int4 sr = m.side[s];
   const double rx = a.rmaxvel, ry = a.rmaxvel * 1.013;
+  const double rc = c ? ry : rx;
   if (!m.side_wet[s]) {
-    for (int k = 0; k < 2 * nv; ++k) u[k] = 0.0;
+    for (int k = 0; k < nv; ++k) u[2 * k + c] = 0.0;
   } else {
     double g1x, g1y, g2x, g2y;
     side_eta_grads(m, sr, a.h0, g1x, g1y, g2x, g2y);
+    const double g1 = c ? g1y : g1x, g2 = c ? g2y : g2x;
     const double grav3 = 0.5 * (m.grav2[sr.x] + m.grav2[sr.y]);
     const double Cd = 0.5 * (m.cd[sr.x] + m.cd[sr.y]);
     const double dt = a.dt, th = a.theta;
     if (kb == nv - 2) {
       const double htot = zc[nv - 1] - zc[kb];
       const double hhat = m.hhat[s];
-      const double hx = u[2 * (nv - 1)] + dt * (0.0 - grav3 * ((1.0 - th) * g1x + th * g2x));
-      const double hy = u[2 * (nv - 1) + 1] + dt * (0.0 - grav3 * ((1.0 - th) * g1y + th * g2y));
-      const double vx = fmin(fmax(hx * hhat / htot, -rx), rx), vy = fmin(fmax(hy * hhat / htot, -ry), ry);
-      for (int k = 0; k < nv; ++k) {
-        u[2 * k] = vx;
-        u[2 * k + 1] = vy;
-      }
+      const double hc = u[2 * (nv - 1) + c] + dt * (0.0 - grav3 * ((1.0 - th) * g1 + th * g2));
+      const double vc = fmin(fmax(hc * hhat / htot, -rc), rc);
+      for (int k = 0; k < nv; ++k) u[2 * k + c] = vc;
     } else {
-      const double Gx = grav3 * (th * g2x + (1.0 - th) * g1x), Gy = grav3 * (th * g2y + (1.0 - th) * g1y);
+      const double G = grav3 * (th * g2 + (1.0 - th) * g1);
       const double chi2 = m.chi[s], dfv0 = a.dfv0;
-      double zm = zc[kb], z0 = zc[kb + 1];                                          // zc[k-1], zc[k]
-      double bmx = u[2 * kb], bmy = u[2 * kb + 1], b0x = u[2 * kb + 2], b0y = u[2 * kb + 3];  // btu[k-1], btu[k]
-      double bet = 1.0, cprev = 0.0, upx = 0.0, upy = 0.0;
+      double zm = zc[kb], z0 = zc[kb + 1];         // zc[k-1], zc[k]
+      double bm = u[2 * kb + c], b0 = u[2 * kb + 2 + c];  // btu[k-1], btu[k] of this component
+      double bet = 1.0, cprev = 0.0, up = 0.0;
       // dz/6, dz/3 and dt dfv0 / dz of layer (k, k+1); the next row reuses them for its layer
       // (k-1, k), whose dz is the same difference of the same two levels
       double d6p = 0.0, d3p = 0.0, tp = 0.0;
       for (int k = kb + 1; k < nv; ++k) {
         const bool top = k < nv - 1;
         const double zp = top ? zc[k + 1] : 0.0;
-        const double bpx = top ? u[2 * (k + 1)] : 0.0, bpy = top ? u[2 * (k + 1) + 1] : 0.0;
+        const double bp = top ? u[2 * (k + 1) + c] : 0.0;
         const double dz_0 = z0 - zm;
         const double dz_1 = top ? zp - z0 : 0.0;
         double aj = 0.0, bj = 0.0, cj = 0.0, d6 = 0.0, d3 = 0.0, tq = 0.0;
@@ -1050,81 +1054,70 @@ __device__ void momentum_side(const DMesh& m, const StepArgs& a, int s, double* 
         double dz = 0.0;
         dz += dz_1;
         if (k > kb + 1) dz += dz_0;
-        double qx = -(0.5 * dz) * dt * Gx, qy = -(0.5 * dz) * dt * Gy;
-        if (top) {
-          qx += d6 * (2.0 * b0x + bpx);
-          qy += d6 * (2.0 * b0y + bpy);
-        }
-        if (k > kb + 1) {
-          qx += d6p * (2.0 * b0x + bmx);
-          qy += d6p * (2.0 * b0y + bmy);
-        }
+        double q = -(0.5 * dz) * dt * G;
+        if (top) q += d6 * (2.0 * b0 + bp);
+        if (k > kb + 1) q += d6p * (2.0 * b0 + bm);
         d6p = d6;
         d3p = d3;
         tp = tq;
-        double ux, uy;
+        double uk;
         if (k == kb + 1) {
           bet = bj;
-          ux = qx / bet;
-          uy = qy / bet;
+          uk = q / bet;
         } else {
           const double gam = cprev / bet;
           bet = bj - aj * gam;
-          ux = (qx - aj * upx) / bet;
-          uy = (qy - aj * upy) / bet;
+          uk = (q - aj * up) / bet;
           zc[k - 1] = gam;
         }
-        u[2 * (k - 1)] = ux;
-        u[2 * (k - 1) + 1] = uy;
-        upx = ux;
-        upy = uy;
+        u[2 * (k - 1) + c] = uk;
+        up = uk;
         cprev = cj;
         zm = z0;
         z0 = zp;
-        bmx = b0x;
-        bmy = b0y;
-        b0x = bpx;
-        b0y = bpy;
+        bm = b0;
+        b0 = bp;
       }
       // back substitution: u_j (slot kb + j) -= gam_{j+1} (zc slot kb + j + 1) u_{j+1}
       for (int k = nv - 3; k >= kb; --k) {
         const double gam = zc[k + 1];
-        upx = u[2 * k] - gam * upx;
-        upy = u[2 * k + 1] - gam * upy;
-        u[2 * k] = upx;
-        u[2 * k + 1] = upy;
+        up = u[2 * k + c] - gam * up;
+        u[2 * k + c] = up;
       }
-      for (int k = nv - 1; k > kb; --k) {
-        u[2 * k] = fmin(fmax(u[2 * (k - 1)], -rx), rx);
-        u[2 * k + 1] = fmin(fmax(u[2 * (k - 1) + 1], -ry), ry);
-      }
+      for (int k = nv - 1; k > kb; --k) u[2 * k + c] = fmin(fmax(u[2 * (k - 1) + c], -rc), rc);
     }
     // bottom condition, imm = 0
     if (Cd == 0.0) {
-      u[2 * kb] = u[2 * (kb + 1)];
-      u[2 * kb + 1] = u[2 * (kb + 1) + 1];
+      u[2 * kb + c] = u[2 * (kb + 1) + c];
     } else if (kb < nv - 2) {
-      u[2 * kb] = 0.0;
-      u[2 * kb + 1] = 0.0;
+      u[2 * kb + c] = 0.0;
     }
-    for (int k = 0; k < kb; ++k) u[2 * k] = u[2 * k + 1] = 0.0;
-  }
-  if (m.side_type[s] == 1) {
-    // free slip: the tangential component, t = (-n.y, n.x), n from populateSides (meshdata.cu:2283-2287)
-    const double vx = -(m.y[sr.x] - m.y[sr.y]), vy = m.x[sr.x] - m.x[sr.y];
-    const double len = sqrt(vx * vx + vy * vy);
-    const double nx = vx / len, ny = vy / len;
-    const double tx = -ny, ty = nx;
-    for (int k = kb; k < nv; ++k) {
-      const double vtan = u[2 * k] * tx + u[2 * k + 1] * ty;
-      u[2 * k] = vtan * tx;
-      u[2 * k + 1] = vtan * ty;
-    }
+    for (int k = 0; k < kb; ++k) u[2 * k + c] = 0.0;
   }
 }
 
-// A workgroup of 64 takes g consecutive sides: their zc and btu rows come into LDS with flat
-// coalesced loads, one thread per side solves in LDS, the u2 rows leave with flat stores.
+// The free-slip projection of a land side (side_type 1), both components, by the side's lane 0
+// after both lanes' momentum_side.
+__device__ void momentum_free_slip(const DMesh& m, int s, double* u) {
+  if (m.side_type[s] != 1) return;
+  const int nv = m.nvrt;
+  const int kb = m.side_kb[s];
+  const int4 sr = m.side[s];
+  // free slip: the tangential component, t = (-n.y, n.x), n from populateSides (meshdata.cu:2283-2287)
+  const double vx = -(m.y[sr.x] - m.y[sr.y]), vy = m.x[sr.x] - m.x[sr.y];
+  const double len = sqrt(vx * vx + vy * vy);
+  const double nx = vx / len, ny = vy / len;
+  const double tx = -ny, ty = nx;
+  for (int k = kb; k < nv; ++k) {
+    const double vtan = u[2 * k] * tx + u[2 * k + 1] * ty;
+    u[2 * k] = vtan * tx;
+    u[2 * k + 1] = vtan * ty;
+  }
+}
+
+// A workgroup of 64 takes g <= 32 consecutive sides: their zc and btu rows come into LDS with flat
+// coalesced loads, two lanes per side solve in LDS (one component each), the u2 rows leave with
+// flat stores.
 __global__ void __launch_bounds__(64) k_momentum(DMesh m, StepArgs a, int g) {
   extern __shared__ double lds[];
   const int nv = m.nvrt;
@@ -1136,8 +1129,10 @@ __global__ void __launch_bounds__(64) k_momentum(DMesh m, StepArgs a, int g) {
   stage_rows(Lz, zs, m.zc + (int64_t)s0 * nv, nv, gc);
   stage_rows(Lu, us, m.btu + 2 * (int64_t)s0 * nv, 2 * nv, gc);
   __syncthreads();
-  const int t = threadIdx.x;
-  if (t < gc) momentum_side(m, a, s0 + t, Lz + t * zs, Lu + t * us);
+  const int t = threadIdx.x, sl = t >> 1, c = t & 1;  // two lanes per side
+  if (sl < gc) momentum_side(m, a, s0 + sl, c, Lz + sl * zs, Lu + sl * us);
+  wave_lds_sync();
+  if (sl < gc && c == 0) momentum_free_slip(m, s0 + sl, Lu + sl * us);
   __syncthreads();
   unstage_rows(m.su2 + 2 * (int64_t)s0 * nv, Lu, us, 2 * nv, gc);
 }
@@ -1199,10 +1194,10 @@ void launch_shapiro(const DMesh& m, double shapiro0, double* out, hipStream_t st
 
 void launch_momentum(const DMesh& m, const StepArgs& a, hipStream_t st) {
   if (m.ns <= 0) return;
-  // sides per workgroup: one per thread, as many as 24 KB of LDS holds (measured best against
+  // sides per workgroup: one per two lanes, as many as 24 KB of LDS holds (measured best against
   // 16-64 KB and 32-256 threads at 3.1M sides x 31 levels)
   const int row = 8 * ((m.nvrt | 1) + ((2 * m.nvrt) | 1));
-  const int g = max(1, min(64, (24 * 1024) / row));
+  const int g = max(1, min(32, (24 * 1024) / row));
   hipLaunchKernelGGL(k_momentum, dim3((unsigned)((m.ns + g - 1) / g)), dim3(64), (size_t)g * row, st, m, a, g);
 }
 
@@ -1234,14 +1229,15 @@ __global__ void __launch_bounds__(256) k_vertical_velocity(DMesh m, double h0, i
   VvElem* Lel = reinterpret_cast<VvElem*>(Lbw + E);  // [E]
   const int e0 = blockIdx.x * E;
   const int ec = min(E, m.ne - e0);
-  if (threadIdx.x < ec) {
-    const int e = e0 + threadIdx.x;
-    VvElem& q = Lel[threadIdx.x];
+  // three threads per element, thread j taking corner j and side j
+  for (int t = threadIdx.x; t < 3 * ec; t += blockDim.x) {
+    const int el = t / 3, j = t - 3 * el;
+    const int e = e0 + el;
+    VvElem& q = Lel[el];
     const int4 r0 = m.elem[2 * e], r1 = m.elem[2 * e + 1];
     const int nd[3] = {r0.x, r0.y, r0.z};
     const int sd[3] = {r0.w, r1.x, r1.y};
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
+    {
       const int i = nd[j];
       q.px[j] = m.x[i];
       q.py[j] = m.y[i];
@@ -1322,11 +1318,21 @@ __global__ void __launch_bounds__(256) k_vertical_velocity(DMesh m, double h0, i
       const double ubx = (bx[0] + bx[1] + bx[2] + 0.0) / 3, uby = (by[0] + by[1] + by[2] + 0.0) / 3;
       const double bottomW = -(ghx * ubx + ghy * uby);
       Lbw[el] = 0.0 + bottomW;
+      // the running sums in level order; the side fluxes of 8 levels are read ahead so that
+      // the dependent additions do not wait for LDS one at a time
       double bottom = 0.0;
-      for (int l = kb; l < nv - 1; ++l) {
-        const double top = -Lsf[el * nvp + l] - bottom;
-        Lsf[el * nvp + l] = top;
-        bottom = -top;
+      double* row = Lsf + el * nvp;
+      for (int l0 = kb; l0 < nvp; l0 += 8) {
+        double f[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) f[u] = (l0 + u < nvp) ? row[l0 + u] : 0.0;
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+          if (l0 + u < nvp) {
+            const double top = -f[u] - bottom;
+            row[l0 + u] = top;
+            bottom = -top;
+          }
       }
     }
   }

@@ -228,14 +228,17 @@ __global__ void __launch_bounds__(256) k_pcg_phase_a(const PcgSys<VT>* __restric
   const gptr<VT> pn = (k & 1) ? S.pb : S.pa;
   const gptr<const VT> po = (k & 1) ? S.pa : S.pb;
   const int base = tr.tile * kTpb * R;
-  // the row pointers of every sub-round first: they depend on nothing, so each sub-round's chunk
-  // then waits for two memory steps (entries, then gathers) instead of three
+  // FP64: the row pointers of every sub-round first: they depend on nothing, so each sub-round's
+  // chunk then waits for two memory steps (entries, then gathers) instead of three. FP32 loads them
+  // per sub-round: its registers would otherwise drop it from 7 to 5 waves per SIMD, which its
+  // batched launches of small systems need more.
+  constexpr bool kPre = sizeof(VT) == 8;
   int cs0[R], ce0[R], ra[R], rb[R];
 #pragma unroll
   for (int j = 0; j < R; ++j) {
     const int r0w = base + j * kTpb + w * 64;
     cs0[j] = ce0[j] = ra[j] = rb[j] = 0;
-    if (r0w < n) {
+    if (kPre && r0w < n) {
       cs0[j] = S.rowptr[r0w];
       ce0[j] = S.rowptr[min(r0w + 64, n)];
       const int i = r0w + lane;
@@ -245,9 +248,24 @@ __global__ void __launch_bounds__(256) k_pcg_phase_a(const PcgSys<VT>* __restric
       }
     }
   }
+  // a sub-round's first and end entries and the thread's row range (FP32: loaded here)
+  auto rows_of = [&](int j, int r0w) {
+    if (!kPre) {
+      cs0[j] = S.rowptr[r0w];
+      ce0[j] = S.rowptr[min(r0w + 64, n)];
+      const int i = r0w + lane;
+      if (i < n) {
+        ra[j] = S.rowptr[i];
+        rb[j] = S.rowptr[i + 1];
+      }
+    }
+  };
   Chunk<VT> c;
   const int r00 = base + w * 64;
-  if (r00 < n) load_chunk(S, po, r00, lane, cs0[0], ce0[0], c);
+  if (r00 < n) {
+    rows_of(0, r00);
+    load_chunk(S, po, r00, lane, cs0[0], ce0[0], c);
+  }
   const double rz_cur = final_reduce(S.part_rz, S.ntiles, red);
   double beta = 0.0;
   if (k > 0) {
@@ -291,7 +309,10 @@ __global__ void __launch_bounds__(256) k_pcg_phase_a(const PcgSys<VT>* __restric
   for (int j = 0; j < R; ++j) {
     const int r0w = base + j * kTpb + w * 64;
     if (r0w >= n) break;
-    if (j > 0) load_chunk(S, po, r0w, lane, cs0[j], ce0[j], c);
+    if (j > 0) {
+      rows_of(j, r0w);
+      load_chunk(S, po, r0w, lane, cs0[j], ce0[j], c);
+    }
     const int i = r0w + lane;
     VT q = (VT)0;
     if (c.e0 - c.s0 <= kWaveCap) {
