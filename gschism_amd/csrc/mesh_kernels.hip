@@ -665,11 +665,41 @@ void launch_backtrack(const DMesh& m, const StepArgs& a, const int32_t* ls, cons
 // wave reads consecutive levels of the same side and element rows.
 __device__ __forceinline__ double clamp_vel(double x, double hi) { return fmin(fmax(x, -hi), hi); }
 
+// The workgroup's threads cover the levels of a few consecutive nodes, whose balls are one
+// contiguous range of the ball arrays: its entries (element, area, wet flag and the three side
+// ids) are staged in LDS once, up to kNvCap of them (the rest read from global memory, the same
+// values), so each (node, level) thread only gathers its level's w and u2.
+constexpr int kNvCap = 512;
+
 __global__ void __launch_bounds__(256) k_nodal_velocity(DMesh m, double rx, double ry) {
-  const uint64_t t = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ double Larea[kNvCap];
+  __shared__ int Le[kNvCap], Ls1[kNvCap], Ls2[kNvCap], Ls3[kNvCap];
+  __shared__ int8_t Lwet[kNvCap];
+  const uint64_t t0 = (uint64_t)blockIdx.x * blockDim.x;
   const uint32_t nv = (uint32_t)m.nvrt;
-  if (t >= (uint64_t)m.n * nv) return;
-  const int i = (t < 0xffffffffull) ? (int)((uint32_t)t / nv) : (int)(t / nv);
+  const uint64_t total = (uint64_t)m.n * nv;
+  auto node_of = [&](uint64_t tt) { return (tt < 0xffffffffull) ? (int)((uint32_t)tt / nv) : (int)(tt / nv); };
+  // sides (r0.w, r1.x, r1.y) are opposite corners 0, 1, 2; s1 is the side opposite the node
+  auto sides_of = [&](int e, int l, int& s1, int& s2, int& s3) {
+    const int4 r0 = m.elem[2 * e], r1 = m.elem[2 * e + 1];
+    s1 = (l == 0) ? r0.w : ((l == 1) ? r1.x : r1.y);
+    s2 = (l == 0) ? r1.x : ((l == 1) ? r1.y : r0.w);
+    s3 = (l == 0) ? r1.y : ((l == 1) ? r0.w : r1.x);
+  };
+  const int ifirst = node_of(t0), ilast = node_of(min(t0 + blockDim.x, total) - 1);
+  const int jb = m.ball_ptr[ifirst];
+  const int nst = min(m.ball_ptr[ilast + 1] - jb, kNvCap);
+  for (int q = threadIdx.x; q < nst; q += blockDim.x) {
+    const int e = m.ball_elem[jb + q];
+    Le[q] = e;
+    Larea[q] = m.area[e];
+    Lwet[q] = m.elem_wet[e];
+    sides_of(e, m.ball_loc[jb + q], Ls1[q], Ls2[q], Ls3[q]);
+  }
+  __syncthreads();
+  const uint64_t t = t0 + threadIdx.x;
+  if (t >= total) return;
+  const int i = node_of(t);
   const int k = (int)(t - (uint64_t)i * nv);
   const int j0 = m.ball_ptr[i], j1 = m.ball_ptr[i + 1];
   const bool node_wet = m.node_wet[i] != 0;
@@ -679,19 +709,23 @@ __global__ void __launch_bounds__(256) k_nodal_velocity(DMesh m, double rx, doub
   double asum = 0.0, ws = 0.0, sx = 0.0, sy = 0.0;
   int count = 0;
   for (int j = j0; j < j1; ++j) {
-    const int e = m.ball_elem[j];
-    const double a = m.area[e];
+    const int q = j - jb;
+    const bool staged = q < nst;
+    const int e = staged ? Le[q] : m.ball_elem[j];
+    const double a = staged ? Larea[q] : m.area[e];
     asum += a;
     if (above) ws += m.ew[(int64_t)e * nv + k] * a;
-    if (!m.elem_wet[e] || !node_wet) continue;
+    if (!(staged ? Lwet[q] : m.elem_wet[e]) || !node_wet) continue;
     ++count;
     if (!above) continue;
-    const int l = m.ball_loc[j];
-    const int4 r0 = m.elem[2 * e], r1 = m.elem[2 * e + 1];
-    // sides (r0.w, r1.x, r1.y) are opposite corners 0, 1, 2
-    const int s1 = (l == 0) ? r0.w : ((l == 1) ? r1.x : r1.y);
-    const int s2 = (l == 0) ? r1.x : ((l == 1) ? r1.y : r0.w);
-    const int s3 = (l == 0) ? r1.y : ((l == 1) ? r0.w : r1.x);
+    int s1, s2, s3;
+    if (staged) {
+      s1 = Ls1[q];
+      s2 = Ls2[q];
+      s3 = Ls3[q];
+    } else {
+      sides_of(e, m.ball_loc[j], s1, s2, s3);
+    }
     const double2 u1 = su[(int64_t)s1 * nv + k];
     const double2 u2 = su[(int64_t)s2 * nv + k];
     const double2 u3 = su[(int64_t)s3 * nv + k];
