@@ -1178,34 +1178,45 @@ __global__ void __launch_bounds__(64) k_momentum(DMesh m, StepArgs a, int g) {
 // elements 0 and 1, summed in that order; kin_j = max(k, kb_j + 1) for a wet neighbour); a 2D
 // side's bottom slot takes bcc[kb+1] (recomputed by its own thread, the same bits); levels below
 // kb - 1 become 0; every other value, and boundary and dry sides, pass through unchanged.
-__device__ __forceinline__ double2 shapiro_value(const DMesh& m, const int (&nb)[4], const double2* u2, int64_t row,
-                                                 int k, double shapiro0) {
-  const int nv = m.nvrt;
-  double sx = 0.0, sy = 0.0;
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int n = nb[j];
-    const int kin = m.side_wet[n] ? max(k, m.side_kb[n] + 1) : k;
-    const double2 v = u2[(int64_t)n * nv + kin];
-    sx += v.x;
-    sy += v.y;
-  }
-  const double2 u = u2[row + k];
-  return make_double2(u.x + shapiro0 * (0.25 * sx - u.x), u.y + shapiro0 * (0.25 * sy - u.y));
-}
+// The workgroup's threads cover the levels of a few consecutive sides; each side's record (its
+// filter flag and kb, its four neighbours and the lowest level each neighbour contributes,
+// kb_j + 1 for a wet neighbour) is read once into LDS, so each (side, level) thread only gathers
+// its five u2 values.
+constexpr int kShSides = 256;  // at most 256 sides per workgroup (one level each)
 
-__global__ void k_shapiro(DMesh m, double shapiro0, double* __restrict__ out) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void __launch_bounds__(256) k_shapiro(DMesh m, double shapiro0, double* __restrict__ out) {
+  __shared__ int Lkb[kShSides], Lnb[kShSides][4], Lkin[kShSides][4];
+  __shared__ int8_t Lf[kShSides];
+  const int64_t t0 = (int64_t)blockIdx.x * blockDim.x;
   const int nv = m.nvrt;
-  if (t >= (int64_t)m.ns * nv) return;
+  const int64_t total = (int64_t)m.ns * nv;
   // a 32-bit division below 2^32 (the 64-bit one is a long instruction sequence)
-  const int s = (t < 0xffffffffll) ? (int)((uint32_t)t / (uint32_t)nv) : (int)(t / nv);
-  const int k = (int)(t - (int64_t)s * nv);
+  auto side_of = [&](int64_t tt) { return (tt < 0xffffffffll) ? (int)((uint32_t)tt / (uint32_t)nv) : (int)(tt / nv); };
+  const int sfirst = side_of(t0), nsd = side_of(min(t0 + (int64_t)blockDim.x, total) - 1) - sfirst + 1;
+  for (int q = threadIdx.x; q < nsd; q += blockDim.x) {
+    const int sd = sfirst + q;
+    const int4 sr = m.side[sd];
+    const bool filtered = m.side_wet[sd] && sr.w >= 0;
+    Lf[q] = filtered;
+    Lkb[q] = m.side_kb[sd];
+    if (filtered) {
+      const int4 nb4 = reinterpret_cast<const int4*>(m.side_nbr)[sd];
+      const int nb[4] = {nb4.x, nb4.y, nb4.z, nb4.w};
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        Lnb[q][j] = nb[j];
+        Lkin[q][j] = m.side_wet[nb[j]] ? m.side_kb[nb[j]] + 1 : -1;
+      }
+    }
+  }
+  __syncthreads();
+  const int64_t t = t0 + threadIdx.x;
+  if (t >= total) return;
+  const int s = side_of(t), k = (int)(t - (int64_t)s * nv), q = s - sfirst;
   const double2* u2 = reinterpret_cast<const double2*>(m.su2);
   double2* o = reinterpret_cast<double2*>(out);
-  const int4 sr = m.side[s];
-  const int kb = m.side_kb[s];
-  const bool filtered = m.side_wet[s] && sr.w >= 0;
+  const int kb = Lkb[q];
+  const bool filtered = Lf[q] != 0;
   const int kf = (kb == nv - 2) ? kb + 1 : k;  // the level whose filtered value this slot takes
   if (!filtered || (k >= kb - 1 && (k < kb || (kb < nv - 2 && k == kb)))) {
     o[t] = u2[t];
@@ -1215,9 +1226,18 @@ __global__ void k_shapiro(DMesh m, double shapiro0, double* __restrict__ out) {
     o[t] = make_double2(0.0, 0.0);
     return;
   }
-  const int4 q = reinterpret_cast<const int4*>(m.side_nbr)[s];
-  const int nb[4] = {q.x, q.y, q.z, q.w};
-  o[t] = shapiro_value(m, nb, u2, (int64_t)s * nv, kf, shapiro0);
+  // u2 + shapiro0 (sum of the four neighbours' u2 / 4 - u2), neighbour j at level max(kf, kb_j + 1)
+  // when wet, in Side::neigh order
+  double sx = 0.0, sy = 0.0;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int kin = max(kf, Lkin[q][j]);
+    const double2 v = u2[(int64_t)Lnb[q][j] * nv + kin];
+    sx += v.x;
+    sy += v.y;
+  }
+  const double2 u = u2[(int64_t)s * nv + kf];
+  o[t] = make_double2(u.x + shapiro0 * (0.25 * sx - u.x), u.y + shapiro0 * (0.25 * sy - u.y));
 }
 
 void launch_shapiro(const DMesh& m, double shapiro0, double* out, hipStream_t st) {
