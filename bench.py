@@ -450,6 +450,7 @@ def served_leg(ctx, L, cases, core, n_req, rates, rtol, mixed_rtol, check_every)
                     st, x, it, lat = srv.poll(tk, pool[which[j]].n)
                     if st != 0:
                         raise RuntimeError(f"request {j} failed with status {st}")
+                    srv.release(tk)
                     done[j] = (x, it, lat, time.perf_counter() - t0)
                 if a == 0 and q == 0 and nxt < n_req:
                     time.sleep(max(0.0, min(0.01, arrive[nxt] - (time.perf_counter() - t0))))
@@ -506,6 +507,7 @@ def served_leg(ctx, L, cases, core, n_req, rates, rtol, mixed_rtol, check_every)
                     st_, _, _, _ = srv.poll(tk)
                     if st_ != 0:
                         raise RuntimeError(f"solve-only request failed with status {st_}")
+                    srv.release(tk)
                     left.discard(tk)
             dt = time.perf_counter() - t
         srv.close()

@@ -160,6 +160,7 @@ SIGNATURES = {
     "gsh_server_step": (_I, [_P, _PI32, _PI32]),
     "gsh_server_finished": (_I, [_P, C.POINTER(C.c_int64), _I32, _PI32]),
     "gsh_server_poll": (_I, [_P, _I64, _PD, _PI32, _PI32, _PD]),
+    "gsh_server_release": (_I, [_P, _I64]),
     "gsh_copy_probe": (_I, [_P, _I64, _I, _PD]),
     "gsh_solver_profile": (_I, [_P, _D, _I, _I, _PD, C.POINTER(_I)]),
     "gsh_solver_set_omega": (_I, [_P, _D]),

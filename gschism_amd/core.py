@@ -519,3 +519,7 @@ class Server:
         check(L.lib().gsh_server_poll(self.h, ticket, dptr(x) if x is not None else None, C.byref(it), C.byref(st),
                                       C.byref(lat)))
         return st.value, (x if st.value == 0 else None), it.value, lat.value
+
+    def release(self, ticket: int) -> None:
+        """Forgets a finished or rejected request, returning its host buffers to the server's pool."""
+        check(L.lib().gsh_server_release(self.h, ticket))

@@ -2275,13 +2275,15 @@ int gsh_batch_solve(gsh_solver* const* solvers, int count, double rtol, int maxi
 namespace {
 struct Request {
   int64_t ticket = 0;
-  int32_t mesh_id = 0;
-  std::vector<double> b;
+  int32_t mesh_id = 0, n = 0;
+  // page-locked host vectors of the server's per-mesh pools: the right-hand side from submit
+  // until the request retires, the solution from retirement until gsh_server_release
+  double* hb = nullptr;
+  double* hx = nullptr;
   bool has_b = false, with_bt = false;
   Clock::time_point t_submit;
   int status = 1;  // 1 queued/active, 0 done, GSH_E_* failed
   bool active = false;
-  std::vector<double> x;
   int32_t iters = 0;
   int32_t run = 0;  // iterations issued for this request
   double latency_ms = 0.0;
@@ -2307,6 +2309,11 @@ struct gsh_server {
   std::vector<gsh_mesh*> meshes;
   std::vector<gsh_step_params> params;           // step parameters of each mesh's admission set-up
   std::vector<std::vector<gsh_solver*>> pool;  // idle solvers per mesh
+  // idle page-locked host vectors of each mesh's length: copies from and to them run
+  // asynchronously at full link speed, where pageable copies block the host and fault in fresh
+  // pages for every request
+  std::vector<std::vector<double*>> hfree;
+  std::vector<Request*> retiring;  // solves ended in this step, their x copies in flight
   std::deque<Request*> queue;
   std::vector<Request*> active;
   std::map<int64_t, std::unique_ptr<Request>> reqs;
@@ -2337,8 +2344,13 @@ struct gsh_server {
   ~gsh_server() {
     for (auto& v : pool)
       for (auto* s : v) gsh_solver_destroy(s);
-    for (auto& kv : reqs)
+    for (auto& kv : reqs) {
       if (kv.second->solver) gsh_solver_destroy(kv.second->solver);
+      if (kv.second->hb) (void)hipHostFree(kv.second->hb);
+      if (kv.second->hx) (void)hipHostFree(kv.second->hx);
+    }
+    for (auto& v : hfree)
+      for (double* p : v) (void)hipHostFree(p);
     (void)hipFree(dsys);
     (void)hipFree(dtiles);
     (void)hipFree(dfresh);
@@ -2379,20 +2391,55 @@ static int64_t solver_bytes(const gsh_mesh* m, int precision, int precond) {
 }
 
 // Retires a request: its solution to the host, its solver back to the mesh's pool.
+// A page-locked host vector of mesh mesh_id's length, from the server's pool.
+static double* host_vec_get(gsh_server* srv, int32_t mesh_id) {
+  auto& f = srv->hfree[mesh_id];
+  if (!f.empty()) {
+    double* p = f.back();
+    f.pop_back();
+    return p;
+  }
+  double* p = nullptr;
+  const size_t nb = sizeof(double) * (size_t)srv->meshes[mesh_id]->hm.n;
+  const hipError_t e = hipHostMalloc((void**)&p, nb, 0);
+  if (e != hipSuccess) throw Error(GSH_E_OOM, "page-locked host vector");
+  // the host mapping of the pages faults in at the first host access: take those faults now,
+  // not at the copy of a request that reuses a vector the device filled
+  std::memset(p, 0, nb);
+  return p;
+}
+
+static void host_vec_put(gsh_server* srv, int32_t mesh_id, double*& p) {
+  if (p) srv->hfree[mesh_id].push_back(p);
+  p = nullptr;
+}
+
+// The request's solve has ended: its x goes to a page-locked host vector on the solver stream.
+// server_finish_retired completes the retirement after one wait for all of a step's copies.
 static void server_retire(gsh_server* srv, Request* r, int32_t iters) {
   gsh_solver* sol = r->solver;
-  r->x.resize(sol->n);
-  hipStream_t st = srv->ctx->stream;
-  HIPCHK(hipMemcpyAsync(r->x.data(), sol->x, sizeof(double) * sol->n, hipMemcpyDeviceToHost, st));
-  HIPCHK(hipStreamSynchronize(st));
+  r->hx = host_vec_get(srv, r->mesh_id);
+  HIPCHK(hipMemcpyAsync(r->hx, sol->x, sizeof(double) * sol->n, hipMemcpyDeviceToHost, srv->ctx->stream));
   r->iters = iters;
-  r->status = 0;
   r->active = false;
-  r->latency_ms = std::chrono::duration<double, std::milli>(Clock::now() - r->t_submit).count();
-  srv->finished.push_back(r->ticket);
-  srv->pool[r->mesh_id].push_back(sol);
-  r->solver = nullptr;
-  srv->used -= r->bytes;
+  srv->retiring.push_back(r);
+}
+
+static void server_finish_retired(gsh_server* srv) {
+  if (srv->retiring.empty()) return;
+  HIPCHK(hipStreamSynchronize(srv->ctx->stream));
+  const auto now = Clock::now();
+  for (Request* r : srv->retiring) {
+    r->status = 0;
+    r->latency_ms = std::chrono::duration<double, std::milli>(now - r->t_submit).count();
+    srv->finished.push_back(r->ticket);
+    // the admission copy of b ran before the set-up the solve waited for
+    host_vec_put(srv, r->mesh_id, r->hb);
+    srv->pool[r->mesh_id].push_back(r->solver);
+    r->solver = nullptr;
+    srv->used -= r->bytes;
+  }
+  srv->retiring.clear();
 }
 
 // One step of an FP64 SSOR server: one window of check_every iterations for every active
@@ -2660,6 +2707,7 @@ int gsh_server_add_mesh(gsh_server* srv, gsh_mesh* mesh, int32_t* id) {
     srv->meshes.push_back(mesh);
     srv->params.push_back(p);
     srv->pool.emplace_back();
+    srv->hfree.emplace_back();
     *id = (int32_t)srv->meshes.size() - 1;
   })
 }
@@ -2680,10 +2728,12 @@ int gsh_server_submit(gsh_server* srv, int32_t mesh_id, const double* b, int wit
     auto r = std::make_unique<Request>();
     r->ticket = srv->next_ticket++;
     r->mesh_id = mesh_id;
+    r->n = srv->meshes[mesh_id]->hm.n;
     r->with_bt = with_bt != 0;
     if (b) {
       r->has_b = true;
-      r->b.assign(b, b + srv->meshes[mesh_id]->hm.n);
+      r->hb = host_vec_get(srv, mesh_id);
+      std::memcpy(r->hb, b, sizeof(double) * (size_t)r->n);
     }
     r->t_submit = Clock::now();
     r->bytes = solver_bytes(srv->meshes[mesh_id], srv->cfg.precision, srv->cfg.precond);
@@ -2727,6 +2777,7 @@ int gsh_server_step(gsh_server* srv, int32_t* active_out, int32_t* queued_out) {
       if (srv->used + r->bytes > srv->budget) {
         if (in_flight == 0) {  // can never fit
           r->status = GSH_E_ADMISSION;
+          host_vec_put(srv, r->mesh_id, r->hb);
           srv->queue.pop_front();
           continue;
         }
@@ -2755,7 +2806,7 @@ int gsh_server_step(gsh_server* srv, int32_t* active_out, int32_t* queued_out) {
       }
       // the request's matrix (and right-hand side unless given) from the mesh state
       launch_assemble(m->d, a, s->val, s->b, sg);
-      if (r->has_b) HIPCHK(hipMemcpyAsync(s->b, r->b.data(), sizeof(double) * s->n, hipMemcpyHostToDevice, sg));
+      if (r->has_b) HIPCHK(hipMemcpyAsync(s->b, r->hb, sizeof(double) * s->n, hipMemcpyHostToDevice, sg));
       if (cfg.precision == GSH_PREC_MIXED) {
         launch_cast_values(s->val, s->val32, s->nnz, sg);
         launch_fill_f64(s->x, 0.0, s->n, sg);
@@ -2838,6 +2889,7 @@ int gsh_server_step(gsh_server* srv, int32_t* active_out, int32_t* queued_out) {
       }
       srv->active.swap(keep);
     }
+    server_finish_retired(srv);
     if (active_out) *active_out = (int32_t)(srv->active.size() + srv->staged.size());
     if (queued_out) *queued_out = (int32_t)srv->queue.size();
   })
@@ -2855,6 +2907,19 @@ int gsh_server_finished(gsh_server* srv, int64_t* tickets, int32_t cap, int32_t*
   })
 }
 
+int gsh_server_release(gsh_server* srv, int64_t ticket) {
+  GSH_API_BODY({
+    GSH_REQUIRE(srv, GSH_E_ARG, "server is NULL");
+    auto it = srv->reqs.find(ticket);
+    GSH_REQUIRE(it != srv->reqs.end(), GSH_E_NOTFOUND, "unknown ticket");
+    Request* r = it->second.get();
+    GSH_REQUIRE(r->status != 1, GSH_E_STATE, "request is still queued or running");
+    host_vec_put(srv, r->mesh_id, r->hx);
+    host_vec_put(srv, r->mesh_id, r->hb);
+    srv->reqs.erase(it);
+  })
+}
+
 int gsh_server_poll(gsh_server* srv, int64_t ticket, double* x_host, int32_t* iters_out, int32_t* status_out,
                     double* latency_ms_out) {
   GSH_API_BODY({
@@ -2864,7 +2929,7 @@ int gsh_server_poll(gsh_server* srv, int64_t ticket, double* x_host, int32_t* it
     Request* r = it->second.get();
     if (status_out) *status_out = r->status;
     if (r->status == 0) {
-      if (x_host) std::memcpy(x_host, r->x.data(), sizeof(double) * r->x.size());
+      if (x_host) std::memcpy(x_host, r->hx, sizeof(double) * (size_t)r->n);
       if (iters_out) *iters_out = r->iters;
       if (latency_ms_out) *latency_ms_out = r->latency_ms;
     }

@@ -436,9 +436,15 @@ int gsh_server_step(gsh_server* srv, int32_t* active_out, int32_t* queued_out);
 /* Tickets of the requests retired since the last call (up to cap, oldest first), so a caller
  * polls only finished requests. Requests rejected at admission are reported by gsh_server_poll. */
 int gsh_server_finished(gsh_server* srv, int64_t* tickets, int32_t cap, int32_t* count);
-/* Result of a finished request: status 0 = done, 1 = pending; x_host may be NULL. */
+/* Result of a finished request: status 0 = done, 1 = pending; x_host may be NULL. A finished
+ * request keeps its solution in page-locked host memory until gsh_server_release. */
 int gsh_server_poll(gsh_server* srv, int64_t ticket, double* x_host, int32_t* iters_out,
                     int32_t* status_out, double* latency_ms_out);
+/* Forgets a finished or rejected request: its page-locked host vectors return to the server's
+ * per-mesh pool for later requests, and the ticket becomes unknown (GSH_E_NOTFOUND).
+ * GSH_E_STATE while the request is queued or running. No counterpart in the reference (it
+ * has no server). */
+int gsh_server_release(gsh_server* srv, int64_t ticket);
 
 /* ---------------------------------------------------------------- measurement */
 /* No counterpart in the reference: measurement helpers of this library.

@@ -52,6 +52,26 @@ def test_alone_vs_mix(ctx):
         assert it == alone[k][1], (k, it, alone[k][1])
         assert np.array_equal(x, alone[k][0]), f"request on mesh {k} differs from the solo solve"
         assert lat > 0.0
+    # released requests return their page-locked vectors to the pool; a second round reuses
+    # them (right-hand sides overwrite the old contents) with the same results
+    for _, t in tickets:
+        srv.release(t)
+        with pytest.raises(L.GshError) as e:
+            srv.poll(t, 1)
+        assert e.value.code == L.GSH_E_NOTFOUND
+    again = [(k, srv.submit(ids[k], rhs[k])) for k in (2, 0, 1)]
+    with pytest.raises(L.GshError) as e:  # queued: not releasable
+        srv.release(again[0][1])
+    assert e.value.code == L.GSH_E_STATE
+    for _ in range(2000):
+        a, q = srv.step()
+        if a == 0 and q == 0:
+            break
+    for k, t in again:
+        st, x, it, lat = srv.poll(t, meshes[k][0].n)
+        assert st == 0 and it == alone[k][1]
+        assert np.array_equal(x, alone[k][0]), f"request on mesh {k} differs after buffer reuse"
+        srv.release(t)
     del srv_vals
 
 
@@ -63,6 +83,7 @@ def test_admission_budget(ctx):
     srv.step()
     st, x, it, lat = srv.poll(t, m.n)
     assert st == L.GSH_E_ADMISSION
+    srv.release(t)  # a rejected request is released like a finished one
 
 
 def test_mixed_alone_vs_mix(ctx):
