@@ -90,14 +90,16 @@ def timed_solves(ctx, solver, steps, rtol, maxiter, check_every):
 def ssor_bytes(mesh, solver):
     """Algorithmic bytes per iteration of the two SSOR launch kinds (real rows, compulsory
     traffic; 8 B value + 4 B column per off-diagonal entry, each half of them in one sweep):
-    backward sweeps read D~, r, p, q, s, y, gather t once and write p, q, y, r, t (colour 0 also
-    u and s); forward sweeps (colours 1..C-1) read D~, t, r, gather u once and write u and s."""
+    backward sweeps read D~, r, p, q, s, gather t once and write p, q, r, t, and read and write
+    y every other iteration (y takes its updates in pairs), 8 B per row on average; colour 0 also
+    writes u and s. Forward sweeps (colours 1..C-1) read D~, t, r, gather u once and write u
+    and s."""
     n, nnz = mesh.n, mesh.nnz
     nc, _, col = solver.ssor_info()
     half = (nnz - n) // 2
     n0 = int(np.count_nonzero(col == 0))
     nf = n - n0
-    bwd = 12 * half + 8 * n + 88 * n + 16 * n0
+    bwd = 12 * half + 8 * n + 80 * n + 16 * n0
     fwd = 12 * half + 48 * nf
     return {"bwd": bwd, "fwd": fwd, "ncolours": nc, "colour0_rows": n0}
 

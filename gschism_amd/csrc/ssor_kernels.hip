@@ -338,17 +338,25 @@ __device__ __forceinline__ void ssor_bwd_body(const SsorView<VT>& S, int tile, i
       pv[j] = vld(S.p, v);
       qv[j] = vld(S.q, v);
       sv[j] = vld(S.s, v);
-      yv[j] = vld(S.y, v);
     }
   }
   if (load_done(st) >= 0) return;
+  // y takes its updates in pairs: an odd iteration k adds alpha_(k-1) p_(k-1) (p as loaded)
+  // and then alpha_k p_k, the same two roundings as one update per iteration, so y is read
+  // and written every other iteration; k_ssor_xsolve adds a last update still pending
+  const int kit = init ? 0 : ((flags & kFirst) ? st->k : st->kk);
+  const bool ypair = !init && (kit & 1);
+  if (ypair) {
+#pragma unroll
+    for (int j = 0; j < J; ++j) yv[j] = vld(S.y, srow<R, W>(tile, j));
+  }
   // the gamma / delta partials of iteration k live in slot k & 1, so the first sweep can read
   // slot k & 1 while the sweeps write the partials of iteration k + 1 into the other slot
-  double alpha = 0.0, beta = 0.0;
+  double alpha = 0.0, beta = 0.0, aprev = 0.0;
   int wslot = 0;
   if (!init) {
     if (flags & kFirst) {
-      const int k = st->k;
+      const int k = kit;
       wslot = (k + 1) & 1;
       double gamma, delta;
       if (flags & kPre)
@@ -375,7 +383,8 @@ __device__ __forceinline__ void ssor_bwd_body(const SsorView<VT>& S, int tile, i
           }
         }
       } else {
-        const double gprev = st->rho[(k - 1) & 1], aprev = st->alpha[(k - 1) & 1];
+        const double gprev = st->rho[(k - 1) & 1];
+        aprev = st->alpha[(k - 1) & 1];
         beta = (gprev != 0.0) ? gamma / gprev : 0.0;
         const double den = delta - beta * gamma / aprev;
         alpha = (aprev != 0.0 && den != 0.0) ? gamma / den : 0.0;
@@ -387,15 +396,16 @@ __device__ __forceinline__ void ssor_bwd_body(const SsorView<VT>& S, int tile, i
         st->kk = k;
       }
     } else {
-      const int kk = st->kk;
+      const int kk = kit;
       alpha = st->alpha[kk & 1];
+      if (ypair) aprev = st->alpha[(kk - 1) & 1];
       beta = st->beta;
       wslot = (kk + 1) & 1;
     }
   }
 #pragma unroll
   for (int j = 0; j < J; ++j) usum[j] = upper_sum(S, ci, srow<R, W>(tile, j), t);
-  const VT av = (VT)alpha, bv = (VT)beta;
+  const VT av = (VT)alpha, bv = (VT)beta, apv = (VT)aprev;
   double g = 0.0, dl = 0.0;
 #pragma unroll
   for (int j = 0; j < J; ++j) {
@@ -407,7 +417,7 @@ __device__ __forceinline__ void ssor_bwd_body(const SsorView<VT>& S, int tile, i
       const VT qn = sv[j] + bv * qv[j];
       vst(S.p, v, pn);
       vst(S.q, v, qn);
-      vst(S.y, v, (VT)(yv[j] + av * pn));
+      if (ypair) vst(S.y, v, (VT)((VT)(yv[j] + apv * pv[j]) + av * pn));
       r1 = rv[j] - av * qn;
       vst(S.r, v, r1);
     }
@@ -619,17 +629,23 @@ __global__ void __launch_bounds__(256) k_ssor_check(const SsorSysT<VT>* __restri
 
 // ------------------------------------------------------------------ back to x
 // x = (D~ + U)^-1 y for one colour (into w), then scattered to the original order; x = 0 when
-// atol == 0 (pcg_solver.py:1089-1095).
+// atol == 0 (pcg_solver.py:1089-1095). When the last iteration run, K = k - 1, is even, its
+// update y += alpha_K p_K is still pending (the sweeps update y in pairs) and is added here.
 template <class VT, int R>
 __global__ void __launch_bounds__(256) k_ssor_xsolve(const SsorSysT<VT>* __restrict__ sys, int tile0, int c) {
   const SsorView<VT> S = sview(sys);
   if (S.st->zero) return;
+  const int K = S.st->k - 1;
+  const bool pend = K >= 0 && !(K & 1);
+  const VT aK = pend ? (VT)S.st->alpha[K & 1] : (VT)0;
   const ColourInfo ci = colour_info(S, c);
   const int tile = tile0 + xcd_block(blockIdx.x, gridDim.x);
   const gptr<const VT> x = C(S.w);
   for (int j = 0; j < R; ++j) {
     const int v = tile_row<R>(tile, j);
-    S.w[v] = (S.y[v] - upper_sum(S, ci, v, x)) / S.dt[v];
+    VT yv = S.y[v];
+    if (pend) yv = (VT)(yv + aK * S.p[v]);
+    S.w[v] = (yv - upper_sum(S, ci, v, x)) / S.dt[v];
   }
 }
 

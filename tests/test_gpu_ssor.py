@@ -77,7 +77,11 @@ def test_ssor_history_omega_and_steps(ctx):
             om.rhs(p)
             val, b = om.assemble(p)
         solver.assemble(p)
-        x, it, st = _compare(solver, ip, ix, val, b, hist=hist, omega=1.3, what=f"step {step}")
+        # windows of 10, 7 and 5 iterations: the solve ends on odd and even iterations, so the
+        # y update left pending after an even one is exercised
+        ce = (10, 7, 5)[step]
+        x, it, st = _compare(solver, ip, ix, val, b, hist=hist, omega=1.3, check_every=ce,
+                             what=f"step {step} check_every {ce}")
         hist = [x, hist[0], hist[1]]
         solver.update_elevation()
         om.set(L.F_ETA2, x)
