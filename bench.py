@@ -89,18 +89,20 @@ def timed_solves(ctx, solver, steps, rtol, maxiter, check_every):
 
 def ssor_bytes(mesh, solver):
     """Algorithmic bytes per iteration of the two SSOR launch kinds (real rows, compulsory
-    traffic; 8 B value + 4 B column per off-diagonal entry, each half of them in one sweep):
-    backward sweeps read D~, r, p, q, s, gather t once and write p, q, r, t, and read and write
-    y every other iteration (y takes its updates in pairs), 8 B per row on average; colour 0 also
-    writes u and s. Forward sweeps (colours 1..C-1) read D~, t, r, gather u once and write u
-    and s."""
+    traffic; 8 B value + 4 B column per off-diagonal entry, each half of them in one sweep).
+    Backward sweeps read D~, r, p, q, s and write p, q, r for every row, read and write y every
+    other iteration (y takes its updates in pairs: 8 B per row on average), gather the t of
+    colours 1..C-1 once and write it; colour 0 writes u and s instead of t. Forward sweeps
+    (colours 1..C-1) read D~, t, r and write s for their rows, gather the u of colours 0..C-2
+    once and write u for colours 1..C-2 (no higher colour gathers the last colour's u)."""
     n, nnz = mesh.n, mesh.nnz
     nc, _, col = solver.ssor_info()
     half = (nnz - n) // 2
     n0 = int(np.count_nonzero(col == 0))
+    nl = int(np.count_nonzero(col == nc - 1)) if nc > 1 else 0
     nf = n - n0
-    bwd = 12 * half + 8 * n + 80 * n + 16 * n0
-    fwd = 12 * half + 48 * nf
+    bwd = 12 * half + 64 * n + 8 * n + 16 * (n - n0) + 16 * n0
+    fwd = 12 * half + 32 * nf + 8 * (n - nl) + 8 * (nf - nl) if nc > 1 else 0
     return {"bwd": bwd, "fwd": fwd, "ncolours": nc, "colour0_rows": n0}
 
 

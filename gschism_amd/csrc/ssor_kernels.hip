@@ -425,7 +425,8 @@ __device__ __forceinline__ void ssor_bwd_body(const SsorView<VT>& S, int tile, i
     const VT zn = dtv[j] * r1;
     sacc<W>(g, (double)r1 * (double)zn);
     const VT tv = (zn - usum[j]) / dtv[j];
-    vst(S.t, v, tv);
+    // t of colour 0 is used only here: no lower colour gathers it and no forward sweep reads it
+    if (!mid) vst(S.t, v, tv);
     if (mid) {
       const VT kdv = S.unit_omega ? dtv[j] : S.kd[v];
       const VT uv = ((zn - kdv * tv) - (VT)0) / dtv[j];
@@ -513,7 +514,8 @@ __device__ __forceinline__ void ssor_fwd_body(const SsorView<VT>& S, int tile, i
     const int v = srow<R, W>(tile, j);
     const VT zn = dtv[j] * rv[j];
     const VT uv = ((zn - kdv[j] * tv[j]) - lsum[j]) / dtv[j];
-    vst(S.u, v, uv);
+    // u of the last colour is gathered by no higher colour
+    if (!(flags & kLast)) vst(S.u, v, uv);
     const VT sn = tv[j] + uv;
     vst(S.s, v, sn);
     sacc<W>(dl, (double)zn * (double)sn);

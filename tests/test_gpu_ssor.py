@@ -415,3 +415,38 @@ def test_mixed_batched_ssor_equals_solo(ctx):
         assert_bitwise(mx.get_solution(), xm, "mixed batched SSOR solution")
     for mx, _ in systems:
         mx.close()
+
+
+def test_ssor_natural_and_shuffled_numbering(ctx):
+    """The same 5-point operator on a 400 x 300 grid in natural numbering (neighbours within a few
+    hundred rows) and in a random numbering (neighbours anywhere in colour blocks of 60k rows):
+    both give the oracle's solution bit for bit, alone and batched together."""
+    nx, ny = 400, 300
+    n = nx * ny
+    T = sp.diags([-np.ones(nx - 1), 2.0 * np.ones(nx), -np.ones(nx - 1)], [-1, 0, 1])
+    A0 = (sp.kron(sp.eye(ny), T) + sp.kron(sp.diags([-np.ones(ny - 1), 2.0 * np.ones(ny), -np.ones(ny - 1)],
+                                                     [-1, 0, 1]), sp.eye(nx)) + 0.01 * sp.eye(n)).tocsr()
+    perm = np.random.default_rng(4).permutation(n)
+    systems = []
+    for name, A in (("natural", A0), ("shuffled", A0[perm][:, perm].tocsr())):
+        A.sort_indices()
+        s = core.Solver(None, ctx=ctx, indptr=A.indptr, indices=A.indices, precond=L.PRECOND_SSOR)
+        s.set_values(A.data)
+        b = np.random.default_rng(9).normal(size=n)
+        s.set_rhs(b)
+        systems.append((name, s, A, b))
+    print(f"[gpu] SSOR natural and shuffled numbering, colours {[s.ssor_info()[0] for _, s, _, _ in systems]}",
+          flush=True)
+    solo = []
+    for name, s, A, b in systems:
+        x, it, st = s.solve(None, rtol=1e-12, maxiter=3000)
+        xo, ito, _, _, _ = O.pcg_ssor(A.indptr, A.indices, A.data, b, rtol=1e-12, maxiter=3000, rows_per_thread=1,
+                                      nthreads=8)
+        assert st == 0 and it == ito, (name, it, ito)
+        assert_bitwise(x, xo, f"{name} numbering SSOR vs oracle")
+        solo.append((x, it))
+        s.reset_history()
+    its = core.batch_solve([s for _, s, _, _ in systems], rtol=1e-12, maxiter=3000)
+    for (name, s, _, _), (x, it), itb in zip(systems, solo, its):
+        assert itb == it
+        assert_bitwise(s.get_solution(), x, f"{name} numbering batched SSOR")
