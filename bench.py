@@ -584,6 +584,10 @@ def main() -> int:
     ms_bt_staged = min(timed(lambda: mesh.backtrack(p)) for _ in range(3))
     mesh.set_backtrack_staging(False)
     walks = mesh.ns * mesh.nvrt
+    # the state and results of these backtracks, for the CPU baseline of backtracking below
+    bt_host = None
+    if not args.no_cpu_baseline:
+        bt_host = {f: mesh.get(f) for f in (L.F_ETA2, L.F_NODE_UVW, L.F_SIDE_U2, L.F_FOOT, L.F_BTU, L.F_BSTATUS)}
     ms_rhs = min(timed(lambda: mesh.rhs_producers(p)) for _ in range(3))
     solver = core.Solver(mesh, L.PREC_FP64, precond=L.PRECOND_SSOR)
     solver.set_tuning(args.ssor_rows_per_thread)
@@ -750,6 +754,30 @@ def main() -> int:
                "single_core": {"value": round(1.0 / one_core, 4), "unit": "solves/s", "cores": 1,
                                "sample": "1 solve", "bitwise_equal": bool(np.array_equal(x1, x_ssor))}}
         log(f"[bench] cpu baseline: {cpu}")
+        # backtracking on the host: the oracle's backtrackAllNodes restatement, all host cores, on
+        # every 4th side (all levels) of the same mesh and state, compared with the GPU's feet,
+        # velocities and status flags for those start points
+        om = O.OracleMesh(mesh.topology(L.T_XY), mesh.topology(L.T_ELNODE), mesh.topology(L.T_DEPTH),
+                          mesh.topology(L.T_OPEN_BND), data.sigma, kb=mesh.topology(L.T_KB))
+        for f in (L.F_ETA2, L.F_NODE_UVW, L.F_SIDE_U2):
+            om.set(f, bt_host[f])
+        om.levels(p)
+        nv_ = mesh.nvrt
+        sides = np.repeat(np.arange(0, mesh.ns, 4, dtype=np.int32), nv_)
+        levs = np.tile(np.arange(nv_, dtype=np.int32), (mesh.ns + 3) // 4)
+        a = time.perf_counter()
+        om.backtrack(p, sides, levs, nthreads=threads)
+        bt_cpu_s = time.perf_counter() - a
+        at = sides.astype(np.int64) * nv_ + levs
+        same = all(np.array_equal(om.get(f).reshape(mesh.ns * nv_, -1)[at], bt_host[f].reshape(mesh.ns * nv_, -1)[at])
+                   for f in (L.F_FOOT, L.F_BTU, L.F_BSTATUS))
+        bt_rec["cpu_baseline"] = {"value": round(sides.size / bt_cpu_s, 1), "unit": "side-levels/s", "cores": threads,
+                                  "kind": "port",
+                                  "sample": f"{sides.size} (side, level) start points (every 4th side, all levels, "
+                                            f"{ndelt} substeps), oracle C restatement, OpenMP over start points",
+                                  "seconds": round(bt_cpu_s, 3), "gpu_bitwise_equal": bool(same)}
+        log(f"[bench] backtrack cpu baseline: {bt_rec['cpu_baseline']}")
+        del om, bt_host
 
     # ---- whole time steps of the rebuilt path on the same mesh (after every measurement above
     # that needs the original state): backtracking, RHS producers, assembly, FP64 SSOR solve,
