@@ -803,6 +803,46 @@ def main() -> int:
     full_step = {"ms": round(ms_step, 3), "steps_per_s": round(1e3 / ms_step, 3),
                  "solver_iters": [int(i) for i, _ in step_its], "status": [int(s) for _, s in step_its]}
     log(f"[bench] full step: {full_step}")
+    if not args.no_cpu_baseline:
+        # the same step on the host: the oracle's restatement of every piece (backtracking,
+        # assembly, the SSOR solve and the nodal velocities on all host cores; RHS producers,
+        # momentum, vertical velocity and levels on one), from the GPU's state before one more
+        # step with a zero solver history, compared bitwise with the GPU's state after it
+        from oracle import pyoracle as O
+
+        solver.reset_history()
+        start = {f: mesh.get(f) for f in (L.F_ETA1, L.F_ETA2, L.F_NODE_UVW, L.F_SIDE_U2)}
+        it_g, _ = one_step()
+        after = {f: mesh.get(f) for f in (L.F_ETA2, L.F_SIDE_U2, L.F_ELEM_W, L.F_NODE_UVW)}
+        om = O.OracleMesh(mesh.topology(L.T_XY), mesh.topology(L.T_ELNODE), mesh.topology(L.T_DEPTH),
+                          mesh.topology(L.T_OPEN_BND), data.sigma, kb=mesh.topology(L.T_KB))
+        for f, v in start.items():
+            om.set(f, v)
+        om.levels(p)
+        ip_o, ix_o = om.topology(L.T_CSR_INDPTR), om.topology(L.T_CSR_INDICES)
+        a = time.perf_counter()
+        om.backtrack(p, nthreads=threads)
+        om.rhs(p)
+        val_o, b_o = om.assemble(p, nthreads=threads)
+        x_o, it_o, _, _, _ = O.pcg_ssor(ip_o, ix_o, val_o, b_o, hist=[np.zeros(n)] * 3, omega=1.0, rtol=rtol,
+                                        maxiter=maxiter, check_every=check_every,
+                                        rows_per_thread=args.ssor_rows_per_thread, nthreads=threads)
+        om.set(L.F_ETA1, om.get(L.F_ETA2))
+        om.set(L.F_ETA2, x_o)
+        om.momentum(p)
+        om.vertical_velocity()
+        om.levels(p)
+        om.nodal_velocity(p, nthreads=threads)
+        cpu_step_s = time.perf_counter() - a
+        same = all(np.array_equal(om.get(f), v) for f, v in after.items()) and int(it_o) == int(it_g)
+        full_step["cpu_baseline"] = {"value": round(1.0 / cpu_step_s, 4), "unit": "steps/s", "cores": threads,
+                                     "kind": "port",
+                                     "sample": "one whole step of the same mesh and state (oracle C restatement; "
+                                               "backtracking, assembly, SSOR solve and nodal velocities on all "
+                                               "cores, the other pieces on one)",
+                                     "seconds": round(cpu_step_s, 3), "gpu_bitwise_equal": bool(same)}
+        log(f"[bench] full step cpu baseline: {full_step['cpu_baseline']}")
+        del om
 
     mixed4m = None
     if not args.no_mixed_4m:
