@@ -635,6 +635,17 @@ def main() -> int:
     solver.reset_history()
     x_ssor, it_ssor, _ = solver.solve(None, rtol, maxiter, check_every)
     rel_ssor = solver.check()
+    # the same solves with b handed over from host memory and x returned to it (the boundary's
+    # host-buffer form): the rate with the PCIe copies, reported beside the value, never as it
+    b_host = solver.get_rhs()
+    ctx.synchronize()
+    a = time.perf_counter()
+    for _ in range(args.steps):
+        solver.reset_history()
+        solver.solve(b_host, rtol, maxiter, check_every)
+    ctx.synchronize()
+    host_buf_rate = args.steps / (time.perf_counter() - a)
+    log(f"[bench] with b from and x to host memory: {host_buf_rate:.2f} solves/s")
 
     # ---- kernel durations per launch kind (kernel-stamped HIP events on the solver stream)
     solver.reset_history()
@@ -899,6 +910,8 @@ def main() -> int:
         "mixed_4m": mixed4m,
         "batched": batched,
         "served": served,
+        "host_buffers": {"solves_per_s": round(host_buf_rate, 3),
+                         "note": "the same solves with b copied from and x copied to host memory per solve"},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
